@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Benchmark of the PHAFD per-timestep hot path on one MI355X.
+
+One "step" is one full timestep of the reference's Integrate::run hook
+sequence (src/integrate.cpp:105-203) over the whole box: grad(phi) by FFT,
+neighbour decision (rebuild: pbc, ghost images, bins, grid and half lists),
+grid-atom + lj/cut pair pass and ghost force fold-back, chain half step,
+Flory-Huggins + r2c(mu), spectral update, chain half step, c2r(phi), NaN check.
+
+Default workload (the plain-FP64-on-GPU configuration): 256^3 periodic grid,
+box 256, phi0 = 0.4 + U(-0.02, 0.02), chi = 2.5, noise off, 5000 chains x 100
+beads (random walks, bond 1), lj/cut sigma 1 cut 2^(1/6), gridatom/gaussian
+cut 3 width 1 epsilon 1, neighbour skin 0.5, semiflexible harmonic chains
+(k = 100, kb = 10), dt = 1e-3.  Inputs are generated on the host, uploaded,
+and resident in HBM before the timed region starts.
+
+Prints ONE JSON line (rank 0).  `value` = grid points x steps / s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-pt·steps/s + beads·steps/s, surrogate/FP32 vs FP64 at ≤1e-3 φ err"
+
+WORKLOADS = {
+    # name: (n, box, nchains, nbeads, phi0, var, chi, gauss_cut, nucwidth, style)
+    "c2": dict(n=256, box=256.0, nchains=5000, nbeads=100, phi0=0.4, var=0.04, chi=2.5,
+               cut=3.0, nucwidth=1.0, ljish=False),
+    "c1": dict(n=64, box=64.0, nchains=20, nbeads=50, phi0=0.5, var=0.02, chi=3.0,
+               cut=3.0, nucwidth=1.0, ljish=False),
+}
+DESCR = {
+    "c2": "plain FP64 on GPU: 256^3 grid, 5000x100-bead semiflexible chains (500k beads), "
+          "gridatom/gaussian width 1 cut 3, lj/cut 2^(1/6), chi 2.5, noise off",
+    "c1": "plain FP64: 64^3 grid, 20x50-bead chains, gridatom/gaussian width 1 cut 3, chi 3, noise off",
+}
+
+
+def setup_inputs(wl):
+    import phafd_amd as P
+    n = wl["n"]
+    phi = P.ic_noisy_constant((n, n, n), wl["phi0"], wl["var"], 1234)
+    xyz, mol = P.ic_random_chains(wl["nchains"], wl["nbeads"], 1.0, 0.9, [0, 0, 0], [wl["box"]] * 3, 42)
+    return phi, xyz, mol
+
+
+def configure(s, wl, phi, xyz, mol, is_oracle=False):
+    nch, nb = wl["nchains"], wl["nbeads"]
+    s.set_field(0, phi)
+    s.set_atoms(xyz, np.zeros(len(xyz), np.int32), tags=np.arange(len(xyz), dtype=np.int32), mols=mol)
+    style = 1 if wl["ljish"] else 0
+    ip = s.add_pair(style, wl["cut"]) if is_oracle else s.pair_gridatom(style, wl["cut"])
+    s.pair_coeff(ip, 0, 1.0, 0.1, wl["nucwidth"])
+    if is_oracle:
+        s.add_pair(2, 1.122462048, 1.0, 1.0)
+    else:
+        s.pair_ljcut(1.122462048, 1.0, 1.0)
+    s.fix_conjugate(5678, 1.0, 1.0, 1.0, 1.0, 0)
+    s.fix_floryhuggins(1.0, 1.0, wl["chi"])
+    st = (np.arange(nch) * nb).astype(np.int32)
+    s.fix_chain(st, st + nb, 1.0, 100.0, 10.0, 1.0)
+    s.neighbor(0.5)
+    s.setup(1e-3)
+
+
+def cpu_baseline(wl, phi, xyz, mol, steps):
+    """The CPU oracle (single thread) on a bounded sample of the same workload."""
+    from oracle import oracle_ctypes as O
+    n = wl["n"]
+    s = O.System(n, n, n, [0, 0, 0], [wl["box"]] * 3)
+    configure(s, wl, phi, xyz, mol, is_oracle=True)
+    t0 = time.perf_counter()
+    s.run(steps)
+    dt = time.perf_counter() - t0
+    s.close()
+    return dt
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (profiles/<kernel>_<workload>_pmc.md), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--probe", default="z_inv", help="kernel timed with HIP events for the roofline")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timers", action="store_true", help="print per-hook GPU times to stderr")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU decomposition is a design, not built", file=sys.stderr)
+        sys.exit(2)
+
+    import phafd_amd as P
+    wl = WORKLOADS[args.workload]
+    n = wl["n"]
+    phi, xyz, mol = setup_inputs(wl)
+    s = P.System(n, n, n, [0, 0, 0], [wl["box"]] * 3)
+    configure(s, wl, phi, xyz, mol)
+    s.run(args.warmup)
+    s.timers(reset=True)
+    s.probe(args.probe)
+    t0 = time.perf_counter()
+    s.run(args.steps)          # every step ends with a stream synchronise (NaN check)
+    elapsed = time.perf_counter() - t0
+    pms, plaunch, pbytes = s.probe(None)
+    timers = s.timers()
+    counts = s.counts()
+    s.close()
+
+    N = n ** 3
+    nbeads = wl["nchains"] * wl["nbeads"]
+    ms_per_step = 1e3 * elapsed / args.steps
+    peak = 8.0e3   # GB/s, MI355X HBM3E
+    avg_ms = pms / max(plaunch, 1)
+    bytes_per_launch = pbytes / max(plaunch, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if plaunch else None
+    traffic = read_traffic(args.probe, args.workload)
+    out = {
+        "metric": METRIC,
+        "value": N * args.steps / elapsed,
+        "unit": "grid-pt·steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": DESCR[args.workload], "grid": [n, n, n], "beads": nbeads,
+                   "dt": 1e-3, "neighbor_builds_in_run": counts["ncalls"]},
+        "beads_steps_per_s": nbeads * args.steps / elapsed,
+        "roofline": {"bound": "hbm", "kernel": args.probe, "achieved": achieved, "peak": peak,
+                     "unit": "GB/s", "frac": (achieved / peak) if achieved else None,
+                     "traffic": traffic, "avg_launch_ms": avg_ms, "launches": plaunch,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+    if args.timers:
+        print(json.dumps({"per_hook_ms_total": timers}), file=sys.stderr)
+    if not args.no_cpu_baseline:
+        t = cpu_baseline(wl, phi, xyz, mol, args.cpu_steps)
+        out["cpu_baseline"] = {"value": N * args.cpu_steps / t, "unit": "grid-pt·steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{args.cpu_steps} full timesteps of the same workload after setup, "
+                                         f"CPU FP64 oracle, one thread ({t:.1f} s)"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,197 @@
+/*
+ * phafd_hip.h -- C ABI of the MI355X (gfx950) implementation of PHAFD's
+ * per-timestep hot path: Model-B pseudo-spectral concentration update,
+ * grid-atom coupling with its binned neighbour lists, bead-bead LJ and
+ * semiflexible-chain forces, for one GPU holding the whole periodic box.
+ *
+ * Plain C: pointers, sizes, int status codes.  No exception, no C++ or torch
+ * type crosses this boundary.  Every entry point returns 0 (or a
+ * non-negative id) on success and a negative PHAFD_E* code on failure; the
+ * message is available from phafd_hip_last_error().
+ *
+ * Each declaration cites the reference interface it replaces (paths relative
+ * to the reference source tree).  The reference runs these as C++ virtual
+ * hooks on one MPI rank (src/fix.hpp:16-29, src/pair.hpp:36-42); here one
+ * context owns all device state and runs the same hook sequence on one HIP
+ * stream.
+ *
+ * Host layouts at this boundary:
+ *   real field : double[Nz][Ny][Nx], x fastest (flat = (z*Ny+y)*Nx+x)
+ *   spectrum   : double[Nz][Ny][Nx/2+1][2] (re, im), kx fastest
+ *   atoms      : double[n][3] positions / forces, int[n] types, tags, ...
+ * Threading: one context per host thread; calls on a context are not
+ * thread-safe.  get_* calls synchronise the context stream.
+ */
+#ifndef PHAFD_HIP_H
+#define PHAFD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct phafd_hip_ctx phafd_hip_ctx;
+
+/* error codes */
+enum {
+  PHAFD_OK = 0,
+  PHAFD_E_ARG = -1,      /* invalid argument / state (std::invalid_argument)  */
+  PHAFD_E_HIP = -2,      /* HIP runtime error                                 */
+  PHAFD_E_NAN = -3,      /* "NAN encountered in phi." (src/integrate.cpp:190-201) */
+  PHAFD_E_RUNTIME = -4,  /* runtime error (std::runtime_error)                */
+  PHAFD_E_OVERFLOW = -5  /* neighbour row longer than oneatom = 2000; the
+                            reference only warns (src/npair_grid_bin.cpp:119-121) */
+};
+
+/* fields (src/grid.hpp:28-36) */
+enum { PHAFD_PHI = 0, PHAFD_MU = 1, PHAFD_GRADX = 2, PHAFD_GRADY = 3, PHAFD_GRADZ = 4 };
+enum { PHAFD_PHIHAT = 0, PHAFD_MUHAT = 1 };
+/* ghost images along z on one GPU (single-rank CommBrick keeps none) */
+enum { PHAFD_ZGHOST_REFERENCE = 0, PHAFD_ZGHOST_PERIODIC = 1 };
+/* thermal noise of grid/conjugate/volfrac */
+enum { PHAFD_NOISE_OFF = 0, PHAFD_NOISE_FIELD = 2 };
+/* grid-atom pair styles */
+enum { PHAFD_PAIR_GAUSSIAN = 0, PHAFD_PAIR_LJISH = 1 };
+enum { PHAFD_FP64 = 0, PHAFD_FP32 = 1 };
+/* hook phases of one timestep (src/integrate.cpp:105-203) */
+enum {
+  PHAFD_PH_START = 0, PHAFD_PH_NEIGH = 1, PHAFD_PH_PAIR = 2, PHAFD_PH_INITIAL = 3,
+  PHAFD_PH_POSTFORCE = 4, PHAFD_PH_PREFINAL = 5, PHAFD_PH_FINAL = 6, PHAFD_PH_POSTFINAL = 7
+};
+
+typedef struct {
+  int nx, ny, nz;          /* grid_style Nx Ny Nz (src/grid.cpp:88-113); powers of two, 4..1024 */
+  double boxlo[3];         /* domain boxorigin (src/domain.cpp:50-56) */
+  double period[3];        /* domain boxdims   (src/domain.cpp:44-49) */
+  int zghost;              /* PHAFD_ZGHOST_* */
+  int precision;           /* PHAFD_FP64 (the drop-in) or PHAFD_FP32 */
+  int device;              /* HIP device ordinal */
+} phafd_hip_desc;
+
+/* ---- lifecycle: PHAFD::create + Domain::set_box + Grid::create
+ *      (src/phafd.cpp:108-121, src/domain.cpp:30-99, src/grid.cpp:88-113,205-286) */
+int phafd_hip_create(const phafd_hip_desc *desc, phafd_hip_ctx **out);
+int phafd_hip_destroy(phafd_hip_ctx *ctx);
+const char *phafd_hip_last_error(const phafd_hip_ctx *ctx);
+
+/* ---- fields: Grid public arrays phi, chempot, gradphi (src/grid.hpp:28-36) */
+int phafd_hip_set_field(phafd_hip_ctx *ctx, int which, const double *host);
+int phafd_hip_get_field(phafd_hip_ctx *ctx, int which, double *host);
+/* ft_phi / ft_chempot (src/grid.hpp:34-35), returned in the canonical layout */
+int phafd_hip_get_spectrum(phafd_hip_ctx *ctx, int which, double *host);
+
+/* ---- atoms: Atom::setup + ReadAtoms + check_tags_and_types
+ *      (src/atom.cpp:26-110,140-252, src/read_atoms.cpp:58-97) */
+int phafd_hip_set_atoms(phafd_hip_ctx *ctx, int n, const double *xyz, const int *type,
+                        const int *tag, const int *mol);
+/* Atom::xs, uxs, images (src/atom.hpp:38-47); any pointer may be NULL */
+int phafd_hip_get_atoms(phafd_hip_ctx *ctx, double *xyz, double *uxyz, int *image);
+/* Atom::Fs of owned atoms after reverse_comm (src/comm_brick.cpp:228-258) */
+int phafd_hip_get_forces(phafd_hip_ctx *ctx, double *f);
+
+/* ---- pair styles, in script order (src/input.cpp:99-153).
+ * pair gridatom/gaussian|gridatom/LJish <cut>: Pair::settings
+ * (src/pair_gridatom_gaussian.cpp:123-140, src/pair_gridatom_lj_ish.cpp:184-204).
+ * Returns the pair id (>= 0). */
+int phafd_hip_pair_gridatom(phafd_hip_ctx *ctx, int style, double cut);
+/* pairCoeffs <type> epsilon e phi p nucwidth w (src/pair_gridatom_gaussian.cpp:143-181);
+ * pass NaN to leave a value at its default */
+int phafd_hip_pair_coeff(phafd_hip_ctx *ctx, int pair, int type, double epsilon, double phi,
+                         double nucwidth);
+/* pair lj/cut <cut> <epsilon> <sigma> (src/pair_lj_cut.cpp:101-117) */
+int phafd_hip_pair_ljcut(phafd_hip_ctx *ctx, double cut, double epsilon, double sigma);
+
+/* ---- fixes, in script order (src/input.cpp:175-230) */
+/* fix grid/conjugate/volfrac <name> <seed> mobility M temp T volFH v gamma g
+ * (src/fixgrid_conjugate.cpp:21-45, src/conjugate_volfrac.cpp:29-69).
+ * noise_mode: PHAFD_NOISE_OFF (new switch: the reference always adds noise)
+ * or PHAFD_NOISE_FIELD (per-mode host field, see phafd_hip_set_noise). */
+int phafd_hip_fix_conjugate(phafd_hip_ctx *ctx, int seed, double mobility, double temp,
+                            double volFH, double gamma, int noise_mode);
+/* host noise field xi in the canonical spectrum layout, used by the next
+ * spectral updates (the value ConjugateVolFrac adds per mode,
+ * src/conjugate_volfrac.cpp:95-96,118); NULL clears it */
+int phafd_hip_set_noise(phafd_hip_ctx *ctx, const double *xi);
+/* fix grid/floryhuggins <name> temp T volFH v chi c (src/fixgrid_floryhuggins.cpp:22-53) */
+int phafd_hip_fix_floryhuggins(phafd_hip_ctx *ctx, double temp, double volFH, double chi);
+/* fix grid/gradphi <name> once|every (src/fixgrid_gradphi.cpp:21-33) */
+int phafd_hip_fix_gradphi(phafd_hip_ctx *ctx, int once);
+/* fix atom/drag <name> <group> <drag> over group chunks [start,end)
+ * (src/fixatom_drag.cpp:19-66, src/group.cpp:31-207) */
+int phafd_hip_fix_drag(phafd_hip_ctx *ctx, int nchunk, const int *start, const int *end,
+                       double drag);
+/* fix atom/semiflexible/harmonic: replaces the bead-rod dynamics of
+ * atom/semiflexible/{notether,singletether,doubletether}
+ * (src/fixatom_semiflexible.cpp:27-191, which delegate to
+ * the absent beadrodpmer library) with harmonic bonds k/2 (r-r0)^2 and
+ * bending kb (1 - t_a.t_b) on unwrapped chain coordinates, integrated
+ * overdamped with the same two half steps as atom/drag. */
+int phafd_hip_fix_chain(phafd_hip_ctx *ctx, int nchunk, const int *start, const int *end,
+                        double r0, double kbond, double kbend, double drag);
+
+/* ---- neighbor <skin>: Neighbor::setup + CommBrick::setup
+ *      (src/neighbor.cpp:39-68, src/comm_brick.cpp:96-187) */
+int phafd_hip_neighbor(phafd_hip_ctx *ctx, double skin);
+
+/* ---- run: Integrate::setup / Integrate::run (src/integrate.cpp:33-219) */
+int phafd_hip_setup(phafd_hip_ctx *ctx, double dt);
+int phafd_hip_run(phafd_hip_ctx *ctx, int nsteps);
+/* one hook phase of a timestep (PHAFD_PH_*); PHAFD_PH_START advances the step */
+int phafd_hip_phase(phafd_hip_ctx *ctx, int phase);
+
+/* ---- inspection (for parity checks and diagnostics) */
+/* {nowned, ngathered, nlocal, nghost, neighbor builds, num_less_zero,
+ *  num_great_zero, list overflow} (src/atom.hpp:27-31, src/neighbor.cpp:105,
+ *  src/fixgrid_floryhuggins.cpp:74-80) */
+int phafd_hip_counts(phafd_hip_ctx *ctx, int64_t *out8);
+/* NPairGridBin list as ilist / numneigh / flattened neighbours
+ * (src/npair_grid_bin.cpp:22-127); call with ilist == NULL to query sizes */
+int phafd_hip_gridlist(phafd_hip_ctx *ctx, int64_t cap, int *ilist, int *numneigh,
+                       int *neigh, int64_t *inum, int64_t *nentries);
+/* NPairHalfBinNewton list (src/npair_half_bin_newton.cpp:38-124) */
+int phafd_hip_halflist(phafd_hip_ctx *ctx, int64_t cap, int *ilist, int *numneigh,
+                       int *neigh, int64_t *inum, int64_t *nentries);
+/* gathered copies / ghost images built by CommBrick::borders
+ * (src/comm_brick.cpp:263-426); returns ngathered */
+int phafd_hip_gathered(phafd_hip_ctx *ctx, double *xyz, int *tag, int *label, int *owner);
+/* NBinStandard geometry and linked lists binhead / bins / atom2bin
+ * (src/nbin_standard.cpp:53-229); info[12] = nbin xyz, mbin xyz, mbinlo xyz,
+ * mbins, full stencil size, half stencil size */
+int phafd_hip_bins(phafd_hip_ctx *ctx, int *info, int *binhead, int *bins, int *atom2bin);
+/* energies defined by this implementation (the reference computes none):
+ * {Flory-Huggins + square-gradient, grid-atom coupling, lj/cut, chain} */
+int phafd_hip_energy(phafd_hip_ctx *ctx, double *out4);
+/* per-hook GPU time in ms accumulated over run() calls (hipEvent timers):
+ * {gradphi, neighbor, pair, integrate, floryhuggins+r2c mu, update, c2r phi, total};
+ * reset when reset != 0 */
+int phafd_hip_timers(phafd_hip_ctx *ctx, double *ms8, int reset);
+/* HIP-event timing of every launch of one kernel during run():
+ * kernel_id >= 0 (re)starts the probe for that kernel (ids in the order
+ * r2c x, y fwd, y inv, z fwd, z inv, c2r x, grad multiply, update, FH,
+ * grid-atom deposit, grid-atom force, grid list count, grid list fill,
+ * half list count, half list fill, lj/cut); kernel_id < 0 stops it.
+ * out3 = {total ms, launches, algorithmic bytes} accumulated so far. */
+int phafd_hip_probe(phafd_hip_ctx *ctx, int kernel_id, double *out3);
+
+/* ---- stateless kernels (unit-level parity; allocate internally) */
+/* fftw_mpi_plan_dft_r2c_3d / _c2r_3d (src/grid.cpp:239-280), unnormalised;
+ * c2r uses only Re() of the kx = 0 and kx = Nx/2 bins */
+int phafd_hip_fft_r2c(int nx, int ny, int nz, const double *in, double *out);
+int phafd_hip_fft_c2r(int nx, int ny, int nz, const double *in, double *out);
+/* FixGridConjugate::start_of_step grad(phi) (src/fixgrid_conjugate.cpp:62-116) */
+int phafd_hip_gradphi(int nx, int ny, int nz, const double *period, const double *phi,
+                      double *gx, double *gy, double *gz);
+/* FixGridFloryHuggins::post_force (src/fixgrid_floryhuggins.cpp:65-85) */
+int phafd_hip_floryhuggins(int64_t n, double *phi, double *mu, double temp, double volFH,
+                           double chi, int64_t *nless, int64_t *ngreat);
+/* Conjugate::update + ConjugateVolFrac (src/conjugate.cpp:163-327,
+ * src/conjugate_volfrac.cpp:72-144) on a canonical-layout spectrum */
+int phafd_hip_spectral_update(int nx, int ny, int nz, const double *period, double *phihat,
+                              const double *muhat, double mobility, double temp, double volFH,
+                              double gamma, double dt, const double *noise);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1532 @@
+/*
+ * oracle.cpp -- CPU FP64 restatement of PHAFD's per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Single-threaded, compiled with
+ * -O3 -ffp-contract=off so that the order of floating-point operations is the
+ * one written here, which follows the reference line by line where it
+ * matters for bit-exact neighbour lists.  Every function cites the reference
+ * file:line it restates (paths relative to the reference source tree).
+ *
+ * Parity pins (what checks each part; the tests live under tests/):
+ *   FFT r2c / c2r ........ numpy rfftn / irfftn golden vectors incl. non-Hermitian
+ *                          c2r inputs, plus brute-force DFT at 4^3 / 8x4x2
+ *   gradphi .............. closed form for single sine modes (3.141592 wavenumber)
+ *   spectral update ...... closed-form single-mode recurrence, exact mass
+ *                          conservation, Hermitian symmetry of the k=0 planes
+ *   Flory-Huggins ........ closed form per point incl. both reflection branches
+ *   noise draw order ..... C++ standard KAT of std::mt19937 (10000th draw) and
+ *                          a per-mode draw-count/ordering test
+ *   bins / stencils ...... brute-force bin membership, hand-enumerated stencils
+ *   grid & half lists .... O(N*M) brute-force pair sets on small boxes
+ *   ghosts ............... hand-enumerated images near faces (z quirk incl.)
+ *   gridatom pairs ....... single atom on a tiny grid (closed-form deposit,
+ *                          zero force for constant phi)
+ *   lj/cut ............... two-body analytic force, Newton's third law
+ *   chain (build-defined). finite-difference gradient of its own energy
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+typedef std::complex<double> cplx;
+
+namespace {
+
+const double PI_REF = 3.141592;   // src/domain.cpp:206-208 truncates pi
+const int IMGMASK = 1023, IMGMAX = 512, IMGBITS = 10, IMG2BITS = 20; // src/domain.cpp:10-13
+const int OWNED = 0, LOCAL = 1, GHOST = 2;                           // src/atom.hpp:18-20
+const double BIG = 1e20;                                             // src/comm_brick.cpp:8
+const double SMALL = 1.0e-6;                                         // src/nbin_standard.cpp:25
+const int ONEATOM = 2000;                                            // src/neighbor.cpp:25
+
+bool is_pow2(int n) { return n >= 1 && (n & (n - 1)) == 0; }
+
+/* ------------------------------------------------------------------ FFT --
+ * Iterative radix-2 complex FFT (Cooley-Tukey, bit-reversed input order).
+ * Twiddles are computed in long double.  Unnormalised, sign -1 forward.   */
+struct FFT1 {
+  int n = 0;
+  std::vector<int> rev;
+  std::vector<cplx> wf, wb;
+  explicit FFT1(int n_) : n(n_) {
+    if (!is_pow2(n)) throw std::runtime_error("oracle FFT needs power-of-two sizes");
+    rev.resize(n);
+    int lg = 0;
+    while ((1 << lg) < n) lg++;
+    for (int i = 0; i < n; i++) {
+      int r = 0;
+      for (int b = 0; b < lg; b++) if (i & (1 << b)) r |= 1 << (lg - 1 - b);
+      rev[i] = r;
+    }
+    wf.resize(n / 2 + 1);
+    wb.resize(n / 2 + 1);
+    for (int k = 0; k <= n / 2; k++) {
+      long double a = 2.0L * 3.14159265358979323846264338327950288L * k / n;
+      wf[k] = cplx((double)cosl(a), (double)-sinl(a));
+      wb[k] = cplx((double)cosl(a), (double)sinl(a));
+    }
+  }
+  void run(cplx *a, int sign) const {
+    for (int i = 0; i < n; i++) if (i < rev[i]) std::swap(a[i], a[rev[i]]);
+    const std::vector<cplx> &w = sign < 0 ? wf : wb;
+    for (int len = 2; len <= n; len <<= 1) {
+      int step = n / len;
+      for (int i = 0; i < n; i += len)
+        for (int j = 0; j < len / 2; j++) {
+          cplx u = a[i + j];
+          cplx t = a[i + j + len / 2] * w[j * step];
+          a[i + j] = u + t;
+          a[i + j + len / 2] = u - t;
+        }
+    }
+  }
+};
+
+/* 3-D r2c of real[Nz][Ny][Nx] into spec[Nz][Ny][Nx/2+1] (unnormalised).
+ * Restates fftw_mpi_plan_dft_r2c_3d (src/grid.cpp:239-242) up to layout.   */
+void fft3_r2c(int nx, int ny, int nz, const double *in, cplx *out) {
+  FFT1 fx(nx), fy(ny), fz(nz);
+  const int hx = nx / 2 + 1;
+  std::vector<cplx> line(std::max(nx, std::max(ny, nz)));
+  for (int z = 0; z < nz; z++)
+    for (int y = 0; y < ny; y++) {
+      const double *r = in + ((size_t)z * ny + y) * nx;
+      for (int x = 0; x < nx; x++) line[x] = cplx(r[x], 0.0);
+      fx.run(line.data(), -1);
+      cplx *o = out + ((size_t)z * ny + y) * hx;
+      for (int k = 0; k < hx; k++) o[k] = line[k];
+    }
+  for (int z = 0; z < nz; z++)
+    for (int k = 0; k < hx; k++) {
+      for (int y = 0; y < ny; y++) line[y] = out[((size_t)z * ny + y) * hx + k];
+      fy.run(line.data(), -1);
+      for (int y = 0; y < ny; y++) out[((size_t)z * ny + y) * hx + k] = line[y];
+    }
+  for (int y = 0; y < ny; y++)
+    for (int k = 0; k < hx; k++) {
+      for (int z = 0; z < nz; z++) line[z] = out[((size_t)z * ny + y) * hx + k];
+      fz.run(line.data(), -1);
+      for (int z = 0; z < nz; z++) out[((size_t)z * ny + y) * hx + k] = line[z];
+    }
+}
+
+/* 3-D c2r (unnormalised) with the half-complex synthesis rule: inverse c2c
+ * along z and y over every kx, then a 1-D c2r along x that uses only the real
+ * part of the kx=0 and kx=Nx/2 bins.  Restates fftw_mpi_plan_dft_c2r_3d
+ * (src/grid.cpp:244-280) for Hermitian input; for the non-Hermitian grad
+ * spectra of src/fixgrid_conjugate.cpp:80-105 this is the stated convention
+ * (numpy irfftn follows the same rule and pins it). The input is preserved. */
+void fft3_c2r(int nx, int ny, int nz, const cplx *in, double *out) {
+  FFT1 fx(nx), fy(ny), fz(nz);
+  const int hx = nx / 2 + 1;
+  std::vector<cplx> w(in, in + (size_t)nz * ny * hx);
+  std::vector<cplx> line(std::max(nx, std::max(ny, nz)));
+  for (int y = 0; y < ny; y++)
+    for (int k = 0; k < hx; k++) {
+      for (int z = 0; z < nz; z++) line[z] = w[((size_t)z * ny + y) * hx + k];
+      fz.run(line.data(), +1);
+      for (int z = 0; z < nz; z++) w[((size_t)z * ny + y) * hx + k] = line[z];
+    }
+  for (int z = 0; z < nz; z++)
+    for (int k = 0; k < hx; k++) {
+      for (int y = 0; y < ny; y++) line[y] = w[((size_t)z * ny + y) * hx + k];
+      fy.run(line.data(), +1);
+      for (int y = 0; y < ny; y++) w[((size_t)z * ny + y) * hx + k] = line[y];
+    }
+  for (int z = 0; z < nz; z++)
+    for (int y = 0; y < ny; y++) {
+      const cplx *c = w.data() + ((size_t)z * ny + y) * hx;
+      line[0] = cplx(c[0].real(), 0.0);
+      for (int k = 1; k < nx / 2; k++) {
+        line[k] = c[k];
+        line[nx - k] = std::conj(c[k]);
+      }
+      if (nx > 1) line[nx / 2] = cplx(c[nx / 2].real(), 0.0);
+      fx.run(line.data(), +1);
+      double *r = out + ((size_t)z * ny + y) * nx;
+      for (int x = 0; x < nx; x++) r[x] = line[x].real();
+    }
+}
+
+/* signed wavenumber index: i > N/2 -> i-N (Nyquist stays positive),
+ * src/fixgrid_conjugate.cpp:82-93, src/conjugate.cpp:146-157               */
+inline int signed_k(int i, int n) { return i > n / 2 ? i - n : i; }
+
+/* grad multiply + three c2r + divide, src/fixgrid_conjugate.cpp:62-116.
+ * Canonical spectrum index (kz,ky,kx); the reference's transposed (i,j,k)
+ * has i = ky, j = kz.  dqx = 2pi/Lx, "dqy" = 2pi/Lz multiplies the kz index,
+ * "dqz" = 2pi/Ly multiplies the ky index (src/domain.cpp:206-208); the c2r of
+ * ft_gradphi[1] lands in gradphi[2] and vice versa (src/grid.cpp:269-280),
+ * so real-space gx, gy, gz are d/dx, d/dy, d/dz.                           */
+void gradphi_impl(int nx, int ny, int nz, const double period[3], const double *phi,
+                  double *gx, double *gy, double *gz, std::vector<cplx> *phihat_out) {
+  const int hx = nx / 2 + 1;
+  const size_t M = (size_t)nz * ny * hx;
+  std::vector<cplx> ph(M), g0(M), g1(M), g2(M);
+  fft3_r2c(nx, ny, nz, phi, ph.data());
+  const cplx idqx(0, 2 * PI_REF / period[0]);
+  const cplx idqy(0, 2 * PI_REF / period[2]);
+  const cplx idqz(0, 2 * PI_REF / period[1]);
+  for (int ky = 0; ky < ny; ky++) {
+    double l = (double)signed_k(ky, ny);
+    for (int kz = 0; kz < nz; kz++) {
+      double m = (double)signed_k(kz, nz);
+      for (int k = 0; k < hx; k++) {
+        double n = k;
+        size_t idx = ((size_t)kz * ny + ky) * hx + k;
+        g0[idx] = ph[idx] * idqx * n;
+        g1[idx] = ph[idx] * idqy * m;
+        g2[idx] = ph[idx] * idqz * l;
+      }
+    }
+  }
+  fft3_c2r(nx, ny, nz, g0.data(), gx);
+  fft3_c2r(nx, ny, nz, g1.data(), gz);   // swapped target, src/grid.cpp:270-273
+  fft3_c2r(nx, ny, nz, g2.data(), gy);   // swapped target, src/grid.cpp:276-279
+  double factor = (double)nx * ny * nz;  // src/fixgrid_conjugate.cpp:111
+  const size_t N = (size_t)nx * ny * nz;
+  for (size_t i = 0; i < N; i++) { gx[i] /= factor; gy[i] /= factor; gz[i] /= factor; }
+  if (phihat_out) phihat_out->swap(ph);
+}
+
+/* FixGridFloryHuggins::post_force, src/fixgrid_floryhuggins.cpp:65-85 */
+void floryhuggins_impl(int64_t n, double *phi, double *mu, double temp, double volFH,
+                       double chi, int64_t &nless, int64_t &ngreat) {
+  for (int64_t i = 0; i < n; i++) {
+    if (phi[i] <= 0) {
+      phi[i] = -phi[i];
+      nless += 1;
+    } else if (phi[i] >= 1) {
+      phi[i] = 1 - (phi[i] - 1);
+      ngreat += 1;
+    }
+    mu[i] += temp / volFH * (log(phi[i] / (1 - phi[i])) + chi * (1 - 2 * phi[i]));
+  }
+}
+
+/* Conjugate::update single-rank traversal, src/conjugate.cpp:163-327, with
+ * ConjugateVolFrac's per-mode formula, src/conjugate_volfrac.cpp:72-144.
+ * Noise modes: off (new flag: the reference has no switch, finding F2),
+ * mt19937 (the reference's draws, in its traversal order), or a per-mode
+ * host field in the canonical spectrum layout.                             */
+struct SpectralParams {
+  int nx, ny, nz;
+  double period[3];
+  double mobility, temp, volFH, gamma, dt;
+  int noise_mode;
+  const double *noise;   // canonical layout (re,im) when noise_mode == FIELD
+  std::mt19937 *gen;     // when noise_mode == MT19937
+  std::vector<double> *record;   // optional: record draws per mode (noise field generation)
+};
+
+struct SpectralUpdater {
+  SpectralParams p;
+  int hx, A, B;          // A = Ny (ky count), B = Nz (kz count)
+  std::vector<double> qzs, qys;   // qzs over ky, qys over kz (src/conjugate.cpp:130-161)
+  double normalization, complexprefactor, realprefactor, sqrtdt;
+  std::uniform_real_distribution<double> real_dist{-0.5, 0.5};
+  cplx *phi;
+  const cplx *mu;
+
+  explicit SpectralUpdater(const SpectralParams &pp) : p(pp) {
+    hx = p.nx / 2 + 1;
+    A = p.ny;
+    B = p.nz;
+    const double dz = 2 * PI_REF / p.period[1];  // Domain::dqz()
+    const double dy = 2 * PI_REF / p.period[2];  // Domain::dqy()
+    qzs.resize(A);
+    qys.resize(B);
+    for (int i = 0; i < A; i++)
+      qzs[i] = (i > A / 2) ? -dz * (A - i) : dz * i;
+    for (int j = 0; j < B; j++)
+      qys[j] = (j > B / 2) ? -dy * (B - j) : dy * j;
+    // ConjugateVolFrac::reset_dt, src/conjugate_volfrac.cpp:72-86
+    double invLcubed = 1.0 / (p.period[0] * p.period[1] * p.period[2]);
+    normalization = 1.0 / ((double)p.nx * p.nz * p.ny);
+    complexprefactor = sqrt(12 * p.temp * p.mobility * invLcubed);
+    realprefactor = sqrt(24 * p.temp * p.mobility * invLcubed);
+    sqrtdt = sqrt(p.dt);
+  }
+  inline size_t idx(int i, int j, int k) const { return ((size_t)j * A + i) * hx + k; }
+  inline double q2_of(int i, int j, int k, double &qx) const {
+    double qz = qzs[i], qy = qys[j];
+    qx = (2 * PI_REF / p.period[0]) * k;
+    return qx * qx + qy * qy + qz * qz;
+  }
+  inline double draw() {
+    double u = real_dist(*p.gen);
+    return u;
+  }
+  // src/conjugate_volfrac.cpp:89-113
+  void complex_update(int i, int j, int k) {
+    double qx, q2 = q2_of(i, j, k, qx);
+    cplx noise(0.0, 0.0);
+    size_t id = idx(i, j, k);
+    if (p.noise_mode == ORC_NOISE_MT19937) {
+      double u1 = draw(), u2 = draw();
+      noise.real(complexprefactor * sqrt(q2) * sqrtdt * u1);
+      noise.imag(complexprefactor * sqrt(q2) * sqrtdt * u2);
+      if (p.record) { (*p.record)[2 * id] = noise.real(); (*p.record)[2 * id + 1] = noise.imag(); }
+    } else if (p.noise_mode == ORC_NOISE_FIELD) {
+      noise = cplx(p.noise[2 * id], p.noise[2 * id + 1]);
+    }
+    phi[id] = (phi[id] - p.mobility * q2 * p.dt *
+                             (mu[id] + p.temp / p.volFH * p.gamma * q2 * phi[id])) *
+                  normalization + noise;
+  }
+  // src/conjugate_volfrac.cpp:116-133
+  void real_update(int i, int j, int k) {
+    double qx, q2 = q2_of(i, j, k, qx);
+    cplx noise(0.0, 0.0);
+    size_t id = idx(i, j, k);
+    if (p.noise_mode == ORC_NOISE_MT19937) {
+      noise = realprefactor * sqrt(q2) * sqrtdt * draw();
+      if (p.record) { (*p.record)[2 * id] = noise.real(); (*p.record)[2 * id + 1] = 0.0; }
+    } else if (p.noise_mode == ORC_NOISE_FIELD) {
+      noise = cplx(p.noise[2 * id], p.noise[2 * id + 1]);
+    }
+    phi[id] = (phi[id] - p.mobility * q2 * p.dt *
+                             (mu[id] + p.temp / p.volFH * p.gamma * q2 * phi[id])) *
+                  normalization + noise;
+  }
+  // src/conjugate_volfrac.cpp:135-141
+  void origin_update() { phi[idx(0, 0, 0)] *= normalization; }
+
+  // src/conjugate.cpp:255-302
+  void single(int nx) {
+    int nz = 0;
+    for (int j = 1; j < B / 2; j++) {
+      complex_update(nz, j, nx);
+      phi[idx(nz, B - j, nx)] = std::conj(phi[idx(nz, j, nx)]);
+    }
+    for (int i = 1; i < A / 2; i++)
+      for (int j = 1; j < B / 2; j++) {
+        complex_update(i, j, nx);
+        phi[idx(A - i, B - j, nx)] = std::conj(phi[idx(i, j, nx)]);
+      }
+    for (int i = A / 2 + 1; i < A; i++)
+      for (int j = 1; j < B / 2; j++) {
+        complex_update(i, j, nx);
+        phi[idx(A - i, B - j, nx)] = std::conj(phi[idx(i, j, nx)]);
+      }
+    nz = A / 2;
+    for (int j = 1; j < B / 2; j++) {
+      complex_update(nz, j, nx);
+      phi[idx(nz, B - j, nx)] = std::conj(phi[idx(nz, j, nx)]);
+    }
+  }
+  // src/conjugate.cpp:304-327
+  void line_single(int nx) {
+    int ny = 0;
+    for (int i = 1; i < A / 2; i++) {
+      complex_update(i, ny, nx);
+      phi[idx(A - i, ny, nx)] = std::conj(phi[idx(i, ny, nx)]);
+    }
+    ny = B / 2;
+    for (int i = A / 2 + 1; i < A; i++) {
+      complex_update(i, ny, nx);
+      phi[idx(A - i, ny, nx)] = std::conj(phi[idx(i, ny, nx)]);
+    }
+  }
+  // src/conjugate.cpp:163-247 (nprocs == 1 branch)
+  void update(cplx *ph, const cplx *m) {
+    phi = ph;
+    mu = m;
+    const int localNx = hx;
+    for (int nz = 0; nz < A; nz++)
+      for (int ny = 0; ny < B; ny++)
+        for (int nx = 1; nx < localNx - 1; nx++) complex_update(nz, ny, nx);
+    for (int nx = 0; nx < localNx; nx += localNx - 1) {
+      single(nx);
+      line_single(nx);
+      if (localNx - 1 == 0) break;
+    }
+    int nx = localNx - 1;
+    real_update(0, B / 2, 0);
+    real_update(0, B / 2, nx);
+    real_update(0, 0, nx);
+    origin_update();
+    int nz = A / 2;
+    real_update(nz, 0, 0);
+    real_update(nz, B / 2, 0);
+    real_update(nz, B / 2, nx);
+    real_update(nz, 0, nx);
+  }
+};
+
+/* utility::make_unique_seed for rank 0, src/utility.cpp:213-235 */
+int make_unique_seed(int baseseed) {
+  std::mt19937 gen;
+  std::uniform_int_distribution<int> integer_dist;
+  gen.seed(baseseed);
+  return integer_dist(gen);
+}
+
+/* ============================================================ system === */
+struct PairStyle {
+  int style;           // ORC_PAIR_*
+  double maxcut;
+  // gridatom
+  std::vector<double> eps, phit, cutsq;
+  // lj/cut, src/pair_lj_cut.cpp:101-117
+  double lj1 = 0, lj2 = 0, ljcutsq = 0;
+};
+
+struct FixDef {
+  enum { CONJ, FH, GRADPHI, DRAG, CHAIN } kind;
+  // conjugate
+  int seed = 0, noise_mode = 0;
+  double mobility = 1, temp = 1, volFH = 1, gamma = 1, chi = 1;
+  std::mt19937 gen;
+  // gradphi
+  int once = 0;
+  // drag / chain
+  std::vector<int> start, end;
+  double drag = 0, r0 = 1, kbond = 0, kbend = 0;
+};
+
+struct Swap {
+  int dim;             // 0,1,2
+  double lo, hi;
+  double shift;        // applied to coordinate `dim` (x/y/z swaps)
+  std::vector<int> sendlist;
+  std::vector<double> pbcz;   // per-entry z shift for the copy swap
+  std::vector<int> labelz;    // per-entry labels for the copy swap
+  int firstrecv = 0;
+};
+
+}  // namespace
+
+struct orc_sys {
+  std::string err;
+  int nx, ny, nz, hx;
+  double boxlo[3], boxhi[3], period[3], sublo[3], subhi[3];
+  int zghost_mode;
+  size_t N, M;
+  std::vector<double> phi, mu, grad[3];
+  std::vector<cplx> ftphi, ftmu;
+  std::vector<double> noisefield;
+  // atoms (src/atom.hpp)
+  int nowned = 0, ngathered = 0, nlocal = 0, nghost = 0, ntypes = 0;
+  std::vector<double> xs, uxs, Fs;
+  std::vector<int> tags, types, images, labels, moltags, owner;
+  // comm
+  double cutghost = 0;
+  std::vector<Swap> swaps;
+  // neighbor
+  bool neigh_set = false;
+  double skin = 0, cutneigh = 0, triggersq = 0;
+  int ago = 0, ncalls = 0;
+  std::vector<double> xholds;
+  int nbinx, nbiny, nbinz, mbinx, mbiny, mbinz, mbinxlo, mbinylo, mbinzlo, mbins;
+  double binsizex, binsizey, binsizez, bininvx, bininvy, bininvz;
+  std::vector<int> binhead, bins, atom2bin;
+  std::set<int> local_atom_indices;
+  std::vector<int> stencil_full, stencil_half;
+  bool need_full = false, need_half = false;
+  // lists (CSR)
+  std::vector<int> g_ilist, g_num, g_off, g_neigh;
+  std::vector<int> h_ilist, h_num, h_off, h_neigh;
+  bool list_overflow = false;
+  // styles
+  std::vector<PairStyle> pairs;
+  std::vector<FixDef> fixes;
+  double dt = 0;
+  int64_t timestep = 0, firststep = 0;
+  int64_t nless = 0, ngreat = 0;
+  bool setup_done = false;
+  bool nan_seen = false;
+
+  double dx() const { return period[0] / nx; }   // src/grid.cpp:187-201
+  double dy() const { return period[1] / ny; }
+  double dz() const { return period[2] / nz; }
+};
+
+namespace {
+
+/* ----------------------------------------------------------- domain --- */
+// Domain::unmap / map, src/domain.cpp:166-193
+void unmap(const orc_sys *s, const double *x, int image, double *u) {
+  int xbox = (image & IMGMASK) - IMGMAX;
+  int ybox = (image >> IMGBITS & IMGMASK) - IMGMAX;
+  int zbox = (image >> IMG2BITS) - IMGMAX;
+  u[0] = x[0] + xbox * s->period[0];
+  u[1] = x[1] + ybox * s->period[1];
+  u[2] = x[2] + zbox * s->period[2];
+}
+void map(const orc_sys *s, const double *u, int image, double *x) {
+  int xbox = (image & IMGMASK) - IMGMAX;
+  int ybox = (image >> IMGBITS & IMGMASK) - IMGMAX;
+  int zbox = (image >> IMG2BITS) - IMGMAX;
+  x[0] = u[0] - xbox * s->period[0];
+  x[1] = u[1] - ybox * s->period[1];
+  x[2] = u[2] - zbox * s->period[2];
+}
+
+// Domain::pbc, src/domain.cpp:102-163
+void pbc(orc_sys *s) {
+  for (int i = 0; i < s->nowned; i++) {
+    double *x = &s->xs[3 * i];
+    int &img = s->images[i];
+    int idim, otherdims;
+    if (x[0] < s->boxlo[0]) {
+      x[0] += s->period[0];
+      idim = img & IMGMASK; otherdims = img ^ idim; idim--; idim &= IMGMASK; img = otherdims | idim;
+    }
+    if (x[0] >= s->boxhi[0]) {
+      x[0] -= s->period[0];
+      x[0] = (x[0] > s->boxlo[0] ? x[0] : s->boxlo[0]);
+      idim = img & IMGMASK; otherdims = img ^ idim; idim++; idim &= IMGMASK; img = otherdims | idim;
+    }
+    if (x[1] < s->boxlo[1]) {
+      x[1] += s->period[1];
+      idim = (img >> IMGBITS) & IMGMASK; otherdims = img ^ (idim << IMGBITS); idim--; idim &= IMGMASK;
+      img = otherdims | (idim << IMGBITS);
+    }
+    if (x[1] >= s->boxhi[1]) {
+      x[1] -= s->period[1];
+      x[1] = (x[1] > s->boxlo[1] ? x[1] : s->boxlo[1]);
+      idim = (img >> IMGBITS) & IMGMASK; otherdims = img ^ (idim << IMGBITS); idim++; idim &= IMGMASK;
+      img = otherdims | (idim << IMGBITS);
+    }
+    if (x[2] < s->boxlo[2]) {
+      x[2] += s->period[2];
+      idim = img >> IMG2BITS; otherdims = img ^ (idim << IMG2BITS); idim--; idim &= IMGMASK;
+      img = otherdims | (idim << IMG2BITS);
+    }
+    if (x[2] >= s->boxhi[2]) {
+      x[2] -= s->period[2];
+      x[2] = (x[2] > s->boxlo[2] ? x[2] : s->boxlo[2]);
+      idim = img >> IMG2BITS; otherdims = img ^ (idim << IMG2BITS); idim++; idim &= IMGMASK;
+      img = otherdims | (idim << IMG2BITS);
+    }
+    unmap(s, x, img, &s->uxs[3 * i]);
+  }
+}
+
+/* ------------------------------------------------------------- comm --- */
+// CommBrick::setup, src/comm_brick.cpp:96-187 (nprocs == 1).  Swap list:
+// [0] copy swap (z), then (periodic mode only) z lo / z hi image swaps, then
+// y lo, y hi, x lo, x hi.  In reference mode the z image swaps are absent,
+// which is what one rank gives (finding F3).
+void comm_setup(orc_sys *s) {
+  s->cutghost = s->cutneigh;
+  s->swaps.clear();
+  Swap z; z.dim = 2; z.shift = 0;
+  int zstart = (int)s->sublo[2];   // int truncation, src/comm_brick.cpp:110-111
+  int zend = (int)s->subhi[2];
+  z.lo = zend - s->cutghost;       // slablo, src/comm_brick.cpp:138-139
+  z.hi = zstart + s->cutghost;     // slabhi, src/comm_brick.cpp:142-143
+  s->swaps.push_back(z);
+  std::vector<int> dims;
+  if (s->zghost_mode == ORC_ZGHOST_PERIODIC) dims.push_back(2);
+  dims.push_back(1);
+  dims.push_back(0);
+  for (int dim : dims) {
+    for (int ineed = 0; ineed < 2; ineed++) {
+      Swap w; w.dim = dim;
+      if (ineed % 2 == 0) {
+        w.lo = -BIG; w.hi = s->boxlo[dim] + s->cutghost; w.shift = s->period[dim];
+      } else {
+        w.lo = s->boxhi[dim] - s->cutghost; w.hi = BIG; w.shift = -s->period[dim];
+      }
+      s->swaps.push_back(w);
+    }
+  }
+}
+
+void grow_gathered(orc_sys *s, int last) {
+  s->xs.resize(3 * (size_t)last);
+  s->Fs.resize(3 * (size_t)last);
+  s->tags.resize(last);
+  s->types.resize(last);
+  s->labels.resize(last);
+  s->owner.resize(last);
+}
+
+// CommBrick::borders, src/comm_brick.cpp:263-426 (sendproc == me branches)
+void borders(orc_sys *s) {
+  s->nlocal = s->ngathered = s->nghost = 0;
+  // copy swap: src/comm_brick.cpp:299-316
+  {
+    Swap &w = s->swaps[0];
+    w.sendlist.clear(); w.pbcz.clear(); w.labelz.clear();
+    int zstart = (int)s->sublo[2], zend = (int)s->subhi[2];
+    const double zprd = s->period[2];
+    int nlocalsend = 0;
+    for (int i = 0; i < s->nowned; i++) {
+      double zc = s->xs[3 * i + 2];
+      if (zc >= zstart && zc <= zend) {
+        w.sendlist.push_back(i); w.pbcz.push_back(0); w.labelz.push_back(LOCAL); nlocalsend++;
+      } else if (zc >= w.lo) {
+        w.sendlist.push_back(i); w.pbcz.push_back(-zprd); w.labelz.push_back(GHOST);
+      } else if (zc <= w.hi) {
+        w.sendlist.push_back(i); w.pbcz.push_back(zprd); w.labelz.push_back(GHOST);
+      }
+    }
+    int first = s->nowned + s->ngathered;
+    int n = (int)w.sendlist.size();
+    grow_gathered(s, first + n);
+    for (int q = 0; q < n; q++) {   // Atom::pack_border_in_z + unpack_border
+      int j = w.sendlist[q];
+      s->xs[3 * (first + q) + 0] = s->xs[3 * j + 0];
+      s->xs[3 * (first + q) + 1] = s->xs[3 * j + 1];
+      s->xs[3 * (first + q) + 2] = s->xs[3 * j + 2] + w.pbcz[q];
+      s->tags[first + q] = s->tags[j];
+      s->types[first + q] = s->types[j];
+      s->labels[first + q] = w.labelz[q];
+      s->owner[first + q] = j;
+    }
+    w.firstrecv = first;
+    s->ngathered += n;
+    s->nlocal += nlocalsend;
+  }
+  // image swaps, src/comm_brick.cpp:337-474 (x/y branch, plus z in periodic mode)
+  size_t iswap = 1;
+  while (iswap < s->swaps.size()) {
+    int dim = s->swaps[iswap].dim;
+    int nfirst = s->nowned, nlast = s->nowned + s->ngathered;
+    for (int ineed = 0; ineed < 2; ineed++, iswap++) {
+      Swap &w = s->swaps[iswap];
+      w.sendlist.clear();
+      for (int i = nfirst; i < nlast; i++) {
+        double c = s->xs[3 * i + dim];
+        if (c >= w.lo && c <= w.hi) w.sendlist.push_back(i);
+      }
+      int first = s->nowned + s->ngathered;
+      int n = (int)w.sendlist.size();
+      grow_gathered(s, first + n);
+      double pbcv[3] = {0, 0, 0};
+      pbcv[dim] = w.shift;
+      for (int q = 0; q < n; q++) {   // Atom::pack_border
+        int j = w.sendlist[q];
+        s->xs[3 * (first + q) + 0] = s->xs[3 * j + 0] + pbcv[0];
+        s->xs[3 * (first + q) + 1] = s->xs[3 * j + 1] + pbcv[1];
+        s->xs[3 * (first + q) + 2] = s->xs[3 * j + 2] + pbcv[2];
+        s->tags[first + q] = s->tags[j];
+        s->types[first + q] = s->types[j];
+        s->labels[first + q] = GHOST;
+        s->owner[first + q] = s->owner[j];
+      }
+      w.firstrecv = first;
+      s->ngathered += n;
+    }
+  }
+  s->nghost = s->ngathered - s->nlocal;
+}
+
+// CommBrick::forward_comm, src/comm_brick.cpp:190-224 (self swaps)
+void forward_comm(orc_sys *s) {
+  for (size_t iswap = 0; iswap < s->swaps.size(); iswap++) {
+    Swap &w = s->swaps[iswap];
+    int n = (int)w.sendlist.size();
+    for (int q = 0; q < n; q++) {
+      int j = w.sendlist[q];
+      double *o = &s->xs[3 * (size_t)(w.firstrecv + q)];
+      if (iswap == 0) {   // Atom::pack_comm_in_z
+        o[0] = s->xs[3 * j + 0];
+        o[1] = s->xs[3 * j + 1];
+        o[2] = s->xs[3 * j + 2] + w.pbcz[q];
+      } else {            // Atom::pack_comm
+        double pbcv[3] = {0, 0, 0};
+        pbcv[w.dim] = w.shift;
+        o[0] = s->xs[3 * j + 0] + pbcv[0];
+        o[1] = s->xs[3 * j + 1] + pbcv[1];
+        o[2] = s->xs[3 * j + 2] + pbcv[2];
+      }
+    }
+  }
+}
+
+// CommBrick::reverse_comm, src/comm_brick.cpp:228-258 + Atom::unpack_reverse
+void reverse_comm(orc_sys *s) {
+  for (int iswap = (int)s->swaps.size() - 1; iswap >= 0; iswap--) {
+    Swap &w = s->swaps[iswap];
+    int n = (int)w.sendlist.size();
+    for (int q = 0; q < n; q++) {
+      int j = w.sendlist[q];
+      const double *b = &s->Fs[3 * (size_t)(w.firstrecv + q)];
+      s->Fs[3 * j + 0] += b[0];
+      s->Fs[3 * j + 1] += b[1];
+      s->Fs[3 * j + 2] += b[2];
+    }
+  }
+}
+
+/* --------------------------------------------------------- neighbor --- */
+// NBin::coord2bin, src/nbin.cpp:61-94 (and NPair::coord2bin, src/npair.cpp:125-158)
+int coord2bin(const orc_sys *s, double x, double y, double z) {
+  int ix, iy, iz;
+  if (x >= s->boxhi[0]) ix = (int)((x - s->boxhi[0]) * s->bininvx) + s->nbinx;
+  else if (x >= s->boxlo[0]) { ix = (int)((x - s->boxlo[0]) * s->bininvx); ix = std::min(ix, s->nbinx - 1); }
+  else ix = (int)((x - s->boxlo[0]) * s->bininvx) - 1;
+  if (y >= s->boxhi[1]) iy = (int)((y - s->boxhi[1]) * s->bininvy) + s->nbiny;
+  else if (y >= s->boxlo[1]) { iy = (int)((y - s->boxlo[1]) * s->bininvy); iy = std::min(iy, s->nbiny - 1); }
+  else iy = (int)((y - s->boxlo[1]) * s->bininvy) - 1;
+  if (z >= s->boxhi[2]) iz = (int)((z - s->boxhi[2]) * s->bininvz) + s->nbinz;
+  else if (z >= s->boxlo[2]) { iz = (int)((z - s->boxlo[2]) * s->bininvz); iz = std::min(iz, s->nbinz - 1); }
+  else iz = (int)((z - s->boxlo[2]) * s->bininvz) - 1;
+  return (iz - s->mbinzlo) * s->mbiny * s->mbinx + (iy - s->mbinylo) * s->mbinx + (ix - s->mbinxlo);
+}
+
+// NBinStandard::setup_bins, src/nbin_standard.cpp:53-175
+void setup_bins(orc_sys *s) {
+  double cutghost = s->cutneigh;
+  double bsubboxlo[3], bsubboxhi[3], bbox[3];
+  for (int d = 0; d < 3; d++) {
+    bsubboxlo[d] = s->sublo[d] - cutghost;
+    bsubboxhi[d] = s->subhi[d] + cutghost;
+    bbox[d] = s->boxhi[d] - s->boxlo[d];
+  }
+  double binsize_optimal = 0.5 * s->cutneigh;
+  if (binsize_optimal == 0.0) binsize_optimal = bbox[0];
+  double binsizeinv = 1.0 / binsize_optimal;
+  s->nbinx = (int)(bbox[0] * binsizeinv);
+  s->nbiny = (int)(bbox[1] * binsizeinv);
+  s->nbinz = (int)(bbox[2] * binsizeinv);
+  if (s->nbinx == 0) s->nbinx = 1;
+  if (s->nbiny == 0) s->nbiny = 1;
+  if (s->nbinz == 0) s->nbinz = 1;
+  s->binsizex = bbox[0] / s->nbinx;
+  s->binsizey = bbox[1] / s->nbiny;
+  s->binsizez = bbox[2] / s->nbinz;
+  s->bininvx = 1.0 / s->binsizex;
+  s->bininvy = 1.0 / s->binsizey;
+  s->bininvz = 1.0 / s->binsizez;
+  if (binsize_optimal * s->bininvx > 100 || binsize_optimal * s->bininvy > 100 ||
+      binsize_optimal * s->bininvz > 100)
+    throw std::runtime_error("Cannot use neighbor bins - box size << cutoff");
+  int mbinxhi, mbinyhi, mbinzhi;
+  double coord;
+  coord = bsubboxlo[0] - SMALL * bbox[0];
+  s->mbinxlo = (int)((coord - s->boxlo[0]) * s->bininvx);
+  if (coord < s->boxlo[0]) s->mbinxlo = s->mbinxlo - 1;
+  coord = bsubboxhi[0] + SMALL * bbox[0];
+  mbinxhi = (int)((coord - s->boxlo[0]) * s->bininvx);
+  coord = bsubboxlo[1] - SMALL * bbox[1];
+  s->mbinylo = (int)((coord - s->boxlo[1]) * s->bininvy);
+  if (coord < s->boxlo[1]) s->mbinylo = s->mbinylo - 1;
+  coord = bsubboxhi[1] + SMALL * bbox[1];
+  mbinyhi = (int)((coord - s->boxlo[1]) * s->bininvy);
+  coord = bsubboxlo[2] - SMALL * bbox[2];
+  s->mbinzlo = (int)((coord - s->boxlo[2]) * s->bininvz);
+  if (coord < s->boxlo[2]) s->mbinzlo = s->mbinzlo - 1;
+  coord = bsubboxhi[2] + SMALL * bbox[2];
+  mbinzhi = (int)((coord - s->boxlo[2]) * s->bininvz);
+  s->mbinxlo -= 1; mbinxhi += 1; s->mbinx = mbinxhi - s->mbinxlo + 1;
+  s->mbinylo -= 1; mbinyhi += 1; s->mbiny = mbinyhi - s->mbinylo + 1;
+  s->mbinzlo -= 1; mbinzhi += 1; s->mbinz = mbinzhi - s->mbinzlo + 1;
+  s->mbins = s->mbinx * s->mbiny * s->mbinz + 1;
+  s->binhead.assign(s->mbins, -1);
+}
+
+// NStencil::create_setup + bin_distance + full/half create,
+// src/nstencil.cpp:101-143, src/nstencil_full_bin_3d.cpp:26-37,
+// src/nstencil_half_bin_3d.cpp:26-38
+double bin_distance(const orc_sys *s, int i, int j, int k) {
+  double delx, dely, delz;
+  if (i > 0) delx = (i - 1) * s->binsizex; else if (i == 0) delx = 0.0; else delx = (i + 1) * s->binsizex;
+  if (j > 0) dely = (j - 1) * s->binsizey; else if (j == 0) dely = 0.0; else dely = (j + 1) * s->binsizey;
+  if (k > 0) delz = (k - 1) * s->binsizez; else if (k == 0) delz = 0.0; else delz = (k + 1) * s->binsizez;
+  return (delx * delx + dely * dely + delz * delz);
+}
+void create_stencils(orc_sys *s) {
+  double cutneighsq = s->cutneigh * s->cutneigh;
+  int sx = (int)(s->cutneigh * s->bininvx); if (sx * s->binsizex < s->cutneigh) sx++;
+  int sy = (int)(s->cutneigh * s->bininvy); if (sy * s->binsizey < s->cutneigh) sy++;
+  int sz = (int)(s->cutneigh * s->bininvz); if (sz * s->binsizez < s->cutneigh) sz++;
+  s->stencil_full.clear();
+  for (int k = -sz; k <= sz; k++)
+    for (int j = -sy; j <= sy; j++)
+      for (int i = -sx; i <= sx; i++)
+        if (bin_distance(s, i, j, k) < cutneighsq)
+          s->stencil_full.push_back(k * s->mbiny * s->mbinx + j * s->mbinx + i);
+  s->stencil_half.clear();
+  for (int k = 0; k <= sz; k++)
+    for (int j = -sy; j <= sy; j++)
+      for (int i = -sx; i <= sx; i++)
+        if (k > 0 || j > 0 || (j == 0 && i > 0))
+          if (bin_distance(s, i, j, k) < cutneighsq)
+            s->stencil_half.push_back(k * s->mbiny * s->mbinx + j * s->mbinx + i);
+}
+
+// NBinStandard::bin_atoms, src/nbin_standard.cpp:181-229
+void bin_atoms(orc_sys *s) {
+  std::fill(s->binhead.begin(), s->binhead.end(), -1);
+  s->local_atom_indices.clear();
+  int nstart = s->nowned, nall = s->nowned + s->ngathered;
+  s->atom2bin.assign(nall - nstart, 0);
+  s->bins.assign(nall - nstart, 0);
+  for (int i = nall - 1; i >= nstart; i--) {
+    if (s->labels[i] == GHOST) {
+      int ibin = coord2bin(s, s->xs[3 * i], s->xs[3 * i + 1], s->xs[3 * i + 2]);
+      s->atom2bin[i - nstart] = ibin;
+      s->bins[i - nstart] = s->binhead[ibin];
+      s->binhead[ibin] = i;
+    } else {
+      s->local_atom_indices.insert(i);
+    }
+  }
+  for (auto rit = s->local_atom_indices.rbegin(); rit != s->local_atom_indices.rend(); ++rit) {
+    int ibin = coord2bin(s, s->xs[3 * *rit], s->xs[3 * *rit + 1], s->xs[3 * *rit + 2]);
+    s->atom2bin[*rit - nstart] = ibin;
+    s->bins[*rit - nstart] = s->binhead[ibin];
+    s->binhead[ibin] = *rit;
+  }
+}
+
+// NPairGridBin::build, src/npair_grid_bin.cpp:22-127 (CSR instead of MyPage)
+void build_gridlist(orc_sys *s) {
+  s->g_ilist.clear(); s->g_num.clear(); s->g_off.clear(); s->g_neigh.clear();
+  const double cutneighsq = s->cutneigh * s->cutneigh;
+  const int nstart = s->nowned;
+  const int nst = (int)s->stencil_full.size();
+  int gridindex = 0;
+  const int local0start = 0;
+  for (int k = 0; k < s->nz; k++) {
+    double ztmp = (k + local0start) * s->dz() + s->boxlo[2];
+    for (int j = 0; j < s->ny; j++) {
+      double ytmp = j * s->dy() + s->boxlo[1];
+      for (int i = 0; i < s->nx; i++) {
+        int n = 0;
+        size_t off = s->g_neigh.size();
+        double xtmp = i * s->dx() + s->boxlo[0];
+        int ibin = coord2bin(s, xtmp, ytmp, ztmp);
+        for (int m = 0; m < nst; m++)
+          for (int ai = s->binhead[ibin + s->stencil_full[m]]; ai >= 0; ai = s->bins[ai - nstart]) {
+            double delx = xtmp - s->xs[3 * ai];
+            double dely = ytmp - s->xs[3 * ai + 1];
+            double delz = ztmp - s->xs[3 * ai + 2];
+            double rsq = delx * delx + dely * dely + delz * delz;
+            if (rsq <= cutneighsq) { s->g_neigh.push_back(ai); n++; }
+          }
+        if (n > 0) {
+          s->g_ilist.push_back(gridindex++);
+          s->g_off.push_back((int)off);
+          s->g_num.push_back(n);
+        } else {
+          gridindex++;
+        }
+        if (n > ONEATOM) s->list_overflow = true;
+      }
+    }
+  }
+}
+
+// NPairHalfBinNewton::build, src/npair_half_bin_newton.cpp:38-124
+void build_halflist(orc_sys *s) {
+  s->h_ilist.clear(); s->h_num.clear(); s->h_off.clear(); s->h_neigh.clear();
+  const double cutneighsq = s->cutneigh * s->cutneigh;
+  const int nstart = s->nowned;
+  const int nst = (int)s->stencil_half.size();
+  for (int i : s->local_atom_indices) {
+    int n = 0;
+    size_t off = s->h_neigh.size();
+    double xtmp = s->xs[3 * i], ytmp = s->xs[3 * i + 1], ztmp = s->xs[3 * i + 2];
+    for (int j = s->bins[i - nstart]; j >= 0; j = s->bins[j - nstart]) {
+      if (!s->local_atom_indices.count(j)) {
+        if (s->xs[3 * j + 2] < ztmp) continue;
+        if (s->xs[3 * j + 2] == ztmp) {
+          if (s->xs[3 * j + 1] < ytmp) continue;
+          if (s->xs[3 * j + 1] == ytmp && s->xs[3 * j] < xtmp) continue;
+        }
+      }
+      double delx = xtmp - s->xs[3 * j];
+      double dely = ytmp - s->xs[3 * j + 1];
+      double delz = ztmp - s->xs[3 * j + 2];
+      double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq <= cutneighsq) { s->h_neigh.push_back(j); n++; }
+    }
+    int ibin = s->atom2bin[i - nstart];
+    for (int k = 0; k < nst; k++)
+      for (int j = s->binhead[ibin + s->stencil_half[k]]; j >= 0; j = s->bins[j - nstart]) {
+        double delx = xtmp - s->xs[3 * j];
+        double dely = ytmp - s->xs[3 * j + 1];
+        double delz = ztmp - s->xs[3 * j + 2];
+        double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq <= cutneighsq) { s->h_neigh.push_back(j); n++; }
+      }
+    s->h_ilist.push_back(i);
+    s->h_off.push_back((int)off);
+    s->h_num.push_back(n);
+    if (n > ONEATOM) s->list_overflow = true;
+  }
+}
+
+// Neighbor::build, src/neighbor.cpp:99-125
+void neighbor_build(orc_sys *s) {
+  s->ago = 0;
+  s->ncalls++;
+  s->xholds.assign(s->xs.begin(), s->xs.begin() + 3 * (size_t)s->nowned);
+  bin_atoms(s);
+  if (s->need_full) build_gridlist(s);
+  if (s->need_half) build_halflist(s);
+}
+
+// Neighbor::decide + check_distance, src/neighbor.cpp:70-97 (delay 0, every 1)
+bool neighbor_decide(orc_sys *s) {
+  s->ago++;
+  int flag = 0;
+  for (int i = 0; i < s->nowned; i++) {
+    double d0 = s->xs[3 * i] - s->xholds[3 * i];
+    double d1 = s->xs[3 * i + 1] - s->xholds[3 * i + 1];
+    double d2 = s->xs[3 * i + 2] - s->xholds[3 * i + 2];
+    double rsq = d0 * d0 + d1 * d1 + d2 * d2;
+    if (rsq > s->triggersq) flag = 1;
+  }
+  return flag != 0;
+}
+
+/* ------------------------------------------------------------ pairs --- */
+// PairGridAtomGaussian::compute (src/pair_gridatom_gaussian.cpp:45-120) and
+// PairGridAtomLJish::compute (src/pair_gridatom_lj_ish.cpp:44-181)
+void pair_gridatom(orc_sys *s, const PairStyle &p, double *energy) {
+  const double Ox = s->sublo[0], Oy = s->sublo[1], Oz = s->sublo[2];
+  const double dv = s->dx() * s->dy() * s->dz();
+  for (size_t ii = 0; ii < s->g_ilist.size(); ii++) {
+    int gridindex = s->g_ilist[ii];
+    int i = gridindex / (s->ny * s->nx);          // reverseFlat: (z, y, x)
+    int j = (gridindex / s->nx) % s->ny;
+    int k = gridindex % s->nx;
+    double xtmp = k * s->dx() + Ox;
+    double ytmp = j * s->dy() + Oy;
+    double ztmp = i * s->dz() + Oz;
+    const int *jlist = &s->g_neigh[s->g_off[ii]];
+    int jnum = s->g_num[ii];
+    for (int ajj = 0; ajj < jnum; ajj++) {
+      int aj = jlist[ajj];
+      int ajtype = s->types[aj];
+      double delx = xtmp - s->xs[3 * aj];
+      double dely = ytmp - s->xs[3 * aj + 1];
+      double delz = ztmp - s->xs[3 * aj + 2];
+      double rsq = delx * delx + dely * dely + delz * delz;
+      bool in = (p.style == ORC_PAIR_GAUSSIAN) ? (rsq <= p.cutsq[ajtype]) : (rsq < p.cutsq[ajtype]);
+      if (!in) continue;
+      rsq /= p.cutsq[ajtype];
+      double expfac;
+      if (p.style == ORC_PAIR_GAUSSIAN)
+        expfac = (exp(-rsq / 2.0) - exp(-0.5)) * exp(0.5) * p.eps[ajtype];
+      else
+        expfac = exp(-1. / (1 - rsq) + 1) * p.eps[ajtype];
+      double delphi = (s->phi[gridindex] - p.phit[ajtype]);
+      s->mu[gridindex] += 2 * delphi * expfac;
+      if (energy) *energy += delphi * delphi * expfac * dv;
+      expfac *= 2 * delphi * dv;
+      s->Fs[3 * aj + 0] -= s->grad[0][gridindex] * expfac;
+      s->Fs[3 * aj + 1] -= s->grad[1][gridindex] * expfac;
+      s->Fs[3 * aj + 2] -= s->grad[2][gridindex] * expfac;
+    }
+  }
+}
+
+// PairLJCut::compute, src/pair_lj_cut.cpp:46-119
+void pair_ljcut(orc_sys *s, const PairStyle &p, double *energy) {
+  for (size_t ii = 0; ii < s->h_ilist.size(); ii++) {
+    int i = s->h_ilist[ii];
+    double xtmp = s->xs[3 * i], ytmp = s->xs[3 * i + 1], ztmp = s->xs[3 * i + 2];
+    const int *jlist = &s->h_neigh[s->h_off[ii]];
+    int jnum = s->h_num[ii];
+    for (int jj = 0; jj < jnum; jj++) {
+      int j = jlist[jj];
+      double delx = xtmp - s->xs[3 * j];
+      double dely = ytmp - s->xs[3 * j + 1];
+      double delz = ztmp - s->xs[3 * j + 2];
+      double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < p.ljcutsq) {
+        double r2inv = 1.0 / rsq;
+        double r6inv = r2inv * r2inv * r2inv;
+        double forcelj = r6inv * (p.lj1 * r6inv - p.lj2);
+        double fpair = forcelj * r2inv;
+        s->Fs[3 * i + 0] += delx * fpair;
+        s->Fs[3 * i + 1] += dely * fpair;
+        s->Fs[3 * i + 2] += delz * fpair;
+        s->Fs[3 * j + 0] -= delx * fpair;
+        s->Fs[3 * j + 1] -= dely * fpair;
+        s->Fs[3 * j + 2] -= delz * fpair;
+        if (energy) *energy += r6inv * (p.lj1 / 12.0 * r6inv - p.lj2 / 6.0);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------- chain fix ---
+ * Build-defined semiflexible chain (the reference's bead-rod dynamics live
+ * in the absent beadrodpmer library, src/fixatom_semiflexible.cpp:144-191).
+ * Harmonic bonds U = k/2 (r - r0)^2 and bending U = kb (1 - t_a . t_b)
+ * between consecutive unit bond vectors, evaluated on unwrapped coordinates
+ * of each chain chunk.  Per-bead force is gathered in a fixed order:
+ * bond(b-1,b), bond(b,b+1), angle centred at b-1, b, b+1.                  */
+inline void bond_force(const double *ua, const double *ub, double k, double r0, double *fa_on_b) {
+  // force on b from bond (a,b)
+  double d0 = ub[0] - ua[0], d1 = ub[1] - ua[1], d2 = ub[2] - ua[2];
+  double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  double c = -k * (r - r0) / r;
+  fa_on_b[0] = c * d0; fa_on_b[1] = c * d1; fa_on_b[2] = c * d2;
+}
+// gradient pieces of cos(theta) for the angle (a, b, c): d1 = b - a, d2 = c - b
+inline void angle_terms(const double *ua, const double *ubb, const double *uc, double ga[3], double gb[3]) {
+  double d1[3] = {ubb[0] - ua[0], ubb[1] - ua[1], ubb[2] - ua[2]};
+  double d2[3] = {uc[0] - ubb[0], uc[1] - ubb[1], uc[2] - ubb[2]};
+  double r1s = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+  double r2s = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+  double r1 = sqrt(r1s), r2 = sqrt(r2s);
+  double dot = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
+  double inv12 = 1.0 / (r1 * r2);
+  double cs = dot * inv12;
+  for (int c = 0; c < 3; c++) {
+    ga[c] = d2[c] * inv12 - cs * d1[c] / r1s;   // d cos / d d1
+    gb[c] = d1[c] * inv12 - cs * d2[c] / r2s;   // d cos / d d2
+  }
+}
+void chain_forces(const double *u, int nb, double k, double r0, double kb, double *f) {
+  for (int b = 0; b < nb; b++) {
+    double acc[3] = {0, 0, 0}, t[3], ga[3], gb[3];
+    if (b >= 1) { bond_force(&u[3 * (b - 1)], &u[3 * b], k, r0, t); for (int c = 0; c < 3; c++) acc[c] += t[c]; }
+    if (b + 1 < nb) { bond_force(&u[3 * (b + 1)], &u[3 * b], k, r0, t); for (int c = 0; c < 3; c++) acc[c] += t[c]; }
+    // angle centred at b-1: atoms (b-2, b-1, b), b is the third atom -> +kb*gb
+    if (b >= 2) { angle_terms(&u[3 * (b - 2)], &u[3 * (b - 1)], &u[3 * b], ga, gb); for (int c = 0; c < 3; c++) acc[c] += kb * gb[c]; }
+    // angle centred at b: atoms (b-1, b, b+1), b is the middle -> kb*(ga - gb)
+    if (b >= 1 && b + 1 < nb) { angle_terms(&u[3 * (b - 1)], &u[3 * b], &u[3 * (b + 1)], ga, gb); for (int c = 0; c < 3; c++) acc[c] += kb * (ga[c] - gb[c]); }
+    // angle centred at b+1: atoms (b, b+1, b+2), b is the first -> -kb*ga
+    if (b + 2 < nb) { angle_terms(&u[3 * b], &u[3 * (b + 1)], &u[3 * (b + 2)], ga, gb); for (int c = 0; c < 3; c++) acc[c] += -kb * ga[c]; }
+    f[3 * b + 0] = acc[0]; f[3 * b + 1] = acc[1]; f[3 * b + 2] = acc[2];
+  }
+}
+double chain_energy(const double *u, int nb, double k, double r0, double kb) {
+  double e = 0;
+  for (int b = 0; b + 1 < nb; b++) {
+    double d0 = u[3 * b + 3] - u[3 * b], d1 = u[3 * b + 4] - u[3 * b + 1], d2 = u[3 * b + 5] - u[3 * b + 2];
+    double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    e += 0.5 * k * (r - r0) * (r - r0);
+  }
+  for (int b = 1; b + 1 < nb; b++) {
+    const double *a = &u[3 * (b - 1)], *m = &u[3 * b], *c = &u[3 * (b + 1)];
+    double d1[3] = {m[0] - a[0], m[1] - a[1], m[2] - a[2]};
+    double d2[3] = {c[0] - m[0], c[1] - m[1], c[2] - m[2]};
+    double r1 = sqrt(d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2]);
+    double r2 = sqrt(d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2]);
+    e += kb * (1 - (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) / (r1 * r2));
+  }
+  return e;
+}
+
+void chain_halfstep(orc_sys *s, const FixDef &f) {
+  for (size_t c = 0; c < f.start.size(); c++) {
+    int st = f.start[c], nb = f.end[c] - f.start[c];
+    std::vector<double> fc(3 * (size_t)nb);
+    chain_forces(&s->uxs[3 * (size_t)st], nb, f.kbond, f.r0, f.kbend, fc.data());
+    for (int b = 0; b < nb; b++) {
+      int j = st + b;
+      for (int q = 0; q < 3; q++) {
+        double tot = s->Fs[3 * j + q] + fc[3 * b + q];
+        s->uxs[3 * j + q] += f.drag * tot * s->dt * 0.5;
+      }
+      map(s, &s->uxs[3 * j], s->images[j], &s->xs[3 * j]);
+    }
+  }
+}
+
+// FixAtomDrag::initial_integrate / final_integrate, src/fixatom_drag.cpp:43-66
+void drag_halfstep(orc_sys *s, const FixDef &f) {
+  for (size_t c = 0; c < f.start.size(); c++)
+    for (int j = f.start[c]; j < f.end[c]; j++)
+      for (int q = 0; q < 3; q++) s->xs[3 * j + q] += f.drag * s->Fs[3 * j + q] * s->dt * 0.5;
+}
+
+/* ----------------------------------------------------------- phases --- */
+void phase_start(orc_sys *s) {
+  for (auto &f : s->fixes) {
+    if (f.kind == FixDef::CONJ ||
+        (f.kind == FixDef::GRADPHI && (!f.once || s->timestep == s->firststep))) {
+      std::vector<cplx> ph;
+      gradphi_impl(s->nx, s->ny, s->nz, s->period, s->phi.data(), s->grad[0].data(),
+                   s->grad[1].data(), s->grad[2].data(), f.kind == FixDef::CONJ ? &ph : nullptr);
+      if (f.kind == FixDef::CONJ) s->ftphi.swap(ph);
+    }
+  }
+}
+
+void phase_neigh(orc_sys *s) {
+  if (!s->neigh_set) return;
+  if (neighbor_decide(s)) {
+    pbc(s);
+    borders(s);
+    neighbor_build(s);
+  } else {
+    forward_comm(s);
+  }
+}
+
+void phase_pair(orc_sys *s) {
+  // Integrate::run, src/integrate.cpp:130-146: the second pass sees the same
+  // inputs as the first (finding F4), so one pass after mu = 0 gives the
+  // state the reference holds after its second pass.
+  std::fill(s->mu.begin(), s->mu.end(), 0.0);
+  std::fill(s->Fs.begin(), s->Fs.end(), 0.0);
+  for (auto &p : s->pairs) {
+    if (p.style == ORC_PAIR_LJCUT) pair_ljcut(s, p, nullptr);
+    else pair_gridatom(s, p, nullptr);
+  }
+  reverse_comm(s);
+}
+
+void phase_integrate(orc_sys *s) {
+  for (auto &f : s->fixes) {
+    if (f.kind == FixDef::DRAG) drag_halfstep(s, f);
+    else if (f.kind == FixDef::CHAIN) chain_halfstep(s, f);
+  }
+}
+
+void phase_postforce(orc_sys *s) {
+  for (auto &f : s->fixes)
+    if (f.kind == FixDef::FH)
+      floryhuggins_impl((int64_t)s->N, s->phi.data(), s->mu.data(), f.temp, f.volFH, f.chi,
+                        s->nless, s->ngreat);
+}
+
+void phase_prefinal(orc_sys *s) {
+  for (auto &f : s->fixes)
+    if (f.kind == FixDef::CONJ) {
+      s->ftmu.resize(s->M);
+      fft3_r2c(s->nx, s->ny, s->nz, s->mu.data(), s->ftmu.data());
+    }
+}
+
+void phase_final(orc_sys *s) {
+  for (auto &f : s->fixes) {
+    if (f.kind == FixDef::CONJ) {
+      SpectralParams p{};
+      p.nx = s->nx; p.ny = s->ny; p.nz = s->nz;
+      for (int d = 0; d < 3; d++) p.period[d] = s->period[d];
+      p.mobility = f.mobility; p.temp = f.temp; p.volFH = f.volFH; p.gamma = f.gamma; p.dt = s->dt;
+      p.noise_mode = f.noise_mode;
+      p.noise = s->noisefield.empty() ? nullptr : s->noisefield.data();
+      if (p.noise_mode == ORC_NOISE_FIELD && !p.noise) p.noise_mode = ORC_NOISE_OFF;
+      p.gen = &f.gen;
+      SpectralUpdater up(p);
+      up.update(s->ftphi.data(), s->ftmu.data());
+    } else if (f.kind == FixDef::DRAG) {
+      drag_halfstep(s, f);
+    } else if (f.kind == FixDef::CHAIN) {
+      chain_halfstep(s, f);
+    }
+  }
+}
+
+void phase_postfinal(orc_sys *s) {
+  for (auto &f : s->fixes)
+    if (f.kind == FixDef::CONJ) fft3_c2r(s->nx, s->ny, s->nz, s->ftphi.data(), s->phi.data());
+}
+
+void run_phase(orc_sys *s, int ph) {
+  switch (ph) {
+    case ORC_PH_START: phase_start(s); break;
+    case ORC_PH_NEIGH: phase_neigh(s); break;
+    case ORC_PH_PAIR: phase_pair(s); break;
+    case ORC_PH_INITIAL: phase_integrate(s); break;
+    case ORC_PH_POSTFORCE: phase_postforce(s); break;
+    case ORC_PH_PREFINAL: phase_prefinal(s); break;
+    case ORC_PH_FINAL: phase_final(s); break;
+    case ORC_PH_POSTFINAL: phase_postfinal(s); break;
+    default: throw std::runtime_error("bad phase");
+  }
+}
+
+template <class F>
+int guard(orc_sys *s, F f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception &e) {
+    if (s) s->err = e.what();
+    return -1;
+  }
+}
+
+}  // namespace
+
+/* ================================================================ C API */
+extern "C" {
+
+int orc_fft_r2c(int nx, int ny, int nz, const double *in, double *out) {
+  return guard(nullptr, [&] { fft3_r2c(nx, ny, nz, in, reinterpret_cast<cplx *>(out)); });
+}
+int orc_fft_c2r(int nx, int ny, int nz, const double *in, double *out) {
+  return guard(nullptr, [&] { fft3_c2r(nx, ny, nz, reinterpret_cast<const cplx *>(in), out); });
+}
+int orc_gradphi(int nx, int ny, int nz, const double period[3], const double *phi, double *gx,
+                double *gy, double *gz) {
+  return guard(nullptr, [&] { gradphi_impl(nx, ny, nz, period, phi, gx, gy, gz, nullptr); });
+}
+int orc_floryhuggins(int64_t n, double *phi, double *mu, double temp, double volFH, double chi,
+                     int64_t *nless, int64_t *ngreat) {
+  int64_t a = 0, b = 0;
+  floryhuggins_impl(n, phi, mu, temp, volFH, chi, a, b);
+  if (nless) *nless = a;
+  if (ngreat) *ngreat = b;
+  return 0;
+}
+int orc_spectral_update(int nx, int ny, int nz, const double period[3], double *phihat,
+                        const double *muhat, double mobility, double temp, double volFH,
+                        double gamma, double dt, int noise_mode, int seed, const double *noise) {
+  return guard(nullptr, [&] {
+    std::mt19937 gen;
+    gen.seed(seed);
+    SpectralParams p{};
+    p.nx = nx; p.ny = ny; p.nz = nz;
+    for (int d = 0; d < 3; d++) p.period[d] = period[d];
+    p.mobility = mobility; p.temp = temp; p.volFH = volFH; p.gamma = gamma; p.dt = dt;
+    p.noise_mode = noise_mode; p.noise = noise; p.gen = &gen;
+    SpectralUpdater up(p);
+    up.update(reinterpret_cast<cplx *>(phihat), reinterpret_cast<const cplx *>(muhat));
+  });
+}
+int orc_noise_field(int nx, int ny, int nz, const double period[3], int seed, double mobility,
+                    double temp, double dt, double *noise) {
+  return guard(nullptr, [&] {
+    const size_t M = (size_t)nz * ny * (nx / 2 + 1);
+    std::vector<cplx> ph(M, cplx(0, 0)), mu(M, cplx(0, 0));
+    std::vector<double> rec(2 * M, 0.0);
+    std::mt19937 gen;
+    gen.seed(seed);
+    SpectralParams p{};
+    p.nx = nx; p.ny = ny; p.nz = nz;
+    for (int d = 0; d < 3; d++) p.period[d] = period[d];
+    p.mobility = mobility; p.temp = temp; p.volFH = 1; p.gamma = 1; p.dt = dt;
+    p.noise_mode = ORC_NOISE_MT19937; p.gen = &gen; p.record = &rec;
+    SpectralUpdater up(p);
+    up.update(ph.data(), mu.data());
+    std::memcpy(noise, rec.data(), sizeof(double) * 2 * M);
+  });
+}
+int orc_mt19937_uniform(int seed, int n, double *out) {
+  std::mt19937 gen;
+  gen.seed(seed);
+  std::uniform_real_distribution<double> d(-0.5, 0.5);
+  for (int i = 0; i < n; i++) out[i] = d(gen);
+  return 0;
+}
+int orc_make_unique_seed(int seed) { return make_unique_seed(seed); }
+int orc_mt19937_raw(int seed, int n, uint32_t *out) {
+  std::mt19937 gen;
+  gen.seed(seed);
+  for (int i = 0; i < n; i++) out[i] = (uint32_t)gen();
+  return 0;
+}
+
+orc_sys *orc_create(int nx, int ny, int nz, const double boxlo[3], const double period[3],
+                    int zghost_mode) {
+  if (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz) || nx < 2 || ny < 2 || nz < 2) return nullptr;
+  orc_sys *s = new orc_sys();
+  s->nx = nx; s->ny = ny; s->nz = nz; s->hx = nx / 2 + 1;
+  for (int d = 0; d < 3; d++) {
+    s->boxlo[d] = boxlo[d];
+    s->period[d] = period[d];
+    s->boxhi[d] = boxlo[d] + period[d];   // src/domain.cpp:71-72
+  }
+  // Domain::set_subbox, src/domain.cpp:78-99 (local0start = 0)
+  double dzz = s->period[2] / s->nz;
+  s->sublo[0] = s->boxlo[0]; s->sublo[1] = s->boxlo[1]; s->sublo[2] = dzz * 0 + s->boxlo[2];
+  s->subhi[0] = s->boxhi[0]; s->subhi[1] = s->boxhi[1]; s->subhi[2] = dzz * (0 + s->nz) + s->boxlo[2];
+  s->zghost_mode = zghost_mode;
+  s->N = (size_t)nx * ny * nz;
+  s->M = (size_t)nz * ny * s->hx;
+  s->phi.assign(s->N, 0.0);
+  s->mu.assign(s->N, 0.0);
+  for (int d = 0; d < 3; d++) s->grad[d].assign(s->N, 0.0);
+  s->ftphi.assign(s->M, cplx(0, 0));
+  s->ftmu.assign(s->M, cplx(0, 0));
+  return s;
+}
+void orc_destroy(orc_sys *s) { delete s; }
+const char *orc_last_error(orc_sys *s) { return s ? s->err.c_str() : ""; }
+
+int orc_set_field(orc_sys *s, int which, const double *data) {
+  std::vector<double> *f = which == ORC_PHI ? &s->phi : which == ORC_MU ? &s->mu
+                           : (which >= 2 && which <= 4) ? &s->grad[which - 2] : nullptr;
+  if (!f) return -1;
+  std::memcpy(f->data(), data, sizeof(double) * s->N);
+  return 0;
+}
+int orc_get_field(orc_sys *s, int which, double *data) {
+  const std::vector<double> *f = which == ORC_PHI ? &s->phi : which == ORC_MU ? &s->mu
+                                 : (which >= 2 && which <= 4) ? &s->grad[which - 2] : nullptr;
+  if (!f) return -1;
+  std::memcpy(data, f->data(), sizeof(double) * s->N);
+  return 0;
+}
+
+// Atom::setup + ReadAtoms + Atom::add_atom + check_tags_and_types,
+// src/atom.cpp:26-110, 140-252
+int orc_set_atoms(orc_sys *s, int n, const double *xyz, const int *type, const int *tag,
+                  const int *mol) {
+  return guard(s, [&] {
+    s->nowned = n;
+    s->ngathered = s->nlocal = s->nghost = 0;
+    s->xs.assign(xyz, xyz + 3 * (size_t)n);
+    s->uxs.assign(3 * (size_t)n, 0.0);
+    s->Fs.assign(3 * (size_t)n, 0.0);
+    s->types.assign(type, type + n);
+    s->tags.resize(n);
+    s->moltags.assign(n, 0);
+    s->labels.assign(n, OWNED);
+    s->owner.resize(n);
+    int img = (IMGMAX << IMG2BITS) | (IMGMAX << IMGBITS) | IMGMAX;   // Domain::set_image
+    s->images.assign(n, img);
+    int tmax = -1;
+    for (int i = 0; i < n; i++) {
+      s->tags[i] = tag ? tag[i] : i;
+      if (mol) s->moltags[i] = mol[i];
+      s->owner[i] = i;
+      tmax = std::max(tmax, type[i]);
+    }
+    std::set<int> tset(s->tags.begin(), s->tags.end());
+    if ((int)tset.size() != n) throw std::runtime_error("duplicate IDs found on processor(s).");
+    s->ntypes = tmax + 1;
+  });
+}
+int orc_get_atoms(orc_sys *s, double *xyz, double *uxyz, int *image) {
+  if (xyz) std::memcpy(xyz, s->xs.data(), sizeof(double) * 3 * s->nowned);
+  if (uxyz) std::memcpy(uxyz, s->uxs.data(), sizeof(double) * 3 * s->nowned);
+  if (image) std::memcpy(image, s->images.data(), sizeof(int) * s->nowned);
+  return 0;
+}
+int orc_get_forces(orc_sys *s, double *f) {
+  std::memcpy(f, s->Fs.data(), sizeof(double) * 3 * s->nowned);
+  return 0;
+}
+
+// Pair settings (src/pair_gridatom_gaussian.cpp:123-140,
+// src/pair_gridatom_lj_ish.cpp:184-204, src/pair_lj_cut.cpp:101-117)
+int orc_add_pair(orc_sys *s, int style, double a, double b, double c) {
+  if (s->ntypes <= 0) { s->err = "Cannot create pair style before atoms are made."; return -1; }
+  PairStyle p;
+  p.style = style;
+  if (style == ORC_PAIR_LJCUT) {
+    p.maxcut = a;
+    p.lj1 = 48 * b * pow(c, 12.0);
+    p.lj2 = 24 * b * pow(c, 6.0);
+    p.ljcutsq = a * a;
+    s->need_half = true;
+  } else {
+    p.maxcut = a;
+    p.eps.assign(s->ntypes, 1.0);
+    p.phit.assign(s->ntypes, 0.1);
+    p.cutsq.assign(s->ntypes, a * a);
+    s->need_full = true;
+  }
+  s->pairs.push_back(p);
+  return (int)s->pairs.size() - 1;
+}
+// pairCoeffs (src/pair_gridatom_gaussian.cpp:143-181)
+int orc_pair_coeff(orc_sys *s, int ipair, int type, double eps, double phit, double nucwidth) {
+  if (ipair < 0 || ipair >= (int)s->pairs.size()) return -1;
+  PairStyle &p = s->pairs[ipair];
+  if (type < 0 || type >= (int)p.eps.size()) { s->err = "atom type out of range."; return -1; }
+  if (!std::isnan(eps)) p.eps[type] = eps;
+  if (!std::isnan(phit)) p.phit[type] = phit;
+  if (!std::isnan(nucwidth)) {
+    if (nucwidth > p.maxcut) { s->err = "nucwidth cannot be larger than maximum cut."; return -1; }
+    p.cutsq[type] = nucwidth * nucwidth;
+  }
+  return 0;
+}
+// FixGridConjugate::init + ConjugateVolFrac::readCoeffs,
+// src/fixgrid_conjugate.cpp:21-45, src/conjugate_volfrac.cpp:29-69
+int orc_add_fix_conjugate(orc_sys *s, int seed, double mobility, double temp, double volFH,
+                          double gamma, int noise_mode) {
+  FixDef f;
+  f.kind = FixDef::CONJ;
+  f.seed = make_unique_seed(seed);
+  f.mobility = mobility; f.temp = temp; f.volFH = volFH; f.gamma = gamma;
+  f.noise_mode = noise_mode;
+  f.gen.seed(f.seed);
+  s->fixes.push_back(f);
+  return 0;
+}
+int orc_set_noise(orc_sys *s, const double *noise) {
+  if (!noise) { s->noisefield.clear(); return 0; }
+  s->noisefield.assign(noise, noise + 2 * s->M);
+  return 0;
+}
+int orc_add_fix_floryhuggins(orc_sys *s, double temp, double volFH, double chi) {
+  FixDef f;
+  f.kind = FixDef::FH;
+  f.temp = temp; f.volFH = volFH; f.chi = chi;
+  s->fixes.push_back(f);
+  return 0;
+}
+int orc_add_fix_gradphi(orc_sys *s, int once) {
+  FixDef f;
+  f.kind = FixDef::GRADPHI;
+  f.once = once;
+  s->fixes.push_back(f);
+  return 0;
+}
+int orc_add_fix_drag(orc_sys *s, int nchunk, const int *start, const int *end, double drag) {
+  FixDef f;
+  f.kind = FixDef::DRAG;
+  f.start.assign(start, start + nchunk);
+  f.end.assign(end, end + nchunk);
+  f.drag = drag;
+  s->fixes.push_back(f);
+  return 0;
+}
+int orc_add_fix_chain(orc_sys *s, int nchunk, const int *start, const int *end, double r0,
+                      double kbond, double kbend, double drag) {
+  FixDef f;
+  f.kind = FixDef::CHAIN;
+  f.start.assign(start, start + nchunk);
+  f.end.assign(end, end + nchunk);
+  f.r0 = r0; f.kbond = kbond; f.kbend = kbend; f.drag = drag;
+  s->fixes.push_back(f);
+  return 0;
+}
+// Neighbor::setup, src/neighbor.cpp:39-68, then CommBrick::setup
+int orc_neighbor(orc_sys *s, double skin) {
+  return guard(s, [&] {
+    s->skin = skin;
+    double cutoff = 0.0;
+    for (auto &p : s->pairs) cutoff = std::max(cutoff, p.maxcut);
+    s->triggersq = 0.25 * skin * skin;
+    if (cutoff > 0.0) s->cutneigh = skin + cutoff;
+    setup_bins(s);
+    create_stencils(s);
+    comm_setup(s);
+    s->neigh_set = true;
+  });
+}
+// Integrate::setup, src/integrate.cpp:33-88
+int orc_setup(orc_sys *s, double dt) {
+  return guard(s, [&] {
+    s->dt = dt;
+    if (s->neigh_set) {
+      pbc(s);
+      borders(s);
+      neighbor_build(s);
+    } else {
+      for (int i = 0; i < s->nowned; i++) unmap(s, &s->xs[3 * i], s->images[i], &s->uxs[3 * i]);
+    }
+    std::fill(s->Fs.begin(), s->Fs.end(), 0.0);
+    s->setup_done = true;
+  });
+}
+// Integrate::run, src/integrate.cpp:90-219
+int orc_run(orc_sys *s, int nsteps) {
+  return guard(s, [&] {
+    for (int i = 0; i < nsteps; i++) {
+      s->timestep++;
+      for (int ph = ORC_PH_START; ph <= ORC_PH_POSTFINAL; ph++) run_phase(s, ph);
+      if (std::isnan(s->phi[0])) { s->nan_seen = true; throw std::runtime_error("NAN encountered in phi."); }
+    }
+  });
+}
+int orc_phase(orc_sys *s, int phase) {
+  return guard(s, [&] {
+    if (phase == ORC_PH_START) s->timestep++;
+    run_phase(s, phase);
+  });
+}
+int orc_counts(orc_sys *s, int64_t *o) {
+  o[0] = s->nowned; o[1] = s->ngathered; o[2] = s->nlocal; o[3] = s->nghost;
+  o[4] = s->ncalls; o[5] = s->nless; o[6] = s->ngreat; o[7] = s->list_overflow ? 1 : 0;
+  return 0;
+}
+static int copy_list(const std::vector<int> &il, const std::vector<int> &num,
+                     const std::vector<int> &off, const std::vector<int> &nb, int64_t cap,
+                     int *ilist, int *numneigh, int *neigh, int64_t *inum, int64_t *nentries) {
+  int64_t tot = 0;
+  for (int v : num) tot += v;
+  *inum = (int64_t)il.size();
+  *nentries = tot;
+  if (!ilist) return 0;
+  if ((int64_t)il.size() > cap || tot > cap) return -1;
+  int64_t w = 0;
+  for (size_t i = 0; i < il.size(); i++) {
+    ilist[i] = il[i];
+    numneigh[i] = num[i];
+    for (int q = 0; q < num[i]; q++) neigh[w++] = nb[off[i] + q];
+  }
+  return 0;
+}
+int orc_gridlist(orc_sys *s, int64_t cap, int *ilist, int *numneigh, int *neigh, int64_t *inum,
+                 int64_t *nentries) {
+  return copy_list(s->g_ilist, s->g_num, s->g_off, s->g_neigh, cap, ilist, numneigh, neigh, inum,
+                   nentries);
+}
+int orc_halflist(orc_sys *s, int64_t cap, int *ilist, int *numneigh, int *neigh, int64_t *inum,
+                 int64_t *nentries) {
+  return copy_list(s->h_ilist, s->h_num, s->h_off, s->h_neigh, cap, ilist, numneigh, neigh, inum,
+                   nentries);
+}
+int orc_gathered(orc_sys *s, double *xyz, int *tag, int *label, int *owner) {
+  int n = s->ngathered, f = s->nowned;
+  if (xyz) std::memcpy(xyz, &s->xs[3 * (size_t)f], sizeof(double) * 3 * n);
+  if (tag) std::memcpy(tag, &s->tags[f], sizeof(int) * n);
+  if (label) std::memcpy(label, &s->labels[f], sizeof(int) * n);
+  if (owner) std::memcpy(owner, &s->owner[f], sizeof(int) * n);
+  return n;
+}
+int orc_bins(orc_sys *s, int *info, int *binhead, int *bins, int *atom2bin) {
+  if (info) {
+    info[0] = s->nbinx; info[1] = s->nbiny; info[2] = s->nbinz;
+    info[3] = s->mbinx; info[4] = s->mbiny; info[5] = s->mbinz;
+    info[6] = s->mbinxlo; info[7] = s->mbinylo; info[8] = s->mbinzlo; info[9] = s->mbins;
+    info[10] = (int)s->stencil_full.size(); info[11] = (int)s->stencil_half.size();
+  }
+  if (binhead) std::memcpy(binhead, s->binhead.data(), sizeof(int) * s->binhead.size());
+  if (bins) std::memcpy(bins, s->bins.data(), sizeof(int) * s->bins.size());
+  if (atom2bin) std::memcpy(atom2bin, s->atom2bin.data(), sizeof(int) * s->atom2bin.size());
+  return 0;
+}
+// Energies defined by this build (the reference computes none, finding F11):
+// out[0] Flory-Huggins + square-gradient free energy, out[1] grid-atom
+// coupling, out[2] lj/cut, out[3] chain bond + bending.
+int orc_energy(orc_sys *s, double *out) {
+  return guard(s, [&] {
+    const double dv = s->dx() * s->dy() * s->dz();
+    double efh = 0;
+    for (auto &f : s->fixes) {
+      if (f.kind == FixDef::FH) {
+        double e = 0;
+        for (size_t i = 0; i < s->N; i++) {
+          double p = s->phi[i];
+          if (p <= 0) p = -p; else if (p >= 1) p = 1 - (p - 1);
+          e += p * log(p) + (1 - p) * log(1 - p) + f.chi * p * (1 - p);
+        }
+        efh += f.temp / f.volFH * e * dv;
+      }
+      if (f.kind == FixDef::CONJ) {
+        double e = 0;
+        for (size_t i = 0; i < s->N; i++)
+          e += s->grad[0][i] * s->grad[0][i] + s->grad[1][i] * s->grad[1][i] + s->grad[2][i] * s->grad[2][i];
+        efh += 0.5 * f.temp / f.volFH * f.gamma * e * dv;
+      }
+    }
+    double ega = 0, elj = 0;
+    std::vector<double> mu_save = s->mu, f_save = s->Fs;
+    for (auto &p : s->pairs) {
+      if (p.style == ORC_PAIR_LJCUT) pair_ljcut(s, p, &elj);
+      else pair_gridatom(s, p, &ega);
+    }
+    s->mu.swap(mu_save);
+    s->Fs.swap(f_save);
+    double ech = 0;
+    for (auto &f : s->fixes)
+      if (f.kind == FixDef::CHAIN)
+        for (size_t c = 0; c < f.start.size(); c++)
+          ech += chain_energy(&s->uxs[3 * (size_t)f.start[c]], f.end[c] - f.start[c], f.kbond, f.r0, f.kbend);
+    out[0] = efh; out[1] = ega; out[2] = elj; out[3] = ech;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+/*
+ * oracle.h -- C API of the CPU FP64 oracle for PHAFD's per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a plain, single-threaded C++
+ * restatement of the reference's CPU algorithm (one MPI rank holding the whole
+ * box).  It is the checker that the HIP path is compared against: only the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark may load it.
+ * Nothing in the product links or calls it.
+ *
+ * Layouts used at this boundary (identical to the product C-ABI):
+ *   real fields   : double[Nz][Ny][Nx], x fastest (flat = (z*Ny+y)*Nx+x, the
+ *                   order of fftw_arr reverseFlat used by the reference's
+ *                   src/npair_grid_bin.cpp:71-116)
+ *   spectra       : complex double[Nz][Ny][Nx/2+1] interleaved (re,im), kx
+ *                   fastest (numpy rfftn layout)
+ *   atoms         : double[n][3] (x,y,z per atom, like Eigen::Matrix3Xd
+ *                   column-major storage in src/atom.hpp:38)
+ */
+#ifndef PHAFD_ORACLE_H
+#define PHAFD_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_sys orc_sys;
+
+enum { ORC_PHI = 0, ORC_MU = 1, ORC_GRADX = 2, ORC_GRADY = 3, ORC_GRADZ = 4 };
+enum { ORC_ZGHOST_REFERENCE = 0, ORC_ZGHOST_PERIODIC = 1 };
+enum { ORC_NOISE_OFF = 0, ORC_NOISE_MT19937 = 1, ORC_NOISE_FIELD = 2 };
+enum { ORC_PAIR_GAUSSIAN = 0, ORC_PAIR_LJISH = 1, ORC_PAIR_LJCUT = 2 };
+
+/* ---- stateless kernels (unit-level parity) ---- */
+int orc_fft_r2c(int nx, int ny, int nz, const double *in, double *out);
+int orc_fft_c2r(int nx, int ny, int nz, const double *in, double *out);
+int orc_gradphi(int nx, int ny, int nz, const double period[3],
+                const double *phi, double *gx, double *gy, double *gz);
+int orc_floryhuggins(int64_t n, double *phi, double *mu, double temp,
+                     double volFH, double chi, int64_t *nless, int64_t *ngreat);
+int orc_spectral_update(int nx, int ny, int nz, const double period[3],
+                        double *phihat, const double *muhat, double mobility,
+                        double temp, double volFH, double gamma, double dt,
+                        int noise_mode, int seed, const double *noise);
+int orc_noise_field(int nx, int ny, int nz, const double period[3], int seed,
+                    double mobility, double temp, double dt, double *noise);
+int orc_mt19937_uniform(int seed, int n, double *out);
+int orc_make_unique_seed(int seed);
+int orc_mt19937_raw(int seed, int n, uint32_t *out);
+
+/* ---- whole-system state machine (mirrors Integrate::setup / run) ---- */
+orc_sys *orc_create(int nx, int ny, int nz, const double boxlo[3],
+                    const double period[3], int zghost_mode);
+void orc_destroy(orc_sys *);
+const char *orc_last_error(orc_sys *);
+int orc_set_field(orc_sys *, int which, const double *data);
+int orc_get_field(orc_sys *, int which, double *data);
+int orc_set_atoms(orc_sys *, int n, const double *xyz, const int *type,
+                  const int *tag, const int *mol);
+int orc_get_atoms(orc_sys *, double *xyz, double *uxyz, int *image);
+int orc_get_forces(orc_sys *, double *f);
+int orc_add_pair(orc_sys *, int style, double a, double b, double c);
+int orc_pair_coeff(orc_sys *, int ipair, int type, double eps, double phit,
+                   double nucwidth);
+int orc_add_fix_conjugate(orc_sys *, int seed, double mobility, double temp,
+                          double volFH, double gamma, int noise_mode);
+int orc_set_noise(orc_sys *, const double *noise);
+int orc_add_fix_floryhuggins(orc_sys *, double temp, double volFH, double chi);
+int orc_add_fix_gradphi(orc_sys *, int once);
+int orc_add_fix_drag(orc_sys *, int nchunk, const int *start, const int *end,
+                     double drag);
+int orc_add_fix_chain(orc_sys *, int nchunk, const int *start, const int *end,
+                      double r0, double kbond, double kbend, double drag);
+int orc_neighbor(orc_sys *, double skin);
+int orc_setup(orc_sys *, double dt);
+int orc_run(orc_sys *, int nsteps);
+int orc_counts(orc_sys *, int64_t *out8);
+int orc_gridlist(orc_sys *, int64_t cap, int *ilist, int *numneigh, int *neigh,
+                 int64_t *inum, int64_t *nentries);
+int orc_halflist(orc_sys *, int64_t cap, int *ilist, int *numneigh, int *neigh,
+                 int64_t *inum, int64_t *nentries);
+int orc_gathered(orc_sys *, double *xyz, int *tag, int *label, int *owner);
+int orc_bins(orc_sys *, int *info, int *binhead, int *bins, int *atom2bin);
+int orc_energy(orc_sys *, double *out4);
+
+/* ---- individual hook phases (for per-hook parity) ---- */
+enum {
+  ORC_PH_START = 0,    /* fixes start_of_step (gradphi)            */
+  ORC_PH_NEIGH = 1,    /* decide -> pbc+borders+build | forward     */
+  ORC_PH_PAIR = 2,     /* mu=0, F=0, pairs compute, reverse_comm    */
+  ORC_PH_INITIAL = 3,  /* fixes initial_integrate                   */
+  ORC_PH_POSTFORCE = 4,/* fixes post_force (Flory-Huggins)          */
+  ORC_PH_PREFINAL = 5, /* r2c mu                                    */
+  ORC_PH_FINAL = 6,    /* spectral update, integrator second half   */
+  ORC_PH_POSTFINAL = 7 /* c2r phi                                   */
+};
+int orc_phase(orc_sys *, int phase);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,402 @@
+"""ctypes binding of libphafd_hip.so (include/phafd_hip.h, include/phafd_host.h).
+
+`System` mirrors the reference's script-level surface (domain / grid_style /
+atom_populate / pair + pairCoeffs / fix / neighbor / run, src/input.cpp:48-299)
+on one MI355X.  The library must have been built (phafd_amd/build.py); there
+is no CPU fallback: a missing library raises immediately.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphafd_hip.so")
+
+PHI, MU, GRADX, GRADY, GRADZ = 0, 1, 2, 3, 4
+PHIHAT, MUHAT = 0, 1
+ZGHOST_REFERENCE, ZGHOST_PERIODIC = 0, 1
+NOISE_OFF, NOISE_FIELD = 0, 2
+PAIR_GAUSSIAN, PAIR_LJISH = 0, 1
+FP64, FP32 = 0, 1
+PH_START, PH_NEIGH, PH_PAIR, PH_INITIAL, PH_POSTFORCE, PH_PREFINAL, PH_FINAL, PH_POSTFINAL = range(8)
+ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "NaN in phi", -4: "runtime error", -5: "list overflow"}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_int64)
+_lib = None
+
+
+class PhafdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("boxlo", ctypes.c_double * 3), ("period", ctypes.c_double * 3),
+                ("zghost", ctypes.c_int), ("precision", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+def lib():
+    """Load libphafd_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with phafd_amd/build.py "
+                           "(__graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    V = ctypes.c_void_p
+    sig = {
+        "phafd_hip_create": ([ctypes.POINTER(Desc), ctypes.POINTER(V)], ctypes.c_int),
+        "phafd_hip_destroy": ([V], ctypes.c_int),
+        "phafd_hip_last_error": ([V], ctypes.c_char_p),
+        "phafd_hip_set_field": ([V, ctypes.c_int, _dp], ctypes.c_int),
+        "phafd_hip_get_field": ([V, ctypes.c_int, _dp], ctypes.c_int),
+        "phafd_hip_get_spectrum": ([V, ctypes.c_int, _dp], ctypes.c_int),
+        "phafd_hip_set_atoms": ([V, ctypes.c_int, _dp, _ip, _ip, _ip], ctypes.c_int),
+        "phafd_hip_get_atoms": ([V, _dp, _dp, _ip], ctypes.c_int),
+        "phafd_hip_get_forces": ([V, _dp], ctypes.c_int),
+        "phafd_hip_pair_gridatom": ([V, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+        "phafd_hip_pair_coeff": ([V, ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 3, ctypes.c_int),
+        "phafd_hip_pair_ljcut": ([V] + [ctypes.c_double] * 3, ctypes.c_int),
+        "phafd_hip_fix_conjugate": ([V, ctypes.c_int] + [ctypes.c_double] * 4 + [ctypes.c_int], ctypes.c_int),
+        "phafd_hip_set_noise": ([V, _dp], ctypes.c_int),
+        "phafd_hip_fix_floryhuggins": ([V] + [ctypes.c_double] * 3, ctypes.c_int),
+        "phafd_hip_fix_gradphi": ([V, ctypes.c_int], ctypes.c_int),
+        "phafd_hip_fix_drag": ([V, ctypes.c_int, _ip, _ip, ctypes.c_double], ctypes.c_int),
+        "phafd_hip_fix_chain": ([V, ctypes.c_int, _ip, _ip] + [ctypes.c_double] * 4, ctypes.c_int),
+        "phafd_hip_neighbor": ([V, ctypes.c_double], ctypes.c_int),
+        "phafd_hip_setup": ([V, ctypes.c_double], ctypes.c_int),
+        "phafd_hip_run": ([V, ctypes.c_int], ctypes.c_int),
+        "phafd_hip_phase": ([V, ctypes.c_int], ctypes.c_int),
+        "phafd_hip_counts": ([V, _lp], ctypes.c_int),
+        "phafd_hip_gridlist": ([V, ctypes.c_int64, _ip, _ip, _ip, _lp, _lp], ctypes.c_int),
+        "phafd_hip_halflist": ([V, ctypes.c_int64, _ip, _ip, _ip, _lp, _lp], ctypes.c_int),
+        "phafd_hip_gathered": ([V, _dp, _ip, _ip, _ip], ctypes.c_int),
+        "phafd_hip_bins": ([V, _ip, _ip, _ip, _ip], ctypes.c_int),
+        "phafd_hip_energy": ([V, _dp], ctypes.c_int),
+        "phafd_hip_timers": ([V, _dp, ctypes.c_int], ctypes.c_int),
+        "phafd_hip_probe": ([V, ctypes.c_int, _dp], ctypes.c_int),
+        "phafd_hip_fft_r2c": ([ctypes.c_int] * 3 + [_dp, _dp], ctypes.c_int),
+        "phafd_hip_fft_c2r": ([ctypes.c_int] * 3 + [_dp, _dp], ctypes.c_int),
+        "phafd_hip_gradphi": ([ctypes.c_int] * 3 + [_dp] * 5, ctypes.c_int),
+        "phafd_hip_floryhuggins": ([ctypes.c_int64, _dp, _dp] + [ctypes.c_double] * 3 + [_lp, _lp], ctypes.c_int),
+        "phafd_hip_spectral_update": ([ctypes.c_int] * 3 + [_dp, _dp, _dp] + [ctypes.c_double] * 5 + [_dp],
+                                      ctypes.c_int),
+        "phafd_make_unique_seed": ([ctypes.c_int], ctypes.c_int),
+        "phafd_ic_noisy_constant": ([ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp],
+                                    ctypes.c_int),
+        "phafd_ic_flat_interface": ([ctypes.c_int] * 3 + [ctypes.c_double] * 5 + [ctypes.c_int, _dp],
+                                    ctypes.c_int),
+        "phafd_ic_sphere": ([ctypes.c_int] * 3 + [_dp, _dp] + [ctypes.c_double] * 4 + [_dp], ctypes.c_int),
+        "phafd_noise_create": ([ctypes.c_int], V),
+        "phafd_noise_destroy": ([V], None),
+        "phafd_noise_next": ([V] + [ctypes.c_int] * 3 + [_dp] + [ctypes.c_double] * 3 + [_dp], ctypes.c_int),
+        "phafd_ic_random_chains": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                    ctypes.c_uint64, _dp, _ip], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names of every entry point the headers declare (checked by the CPU tests)."""
+    names = []
+    for h in ("phafd_hip.h", "phafd_host.h"):
+        path = os.path.join(os.path.dirname(HERE), "include", h)
+        import re
+        txt = open(path).read()
+        names += re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\*?\s*(phafd_[a-z0-9_]+)\s*\(", txt, re.M)
+    return sorted(set(names))
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+def _c3(v):
+    return (ctypes.c_double * 3)(*[float(x) for x in v])
+
+
+def _check(rc, ctx=None):
+    if rc < 0:
+        msg = lib().phafd_hip_last_error(ctx)
+        raise PhafdError(rc, (msg or b"").decode())
+    return rc
+
+
+# --------------------------------------------------------------- host helpers
+def make_unique_seed(seed):
+    return lib().phafd_make_unique_seed(seed)
+
+
+def ic_noisy_constant(shape, average, variance, seed):
+    """grid_populate constant concentration <average> <variance> <seed>."""
+    out = np.empty(int(np.prod(shape)))
+    _check(lib().phafd_ic_noisy_constant(out.size, average, variance, seed, _d(out)))
+    return out.reshape(shape)
+
+
+def ic_random_chains(nchains, nbeads, bond, minsep, boxlo, period, seed):
+    n = nchains * nbeads
+    xyz = np.zeros((n, 3))
+    mol = np.zeros(n, dtype=np.int32)
+    lo = np.ascontiguousarray(boxlo, dtype=np.float64)
+    pe = np.ascontiguousarray(period, dtype=np.float64)
+    _check(lib().phafd_ic_random_chains(nchains, nbeads, bond, minsep, _d(lo), _d(pe), seed, _d(xyz), _i(mol)))
+    return xyz, mol
+
+
+class NoiseGen:
+    """Reference-order mt19937 thermal noise of grid/conjugate/volfrac."""
+
+    def __init__(self, seed):
+        self.h = lib().phafd_noise_create(seed)
+
+    def next(self, nx, ny, nz, period, mobility, temp, dt):
+        out = np.zeros((nz, ny, nx // 2 + 1), dtype=np.complex128)
+        pe = np.ascontiguousarray(period, dtype=np.float64)
+        _check(lib().phafd_noise_next(self.h, nx, ny, nz, _d(pe), mobility, temp, dt,
+                                      out.ctypes.data_as(_dp)))
+        return out
+
+    def __del__(self):
+        try:
+            lib().phafd_noise_destroy(self.h)
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ stateless kernels
+def fft_r2c(phi):
+    nz, ny, nx = phi.shape
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    out = np.zeros((nz, ny, nx // 2 + 1), dtype=np.complex128)
+    _check(lib().phafd_hip_fft_r2c(nx, ny, nz, _d(phi), out.ctypes.data_as(_dp)))
+    return out
+
+
+def fft_c2r(spec, nx):
+    nz, ny, hx = spec.shape
+    spec = np.ascontiguousarray(spec, dtype=np.complex128)
+    out = np.zeros((nz, ny, nx))
+    _check(lib().phafd_hip_fft_c2r(nx, ny, nz, spec.ctypes.data_as(_dp), _d(out)))
+    return out
+
+
+def gradphi(phi, period):
+    nz, ny, nx = phi.shape
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    g = [np.zeros_like(phi) for _ in range(3)]
+    pe = np.ascontiguousarray(period, dtype=np.float64)
+    _check(lib().phafd_hip_gradphi(nx, ny, nz, _d(pe), _d(phi), _d(g[0]), _d(g[1]), _d(g[2])))
+    return g
+
+
+def floryhuggins(phi, mu, temp, volFH, chi):
+    phi = np.array(phi, dtype=np.float64, order="C")
+    mu = np.array(mu, dtype=np.float64, order="C")
+    a = ctypes.c_int64(0)
+    b = ctypes.c_int64(0)
+    _check(lib().phafd_hip_floryhuggins(phi.size, _d(phi), _d(mu), temp, volFH, chi,
+                                        ctypes.byref(a), ctypes.byref(b)))
+    return phi, mu, a.value, b.value
+
+
+def spectral_update(phihat, muhat, nx, period, mobility, temp, volFH, gamma, dt, noise=None):
+    nz, ny, hx = phihat.shape
+    ph = np.array(phihat, dtype=np.complex128, order="C")
+    mu = np.ascontiguousarray(muhat, dtype=np.complex128)
+    pe = np.ascontiguousarray(period, dtype=np.float64)
+    nzz = None if noise is None else np.ascontiguousarray(noise, dtype=np.complex128)
+    _check(lib().phafd_hip_spectral_update(nx, ny, nz, _d(pe), ph.ctypes.data_as(_dp), mu.ctypes.data_as(_dp),
+                                           mobility, temp, volFH, gamma, dt,
+                                           None if nzz is None else nzz.ctypes.data_as(_dp)))
+    return ph
+
+
+# ------------------------------------------------------------------- system
+class System:
+    """One whole-box subdomain on one GPU; method names follow the script
+    commands of the reference (src/input.cpp:48-299)."""
+
+    def __init__(self, nx, ny, nz, boxlo, period, zghost=ZGHOST_REFERENCE, precision=FP64, device=0):
+        d = Desc(nx, ny, nz, _c3(boxlo), _c3(period), zghost, precision, device)
+        h = ctypes.c_void_p()
+        _check(lib().phafd_hip_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        self.shape = (nz, ny, nx)
+        self.natoms = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().phafd_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        return _check(rc, self.h)
+
+    def set_field(self, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.shape)
+        self._c(lib().phafd_hip_set_field(self.h, which, _d(a)))
+
+    def get_field(self, which):
+        a = np.zeros(self.shape)
+        self._c(lib().phafd_hip_get_field(self.h, which, _d(a)))
+        return a
+
+    def get_spectrum(self, which):
+        nz, ny, nx = self.shape
+        a = np.zeros((nz, ny, nx // 2 + 1), dtype=np.complex128)
+        self._c(lib().phafd_hip_get_spectrum(self.h, which, a.ctypes.data_as(_dp)))
+        return a
+
+    def set_atoms(self, xyz, types, tags=None, mols=None):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        n = xyz.shape[0]
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        tg = None if tags is None else np.ascontiguousarray(tags, dtype=np.int32)
+        ml = None if mols is None else np.ascontiguousarray(mols, dtype=np.int32)
+        self.natoms = n
+        self._c(lib().phafd_hip_set_atoms(self.h, n, _d(xyz), _i(t), _i(tg), _i(ml)))
+
+    def get_atoms(self):
+        x = np.zeros((self.natoms, 3))
+        u = np.zeros((self.natoms, 3))
+        im = np.zeros(self.natoms, dtype=np.int32)
+        self._c(lib().phafd_hip_get_atoms(self.h, _d(x), _d(u), _i(im)))
+        return x, u, im
+
+    def get_forces(self):
+        f = np.zeros((self.natoms, 3))
+        self._c(lib().phafd_hip_get_forces(self.h, _d(f)))
+        return f
+
+    def pair_gridatom(self, style, cut):
+        return self._c(lib().phafd_hip_pair_gridatom(self.h, style, cut))
+
+    def pair_coeff(self, ipair, itype, eps=float("nan"), phit=float("nan"), nucwidth=float("nan")):
+        self._c(lib().phafd_hip_pair_coeff(self.h, ipair, itype, eps, phit, nucwidth))
+
+    def pair_ljcut(self, cut, eps, sigma):
+        return self._c(lib().phafd_hip_pair_ljcut(self.h, cut, eps, sigma))
+
+    def fix_conjugate(self, seed, mobility=1.0, temp=1.0, volFH=1.0, gamma=1.0, noise=NOISE_OFF):
+        self._c(lib().phafd_hip_fix_conjugate(self.h, seed, mobility, temp, volFH, gamma, noise))
+
+    def set_noise(self, xi):
+        if xi is None:
+            self._c(lib().phafd_hip_set_noise(self.h, None))
+        else:
+            a = np.ascontiguousarray(xi, dtype=np.complex128)
+            self._c(lib().phafd_hip_set_noise(self.h, a.ctypes.data_as(_dp)))
+
+    def fix_floryhuggins(self, temp=1.0, volFH=1.0, chi=1.0):
+        self._c(lib().phafd_hip_fix_floryhuggins(self.h, temp, volFH, chi))
+
+    def fix_gradphi(self, once):
+        self._c(lib().phafd_hip_fix_gradphi(self.h, 1 if once else 0))
+
+    def fix_drag(self, starts, ends, drag):
+        s = np.ascontiguousarray(starts, dtype=np.int32)
+        e = np.ascontiguousarray(ends, dtype=np.int32)
+        self._c(lib().phafd_hip_fix_drag(self.h, len(s), _i(s), _i(e), drag))
+
+    def fix_chain(self, starts, ends, r0, kbond, kbend, drag):
+        s = np.ascontiguousarray(starts, dtype=np.int32)
+        e = np.ascontiguousarray(ends, dtype=np.int32)
+        self._c(lib().phafd_hip_fix_chain(self.h, len(s), _i(s), _i(e), r0, kbond, kbend, drag))
+
+    def neighbor(self, skin):
+        self._c(lib().phafd_hip_neighbor(self.h, skin))
+
+    def setup(self, dt):
+        self._c(lib().phafd_hip_setup(self.h, dt))
+
+    def run(self, nsteps):
+        self._c(lib().phafd_hip_run(self.h, nsteps))
+
+    def phase(self, ph):
+        self._c(lib().phafd_hip_phase(self.h, ph))
+
+    def counts(self):
+        o = np.zeros(8, dtype=np.int64)
+        self._c(lib().phafd_hip_counts(self.h, o.ctypes.data_as(_lp)))
+        keys = ["nowned", "ngathered", "nlocal", "nghost", "ncalls", "nless", "ngreat", "overflow"]
+        return dict(zip(keys, [int(v) for v in o]))
+
+    def _list(self, fn):
+        inum = ctypes.c_int64(0)
+        nent = ctypes.c_int64(0)
+        self._c(fn(self.h, 0, None, None, None, ctypes.byref(inum), ctypes.byref(nent)))
+        cap = max(inum.value, nent.value, 1)
+        il = np.zeros(max(inum.value, 1), dtype=np.int32)
+        nn = np.zeros(max(inum.value, 1), dtype=np.int32)
+        nb = np.zeros(max(nent.value, 1), dtype=np.int32)
+        self._c(fn(self.h, cap, _i(il), _i(nn), _i(nb), ctypes.byref(inum), ctypes.byref(nent)))
+        return il[:inum.value], nn[:inum.value], nb[:nent.value]
+
+    def gridlist(self):
+        return self._list(lib().phafd_hip_gridlist)
+
+    def halflist(self):
+        return self._list(lib().phafd_hip_halflist)
+
+    def gathered(self):
+        n = self.counts()["ngathered"]
+        x = np.zeros((max(n, 1), 3))
+        t = np.zeros(max(n, 1), dtype=np.int32)
+        lab = np.zeros(max(n, 1), dtype=np.int32)
+        own = np.zeros(max(n, 1), dtype=np.int32)
+        self._c(lib().phafd_hip_gathered(self.h, _d(x), _i(t), _i(lab), _i(own)))
+        return x[:n], t[:n], lab[:n], own[:n]
+
+    def bins(self):
+        info = np.zeros(12, dtype=np.int32)
+        self._c(lib().phafd_hip_bins(self.h, _i(info), None, None, None))
+        n = self.counts()["ngathered"]
+        bh = np.zeros(info[9], dtype=np.int32)
+        b = np.zeros(max(n, 1), dtype=np.int32)
+        a2b = np.zeros(max(n, 1), dtype=np.int32)
+        self._c(lib().phafd_hip_bins(self.h, _i(info), _i(bh), _i(b), _i(a2b)))
+        return info, bh, b[:n], a2b[:n]
+
+    def energy(self):
+        o = np.zeros(4)
+        self._c(lib().phafd_hip_energy(self.h, _d(o)))
+        return o
+
+    KERNELS = ["r2c_x", "y_fwd", "y_inv", "z_fwd", "z_inv", "c2r_x", "grad_multiply", "update", "fh",
+               "ga_deposit", "ga_force", "grid_count", "grid_fill", "half_count", "half_fill", "lj"]
+
+    def probe(self, kernel=None):
+        """Start (kernel name) or stop (None) HIP-event timing of one kernel;
+        returns (ms, launches, algorithmic bytes) accumulated so far."""
+        kid = -1 if kernel is None else self.KERNELS.index(kernel)
+        o = np.zeros(3)
+        self._c(lib().phafd_hip_probe(self.h, kid, _d(o)))
+        return o[0], int(o[1]), o[2]
+
+    def timers(self, reset=False):
+        o = np.zeros(8)
+        self._c(lib().phafd_hip_timers(self.h, _d(o), 1 if reset else 0))
+        keys = ["gradphi", "neighbor", "pair", "integrate", "fh_r2c", "update", "c2r_phi", "total"]
+        return dict(zip(keys, o.tolist()))

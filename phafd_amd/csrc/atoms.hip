@@ -1,0 +1,817 @@
+// atoms.hip -- atom-side kernels of the per-timestep hot path (FP64):
+// periodic wrap, ghost images, binning, neighbour lists, pair forces,
+// force fold-back and the overdamped integrators.
+//
+// Every kernel that decides list membership or position arithmetic uses the
+// reference's exact expressions (built with -ffp-contract=off), so lists and
+// bins are bit-identical to the reference's on the same positions.
+#include "atoms.hpp"
+
+namespace phafd {
+namespace {
+
+constexpr int TPB = 256;
+constexpr int IMGMASK = 1023, IMGMAX = 512, IMGBITS = 10, IMG2BITS = 20;   // src/domain.cpp:10-13
+
+int nblk(size_t n) {
+  size_t b = (n + TPB - 1) / TPB;
+  if (b < 1) b = 1;
+  if (b > 65535 * 4) b = 65535 * 4;
+  return (int)b;
+}
+#define GRID_STRIDE(i, n) \
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)(n); i += (size_t)gridDim.x * blockDim.x)
+
+// ------------------------------------------------------------ domain
+// Domain::pbc + unmap, src/domain.cpp:102-178
+__global__ void k_pbc(int n, double4 *x, double4 *ux, int *image, Box b) {
+  GRID_STRIDE(i, n) {
+    double4 p = x[i];
+    int img = image[i];
+    int idim, otherdims;
+    if (p.x < b.lo[0]) {
+      p.x += b.period[0];
+      idim = img & IMGMASK; otherdims = img ^ idim; idim--; idim &= IMGMASK; img = otherdims | idim;
+    }
+    if (p.x >= b.hi[0]) {
+      p.x -= b.period[0];
+      p.x = (p.x > b.lo[0] ? p.x : b.lo[0]);
+      idim = img & IMGMASK; otherdims = img ^ idim; idim++; idim &= IMGMASK; img = otherdims | idim;
+    }
+    if (p.y < b.lo[1]) {
+      p.y += b.period[1];
+      idim = (img >> IMGBITS) & IMGMASK; otherdims = img ^ (idim << IMGBITS); idim--; idim &= IMGMASK;
+      img = otherdims | (idim << IMGBITS);
+    }
+    if (p.y >= b.hi[1]) {
+      p.y -= b.period[1];
+      p.y = (p.y > b.lo[1] ? p.y : b.lo[1]);
+      idim = (img >> IMGBITS) & IMGMASK; otherdims = img ^ (idim << IMGBITS); idim++; idim &= IMGMASK;
+      img = otherdims | (idim << IMGBITS);
+    }
+    if (p.z < b.lo[2]) {
+      p.z += b.period[2];
+      idim = img >> IMG2BITS; otherdims = img ^ (idim << IMG2BITS); idim--; idim &= IMGMASK;
+      img = otherdims | (idim << IMG2BITS);
+    }
+    if (p.z >= b.hi[2]) {
+      p.z -= b.period[2];
+      p.z = (p.z > b.lo[2] ? p.z : b.lo[2]);
+      idim = img >> IMG2BITS; otherdims = img ^ (idim << IMG2BITS); idim++; idim &= IMGMASK;
+      img = otherdims | (idim << IMG2BITS);
+    }
+    x[i] = p;
+    image[i] = img;
+    const int xb = (img & IMGMASK) - IMGMAX, yb = (img >> IMGBITS & IMGMASK) - IMGMAX,
+              zb = (img >> IMG2BITS) - IMGMAX;
+    ux[i] = make_double4(p.x + xb * b.period[0], p.y + yb * b.period[1], p.z + zb * b.period[2], 0.0);
+  }
+}
+
+__global__ void k_unmap(int n, const double4 *x, double4 *ux, const int *image, Box b) {
+  GRID_STRIDE(i, n) {
+    const double4 p = x[i];
+    const int img = image[i];
+    const int xb = (img & IMGMASK) - IMGMAX, yb = (img >> IMGBITS & IMGMASK) - IMGMAX,
+              zb = (img >> IMG2BITS) - IMGMAX;
+    ux[i] = make_double4(p.x + xb * b.period[0], p.y + yb * b.period[1], p.z + zb * b.period[2], 0.0);
+  }
+}
+
+// Neighbor::check_distance, src/neighbor.cpp:79-97
+__global__ void k_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag) {
+  int f = 0;
+  GRID_STRIDE(i, n) {
+    const double4 a = x[i], h = xh[i];
+    const double d0 = a.x - h.x, d1 = a.y - h.y, d2 = a.z - h.z;
+    const double rsq = d0 * d0 + d1 * d1 + d2 * d2;
+    if (rsq > triggersq) f |= 1;
+    if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z)) f |= 2;   // reported as an error
+  }
+  if (f) atomicOr(flag, f);
+}
+
+__global__ void k_copy4(int n, const double4 *a, double4 *b) {
+  GRID_STRIDE(i, n) b[i] = a[i];
+}
+
+// ------------------------------------------------------------- borders
+// copy swap classification, src/comm_brick.cpp:299-316 (one rank):
+// 1 = LOCAL copy, 2 = GHOST at z - Lz, 3 = GHOST at z + Lz, 0 = not sent
+__global__ void k_copy_classify(int n, const double4 *x, double zstart, double zend, double lo,
+                                double hi, int *cat, int *flag) {
+  GRID_STRIDE(i, n) {
+    const double z = x[i].z;
+    int c = 0;
+    if (z >= zstart && z <= zend) c = 1;
+    else if (z >= lo) c = 2;
+    else if (z <= hi) c = 3;
+    cat[i] = c;
+    flag[i] = c != 0;
+  }
+}
+__global__ void k_copy_scatter(int n, const double4 *x, const int *type, const int *cat,
+                               const int *pos, double zprd, int *send, GatheredView g) {
+  GRID_STRIDE(i, n) {
+    const int c = cat[i];
+    if (!c) continue;
+    const int q = pos[i];
+    const double pz = c == 1 ? 0.0 : (c == 2 ? -zprd : zprd);
+    const double4 p = x[i];
+    send[q] = (int)i;
+    g.x[q] = make_double4(p.x, p.y, p.z + pz, (double)type[i]);
+    g.shift[q] = make_double4(0.0, 0.0, pz, 0.0);
+    g.owner[q] = (int)i;
+    g.label[q] = c == 1 ? 1 : 2;
+  }
+}
+// image swap selection, src/comm_brick.cpp:420-428 (inclusive slab test)
+__global__ void k_img_flag(int nscan, const double4 *gx, int dim, double lo, double hi, int *flag) {
+  GRID_STRIDE(i, nscan) {
+    const double4 p = gx[i];
+    const double c = dim == 0 ? p.x : (dim == 1 ? p.y : p.z);
+    flag[i] = (c >= lo && c <= hi) ? 1 : 0;
+  }
+}
+// Atom::pack_border + unpack_border, src/atom.cpp:327-376
+__global__ void k_img_scatter(int nscan, const int *flag, const int *pos, int dim, double shift,
+                              int first, int *send, GatheredView g) {
+  GRID_STRIDE(i, nscan) {
+    if (!flag[i]) continue;
+    const int q = pos[i];
+    const int d = first + q;
+    double4 p = g.x[i];
+    double4 s = g.shift[i];
+    if (dim == 0) { p.x = p.x + shift; s.x = shift; }
+    else if (dim == 1) { p.y = p.y + shift; s.y = shift; }
+    else { p.z = p.z + shift; s.z = shift; }
+    send[q] = (int)i;
+    g.x[d] = p;
+    g.shift[d] = s;
+    g.owner[d] = g.owner[i];
+    g.label[d] = 2;
+  }
+}
+
+// CommBrick::forward_comm, src/comm_brick.cpp:190-224: every gathered entry
+// is its owner's position plus one periodic shift per coordinate (the other
+// swaps add 0.0, which is exact), so one pass reproduces the swap sequence.
+__global__ void k_forward(int ng, const double4 *x, GatheredView g) {
+  GRID_STRIDE(i, ng) {
+    const double4 p = x[g.owner[i]];
+    const double4 s = g.shift[i];
+    const double w = g.x[i].w;
+    g.x[i] = make_double4(p.x + s.x, p.y + s.y, p.z + s.z, w);
+  }
+}
+
+// CommBrick::reverse_comm for one swap, src/comm_brick.cpp:228-258 +
+// Atom::unpack_reverse, src/atom.cpp:255-269 (targets within a swap are distinct)
+__global__ void k_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF) {
+  GRID_STRIDE(q, cnt) {
+    const int j = send[q];
+    const double4 b = gF[first + q];
+    double4 t = target[j];
+    t.x += b.x;
+    t.y += b.y;
+    t.z += b.z;
+    target[j] = t;
+  }
+}
+
+// ---------------------------------------------------------------- bins
+// NBin::coord2bin, src/nbin.cpp:61-94
+__device__ __forceinline__ int coord2bin(const BinGeom &b, double x, double y, double z) {
+  int ix, iy, iz;
+  if (x >= b.boxhi[0]) ix = (int)((x - b.boxhi[0]) * b.bininvx) + b.nbinx;
+  else if (x >= b.boxlo[0]) { ix = (int)((x - b.boxlo[0]) * b.bininvx); ix = ix < b.nbinx - 1 ? ix : b.nbinx - 1; }
+  else ix = (int)((x - b.boxlo[0]) * b.bininvx) - 1;
+  if (y >= b.boxhi[1]) iy = (int)((y - b.boxhi[1]) * b.bininvy) + b.nbiny;
+  else if (y >= b.boxlo[1]) { iy = (int)((y - b.boxlo[1]) * b.bininvy); iy = iy < b.nbiny - 1 ? iy : b.nbiny - 1; }
+  else iy = (int)((y - b.boxlo[1]) * b.bininvy) - 1;
+  if (z >= b.boxhi[2]) iz = (int)((z - b.boxhi[2]) * b.bininvz) + b.nbinz;
+  else if (z >= b.boxlo[2]) { iz = (int)((z - b.boxlo[2]) * b.bininvz); iz = iz < b.nbinz - 1 ? iz : b.nbinz - 1; }
+  else iz = (int)((z - b.boxlo[2]) * b.bininvz) - 1;
+  return (iz - b.mbinzlo) * b.mbiny * b.mbinx + (iy - b.mbinylo) * b.mbinx + (ix - b.mbinxlo);
+}
+
+// bins outside [0, mbins-1) (non-finite or escaped positions) are clamped to
+// the spare last bin and flagged, so no kernel ever indexes out of range
+__global__ void k_atom2bin(int ng, const double4 *gx, BinGeom b, int *a2b, int *bincount, int *bad) {
+  GRID_STRIDE(i, ng) {
+    const double4 p = gx[i];
+    int bin = 0;
+    const bool finite = isfinite(p.x) && isfinite(p.y) && isfinite(p.z);
+    if (finite) bin = coord2bin(b, p.x, p.y, p.z);
+    if (!finite || bin < 0 || bin >= b.mbins - 1) {
+      bin = b.mbins - 1;
+      atomicOr(bad, 1);
+    }
+    a2b[i] = bin;
+    atomicAdd(&bincount[bin], 1);
+  }
+}
+__global__ void k_bin_place(int ng, const int *a2b, const int *binstart, int *cursor, int *binatoms) {
+  GRID_STRIDE(i, ng) {
+    const int bin = a2b[i];
+    const int slot = binstart[bin] + atomicAdd(&cursor[bin], 1);
+    binatoms[slot] = (int)i;
+  }
+}
+// NBinStandard::bin_atoms order (src/nbin_standard.cpp:181-229): each bin
+// lists LOCAL atoms in ascending index, then GHOST atoms in ascending index.
+__global__ void k_bin_sort(int nbins, const int *binstart, int *binatoms, const int *label) {
+  GRID_STRIDE(b, nbins) {
+    const int s = binstart[b], e = binstart[b + 1];
+    for (int i = s + 1; i < e; i++) {
+      const int v = binatoms[i];
+      const long long kv = ((long long)(label[v] == 2) << 32) | (unsigned)v;
+      int j = i - 1;
+      while (j >= s) {
+        const int u = binatoms[j];
+        const long long ku = ((long long)(label[u] == 2) << 32) | (unsigned)u;
+        if (ku <= kv) break;
+        binatoms[j + 1] = u;
+        j--;
+      }
+      binatoms[j + 1] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------ grid list
+// NPairGridBin::build, src/npair_grid_bin.cpp:74-125: for every grid point in
+// (z, y, x) order, walk the full stencil in order and each bin in its list
+// order; keep atoms with rsq <= cutneighsq.  Two passes: count, then fill.
+template <bool FILL>
+__global__ void k_grid_list(GridView gv, BinGeom b, const int *stencil, int nst,
+                            const int *binstart, const int *binatoms, const double4 *gx,
+                            double cutneighsq, int nowned, int *cnt, const int64_t *off,
+                            int *neigh, int *overflow) {
+  const size_t N = (size_t)gv.nx * gv.ny * gv.nz;
+  GRID_STRIDE(p, N) {
+    const int i = (int)(p % gv.nx);
+    const int j = (int)((p / gv.nx) % gv.ny);
+    const int k = (int)(p / ((size_t)gv.nx * gv.ny));
+    const double ztmp = (k + 0) * gv.dz + gv.lo[2];
+    const double ytmp = j * gv.dy + gv.lo[1];
+    const double xtmp = i * gv.dx + gv.lo[0];
+    const int ibin = coord2bin(b, xtmp, ytmp, ztmp);
+    int n = 0;
+    int64_t w = FILL ? off[p] : 0;
+    for (int m = 0; m < nst; m++) {
+      const int bb = ibin + stencil[m];
+      const int s = binstart[bb], e = binstart[bb + 1];
+      for (int q = s; q < e; q++) {
+        const int a = binatoms[q];
+        const double4 x = gx[a];
+        const double delx = xtmp - x.x;
+        const double dely = ytmp - x.y;
+        const double delz = ztmp - x.z;
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq <= cutneighsq) {
+          if (FILL) neigh[w++] = nowned + a;
+          n++;
+        }
+      }
+    }
+    if (!FILL) {
+      cnt[p] = n;
+      if (n > 2000) atomicOr(overflow, 1);
+    }
+  }
+}
+
+// ------------------------------------------------------------ half list
+// local (non-GHOST) gathered atoms in ascending order: the std::set
+// local_atom_indices of src/nbin_standard.cpp:236-246
+__global__ void k_local_flag(int ng, const int *label, int *flag) {
+  GRID_STRIDE(i, ng) flag[i] = label[i] != 2;
+}
+__global__ void k_local_scatter(int ng, const int *flag, const int *pos, int *ilist, int *rowof) {
+  GRID_STRIDE(i, ng) {
+    if (flag[i]) {
+      ilist[pos[i]] = (int)i;
+      rowof[i] = pos[i];
+    } else {
+      rowof[i] = -1;
+    }
+  }
+}
+// NPairHalfBinNewton::build, src/npair_half_bin_newton.cpp:63-120
+template <bool FILL>
+__global__ void k_half_list(int nrows, const int *ilist, const int *a2b, const int *binstart,
+                            const int *binatoms, const int *label, const int *stencil, int nst,
+                            const double4 *gx, double cutneighsq, int nowned, int *cnt,
+                            const int *off, int *neigh, int *overflow) {
+  GRID_STRIDE(r, nrows) {
+    const int i = ilist[r];
+    const double4 xi = gx[i];
+    const double xtmp = xi.x, ytmp = xi.y, ztmp = xi.z;
+    int n = 0;
+    int w = FILL ? off[r] : 0;
+    const int ibin = a2b[i];
+    // rest of i's own bin (entries after i in the bin's list order)
+    const int s0 = binstart[ibin], e0 = binstart[ibin + 1];
+    int q = s0;
+    while (q < e0 && binatoms[q] != i) q++;
+    for (q = q + 1; q < e0; q++) {
+      const int j = binatoms[q];
+      const double4 xj = gx[j];
+      if (label[j] == 2) {
+        if (xj.z < ztmp) continue;
+        if (xj.z == ztmp) {
+          if (xj.y < ytmp) continue;
+          if (xj.y == ytmp && xj.x < xtmp) continue;
+        }
+      }
+      const double delx = xtmp - xj.x, dely = ytmp - xj.y, delz = ztmp - xj.z;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq <= cutneighsq) {
+        if (FILL) neigh[w++] = nowned + j;
+        n++;
+      }
+    }
+    for (int k = 0; k < nst; k++) {
+      const int bb = ibin + stencil[k];
+      const int s = binstart[bb], e = binstart[bb + 1];
+      for (int qq = s; qq < e; qq++) {
+        const int j = binatoms[qq];
+        const double4 xj = gx[j];
+        const double delx = xtmp - xj.x, dely = ytmp - xj.y, delz = ztmp - xj.z;
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq <= cutneighsq) {
+          if (FILL) neigh[w++] = nowned + j;
+          n++;
+        }
+      }
+    }
+    if (!FILL) {
+      cnt[r] = n;
+      if (n > 2000) atomicOr(overflow, 1);
+    }
+  }
+}
+// transposed half list: for every atom j, the pair slots e (ascending) that
+// name it as a neighbour, with the row atom i
+__global__ void k_t_count(int nent, const int *neigh, int nowned, int *tcnt) {
+  GRID_STRIDE(e, nent) atomicAdd(&tcnt[neigh[e] - nowned], 1);
+}
+__global__ void k_t_place(int nrows, const int *ilist, const int *off, const int *neigh,
+                          int nowned, const int *toff, int *tcur, int *tpair, int *ti) {
+  GRID_STRIDE(r, nrows) {
+    for (int e = off[r]; e < off[r + 1]; e++) {
+      const int j = neigh[e] - nowned;
+      const int slot = toff[j] + atomicAdd(&tcur[j], 1);
+      tpair[slot] = e;
+      ti[slot] = ilist[r];
+    }
+  }
+}
+__global__ void k_t_sort(int ng, const int *toff, int *tpair, int *ti) {
+  GRID_STRIDE(g, ng) {
+    const int s = toff[g], e = toff[g + 1];
+    for (int i = s + 1; i < e; i++) {
+      const int v = tpair[i], vi = ti[i];
+      int j = i - 1;
+      while (j >= s && tpair[j] > v) {
+        tpair[j + 1] = tpair[j];
+        ti[j + 1] = ti[j];
+        j--;
+      }
+      tpair[j + 1] = v;
+      ti[j + 1] = vi;
+    }
+  }
+}
+
+// --------------------------------------------------------- pair kernels
+// PairGridAtomGaussian / PairGridAtomLJish compute, the mu side
+// (src/pair_gridatom_gaussian.cpp:69-118, src/pair_gridatom_lj_ish.cpp:72-178):
+// one thread per grid point walks its list in order and accumulates mu.
+__global__ void k_ga_deposit(GridView gv, GAParams ga, const int *cnt, const int64_t *off,
+                             const int *neigh, const double4 *gx, int nowned, double *energy) {
+  const size_t N = (size_t)gv.nx * gv.ny * gv.nz;
+  double esum = 0.0;
+  GRID_STRIDE(p, N) {
+    const int c = cnt[p];
+    if (c == 0) continue;
+    const int k = (int)(p % gv.nx);
+    const int j = (int)((p / gv.nx) % gv.ny);
+    const int i = (int)(p / ((size_t)gv.nx * gv.ny));
+    const double xtmp = k * gv.dx + gv.lo[0];
+    const double ytmp = j * gv.dy + gv.lo[1];
+    const double ztmp = i * gv.dz + gv.lo[2];
+    const double ph = gv.phi[p];
+    double m = gv.mu[p];
+    const int64_t s = off[p];
+    for (int q = 0; q < c; q++) {
+      const int a = neigh[s + q] - nowned;
+      const double4 x = gx[a];
+      const int t = (int)x.w;
+      const double delx = xtmp - x.x, dely = ytmp - x.y, delz = ztmp - x.z;
+      double rsq = delx * delx + dely * dely + delz * delz;
+      const double cs = ga.cutsq[t];
+      const bool in = ga.style == 0 ? (rsq <= cs) : (rsq < cs);
+      if (!in) continue;
+      rsq /= cs;
+      double expfac;
+      if (ga.style == 0) expfac = (exp(-rsq / 2.0) - ga.em05) * ga.e05 * ga.eps[t];
+      else expfac = exp(-1. / (1 - rsq) + 1) * ga.eps[t];
+      const double delphi = (ph - ga.phit[t]);
+      m += 2 * delphi * expfac;
+      if (energy) esum += delphi * delphi * expfac * gv.dv;
+    }
+    gv.mu[p] = m;
+  }
+  if (energy) atomicAdd(energy, esum);
+}
+
+// The force side of the same loop, gathered per atom: the reference adds to
+// F(aj) once per listed grid point, in ascending grid index.  Every grid point
+// with rsq <= cutsq(type) is in the atom's list (the list cut is
+// cutsq + skin and no atom has moved more than skin/2 since the build), so
+// walking the in-box grid points around the atom in (z, y, x) order gives the
+// same terms in the same order.
+__global__ void k_ga_force(int ng, GridView gv, GAParams ga, const double4 *gx, double4 *gF) {
+  GRID_STRIDE(a, ng) {
+    const double4 x = gx[a];
+    if (!isfinite(x.x) || !isfinite(x.y) || !isfinite(x.z)) continue;   // never walk an unbounded range
+    const int t = (int)x.w;
+    const double cs = ga.cutsq[t];
+    const double w = sqrt(cs);
+    const double mg = 1e-9;
+    int i0 = (int)ceil((x.z - w - gv.lo[2]) / gv.dz - mg), i1 = (int)floor((x.z + w - gv.lo[2]) / gv.dz + mg);
+    int j0 = (int)ceil((x.y - w - gv.lo[1]) / gv.dy - mg), j1 = (int)floor((x.y + w - gv.lo[1]) / gv.dy + mg);
+    int k0 = (int)ceil((x.x - w - gv.lo[0]) / gv.dx - mg), k1 = (int)floor((x.x + w - gv.lo[0]) / gv.dx + mg);
+    i0 = i0 < 0 ? 0 : i0; j0 = j0 < 0 ? 0 : j0; k0 = k0 < 0 ? 0 : k0;
+    i1 = i1 > gv.nz - 1 ? gv.nz - 1 : i1;
+    j1 = j1 > gv.ny - 1 ? gv.ny - 1 : j1;
+    k1 = k1 > gv.nx - 1 ? gv.nx - 1 : k1;
+    double4 F = gF[a];
+    for (int i = i0; i <= i1; i++) {
+      const double ztmp = i * gv.dz + gv.lo[2];
+      const double delz = ztmp - x.z;
+      for (int j = j0; j <= j1; j++) {
+        const double ytmp = j * gv.dy + gv.lo[1];
+        const double dely = ytmp - x.y;
+        for (int k = k0; k <= k1; k++) {
+          const double xtmp = k * gv.dx + gv.lo[0];
+          const double delx = xtmp - x.x;
+          double rsq = delx * delx + dely * dely + delz * delz;
+          const bool in = ga.style == 0 ? (rsq <= cs) : (rsq < cs);
+          if (!in) continue;
+          rsq /= cs;
+          double expfac;
+          if (ga.style == 0) expfac = (exp(-rsq / 2.0) - ga.em05) * ga.e05 * ga.eps[t];
+          else expfac = exp(-1. / (1 - rsq) + 1) * ga.eps[t];
+          const size_t p = ((size_t)i * gv.ny + j) * gv.nx + k;
+          const double delphi = (gv.phi[p] - ga.phit[t]);
+          expfac *= 2 * delphi * gv.dv;
+          F.x -= gv.gx[p] * expfac;
+          F.y -= gv.gy[p] * expfac;
+          F.z -= gv.gz[p] * expfac;
+        }
+      }
+    }
+    gF[a] = F;
+  }
+}
+
+// PairLJCut::compute, src/pair_lj_cut.cpp:46-119, gathered per atom: atom k
+// receives +d*fpair from its own row and -d*fpair from every row that lists
+// it; the reference adds these in ascending pair-slot order, which the merge
+// of the (sorted) transposed slots with the own row reproduces.
+__device__ __forceinline__ bool lj_pair(const double4 &xi, const double4 &xj, const LJParams &lj,
+                                        double &fx, double &fy, double &fz, double &e) {
+  const double delx = xi.x - xj.x, dely = xi.y - xj.y, delz = xi.z - xj.z;
+  const double rsq = delx * delx + dely * dely + delz * delz;
+  if (!(rsq < lj.cutsq)) return false;
+  const double r2inv = 1.0 / rsq;
+  const double r6inv = r2inv * r2inv * r2inv;
+  const double forcelj = r6inv * (lj.lj1 * r6inv - lj.lj2);
+  const double fpair = forcelj * r2inv;
+  fx = delx * fpair;
+  fy = dely * fpair;
+  fz = delz * fpair;
+  e = r6inv * (lj.lj1 / 12.0 * r6inv - lj.lj2 / 6.0);
+  return true;
+}
+__global__ void k_lj_force(int ng, LJParams lj, const double4 *gx, const int *rowof,
+                           const int *hoff, const int *hneigh, const int *toff, const int *tpair,
+                           const int *ti, int nowned, double4 *gF, double *energy) {
+  double esum = 0.0;
+  GRID_STRIDE(a, ng) {
+    const double4 xa = gx[a];
+    double4 F = gF[a];
+    const int row = rowof[a];
+    const int rs = row >= 0 ? hoff[row] : 0x7fffffff;
+    const int re = row >= 0 ? hoff[row + 1] : 0x7fffffff;
+    int q = toff[a];
+    const int q1 = toff[a + 1];
+    double fx, fy, fz, e;
+    for (; q < q1 && tpair[q] < rs; q++) {
+      if (lj_pair(gx[ti[q]], xa, lj, fx, fy, fz, e)) { F.x -= fx; F.y -= fy; F.z -= fz; }
+    }
+    if (row >= 0) {
+      for (int s = rs; s < re; s++) {
+        if (lj_pair(xa, gx[hneigh[s] - nowned], lj, fx, fy, fz, e)) {
+          F.x += fx; F.y += fy; F.z += fz;
+          if (energy) esum += e;
+        }
+      }
+    }
+    for (; q < q1; q++) {
+      if (lj_pair(gx[ti[q]], xa, lj, fx, fy, fz, e)) { F.x -= fx; F.y -= fy; F.z -= fz; }
+    }
+    gF[a] = F;
+  }
+  if (energy) atomicAdd(energy, esum);
+}
+
+// ---------------------------------------------------------- integrators
+// FixAtomDrag half step, src/fixatom_drag.cpp:43-66
+__global__ void k_drag(int n, const int *members, double4 *x, const double4 *F, double drag,
+                       double dt) {
+  GRID_STRIDE(q, n) {
+    const int j = members[q];
+    double4 p = x[j];
+    const double4 f = F[j];
+    p.x += drag * f.x * dt * 0.5;
+    p.y += drag * f.y * dt * 0.5;
+    p.z += drag * f.z * dt * 0.5;
+    x[j] = p;
+  }
+}
+
+// chain forces (build-defined semiflexible style), gathered per bead in a
+// fixed order: bond(b-1,b), bond(b,b+1), angles centred at b-1, b, b+1
+__device__ __forceinline__ void bond_f(const double4 &ua, const double4 &ub, double k, double r0,
+                                       double *o) {
+  const double d0 = ub.x - ua.x, d1 = ub.y - ua.y, d2 = ub.z - ua.z;
+  const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  const double c = -k * (r - r0) / r;
+  o[0] = c * d0; o[1] = c * d1; o[2] = c * d2;
+}
+__device__ __forceinline__ void angle_t(const double4 &ua, const double4 &ub, const double4 &uc,
+                                        double *ga, double *gb) {
+  const double d1[3] = {ub.x - ua.x, ub.y - ua.y, ub.z - ua.z};
+  const double d2[3] = {uc.x - ub.x, uc.y - ub.y, uc.z - ub.z};
+  const double r1s = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+  const double r2s = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+  const double r1 = sqrt(r1s), r2 = sqrt(r2s);
+  const double dot = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
+  const double inv12 = 1.0 / (r1 * r2);
+  const double cs = dot * inv12;
+  for (int c = 0; c < 3; c++) {
+    ga[c] = d2[c] * inv12 - cs * d1[c] / r1s;
+    gb[c] = d1[c] * inv12 - cs * d2[c] / r2s;
+  }
+}
+__global__ void k_chain_force(int n, const int *members, const int *cstart, const int *cend,
+                              const double4 *ux, ChainParams cp, double4 *fc) {
+  GRID_STRIDE(q, n) {
+    const int j = members[q];
+    const int s = cstart[q], e = cend[q];
+    const int b = j - s, nb = e - s;
+    const double4 *u = ux + s;
+    double acc[3] = {0, 0, 0}, t[3], ga[3], gb[3];
+    if (b >= 1) { bond_f(u[b - 1], u[b], cp.kbond, cp.r0, t); for (int c = 0; c < 3; c++) acc[c] += t[c]; }
+    if (b + 1 < nb) { bond_f(u[b + 1], u[b], cp.kbond, cp.r0, t); for (int c = 0; c < 3; c++) acc[c] += t[c]; }
+    if (b >= 2) { angle_t(u[b - 2], u[b - 1], u[b], ga, gb); for (int c = 0; c < 3; c++) acc[c] += cp.kbend * gb[c]; }
+    if (b >= 1 && b + 1 < nb) { angle_t(u[b - 1], u[b], u[b + 1], ga, gb); for (int c = 0; c < 3; c++) acc[c] += cp.kbend * (ga[c] - gb[c]); }
+    if (b + 2 < nb) { angle_t(u[b], u[b + 1], u[b + 2], ga, gb); for (int c = 0; c < 3; c++) acc[c] += -cp.kbend * ga[c]; }
+    fc[q] = make_double4(acc[0], acc[1], acc[2], 0.0);
+  }
+}
+__global__ void k_chain_move(int n, const int *members, const double4 *fc, const double4 *F,
+                             double4 *ux, double4 *x, const int *image, ChainParams cp, double dt,
+                             Box bx) {
+  GRID_STRIDE(q, n) {
+    const int j = members[q];
+    const double4 f = F[j], c = fc[q];
+    double4 u = ux[j];
+    const double tx = f.x + c.x, ty = f.y + c.y, tz = f.z + c.z;
+    u.x += cp.drag * tx * dt * 0.5;
+    u.y += cp.drag * ty * dt * 0.5;
+    u.z += cp.drag * tz * dt * 0.5;
+    ux[j] = u;
+    // Domain::map, src/domain.cpp:181-193
+    const int img = image[j];
+    const int xb = (img & IMGMASK) - IMGMAX, yb = (img >> IMGBITS & IMGMASK) - IMGMAX,
+              zb = (img >> IMG2BITS) - IMGMAX;
+    double4 p = x[j];
+    p.x = u.x - xb * bx.period[0];
+    p.y = u.y - yb * bx.period[1];
+    p.z = u.z - zb * bx.period[2];
+    x[j] = p;
+  }
+}
+__global__ void k_chain_energy(int n, const int *members, const int *cstart, const int *cend,
+                               const double4 *ux, ChainParams cp, double *energy) {
+  double e = 0;
+  GRID_STRIDE(q, n) {
+    const int j = members[q];
+    const int s = cstart[q], eidx = cend[q];
+    const int b = j - s, nb = eidx - s;
+    const double4 *u = ux + s;
+    if (b + 1 < nb) {
+      const double d0 = u[b + 1].x - u[b].x, d1 = u[b + 1].y - u[b].y, d2 = u[b + 1].z - u[b].z;
+      const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+      e += 0.5 * cp.kbond * (r - cp.r0) * (r - cp.r0);
+    }
+    if (b >= 1 && b + 1 < nb) {
+      const double4 a = u[b - 1], m = u[b], c = u[b + 1];
+      const double d1[3] = {m.x - a.x, m.y - a.y, m.z - a.z};
+      const double d2[3] = {c.x - m.x, c.y - m.y, c.z - m.z};
+      const double r1 = sqrt(d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2]);
+      const double r2 = sqrt(d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2]);
+      e += cp.kbend * (1 - (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) / (r1 * r2));
+    }
+  }
+  atomicAdd(energy, e);
+}
+
+__global__ void k_fill_i32(int n, int *p, int v) { GRID_STRIDE(i, n) p[i] = v; }
+__global__ void k_fill_d4(int n, double4 *p) { GRID_STRIDE(i, n) p[i] = make_double4(0, 0, 0, 0); }
+
+}  // namespace
+
+// ================================================================ launchers
+void launch_pbc(int n, double4 *x, double4 *ux, int *image, const Box &b, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pbc, dim3(nblk(n)), dim3(TPB), 0, s, n, x, ux, image, b);
+  HIPCHK(hipGetLastError());
+}
+void launch_unmap(int n, const double4 *x, double4 *ux, const int *image, const Box &b, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_unmap, dim3(nblk(n)), dim3(TPB), 0, s, n, x, ux, image, b);
+  HIPCHK(hipGetLastError());
+}
+void launch_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_decide, dim3(nblk(n)), dim3(TPB), 0, s, n, x, xh, triggersq, flag);
+  HIPCHK(hipGetLastError());
+}
+void launch_copy4(int n, const double4 *a, double4 *b, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy4, dim3(nblk(n)), dim3(TPB), 0, s, n, a, b);
+  HIPCHK(hipGetLastError());
+}
+void launch_copy_classify(int n, const double4 *x, double zstart, double zend, double lo, double hi,
+                          int *cat, int *flag, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy_classify, dim3(nblk(n)), dim3(TPB), 0, s, n, x, zstart, zend, lo, hi, cat, flag);
+  HIPCHK(hipGetLastError());
+}
+void launch_copy_scatter(int n, const double4 *x, const int *type, const int *cat, const int *pos,
+                         double zprd, int *send, GatheredView g, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy_scatter, dim3(nblk(n)), dim3(TPB), 0, s, n, x, type, cat, pos, zprd, send, g);
+  HIPCHK(hipGetLastError());
+}
+void launch_img_flag(int nscan, const double4 *gx, int dim, double lo, double hi, int *flag, hipStream_t s) {
+  if (nscan <= 0) return;
+  hipLaunchKernelGGL(k_img_flag, dim3(nblk(nscan)), dim3(TPB), 0, s, nscan, gx, dim, lo, hi, flag);
+  HIPCHK(hipGetLastError());
+}
+void launch_img_scatter(int nscan, const int *flag, const int *pos, int dim, double shift, int first,
+                        int *send, GatheredView g, hipStream_t s) {
+  if (nscan <= 0) return;
+  hipLaunchKernelGGL(k_img_scatter, dim3(nblk(nscan)), dim3(TPB), 0, s, nscan, flag, pos, dim, shift, first, send, g);
+  HIPCHK(hipGetLastError());
+}
+void launch_forward(int ng, const double4 *x, GatheredView g, hipStream_t s) {
+  if (ng <= 0) return;
+  hipLaunchKernelGGL(k_forward, dim3(nblk(ng)), dim3(TPB), 0, s, ng, x, g);
+  HIPCHK(hipGetLastError());
+}
+void launch_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF, hipStream_t s) {
+  if (cnt <= 0) return;
+  hipLaunchKernelGGL(k_reverse, dim3(nblk(cnt)), dim3(TPB), 0, s, cnt, send, first, target, gF);
+  HIPCHK(hipGetLastError());
+}
+void launch_atom2bin(int ng, const double4 *gx, const BinGeom &b, int *a2b, int *bincount, int *bad,
+                     hipStream_t s) {
+  if (ng <= 0) return;
+  hipLaunchKernelGGL(k_atom2bin, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gx, b, a2b, bincount, bad);
+  HIPCHK(hipGetLastError());
+}
+void launch_bin_place(int ng, const int *a2b, const int *binstart, int *cursor, int *binatoms, hipStream_t s) {
+  if (ng <= 0) return;
+  hipLaunchKernelGGL(k_bin_place, dim3(nblk(ng)), dim3(TPB), 0, s, ng, a2b, binstart, cursor, binatoms);
+  HIPCHK(hipGetLastError());
+}
+void launch_bin_sort(int nbins, const int *binstart, int *binatoms, const int *label, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_sort, dim3(nblk(nbins)), dim3(TPB), 0, s, nbins, binstart, binatoms, label);
+  HIPCHK(hipGetLastError());
+}
+void launch_grid_list(bool fill, const GridView &gv, const BinGeom &b, const int *stencil, int nst,
+                      const int *binstart, const int *binatoms, const double4 *gx, double cutneighsq,
+                      int nowned, int *cnt, const int64_t *off, int *neigh, int *overflow, hipStream_t s) {
+  const size_t N = (size_t)gv.nx * gv.ny * gv.nz;
+  if (fill)
+    hipLaunchKernelGGL((k_grid_list<true>), dim3(nblk(N)), dim3(TPB), 0, s, gv, b, stencil, nst, binstart,
+                       binatoms, gx, cutneighsq, nowned, cnt, off, neigh, overflow);
+  else
+    hipLaunchKernelGGL((k_grid_list<false>), dim3(nblk(N)), dim3(TPB), 0, s, gv, b, stencil, nst, binstart,
+                       binatoms, gx, cutneighsq, nowned, cnt, off, neigh, overflow);
+  HIPCHK(hipGetLastError());
+}
+void launch_local_flag(int ng, const int *label, int *flag, hipStream_t s) {
+  if (ng <= 0) return;
+  hipLaunchKernelGGL(k_local_flag, dim3(nblk(ng)), dim3(TPB), 0, s, ng, label, flag);
+  HIPCHK(hipGetLastError());
+}
+void launch_local_scatter(int ng, const int *flag, const int *pos, int *ilist, int *rowof, hipStream_t s) {
+  if (ng <= 0) return;
+  hipLaunchKernelGGL(k_local_scatter, dim3(nblk(ng)), dim3(TPB), 0, s, ng, flag, pos, ilist, rowof);
+  HIPCHK(hipGetLastError());
+}
+void launch_half_list(bool fill, int nrows, const int *ilist, const int *a2b, const int *binstart,
+                      const int *binatoms, const int *label, const int *stencil, int nst, const double4 *gx,
+                      double cutneighsq, int nowned, int *cnt, const int *off, int *neigh, int *overflow,
+                      hipStream_t s) {
+  if (nrows <= 0) return;
+  if (fill)
+    hipLaunchKernelGGL((k_half_list<true>), dim3(nblk(nrows)), dim3(TPB), 0, s, nrows, ilist, a2b, binstart,
+                       binatoms, label, stencil, nst, gx, cutneighsq, nowned, cnt, off, neigh, overflow);
+  else
+    hipLaunchKernelGGL((k_half_list<false>), dim3(nblk(nrows)), dim3(TPB), 0, s, nrows, ilist, a2b, binstart,
+                       binatoms, label, stencil, nst, gx, cutneighsq, nowned, cnt, off, neigh, overflow);
+  HIPCHK(hipGetLastError());
+}
+void launch_t_count(int nent, const int *neigh, int nowned, int *tcnt, hipStream_t s) {
+  if (nent <= 0) return;
+  hipLaunchKernelGGL(k_t_count, dim3(nblk(nent)), dim3(TPB), 0, s, nent, neigh, nowned, tcnt);
+  HIPCHK(hipGetLastError());
+}
+void launch_t_place(int nrows, const int *ilist, const int *off, const int *neigh, int nowned,
+                    const int *toff, int *tcur, int *tpair, int *ti, hipStream_t s) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_t_place, dim3(nblk(nrows)), dim3(TPB), 0, s, nrows, ilist, off, neigh, nowned, toff,
+                     tcur, tpair, ti);
+  HIPCHK(hipGetLastError());
+}
+void launch_t_sort(int ng, const int *toff, int *tpair, int *ti, hipStream_t s) {
+  if (ng <= 0) return;
+  hipLaunchKernelGGL(k_t_sort, dim3(nblk(ng)), dim3(TPB), 0, s, ng, toff, tpair, ti);
+  HIPCHK(hipGetLastError());
+}
+void launch_ga_deposit(const GridView &gv, const GAParams &ga, const int *cnt, const int64_t *off,
+                       const int *neigh, const double4 *gx, int nowned, double *energy, hipStream_t s) {
+  const size_t N = (size_t)gv.nx * gv.ny * gv.nz;
+  hipLaunchKernelGGL(k_ga_deposit, dim3(nblk(N)), dim3(TPB), 0, s, gv, ga, cnt, off, neigh, gx, nowned, energy);
+  HIPCHK(hipGetLastError());
+}
+void launch_ga_force(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, double4 *gF,
+                     hipStream_t s) {
+  if (ng <= 0) return;
+  hipLaunchKernelGGL(k_ga_force, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, gF);
+  HIPCHK(hipGetLastError());
+}
+void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,
+                     const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
+                     double4 *gF, double *energy, hipStream_t s) {
+  if (ng <= 0) return;
+  hipLaunchKernelGGL(k_lj_force, dim3(nblk(ng)), dim3(TPB), 0, s, ng, lj, gx, rowof, hoff, hneigh, toff,
+                     tpair, ti, nowned, gF, energy);
+  HIPCHK(hipGetLastError());
+}
+void launch_drag(int n, const int *members, double4 *x, const double4 *F, double drag, double dt,
+                 hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_drag, dim3(nblk(n)), dim3(TPB), 0, s, n, members, x, F, drag, dt);
+  HIPCHK(hipGetLastError());
+}
+void launch_chain_force(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,
+                        const ChainParams &cp, double4 *fc, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_chain_force, dim3(nblk(n)), dim3(TPB), 0, s, n, members, cstart, cend, ux, cp, fc);
+  HIPCHK(hipGetLastError());
+}
+void launch_chain_move(int n, const int *members, const double4 *fc, const double4 *F, double4 *ux,
+                       double4 *x, const int *image, const ChainParams &cp, double dt, const Box &b,
+                       hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_chain_move, dim3(nblk(n)), dim3(TPB), 0, s, n, members, fc, F, ux, x, image, cp, dt, b);
+  HIPCHK(hipGetLastError());
+}
+void launch_chain_energy(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,
+                         const ChainParams &cp, double *energy, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_chain_energy, dim3(nblk(n)), dim3(TPB), 0, s, n, members, cstart, cend, ux, cp, energy);
+  HIPCHK(hipGetLastError());
+}
+void launch_fill_i32(int n, int *p, int v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(TPB), 0, s, n, p, v);
+  HIPCHK(hipGetLastError());
+}
+void launch_fill_d4(int n, double4 *p, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_d4, dim3(nblk(n)), dim3(TPB), 0, s, n, p);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace phafd

@@ -1,0 +1,91 @@
+// atoms.hpp -- kernel parameter blocks and launchers of atoms.hip.
+#pragma once
+#include "internal.hpp"
+
+namespace phafd {
+
+struct Box {
+  double lo[3], hi[3], period[3];
+};
+
+// gathered atoms (LOCAL copies and GHOST images), indexed relative to nowned
+struct GatheredView {
+  double4 *x;       // x, y, z, type
+  double4 *shift;   // periodic shift applied to the owner's position
+  int *owner;       // owned index
+  int *label;       // 1 LOCAL, 2 GHOST
+};
+
+struct GridView {
+  int nx, ny, nz;
+  double dx, dy, dz, dv;
+  double lo[3];     // sublo (grid point (0,0,0))
+  const double *phi;
+  double *mu;
+  const double *gx, *gy, *gz;
+};
+
+struct GAParams {
+  int style;        // 0 gaussian, 1 LJish
+  const double *eps, *phit, *cutsq;
+  double em05, e05;   // exp(-0.5), exp(0.5) evaluated on the host
+};
+
+struct LJParams {
+  double lj1, lj2, cutsq;
+};
+
+struct ChainParams {
+  double r0, kbond, kbend, drag;
+};
+
+void launch_pbc(int n, double4 *x, double4 *ux, int *image, const Box &b, hipStream_t s);
+void launch_unmap(int n, const double4 *x, double4 *ux, const int *image, const Box &b, hipStream_t s);
+void launch_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag, hipStream_t s);
+void launch_copy4(int n, const double4 *a, double4 *b, hipStream_t s);
+void launch_copy_classify(int n, const double4 *x, double zstart, double zend, double lo, double hi,
+                          int *cat, int *flag, hipStream_t s);
+void launch_copy_scatter(int n, const double4 *x, const int *type, const int *cat, const int *pos,
+                         double zprd, int *send, GatheredView g, hipStream_t s);
+void launch_img_flag(int nscan, const double4 *gx, int dim, double lo, double hi, int *flag, hipStream_t s);
+void launch_img_scatter(int nscan, const int *flag, const int *pos, int dim, double shift, int first,
+                        int *send, GatheredView g, hipStream_t s);
+void launch_forward(int ng, const double4 *x, GatheredView g, hipStream_t s);
+void launch_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF, hipStream_t s);
+void launch_atom2bin(int ng, const double4 *gx, const BinGeom &b, int *a2b, int *bincount, int *bad,
+                     hipStream_t s);
+void launch_bin_place(int ng, const int *a2b, const int *binstart, int *cursor, int *binatoms, hipStream_t s);
+void launch_bin_sort(int nbins, const int *binstart, int *binatoms, const int *label, hipStream_t s);
+void launch_grid_list(bool fill, const GridView &gv, const BinGeom &b, const int *stencil, int nst,
+                      const int *binstart, const int *binatoms, const double4 *gx, double cutneighsq,
+                      int nowned, int *cnt, const int64_t *off, int *neigh, int *overflow, hipStream_t s);
+void launch_local_flag(int ng, const int *label, int *flag, hipStream_t s);
+void launch_local_scatter(int ng, const int *flag, const int *pos, int *ilist, int *rowof, hipStream_t s);
+void launch_half_list(bool fill, int nrows, const int *ilist, const int *a2b, const int *binstart,
+                      const int *binatoms, const int *label, const int *stencil, int nst, const double4 *gx,
+                      double cutneighsq, int nowned, int *cnt, const int *off, int *neigh, int *overflow,
+                      hipStream_t s);
+void launch_t_count(int nent, const int *neigh, int nowned, int *tcnt, hipStream_t s);
+void launch_t_place(int nrows, const int *ilist, const int *off, const int *neigh, int nowned,
+                    const int *toff, int *tcur, int *tpair, int *ti, hipStream_t s);
+void launch_t_sort(int ng, const int *toff, int *tpair, int *ti, hipStream_t s);
+void launch_ga_deposit(const GridView &gv, const GAParams &ga, const int *cnt, const int64_t *off,
+                       const int *neigh, const double4 *gx, int nowned, double *energy, hipStream_t s);
+void launch_ga_force(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, double4 *gF,
+                     hipStream_t s);
+void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,
+                     const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
+                     double4 *gF, double *energy, hipStream_t s);
+void launch_drag(int n, const int *members, double4 *x, const double4 *F, double drag, double dt,
+                 hipStream_t s);
+void launch_chain_force(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,
+                        const ChainParams &cp, double4 *fc, hipStream_t s);
+void launch_chain_move(int n, const int *members, const double4 *fc, const double4 *F, double4 *ux,
+                       double4 *x, const int *image, const ChainParams &cp, double dt, const Box &b,
+                       hipStream_t s);
+void launch_chain_energy(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,
+                         const ChainParams &cp, double *energy, hipStream_t s);
+void launch_fill_i32(int n, int *p, int v, hipStream_t s);
+void launch_fill_d4(int n, double4 *p, hipStream_t s);
+
+}  // namespace phafd

@@ -1,0 +1,1370 @@
+// ctx.hip -- the phafd_hip context: device state of one whole-box subdomain,
+// the per-timestep hook sequence of Integrate::run (src/integrate.cpp:105-203)
+// and the C ABI declared in include/phafd_hip.h.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <set>
+
+#include "../../include/phafd_hip.h"
+#include "atoms.hpp"
+#include "internal.hpp"
+
+using namespace phafd;
+
+namespace {
+
+thread_local std::string g_create_err;
+const double PI_REF = 3.141592;   // src/domain.cpp:206-208
+const double BIG = 1e20;          // src/comm_brick.cpp:8
+const double SMALL = 1.0e-6;      // src/nbin_standard.cpp:25
+
+struct PairDef {
+  int style;   // 0 gaussian, 1 LJish, 2 lj/cut
+  double maxcut = 0;
+  std::vector<double> eps, phit, cutsq;
+  DBuf<double> d_eps, d_phit, d_cutsq;
+  double lj1 = 0, lj2 = 0, ljcutsq = 0;
+};
+
+struct FixDef {
+  enum Kind { CONJ, FH, GRADPHI, DRAG, CHAIN } kind;
+  double mobility = 1, temp = 1, volFH = 1, gamma = 1, chi = 1;
+  int seed = 0, noise_mode = 0, once = 0;
+  std::vector<int> start, end;
+  double drag = 0, r0 = 1, kbond = 0, kbend = 0;
+  int nmem = 0;
+  DBuf<int> members, cstart, cend;
+  DBuf<double4> fc;
+};
+
+struct SwapDev {
+  int dim = 2;
+  double lo = 0, hi = 0, shift = 0;
+  int first = 0;   // relative gathered index of the first received entry
+  int count = 0;
+  DBuf<int> send;  // sources: owned index (copy swap) or relative gathered index
+};
+
+}  // namespace
+
+struct phafd_hip_ctx {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nx = 0, ny = 0, nz = 0;
+  double boxlo[3], boxhi[3], period[3], sublo[3], subhi[3];
+  int zghost = 0;
+  FFTPlan plan;
+  size_t N = 0, M = 0;
+  DBuf<double> phi, mu, grad[3];
+  DBuf<double2> phihat, muhat, ghat[3], noise_k, canon_tmp;
+  bool noise_set = false;
+  DBuf<unsigned long long> counters;   // [nless, ngreat]
+  DBuf<int> flags;                     // [0] rebuild, [1] overflow
+  DBuf<double> esum;                   // energy accumulators
+  int *h_flags = nullptr;              // pinned: [rebuild, overflow, pad, pad] + phi0 as 2 ints
+  // owned atoms
+  int nowned = 0, ntypes = 0;
+  bool atoms_set = false;
+  std::vector<int> h_tag, h_type, h_mol;
+  DBuf<double4> x, ux, F, xhold;
+  DBuf<int> image, type;
+  // gathered
+  int ngathered = 0, nlocal = 0, nghost = 0, gcap = 0;
+  DBuf<double4> gx, gshift, gF;
+  DBuf<int> gowner, glabel;
+  DBuf<int> tmp_flag, tmp_pos, tmp_cat;
+  std::vector<SwapDev> swaps;
+  double cutghost = 0;
+  // neighbour
+  bool neigh_set = false, need_full = false, need_half = false;
+  double skin = 0, cutneigh = 0, triggersq = 0;
+  int64_t ncalls = 0;
+  BinGeom bg{};
+  std::vector<int> stencil_full, stencil_half;
+  DBuf<int> d_stencil_full, d_stencil_half;
+  DBuf<int> a2b, bincount, binstart, bincursor, binatoms;
+  DBuf<int> g_cnt, g_neigh;
+  DBuf<int64_t> g_off;
+  int64_t g_nent = 0;
+  DBuf<int> h_ilist, h_rowof, h_cnt, h_off, h_neigh;
+  int h_nrows = 0, h_nent = 0;
+  DBuf<int> t_cnt, t_off, t_cur, t_pair, t_i;
+  bool lists_valid = false;
+  int pending_decision = -1;   // rebuild decision prefetched at the end of the last step
+  // styles
+  std::vector<std::unique_ptr<PairDef>> pairs;
+  std::vector<std::unique_ptr<FixDef>> fixes;
+  double dt = 0;
+  int64_t timestep = 0, firststep = 0;
+  bool setup_done = false;
+  // timers
+  hipEvent_t ev[9] = {};
+  double tacc[8] = {0};
+  bool fuse_fh = true;
+
+  double dx() const { return period[0] / nx; }   // src/grid.cpp:187-201
+  double dy() const { return period[1] / ny; }
+  double dz() const { return period[2] / nz; }
+  Box box() const {
+    Box b;
+    for (int d = 0; d < 3; d++) { b.lo[d] = boxlo[d]; b.hi[d] = boxhi[d]; b.period[d] = period[d]; }
+    return b;
+  }
+  GatheredView gview() { return GatheredView{gx.p, gshift.p, gowner.p, glabel.p}; }
+  GridView gridview() {
+    GridView v;
+    v.nx = nx; v.ny = ny; v.nz = nz;
+    v.dx = dx(); v.dy = dy(); v.dz = dz();
+    v.dv = dx() * dy() * dz();
+    for (int d = 0; d < 3; d++) v.lo[d] = sublo[d];
+    v.phi = phi.p; v.mu = mu.p; v.gx = grad[0].p; v.gy = grad[1].p; v.gz = grad[2].p;
+    return v;
+  }
+};
+
+namespace {
+
+template <class Fn>
+int guard(phafd_hip_ctx *c, Fn fn) {
+  try {
+    if (c) (void)hipSetDevice(c->device);
+    return fn();
+  } catch (const HipError &e) {
+    if (c) c->err = e.what();
+    return PHAFD_E_HIP;
+  } catch (const std::invalid_argument &e) {
+    if (c) c->err = e.what();
+    return PHAFD_E_ARG;
+  } catch (const std::exception &e) {
+    if (c) c->err = e.what();
+    std::string w = e.what();
+    if (w.find("NAN") != std::string::npos) return PHAFD_E_NAN;
+    return PHAFD_E_RUNTIME;
+  }
+}
+
+void sync(phafd_hip_ctx *c) { HIPCHK(hipStreamSynchronize(c->stream)); }
+
+int read_int(phafd_hip_ctx *c, const int *dptr) {
+  HIPCHK(hipMemcpyAsync(c->h_flags + 4, dptr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  return c->h_flags[4];
+}
+
+// ---------------------------------------------------------------- comm
+// CommBrick::setup (src/comm_brick.cpp:96-187, nprocs == 1): the copy swap,
+// then (periodic mode only) two z image swaps, then y and x image swaps.
+void comm_setup(phafd_hip_ctx *c) {
+  c->cutghost = c->cutneigh;
+  c->swaps.clear();
+  c->swaps.resize(1);
+  SwapDev &z = c->swaps[0];
+  z.dim = 2;
+  const int zstart = (int)c->sublo[2], zend = (int)c->subhi[2];
+  z.lo = zend - c->cutghost;
+  z.hi = zstart + c->cutghost;
+  std::vector<int> dims;
+  if (c->zghost == PHAFD_ZGHOST_PERIODIC) dims.push_back(2);
+  dims.push_back(1);
+  dims.push_back(0);
+  for (int dim : dims)
+    for (int ineed = 0; ineed < 2; ineed++) {
+      c->swaps.emplace_back();
+      SwapDev &w = c->swaps.back();
+      w.dim = dim;
+      if (ineed == 0) { w.lo = -BIG; w.hi = c->boxlo[dim] + c->cutghost; w.shift = c->period[dim]; }
+      else { w.lo = c->boxhi[dim] - c->cutghost; w.hi = BIG; w.shift = -c->period[dim]; }
+    }
+}
+
+void ensure_gathered(phafd_hip_ctx *c, int need) {
+  if (need <= c->gcap) return;
+  int cap = std::max(need + need / 4 + 1024, 2 * c->nowned + 1024);
+  // grow preserving the first ngathered entries
+  DBuf<double4> nx_, ns_;
+  DBuf<int> no_, nl_;
+  nx_.alloc(cap); ns_.alloc(cap); no_.alloc(cap); nl_.alloc(cap);
+  if (c->ngathered > 0) {
+    HIPCHK(hipMemcpyAsync(nx_.p, c->gx.p, sizeof(double4) * c->ngathered, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(ns_.p, c->gshift.p, sizeof(double4) * c->ngathered, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(no_.p, c->gowner.p, sizeof(int) * c->ngathered, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(nl_.p, c->glabel.p, sizeof(int) * c->ngathered, hipMemcpyDeviceToDevice, c->stream));
+    sync(c);
+  }
+  std::swap(c->gx.p, nx_.p); std::swap(c->gx.n, nx_.n);
+  std::swap(c->gshift.p, ns_.p); std::swap(c->gshift.n, ns_.n);
+  std::swap(c->gowner.p, no_.p); std::swap(c->gowner.n, no_.n);
+  std::swap(c->glabel.p, nl_.p); std::swap(c->glabel.n, nl_.n);
+  c->gF.alloc(cap);
+  c->gcap = cap;
+}
+
+// ordered compaction of flags[0..n): positions into tmp_pos, returns count
+int compact_count(phafd_hip_ctx *c, int n) {
+  scan_exclusive_i32(c->tmp_flag.p, c->tmp_pos.p, (size_t)n, c->stream);
+  return read_int(c, c->tmp_pos.p + n);
+}
+
+// CommBrick::borders, src/comm_brick.cpp:263-426 (sendproc == me branches)
+void borders(phafd_hip_ctx *c) {
+  const int n = c->nowned;
+  c->tmp_flag.alloc((size_t)std::max(n, 1) * 4 + 16);
+  c->tmp_pos.alloc((size_t)std::max(n, 1) * 4 + 16);
+  c->tmp_cat.alloc((size_t)std::max(n, 1) + 1);
+  c->ngathered = c->nlocal = c->nghost = 0;
+  ensure_gathered(c, n + 16);
+  // copy swap
+  SwapDev &z = c->swaps[0];
+  const int zstart = (int)c->sublo[2], zend = (int)c->subhi[2];
+  launch_copy_classify(n, c->x.p, (double)zstart, (double)zend, z.lo, z.hi, c->tmp_cat.p, c->tmp_flag.p, c->stream);
+  const int n0 = compact_count(c, n);
+  z.send.alloc(n0 + 1);
+  launch_copy_scatter(n, c->x.p, c->type.p, c->tmp_cat.p, c->tmp_pos.p, c->period[2], z.send.p, c->gview(), c->stream);
+  z.first = 0;
+  z.count = n0;
+  c->ngathered = n0;
+  // nlocal = LOCAL entries of the copy swap
+  {
+    std::vector<int> lab(n0);
+    if (n0) {
+      HIPCHK(hipMemcpyAsync(lab.data(), c->glabel.p, sizeof(int) * n0, hipMemcpyDeviceToHost, c->stream));
+      sync(c);
+    }
+    c->nlocal = (int)std::count(lab.begin(), lab.end(), 1);
+  }
+  // image swaps
+  size_t iswap = 1;
+  while (iswap < c->swaps.size()) {
+    const int nscan = c->ngathered;
+    for (int ineed = 0; ineed < 2; ineed++, iswap++) {
+      SwapDev &w = c->swaps[iswap];
+      c->tmp_flag.alloc((size_t)nscan + 16);
+      c->tmp_pos.alloc((size_t)nscan + 16);
+      launch_img_flag(nscan, c->gx.p, w.dim, w.lo, w.hi, c->tmp_flag.p, c->stream);
+      const int cnt = compact_count(c, nscan);
+      ensure_gathered(c, c->ngathered + cnt + 16);
+      w.send.alloc(cnt + 1);
+      w.first = c->ngathered;
+      w.count = cnt;
+      launch_img_scatter(nscan, c->tmp_flag.p, c->tmp_pos.p, w.dim, w.shift, w.first, w.send.p, c->gview(), c->stream);
+      c->ngathered += cnt;
+    }
+  }
+  c->nghost = c->ngathered - c->nlocal;
+}
+
+void forward_comm(phafd_hip_ctx *c) { launch_forward(c->ngathered, c->x.p, c->gview(), c->stream); }
+
+void reverse_comm(phafd_hip_ctx *c) {
+  for (int iswap = (int)c->swaps.size() - 1; iswap >= 1; iswap--) {
+    SwapDev &w = c->swaps[iswap];
+    launch_reverse(w.count, w.send.p, w.first, c->gF.p, c->gF.p, c->stream);
+  }
+  SwapDev &z = c->swaps[0];
+  launch_reverse(z.count, z.send.p, z.first, c->F.p, c->gF.p, c->stream);
+}
+
+// ------------------------------------------------------------ neighbour
+// NBinStandard::setup_bins, src/nbin_standard.cpp:53-175
+void setup_bins(phafd_hip_ctx *c) {
+  BinGeom &b = c->bg;
+  double cutghost = c->cutneigh;
+  double bsubboxlo[3], bsubboxhi[3], bbox[3];
+  for (int d = 0; d < 3; d++) {
+    bsubboxlo[d] = c->sublo[d] - cutghost;
+    bsubboxhi[d] = c->subhi[d] + cutghost;
+    bbox[d] = c->boxhi[d] - c->boxlo[d];
+    b.boxlo[d] = c->boxlo[d];
+    b.boxhi[d] = c->boxhi[d];
+  }
+  double binsize_optimal = 0.5 * c->cutneigh;
+  if (binsize_optimal == 0.0) binsize_optimal = bbox[0];
+  double binsizeinv = 1.0 / binsize_optimal;
+  b.nbinx = (int)(bbox[0] * binsizeinv);
+  b.nbiny = (int)(bbox[1] * binsizeinv);
+  b.nbinz = (int)(bbox[2] * binsizeinv);
+  if (b.nbinx == 0) b.nbinx = 1;
+  if (b.nbiny == 0) b.nbiny = 1;
+  if (b.nbinz == 0) b.nbinz = 1;
+  b.binsizex = bbox[0] / b.nbinx;
+  b.binsizey = bbox[1] / b.nbiny;
+  b.binsizez = bbox[2] / b.nbinz;
+  b.bininvx = 1.0 / b.binsizex;
+  b.bininvy = 1.0 / b.binsizey;
+  b.bininvz = 1.0 / b.binsizez;
+  if (binsize_optimal * b.bininvx > 100 || binsize_optimal * b.bininvy > 100 ||
+      binsize_optimal * b.bininvz > 100)
+    throw std::runtime_error("Cannot use neighbor bins - box size << cutoff");
+  int hi[3];
+  int *lo[3] = {&b.mbinxlo, &b.mbinylo, &b.mbinzlo};
+  const double inv[3] = {b.bininvx, b.bininvy, b.bininvz};
+  for (int d = 0; d < 3; d++) {
+    double coord = bsubboxlo[d] - SMALL * bbox[d];
+    *lo[d] = (int)((coord - c->boxlo[d]) * inv[d]);
+    if (coord < c->boxlo[d]) *lo[d] = *lo[d] - 1;
+    coord = bsubboxhi[d] + SMALL * bbox[d];
+    hi[d] = (int)((coord - c->boxlo[d]) * inv[d]);
+    *lo[d] -= 1;
+    hi[d] += 1;
+  }
+  b.mbinx = hi[0] - b.mbinxlo + 1;
+  b.mbiny = hi[1] - b.mbinylo + 1;
+  b.mbinz = hi[2] - b.mbinzlo + 1;
+  b.mbins = b.mbinx * b.mbiny * b.mbinz + 1;
+}
+
+// NStencil create_setup / bin_distance / full + half create,
+// src/nstencil.cpp:101-143, src/nstencil_full_bin_3d.cpp:26-37,
+// src/nstencil_half_bin_3d.cpp:26-38
+void create_stencils(phafd_hip_ctx *c) {
+  const BinGeom &b = c->bg;
+  const double cutneighsq = c->cutneigh * c->cutneigh;
+  auto bin_distance = [&](int i, int j, int k) {
+    double delx, dely, delz;
+    if (i > 0) delx = (i - 1) * b.binsizex; else if (i == 0) delx = 0.0; else delx = (i + 1) * b.binsizex;
+    if (j > 0) dely = (j - 1) * b.binsizey; else if (j == 0) dely = 0.0; else dely = (j + 1) * b.binsizey;
+    if (k > 0) delz = (k - 1) * b.binsizez; else if (k == 0) delz = 0.0; else delz = (k + 1) * b.binsizez;
+    return (delx * delx + dely * dely + delz * delz);
+  };
+  int sx = (int)(c->cutneigh * b.bininvx); if (sx * b.binsizex < c->cutneigh) sx++;
+  int sy = (int)(c->cutneigh * b.bininvy); if (sy * b.binsizey < c->cutneigh) sy++;
+  int sz = (int)(c->cutneigh * b.bininvz); if (sz * b.binsizez < c->cutneigh) sz++;
+  c->stencil_full.clear();
+  for (int k = -sz; k <= sz; k++)
+    for (int j = -sy; j <= sy; j++)
+      for (int i = -sx; i <= sx; i++)
+        if (bin_distance(i, j, k) < cutneighsq)
+          c->stencil_full.push_back(k * b.mbiny * b.mbinx + j * b.mbinx + i);
+  c->stencil_half.clear();
+  for (int k = 0; k <= sz; k++)
+    for (int j = -sy; j <= sy; j++)
+      for (int i = -sx; i <= sx; i++)
+        if (k > 0 || j > 0 || (j == 0 && i > 0))
+          if (bin_distance(i, j, k) < cutneighsq)
+            c->stencil_half.push_back(k * b.mbiny * b.mbinx + j * b.mbinx + i);
+  c->d_stencil_full.alloc(c->stencil_full.size() + 1);
+  c->d_stencil_half.alloc(c->stencil_half.size() + 1);
+  if (!c->stencil_full.empty())
+    HIPCHK(hipMemcpy(c->d_stencil_full.p, c->stencil_full.data(), sizeof(int) * c->stencil_full.size(), hipMemcpyHostToDevice));
+  if (!c->stencil_half.empty())
+    HIPCHK(hipMemcpy(c->d_stencil_half.p, c->stencil_half.data(), sizeof(int) * c->stencil_half.size(), hipMemcpyHostToDevice));
+}
+
+// NBinStandard::bin_atoms as a counting sort, src/nbin_standard.cpp:181-229
+void bin_atoms(phafd_hip_ctx *c) {
+  const int ng = c->ngathered, nb = c->bg.mbins;
+  c->a2b.alloc((size_t)ng + 1);
+  c->binatoms.alloc((size_t)ng + 1);
+  c->bincount.alloc((size_t)nb + 1);
+  c->bincursor.alloc((size_t)nb + 1);
+  c->binstart.alloc((size_t)nb + 2);
+  launch_fill_i32(nb, c->bincount.p, 0, c->stream);
+  launch_fill_i32(nb, c->bincursor.p, 0, c->stream);
+  launch_fill_i32(1, c->flags.p + 2, 0, c->stream);
+  launch_atom2bin(ng, c->gx.p, c->bg, c->a2b.p, c->bincount.p, c->flags.p + 2, c->stream);
+  if (read_int(c, c->flags.p + 2))
+    throw std::runtime_error("Non-numeric positions - simulation unstable (atom outside the bin grid)");
+  scan_exclusive_i32(c->bincount.p, c->binstart.p, (size_t)nb, c->stream);
+  launch_bin_place(ng, c->a2b.p, c->binstart.p, c->bincursor.p, c->binatoms.p, c->stream);
+  launch_bin_sort(nb, c->binstart.p, c->binatoms.p, c->glabel.p, c->stream);
+}
+
+// NPairGridBin::build (src/npair_grid_bin.cpp:22-127)
+void build_gridlist(phafd_hip_ctx *c) {
+  GridView gv = c->gridview();
+  const double cutneighsq = c->cutneigh * c->cutneigh;
+  c->g_cnt.alloc(c->N + 1);
+  c->g_off.alloc(c->N + 1);
+  probe_begin(K_GRID_COUNT, c->stream);
+  launch_grid_list(false, gv, c->bg, c->d_stencil_full.p, (int)c->stencil_full.size(), c->binstart.p,
+                   c->binatoms.p, c->gx.p, cutneighsq, c->nowned, c->g_cnt.p, nullptr, nullptr,
+                   c->flags.p + 1, c->stream);
+  probe_end(K_GRID_COUNT, c->stream, 4.0 * c->N);
+  scan_exclusive_i32_to_i64(c->g_cnt.p, c->g_off.p, c->N, c->stream);
+  int64_t tot = 0;
+  HIPCHK(hipMemcpyAsync(&tot, c->g_off.p + c->N, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  c->g_nent = tot;
+  c->g_neigh.alloc((size_t)tot + 1);
+  probe_begin(K_GRID_FILL, c->stream);
+  launch_grid_list(true, gv, c->bg, c->d_stencil_full.p, (int)c->stencil_full.size(), c->binstart.p,
+                   c->binatoms.p, c->gx.p, cutneighsq, c->nowned, c->g_cnt.p, c->g_off.p, c->g_neigh.p,
+                   c->flags.p + 1, c->stream);
+  probe_end(K_GRID_FILL, c->stream, 8.0 * c->N + 4.0 * tot);
+}
+
+// NPairHalfBinNewton::build (src/npair_half_bin_newton.cpp:38-124) plus the
+// transposed slots used by the deterministic force gather
+void build_halflist(phafd_hip_ctx *c) {
+  const int ng = c->ngathered;
+  const double cutneighsq = c->cutneigh * c->cutneigh;
+  c->tmp_flag.alloc((size_t)ng + 16);
+  c->tmp_pos.alloc((size_t)ng + 16);
+  launch_local_flag(ng, c->glabel.p, c->tmp_flag.p, c->stream);
+  const int nrows = compact_count(c, ng);
+  c->h_nrows = nrows;
+  c->h_ilist.alloc((size_t)nrows + 1);
+  c->h_rowof.alloc((size_t)ng + 1);
+  launch_local_scatter(ng, c->tmp_flag.p, c->tmp_pos.p, c->h_ilist.p, c->h_rowof.p, c->stream);
+  c->h_cnt.alloc((size_t)nrows + 1);
+  c->h_off.alloc((size_t)nrows + 2);
+  launch_half_list(false, nrows, c->h_ilist.p, c->a2b.p, c->binstart.p, c->binatoms.p, c->glabel.p,
+                   c->d_stencil_half.p, (int)c->stencil_half.size(), c->gx.p, cutneighsq, c->nowned,
+                   c->h_cnt.p, nullptr, nullptr, c->flags.p + 1, c->stream);
+  scan_exclusive_i32(c->h_cnt.p, c->h_off.p, (size_t)nrows, c->stream);
+  const int nent = read_int(c, c->h_off.p + nrows);
+  c->h_nent = nent;
+  c->h_neigh.alloc((size_t)nent + 1);
+  launch_half_list(true, nrows, c->h_ilist.p, c->a2b.p, c->binstart.p, c->binatoms.p, c->glabel.p,
+                   c->d_stencil_half.p, (int)c->stencil_half.size(), c->gx.p, cutneighsq, c->nowned,
+                   c->h_cnt.p, c->h_off.p, c->h_neigh.p, c->flags.p + 1, c->stream);
+  c->t_cnt.alloc((size_t)ng + 1);
+  c->t_cur.alloc((size_t)ng + 1);
+  c->t_off.alloc((size_t)ng + 2);
+  c->t_pair.alloc((size_t)nent + 1);
+  c->t_i.alloc((size_t)nent + 1);
+  launch_fill_i32(ng, c->t_cnt.p, 0, c->stream);
+  launch_fill_i32(ng, c->t_cur.p, 0, c->stream);
+  launch_t_count(nent, c->h_neigh.p, c->nowned, c->t_cnt.p, c->stream);
+  scan_exclusive_i32(c->t_cnt.p, c->t_off.p, (size_t)ng, c->stream);
+  launch_t_place(nrows, c->h_ilist.p, c->h_off.p, c->h_neigh.p, c->nowned, c->t_off.p, c->t_cur.p,
+                 c->t_pair.p, c->t_i.p, c->stream);
+  launch_t_sort(ng, c->t_off.p, c->t_pair.p, c->t_i.p, c->stream);
+}
+
+// Neighbor::build, src/neighbor.cpp:99-125
+void neighbor_build(phafd_hip_ctx *c) {
+  c->ncalls++;
+  launch_copy4(c->nowned, c->x.p, c->xhold.p, c->stream);
+  bin_atoms(c);
+  if (c->need_full) build_gridlist(c);
+  if (c->need_half) build_halflist(c);
+  c->lists_valid = true;
+}
+
+bool neighbor_decide(phafd_hip_ctx *c) {
+  if (c->pending_decision >= 0) {
+    bool r = c->pending_decision != 0;
+    c->pending_decision = -1;
+    return r;
+  }
+  launch_fill_i32(1, c->flags.p, 0, c->stream);
+  launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->flags.p, c->stream);
+  const int f = read_int(c, c->flags.p);
+  if (f & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
+  return (f & 1) != 0;
+}
+
+void upload_pair(PairDef &p) {
+  int nt = (int)p.eps.size();
+  p.d_eps.alloc(nt); p.d_phit.alloc(nt); p.d_cutsq.alloc(nt);
+  HIPCHK(hipMemcpy(p.d_eps.p, p.eps.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.d_phit.p, p.phit.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.d_cutsq.p, p.cutsq.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
+}
+
+GAParams ga_params(const PairDef &p) {
+  GAParams g;
+  g.style = p.style;
+  g.eps = p.d_eps.p; g.phit = p.d_phit.p; g.cutsq = p.d_cutsq.p;
+  g.em05 = exp(-0.5);
+  g.e05 = exp(0.5);
+  return g;
+}
+
+// ---------------------------------------------------------- grid hooks
+SpecConst spec_const(const phafd_hip_ctx *c) {
+  SpecConst k;
+  k.dqx = 2 * PI_REF / c->period[0];      // Domain::dqx
+  k.dqy_kz = 2 * PI_REF / c->period[2];   // Domain::dqy
+  k.dqz_ky = 2 * PI_REF / c->period[1];   // Domain::dqz
+  return k;
+}
+
+// FixGridConjugate::start_of_step / FixGridGradPhi::start_of_step,
+// src/fixgrid_conjugate.cpp:62-116: r2c, grad multiply, three c2r, / N
+void compute_gradphi(phafd_hip_ctx *c, double2 *phihat_dst) {
+  fft_r2c(c->plan, c->phi.p, phihat_dst, c->stream);
+  grad_multiply(c->plan, phihat_dst, c->ghat[0].p, c->ghat[1].p, c->ghat[2].p, spec_const(c), c->stream);
+  double2 *ins[3] = {c->ghat[0].p, c->ghat[1].p, c->ghat[2].p};
+  double *outs[3] = {c->grad[0].p, c->grad[1].p, c->grad[2].p};
+  const double factor = (double)c->nx * c->ny * c->nz;
+  fft_c2r_batch(c->plan, ins, outs, 3, factor, c->stream);
+}
+
+UpdateParams update_params(const phafd_hip_ctx *c, const FixDef &f) {
+  UpdateParams u;
+  u.mobility = f.mobility; u.temp = f.temp; u.volFH = f.volFH; u.gamma = f.gamma; u.dt = c->dt;
+  u.normalization = 1.0 / ((double)c->nx * c->nz * c->ny);   // src/conjugate_volfrac.cpp:78
+  u.qx_scale = 2 * PI_REF / c->period[0];
+  u.qky_scale = 2 * PI_REF / c->period[1];
+  u.qkz_scale = 2 * PI_REF / c->period[2];
+  return u;
+}
+
+// ---------------------------------------------------------------- phases
+void phase_start(phafd_hip_ctx *c) {
+  for (auto &fp : c->fixes) {
+    FixDef &f = *fp;
+    if (f.kind == FixDef::CONJ) compute_gradphi(c, c->phihat.p);
+    else if (f.kind == FixDef::GRADPHI && (!f.once || c->timestep == c->firststep))
+      compute_gradphi(c, c->muhat.p);
+  }
+}
+
+void phase_neigh(phafd_hip_ctx *c) {
+  if (!c->neigh_set) return;
+  if (neighbor_decide(c)) {
+    launch_pbc(c->nowned, c->x.p, c->ux.p, c->image.p, c->box(), c->stream);
+    borders(c);
+    neighbor_build(c);
+  } else {
+    forward_comm(c);
+  }
+}
+
+// Integrate::run pair block, src/integrate.cpp:130-146 (one pass, finding F4)
+void phase_pair(phafd_hip_ctx *c, double *energy_ga = nullptr, double *energy_lj = nullptr) {
+  fill_zero(c->mu.p, sizeof(double) * c->N, c->stream);
+  launch_fill_d4(c->ngathered, c->gF.p, c->stream);
+  launch_fill_d4(c->nowned, c->F.p, c->stream);
+  if (!c->neigh_set || !c->lists_valid) return;
+  GridView gv = c->gridview();
+  for (auto &pp : c->pairs) {
+    PairDef &p = *pp;
+    if (p.style == 2) {
+      LJParams lj{p.lj1, p.lj2, p.ljcutsq};
+      probe_begin(K_LJ, c->stream);
+      launch_lj_force(c->ngathered, lj, c->gx.p, c->h_rowof.p, c->h_off.p, c->h_neigh.p, c->t_off.p,
+                      c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, energy_lj, c->stream);
+      probe_end(K_LJ, c->stream, 64.0 * c->ngathered + 4.0 * c->h_nent);
+    } else {
+      GAParams ga = ga_params(p);
+      probe_begin(K_GA_DEPOSIT, c->stream);
+      launch_ga_deposit(gv, ga, c->g_cnt.p, c->g_off.p, c->g_neigh.p, c->gx.p, c->nowned, energy_ga, c->stream);
+      probe_end(K_GA_DEPOSIT, c->stream, 28.0 * c->N + 4.0 * c->g_nent);
+      probe_begin(K_GA_FORCE, c->stream);
+      launch_ga_force(c->ngathered, gv, ga, c->gx.p, c->gF.p, c->stream);
+      probe_end(K_GA_FORCE, c->stream, 64.0 * c->ngathered);
+    }
+  }
+  reverse_comm(c);
+}
+
+void integrate_half(phafd_hip_ctx *c, FixDef &f) {
+  if (f.kind == FixDef::DRAG) {
+    launch_drag(f.nmem, f.members.p, c->x.p, c->F.p, f.drag, c->dt, c->stream);
+  } else if (f.kind == FixDef::CHAIN) {
+    ChainParams cp{f.r0, f.kbond, f.kbend, f.drag};
+    launch_chain_force(f.nmem, f.members.p, f.cstart.p, f.cend.p, c->ux.p, cp, f.fc.p, c->stream);
+    launch_chain_move(f.nmem, f.members.p, f.fc.p, c->F.p, c->ux.p, c->x.p, c->image.p, cp, c->dt, c->box(), c->stream);
+  }
+}
+
+void phase_initial(phafd_hip_ctx *c) {
+  for (auto &f : c->fixes) integrate_half(c, *f);
+}
+
+void phase_postforce(phafd_hip_ctx *c) {
+  for (auto &fp : c->fixes)
+    if (fp->kind == FixDef::FH)
+      floryhuggins(c->N, c->phi.p, c->mu.p, fp->temp / fp->volFH, fp->chi, c->counters.p, c->stream);
+}
+
+void phase_prefinal(phafd_hip_ctx *c, const FHPre *pre = nullptr) {
+  for (auto &fp : c->fixes)
+    if (fp->kind == FixDef::CONJ) fft_r2c(c->plan, c->mu.p, c->muhat.p, c->stream, pre);
+}
+
+void phase_final(phafd_hip_ctx *c) {
+  for (auto &fp : c->fixes) {
+    FixDef &f = *fp;
+    if (f.kind == FixDef::CONJ) {
+      const double2 *nz = (f.noise_mode == PHAFD_NOISE_FIELD && c->noise_set) ? c->noise_k.p : nullptr;
+      spectral_update(c->plan, c->phihat.p, c->muhat.p, nz, update_params(c, f), c->stream);
+    } else {
+      integrate_half(c, f);
+    }
+  }
+}
+
+void phase_postfinal(phafd_hip_ctx *c) {
+  for (auto &fp : c->fixes)
+    if (fp->kind == FixDef::CONJ) fft_c2r(c->plan, c->phihat.p, c->phi.p, 0.0, c->stream);
+}
+
+void run_phase(phafd_hip_ctx *c, int ph) {
+  switch (ph) {
+    case PHAFD_PH_START: phase_start(c); break;
+    case PHAFD_PH_NEIGH: phase_neigh(c); break;
+    case PHAFD_PH_PAIR: phase_pair(c); break;
+    case PHAFD_PH_INITIAL: phase_initial(c); break;
+    case PHAFD_PH_POSTFORCE: phase_postforce(c); break;
+    case PHAFD_PH_PREFINAL: phase_prefinal(c); break;
+    case PHAFD_PH_FINAL: phase_final(c); break;
+    case PHAFD_PH_POSTFINAL: phase_postfinal(c); break;
+    default: throw std::invalid_argument("bad phase");
+  }
+}
+
+void require_setup(phafd_hip_ctx *c) {
+  if (!c->setup_done) throw std::invalid_argument("call phafd_hip_setup before running");
+}
+
+// FH fused into the r2c of mu when the two hooks are adjacent and alone
+bool can_fuse_fh(phafd_hip_ctx *c, FHPre &pre) {
+  if (!c->fuse_fh) return false;
+  int nconj = 0, nfh = 0;
+  const FixDef *fh = nullptr;
+  for (auto &f : c->fixes) {
+    if (f->kind == FixDef::CONJ) nconj++;
+    if (f->kind == FixDef::FH) { nfh++; fh = f.get(); }
+  }
+  if (nconj != 1 || nfh != 1) return false;
+  pre.on = true;
+  pre.phi = c->phi.p;
+  pre.mu = c->mu.p;
+  pre.tv = fh->temp / fh->volFH;
+  pre.chi = fh->chi;
+  pre.counters = c->counters.p;
+  return true;
+}
+
+void run_steps(phafd_hip_ctx *c, int nsteps) {
+  require_setup(c);
+  FHPre pre;
+  const bool fused = can_fuse_fh(c, pre);
+  for (int it = 0; it < nsteps; it++) {
+    c->timestep++;
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    phase_start(c);
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    phase_neigh(c);
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    phase_pair(c);
+    HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    phase_initial(c);
+    HIPCHK(hipEventRecord(c->ev[4], c->stream));
+    if (fused) {
+      phase_prefinal(c, &pre);
+    } else {
+      phase_postforce(c);
+      phase_prefinal(c);
+    }
+    HIPCHK(hipEventRecord(c->ev[5], c->stream));
+    phase_final(c);
+    HIPCHK(hipEventRecord(c->ev[6], c->stream));
+    phase_postfinal(c);
+    HIPCHK(hipEventRecord(c->ev[7], c->stream));
+    // next step's rebuild decision (Neighbor::decide) + NaN check of
+    // phi(0,0,0) (src/integrate.cpp:190-201), read back together
+    if (c->neigh_set) {
+      launch_fill_i32(1, c->flags.p, 0, c->stream);
+      launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->flags.p, c->stream);
+      HIPCHK(hipMemcpyAsync(c->h_flags, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->phi.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(c->ev[8], c->stream));
+    sync(c);
+    probe_collect();
+    for (int k = 0; k < 7; k++) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
+      c->tacc[k] += ms;
+    }
+    float tot = 0;
+    HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[8]));
+    c->tacc[7] += tot;
+    if (c->neigh_set) {
+      if (c->h_flags[0] & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
+      c->pending_decision = c->h_flags[0] & 1;
+    }
+    double phi0;
+    std::memcpy(&phi0, c->h_flags + 2, sizeof(double));
+    if (std::isnan(phi0)) throw std::runtime_error("NAN encountered in phi.");
+  }
+}
+
+// flatten group chunks (src/group.cpp:31-207) into member lists
+void build_members(phafd_hip_ctx *c, FixDef &f) {
+  std::vector<int> mem, cs, ce;
+  for (size_t k = 0; k < f.start.size(); k++) {
+    if (f.start[k] < 0 || f.end[k] > c->nowned || f.end[k] < f.start[k])
+      throw std::invalid_argument("fix group chunk out of range");
+    for (int j = f.start[k]; j < f.end[k]; j++) {
+      mem.push_back(j);
+      cs.push_back(f.start[k]);
+      ce.push_back(f.end[k]);
+    }
+  }
+  f.nmem = (int)mem.size();
+  f.members.alloc(mem.size() + 1);
+  f.cstart.alloc(mem.size() + 1);
+  f.cend.alloc(mem.size() + 1);
+  f.fc.alloc(mem.size() + 1);
+  if (!mem.empty()) {
+    HIPCHK(hipMemcpy(f.members.p, mem.data(), sizeof(int) * mem.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(f.cstart.p, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(f.cend.p, ce.data(), sizeof(int) * ce.size(), hipMemcpyHostToDevice));
+  }
+}
+
+void copy_field_in(phafd_hip_ctx *c, double *d, const double *h) {
+  HIPCHK(hipMemcpyAsync(d, h, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
+  sync(c);
+}
+
+double *field_ptr(phafd_hip_ctx *c, int which) {
+  switch (which) {
+    case PHAFD_PHI: return c->phi.p;
+    case PHAFD_MU: return c->mu.p;
+    case PHAFD_GRADX: return c->grad[0].p;
+    case PHAFD_GRADY: return c->grad[1].p;
+    case PHAFD_GRADZ: return c->grad[2].p;
+  }
+  throw std::invalid_argument("unknown field");
+}
+
+std::vector<double4> to_d4(const double *xyz, int n) {
+  std::vector<double4> v(n);
+  for (int i = 0; i < n; i++) v[i] = make_double4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.0);
+  return v;
+}
+void from_d4(const std::vector<double4> &v, double *xyz) {
+  for (size_t i = 0; i < v.size(); i++) {
+    xyz[3 * i] = v[i].x;
+    xyz[3 * i + 1] = v[i].y;
+    xyz[3 * i + 2] = v[i].z;
+  }
+}
+template <class T>
+std::vector<T> download(phafd_hip_ctx *c, const T *d, size_t n) {
+  std::vector<T> h(n);
+  if (n) {
+    HIPCHK(hipMemcpyAsync(h.data(), d, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+  }
+  return h;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+int phafd_hip_create(const phafd_hip_desc *desc, phafd_hip_ctx **out) {
+  if (!desc || !out) return PHAFD_E_ARG;
+  *out = nullptr;
+  auto c = std::make_unique<phafd_hip_ctx>();
+  c->device = desc->device;
+  int rc = guard(c.get(), [&] {
+    if (desc->precision != PHAFD_FP64) throw std::invalid_argument("only PHAFD_FP64 is built");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (desc->device < 0 || desc->device >= ndev) throw std::invalid_argument("no such HIP device");
+    HIPCHK(hipSetDevice(desc->device));
+    c->plan.init(desc->nx, desc->ny, desc->nz);
+    c->nx = desc->nx; c->ny = desc->ny; c->nz = desc->nz;
+    for (int d = 0; d < 3; d++) {
+      if (!(desc->period[d] > 0)) throw std::invalid_argument("need to set box dimensions.");
+      c->boxlo[d] = desc->boxlo[d];
+      c->period[d] = desc->period[d];
+      c->boxhi[d] = c->boxlo[d] + c->period[d];   // src/domain.cpp:71-72
+    }
+    // Domain::set_subbox, src/domain.cpp:78-99 (local0start = 0)
+    const double dzz = c->period[2] / c->nz;
+    c->sublo[0] = c->boxlo[0]; c->sublo[1] = c->boxlo[1]; c->sublo[2] = dzz * 0 + c->boxlo[2];
+    c->subhi[0] = c->boxhi[0]; c->subhi[1] = c->boxhi[1]; c->subhi[2] = dzz * (0 + c->nz) + c->boxlo[2];
+    c->zghost = desc->zghost;
+    c->N = c->plan.N();
+    c->M = c->plan.M();
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->phi.alloc(c->N); c->mu.alloc(c->N);
+    for (int d = 0; d < 3; d++) { c->grad[d].alloc(c->N); c->ghat[d].alloc(c->M); }
+    c->phihat.alloc(c->M); c->muhat.alloc(c->M);
+    c->counters.alloc(2);
+    c->flags.alloc(4);
+    c->esum.alloc(8);
+    HIPCHK(hipHostMalloc((void **)&c->h_flags, 64, hipHostMallocDefault));
+    for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+    fill_zero(c->phi.p, sizeof(double) * c->N, c->stream);
+    fill_zero(c->mu.p, sizeof(double) * c->N, c->stream);
+    for (int d = 0; d < 3; d++) fill_zero(c->grad[d].p, sizeof(double) * c->N, c->stream);
+    fill_zero(c->counters.p, 16, c->stream);
+    fill_zero(c->flags.p, 16, c->stream);
+    sync(c.get());
+    return 0;
+  });
+  if (rc != 0) {
+    g_create_err = c->err;   // reported by phafd_hip_last_error(NULL)
+    return rc;
+  }
+  *out = c.release();
+  return 0;
+}
+
+int phafd_hip_destroy(phafd_hip_ctx *c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+  if (c->h_flags) (void)hipHostFree(c->h_flags);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+const char *phafd_hip_last_error(const phafd_hip_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int phafd_hip_set_field(phafd_hip_ctx *c, int which, const double *h) {
+  return guard(c, [&] {
+    copy_field_in(c, field_ptr(c, which), h);
+    return 0;
+  });
+}
+int phafd_hip_get_field(phafd_hip_ctx *c, int which, double *h) {
+  return guard(c, [&] {
+    HIPCHK(hipMemcpyAsync(h, field_ptr(c, which), sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return 0;
+  });
+}
+int phafd_hip_get_spectrum(phafd_hip_ctx *c, int which, double *h) {
+  return guard(c, [&] {
+    const double2 *k = which == PHAFD_PHIHAT ? c->phihat.p : which == PHAFD_MUHAT ? c->muhat.p : nullptr;
+    if (!k) throw std::invalid_argument("unknown spectrum");
+    c->canon_tmp.alloc(c->M);
+    spec_k_to_canon(c->plan, k, c->canon_tmp.p, c->stream);
+    HIPCHK(hipMemcpyAsync(h, c->canon_tmp.p, sizeof(double2) * c->M, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return 0;
+  });
+}
+
+int phafd_hip_set_atoms(phafd_hip_ctx *c, int n, const double *xyz, const int *type, const int *tag,
+                        const int *mol) {
+  return guard(c, [&] {
+    if (n < 0 || (n > 0 && (!xyz || !type))) throw std::invalid_argument("bad atom arrays");
+    c->nowned = n;
+    c->h_type.assign(type, type + n);
+    c->h_tag.resize(n);
+    c->h_mol.assign(n, 0);
+    int tmax = -1;
+    for (int i = 0; i < n; i++) {
+      c->h_tag[i] = tag ? tag[i] : i;
+      if (mol) c->h_mol[i] = mol[i];
+      if (type[i] < 0) throw std::invalid_argument("negative atom type");
+      tmax = std::max(tmax, type[i]);
+    }
+    std::set<int> tset(c->h_tag.begin(), c->h_tag.end());
+    if ((int)tset.size() != n) throw std::runtime_error("duplicate IDs found on processor(s).");
+    c->ntypes = tmax + 1;   // src/atom.cpp:190-203
+    c->x.alloc(n + 1); c->ux.alloc(n + 1); c->F.alloc(n + 1); c->xhold.alloc(n + 1);
+    c->image.alloc(n + 1); c->type.alloc(n + 1);
+    if (n) {
+      std::vector<double4> v = to_d4(xyz, n);
+      HIPCHK(hipMemcpy(c->x.p, v.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->xhold.p, v.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+      std::vector<int> img(n, (512 << 20) | (512 << 10) | 512);   // Domain::set_image
+      HIPCHK(hipMemcpy(c->image.p, img.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->type.p, type, sizeof(int) * n, hipMemcpyHostToDevice));
+      launch_fill_d4(n, c->F.p, c->stream);
+      launch_fill_d4(n, c->ux.p, c->stream);
+      sync(c);
+    }
+    c->gcap = 0;
+    c->ngathered = 0;
+    c->atoms_set = true;
+    return 0;
+  });
+}
+int phafd_hip_get_atoms(phafd_hip_ctx *c, double *xyz, double *uxyz, int *image) {
+  return guard(c, [&] {
+    if (xyz) from_d4(download(c, c->x.p, c->nowned), xyz);
+    if (uxyz) from_d4(download(c, c->ux.p, c->nowned), uxyz);
+    if (image && c->nowned) {
+      auto v = download(c, c->image.p, c->nowned);
+      std::memcpy(image, v.data(), sizeof(int) * c->nowned);
+    }
+    return 0;
+  });
+}
+int phafd_hip_get_forces(phafd_hip_ctx *c, double *f) {
+  return guard(c, [&] {
+    from_d4(download(c, c->F.p, c->nowned), f);
+    return 0;
+  });
+}
+
+int phafd_hip_pair_gridatom(phafd_hip_ctx *c, int style, double cut) {
+  return guard(c, [&] {
+    if (c->ntypes <= 0) throw std::runtime_error("Cannot create pair style before atoms are made.");
+    if (style != PHAFD_PAIR_GAUSSIAN && style != PHAFD_PAIR_LJISH) throw std::invalid_argument("Unrecognised pair style.");
+    auto p = std::make_unique<PairDef>();
+    p->style = style;
+    p->maxcut = cut;
+    p->eps.assign(c->ntypes, 1.0);      // defaults, src/pair_gridatom_gaussian.cpp:134-139
+    p->phit.assign(c->ntypes, 0.1);
+    p->cutsq.assign(c->ntypes, cut * cut);
+    c->pairs.push_back(std::move(p));
+    c->need_full = true;
+    return (int)c->pairs.size() - 1;
+  });
+}
+int phafd_hip_pair_coeff(phafd_hip_ctx *c, int ipair, int type, double eps, double phit, double nucwidth) {
+  return guard(c, [&] {
+    if (ipair < 0 || ipair >= (int)c->pairs.size()) throw std::invalid_argument("no such pair");
+    PairDef &p = *c->pairs[ipair];
+    if (p.style == 2) return 0;   // PairLJCut::coeff is empty (src/pair_lj_cut.cpp:119-125)
+    if (type < 0 || type >= (int)p.eps.size()) throw std::runtime_error("Pair grid/atom atom type out of range.");
+    if (!std::isnan(eps)) p.eps[type] = eps;
+    if (!std::isnan(phit)) p.phit[type] = phit;
+    if (!std::isnan(nucwidth)) {
+      if (nucwidth > p.maxcut) throw std::runtime_error("Pair grid/atom nucwidth cannot be larger than maximum cut.");
+      p.cutsq[type] = nucwidth * nucwidth;
+    }
+    return 0;
+  });
+}
+int phafd_hip_pair_ljcut(phafd_hip_ctx *c, double cut, double eps, double sigma) {
+  return guard(c, [&] {
+    if (c->ntypes <= 0) throw std::runtime_error("Cannot create pair style before atoms are made.");
+    auto p = std::make_unique<PairDef>();
+    p->style = 2;
+    p->maxcut = cut;
+    p->lj1 = 48 * eps * pow(sigma, 12.0);
+    p->lj2 = 24 * eps * pow(sigma, 6.0);
+    p->ljcutsq = cut * cut;
+    c->pairs.push_back(std::move(p));
+    c->need_half = true;
+    return (int)c->pairs.size() - 1;
+  });
+}
+
+int phafd_hip_fix_conjugate(phafd_hip_ctx *c, int seed, double mobility, double temp, double volFH,
+                            double gamma, int noise_mode) {
+  return guard(c, [&] {
+    if (noise_mode != PHAFD_NOISE_OFF && noise_mode != PHAFD_NOISE_FIELD)
+      throw std::invalid_argument("noise mode must be off or field");
+    auto f = std::make_unique<FixDef>();
+    f->kind = FixDef::CONJ;
+    f->seed = seed;
+    f->mobility = mobility; f->temp = temp; f->volFH = volFH; f->gamma = gamma;
+    f->noise_mode = noise_mode;
+    c->fixes.push_back(std::move(f));
+    return 0;
+  });
+}
+int phafd_hip_set_noise(phafd_hip_ctx *c, const double *xi) {
+  return guard(c, [&] {
+    if (!xi) { c->noise_set = false; return 0; }
+    c->noise_k.alloc(c->M);
+    c->canon_tmp.alloc(c->M);
+    HIPCHK(hipMemcpyAsync(c->canon_tmp.p, xi, sizeof(double2) * c->M, hipMemcpyHostToDevice, c->stream));
+    spec_canon_to_k(c->plan, c->canon_tmp.p, c->noise_k.p, c->stream);
+    sync(c);
+    c->noise_set = true;
+    return 0;
+  });
+}
+int phafd_hip_fix_floryhuggins(phafd_hip_ctx *c, double temp, double volFH, double chi) {
+  return guard(c, [&] {
+    auto f = std::make_unique<FixDef>();
+    f->kind = FixDef::FH;
+    f->temp = temp; f->volFH = volFH; f->chi = chi;
+    c->fixes.push_back(std::move(f));
+    return 0;
+  });
+}
+int phafd_hip_fix_gradphi(phafd_hip_ctx *c, int once) {
+  return guard(c, [&] {
+    auto f = std::make_unique<FixDef>();
+    f->kind = FixDef::GRADPHI;
+    f->once = once;
+    c->fixes.push_back(std::move(f));
+    return 0;
+  });
+}
+int phafd_hip_fix_drag(phafd_hip_ctx *c, int nchunk, const int *start, const int *end, double drag) {
+  return guard(c, [&] {
+    if (c->ntypes <= 0) throw std::runtime_error("Fix requires atoms to be created.");
+    auto f = std::make_unique<FixDef>();
+    f->kind = FixDef::DRAG;
+    f->start.assign(start, start + nchunk);
+    f->end.assign(end, end + nchunk);
+    f->drag = drag;
+    build_members(c, *f);
+    c->fixes.push_back(std::move(f));
+    return 0;
+  });
+}
+int phafd_hip_fix_chain(phafd_hip_ctx *c, int nchunk, const int *start, const int *end, double r0,
+                        double kbond, double kbend, double drag) {
+  return guard(c, [&] {
+    if (c->ntypes <= 0) throw std::runtime_error("Fix requires atoms to be created.");
+    auto f = std::make_unique<FixDef>();
+    f->kind = FixDef::CHAIN;
+    f->start.assign(start, start + nchunk);
+    f->end.assign(end, end + nchunk);
+    f->r0 = r0; f->kbond = kbond; f->kbend = kbend; f->drag = drag;
+    build_members(c, *f);
+    c->fixes.push_back(std::move(f));
+    return 0;
+  });
+}
+
+int phafd_hip_neighbor(phafd_hip_ctx *c, double skin) {
+  return guard(c, [&] {
+    if (c->pairs.empty() && c->ntypes > 0) throw std::runtime_error("Must call neighbor after pairs are given.");
+    c->skin = skin;
+    double cutoff = 0.0;
+    for (auto &p : c->pairs) cutoff = std::max(cutoff, p->maxcut);
+    c->triggersq = 0.25 * skin * skin;
+    if (cutoff > 0.0) c->cutneigh = skin + cutoff;
+    setup_bins(c);
+    create_stencils(c);
+    comm_setup(c);
+    for (auto &p : c->pairs) if (p->style != 2) upload_pair(*p);
+    c->neigh_set = true;
+    return 0;
+  });
+}
+
+int phafd_hip_setup(phafd_hip_ctx *c, double dt) {
+  return guard(c, [&] {
+    if (dt <= 0.0) throw std::runtime_error("Cannot run simulation without setting dt > 0.");
+    c->dt = dt;
+    for (auto &p : c->pairs) if (p->style != 2) upload_pair(*p);
+    if (c->neigh_set && c->nowned > 0) {
+      launch_pbc(c->nowned, c->x.p, c->ux.p, c->image.p, c->box(), c->stream);
+      borders(c);
+      neighbor_build(c);
+    } else if (c->nowned > 0) {
+      launch_unmap(c->nowned, c->x.p, c->ux.p, c->image.p, c->box(), c->stream);
+    }
+    if (c->nowned > 0) launch_fill_d4(c->nowned, c->F.p, c->stream);
+    if (c->ngathered > 0) launch_fill_d4(c->ngathered, c->gF.p, c->stream);
+    c->pending_decision = -1;
+    sync(c);
+    c->setup_done = true;
+    return 0;
+  });
+}
+
+int phafd_hip_run(phafd_hip_ctx *c, int nsteps) {
+  return guard(c, [&] {
+    run_steps(c, nsteps);
+    return 0;
+  });
+}
+
+int phafd_hip_phase(phafd_hip_ctx *c, int phase) {
+  return guard(c, [&] {
+    require_setup(c);
+    if (phase == PHAFD_PH_START) c->timestep++;
+    c->pending_decision = -1;
+    run_phase(c, phase);
+    sync(c);
+    return 0;
+  });
+}
+
+int phafd_hip_counts(phafd_hip_ctx *c, int64_t *o) {
+  return guard(c, [&] {
+    unsigned long long cnt[2] = {0, 0};
+    int fl[4] = {0};
+    HIPCHK(hipMemcpyAsync(cnt, c->counters.p, 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(fl, c->flags.p, 16, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    o[0] = c->nowned; o[1] = c->ngathered; o[2] = c->nlocal; o[3] = c->nghost;
+    o[4] = c->ncalls; o[5] = (int64_t)cnt[0]; o[6] = (int64_t)cnt[1]; o[7] = fl[1];
+    return 0;
+  });
+}
+
+int phafd_hip_gridlist(phafd_hip_ctx *c, int64_t cap, int *ilist, int *numneigh, int *neigh,
+                       int64_t *inum, int64_t *nentries) {
+  return guard(c, [&] {
+    if (!c->lists_valid || !c->need_full) { *inum = 0; *nentries = 0; return 0; }
+    auto cnt = download(c, c->g_cnt.p, c->N);
+    int64_t ni = 0;
+    for (int v : cnt) ni += v > 0;
+    *inum = ni;
+    *nentries = c->g_nent;
+    if (!ilist) return 0;
+    if (ni > cap || c->g_nent > cap) throw std::invalid_argument("capacity too small");
+    auto nb = download(c, c->g_neigh.p, (size_t)c->g_nent);
+    int64_t w = 0;
+    for (size_t p = 0; p < c->N; p++)
+      if (cnt[p] > 0) { ilist[w] = (int)p; numneigh[w] = cnt[p]; w++; }
+    std::memcpy(neigh, nb.data(), sizeof(int) * nb.size());
+    return 0;
+  });
+}
+
+int phafd_hip_halflist(phafd_hip_ctx *c, int64_t cap, int *ilist, int *numneigh, int *neigh,
+                       int64_t *inum, int64_t *nentries) {
+  return guard(c, [&] {
+    if (!c->lists_valid || !c->need_half) { *inum = 0; *nentries = 0; return 0; }
+    *inum = c->h_nrows;
+    *nentries = c->h_nent;
+    if (!ilist) return 0;
+    if (c->h_nrows > cap || c->h_nent > cap) throw std::invalid_argument("capacity too small");
+    auto il = download(c, c->h_ilist.p, c->h_nrows);
+    auto cn = download(c, c->h_cnt.p, c->h_nrows);
+    auto nb = download(c, c->h_neigh.p, c->h_nent);
+    for (int r = 0; r < c->h_nrows; r++) { ilist[r] = il[r] + c->nowned; numneigh[r] = cn[r]; }
+    if (!nb.empty()) std::memcpy(neigh, nb.data(), sizeof(int) * nb.size());
+    return 0;
+  });
+}
+
+int phafd_hip_gathered(phafd_hip_ctx *c, double *xyz, int *tag, int *label, int *owner) {
+  return guard(c, [&] {
+    const int ng = c->ngathered;
+    if (xyz) from_d4(download(c, c->gx.p, ng), xyz);
+    auto own = download(c, c->gowner.p, ng);
+    if (owner && ng) std::memcpy(owner, own.data(), sizeof(int) * ng);
+    if (tag) for (int i = 0; i < ng; i++) tag[i] = c->h_tag[own[i]];
+    if (label && ng) {
+      auto lab = download(c, c->glabel.p, ng);
+      std::memcpy(label, lab.data(), sizeof(int) * ng);
+    }
+    return ng;
+  });
+}
+
+int phafd_hip_bins(phafd_hip_ctx *c, int *info, int *binhead, int *bins, int *atom2bin) {
+  return guard(c, [&] {
+    const BinGeom &b = c->bg;
+    if (info) {
+      int v[12] = {b.nbinx, b.nbiny, b.nbinz, b.mbinx, b.mbiny, b.mbinz, b.mbinxlo, b.mbinylo, b.mbinzlo,
+                   b.mbins, (int)c->stencil_full.size(), (int)c->stencil_half.size()};
+      std::memcpy(info, v, sizeof(v));
+    }
+    if (!c->lists_valid) return 0;
+    const int ng = c->ngathered, nb = b.mbins;
+    auto bs = download(c, c->binstart.p, (size_t)nb + 1);
+    auto ba = download(c, c->binatoms.p, (size_t)ng);
+    if (binhead) for (int k = 0; k < nb; k++) binhead[k] = bs[k] < bs[k + 1] ? ba[bs[k]] + c->nowned : -1;
+    if (bins) {
+      for (int k = 0; k < nb; k++)
+        for (int q = bs[k]; q < bs[k + 1]; q++) bins[ba[q]] = (q + 1 < bs[k + 1]) ? ba[q + 1] + c->nowned : -1;
+    }
+    if (atom2bin && ng) {
+      auto a = download(c, c->a2b.p, (size_t)ng);
+      std::memcpy(atom2bin, a.data(), sizeof(int) * ng);
+    }
+    return 0;
+  });
+}
+
+int phafd_hip_energy(phafd_hip_ctx *c, double *out) {
+  return guard(c, [&] {
+    fill_zero(c->esum.p, 8 * sizeof(double), c->stream);
+    for (auto &f : c->fixes) {
+      if (f->kind == FixDef::FH) {
+        // FH part only here; gradient part added by the CONJ fix below
+        reduce_energy_fh(c->N, c->phi.p, nullptr, nullptr, nullptr, f->temp / f->volFH, f->chi, 0.0,
+                         c->esum.p, c->stream);
+      } else if (f->kind == FixDef::CONJ) {
+        // square-gradient term: pass phi with chi = 0 would add FH terms, so
+        // accumulate only the gradient sum into esum[1] then move it
+        reduce_energy_fh(c->N, c->phi.p, c->grad[0].p, c->grad[1].p, c->grad[2].p, 0.0, 0.0,
+                         f->temp / f->volFH * f->gamma, c->esum.p + 2, c->stream);
+      } else if (f->kind == FixDef::CHAIN) {
+        ChainParams cp{f->r0, f->kbond, f->kbend, f->drag};
+        launch_chain_energy(f->nmem, f->members.p, f->cstart.p, f->cend.p, c->ux.p, cp, c->esum.p + 6, c->stream);
+      }
+    }
+    // pair energies: recompute the pair pass into scratch copies of mu / gF
+    if (c->neigh_set && c->lists_valid) {
+      DBuf<double> mu_save;
+      DBuf<double4> gf_save, f_save;
+      mu_save.alloc(c->N);
+      gf_save.alloc((size_t)c->ngathered + 1);
+      f_save.alloc((size_t)c->nowned + 1);
+      HIPCHK(hipMemcpyAsync(mu_save.p, c->mu.p, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
+      if (c->ngathered) HIPCHK(hipMemcpyAsync(gf_save.p, c->gF.p, sizeof(double4) * c->ngathered, hipMemcpyDeviceToDevice, c->stream));
+      if (c->nowned) HIPCHK(hipMemcpyAsync(f_save.p, c->F.p, sizeof(double4) * c->nowned, hipMemcpyDeviceToDevice, c->stream));
+      phase_pair(c, c->esum.p + 4, c->esum.p + 5);
+      HIPCHK(hipMemcpyAsync(c->mu.p, mu_save.p, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
+      if (c->ngathered) HIPCHK(hipMemcpyAsync(c->gF.p, gf_save.p, sizeof(double4) * c->ngathered, hipMemcpyDeviceToDevice, c->stream));
+      if (c->nowned) HIPCHK(hipMemcpyAsync(c->F.p, f_save.p, sizeof(double4) * c->nowned, hipMemcpyDeviceToDevice, c->stream));
+      sync(c);
+    }
+    double e[8];
+    HIPCHK(hipMemcpyAsync(e, c->esum.p, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    const double dv = c->dx() * c->dy() * c->dz();
+    out[0] = (e[0] + e[3]) * dv;   // FH (esum[0]) + gradient (esum[3])
+    out[1] = e[4];
+    out[2] = e[5];
+    out[3] = e[6];
+    return 0;
+  });
+}
+
+int phafd_hip_probe(phafd_hip_ctx *c, int kernel_id, double *out3) {
+  return guard(c, [&] {
+    static KernelProbe probe;
+    if (kernel_id >= 0) {
+      if (kernel_id >= K_NKERNELS) throw std::invalid_argument("unknown kernel id");
+      probe.id = kernel_id;
+      probe.ms = 0; probe.total_bytes = 0; probe.launches = 0; probe.used = 0; probe.bytes.clear();
+      g_probe = &probe;
+    } else {
+      g_probe = nullptr;
+    }
+    if (out3) { out3[0] = probe.ms; out3[1] = (double)probe.launches; out3[2] = probe.total_bytes; }
+    return 0;
+  });
+}
+
+int phafd_hip_timers(phafd_hip_ctx *c, double *ms, int reset) {
+  if (!c) return PHAFD_E_ARG;
+  if (ms) for (int k = 0; k < 8; k++) ms[k] = c->tacc[k];
+  if (reset) for (double &t : c->tacc) t = 0;
+  return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------- stateless kernels
+namespace {
+struct Scratch {
+  FFTPlan plan;
+  hipStream_t s = nullptr;
+  Scratch(int nx, int ny, int nz) {
+    plan.init(nx, ny, nz);
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  }
+  ~Scratch() { if (s) (void)hipStreamDestroy(s); }
+};
+template <class Fn>
+int sguard(Fn fn) {
+  try {
+    return fn();
+  } catch (const HipError &) {
+    return PHAFD_E_HIP;
+  } catch (const std::invalid_argument &) {
+    return PHAFD_E_ARG;
+  } catch (const std::exception &) {
+    return PHAFD_E_RUNTIME;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int phafd_hip_fft_r2c(int nx, int ny, int nz, const double *in, double *out) {
+  return sguard([&] {
+    Scratch sc(nx, ny, nz);
+    DBuf<double> r;
+    DBuf<double2> k, cn;
+    r.alloc(sc.plan.N()); k.alloc(sc.plan.M()); cn.alloc(sc.plan.M());
+    HIPCHK(hipMemcpy(r.p, in, sizeof(double) * sc.plan.N(), hipMemcpyHostToDevice));
+    fft_r2c(sc.plan, r.p, k.p, sc.s);
+    spec_k_to_canon(sc.plan, k.p, cn.p, sc.s);
+    HIPCHK(hipStreamSynchronize(sc.s));
+    HIPCHK(hipMemcpy(out, cn.p, sizeof(double2) * sc.plan.M(), hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+int phafd_hip_fft_c2r(int nx, int ny, int nz, const double *in, double *out) {
+  return sguard([&] {
+    Scratch sc(nx, ny, nz);
+    DBuf<double> r;
+    DBuf<double2> k, cn;
+    r.alloc(sc.plan.N()); k.alloc(sc.plan.M()); cn.alloc(sc.plan.M());
+    HIPCHK(hipMemcpy(cn.p, in, sizeof(double2) * sc.plan.M(), hipMemcpyHostToDevice));
+    spec_canon_to_k(sc.plan, cn.p, k.p, sc.s);
+    fft_c2r(sc.plan, k.p, r.p, 0.0, sc.s);
+    HIPCHK(hipStreamSynchronize(sc.s));
+    HIPCHK(hipMemcpy(out, r.p, sizeof(double) * sc.plan.N(), hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+int phafd_hip_gradphi(int nx, int ny, int nz, const double *period, const double *phi, double *gx,
+                      double *gy, double *gz) {
+  return sguard([&] {
+    Scratch sc(nx, ny, nz);
+    const size_t N = sc.plan.N(), M = sc.plan.M();
+    DBuf<double> r, g[3];
+    DBuf<double2> ph, gh[3];
+    r.alloc(N); ph.alloc(M);
+    for (int d = 0; d < 3; d++) { g[d].alloc(N); gh[d].alloc(M); }
+    HIPCHK(hipMemcpy(r.p, phi, sizeof(double) * N, hipMemcpyHostToDevice));
+    fft_r2c(sc.plan, r.p, ph.p, sc.s);
+    SpecConst k{2 * PI_REF / period[0], 2 * PI_REF / period[2], 2 * PI_REF / period[1]};
+    grad_multiply(sc.plan, ph.p, gh[0].p, gh[1].p, gh[2].p, k, sc.s);
+    double2 *ins[3] = {gh[0].p, gh[1].p, gh[2].p};
+    double *outs[3] = {g[0].p, g[1].p, g[2].p};
+    fft_c2r_batch(sc.plan, ins, outs, 3, (double)nx * ny * nz, sc.s);
+    HIPCHK(hipStreamSynchronize(sc.s));
+    HIPCHK(hipMemcpy(gx, g[0].p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(gy, g[1].p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(gz, g[2].p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+int phafd_hip_floryhuggins(int64_t n, double *phi, double *mu, double temp, double volFH, double chi,
+                           int64_t *nless, int64_t *ngreat) {
+  return sguard([&] {
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    DBuf<double> p, m;
+    DBuf<unsigned long long> cn;
+    p.alloc(n); m.alloc(n); cn.alloc(2);
+    HIPCHK(hipMemcpy(p.p, phi, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m.p, mu, sizeof(double) * n, hipMemcpyHostToDevice));
+    fill_zero(cn.p, 16, s);
+    floryhuggins((size_t)n, p.p, m.p, temp / volFH, chi, cn.p, s);
+    HIPCHK(hipStreamSynchronize(s));
+    unsigned long long h[2];
+    HIPCHK(hipMemcpy(phi, p.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(mu, m.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h, cn.p, 16, hipMemcpyDeviceToHost));
+    HIPCHK(hipStreamDestroy(s));
+    if (nless) *nless = (int64_t)h[0];
+    if (ngreat) *ngreat = (int64_t)h[1];
+    return 0;
+  });
+}
+int phafd_hip_spectral_update(int nx, int ny, int nz, const double *period, double *phihat,
+                              const double *muhat, double mobility, double temp, double volFH,
+                              double gamma, double dt, const double *noise) {
+  return sguard([&] {
+    Scratch sc(nx, ny, nz);
+    const size_t M = sc.plan.M();
+    DBuf<double2> cn, pk, mk, nk;
+    cn.alloc(M); pk.alloc(M); mk.alloc(M);
+    HIPCHK(hipMemcpy(cn.p, phihat, sizeof(double2) * M, hipMemcpyHostToDevice));
+    spec_canon_to_k(sc.plan, cn.p, pk.p, sc.s);
+    HIPCHK(hipStreamSynchronize(sc.s));
+    HIPCHK(hipMemcpy(cn.p, muhat, sizeof(double2) * M, hipMemcpyHostToDevice));
+    spec_canon_to_k(sc.plan, cn.p, mk.p, sc.s);
+    if (noise) {
+      HIPCHK(hipStreamSynchronize(sc.s));
+      nk.alloc(M);
+      HIPCHK(hipMemcpy(cn.p, noise, sizeof(double2) * M, hipMemcpyHostToDevice));
+      spec_canon_to_k(sc.plan, cn.p, nk.p, sc.s);
+    }
+    UpdateParams u;
+    u.mobility = mobility; u.temp = temp; u.volFH = volFH; u.gamma = gamma; u.dt = dt;
+    u.normalization = 1.0 / ((double)nx * nz * ny);
+    u.qx_scale = 2 * PI_REF / period[0];
+    u.qky_scale = 2 * PI_REF / period[1];
+    u.qkz_scale = 2 * PI_REF / period[2];
+    spectral_update(sc.plan, pk.p, mk.p, noise ? nk.p : nullptr, u, sc.s);
+    spec_k_to_canon(sc.plan, pk.p, cn.p, sc.s);
+    HIPCHK(hipStreamSynchronize(sc.s));
+    HIPCHK(hipMemcpy(phihat, cn.p, sizeof(double2) * M, hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+
+}  // extern "C"

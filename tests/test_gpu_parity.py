@@ -1,0 +1,178 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Tolerances (relative L2 unless noted):
+  FFT / grad(phi) ....... 1e-12   (different FFT algorithms, FP64)
+  spectral update ....... bit-exact given the same spectra
+  Flory-Huggins ......... 1e-15   (device log vs glibc log, <= 1 ulp)
+  gathered atoms, bins,
+  grid and half lists ... bit-exact (integer / index results)
+  mu, forces, one step .. 1e-10   (the stated FP64 tolerance)
+  200-step trajectory ... 1e-10 for phi, positions
+"""
+import os
+
+import numpy as np
+import pytest
+
+import phafd_amd as P
+from oracle import oracle_ctypes as O
+from tests.helpers import workload, make_gpu, make_oracle, rel_l2
+
+pytestmark = pytest.mark.gpu
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "fft_golden.npz"))
+
+
+def _tags():
+    return sorted({k.split("_", 2)[2] for k in GOLD.files if min(map(int, k.split("_", 2)[2].split("x"))) >= 4})
+
+
+@pytest.mark.parametrize("tag", _tags())
+def test_fft_golden(tag):
+    a = GOLD[f"r2c_in_{tag}"]
+    ref = GOLD[f"r2c_out_{tag}"]
+    assert rel_l2(P.fft_r2c(a).view(float), ref.view(float)) < 1e-13
+    c = GOLD[f"c2r_in_{tag}"]
+    nx = int(tag.split("x")[2])
+    assert rel_l2(P.fft_c2r(c, nx), GOLD[f"c2r_out_{tag}"]) < 1e-13
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 64), (32, 16, 128), (128, 8, 4), (4, 256, 512), (16, 1024, 8)])
+def test_fft_vs_oracle(shape):
+    rng = np.random.default_rng(sum(shape))
+    a = rng.standard_normal(shape)
+    ref = O.fft_r2c(a)
+    got = P.fft_r2c(a)
+    assert rel_l2(got.view(float), ref.view(float)) < 1e-13
+    c = ref * (1 + 0.1j)
+    assert rel_l2(P.fft_c2r(c, shape[2]), O.fft_c2r(c, shape[2])) < 1e-13
+
+
+def test_gradphi_vs_oracle():
+    shape = (32, 16, 64)
+    L = [64.0, 12.0, 40.0]
+    phi = np.random.default_rng(1).standard_normal(shape)
+    for g, r in zip(P.gradphi(phi, L), O.gradphi(phi, L)):
+        assert rel_l2(g, r) < 1e-12
+
+
+def test_floryhuggins_vs_oracle():
+    rng = np.random.default_rng(2)
+    phi = rng.uniform(-0.2, 1.2, 100000)
+    mu = rng.standard_normal(100000)
+    g = P.floryhuggins(phi, mu, 1.1, 0.9, 2.7)
+    r = O.floryhuggins(phi, mu, 1.1, 0.9, 2.7)
+    assert np.array_equal(g[0], r[0])
+    assert g[2:] == r[2:]
+    assert rel_l2(g[1], r[1]) < 1e-15
+
+
+@pytest.mark.parametrize("noise", [False, True])
+def test_spectral_update_bit_exact(noise):
+    nx, ny, nz = 16, 8, 32
+    L = [16.0, 12.0, 40.0]
+    rng = np.random.default_rng(3)
+    ph = O.fft_r2c(rng.standard_normal((nz, ny, nx)))
+    mh = O.fft_r2c(rng.standard_normal((nz, ny, nx)))
+    xi = O.noise_field(nx, ny, nz, L, 77, 1.0, 1.0, 1e-3) if noise else None
+    mode = O.NOISE_FIELD if noise else O.NOISE_OFF
+    r = O.spectral_update(ph, mh, nx, L, 1.2, 0.8, 1.1, 0.7, 1e-3, mode, 0, xi)
+    g = P.spectral_update(ph, mh, nx, L, 1.2, 0.8, 1.1, 0.7, 1e-3, xi)
+    assert np.array_equal(g, r)
+
+
+def _lists_equal(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+@pytest.mark.parametrize("zghost", [0, 1])
+def test_setup_ghosts_bins_lists_bit_exact(zghost):
+    w = workload(n=32, nchains=12, nbeads=25, zghost=zghost, seed=3)
+    g = make_gpu(w)
+    o = make_oracle(w)
+    cg, co = g.counts(), o.counts()
+    for k in ("nowned", "ngathered", "nlocal", "nghost", "ncalls"):
+        assert cg[k] == co[k], k
+    for a, b in zip(g.gathered(), o.gathered()):
+        assert np.array_equal(a, b)
+    for a, b in zip(g.bins(), o.bins()):
+        assert np.array_equal(a, b)
+    assert _lists_equal(g.gridlist(), o.gridlist())
+    assert _lists_equal(g.halflist(), o.halflist())
+
+
+@pytest.mark.parametrize("ljish", [False, True])
+def test_one_step_phases(ljish):
+    w = workload(n=32, nchains=10, nbeads=30, ljish=ljish, seed=4, nucwidth=2.0)
+    g = make_gpu(w)
+    o = make_oracle(w)
+    for ph in range(8):
+        g.phase(ph)
+        o.phase(ph)
+        if ph == P.PH_START:
+            for f in (P.GRADX, P.GRADY, P.GRADZ):
+                assert rel_l2(g.get_field(f), o.get_field(f)) < 1e-12
+        if ph == P.PH_PAIR:
+            assert rel_l2(g.get_field(P.MU), o.get_field(P.MU)) < 1e-10
+            assert rel_l2(g.get_forces(), o.get_forces()) < 1e-10
+        if ph == P.PH_POSTFORCE:
+            assert rel_l2(g.get_field(P.MU), o.get_field(P.MU)) < 1e-10
+        if ph in (P.PH_INITIAL, P.PH_FINAL):
+            assert rel_l2(g.get_atoms()[0], o.get_atoms()[0]) < 1e-12
+    assert rel_l2(g.get_field(P.PHI), o.get_field(P.PHI)) < 1e-12
+
+
+def test_pair_pass_bit_exact_with_identical_inputs():
+    # With grad(phi) taken from the oracle, mu and forces must match the
+    # reference loop order exactly (exp() aside: compared at 1e-15).
+    w = workload(n=32, nchains=10, nbeads=30, seed=5, nucwidth=2.5)
+    g = make_gpu(w)
+    o = make_oracle(w)
+    for s in (g, o):
+        s.phase(P.PH_START)
+        s.phase(P.PH_NEIGH)
+    for f in (P.GRADX, P.GRADY, P.GRADZ):
+        g.set_field(f, o.get_field(f))
+    g.phase(P.PH_PAIR)
+    o.phase(P.PH_PAIR)
+    assert rel_l2(g.get_field(P.MU), o.get_field(P.MU)) < 1e-15
+    assert rel_l2(g.get_forces(), o.get_forces()) < 1e-14
+
+
+@pytest.mark.parametrize("zghost", [0, 1])
+def test_trajectory_200_steps(zghost):
+    w = workload(n=64, nchains=20, nbeads=50, seed=42, zghost=zghost, chi=3.0)
+    g = make_gpu(w)
+    o = make_oracle(w)
+    g.run(200)
+    o.run(200)
+    assert rel_l2(g.get_field(P.PHI), o.get_field(P.PHI)) < 1e-10
+    assert rel_l2(g.get_atoms()[0], o.get_atoms()[0]) < 1e-10
+    assert rel_l2(g.get_forces(), o.get_forces()) < 1e-8
+    assert g.counts()["ncalls"] == o.counts()["ncalls"]
+    assert _lists_equal(g.gridlist(), o.gridlist())
+    ge, oe = g.energy(), o.energy()
+    assert np.allclose(ge, oe, rtol=1e-9, atol=1e-12)
+
+
+def test_reference_noise_field_steps():
+    w = workload(n=32, nchains=5, nbeads=20, seed=6, noise=P.NOISE_FIELD)
+    g = make_gpu(w)
+    w2 = dict(w)
+    w2["noise"] = O.NOISE_MT19937
+    o = make_oracle(w2)
+    gen = P.NoiseGen(5678)
+    for _ in range(3):
+        g.set_noise(gen.next(32, 32, 32, [32.0] * 3, 1.0, 1.0, 1e-3))
+        g.run(1)
+        o.run(1)
+    assert rel_l2(g.get_field(P.PHI), o.get_field(P.PHI)) < 1e-12
+
+
+def test_nan_is_reported():
+    w = workload(n=16, nchains=2, nbeads=5, seed=8)
+    w["phi"] = w["phi"].copy()
+    w["phi"][0, 0, 0] = np.nan
+    g = make_gpu(w)
+    with pytest.raises(P.PhafdError) as e:
+        g.run(1)
+    assert e.value.code == -3

@@ -1,0 +1,127 @@
+"""Pins of the oracle's pair styles and chain fix (CPU only)."""
+import numpy as np
+
+from oracle import oracle_ctypes as O
+
+
+def one_atom(pos, phi, style=O.PAIR_GAUSSIAN, cut=3.0, nucwidth=1.5, n=8, L=8.0):
+    s = O.System(n, n, n, [0, 0, 0], [L] * 3)
+    s.set_field(O.PHI, phi)
+    s.set_atoms(np.array([pos]), np.zeros(1, np.int32))
+    p = s.add_pair(style, cut)
+    s.pair_coeff(p, 0, 2.0, 0.1, nucwidth)
+    s.fix_gradphi(False)
+    s.neighbor(0.5)
+    s.setup(1e-3)
+    s.phase(O.PH_START)
+    s.phase(O.PH_NEIGH)
+    s.phase(O.PH_PAIR)
+    return s
+
+
+def brute(pos, phi, grads, style, w, eps=2.0, phit=0.1, n=8, L=8.0):
+    g = np.arange(n) * L / n
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    mu = np.zeros_like(phi)
+    F = np.zeros(3)
+    # the central copy plus its periodic images in x and y (no z images on one rank)
+    for sx in (-L, 0, L):
+        for sy in (-L, 0, L):
+            p = np.array(pos) + [sx, sy, 0]
+            r2 = (X - p[0]) ** 2 + (Y - p[1]) ** 2 + (Z - p[2]) ** 2
+            s = r2 / w ** 2
+            if style == O.PAIR_GAUSSIAN:
+                m = r2 <= w ** 2
+                e = (np.exp(-s / 2) - np.exp(-0.5)) * np.exp(0.5) * eps
+            else:
+                m = r2 < w ** 2
+                with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
+                    e = np.exp(-1.0 / (1 - s) + 1) * eps
+            e = np.where(m, e, 0.0)
+            mu += 2 * (phi - phit) * e
+            for c in range(3):
+                F[c] -= (grads[c] * e * 2 * (phi - phit) * (L / n) ** 3).sum()
+    return mu, F
+
+
+def test_gridatom_deposit_and_force_bruteforce():
+    n, L = 8, 8.0
+    g = np.arange(n) * L / n
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    phi = 0.4 + 0.1 * np.sin(2 * np.pi * X / L) + 0.05 * np.cos(2 * np.pi * Y / L)
+    for style in (O.PAIR_GAUSSIAN, O.PAIR_LJISH):
+        s = one_atom([0.7, 7.6, 3.3], phi, style)
+        grads = [s.get_field(O.GRADX), s.get_field(O.GRADY), s.get_field(O.GRADZ)]
+        mu_ref, F_ref = brute([0.7, 7.6, 3.3], phi, grads, style, 1.5)
+        assert np.allclose(s.get_field(O.MU), mu_ref, rtol=1e-12, atol=1e-14)
+        assert np.allclose(s.get_forces()[0], F_ref, rtol=1e-10, atol=1e-14)
+
+
+def test_gridatom_zero_force_for_constant_phi():
+    s = one_atom([3.3, 4.1, 2.2], np.full((8, 8, 8), 0.37))
+    assert np.abs(s.get_forces()).max() == 0.0
+    assert s.get_field(O.MU).max() > 0
+
+
+def two_lj(p0, p1, L=16.0):
+    s = O.System(16, 16, 16, [0, 0, 0], [L] * 3)
+    s.set_field(O.PHI, np.full((16, 16, 16), 0.5))
+    s.set_atoms(np.array([p0, p1]), np.zeros(2, np.int32))
+    s.add_pair(O.PAIR_LJCUT, 1.5, 1.0, 1.0)
+    s.neighbor(0.5)
+    s.setup(1e-3)
+    s.phase(O.PH_START)
+    s.phase(O.PH_NEIGH)
+    s.phase(O.PH_PAIR)
+    return s.get_forces()
+
+
+def lj_force(d):
+    r = np.linalg.norm(d)
+    return 24 * (2 * r ** -12 - r ** -6) / r * (d / r)
+
+
+def test_lj_two_body_and_newton():
+    F = two_lj([5.0, 5.0, 5.0], [6.1, 5.2, 4.9])
+    d = np.array([5.0, 5.0, 5.0]) - np.array([6.1, 5.2, 4.9])
+    assert np.allclose(F[0], lj_force(d), rtol=1e-13)
+    assert np.allclose(F[0], -F[1], rtol=0, atol=0)
+
+
+def test_lj_across_x_boundary_and_not_z():
+    F = two_lj([0.3, 5.0, 5.0], [15.5, 5.0, 5.0])    # x images interact
+    assert np.allclose(F[0], lj_force(np.array([0.8, 0.0, 0.0])), rtol=1e-12)
+    assert np.allclose(F[0], -F[1], atol=1e-15)
+    F = two_lj([5.0, 5.0, 0.3], [5.0, 5.0, 15.5])    # no z images on one rank
+    assert np.abs(F).max() == 0.0
+
+
+def test_chain_force_is_minus_energy_gradient():
+    rng = np.random.default_rng(2)
+    nb = 7
+    xyz = np.cumsum(rng.normal(size=(nb, 3)) * 0.6, 0) + 8.0
+    k, r0, kb = 100.0, 1.0, 10.0
+
+    def energy(u):
+        s = O.System(8, 8, 8, [0, 0, 0], [16.0] * 3)
+        s.set_field(O.PHI, np.full((8, 8, 8), 0.5))
+        s.set_atoms(u, np.zeros(nb, np.int32))
+        s.fix_chain([0], [nb], r0, k, kb, 1.0)
+        s.setup(1.0)
+        return s.energy()[3]
+
+    s = O.System(8, 8, 8, [0, 0, 0], [16.0] * 3)
+    s.set_field(O.PHI, np.full((8, 8, 8), 0.5))
+    s.set_atoms(xyz, np.zeros(nb, np.int32))
+    s.fix_chain([0], [nb], r0, k, kb, 1.0)
+    s.setup(1.0)     # F = 0, so one half step moves u by drag * Fchain * dt / 2
+    s.phase(O.PH_INITIAL)
+    _, u1, _ = s.get_atoms()
+    fchain = (u1 - xyz) / 0.5
+    h = 1e-6
+    for b in range(nb):
+        for c in range(3):
+            up = xyz.copy(); up[b, c] += h
+            dn = xyz.copy(); dn[b, c] -= h
+            g = (energy(up) - energy(dn)) / (2 * h)
+            assert abs(fchain[b, c] + g) < 1e-5 * max(1.0, abs(g))
