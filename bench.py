@@ -9,9 +9,9 @@ Flory-Huggins + r2c(mu), spectral update, chain half step, c2r(phi), NaN check.
 
 Default workload (the plain-FP64-on-GPU configuration): 256^3 periodic grid,
 box 256, phi0 = 0.4 + U(-0.02, 0.02), chi = 2.5, noise off, 5000 chains x 100
-beads (random walks, bond 1), lj/cut sigma 1 cut 2^(1/6), gridatom/gaussian
+beads (random walks, bond 1, no two beads closer than 1), lj/cut sigma 1 cut 2^(1/6), gridatom/gaussian
 cut 3 width 1 epsilon 1, neighbour skin 0.5, semiflexible harmonic chains
-(k = 100, kb = 10), dt = 1e-3.  Inputs are generated on the host, uploaded,
+(k = 100, kb = 10), periodic ghost images along z, dt = 1e-3.  Inputs are generated on the host, uploaded,
 and resident in HBM before the timed region starts.
 
 Prints ONE JSON line (rank 0).  `value` = grid points x steps / s.
@@ -30,11 +30,15 @@ sys.path.insert(0, ROOT)
 METRIC = "grid-pt·steps/s + beads·steps/s, surrogate/FP32 vs FP64 at ≤1e-3 φ err"
 
 WORKLOADS = {
-    # name: (n, box, nchains, nbeads, phi0, var, chi, gauss_cut, nucwidth, style)
+    # zghost=1: periodic ghost images along z.  One reference rank makes none
+    # (its copy swap marks every in-box atom LOCAL), which drops lj/cut across
+    # the z boundary; with stiff bonds at dt = 1e-3 a bond straddling z = 0
+    # then compresses below the WCA stability limit and the run blows up at
+    # its first re-wrap.  Periodic images are what two or more ranks compute.
     "c2": dict(n=256, box=256.0, nchains=5000, nbeads=100, phi0=0.4, var=0.04, chi=2.5,
-               cut=3.0, nucwidth=1.0, ljish=False),
+               cut=3.0, nucwidth=1.0, ljish=False, zghost=1),
     "c1": dict(n=64, box=64.0, nchains=20, nbeads=50, phi0=0.5, var=0.02, chi=3.0,
-               cut=3.0, nucwidth=1.0, ljish=False),
+               cut=3.0, nucwidth=1.0, ljish=False, zghost=1),
 }
 DESCR = {
     "c2": "plain FP64 on GPU: 256^3 grid, 5000x100-bead semiflexible chains (500k beads), "
@@ -47,7 +51,7 @@ def setup_inputs(wl):
     import phafd_amd as P
     n = wl["n"]
     phi = P.ic_noisy_constant((n, n, n), wl["phi0"], wl["var"], 1234)
-    xyz, mol = P.ic_random_chains(wl["nchains"], wl["nbeads"], 1.0, 0.9, [0, 0, 0], [wl["box"]] * 3, 42)
+    xyz, mol = P.ic_random_chains(wl["nchains"], wl["nbeads"], 1.0, 1.0, [0, 0, 0], [wl["box"]] * 3, 42)
     return phi, xyz, mol
 
 
@@ -74,7 +78,7 @@ def cpu_baseline(wl, phi, xyz, mol, steps):
     """The CPU oracle (single thread) on a bounded sample of the same workload."""
     from oracle import oracle_ctypes as O
     n = wl["n"]
-    s = O.System(n, n, n, [0, 0, 0], [wl["box"]] * 3)
+    s = O.System(n, n, n, [0, 0, 0], [wl["box"]] * 3, zghost=wl["zghost"])
     configure(s, wl, phi, xyz, mol, is_oracle=True)
     t0 = time.perf_counter()
     s.run(steps)
@@ -115,7 +119,7 @@ def main():
     wl = WORKLOADS[args.workload]
     n = wl["n"]
     phi, xyz, mol = setup_inputs(wl)
-    s = P.System(n, n, n, [0, 0, 0], [wl["box"]] * 3)
+    s = P.System(n, n, n, [0, 0, 0], [wl["box"]] * 3, zghost=wl["zghost"])
     configure(s, wl, phi, xyz, mol)
     s.run(args.warmup)
     s.timers(reset=True)

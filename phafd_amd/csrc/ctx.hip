@@ -678,13 +678,13 @@ void run_steps(phafd_hip_ctx *c, int nsteps) {
     float tot = 0;
     HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[8]));
     c->tacc[7] += tot;
+    double phi0;
+    std::memcpy(&phi0, c->h_flags + 2, sizeof(double));
+    if (std::isnan(phi0)) throw std::runtime_error("NAN encountered in phi.");
     if (c->neigh_set) {
       if (c->h_flags[0] & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
       c->pending_decision = c->h_flags[0] & 1;
     }
-    double phi0;
-    std::memcpy(&phi0, c->h_flags + 2, sizeof(double));
-    if (std::isnan(phi0)) throw std::runtime_error("NAN encountered in phi.");
   }
 }
 

@@ -4,6 +4,7 @@
 //     std::mt19937 draw order (src/conjugate.cpp:163-327,
 //     src/conjugate_volfrac.cpp:72-133), for PHAFD_NOISE_FIELD
 //   * a synthetic random-walk chain generator for benchmark workloads
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -177,8 +178,9 @@ int phafd_noise_next(phafd_noise_gen *g, int nx, int ny, int nz, const double *p
 // ------------------------------------------------------ chain generator
 // Synthetic polymer workload: nchains random walks of nbeads with fixed bond
 // length, each bead placed at a uniformly random direction from the previous
-// one and rejected (up to 100 tries, then accepted) when closer than minsep
-// to any already placed bead.  Positions are unwrapped walks started at a
+// one and rejected when closer than minsep to any already placed bead; a
+// chain that cannot place a bead within 100 tries is regrown from a new
+// random start, so no two beads end up closer than minsep.  Positions are unwrapped walks started at a
 // uniform point of the box; wrap them with the periodic box as the reference
 // does in Domain::pbc.  Deterministic for a given seed (std::mt19937_64).
 int phafd_ic_random_chains(int nchains, int nbeads, double bond, double minsep,
@@ -234,9 +236,12 @@ int phafd_ic_random_chains(int nchains, int nbeads, double bond, double minsep,
   };
   int idx = 0;
   for (int c = 0; c < nchains; c++) {
+    const int first = idx;
+    int regrow = 0;
     for (int b = 0; b < nbeads; b++, idx++) {
       double p[3];
-      for (int tries = 0;; tries++) {
+      bool placed = false;
+      for (int tries = 0; tries < 100; tries++) {
         if (b == 0) {
           for (int d = 0; d < 3; d++) p[d] = boxlo[d] + u01(gen) * period[d];
         } else {
@@ -246,7 +251,18 @@ int phafd_ic_random_chains(int nchains, int nbeads, double bond, double minsep,
           p[1] = q[1] + bond * s * sin(ph);
           p[2] = q[2] + bond * z;
         }
-        if (tries >= 100 || !too_close(p)) break;
+        if (!too_close(p)) { placed = true; break; }
+      }
+      if (!placed) {
+        // remove this chain's beads from the cell map and start it again
+        for (int j = first; j < idx; j++) {
+          auto &v = grid[key(&xyz[3 * (size_t)j])];
+          v.erase(std::remove(v.begin(), v.end(), j), v.end());
+        }
+        if (++regrow > 1000) return -2;
+        idx = first - 1;
+        b = -1;
+        continue;
       }
       for (int d = 0; d < 3; d++) xyz[3 * (size_t)idx + d] = p[d];
       if (mol) mol[idx] = c;
