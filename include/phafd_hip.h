@@ -166,6 +166,16 @@ int phafd_hip_energy(phafd_hip_ctx *ctx, double *out4);
  * {gradphi, neighbor, pair, integrate, floryhuggins+r2c mu, update, c2r phi, total};
  * reset when reset != 0 */
 int phafd_hip_timers(phafd_hip_ctx *ctx, double *ms8, int reset);
+/* run-time options (defaults in brackets):
+ *  PHAFD_OPT_EAGER_GRIDLIST [0]: store the NPairGridBin list at every build
+ *    and walk it for the mu deposit.  By default the step walks the same bins
+ *    in the same order without storing the list (bit-identical mu) and the
+ *    list is materialised from the last build only when phafd_hip_gridlist()
+ *    asks for it.
+ *  PHAFD_OPT_FUSE_FH [1]: apply Flory-Huggins inside the load of the r2c of
+ *    mu when the two hooks are adjacent (same arithmetic per point). */
+enum { PHAFD_OPT_EAGER_GRIDLIST = 0, PHAFD_OPT_FUSE_FH = 1 };
+int phafd_hip_set_option(phafd_hip_ctx *ctx, int key, double value);
 /* HIP-event timing of every launch of one kernel during run():
  * kernel_id >= 0 (re)starts the probe for that kernel (ids in the order
  * r2c x, y fwd, y inv, z fwd, z inv, c2r x, grad multiply, update, FH,

@@ -19,6 +19,7 @@ ZGHOST_REFERENCE, ZGHOST_PERIODIC = 0, 1
 NOISE_OFF, NOISE_FIELD = 0, 2
 PAIR_GAUSSIAN, PAIR_LJISH = 0, 1
 FP64, FP32 = 0, 1
+OPT_EAGER_GRIDLIST, OPT_FUSE_FH = 0, 1
 PH_START, PH_NEIGH, PH_PAIR, PH_INITIAL, PH_POSTFORCE, PH_PREFINAL, PH_FINAL, PH_POSTFINAL = range(8)
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "NaN in phi", -4: "runtime error", -5: "list overflow"}
 
@@ -81,6 +82,7 @@ def lib():
         "phafd_hip_energy": ([V, _dp], ctypes.c_int),
         "phafd_hip_timers": ([V, _dp, ctypes.c_int], ctypes.c_int),
         "phafd_hip_probe": ([V, ctypes.c_int, _dp], ctypes.c_int),
+        "phafd_hip_set_option": ([V, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "phafd_hip_fft_r2c": ([ctypes.c_int] * 3 + [_dp, _dp], ctypes.c_int),
         "phafd_hip_fft_c2r": ([ctypes.c_int] * 3 + [_dp, _dp], ctypes.c_int),
         "phafd_hip_gradphi": ([ctypes.c_int] * 3 + [_dp] * 5, ctypes.c_int),
@@ -383,6 +385,9 @@ class System:
         o = np.zeros(4)
         self._c(lib().phafd_hip_energy(self.h, _d(o)))
         return o
+
+    def set_option(self, key, value):
+        self._c(lib().phafd_hip_set_option(self.h, key, float(value)))
 
     KERNELS = ["r2c_x", "y_fwd", "y_inv", "z_fwd", "z_inv", "c2r_x", "grad_multiply", "update", "fh",
                "ga_deposit", "ga_force", "grid_count", "grid_fill", "half_count", "half_fill", "lj"]

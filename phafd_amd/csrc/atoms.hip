@@ -427,46 +427,105 @@ __global__ void k_ga_deposit(GridView gv, GAParams ga, const int *cnt, const int
   if (energy) atomicAdd(energy, esum);
 }
 
-// The force side of the same loop, gathered per atom: the reference adds to
-// F(aj) once per listed grid point, in ascending grid index.  Every grid point
-// with rsq <= cutsq(type) is in the atom's list (the list cut is
-// cutsq + skin and no atom has moved more than skin/2 since the build), so
-// walking the in-box grid points around the atom in (z, y, x) order gives the
-// same terms in the same order.
-__global__ void k_ga_force(int ng, GridView gv, GAParams ga, const double4 *gx, double4 *gF) {
+__device__ __forceinline__ int bin1(double c, double lo, double hi, double inv, int nbin) {
+  if (c >= hi) return (int)((c - hi) * inv) + nbin;
+  if (c >= lo) { int i = (int)((c - lo) * inv); return i < nbin - 1 ? i : nbin - 1; }
+  return (int)((c - lo) * inv) - 1;
+}
+
+// The force side of the grid-atom loop, gathered per atom, plus the mu side
+// as emitted terms.  The reference adds to F(aj) once per listed grid point,
+// in ascending grid index.  Every grid point with rsq <= cutsq(type) is in
+// the atom's list (the list cut is cutsq + skin and no atom has moved more
+// than skin/2 since the build), so walking the in-box grid points around the
+// atom in (z, y, x) order gives the same force terms in the same order.
+// Each in-range pair also emits its mu term 2*delphi*expfac keyed by the
+// atom's position in the bin-sorted atom array.  A grid point's list walks
+// the stencil bins in (dz, dy, dx) lexicographic order
+// (src/nstencil_full_bin_3d.cpp:30-36) and each bin in its list order
+// (src/nbin_standard.cpp:181-229); bins are numbered z-major, so that order is
+// exactly ascending position in the bin-sorted array.  Summing each grid
+// point's terms in key order reproduces the reference's mu bit for bit.
+struct GABox {
+  int i0, i1, j0, j1, k0, k1;
+};
+__device__ __forceinline__ GABox ga_box(const GridView &gv, const double4 &x, double cs) {
+  const double w = sqrt(cs);
+  const double mg = 1e-9;
+  GABox b;
+  b.i0 = (int)ceil((x.z - w - gv.lo[2]) / gv.dz - mg); b.i1 = (int)floor((x.z + w - gv.lo[2]) / gv.dz + mg);
+  b.j0 = (int)ceil((x.y - w - gv.lo[1]) / gv.dy - mg); b.j1 = (int)floor((x.y + w - gv.lo[1]) / gv.dy + mg);
+  b.k0 = (int)ceil((x.x - w - gv.lo[0]) / gv.dx - mg); b.k1 = (int)floor((x.x + w - gv.lo[0]) / gv.dx + mg);
+  b.i0 = b.i0 < 0 ? 0 : b.i0; b.j0 = b.j0 < 0 ? 0 : b.j0; b.k0 = b.k0 < 0 ? 0 : b.k0;
+  b.i1 = b.i1 > gv.nz - 1 ? gv.nz - 1 : b.i1;
+  b.j1 = b.j1 > gv.ny - 1 ? gv.ny - 1 : b.j1;
+  b.k1 = b.k1 > gv.nx - 1 ? gv.nx - 1 : b.k1;
+  return b;
+}
+__device__ __forceinline__ bool ga_in(const GAParams &ga, double rsq, double cs) {
+  return ga.style == 0 ? (rsq <= cs) : (rsq < cs);
+}
+
+__global__ void k_ga_count(int ng, GridView gv, GAParams ga, const double4 *gx, int *acnt) {
+  GRID_STRIDE(a, ng) {
+    const double4 x = gx[a];
+    int n = 0;
+    if (isfinite(x.x) && isfinite(x.y) && isfinite(x.z)) {
+      const double cs = ga.cutsq[(int)x.w];
+      const GABox b = ga_box(gv, x, cs);
+      for (int i = b.i0; i <= b.i1; i++) {
+        const double delz = (i * gv.dz + gv.lo[2]) - x.z;
+        for (int j = b.j0; j <= b.j1; j++) {
+          const double dely = (j * gv.dy + gv.lo[1]) - x.y;
+          for (int k = b.k0; k <= b.k1; k++) {
+            const double delx = (k * gv.dx + gv.lo[0]) - x.x;
+            const double rsq = delx * delx + dely * dely + delz * delz;
+            n += ga_in(ga, rsq, cs) ? 1 : 0;
+          }
+        }
+      }
+    }
+    acnt[a] = n;
+  }
+}
+
+__global__ void k_ga_force_emit(int ng, GridView gv, GAParams ga, const double4 *gx, const int *binpos,
+                                const int *aoff, int *pg, int *pkey, double *pv, double4 *gF, bool emit,
+                                double *energy) {
+  double esum = 0.0;
   GRID_STRIDE(a, ng) {
     const double4 x = gx[a];
     if (!isfinite(x.x) || !isfinite(x.y) || !isfinite(x.z)) continue;   // never walk an unbounded range
     const int t = (int)x.w;
     const double cs = ga.cutsq[t];
-    const double w = sqrt(cs);
-    const double mg = 1e-9;
-    int i0 = (int)ceil((x.z - w - gv.lo[2]) / gv.dz - mg), i1 = (int)floor((x.z + w - gv.lo[2]) / gv.dz + mg);
-    int j0 = (int)ceil((x.y - w - gv.lo[1]) / gv.dy - mg), j1 = (int)floor((x.y + w - gv.lo[1]) / gv.dy + mg);
-    int k0 = (int)ceil((x.x - w - gv.lo[0]) / gv.dx - mg), k1 = (int)floor((x.x + w - gv.lo[0]) / gv.dx + mg);
-    i0 = i0 < 0 ? 0 : i0; j0 = j0 < 0 ? 0 : j0; k0 = k0 < 0 ? 0 : k0;
-    i1 = i1 > gv.nz - 1 ? gv.nz - 1 : i1;
-    j1 = j1 > gv.ny - 1 ? gv.ny - 1 : j1;
-    k1 = k1 > gv.nx - 1 ? gv.nx - 1 : k1;
+    const GABox b = ga_box(gv, x, cs);
+    const int key = emit ? binpos[a] : 0;
+    int w = emit ? aoff[a] : 0;
     double4 F = gF[a];
-    for (int i = i0; i <= i1; i++) {
+    for (int i = b.i0; i <= b.i1; i++) {
       const double ztmp = i * gv.dz + gv.lo[2];
       const double delz = ztmp - x.z;
-      for (int j = j0; j <= j1; j++) {
+      for (int j = b.j0; j <= b.j1; j++) {
         const double ytmp = j * gv.dy + gv.lo[1];
         const double dely = ytmp - x.y;
-        for (int k = k0; k <= k1; k++) {
+        for (int k = b.k0; k <= b.k1; k++) {
           const double xtmp = k * gv.dx + gv.lo[0];
           const double delx = xtmp - x.x;
           double rsq = delx * delx + dely * dely + delz * delz;
-          const bool in = ga.style == 0 ? (rsq <= cs) : (rsq < cs);
-          if (!in) continue;
+          if (!ga_in(ga, rsq, cs)) continue;
           rsq /= cs;
           double expfac;
           if (ga.style == 0) expfac = (exp(-rsq / 2.0) - ga.em05) * ga.e05 * ga.eps[t];
           else expfac = exp(-1. / (1 - rsq) + 1) * ga.eps[t];
           const size_t p = ((size_t)i * gv.ny + j) * gv.nx + k;
           const double delphi = (gv.phi[p] - ga.phit[t]);
+          if (emit) {   // mu term of PairGridAtom*::compute, summed later in list order
+            pg[w] = (int)p;
+            pkey[w] = key;
+            pv[w] = 2 * delphi * expfac;
+            w++;
+          }
+          if (energy) esum += delphi * delphi * expfac * gv.dv;
           expfac *= 2 * delphi * gv.dv;
           F.x -= gv.gx[p] * expfac;
           F.y -= gv.gy[p] * expfac;
@@ -475,6 +534,51 @@ __global__ void k_ga_force(int ng, GridView gv, GAParams ga, const double4 *gx, 
       }
     }
     gF[a] = F;
+  }
+  if (energy && esum != 0.0) atomicAdd(energy, esum);
+}
+
+// slot of every emitted term inside its grid point's group (arbitrary order,
+// restored by the key in k_ga_sum)
+__global__ void k_ga_rank(int npairs, const int *pg, int *gcnt, int *ppos) {
+  GRID_STRIDE(q, npairs) ppos[q] = atomicAdd(&gcnt[pg[q]], 1);
+}
+
+__global__ void k_binpos(int ng, const int *binatoms, int *binpos) {
+  GRID_STRIDE(q, ng) binpos[binatoms[q]] = (int)q;
+}
+
+__global__ void k_ga_place(int npairs, const int *pg, const int *ppos, const int *goff, const int *pkey,
+                           const double *pv, int *skey, double *sv) {
+  GRID_STRIDE(q, npairs) {
+    const int slot = goff[pg[q]] + ppos[q];
+    skey[slot] = pkey[q];
+    sv[slot] = pv[q];
+  }
+}
+
+// mu(g) += its terms in ascending key (list) order; a grid point holds few
+// terms, so a selection over the group is cheap
+__global__ void k_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu) {
+  GRID_STRIDE(p, N) {
+    const int s = goff[p], e = goff[p + 1];
+    if (s == e) continue;
+    double m = mu[p];
+    if (e - s == 1) {
+      m += sv[s];
+    } else {
+      int last = -1;
+      for (int it = s; it < e; it++) {
+        int best = 0x7fffffff, bi = s;
+        for (int q = s; q < e; q++) {
+          const int kq = skey[q];
+          if (kq > last && kq < best) { best = kq; bi = q; }
+        }
+        m += sv[bi];
+        last = best;
+      }
+    }
+    mu[p] = m;
   }
 }
 
@@ -764,10 +868,38 @@ void launch_ga_deposit(const GridView &gv, const GAParams &ga, const int *cnt, c
   hipLaunchKernelGGL(k_ga_deposit, dim3(nblk(N)), dim3(TPB), 0, s, gv, ga, cnt, off, neigh, gx, nowned, energy);
   HIPCHK(hipGetLastError());
 }
-void launch_ga_force(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, double4 *gF,
+void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, int *acnt,
                      hipStream_t s) {
   if (ng <= 0) return;
-  hipLaunchKernelGGL(k_ga_force, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, gF);
+  hipLaunchKernelGGL(k_ga_count, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, acnt);
+  HIPCHK(hipGetLastError());
+}
+void launch_ga_force_emit(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *binpos,
+                          const int *aoff, int *pg, int *pkey, double *pv, double4 *gF, bool emit, double *energy,
+                          hipStream_t s) {
+  if (ng <= 0) return;
+  hipLaunchKernelGGL(k_ga_force_emit, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, binpos, aoff, pg, pkey, pv,
+                     gF, emit, energy);
+  HIPCHK(hipGetLastError());
+}
+void launch_ga_rank(int npairs, const int *pg, int *gcnt, int *ppos, hipStream_t s) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(k_ga_rank, dim3(nblk(npairs)), dim3(TPB), 0, s, npairs, pg, gcnt, ppos);
+  HIPCHK(hipGetLastError());
+}
+void launch_binpos(int ng, const int *binatoms, int *binpos, hipStream_t s) {
+  if (ng <= 0) return;
+  hipLaunchKernelGGL(k_binpos, dim3(nblk(ng)), dim3(TPB), 0, s, ng, binatoms, binpos);
+  HIPCHK(hipGetLastError());
+}
+void launch_ga_place(int npairs, const int *pg, const int *ppos, const int *goff, const int *pkey,
+                     const double *pv, int *skey, double *sv, hipStream_t s) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(k_ga_place, dim3(nblk(npairs)), dim3(TPB), 0, s, npairs, pg, ppos, goff, pkey, pv, skey, sv);
+  HIPCHK(hipGetLastError());
+}
+void launch_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu, hipStream_t s) {
+  hipLaunchKernelGGL(k_ga_sum, dim3(nblk(N)), dim3(TPB), 0, s, N, goff, skey, sv, mu);
   HIPCHK(hipGetLastError());
 }
 void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,

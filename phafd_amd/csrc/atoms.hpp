@@ -71,8 +71,16 @@ void launch_t_place(int nrows, const int *ilist, const int *off, const int *neig
 void launch_t_sort(int ng, const int *toff, int *tpair, int *ti, hipStream_t s);
 void launch_ga_deposit(const GridView &gv, const GAParams &ga, const int *cnt, const int64_t *off,
                        const int *neigh, const double4 *gx, int nowned, double *energy, hipStream_t s);
-void launch_ga_force(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, double4 *gF,
+void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, int *acnt,
                      hipStream_t s);
+void launch_ga_force_emit(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *binpos,
+                          const int *aoff, int *pg, int *pkey, double *pv, double4 *gF, bool emit, double *energy,
+                          hipStream_t s);
+void launch_ga_rank(int npairs, const int *pg, int *gcnt, int *ppos, hipStream_t s);
+void launch_binpos(int ng, const int *binatoms, int *binpos, hipStream_t s);
+void launch_ga_place(int npairs, const int *pg, const int *ppos, const int *goff, const int *pkey,
+                     const double *pv, int *skey, double *sv, hipStream_t s);
+void launch_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu, hipStream_t s);
 void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,
                      const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
                      double4 *gF, double *energy, hipStream_t s);

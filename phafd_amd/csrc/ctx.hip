@@ -84,7 +84,13 @@ struct phafd_hip_ctx {
   int64_t ncalls = 0;
   BinGeom bg{};
   std::vector<int> stencil_full, stencil_half;
-  DBuf<int> d_stencil_full, d_stencil_half;
+  DBuf<int> d_stencil_full, d_stencil_half, d_sfx, d_sfy, d_sfz;
+  DBuf<double4> gx_build;             // gathered positions at the last build
+  bool grid_built = false;            // grid list materialised for the last build
+  bool eager_gridlist = false;        // build (and use) the stored grid list in the step
+  DBuf<int> ga_acnt, ga_aoff, ga_pg, ga_ppos, ga_gcnt, ga_goff;   // grid-atom emitted terms
+  DBuf<int> ga_pkey, ga_skey, binpos;
+  DBuf<double> ga_pv, ga_sv;
   DBuf<int> a2b, bincount, binstart, bincursor, binatoms;
   DBuf<int> g_cnt, g_neigh;
   DBuf<int64_t> g_off;
@@ -333,11 +339,20 @@ void create_stencils(phafd_hip_ctx *c) {
   int sy = (int)(c->cutneigh * b.bininvy); if (sy * b.binsizey < c->cutneigh) sy++;
   int sz = (int)(c->cutneigh * b.bininvz); if (sz * b.binsizez < c->cutneigh) sz++;
   c->stencil_full.clear();
+  std::vector<int> fx, fy, fz;
   for (int k = -sz; k <= sz; k++)
     for (int j = -sy; j <= sy; j++)
       for (int i = -sx; i <= sx; i++)
-        if (bin_distance(i, j, k) < cutneighsq)
+        if (bin_distance(i, j, k) < cutneighsq) {
           c->stencil_full.push_back(k * b.mbiny * b.mbinx + j * b.mbinx + i);
+          fx.push_back(i); fy.push_back(j); fz.push_back(k);
+        }
+  c->d_sfx.alloc(fx.size() + 1); c->d_sfy.alloc(fx.size() + 1); c->d_sfz.alloc(fx.size() + 1);
+  if (!fx.empty()) {
+    HIPCHK(hipMemcpy(c->d_sfx.p, fx.data(), sizeof(int) * fx.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_sfy.p, fy.data(), sizeof(int) * fy.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_sfz.p, fz.data(), sizeof(int) * fz.size(), hipMemcpyHostToDevice));
+  }
   c->stencil_half.clear();
   for (int k = 0; k <= sz; k++)
     for (int j = -sy; j <= sy; j++)
@@ -370,17 +385,20 @@ void bin_atoms(phafd_hip_ctx *c) {
   scan_exclusive_i32(c->bincount.p, c->binstart.p, (size_t)nb, c->stream);
   launch_bin_place(ng, c->a2b.p, c->binstart.p, c->bincursor.p, c->binatoms.p, c->stream);
   launch_bin_sort(nb, c->binstart.p, c->binatoms.p, c->glabel.p, c->stream);
+  c->binpos.alloc((size_t)ng + 1);
+  launch_binpos(ng, c->binatoms.p, c->binpos.p, c->stream);
 }
 
 // NPairGridBin::build (src/npair_grid_bin.cpp:22-127)
 void build_gridlist(phafd_hip_ctx *c) {
   GridView gv = c->gridview();
+  const double4 *gxb = c->gx_build.p;
   const double cutneighsq = c->cutneigh * c->cutneigh;
   c->g_cnt.alloc(c->N + 1);
   c->g_off.alloc(c->N + 1);
   probe_begin(K_GRID_COUNT, c->stream);
   launch_grid_list(false, gv, c->bg, c->d_stencil_full.p, (int)c->stencil_full.size(), c->binstart.p,
-                   c->binatoms.p, c->gx.p, cutneighsq, c->nowned, c->g_cnt.p, nullptr, nullptr,
+                   c->binatoms.p, gxb, cutneighsq, c->nowned, c->g_cnt.p, nullptr, nullptr,
                    c->flags.p + 1, c->stream);
   probe_end(K_GRID_COUNT, c->stream, 4.0 * c->N);
   scan_exclusive_i32_to_i64(c->g_cnt.p, c->g_off.p, c->N, c->stream);
@@ -391,8 +409,9 @@ void build_gridlist(phafd_hip_ctx *c) {
   c->g_neigh.alloc((size_t)tot + 1);
   probe_begin(K_GRID_FILL, c->stream);
   launch_grid_list(true, gv, c->bg, c->d_stencil_full.p, (int)c->stencil_full.size(), c->binstart.p,
-                   c->binatoms.p, c->gx.p, cutneighsq, c->nowned, c->g_cnt.p, c->g_off.p, c->g_neigh.p,
+                   c->binatoms.p, gxb, cutneighsq, c->nowned, c->g_cnt.p, c->g_off.p, c->g_neigh.p,
                    c->flags.p + 1, c->stream);
+  c->grid_built = true;
   probe_end(K_GRID_FILL, c->stream, 8.0 * c->N + 4.0 * tot);
 }
 
@@ -440,7 +459,10 @@ void neighbor_build(phafd_hip_ctx *c) {
   c->ncalls++;
   launch_copy4(c->nowned, c->x.p, c->xhold.p, c->stream);
   bin_atoms(c);
-  if (c->need_full) build_gridlist(c);
+  c->gx_build.alloc((size_t)c->ngathered + 1);
+  launch_copy4(c->ngathered, c->gx.p, c->gx_build.p, c->stream);
+  c->grid_built = false;
+  if (c->need_full && c->eager_gridlist) build_gridlist(c);
   if (c->need_half) build_halflist(c);
   c->lists_valid = true;
 }
@@ -526,13 +548,55 @@ void phase_neigh(phafd_hip_ctx *c) {
   }
 }
 
+// One grid-atom pair style (PairGridAtomGaussian / PairGridAtomLJish
+// compute): atom-centric force terms in reference order, and the mu terms
+// emitted with their list-order keys, grouped per grid point by a counting
+// sort and summed in key order.  With the eager option the mu side walks the
+// stored list instead (the two are bit-identical).
+void grid_atom_pass(phafd_hip_ctx *c, PairDef &p, double *energy) {
+  GridView gv = c->gridview();
+  GAParams ga = ga_params(p);
+  const int ng = c->ngathered;
+  const bool emit = !(c->eager_gridlist && c->grid_built);
+  int npairs = 0;
+  if (emit) {
+    c->ga_acnt.alloc((size_t)ng + 1);
+    c->ga_aoff.alloc((size_t)ng + 2);
+    launch_ga_count(ng, gv, ga, c->gx.p, c->ga_acnt.p, c->stream);
+    scan_exclusive_i32(c->ga_acnt.p, c->ga_aoff.p, (size_t)ng, c->stream);
+    npairs = read_int(c, c->ga_aoff.p + ng);
+    c->ga_pg.alloc((size_t)npairs + 1); c->ga_ppos.alloc((size_t)npairs + 1);
+    c->ga_pkey.alloc((size_t)npairs + 1); c->ga_pv.alloc((size_t)npairs + 1);
+    c->ga_skey.alloc((size_t)npairs + 1); c->ga_sv.alloc((size_t)npairs + 1);
+    c->ga_gcnt.alloc(c->N + 1);
+    c->ga_goff.alloc(c->N + 2);
+    fill_zero(c->ga_gcnt.p, sizeof(int) * c->N, c->stream);
+  } else {
+    probe_begin(K_GA_DEPOSIT, c->stream);
+    launch_ga_deposit(gv, ga, c->g_cnt.p, c->g_off.p, c->g_neigh.p, c->gx.p, c->nowned, nullptr, c->stream);
+    probe_end(K_GA_DEPOSIT, c->stream, 28.0 * c->N + 4.0 * c->g_nent);
+  }
+  probe_begin(K_GA_FORCE, c->stream);
+  launch_ga_force_emit(ng, gv, ga, c->gx.p, c->binpos.p, c->ga_aoff.p, c->ga_pg.p, c->ga_pkey.p, c->ga_pv.p,
+                       c->gF.p, emit, energy, c->stream);
+  probe_end(K_GA_FORCE, c->stream, 64.0 * ng + 16.0 * npairs);
+  if (emit) {
+    probe_begin(K_GA_DEPOSIT, c->stream);
+    launch_ga_rank(npairs, c->ga_pg.p, c->ga_gcnt.p, c->ga_ppos.p, c->stream);
+    scan_exclusive_i32(c->ga_gcnt.p, c->ga_goff.p, c->N, c->stream);
+    launch_ga_place(npairs, c->ga_pg.p, c->ga_ppos.p, c->ga_goff.p, c->ga_pkey.p, c->ga_pv.p, c->ga_skey.p,
+                    c->ga_sv.p, c->stream);
+    launch_ga_sum(c->N, c->ga_goff.p, c->ga_skey.p, c->ga_sv.p, c->mu.p, c->stream);
+    probe_end(K_GA_DEPOSIT, c->stream, 16.0 * c->N + 40.0 * npairs);
+  }
+}
+
 // Integrate::run pair block, src/integrate.cpp:130-146 (one pass, finding F4)
 void phase_pair(phafd_hip_ctx *c, double *energy_ga = nullptr, double *energy_lj = nullptr) {
   fill_zero(c->mu.p, sizeof(double) * c->N, c->stream);
   launch_fill_d4(c->ngathered, c->gF.p, c->stream);
   launch_fill_d4(c->nowned, c->F.p, c->stream);
   if (!c->neigh_set || !c->lists_valid) return;
-  GridView gv = c->gridview();
   for (auto &pp : c->pairs) {
     PairDef &p = *pp;
     if (p.style == 2) {
@@ -542,13 +606,7 @@ void phase_pair(phafd_hip_ctx *c, double *energy_ga = nullptr, double *energy_lj
                       c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, energy_lj, c->stream);
       probe_end(K_LJ, c->stream, 64.0 * c->ngathered + 4.0 * c->h_nent);
     } else {
-      GAParams ga = ga_params(p);
-      probe_begin(K_GA_DEPOSIT, c->stream);
-      launch_ga_deposit(gv, ga, c->g_cnt.p, c->g_off.p, c->g_neigh.p, c->gx.p, c->nowned, energy_ga, c->stream);
-      probe_end(K_GA_DEPOSIT, c->stream, 28.0 * c->N + 4.0 * c->g_nent);
-      probe_begin(K_GA_FORCE, c->stream);
-      launch_ga_force(c->ngathered, gv, ga, c->gx.p, c->gF.p, c->stream);
-      probe_end(K_GA_FORCE, c->stream, 64.0 * c->ngathered);
+      grid_atom_pass(c, p, energy_ga);
     }
   }
   reverse_comm(c);
@@ -878,6 +936,9 @@ int phafd_hip_set_atoms(phafd_hip_ctx *c, int n, const double *xyz, const int *t
     c->gcap = 0;
     c->ngathered = 0;
     c->atoms_set = true;
+    c->setup_done = false;
+    c->lists_valid = false;
+    c->pending_decision = -1;
     return 0;
   });
 }
@@ -1089,6 +1150,7 @@ int phafd_hip_gridlist(phafd_hip_ctx *c, int64_t cap, int *ilist, int *numneigh,
                        int64_t *inum, int64_t *nentries) {
   return guard(c, [&] {
     if (!c->lists_valid || !c->need_full) { *inum = 0; *nentries = 0; return 0; }
+    if (!c->grid_built) build_gridlist(c);   // from the bins and positions of the last build
     auto cnt = download(c, c->g_cnt.p, c->N);
     int64_t ni = 0;
     for (int v : cnt) ni += v > 0;
@@ -1220,6 +1282,17 @@ int phafd_hip_probe(phafd_hip_ctx *c, int kernel_id, double *out3) {
       g_probe = nullptr;
     }
     if (out3) { out3[0] = probe.ms; out3[1] = (double)probe.launches; out3[2] = probe.total_bytes; }
+    return 0;
+  });
+}
+
+int phafd_hip_set_option(phafd_hip_ctx *c, int key, double value) {
+  return guard(c, [&] {
+    switch (key) {
+      case PHAFD_OPT_EAGER_GRIDLIST: c->eager_gridlist = value != 0; break;
+      case PHAFD_OPT_FUSE_FH: c->fuse_fh = value != 0; break;
+      default: throw std::invalid_argument("unknown option");
+    }
     return 0;
   });
 }

@@ -176,3 +176,20 @@ def test_nan_is_reported():
     with pytest.raises(P.PhafdError) as e:
         g.run(1)
     assert e.value.code == -3
+
+
+def test_list_free_deposit_matches_stored_list_bitwise():
+    # the step's list-free mu deposit against the stored NPairGridBin list walk,
+    # over several neighbour rebuilds
+    w = workload(n=32, nchains=12, nbeads=30, seed=12, nucwidth=2.0, zghost=1)
+    a = make_gpu(w)
+    b = make_gpu(w)
+    b.set_option(P.OPT_EAGER_GRIDLIST, 1)
+    for _ in range(4):
+        a.run(10)
+        b.run(10)
+        assert np.array_equal(a.get_field(P.MU), b.get_field(P.MU))
+        assert np.array_equal(a.get_forces(), b.get_forces())
+        assert np.array_equal(a.get_field(P.PHI), b.get_field(P.PHI))
+    assert a.counts()["ncalls"] > 1
+    assert _lists_equal(a.gridlist(), b.gridlist())
