@@ -173,14 +173,22 @@ int phafd_hip_timers(phafd_hip_ctx *ctx, double *ms8, int reset);
  *    list is materialised from the last build only when phafd_hip_gridlist()
  *    asks for it.
  *  PHAFD_OPT_FUSE_FH [1]: apply Flory-Huggins inside the load of the r2c of
- *    mu when the two hooks are adjacent (same arithmetic per point). */
-enum { PHAFD_OPT_EAGER_GRIDLIST = 0, PHAFD_OPT_FUSE_FH = 1 };
+ *    mu when the two hooks are adjacent (same arithmetic per point).
+ *  PHAFD_OPT_FUSE_SPECTRAL [1]: fold the k-space gradient multiply and the
+ *    spectral update into the z pass of the FFT (forward z, pointwise map,
+ *    inverse z in one kernel) where the grid allows it (Nz in {16, 64, 256}).
+ *    Same per-mode arithmetic; the inverse z transform factors differently,
+ *    so results agree with the unfused path to rounding, not bitwise.  With
+ *    it on, phafd_hip_run() leaves muhat complete and phihat consumed by the
+ *    c2r, as without it. */
+enum { PHAFD_OPT_EAGER_GRIDLIST = 0, PHAFD_OPT_FUSE_FH = 1, PHAFD_OPT_FUSE_SPECTRAL = 2 };
 int phafd_hip_set_option(phafd_hip_ctx *ctx, int key, double value);
 /* HIP-event timing of every launch of one kernel during run():
  * kernel_id >= 0 (re)starts the probe for that kernel (ids in the order
  * r2c x, y fwd, y inv, z fwd, z inv, c2r x, grad multiply, update, FH,
  * grid-atom deposit, grid-atom force, grid list count, grid list fill,
- * half list count, half list fill, lj/cut); kernel_id < 0 stops it.
+ * half list count, half list fill, lj/cut, fused z gradient, fused z
+ * update); kernel_id < 0 stops it.
  * out3 = {total ms, launches, algorithmic bytes} accumulated so far. */
 int phafd_hip_probe(phafd_hip_ctx *ctx, int kernel_id, double *out3);
 

@@ -19,7 +19,7 @@ ZGHOST_REFERENCE, ZGHOST_PERIODIC = 0, 1
 NOISE_OFF, NOISE_FIELD = 0, 2
 PAIR_GAUSSIAN, PAIR_LJISH = 0, 1
 FP64, FP32 = 0, 1
-OPT_EAGER_GRIDLIST, OPT_FUSE_FH = 0, 1
+OPT_EAGER_GRIDLIST, OPT_FUSE_FH, OPT_FUSE_SPECTRAL = 0, 1, 2
 PH_START, PH_NEIGH, PH_PAIR, PH_INITIAL, PH_POSTFORCE, PH_PREFINAL, PH_FINAL, PH_POSTFINAL = range(8)
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "NaN in phi", -4: "runtime error", -5: "list overflow"}
 
@@ -390,7 +390,8 @@ class System:
         self._c(lib().phafd_hip_set_option(self.h, key, float(value)))
 
     KERNELS = ["r2c_x", "y_fwd", "y_inv", "z_fwd", "z_inv", "c2r_x", "grad_multiply", "update", "fh",
-               "ga_deposit", "ga_force", "grid_count", "grid_fill", "half_count", "half_fill", "lj"]
+               "ga_deposit", "ga_force", "grid_count", "grid_fill", "half_count", "half_fill", "lj",
+               "z_grad", "z_update"]
 
     def probe(self, kernel=None):
         """Start (kernel name) or stop (None) HIP-event timing of one kernel;

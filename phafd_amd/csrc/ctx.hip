@@ -110,6 +110,7 @@ struct phafd_hip_ctx {
   hipEvent_t ev[9] = {};
   double tacc[8] = {0};
   bool fuse_fh = true;
+  bool fuse_spectral = true;
 
   double dx() const { return period[0] / nx; }   // src/grid.cpp:187-201
   double dy() const { return period[1] / ny; }
@@ -508,12 +509,19 @@ SpecConst spec_const(const phafd_hip_ctx *c) {
 
 // FixGridConjugate::start_of_step / FixGridGradPhi::start_of_step,
 // src/fixgrid_conjugate.cpp:62-116: r2c, grad multiply, three c2r, / N
+bool spectral_fused(const phafd_hip_ctx *c) { return c->fuse_spectral && fft_fused_z_ok(c->plan); }
+
 void compute_gradphi(phafd_hip_ctx *c, double2 *phihat_dst) {
-  fft_r2c(c->plan, c->phi.p, phihat_dst, c->stream);
-  grad_multiply(c->plan, phihat_dst, c->ghat[0].p, c->ghat[1].p, c->ghat[2].p, spec_const(c), c->stream);
   double2 *ins[3] = {c->ghat[0].p, c->ghat[1].p, c->ghat[2].p};
   double *outs[3] = {c->grad[0].p, c->grad[1].p, c->grad[2].p};
   const double factor = (double)c->nx * c->ny * c->nz;
+  if (c->fuse_spectral && fft_sep_grad_ok(c->plan)) {
+    fft_grad_sep(c->plan, c->phi.p, phihat_dst, ins[1], ins[2], outs[0], outs[1], outs[2], spec_const(c),
+                 c->stream);
+    return;
+  }
+  fft_r2c(c->plan, c->phi.p, phihat_dst, c->stream);
+  grad_multiply(c->plan, phihat_dst, c->ghat[0].p, c->ghat[1].p, c->ghat[2].p, spec_const(c), c->stream);
   fft_c2r_batch(c->plan, ins, outs, 3, factor, c->stream);
 }
 
@@ -654,6 +662,29 @@ void phase_postfinal(phafd_hip_ctx *c) {
     if (fp->kind == FixDef::CONJ) fft_c2r(c->plan, c->phihat.p, c->phi.p, 0.0, c->stream);
 }
 
+// The same three hooks with the z pass of r2c(mu), the update and the z pass
+// of c2r(phihat) folded into one kernel; used by run_steps, where the hooks
+// always run back to back (one conjugate fix).
+void phase_prefinal_split(phafd_hip_ctx *c, const FHPre *pre) {
+  fft_r2c_xy(c->plan, c->mu.p, c->muhat.p, c->stream, pre);
+}
+void phase_final_split(phafd_hip_ctx *c) {
+  for (auto &fp : c->fixes) {
+    FixDef &f = *fp;
+    if (f.kind == FixDef::CONJ) {
+      const double2 *nz = (f.noise_mode == PHAFD_NOISE_FIELD && c->noise_set) ? c->noise_k.p : nullptr;
+      fft_z_update(c->plan, c->muhat.p, c->phihat.p, nz, update_params(c, f), c->stream);
+    } else {
+      integrate_half(c, f);
+    }
+  }
+}
+void phase_postfinal_split(phafd_hip_ctx *c) {
+  double2 *in = c->phihat.p;
+  double *out = c->phi.p;
+  fft_c2r_yx_batch(c->plan, &in, &out, 1, 0.0, c->stream);
+}
+
 void run_phase(phafd_hip_ctx *c, int ph) {
   switch (ph) {
     case PHAFD_PH_START: phase_start(c); break;
@@ -695,6 +726,9 @@ void run_steps(phafd_hip_ctx *c, int nsteps) {
   require_setup(c);
   FHPre pre;
   const bool fused = can_fuse_fh(c, pre);
+  int nconj = 0;
+  for (auto &f : c->fixes) nconj += f->kind == FixDef::CONJ;
+  const bool zsplit = nconj == 1 && spectral_fused(c);
   for (int it = 0; it < nsteps; it++) {
     c->timestep++;
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
@@ -706,16 +740,15 @@ void run_steps(phafd_hip_ctx *c, int nsteps) {
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
     phase_initial(c);
     HIPCHK(hipEventRecord(c->ev[4], c->stream));
-    if (fused) {
-      phase_prefinal(c, &pre);
-    } else {
-      phase_postforce(c);
-      phase_prefinal(c);
-    }
+    if (!fused) phase_postforce(c);
+    if (zsplit) phase_prefinal_split(c, fused ? &pre : nullptr);
+    else phase_prefinal(c, fused ? &pre : nullptr);
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
-    phase_final(c);
+    if (zsplit) phase_final_split(c);
+    else phase_final(c);
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
-    phase_postfinal(c);
+    if (zsplit) phase_postfinal_split(c);
+    else phase_postfinal(c);
     HIPCHK(hipEventRecord(c->ev[7], c->stream));
     // next step's rebuild decision (Neighbor::decide) + NaN check of
     // phi(0,0,0) (src/integrate.cpp:190-201), read back together
@@ -1291,6 +1324,7 @@ int phafd_hip_set_option(phafd_hip_ctx *c, int key, double value) {
     switch (key) {
       case PHAFD_OPT_EAGER_GRIDLIST: c->eager_gridlist = value != 0; break;
       case PHAFD_OPT_FUSE_FH: c->fuse_fh = value != 0; break;
+      case PHAFD_OPT_FUSE_SPECTRAL: c->fuse_spectral = value != 0; break;
       default: throw std::invalid_argument("unknown option");
     }
     return 0;

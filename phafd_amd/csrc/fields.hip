@@ -4,16 +4,15 @@
 // built with -ffp-contract=off, a kernel given the same inputs as the
 // reference loop reproduces it (log/exp aside, which may differ by an ulp).
 #include "internal.hpp"
+#include "spectral_ops.hpp"
 
 namespace phafd {
 namespace {
 
 constexpr int TPB = 256;
 
-__device__ __forceinline__ int signed_k(int i, int n) { return i > n / 2 ? i - n : i; }
-
-// grad multiply, src/fixgrid_conjugate.cpp:80-105: (phihat * (0, dq)) * n with
-// std::complex semantics.  K layout index = (kx*nz + kz)*ny + ky.
+// grad multiply, src/fixgrid_conjugate.cpp:80-105, for the three components.
+// K layout index = (kx*nz + kz)*ny + ky.
 __global__ void k_grad_multiply(const double2 *__restrict__ ph, double2 *__restrict__ gx,
                                 double2 *__restrict__ gy, double2 *__restrict__ gz, int hx,
                                 int ny, int nz, SpecConst c) {
@@ -25,37 +24,10 @@ __global__ void k_grad_multiply(const double2 *__restrict__ ph, double2 *__restr
     const int kz = (int)(t % nz);
     const int kx = (int)(t / nz);
     const double2 v = ph[i];
-    const double n = (double)kx;
-    const double m = (double)signed_k(kz, nz);
-    const double l = (double)signed_k(ky, ny);
-    // (a + ib) * (0 + id) = (a*0 - b*d, a*d + b*0), then * scalar
-    double re, im;
-    re = v.x * 0.0 - v.y * c.dqx;
-    im = v.x * c.dqx + v.y * 0.0;
-    gx[i] = make_double2(re * n, im * n);
-    re = v.x * 0.0 - v.y * c.dqy_kz;
-    im = v.x * c.dqy_kz + v.y * 0.0;
-    gz[i] = make_double2(re * m, im * m);   // ft_gradphi[1] -> gradphi[2]
-    re = v.x * 0.0 - v.y * c.dqz_ky;
-    im = v.x * c.dqz_ky + v.y * 0.0;
-    gy[i] = make_double2(re * l, im * l);   // ft_gradphi[2] -> gradphi[1]
+    gx[i] = grad_mode(v, c.dqx, (double)kx);
+    gz[i] = grad_mode(v, c.dqy_kz, (double)signed_k(kz, nz));   // ft_gradphi[1] -> gradphi[2]
+    gy[i] = grad_mode(v, c.dqz_ky, (double)signed_k(ky, ny));   // ft_gradphi[2] -> gradphi[1]
   }
-}
-
-// one mode of ConjugateVolFrac::complex_update / real_update,
-// src/conjugate_volfrac.cpp:89-133 (reference indices i = ky, j = kz, k = kx)
-__device__ __forceinline__ double2 mode_update(double2 phi, double2 mu, double2 noise, int i,
-                                               int j, int k, int A, int B,
-                                               const UpdateParams &u) {
-  const double qz = (i > A / 2) ? -u.qky_scale * (A - i) : u.qky_scale * i;   // qzs[i]
-  const double qy = (j > B / 2) ? -u.qkz_scale * (B - j) : u.qkz_scale * j;   // qys[j]
-  const double qx = u.qx_scale * k;
-  const double q2 = qx * qx + qy * qy + qz * qz;
-  const double s = u.temp / u.volFH * u.gamma * q2;
-  const double c = u.mobility * q2 * u.dt;
-  const double tx = mu.x + s * phi.x, tyy = mu.y + s * phi.y;
-  const double ux = phi.x - c * tx, uy = phi.y - c * tyy;
-  return make_double2(ux * u.normalization + noise.x, uy * u.normalization + noise.y);
 }
 
 // Conjugate::update (src/conjugate.cpp:163-327, one rank) as a pointwise map.
@@ -63,39 +35,56 @@ __device__ __forceinline__ double2 mode_update(double2 phi, double2 mu, double2 
 // "primary" modes (the ones single()/line_single() update) write themselves
 // and conj() into their mirror (-ky, -kz); mirror threads do nothing; the
 // eight self-conjugate modes use real_update; the origin is only rescaled.
+__device__ __forceinline__ void update_index(double2 *__restrict__ ph, const double2 *__restrict__ mu,
+                                             const double2 *__restrict__ noise, size_t idx, int hx,
+                                             int ny, int nz, const UpdateParams &u) {
+  const int A = ny, B = nz, H = hx - 1;
+  const int i = (int)(idx % ny);   // ky
+  const size_t t = idx / ny;
+  const int j = (int)(t % nz);     // kz
+  const int k = (int)(t / nz);     // kx
+  const double2 nz0 = make_double2(0.0, 0.0);
+  if (k != 0 && k != H) {
+    ph[idx] = mode_update(ph[idx], mu[idx], noise ? noise[idx] : nz0, i, j, k, A, B, u);
+    return;
+  }
+  const bool self = (i == 0 || i == A / 2) && (j == 0 || j == B / 2);
+  if (self) {
+    if (k == 0 && i == 0 && j == 0) {   // origin_update
+      const double2 v = ph[idx];
+      ph[idx] = make_double2(v.x * u.normalization, v.y * u.normalization);
+    } else {
+      ph[idx] = mode_update(ph[idx], mu[idx], noise ? noise[idx] : nz0, i, j, k, A, B, u);
+    }
+    return;
+  }
+  const bool primary = (j >= 1 && j < B / 2) || (j == 0 && i >= 1 && i < A / 2) ||
+                       (j == B / 2 && i > A / 2);
+  if (!primary) return;
+  const double2 r = mode_update(ph[idx], mu[idx], noise ? noise[idx] : nz0, i, j, k, A, B, u);
+  ph[idx] = r;
+  const int im = (A - i) % A, jm = (B - j) % B;
+  ph[((size_t)k * nz + jm) * ny + im] = make_double2(r.x, -r.y);
+}
+
 __global__ void k_update(double2 *__restrict__ ph, const double2 *__restrict__ mu,
                          const double2 *__restrict__ noise, int hx, int ny, int nz,
                          UpdateParams u) {
   const size_t M = (size_t)hx * nz * ny;
-  const int A = ny, B = nz, H = hx - 1;
   for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < M;
-       idx += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % ny);   // ky
-    const size_t t = idx / ny;
-    const int j = (int)(t % nz);     // kz
-    const int k = (int)(t / nz);     // kx
-    const double2 nz0 = make_double2(0.0, 0.0);
-    if (k != 0 && k != H) {
-      ph[idx] = mode_update(ph[idx], mu[idx], noise ? noise[idx] : nz0, i, j, k, A, B, u);
-      continue;
-    }
-    const bool self = (i == 0 || i == A / 2) && (j == 0 || j == B / 2);
-    if (self) {
-      if (k == 0 && i == 0 && j == 0) {   // origin_update
-        const double2 v = ph[idx];
-        ph[idx] = make_double2(v.x * u.normalization, v.y * u.normalization);
-      } else {
-        ph[idx] = mode_update(ph[idx], mu[idx], noise ? noise[idx] : nz0, i, j, k, A, B, u);
-      }
-      continue;
-    }
-    const bool primary = (j >= 1 && j < B / 2) || (j == 0 && i >= 1 && i < A / 2) ||
-                         (j == B / 2 && i > A / 2);
-    if (!primary) continue;
-    const double2 r = mode_update(ph[idx], mu[idx], noise ? noise[idx] : nz0, i, j, k, A, B, u);
-    ph[idx] = r;
-    const int im = (A - i) % A, jm = (B - j) % B;
-    ph[((size_t)k * nz + jm) * ny + im] = make_double2(r.x, -r.y);
+       idx += (size_t)gridDim.x * blockDim.x)
+    update_index(ph, mu, noise, idx, hx, ny, nz, u);
+}
+
+// only the planes kx = 0 and kx = Nx/2 (the bulk is fused into the z pass)
+__global__ void k_update_edges(double2 *__restrict__ ph, const double2 *__restrict__ mu,
+                               const double2 *__restrict__ noise, int hx, int ny, int nz,
+                               UpdateParams u) {
+  const size_t P = (size_t)nz * ny;
+  for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < 2 * P;
+       w += (size_t)gridDim.x * blockDim.x) {
+    const size_t idx = w < P ? w : (size_t)(hx - 1) * P + (w - P);
+    update_index(ph, mu, noise, idx, hx, ny, nz, u);
   }
 }
 
@@ -195,6 +184,13 @@ void spectral_update(const FFTPlan &p, double2 *phihat, const double2 *muhat,
   hipLaunchKernelGGL(k_update, dim3(grid_for(p.M())), dim3(TPB), 0, s, phihat, muhat, noise_k,
                      p.hx, p.ny, p.nz, u);
   probe_end(K_UPDATE, s, (noise_k ? 64.0 : 48.0) * p.M());
+  HIPCHK(hipGetLastError());
+}
+
+void spectral_update_edges(const FFTPlan &p, double2 *phihat, const double2 *muhat,
+                           const double2 *noise_k, UpdateParams u, hipStream_t s) {
+  hipLaunchKernelGGL(k_update_edges, dim3(grid_for(2 * (size_t)p.ny * p.nz)), dim3(TPB), 0, s, phihat,
+                     muhat, noise_k, p.hx, p.ny, p.nz, u);
   HIPCHK(hipGetLastError());
 }
 

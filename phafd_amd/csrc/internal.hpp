@@ -57,7 +57,7 @@ struct DBuf {   // device buffer owned by the context
 enum KernelId {
   K_R2C_X = 0, K_Y_FWD, K_Y_INV, K_Z_FWD, K_Z_INV, K_C2R_X, K_GRAD_MUL, K_UPDATE, K_FH,
   K_GA_DEPOSIT, K_GA_FORCE, K_GRID_COUNT, K_GRID_FILL, K_HALF_COUNT, K_HALF_FILL, K_LJ,
-  K_NKERNELS
+  K_Z_GRAD, K_Z_UPDATE, K_NKERNELS
 };
 struct KernelProbe {
   int id = -1;
@@ -104,6 +104,27 @@ void fft_c2r(const FFTPlan &p, double2 *in, double *out, double divisor, hipStre
 void fft_c2r_batch(const FFTPlan &p, double2 *const *in, double *const *out, int batch,
                    double divisor, hipStream_t s);
 
+// Split transforms for the fused z pass.  fft_r2c_xy does the x and y passes
+// of an r2c; fft_z_update finishes mu's transform in place, applies the
+// spectral update to phihat and leaves it z-inverted (finish with
+// fft_c2r_yx_batch).
+struct SpecConst;
+struct UpdateParams;
+bool fft_fused_z_ok(const FFTPlan &p);
+void fft_r2c_xy(const FFTPlan &p, const double *in, double2 *out, hipStream_t s,
+                const FHPre *pre = nullptr);
+// Gradient of phi by separable spectral derivatives (grids with
+// fft_sep_grad_ok): gx from one x-line kernel, gy from the y pass, gz from the
+// z pass; ph receives phihat (the full r2c of phi); gy_k, gz_k are scratch
+// spectra.  Equal in exact arithmetic to r2c, grad multiply, three c2r / N.
+bool fft_sep_grad_ok(const FFTPlan &p);
+void fft_grad_sep(const FFTPlan &p, const double *phi, double2 *ph, double2 *gy_k, double2 *gz_k,
+                  double *gx, double *gy, double *gz, SpecConst sc, hipStream_t s);
+void fft_z_update(const FFTPlan &p, double2 *mu, double2 *ph, const double2 *noise, UpdateParams u,
+                  hipStream_t s);
+void fft_c2r_yx_batch(const FFTPlan &p, double2 *const *in, double *const *out, int batch,
+                      double divisor, hipStream_t s);
+
 // K layout <-> canonical layout (host-visible order) conversions on device
 void spec_k_to_canon(const FFTPlan &p, const double2 *k, double2 *canon, hipStream_t s);
 void spec_canon_to_k(const FFTPlan &p, const double2 *canon, double2 *k, hipStream_t s);
@@ -120,6 +141,8 @@ struct UpdateParams {
 };
 void spectral_update(const FFTPlan &p, double2 *phihat, const double2 *muhat,
                      const double2 *noise_k, UpdateParams u, hipStream_t s);
+void spectral_update_edges(const FFTPlan &p, double2 *phihat, const double2 *muhat,
+                           const double2 *noise_k, UpdateParams u, hipStream_t s);
 void floryhuggins(size_t n, double *phi, double *mu, double tv, double chi,
                   unsigned long long *counters, hipStream_t s);
 void fill_zero(void *p, size_t bytes, hipStream_t s);

@@ -193,3 +193,54 @@ def test_list_free_deposit_matches_stored_list_bitwise():
         assert np.array_equal(a.get_field(P.PHI), b.get_field(P.PHI))
     assert a.counts()["ncalls"] > 1
     assert _lists_equal(a.gridlist(), b.gridlist())
+
+
+def _fields_only(cls, shape, period, noise, seed=0):
+    nz, ny, nx = shape
+    s = cls(nx, ny, nz, [0.0, 0.0, 0.0], period)
+    s.set_field(0, P.ic_noisy_constant(shape, 0.5, 0.05, 99 + seed))
+    z = np.zeros(0, np.int32)
+    s.set_atoms(np.zeros((0, 3)), z, tags=z, mols=z)
+    s.fix_conjugate(5678, 1.0, 1.0, 1.0, 1.0, noise)
+    s.fix_floryhuggins(1.0, 1.0, 3.0)
+    s.setup(1e-3)
+    return s
+
+
+# grids whose z pass takes the fused kernels (Nz in {16, 64, 256}); the
+# anisotropic periods check that the wave-number scales go to the right axes
+@pytest.mark.parametrize("shape,period", [((16, 64, 32), [32.0, 96.0, 12.0]),
+                                          ((256, 16, 8), [8.0, 16.0, 256.0]),
+                                          ((64, 64, 64), [64.0, 64.0, 64.0])])
+def test_fused_spectral_path(shape, period):
+    nz, ny, nx = shape
+    fused = _fields_only(P.System, shape, period, P.NOISE_FIELD)
+    plain = _fields_only(P.System, shape, period, P.NOISE_FIELD)
+    plain.set_option(P.OPT_FUSE_SPECTRAL, 0)
+    o = _fields_only(O.System, shape, period, O.NOISE_MT19937)
+    gen = P.NoiseGen(5678)
+    for _ in range(4):
+        xi = gen.next(nx, ny, nz, period, 1.0, 1.0, 1e-3)
+        fused.set_noise(xi)
+        plain.set_noise(xi)
+        fused.run(1)
+        plain.run(1)
+        o.run(1)
+        for f in (P.PHI, P.MU, P.GRADX, P.GRADY, P.GRADZ):
+            assert rel_l2(fused.get_field(f), plain.get_field(f)) < 1e-13, f
+        assert rel_l2(fused.get_spectrum(P.MUHAT), plain.get_spectrum(P.MUHAT)) < 1e-13
+    for f in (P.PHI, P.MU, P.GRADX, P.GRADY, P.GRADZ):
+        assert rel_l2(fused.get_field(f), o.get_field(f)) < 1e-12, f
+
+
+def test_fused_spectral_probe_counts():
+    # the fused kernels are the ones that run on a 64^3 grid
+    w = workload(n=64, nchains=4, nbeads=10, seed=9)
+    g = make_gpu(w)
+    g.probe("z_update")
+    g.run(3)
+    ms, launches, nbytes = g.probe(None)
+    assert launches == 3 and nbytes == 3 * 64.0 * 31 * 64 * 64
+    g.probe("z_grad")
+    g.run(2)
+    assert g.probe(None)[1] == 2
