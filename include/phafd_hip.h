@@ -180,8 +180,14 @@ int phafd_hip_timers(phafd_hip_ctx *ctx, double *ms8, int reset);
  *    Same per-mode arithmetic; the inverse z transform factors differently,
  *    so results agree with the unfused path to rounding, not bitwise.  With
  *    it on, phafd_hip_run() leaves muhat complete and phihat consumed by the
- *    c2r, as without it. */
-enum { PHAFD_OPT_EAGER_GRIDLIST = 0, PHAFD_OPT_FUSE_FH = 1, PHAFD_OPT_FUSE_SPECTRAL = 2 };
+ *    c2r, as without it.
+ *  PHAFD_OPT_EAGER_HALFLIST [0]: build the NPairHalfBinNewton list with the
+ *    full neighbour cut at every build and walk it for lj/cut.  By default the
+ *    step walks a pruned copy (pairs closer than lj cut + skin at the build, in
+ *    the same relative order: bit-identical forces) and the full list is
+ *    materialised from the last build when phafd_hip_halflist() asks. */
+enum { PHAFD_OPT_EAGER_GRIDLIST = 0, PHAFD_OPT_FUSE_FH = 1, PHAFD_OPT_FUSE_SPECTRAL = 2,
+       PHAFD_OPT_EAGER_HALFLIST = 3 };
 int phafd_hip_set_option(phafd_hip_ctx *ctx, int key, double value);
 /* HIP-event timing of every launch of one kernel during run():
  * kernel_id >= 0 (re)starts the probe for that kernel (ids in the order

@@ -465,6 +465,20 @@ __device__ __forceinline__ GABox ga_box(const GridView &gv, const double4 &x, do
 __device__ __forceinline__ bool ga_in(const GAParams &ga, double rsq, double cs) {
   return ga.style == 0 ? (rsq <= cs) : (rsq < cs);
 }
+// x range of one (z, y) row of the box that can be in range: the sphere's
+// chord widened by one grid point each side (the exact test stays in the
+// loop, so the widening only has to cover the rounding of this estimate)
+__device__ __forceinline__ bool ga_row(const GridView &gv, const GABox &b, double xx, double cs, double dyz,
+                                       int &k0, int &k1) {
+  const double rem = cs - dyz;
+  if (rem < -1e-9 * cs) return false;
+  const double w = sqrt(rem > 0 ? rem : 0.0);
+  const int a0 = (int)floor((xx - w - gv.lo[0]) / gv.dx) - 1;
+  const int a1 = (int)ceil((xx + w - gv.lo[0]) / gv.dx) + 1;
+  k0 = a0 > b.k0 ? a0 : b.k0;
+  k1 = a1 < b.k1 ? a1 : b.k1;
+  return k0 <= k1;
+}
 
 __global__ void k_ga_count(int ng, GridView gv, GAParams ga, const double4 *gx, int *acnt) {
   GRID_STRIDE(a, ng) {
@@ -477,7 +491,9 @@ __global__ void k_ga_count(int ng, GridView gv, GAParams ga, const double4 *gx, 
         const double delz = (i * gv.dz + gv.lo[2]) - x.z;
         for (int j = b.j0; j <= b.j1; j++) {
           const double dely = (j * gv.dy + gv.lo[1]) - x.y;
-          for (int k = b.k0; k <= b.k1; k++) {
+          int rk0, rk1;
+          if (!ga_row(gv, b, x.x, cs, dely * dely + delz * delz, rk0, rk1)) continue;
+          for (int k = rk0; k <= rk1; k++) {
             const double delx = (k * gv.dx + gv.lo[0]) - x.x;
             const double rsq = delx * delx + dely * dely + delz * delz;
             n += ga_in(ga, rsq, cs) ? 1 : 0;
@@ -508,7 +524,9 @@ __global__ void k_ga_force_emit(int ng, GridView gv, GAParams ga, const double4 
       for (int j = b.j0; j <= b.j1; j++) {
         const double ytmp = j * gv.dy + gv.lo[1];
         const double dely = ytmp - x.y;
-        for (int k = b.k0; k <= b.k1; k++) {
+        int rk0, rk1;
+        if (!ga_row(gv, b, x.x, cs, dely * dely + delz * delz, rk0, rk1)) continue;
+        for (int k = rk0; k <= rk1; k++) {
           const double xtmp = k * gv.dx + gv.lo[0];
           const double delx = xtmp - x.x;
           double rsq = delx * delx + dely * dely + delz * delz;
@@ -559,11 +577,14 @@ __global__ void k_ga_place(int npairs, const int *pg, const int *ppos, const int
 
 // mu(g) += its terms in ascending key (list) order; a grid point holds few
 // terms, so a selection over the group is cheap
-__global__ void k_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu) {
+__global__ void k_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu, bool first) {
   GRID_STRIDE(p, N) {
     const int s = goff[p], e = goff[p + 1];
-    if (s == e) continue;
-    double m = mu[p];
+    if (s == e) {
+      if (first) mu[p] = 0.0;
+      continue;
+    }
+    double m = first ? 0.0 : mu[p];
     if (e - s == 1) {
       m += sv[s];
     } else {
@@ -898,8 +919,9 @@ void launch_ga_place(int npairs, const int *pg, const int *ppos, const int *goff
   hipLaunchKernelGGL(k_ga_place, dim3(nblk(npairs)), dim3(TPB), 0, s, npairs, pg, ppos, goff, pkey, pv, skey, sv);
   HIPCHK(hipGetLastError());
 }
-void launch_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu, hipStream_t s) {
-  hipLaunchKernelGGL(k_ga_sum, dim3(nblk(N)), dim3(TPB), 0, s, N, goff, skey, sv, mu);
+void launch_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu, bool first,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_ga_sum, dim3(nblk(N)), dim3(TPB), 0, s, N, goff, skey, sv, mu, first);
   HIPCHK(hipGetLastError());
 }
 void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,

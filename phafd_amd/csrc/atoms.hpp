@@ -80,7 +80,9 @@ void launch_ga_rank(int npairs, const int *pg, int *gcnt, int *ppos, hipStream_t
 void launch_binpos(int ng, const int *binatoms, int *binpos, hipStream_t s);
 void launch_ga_place(int npairs, const int *pg, const int *ppos, const int *goff, const int *pkey,
                      const double *pv, int *skey, double *sv, hipStream_t s);
-void launch_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu, hipStream_t s);
+// first: mu is written (terms summed from 0) instead of added to
+void launch_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu, bool first,
+                   hipStream_t s);
 void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,
                      const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
                      double4 *gF, double *energy, hipStream_t s);

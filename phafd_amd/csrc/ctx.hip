@@ -83,8 +83,16 @@ struct phafd_hip_ctx {
   double skin = 0, cutneigh = 0, triggersq = 0;
   int64_t ncalls = 0;
   BinGeom bg{};
-  std::vector<int> stencil_full, stencil_half;
-  DBuf<int> d_stencil_full, d_stencil_half, d_sfx, d_sfy, d_sfz;
+  std::vector<int> stencil_full, stencil_half, stencil_lj;
+  DBuf<int> d_stencil_full, d_stencil_half, d_stencil_lj, d_sfx, d_sfy, d_sfz;
+  // The step's lj/cut list keeps only the half-list pairs closer than
+  // (lj cut + skin) at the build (prune_sq), walking only the half-stencil bins
+  // that can hold them: every pair that can come inside the lj cut before the
+  // next build is kept, in the reference's relative order, so the forces are
+  // the same bit for bit.  The full reference list is materialised from the
+  // build snapshot when asked for (half_full) or at every build (eager).
+  double prune_sq = 0;
+  bool half_full = false, eager_halflist = false;
   DBuf<double4> gx_build;             // gathered positions at the last build
   bool grid_built = false;            // grid list materialised for the last build
   bool eager_gridlist = false;        // build (and use) the stored grid list in the step
@@ -361,6 +369,21 @@ void create_stencils(phafd_hip_ctx *c) {
         if (k > 0 || j > 0 || (j == 0 && i > 0))
           if (bin_distance(i, j, k) < cutneighsq)
             c->stencil_half.push_back(k * b.mbiny * b.mbinx + j * b.mbinx + i);
+  double ljcut = 0.0;
+  for (auto &p : c->pairs)
+    if (p->style == 2) ljcut = std::max(ljcut, std::sqrt(p->ljcutsq));
+  const double prune = ljcut + c->skin;
+  c->prune_sq = prune < c->cutneigh ? prune * prune : cutneighsq;
+  c->stencil_lj.clear();
+  for (int k = 0; k <= sz; k++)
+    for (int j = -sy; j <= sy; j++)
+      for (int i = -sx; i <= sx; i++)
+        if (k > 0 || j > 0 || (j == 0 && i > 0))
+          if (bin_distance(i, j, k) < cutneighsq && bin_distance(i, j, k) <= c->prune_sq)
+            c->stencil_lj.push_back(k * b.mbiny * b.mbinx + j * b.mbinx + i);
+  c->d_stencil_lj.alloc(c->stencil_lj.size() + 1);
+  if (!c->stencil_lj.empty())
+    HIPCHK(hipMemcpy(c->d_stencil_lj.p, c->stencil_lj.data(), sizeof(int) * c->stencil_lj.size(), hipMemcpyHostToDevice));
   c->d_stencil_full.alloc(c->stencil_full.size() + 1);
   c->d_stencil_half.alloc(c->stencil_half.size() + 1);
   if (!c->stencil_full.empty())
@@ -418,9 +441,14 @@ void build_gridlist(phafd_hip_ctx *c) {
 
 // NPairHalfBinNewton::build (src/npair_half_bin_newton.cpp:38-124) plus the
 // transposed slots used by the deterministic force gather
-void build_halflist(phafd_hip_ctx *c) {
+// full = the reference's list (cut = cutneigh), else the pruned lj/cut list;
+// both from the bins and positions of the last build
+void build_halflist(phafd_hip_ctx *c, bool full) {
   const int ng = c->ngathered;
-  const double cutneighsq = c->cutneigh * c->cutneigh;
+  const double cutneighsq = full ? c->cutneigh * c->cutneigh : c->prune_sq;
+  const DBuf<int> &st = full ? c->d_stencil_half : c->d_stencil_lj;
+  const int nst = (int)(full ? c->stencil_half.size() : c->stencil_lj.size());
+  const double4 *pos = c->gx_build.p;
   c->tmp_flag.alloc((size_t)ng + 16);
   c->tmp_pos.alloc((size_t)ng + 16);
   launch_local_flag(ng, c->glabel.p, c->tmp_flag.p, c->stream);
@@ -432,15 +460,14 @@ void build_halflist(phafd_hip_ctx *c) {
   c->h_cnt.alloc((size_t)nrows + 1);
   c->h_off.alloc((size_t)nrows + 2);
   launch_half_list(false, nrows, c->h_ilist.p, c->a2b.p, c->binstart.p, c->binatoms.p, c->glabel.p,
-                   c->d_stencil_half.p, (int)c->stencil_half.size(), c->gx.p, cutneighsq, c->nowned,
-                   c->h_cnt.p, nullptr, nullptr, c->flags.p + 1, c->stream);
+                   st.p, nst, pos, cutneighsq, c->nowned, c->h_cnt.p, nullptr, nullptr, c->flags.p + 1, c->stream);
   scan_exclusive_i32(c->h_cnt.p, c->h_off.p, (size_t)nrows, c->stream);
   const int nent = read_int(c, c->h_off.p + nrows);
   c->h_nent = nent;
   c->h_neigh.alloc((size_t)nent + 1);
   launch_half_list(true, nrows, c->h_ilist.p, c->a2b.p, c->binstart.p, c->binatoms.p, c->glabel.p,
-                   c->d_stencil_half.p, (int)c->stencil_half.size(), c->gx.p, cutneighsq, c->nowned,
-                   c->h_cnt.p, c->h_off.p, c->h_neigh.p, c->flags.p + 1, c->stream);
+                   st.p, nst, pos, cutneighsq, c->nowned, c->h_cnt.p, c->h_off.p, c->h_neigh.p, c->flags.p + 1,
+                   c->stream);
   c->t_cnt.alloc((size_t)ng + 1);
   c->t_cur.alloc((size_t)ng + 1);
   c->t_off.alloc((size_t)ng + 2);
@@ -453,6 +480,7 @@ void build_halflist(phafd_hip_ctx *c) {
   launch_t_place(nrows, c->h_ilist.p, c->h_off.p, c->h_neigh.p, c->nowned, c->t_off.p, c->t_cur.p,
                  c->t_pair.p, c->t_i.p, c->stream);
   launch_t_sort(ng, c->t_off.p, c->t_pair.p, c->t_i.p, c->stream);
+  c->half_full = full;
 }
 
 // Neighbor::build, src/neighbor.cpp:99-125
@@ -464,7 +492,7 @@ void neighbor_build(phafd_hip_ctx *c) {
   launch_copy4(c->ngathered, c->gx.p, c->gx_build.p, c->stream);
   c->grid_built = false;
   if (c->need_full && c->eager_gridlist) build_gridlist(c);
-  if (c->need_half) build_halflist(c);
+  if (c->need_half) build_halflist(c, c->eager_halflist);
   c->lists_valid = true;
 }
 
@@ -561,11 +589,15 @@ void phase_neigh(phafd_hip_ctx *c) {
 // emitted with their list-order keys, grouped per grid point by a counting
 // sort and summed in key order.  With the eager option the mu side walks the
 // stored list instead (the two are bit-identical).
-void grid_atom_pass(phafd_hip_ctx *c, PairDef &p, double *energy) {
+// first: mu has not been zeroed this step; the pass writes every point
+bool ga_uses_emit(const phafd_hip_ctx *c) { return !(c->eager_gridlist && c->grid_built); }
+
+void grid_atom_pass(phafd_hip_ctx *c, PairDef &p, double *energy, bool first) {
   GridView gv = c->gridview();
   GAParams ga = ga_params(p);
   const int ng = c->ngathered;
-  const bool emit = !(c->eager_gridlist && c->grid_built);
+  const bool emit = ga_uses_emit(c);
+  if (first && !emit) fill_zero(c->mu.p, sizeof(double) * c->N, c->stream);
   int npairs = 0;
   if (emit) {
     c->ga_acnt.alloc((size_t)ng + 1);
@@ -594,14 +626,24 @@ void grid_atom_pass(phafd_hip_ctx *c, PairDef &p, double *energy) {
     scan_exclusive_i32(c->ga_gcnt.p, c->ga_goff.p, c->N, c->stream);
     launch_ga_place(npairs, c->ga_pg.p, c->ga_ppos.p, c->ga_goff.p, c->ga_pkey.p, c->ga_pv.p, c->ga_skey.p,
                     c->ga_sv.p, c->stream);
-    launch_ga_sum(c->N, c->ga_goff.p, c->ga_skey.p, c->ga_sv.p, c->mu.p, c->stream);
+    launch_ga_sum(c->N, c->ga_goff.p, c->ga_skey.p, c->ga_sv.p, c->mu.p, first, c->stream);
     probe_end(K_GA_DEPOSIT, c->stream, 16.0 * c->N + 40.0 * npairs);
   }
 }
 
 // Integrate::run pair block, src/integrate.cpp:130-146 (one pass, finding F4)
 void phase_pair(phafd_hip_ctx *c, double *energy_ga = nullptr, double *energy_lj = nullptr) {
-  fill_zero(c->mu.p, sizeof(double) * c->N, c->stream);
+  // mu = 0 before the pair computes: done by the first grid-atom pass, which
+  // writes every grid point, when there is one
+  bool mu_pending = true;
+  const bool any_ga = c->neigh_set && c->lists_valid &&
+                      std::any_of(c->pairs.begin(), c->pairs.end(), [](const std::unique_ptr<PairDef> &p) {
+                        return p->style != 2;
+                      });
+  if (!any_ga) {
+    fill_zero(c->mu.p, sizeof(double) * c->N, c->stream);
+    mu_pending = false;
+  }
   launch_fill_d4(c->ngathered, c->gF.p, c->stream);
   launch_fill_d4(c->nowned, c->F.p, c->stream);
   if (!c->neigh_set || !c->lists_valid) return;
@@ -614,7 +656,8 @@ void phase_pair(phafd_hip_ctx *c, double *energy_ga = nullptr, double *energy_lj
                       c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, energy_lj, c->stream);
       probe_end(K_LJ, c->stream, 64.0 * c->ngathered + 4.0 * c->h_nent);
     } else {
-      grid_atom_pass(c, p, energy_ga);
+      grid_atom_pass(c, p, energy_ga, mu_pending);
+      mu_pending = false;
     }
   }
   reverse_comm(c);
@@ -1204,6 +1247,7 @@ int phafd_hip_halflist(phafd_hip_ctx *c, int64_t cap, int *ilist, int *numneigh,
                        int64_t *inum, int64_t *nentries) {
   return guard(c, [&] {
     if (!c->lists_valid || !c->need_half) { *inum = 0; *nentries = 0; return 0; }
+    if (!c->half_full) build_halflist(c, true);   // from the bins and positions of the last build
     *inum = c->h_nrows;
     *nentries = c->h_nent;
     if (!ilist) return 0;
@@ -1325,6 +1369,7 @@ int phafd_hip_set_option(phafd_hip_ctx *c, int key, double value) {
       case PHAFD_OPT_EAGER_GRIDLIST: c->eager_gridlist = value != 0; break;
       case PHAFD_OPT_FUSE_FH: c->fuse_fh = value != 0; break;
       case PHAFD_OPT_FUSE_SPECTRAL: c->fuse_spectral = value != 0; break;
+      case PHAFD_OPT_EAGER_HALFLIST: c->eager_halflist = value != 0; break;
       default: throw std::invalid_argument("unknown option");
     }
     return 0;

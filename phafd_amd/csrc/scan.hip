@@ -1,6 +1,7 @@
 // scan.hip -- device-wide exclusive prefix sums (stream compaction, CSR
 // offsets, counting sorts).  Three phases: per-block sums, scan of the block
-// sums (recursive), per-block scan plus block offset.  4096 items per block.
+// sums (recursive), per-block scan plus block offset.  4096 items per block,
+// 16 consecutive items per thread.
 #include "internal.hpp"
 
 namespace phafd {
@@ -10,64 +11,95 @@ constexpr int ST = 256;            // threads per block
 constexpr int SQ = 16;             // items per thread
 constexpr int SCHUNK = ST * SQ;    // 4096 items per block
 
+// Each item owns SQ consecutive entries (vector loads where the run is whole),
+// keeps its run in registers, and the block combines item totals with wave
+// shuffles plus one LDS exchange between the four waves.
+template <class TI>
+__device__ __forceinline__ void load_run(const TI *__restrict__ in, size_t base, size_t n, TI *v) {
+  if (base + SQ <= n && (sizeof(TI) * SQ) % 16 == 0 && ((uintptr_t)in & 15) == 0) {
+    const int4 *p = reinterpret_cast<const int4 *>(in + base);
+    constexpr int NV = (int)(sizeof(TI) * SQ / 16);
+#pragma unroll
+    for (int q = 0; q < NV; q++) reinterpret_cast<int4 *>(v)[q] = p[q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < SQ; q++) v[q] = base + q < n ? in[base + q] : (TI)0;
+  }
+}
+
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
 template <class TI>
 __global__ __launch_bounds__(ST) void k_reduce(const TI *__restrict__ in, size_t n,
                                                int64_t *__restrict__ bsum) {
-  __shared__ int64_t sh[ST];
-  const size_t base = (size_t)blockIdx.x * SCHUNK;
+  __shared__ int64_t ws[ST / 64];
+  const size_t base = (size_t)blockIdx.x * SCHUNK + (size_t)threadIdx.x * SQ;
+  TI v[SQ];
+  load_run(in, base, n, v);
   int64_t acc = 0;
 #pragma unroll
-  for (int q = 0; q < SQ; q++) {
-    const size_t i = base + (size_t)q * ST + threadIdx.x;
-    if (i < n) acc += (int64_t)in[i];
-  }
-  sh[threadIdx.x] = acc;
+  for (int q = 0; q < SQ; q++) acc += (int64_t)v[q];
+  acc = wave_incl_scan(acc);
+  if ((threadIdx.x & 63) == 63) ws[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int s = ST / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+#pragma unroll
+    for (int w = 0; w < ST / 64; w++) t += ws[w];
+    bsum[blockIdx.x] = t;
   }
-  if (threadIdx.x == 0) bsum[blockIdx.x] = sh[0];
 }
 
 template <class TI, class TO>
 __global__ __launch_bounds__(ST) void k_scan_apply(const TI *__restrict__ in, size_t n,
                                                    const int64_t *__restrict__ boff,
                                                    TO *__restrict__ out) {
-  __shared__ int64_t sh[SCHUNK];
-  __shared__ int64_t ts[ST];
-  const size_t base = (size_t)blockIdx.x * SCHUNK;
+  __shared__ int64_t ws[ST / 64];
+  const size_t base = (size_t)blockIdx.x * SCHUNK + (size_t)threadIdx.x * SQ;
+  TI v[SQ];
+  load_run(in, base, n, v);
+  int64_t run = 0;
 #pragma unroll
-  for (int q = 0; q < SQ; q++) {
-    const size_t i = base + (size_t)q * ST + threadIdx.x;
-    sh[q * ST + threadIdx.x] = i < n ? (int64_t)in[i] : 0;
-  }
+  for (int q = 0; q < SQ; q++) run += (int64_t)v[q];
+  const int64_t incl = wave_incl_scan(run);
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 63) ws[wid] = incl;
   __syncthreads();
-  int64_t s = 0;
-  for (int q = 0; q < SQ; q++) {
-    const int64_t v = sh[threadIdx.x * SQ + q];
-    sh[threadIdx.x * SQ + q] = s;
-    s += v;
-  }
-  ts[threadIdx.x] = s;
-  __syncthreads();
-  // inclusive Hillis-Steele scan of the per-thread totals
-  for (int d = 1; d < ST; d <<= 1) {
-    int64_t v = (int)threadIdx.x >= d ? ts[threadIdx.x - d] : 0;
-    __syncthreads();
-    ts[threadIdx.x] += v;
-    __syncthreads();
-  }
-  const int64_t excl = (threadIdx.x == 0 ? 0 : ts[threadIdx.x - 1]) + (boff ? boff[blockIdx.x] : 0);
-  for (int q = 0; q < SQ; q++) sh[threadIdx.x * SQ + q] += excl;
-  __syncthreads();
+  int64_t woff = 0, total = 0;
 #pragma unroll
-  for (int q = 0; q < SQ; q++) {
-    const size_t i = base + (size_t)q * ST + threadIdx.x;
-    if (i < n) out[i] = (TO)sh[q * ST + threadIdx.x];
+  for (int w = 0; w < ST / 64; w++) {
+    woff += w < wid ? ws[w] : 0;
+    total += ws[w];
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == ST - 1)
-    out[n] = (TO)(ts[ST - 1] + (boff ? boff[blockIdx.x] : 0));
+  const int64_t b0 = boff ? boff[blockIdx.x] : 0;
+  int64_t e = b0 + woff + incl - run;
+  if (base + SQ <= n && (sizeof(TO) * SQ) % 16 == 0 && ((uintptr_t)out & 15) == 0) {
+    TO o[SQ];
+#pragma unroll
+    for (int q = 0; q < SQ; q++) {
+      o[q] = (TO)e;
+      e += (int64_t)v[q];
+    }
+    int4 *p = reinterpret_cast<int4 *>(out + base);
+    constexpr int NV = (int)(sizeof(TO) * SQ / 16);
+#pragma unroll
+    for (int q = 0; q < NV; q++) p[q] = reinterpret_cast<const int4 *>(o)[q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < SQ; q++) {
+      if (base + q < n) out[base + q] = (TO)e;
+      e += (int64_t)v[q];
+    }
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == ST - 1) out[n] = (TO)(b0 + total);
 }
 
 // workspace per recursion level (grow-only; one context stream at a time)

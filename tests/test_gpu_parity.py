@@ -244,3 +244,23 @@ def test_fused_spectral_probe_counts():
     g.probe("z_grad")
     g.run(2)
     assert g.probe(None)[1] == 2
+
+
+def test_pruned_lj_list_matches_full_list_bitwise():
+    # the step's pruned lj/cut list against the full NPairHalfBinNewton list,
+    # over several neighbour rebuilds; the lazily materialised full list
+    # equals the eager one
+    w = workload(n=32, nchains=12, nbeads=30, seed=13, zghost=1)
+    a = make_gpu(w)
+    b = make_gpu(w)
+    b.set_option(P.OPT_EAGER_HALFLIST, 1)
+    for _ in range(4):
+        a.run(10)
+        b.run(10)
+        assert np.array_equal(a.get_forces(), b.get_forces())
+        assert np.array_equal(a.get_atoms()[0], b.get_atoms()[0])
+    assert a.counts()["ncalls"] > 1
+    assert _lists_equal(a.halflist(), b.halflist())
+    a.run(5)
+    b.run(5)
+    assert np.array_equal(a.get_atoms()[0], b.get_atoms()[0])
