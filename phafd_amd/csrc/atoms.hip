@@ -433,19 +433,8 @@ __device__ __forceinline__ int bin1(double c, double lo, double hi, double inv, 
   return (int)((c - lo) * inv) - 1;
 }
 
-// The force side of the grid-atom loop, gathered per atom, plus the mu side
-// as emitted terms.  The reference adds to F(aj) once per listed grid point,
-// in ascending grid index.  Every grid point with rsq <= cutsq(type) is in
-// the atom's list (the list cut is cutsq + skin and no atom has moved more
-// than skin/2 since the build), so walking the in-box grid points around the
-// atom in (z, y, x) order gives the same force terms in the same order.
-// Each in-range pair also emits its mu term 2*delphi*expfac keyed by the
-// atom's position in the bin-sorted atom array.  A grid point's list walks
-// the stencil bins in (dz, dy, dx) lexicographic order
-// (src/nstencil_full_bin_3d.cpp:30-36) and each bin in its list order
-// (src/nbin_standard.cpp:181-229); bins are numbered z-major, so that order is
-// exactly ascending position in the bin-sorted array.  Summing each grid
-// point's terms in key order reproduces the reference's mu bit for bit.
+// walk box of an atom: the grid points within sqrt(cs) of it, clipped to the
+// grid (a hair wider than the sphere; the exact test decides)
 struct GABox {
   int i0, i1, j0, j1, k0, k1;
 };
@@ -480,12 +469,31 @@ __device__ __forceinline__ bool ga_row(const GridView &gv, const GABox &b, doubl
   return k0 <= k1;
 }
 
-__global__ void k_ga_count(int ng, GridView gv, GAParams ga, const double4 *gx, int *acnt) {
-  GRID_STRIDE(a, ng) {
-    const double4 x = gx[a];
+// Grid-atom pass (PairGridAtomGaussian / PairGridAtomLJish::compute,
+// src/pair_gridatom_gaussian.cpp:69-118, src/pair_gridatom_lj_ish.cpp:72-178).
+// The reference adds to F(aj) once per listed grid point, in ascending grid
+// index, and to mu(g) once per listed atom, in list order.  Every grid point
+// with rsq <= cutsq(type) is in the atom's list (the list cut is cutsq + skin
+// and no atom has moved more than skin/2 since the build), so walking the
+// in-box grid points around the atom in (z, y, x) order finds the same pairs
+// in the force's order.  A grid point's list walks the stencil bins in
+// (dz, dy, dx) lexicographic order (src/nstencil_full_bin_3d.cpp:30-36) and
+// each bin in its list order (src/nbin_standard.cpp:181-229); bins are
+// numbered z-major, so that order is ascending position q in the bin-sorted
+// atom array: the key mu terms are summed by.
+//
+// Pipeline: per atom (bin order) count then enumerate its in-range points;
+// per pair the expensive part (exp, field gathers, the mu term, the force
+// term, the pair's rank among its grid point's terms); per atom the force
+// fold in pair order; per pair the placement by grid point; per grid point
+// the sum in key order.  `order` is the bin-sorted atom array.
+__global__ void k_ga_count(int ng, GridView gv, GAParams ga, const double4 *gx, const int *order,
+                           const double *cutsq, int *acnt) {
+  GRID_STRIDE(q, ng) {
+    const double4 x = gx[order[q]];
     int n = 0;
     if (isfinite(x.x) && isfinite(x.y) && isfinite(x.z)) {
-      const double cs = ga.cutsq[(int)x.w];
+      const double cs = cutsq[(int)x.w];
       const GABox b = ga_box(gv, x, cs);
       for (int i = b.i0; i <= b.i1; i++) {
         const double delz = (i * gv.dz + gv.lo[2]) - x.z;
@@ -501,105 +509,141 @@ __global__ void k_ga_count(int ng, GridView gv, GAParams ga, const double4 *gx, 
         }
       }
     }
-    acnt[a] = n;
+    acnt[q] = n;
   }
 }
 
-__global__ void k_ga_force_emit(int ng, GridView gv, GAParams ga, const double4 *gx, const int *binpos,
-                                const int *aoff, int *pg, int *pkey, double *pv, double4 *gF, bool emit,
-                                double *energy) {
-  double esum = 0.0;
-  GRID_STRIDE(a, ng) {
-    const double4 x = gx[a];
-    if (!isfinite(x.x) || !isfinite(x.y) || !isfinite(x.z)) continue;   // never walk an unbounded range
-    const int t = (int)x.w;
-    const double cs = ga.cutsq[t];
+// in-range grid point and atom position of every pair, atom by atom
+__global__ void k_ga_enum(int ng, GridView gv, GAParams ga, const double4 *gx, const int *order, const int *aoff,
+                          int64_t cap, int *pg, int *pq, int *overflow) {
+  GRID_STRIDE(q, ng) {
+    const double4 x = gx[order[q]];
+    if (!isfinite(x.x) || !isfinite(x.y) || !isfinite(x.z)) continue;
+    const double cs = ga.cutsq[(int)x.w];
     const GABox b = ga_box(gv, x, cs);
-    const int key = emit ? binpos[a] : 0;
-    int w = emit ? aoff[a] : 0;
-    double4 F = gF[a];
+    int w = aoff[q];
+    if (aoff[q + 1] > cap) {
+      atomicOr(overflow, 1);
+      continue;
+    }
     for (int i = b.i0; i <= b.i1; i++) {
-      const double ztmp = i * gv.dz + gv.lo[2];
-      const double delz = ztmp - x.z;
+      const double delz = (i * gv.dz + gv.lo[2]) - x.z;
       for (int j = b.j0; j <= b.j1; j++) {
-        const double ytmp = j * gv.dy + gv.lo[1];
-        const double dely = ytmp - x.y;
+        const double dely = (j * gv.dy + gv.lo[1]) - x.y;
         int rk0, rk1;
         if (!ga_row(gv, b, x.x, cs, dely * dely + delz * delz, rk0, rk1)) continue;
         for (int k = rk0; k <= rk1; k++) {
-          const double xtmp = k * gv.dx + gv.lo[0];
-          const double delx = xtmp - x.x;
-          double rsq = delx * delx + dely * dely + delz * delz;
+          const double delx = (k * gv.dx + gv.lo[0]) - x.x;
+          const double rsq = delx * delx + dely * dely + delz * delz;
           if (!ga_in(ga, rsq, cs)) continue;
-          rsq /= cs;
-          double expfac;
-          if (ga.style == 0) expfac = (exp(-rsq / 2.0) - ga.em05) * ga.e05 * ga.eps[t];
-          else expfac = exp(-1. / (1 - rsq) + 1) * ga.eps[t];
-          const size_t p = ((size_t)i * gv.ny + j) * gv.nx + k;
-          const double delphi = (gv.phi[p] - ga.phit[t]);
-          if (emit) {   // mu term of PairGridAtom*::compute, summed later in list order
-            pg[w] = (int)p;
-            pkey[w] = key;
-            pv[w] = 2 * delphi * expfac;
-            w++;
-          }
-          if (energy) esum += delphi * delphi * expfac * gv.dv;
-          expfac *= 2 * delphi * gv.dv;
-          F.x -= gv.gx[p] * expfac;
-          F.y -= gv.gy[p] * expfac;
-          F.z -= gv.gz[p] * expfac;
+          pg[w] = (int)(((size_t)i * gv.ny + j) * gv.nx + k);
+          pq[w] = (int)q;
+          w++;
         }
       }
     }
-    gF[a] = F;
+  }
+}
+
+// one pair: the loop body of the reference's compute, with the same
+// arithmetic (rsq rebuilt from the grid indices exactly as the walk made it)
+__global__ void k_ga_pair(const int *npairs, GridView gv, GAParams ga, const double4 *gx, const int *order,
+                          const int *pg, const int *pq, double4 *ft, double *pv, int *ppos, int *gcnt, bool emit,
+                          double *energy) {
+  const int np = *npairs;
+  double esum = 0.0;
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < np; w += gridDim.x * blockDim.x) {
+    const int p = pg[w];
+    const double4 x = gx[order[pq[w]]];
+    const int t = (int)x.w;
+    const double cs = ga.cutsq[t];
+    const int k = p % gv.nx, j = (p / gv.nx) % gv.ny, i = p / (gv.nx * gv.ny);
+    const double ztmp = i * gv.dz + gv.lo[2];
+    const double delz = ztmp - x.z;
+    const double ytmp = j * gv.dy + gv.lo[1];
+    const double dely = ytmp - x.y;
+    const double xtmp = k * gv.dx + gv.lo[0];
+    const double delx = xtmp - x.x;
+    double rsq = delx * delx + dely * dely + delz * delz;
+    rsq /= cs;
+    double expfac;
+    if (ga.style == 0) expfac = (exp(-rsq / 2.0) - ga.em05) * ga.e05 * ga.eps[t];
+    else expfac = exp(-1. / (1 - rsq) + 1) * ga.eps[t];
+    const double delphi = (gv.phi[p] - ga.phit[t]);
+    if (emit) {   // mu term, summed later in list order
+      pv[w] = 2 * delphi * expfac;
+      ppos[w] = atomicAdd(&gcnt[p], 1);
+    }
+    if (energy) esum += delphi * delphi * expfac * gv.dv;
+    expfac *= 2 * delphi * gv.dv;
+    ft[w] = make_double4(gv.gx[p] * expfac, gv.gy[p] * expfac, gv.gz[p] * expfac, 0.0);
   }
   if (energy && esum != 0.0) atomicAdd(energy, esum);
 }
 
-// slot of every emitted term inside its grid point's group (arbitrary order,
-// restored by the key in k_ga_sum)
-__global__ void k_ga_rank(int npairs, const int *pg, int *gcnt, int *ppos) {
-  GRID_STRIDE(q, npairs) ppos[q] = atomicAdd(&gcnt[pg[q]], 1);
-}
-
-__global__ void k_binpos(int ng, const int *binatoms, int *binpos) {
-  GRID_STRIDE(q, ng) binpos[binatoms[q]] = (int)q;
-}
-
-__global__ void k_ga_place(int npairs, const int *pg, const int *ppos, const int *goff, const int *pkey,
-                           const double *pv, int *skey, double *sv) {
-  GRID_STRIDE(q, npairs) {
-    const int slot = goff[pg[q]] + ppos[q];
-    skey[slot] = pkey[q];
-    sv[slot] = pv[q];
+// F(a) -= its pair terms in ascending grid index, as the reference adds them
+__global__ void k_ga_fold(int ng, const int *order, const int *aoff, const double4 *ft, double4 *gF) {
+  GRID_STRIDE(q, ng) {
+    const int a = order[q];
+    const int s = aoff[q], e = aoff[q + 1];
+    if (s == e) continue;
+    double4 F = gF[a];
+    for (int w = s; w < e; w++) {
+      const double4 f = ft[w];
+      F.x -= f.x;
+      F.y -= f.y;
+      F.z -= f.z;
+    }
+    gF[a] = F;
   }
 }
 
-// mu(g) += its terms in ascending key (list) order; a grid point holds few
-// terms, so a selection over the group is cheap
+// every term to its grid point's segment, keyed by the emitting atom's q
+__global__ void k_ga_place(const int *npairs, const int *pg, const int *ppos, const int *pq, const double *pv,
+                           const int *goff, int *skey, double *sv) {
+  const int np = *npairs;
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < np; w += gridDim.x * blockDim.x) {
+    const int slot = goff[pg[w]] + ppos[w];
+    skey[slot] = pq[w];
+    sv[slot] = pv[w];
+  }
+}
+
+// mu(g) (+)= its terms in ascending key (list) order, four grid points per
+// item (a point holds few terms, so a selection over the group is cheap)
+__device__ __forceinline__ double ga_point_sum(double m, int s, int n, const int *skey, const double *sv) {
+  if (n == 1) return m + sv[s];
+  int last = -1;
+  for (int it = 0; it < n; it++) {
+    int best = 0x7fffffff, bi = s;
+    for (int q = s; q < s + n; q++) {
+      const int kq = skey[q];
+      if (kq > last && kq < best) { best = kq; bi = q; }
+    }
+    m += sv[bi];
+    last = best;
+  }
+  return m;
+}
 __global__ void k_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu, bool first) {
-  GRID_STRIDE(p, N) {
-    const int s = goff[p], e = goff[p + 1];
-    if (s == e) {
-      if (first) mu[p] = 0.0;
-      continue;
-    }
-    double m = first ? 0.0 : mu[p];
-    if (e - s == 1) {
-      m += sv[s];
+  const size_t N4 = N / 4;
+  for (size_t u = blockIdx.x * (size_t)blockDim.x + threadIdx.x; u < N4; u += (size_t)gridDim.x * blockDim.x) {
+    const size_t p0 = 4 * u;
+    const int4 o = *reinterpret_cast<const int4 *>(goff + p0);
+    const int o4 = goff[p0 + 4];
+    const int st[5] = {o.x, o.y, o.z, o.w, o4};
+    if (!first && o.x == o4) continue;
+    double m[4];
+    if (first) {
+      m[0] = m[1] = m[2] = m[3] = 0.0;
     } else {
-      int last = -1;
-      for (int it = s; it < e; it++) {
-        int best = 0x7fffffff, bi = s;
-        for (int q = s; q < e; q++) {
-          const int kq = skey[q];
-          if (kq > last && kq < best) { best = kq; bi = q; }
-        }
-        m += sv[bi];
-        last = best;
-      }
+      const double4 v = *reinterpret_cast<const double4 *>(mu + p0);
+      m[0] = v.x; m[1] = v.y; m[2] = v.z; m[3] = v.w;
     }
-    mu[p] = m;
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      if (st[e + 1] > st[e]) m[e] = ga_point_sum(m[e], st[e], st[e + 1] - st[e], skey, sv);
+    *reinterpret_cast<double4 *>(mu + p0) = make_double4(m[0], m[1], m[2], m[3]);
   }
 }
 
@@ -889,39 +933,40 @@ void launch_ga_deposit(const GridView &gv, const GAParams &ga, const int *cnt, c
   hipLaunchKernelGGL(k_ga_deposit, dim3(nblk(N)), dim3(TPB), 0, s, gv, ga, cnt, off, neigh, gx, nowned, energy);
   HIPCHK(hipGetLastError());
 }
-void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, int *acnt,
-                     hipStream_t s) {
+void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
+                     const double *cutsq, int *acnt, hipStream_t s) {
   if (ng <= 0) return;
-  hipLaunchKernelGGL(k_ga_count, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, acnt);
+  hipLaunchKernelGGL(k_ga_count, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, order, cutsq, acnt);
   HIPCHK(hipGetLastError());
 }
-void launch_ga_force_emit(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *binpos,
-                          const int *aoff, int *pg, int *pkey, double *pv, double4 *gF, bool emit, double *energy,
-                          hipStream_t s) {
+void launch_ga_enum(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
+                    const int *aoff, int64_t cap, int *pg, int *pq, int *overflow, hipStream_t s) {
   if (ng <= 0) return;
-  hipLaunchKernelGGL(k_ga_force_emit, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, binpos, aoff, pg, pkey, pv,
-                     gF, emit, energy);
+  hipLaunchKernelGGL(k_ga_enum, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, order, aoff, cap, pg, pq,
+                     overflow);
   HIPCHK(hipGetLastError());
 }
-void launch_ga_rank(int npairs, const int *pg, int *gcnt, int *ppos, hipStream_t s) {
-  if (npairs <= 0) return;
-  hipLaunchKernelGGL(k_ga_rank, dim3(nblk(npairs)), dim3(TPB), 0, s, npairs, pg, gcnt, ppos);
+void launch_ga_pair(int64_t cap, const int *npairs, const GridView &gv, const GAParams &ga, const double4 *gx,
+                    const int *order, const int *pg, const int *pq, double4 *ft, double *pv, int *ppos, int *gcnt,
+                    bool emit, double *energy, hipStream_t s) {
+  hipLaunchKernelGGL(k_ga_pair, dim3(nblk((size_t)cap)), dim3(TPB), 0, s, npairs, gv, ga, gx, order, pg, pq, ft, pv,
+                     ppos, gcnt, emit, energy);
   HIPCHK(hipGetLastError());
 }
-void launch_binpos(int ng, const int *binatoms, int *binpos, hipStream_t s) {
+void launch_ga_fold(int ng, const int *order, const int *aoff, const double4 *ft, double4 *gF, hipStream_t s) {
   if (ng <= 0) return;
-  hipLaunchKernelGGL(k_binpos, dim3(nblk(ng)), dim3(TPB), 0, s, ng, binatoms, binpos);
+  hipLaunchKernelGGL(k_ga_fold, dim3(nblk(ng)), dim3(TPB), 0, s, ng, order, aoff, ft, gF);
   HIPCHK(hipGetLastError());
 }
-void launch_ga_place(int npairs, const int *pg, const int *ppos, const int *goff, const int *pkey,
-                     const double *pv, int *skey, double *sv, hipStream_t s) {
-  if (npairs <= 0) return;
-  hipLaunchKernelGGL(k_ga_place, dim3(nblk(npairs)), dim3(TPB), 0, s, npairs, pg, ppos, goff, pkey, pv, skey, sv);
+void launch_ga_place(int64_t cap, const int *npairs, const int *pg, const int *ppos, const int *pq, const double *pv,
+                     const int *goff, int *skey, double *sv, hipStream_t s) {
+  hipLaunchKernelGGL(k_ga_place, dim3(nblk((size_t)cap)), dim3(TPB), 0, s, npairs, pg, ppos, pq, pv, goff, skey, sv);
   HIPCHK(hipGetLastError());
 }
 void launch_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu, bool first,
                    hipStream_t s) {
-  hipLaunchKernelGGL(k_ga_sum, dim3(nblk(N)), dim3(TPB), 0, s, N, goff, skey, sv, mu, first);
+  if (N % 4) throw std::invalid_argument("grid size must be a multiple of 4");
+  hipLaunchKernelGGL(k_ga_sum, dim3(nblk(N / 4)), dim3(TPB), 0, s, N, goff, skey, sv, mu, first);
   HIPCHK(hipGetLastError());
 }
 void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,

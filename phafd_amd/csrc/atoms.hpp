@@ -71,15 +71,19 @@ void launch_t_place(int nrows, const int *ilist, const int *off, const int *neig
 void launch_t_sort(int ng, const int *toff, int *tpair, int *ti, hipStream_t s);
 void launch_ga_deposit(const GridView &gv, const GAParams &ga, const int *cnt, const int64_t *off,
                        const int *neigh, const double4 *gx, int nowned, double *energy, hipStream_t s);
-void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, int *acnt,
-                     hipStream_t s);
-void launch_ga_force_emit(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *binpos,
-                          const int *aoff, int *pg, int *pkey, double *pv, double4 *gF, bool emit, double *energy,
-                          hipStream_t s);
-void launch_ga_rank(int npairs, const int *pg, int *gcnt, int *ppos, hipStream_t s);
-void launch_binpos(int ng, const int *binatoms, int *binpos, hipStream_t s);
-void launch_ga_place(int npairs, const int *pg, const int *ppos, const int *goff, const int *pkey,
-                     const double *pv, int *skey, double *sv, hipStream_t s);
+// grid-atom pass; order = bin-sorted atom array, npairs = device pointer to
+// the number of pairs (the last entry of the scanned counts), cap = capacity
+// of the pair arrays
+void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
+                     const double *cutsq, int *acnt, hipStream_t s);
+void launch_ga_enum(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
+                    const int *aoff, int64_t cap, int *pg, int *pq, int *overflow, hipStream_t s);
+void launch_ga_pair(int64_t cap, const int *npairs, const GridView &gv, const GAParams &ga, const double4 *gx,
+                    const int *order, const int *pg, const int *pq, double4 *ft, double *pv, int *ppos, int *gcnt,
+                    bool emit, double *energy, hipStream_t s);
+void launch_ga_fold(int ng, const int *order, const int *aoff, const double4 *ft, double4 *gF, hipStream_t s);
+void launch_ga_place(int64_t cap, const int *npairs, const int *pg, const int *ppos, const int *pq, const double *pv,
+                     const int *goff, int *skey, double *sv, hipStream_t s);
 // first: mu is written (terms summed from 0) instead of added to
 void launch_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu, bool first,
                    hipStream_t s);

@@ -24,7 +24,9 @@ struct PairDef {
   int style;   // 0 gaussian, 1 LJish, 2 lj/cut
   double maxcut = 0;
   std::vector<double> eps, phit, cutsq;
-  DBuf<double> d_eps, d_phit, d_cutsq;
+  DBuf<double> d_eps, d_phit, d_cutsq, d_cutsq_bound;
+  int64_t cap_build = -1;   // ncalls of the build the pair capacity was sized at
+  int64_t cap = 0;
   double lj1 = 0, lj2 = 0, ljcutsq = 0;
 };
 
@@ -96,8 +98,11 @@ struct phafd_hip_ctx {
   DBuf<double4> gx_build;             // gathered positions at the last build
   bool grid_built = false;            // grid list materialised for the last build
   bool eager_gridlist = false;        // build (and use) the stored grid list in the step
-  DBuf<int> ga_acnt, ga_aoff, ga_pg, ga_ppos, ga_gcnt, ga_goff;   // grid-atom emitted terms
-  DBuf<int> ga_pkey, ga_skey, binpos;
+  // grid-atom pairs: per-atom counts and offsets (bin order), per pair the
+  // grid point, atom position, rank, force term and mu term; per grid point
+  // counts and offsets; terms sorted by grid point (key, value)
+  DBuf<int> ga_acnt, ga_aoff, ga_pg, ga_pq, ga_ppos, ga_gcnt, ga_goff, ga_skey;
+  DBuf<double4> ga_ft;
   DBuf<double> ga_pv, ga_sv;
   DBuf<int> a2b, bincount, binstart, bincursor, binatoms;
   DBuf<int> g_cnt, g_neigh;
@@ -409,8 +414,6 @@ void bin_atoms(phafd_hip_ctx *c) {
   scan_exclusive_i32(c->bincount.p, c->binstart.p, (size_t)nb, c->stream);
   launch_bin_place(ng, c->a2b.p, c->binstart.p, c->bincursor.p, c->binatoms.p, c->stream);
   launch_bin_sort(nb, c->binstart.p, c->binatoms.p, c->glabel.p, c->stream);
-  c->binpos.alloc((size_t)ng + 1);
-  launch_binpos(ng, c->binatoms.p, c->binpos.p, c->stream);
 }
 
 // NPairGridBin::build (src/npair_grid_bin.cpp:22-127)
@@ -509,12 +512,20 @@ bool neighbor_decide(phafd_hip_ctx *c) {
   return (f & 1) != 0;
 }
 
-void upload_pair(PairDef &p) {
+void upload_pair(PairDef &p, double skin) {
   int nt = (int)p.eps.size();
-  p.d_eps.alloc(nt); p.d_phit.alloc(nt); p.d_cutsq.alloc(nt);
+  p.d_eps.alloc(nt); p.d_phit.alloc(nt); p.d_cutsq.alloc(nt); p.d_cutsq_bound.alloc(nt);
   HIPCHK(hipMemcpy(p.d_eps.p, p.eps.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p.d_phit.p, p.phit.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p.d_cutsq.p, p.cutsq.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
+  // (cut + skin/2 + margin)^2: holds every pair any step before the next
+  // build can have (no atom moves more than skin/2 in between)
+  std::vector<double> b(nt);
+  for (int t = 0; t < nt; t++) {
+    const double r = std::sqrt(p.cutsq[t]) + 0.5 * skin;
+    b[t] = r * r * (1 + 1e-9) + 1e-12;
+  }
+  HIPCHK(hipMemcpy(p.d_cutsq_bound.p, b.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
 }
 
 GAParams ga_params(const PairDef &p) {
@@ -597,38 +608,46 @@ void grid_atom_pass(phafd_hip_ctx *c, PairDef &p, double *energy, bool first) {
   GAParams ga = ga_params(p);
   const int ng = c->ngathered;
   const bool emit = ga_uses_emit(c);
-  if (first && !emit) fill_zero(c->mu.p, sizeof(double) * c->N, c->stream);
-  int npairs = 0;
-  if (emit) {
-    c->ga_acnt.alloc((size_t)ng + 1);
-    c->ga_aoff.alloc((size_t)ng + 2);
-    launch_ga_count(ng, gv, ga, c->gx.p, c->ga_acnt.p, c->stream);
+  const int *order = c->binatoms.p;
+  c->ga_acnt.alloc((size_t)ng + 1);
+  c->ga_aoff.alloc((size_t)ng + 2);
+  if (p.cap_build != c->ncalls) {
+    // pair capacity for the steps until the next build: the pairs within
+    // cut + skin/2 of the build positions, counted once (one read-back per build)
+    launch_ga_count(ng, gv, ga, c->gx.p, order, p.d_cutsq_bound.p, c->ga_acnt.p, c->stream);
     scan_exclusive_i32(c->ga_acnt.p, c->ga_aoff.p, (size_t)ng, c->stream);
-    npairs = read_int(c, c->ga_aoff.p + ng);
-    c->ga_pg.alloc((size_t)npairs + 1); c->ga_ppos.alloc((size_t)npairs + 1);
-    c->ga_pkey.alloc((size_t)npairs + 1); c->ga_pv.alloc((size_t)npairs + 1);
-    c->ga_skey.alloc((size_t)npairs + 1); c->ga_sv.alloc((size_t)npairs + 1);
+    p.cap = std::max(1, read_int(c, c->ga_aoff.p + ng));
+    p.cap_build = c->ncalls;
+  }
+  const int64_t cap = p.cap;
+  c->ga_pg.alloc(cap); c->ga_pq.alloc(cap); c->ga_ft.alloc(cap);
+  probe_begin(K_GA_FORCE, c->stream);
+  launch_ga_count(ng, gv, ga, c->gx.p, order, ga.cutsq, c->ga_acnt.p, c->stream);
+  scan_exclusive_i32(c->ga_acnt.p, c->ga_aoff.p, (size_t)ng, c->stream);
+  const int *npairs = c->ga_aoff.p + ng;
+  launch_ga_enum(ng, gv, ga, c->gx.p, order, c->ga_aoff.p, cap, c->ga_pg.p, c->ga_pq.p, c->flags.p + 3, c->stream);
+  if (emit) {
+    c->ga_ppos.alloc(cap); c->ga_pv.alloc(cap); c->ga_skey.alloc(cap); c->ga_sv.alloc(cap);
     c->ga_gcnt.alloc(c->N + 1);
     c->ga_goff.alloc(c->N + 2);
     fill_zero(c->ga_gcnt.p, sizeof(int) * c->N, c->stream);
   } else {
-    probe_begin(K_GA_DEPOSIT, c->stream);
-    launch_ga_deposit(gv, ga, c->g_cnt.p, c->g_off.p, c->g_neigh.p, c->gx.p, c->nowned, nullptr, c->stream);
-    probe_end(K_GA_DEPOSIT, c->stream, 28.0 * c->N + 4.0 * c->g_nent);
+    if (first) fill_zero(c->mu.p, sizeof(double) * c->N, c->stream);
   }
-  probe_begin(K_GA_FORCE, c->stream);
-  launch_ga_force_emit(ng, gv, ga, c->gx.p, c->binpos.p, c->ga_aoff.p, c->ga_pg.p, c->ga_pkey.p, c->ga_pv.p,
-                       c->gF.p, emit, energy, c->stream);
-  probe_end(K_GA_FORCE, c->stream, 64.0 * ng + 16.0 * npairs);
+  launch_ga_pair(cap, npairs, gv, ga, c->gx.p, order, c->ga_pg.p, c->ga_pq.p, c->ga_ft.p, c->ga_pv.p, c->ga_ppos.p,
+                 c->ga_gcnt.p, emit, energy, c->stream);
+  launch_ga_fold(ng, order, c->ga_aoff.p, c->ga_ft.p, c->gF.p, c->stream);
+  probe_end(K_GA_FORCE, c->stream, 64.0 * ng);
+  probe_begin(K_GA_DEPOSIT, c->stream);
   if (emit) {
-    probe_begin(K_GA_DEPOSIT, c->stream);
-    launch_ga_rank(npairs, c->ga_pg.p, c->ga_gcnt.p, c->ga_ppos.p, c->stream);
     scan_exclusive_i32(c->ga_gcnt.p, c->ga_goff.p, c->N, c->stream);
-    launch_ga_place(npairs, c->ga_pg.p, c->ga_ppos.p, c->ga_goff.p, c->ga_pkey.p, c->ga_pv.p, c->ga_skey.p,
+    launch_ga_place(cap, npairs, c->ga_pg.p, c->ga_ppos.p, c->ga_pq.p, c->ga_pv.p, c->ga_goff.p, c->ga_skey.p,
                     c->ga_sv.p, c->stream);
     launch_ga_sum(c->N, c->ga_goff.p, c->ga_skey.p, c->ga_sv.p, c->mu.p, first, c->stream);
-    probe_end(K_GA_DEPOSIT, c->stream, 16.0 * c->N + 40.0 * npairs);
+  } else {
+    launch_ga_deposit(gv, ga, c->g_cnt.p, c->g_off.p, c->g_neigh.p, c->gx.p, c->nowned, nullptr, c->stream);
   }
+  probe_end(K_GA_DEPOSIT, c->stream, 16.0 * c->N);
 }
 
 // Integrate::run pair block, src/integrate.cpp:130-146 (one pass, finding F4)
@@ -798,8 +817,8 @@ void run_steps(phafd_hip_ctx *c, int nsteps) {
     if (c->neigh_set) {
       launch_fill_i32(1, c->flags.p, 0, c->stream);
       launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->flags.p, c->stream);
-      HIPCHK(hipMemcpyAsync(c->h_flags, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     }
+    HIPCHK(hipMemcpyAsync(c->h_flags + 8, c->flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->phi.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipEventRecord(c->ev[8], c->stream));
     sync(c);
@@ -815,9 +834,10 @@ void run_steps(phafd_hip_ctx *c, int nsteps) {
     double phi0;
     std::memcpy(&phi0, c->h_flags + 2, sizeof(double));
     if (std::isnan(phi0)) throw std::runtime_error("NAN encountered in phi.");
+    if (c->h_flags[11]) throw std::runtime_error("grid-atom term capacity exceeded");
     if (c->neigh_set) {
-      if (c->h_flags[0] & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
-      c->pending_decision = c->h_flags[0] & 1;
+      if (c->h_flags[8] & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
+      c->pending_decision = c->h_flags[8] & 1;
     }
   }
 }
@@ -1164,7 +1184,7 @@ int phafd_hip_neighbor(phafd_hip_ctx *c, double skin) {
     setup_bins(c);
     create_stencils(c);
     comm_setup(c);
-    for (auto &p : c->pairs) if (p->style != 2) upload_pair(*p);
+    for (auto &p : c->pairs) if (p->style != 2) upload_pair(*p, c->skin);
     c->neigh_set = true;
     return 0;
   });
@@ -1174,7 +1194,7 @@ int phafd_hip_setup(phafd_hip_ctx *c, double dt) {
   return guard(c, [&] {
     if (dt <= 0.0) throw std::runtime_error("Cannot run simulation without setting dt > 0.");
     c->dt = dt;
-    for (auto &p : c->pairs) if (p->style != 2) upload_pair(*p);
+    for (auto &p : c->pairs) if (p->style != 2) upload_pair(*p, c->skin);
     if (c->neigh_set && c->nowned > 0) {
       launch_pbc(c->nowned, c->x.p, c->ux.p, c->image.p, c->box(), c->stream);
       borders(c);

@@ -264,3 +264,25 @@ def test_pruned_lj_list_matches_full_list_bitwise():
     a.run(5)
     b.run(5)
     assert np.array_equal(a.get_atoms()[0], b.get_atoms()[0])
+
+
+@pytest.mark.parametrize("ljish", [False, True])
+def test_dense_tile_mu_terms(ljish):
+    # 400 atoms packed into a 5^3 corner (across the periodic boundary) with a
+    # long grid-atom cut: some grid tiles hold more mu terms than the ordered
+    # sum keeps in LDS and take the in-memory selection instead
+    w = workload(n=32, nchains=40, nbeads=10, seed=21, lj=False, chain=False, gauss_cut=3.0,
+                 nucwidth=3.0, ljish=ljish, zghost=1)
+    rng = np.random.default_rng(5)
+    w["xyz"] = np.mod(rng.uniform(-2.5, 2.5, size=w["xyz"].shape), 32.0)
+    a = make_gpu(w)
+    b = make_gpu(w)
+    b.set_option(P.OPT_EAGER_GRIDLIST, 1)
+    o = make_oracle(w)
+    for s in (a, b, o):
+        s.run(2)
+    assert np.array_equal(a.get_field(P.MU), b.get_field(P.MU))
+    assert np.array_equal(a.get_forces(), b.get_forces())
+    assert rel_l2(a.get_field(P.MU), o.get_field(P.MU)) < 1e-10
+    assert rel_l2(a.get_forces(), o.get_forces()) < 1e-10
+    assert rel_l2(a.get_field(P.PHI), o.get_field(P.PHI)) < 1e-10
