@@ -87,6 +87,21 @@ def cpu_baseline(wl, phi, xyz, mol, steps):
     return dt
 
 
+def copy_calibration(nbytes, reps):
+    """A plain device-to-device copy of `nbytes`, `reps` times: the reference
+    kernel whose FETCH_SIZE / WRITE_SIZE counters scale the ones of the FFT
+    kernels into bytes."""
+    import torch
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "reps": reps}), flush=True)
+
+
 def read_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (profiles/<kernel>_<workload>_pmc.md), or None."""
@@ -105,7 +120,9 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
-    ap.add_argument("--probe", default="z_inv", help="kernel timed with HIP events for the roofline")
+    ap.add_argument("--probe", default="c2r_x", help="kernel timed with HIP events for the roofline")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (counter calibration) and exit")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timers", action="store_true", help="print per-hook GPU times to stderr")
@@ -114,6 +131,10 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU decomposition is a design, not built", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, 5)
+        return
 
     import phafd_amd as P
     wl = WORKLOADS[args.workload]

@@ -23,6 +23,18 @@
  *                          zero force for constant phi)
  *   lj/cut ............... two-body analytic force, Newton's third law
  *   chain (build-defined). finite-difference gradient of its own energy
+ *
+ * Parity unpinned (nothing in the reference tree or in mathematics fixes it):
+ *   - the semiflexible chain dynamics: the reference delegates them to the
+ *     external beadrodpmer library (no source, no version, no test in the
+ *     tree); the chain fix here is a build-defined harmonic bond + bending
+ *     model, pinned only by its own energy's finite-difference gradient;
+ *   - FFTW's c2r output on non-Hermitian input at the kx = 0 and kx = Nx/2
+ *     bins: the standard half-complex synthesis (only Re() of those bins
+ *     enters) is assumed, as numpy's irfftn does;
+ *   - the fftw_arr container layout (padding, reverseFlat): inferred from its
+ *     call sites, src/npair_grid_bin.cpp:71-116 and
+ *     src/pair_gridatom_gaussian.cpp:71-77.
  */
 #include "oracle.h"
 
