@@ -125,6 +125,7 @@ def main():
                     help="only run a plain device copy of BYTES (counter calibration) and exit")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form leg")
     ap.add_argument("--timers", action="store_true", help="print per-hook GPU times to stderr")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -151,9 +152,28 @@ def main():
     pms, plaunch, pbytes = s.probe(None)
     timers = s.timers()
     counts = s.counts()
+    phi64 = s.get_field(P.PHI)
     s.close()
 
     N = n ** 3
+    fp32 = None
+    if not args.no_fp32:
+        # the same workload, steps and initial state in the FP32 form; its phi
+        # error is taken against the FP64 run above
+        s32 = P.System(n, n, n, [0, 0, 0], [wl["box"]] * 3, zghost=wl["zghost"], precision=P.FP32)
+        configure(s32, wl, phi, xyz, mol)
+        s32.run(args.warmup)
+        t1 = time.perf_counter()
+        s32.run(args.steps)
+        el32 = time.perf_counter() - t1
+        phi32 = s32.get_field(P.PHI)
+        s32.close()
+        err = float(np.linalg.norm(phi32 - phi64) / np.linalg.norm(phi64))
+        fp32 = {"value": N * args.steps / el32, "unit": "grid-pt·steps/s", "ms_per_step": 1e3 * el32 / args.steps,
+                "speedup_vs_fp64": elapsed / el32, "phi_rel_l2_vs_fp64": err,
+                "steps_compared": args.warmup + args.steps,
+                "form": "FP32 FFT + fields, Kahan-compensated grid-atom sums, FP64 k=0 mode, atoms and energies"}
+
     nbeads = wl["nchains"] * wl["nbeads"]
     ms_per_step = 1e3 * elapsed / args.steps
     peak = 8.0e3   # GB/s, MI355X HBM3E
@@ -182,6 +202,8 @@ def main():
                      "traffic": traffic, "avg_launch_ms": avg_ms, "launches": plaunch,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
+    if fp32 is not None:
+        out["fp32"] = fp32
     if args.timers:
         print(json.dumps({"per_hook_ms_total": timers}), file=sys.stderr)
     if not args.no_cpu_baseline:

@@ -545,11 +545,20 @@ __global__ void k_ga_enum(int ng, GridView gv, GAParams ga, const double4 *gx, c
   }
 }
 
+__device__ __forceinline__ float fexp(float x) { return expf(x); }
+__device__ __forceinline__ double fexp(double x) { return exp(x); }
+template <class R> struct Vec4;
+template <> struct Vec4<double> { using T = double4; };
+template <> struct Vec4<float> { using T = float4; };
+
 // one pair: the loop body of the reference's compute, with the same
-// arithmetic (rsq rebuilt from the grid indices exactly as the walk made it)
-__global__ void k_ga_pair(const int *npairs, GridView gv, GAParams ga, const double4 *gx, const int *order,
-                          const int *pg, const int *pq, double4 *ft, double *pv, int *ppos, int *gcnt, bool emit,
-                          double *energy) {
+// arithmetic (rsq rebuilt from the grid indices exactly as the walk made it).
+// FP32 form: rsq in FP64 from the FP64 positions, everything after it in
+// FP32; the energy term in FP64.
+template <class R>
+__global__ void k_ga_pair(const int *npairs, GridViewT<R> gv, GAParams ga, const double4 *gx, const int *order,
+                          const int *pg, const int *pq, typename Vec4<R>::T *ft, R *pv, int *ppos, int *gcnt,
+                          bool emit, double *energy) {
   const int np = *npairs;
   double esum = 0.0;
   for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < np; w += gridDim.x * blockDim.x) {
@@ -566,22 +575,41 @@ __global__ void k_ga_pair(const int *npairs, GridView gv, GAParams ga, const dou
     const double delx = xtmp - x.x;
     double rsq = delx * delx + dely * dely + delz * delz;
     rsq /= cs;
-    double expfac;
-    if (ga.style == 0) expfac = (exp(-rsq / 2.0) - ga.em05) * ga.e05 * ga.eps[t];
-    else expfac = exp(-1. / (1 - rsq) + 1) * ga.eps[t];
-    const double delphi = (gv.phi[p] - ga.phit[t]);
+    const R r = (R)rsq;
+    R expfac;
+    if (ga.style == 0) expfac = (fexp(-r / R(2)) - (R)ga.em05) * (R)ga.e05 * (R)ga.eps[t];
+    else expfac = fexp(R(-1) / (1 - r) + 1) * (R)ga.eps[t];
+    const R delphi = (gv.phi[p] - (R)ga.phit[t]);
     if (emit) {   // mu term, summed later in list order
       pv[w] = 2 * delphi * expfac;
       ppos[w] = atomicAdd(&gcnt[p], 1);
     }
-    if (energy) esum += delphi * delphi * expfac * gv.dv;
-    expfac *= 2 * delphi * gv.dv;
-    ft[w] = make_double4(gv.gx[p] * expfac, gv.gy[p] * expfac, gv.gz[p] * expfac, 0.0);
+    if (energy) esum += (double)delphi * (double)delphi * (double)expfac * gv.dv;
+    expfac *= 2 * delphi * (R)gv.dv;
+    typename Vec4<R>::T f;
+    f.x = gv.gx[p] * expfac;
+    f.y = gv.gy[p] * expfac;
+    f.z = gv.gz[p] * expfac;
+    f.w = R(0);
+    ft[w] = f;
   }
   if (energy && esum != 0.0) atomicAdd(energy, esum);
 }
 
+// Kahan-compensated running sum (FP32 form)
+struct Kahan {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
 // F(a) -= its pair terms in ascending grid index, as the reference adds them
+// (FP64); FP32 form: the terms summed with Kahan compensation, then taken
+// from F in FP64
 __global__ void k_ga_fold(int ng, const int *order, const int *aoff, const double4 *ft, double4 *gF) {
   GRID_STRIDE(q, ng) {
     const int a = order[q];
@@ -597,10 +625,30 @@ __global__ void k_ga_fold(int ng, const int *order, const int *aoff, const doubl
     gF[a] = F;
   }
 }
+__global__ void k_ga_fold(int ng, const int *order, const int *aoff, const float4 *ft, double4 *gF) {
+  GRID_STRIDE(q, ng) {
+    const int a = order[q];
+    const int s = aoff[q], e = aoff[q + 1];
+    if (s == e) continue;
+    Kahan kx, ky, kz;
+    for (int w = s; w < e; w++) {
+      const float4 f = ft[w];
+      kx.add(f.x);
+      ky.add(f.y);
+      kz.add(f.z);
+    }
+    double4 F = gF[a];
+    F.x -= (double)kx.s;
+    F.y -= (double)ky.s;
+    F.z -= (double)kz.s;
+    gF[a] = F;
+  }
+}
 
 // every term to its grid point's segment, keyed by the emitting atom's q
-__global__ void k_ga_place(const int *npairs, const int *pg, const int *ppos, const int *pq, const double *pv,
-                           const int *goff, int *skey, double *sv) {
+template <class R>
+__global__ void k_ga_place(const int *npairs, const int *pg, const int *ppos, const int *pq, const R *pv,
+                           const int *goff, int *skey, R *sv) {
   const int np = *npairs;
   for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < np; w += gridDim.x * blockDim.x) {
     const int slot = goff[pg[w]] + ppos[w];
@@ -610,9 +658,33 @@ __global__ void k_ga_place(const int *npairs, const int *pg, const int *ppos, co
 }
 
 // mu(g) (+)= its terms in ascending key (list) order, four grid points per
-// item (a point holds few terms, so a selection over the group is cheap)
-__device__ __forceinline__ double ga_point_sum(double m, int s, int n, const int *skey, const double *sv) {
-  if (n == 1) return m + sv[s];
+// item (a point holds few terms, so a selection over the group is cheap).
+// FP32 form: Kahan-compensated.
+template <class R>
+struct MuAcc {   // FP64: plain running sum in order
+  R m;
+  __device__ __forceinline__ void add(R v) { m += v; }
+  __device__ __forceinline__ R get() const { return m; }
+};
+template <>
+struct MuAcc<float> {
+  float m, c = 0.f;
+  __device__ __forceinline__ void add(float v) {
+    const float y = v - c;
+    const float t = m + y;
+    c = (t - m) - y;
+    m = t;
+  }
+  __device__ __forceinline__ float get() const { return m; }
+};
+template <class R>
+__device__ __forceinline__ R ga_point_sum(R m0, int s, int n, const int *skey, const R *sv) {
+  MuAcc<R> acc;
+  acc.m = m0;
+  if (n == 1) {
+    acc.add(sv[s]);
+    return acc.get();
+  }
   int last = -1;
   for (int it = 0; it < n; it++) {
     int best = 0x7fffffff, bi = s;
@@ -620,12 +692,13 @@ __device__ __forceinline__ double ga_point_sum(double m, int s, int n, const int
       const int kq = skey[q];
       if (kq > last && kq < best) { best = kq; bi = q; }
     }
-    m += sv[bi];
+    acc.add(sv[bi]);
     last = best;
   }
-  return m;
+  return acc.get();
 }
-__global__ void k_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu, bool first) {
+template <class R>
+__global__ void k_ga_sum(size_t N, const int *goff, const int *skey, const R *sv, R *mu, bool first) {
   const size_t N4 = N / 4;
   for (size_t u = blockIdx.x * (size_t)blockDim.x + threadIdx.x; u < N4; u += (size_t)gridDim.x * blockDim.x) {
     const size_t p0 = 4 * u;
@@ -633,17 +706,14 @@ __global__ void k_ga_sum(size_t N, const int *goff, const int *skey, const doubl
     const int o4 = goff[p0 + 4];
     const int st[5] = {o.x, o.y, o.z, o.w, o4};
     if (!first && o.x == o4) continue;
-    double m[4];
-    if (first) {
-      m[0] = m[1] = m[2] = m[3] = 0.0;
-    } else {
-      const double4 v = *reinterpret_cast<const double4 *>(mu + p0);
-      m[0] = v.x; m[1] = v.y; m[2] = v.z; m[3] = v.w;
-    }
+    R m[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) m[e] = first ? R(0) : mu[p0 + e];
 #pragma unroll
     for (int e = 0; e < 4; e++)
       if (st[e + 1] > st[e]) m[e] = ga_point_sum(m[e], st[e], st[e + 1] - st[e], skey, sv);
-    *reinterpret_cast<double4 *>(mu + p0) = make_double4(m[0], m[1], m[2], m[3]);
+#pragma unroll
+    for (int e = 0; e < 4; e++) mu[p0 + e] = m[e];
   }
 }
 
@@ -946,29 +1016,35 @@ void launch_ga_enum(int ng, const GridView &gv, const GAParams &ga, const double
                      overflow);
   HIPCHK(hipGetLastError());
 }
-void launch_ga_pair(int64_t cap, const int *npairs, const GridView &gv, const GAParams &ga, const double4 *gx,
-                    const int *order, const int *pg, const int *pq, double4 *ft, double *pv, int *ppos, int *gcnt,
-                    bool emit, double *energy, hipStream_t s) {
-  hipLaunchKernelGGL(k_ga_pair, dim3(nblk((size_t)cap)), dim3(TPB), 0, s, npairs, gv, ga, gx, order, pg, pq, ft, pv,
-                     ppos, gcnt, emit, energy);
-  HIPCHK(hipGetLastError());
-}
-void launch_ga_fold(int ng, const int *order, const int *aoff, const double4 *ft, double4 *gF, hipStream_t s) {
-  if (ng <= 0) return;
-  hipLaunchKernelGGL(k_ga_fold, dim3(nblk(ng)), dim3(TPB), 0, s, ng, order, aoff, ft, gF);
-  HIPCHK(hipGetLastError());
-}
-void launch_ga_place(int64_t cap, const int *npairs, const int *pg, const int *ppos, const int *pq, const double *pv,
-                     const int *goff, int *skey, double *sv, hipStream_t s) {
-  hipLaunchKernelGGL(k_ga_place, dim3(nblk((size_t)cap)), dim3(TPB), 0, s, npairs, pg, ppos, pq, pv, goff, skey, sv);
-  HIPCHK(hipGetLastError());
-}
-void launch_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu, bool first,
-                   hipStream_t s) {
-  if (N % 4) throw std::invalid_argument("grid size must be a multiple of 4");
-  hipLaunchKernelGGL(k_ga_sum, dim3(nblk(N / 4)), dim3(TPB), 0, s, N, goff, skey, sv, mu, first);
-  HIPCHK(hipGetLastError());
-}
+#define PHAFD_GA_DEF(RT, V4)                                                                                    \
+  void launch_ga_pair(int64_t cap, const int *npairs, const GridViewT<RT> &gv, const GAParams &ga,             \
+                      const double4 *gx, const int *order, const int *pg, const int *pq, V4 *ft, RT *pv,      \
+                      int *ppos, int *gcnt, bool emit, double *energy, hipStream_t s) {                       \
+    hipLaunchKernelGGL(k_ga_pair<RT>, dim3(nblk((size_t)cap)), dim3(TPB), 0, s, npairs, gv, ga, gx, order, pg, pq, \
+                       ft, pv, ppos, gcnt, emit, energy);                                                    \
+    HIPCHK(hipGetLastError());                                                                                \
+  }                                                                                                           \
+  void launch_ga_fold(int ng, const int *order, const int *aoff, const V4 *ft, double4 *gF, hipStream_t s) {  \
+    if (ng <= 0) return;                                                                                      \
+    const V4 *f = ft;                                                                                         \
+    hipLaunchKernelGGL(k_ga_fold, dim3(nblk(ng)), dim3(TPB), 0, s, ng, order, aoff, f, gF);                   \
+    HIPCHK(hipGetLastError());                                                                                \
+  }                                                                                                           \
+  void launch_ga_place(int64_t cap, const int *npairs, const int *pg, const int *ppos, const int *pq,         \
+                       const RT *pv, const int *goff, int *skey, RT *sv, hipStream_t s) {                     \
+    hipLaunchKernelGGL(k_ga_place<RT>, dim3(nblk((size_t)cap)), dim3(TPB), 0, s, npairs, pg, ppos, pq, pv, goff, \
+                       skey, sv);                                                                            \
+    HIPCHK(hipGetLastError());                                                                                \
+  }                                                                                                           \
+  void launch_ga_sum(size_t N, const int *goff, const int *skey, const RT *sv, RT *mu, bool first,           \
+                     hipStream_t s) {                                                                         \
+    if (N % 4) throw std::invalid_argument("grid size must be a multiple of 4");                              \
+    hipLaunchKernelGGL(k_ga_sum<RT>, dim3(nblk(N / 4)), dim3(TPB), 0, s, N, goff, skey, sv, mu, first);      \
+    HIPCHK(hipGetLastError());                                                                                \
+  }
+PHAFD_GA_DEF(double, double4)
+PHAFD_GA_DEF(float, float4)
+#undef PHAFD_GA_DEF
 void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,
                      const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
                      double4 *gF, double *energy, hipStream_t s) {

@@ -16,14 +16,17 @@ struct GatheredView {
   int *label;       // 1 LOCAL, 2 GHOST
 };
 
-struct GridView {
+// grid geometry (always FP64) and the fields of one precision
+template <class R>
+struct GridViewT {
   int nx, ny, nz;
   double dx, dy, dz, dv;
   double lo[3];     // sublo (grid point (0,0,0))
-  const double *phi;
-  double *mu;
-  const double *gx, *gy, *gz;
+  const R *phi;
+  R *mu;
+  const R *gx, *gy, *gz;
 };
+using GridView = GridViewT<double>;
 
 struct GAParams {
   int style;        // 0 gaussian, 1 LJish
@@ -78,15 +81,21 @@ void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const doubl
                      const double *cutsq, int *acnt, hipStream_t s);
 void launch_ga_enum(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
                     const int *aoff, int64_t cap, int *pg, int *pq, int *overflow, hipStream_t s);
-void launch_ga_pair(int64_t cap, const int *npairs, const GridView &gv, const GAParams &ga, const double4 *gx,
-                    const int *order, const int *pg, const int *pq, double4 *ft, double *pv, int *ppos, int *gcnt,
-                    bool emit, double *energy, hipStream_t s);
-void launch_ga_fold(int ng, const int *order, const int *aoff, const double4 *ft, double4 *gF, hipStream_t s);
-void launch_ga_place(int64_t cap, const int *npairs, const int *pg, const int *ppos, const int *pq, const double *pv,
-                     const int *goff, int *skey, double *sv, hipStream_t s);
+// per pair terms; the FP32 form (float fields, float4 terms) folds forces and
+// sums mu with Kahan compensation, energies stay FP64.  launch_ga_sum with
 // first: mu is written (terms summed from 0) instead of added to
-void launch_ga_sum(size_t N, const int *goff, const int *skey, const double *sv, double *mu, bool first,
-                   hipStream_t s);
+#define PHAFD_GA_DECL(RT, V4)                                                                                   \
+  void launch_ga_pair(int64_t cap, const int *npairs, const GridViewT<RT> &gv, const GAParams &ga,             \
+                      const double4 *gx, const int *order, const int *pg, const int *pq, V4 *ft, RT *pv,      \
+                      int *ppos, int *gcnt, bool emit, double *energy, hipStream_t s);                        \
+  void launch_ga_fold(int ng, const int *order, const int *aoff, const V4 *ft, double4 *gF, hipStream_t s);   \
+  void launch_ga_place(int64_t cap, const int *npairs, const int *pg, const int *ppos, const int *pq,         \
+                       const RT *pv, const int *goff, int *skey, RT *sv, hipStream_t s);                      \
+  void launch_ga_sum(size_t N, const int *goff, const int *skey, const RT *sv, RT *mu, bool first,           \
+                     hipStream_t s);
+PHAFD_GA_DECL(double, double4)
+PHAFD_GA_DECL(float, float4)
+#undef PHAFD_GA_DECL
 void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,
                      const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
                      double4 *gF, double *energy, hipStream_t s);

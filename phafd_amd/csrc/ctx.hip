@@ -51,6 +51,23 @@ struct SwapDev {
 
 }  // namespace
 
+template <class R> struct Vec2;
+template <> struct Vec2<double> { using T = double2; };
+template <> struct Vec2<float> { using T = float2; };
+
+// fields and spectra of one precision
+template <class R>
+struct FieldSet {
+  using C = typename Vec2<R>::T;
+  DBuf<R> phi, mu, grad[3];
+  DBuf<C> phihat, muhat, ghat[3], noise_k, canon_tmp;
+  void alloc(size_t N, size_t M) {
+    phi.alloc(N); mu.alloc(N);
+    for (int d = 0; d < 3; d++) { grad[d].alloc(N); ghat[d].alloc(M); }
+    phihat.alloc(M); muhat.alloc(M);
+  }
+};
+
 struct phafd_hip_ctx {
   std::string err;
   int device = 0;
@@ -60,8 +77,10 @@ struct phafd_hip_ctx {
   int zghost = 0;
   FFTPlan plan;
   size_t N = 0, M = 0;
-  DBuf<double> phi, mu, grad[3];
-  DBuf<double2> phihat, muhat, ghat[3], noise_k, canon_tmp;
+  bool fp32 = false;         // PHAFD_FP32: fields, spectra and grid-atom terms in FP32
+  FieldSet<double> fd;
+  FieldSet<float> fs;
+  DBuf<double> mass64;       // FP32 form: sum of phi in FP64, the k = 0 mode
   bool noise_set = false;
   DBuf<unsigned long long> counters;   // [nless, ngreat]
   DBuf<int> flags;                     // [0] rebuild, [1] overflow
@@ -104,6 +123,8 @@ struct phafd_hip_ctx {
   DBuf<int> ga_acnt, ga_aoff, ga_pg, ga_pq, ga_ppos, ga_gcnt, ga_goff, ga_skey;
   DBuf<double4> ga_ft;
   DBuf<double> ga_pv, ga_sv;
+  DBuf<float4> ga_ft32;
+  DBuf<float> ga_pv32, ga_sv32;
   DBuf<int> a2b, bincount, binstart, bincursor, binatoms;
   DBuf<int> g_cnt, g_neigh;
   DBuf<int64_t> g_off;
@@ -134,16 +155,21 @@ struct phafd_hip_ctx {
     return b;
   }
   GatheredView gview() { return GatheredView{gx.p, gshift.p, gowner.p, glabel.p}; }
-  GridView gridview() {
-    GridView v;
+  template <class R> FieldSet<R> &fields();
+  template <class R> GridViewT<R> gridview() {
+    GridViewT<R> v;
     v.nx = nx; v.ny = ny; v.nz = nz;
     v.dx = dx(); v.dy = dy(); v.dz = dz();
     v.dv = dx() * dy() * dz();
     for (int d = 0; d < 3; d++) v.lo[d] = sublo[d];
-    v.phi = phi.p; v.mu = mu.p; v.gx = grad[0].p; v.gy = grad[1].p; v.gz = grad[2].p;
+    FieldSet<R> &f = fields<R>();
+    v.phi = f.phi.p; v.mu = f.mu.p; v.gx = f.grad[0].p; v.gy = f.grad[1].p; v.gz = f.grad[2].p;
     return v;
   }
+  size_t real_bytes() const { return fp32 ? sizeof(float) : sizeof(double); }
 };
+template <> FieldSet<double> &phafd_hip_ctx::fields<double>() { return fd; }
+template <> FieldSet<float> &phafd_hip_ctx::fields<float>() { return fs; }
 
 namespace {
 
@@ -418,7 +444,7 @@ void bin_atoms(phafd_hip_ctx *c) {
 
 // NPairGridBin::build (src/npair_grid_bin.cpp:22-127)
 void build_gridlist(phafd_hip_ctx *c) {
-  GridView gv = c->gridview();
+  GridView gv = c->gridview<double>();
   const double4 *gxb = c->gx_build.p;
   const double cutneighsq = c->cutneigh * c->cutneigh;
   c->g_cnt.alloc(c->N + 1);
@@ -550,19 +576,30 @@ SpecConst spec_const(const phafd_hip_ctx *c) {
 // src/fixgrid_conjugate.cpp:62-116: r2c, grad multiply, three c2r, / N
 bool spectral_fused(const phafd_hip_ctx *c) { return c->fuse_spectral && fft_fused_z_ok(c->plan); }
 
-void compute_gradphi(phafd_hip_ctx *c, double2 *phihat_dst) {
-  double2 *ins[3] = {c->ghat[0].p, c->ghat[1].p, c->ghat[2].p};
-  double *outs[3] = {c->grad[0].p, c->grad[1].p, c->grad[2].p};
+// FP32 form: the k = 0 mode of the spectral update comes from the FP64 sum
+// of phi taken here (mass), so the mean of phi does not drift with FP32
+// rounding of the transform.
+template <class R>
+void compute_gradphi(phafd_hip_ctx *c, typename Vec2<R>::T *phihat_dst, bool mass) {
+  using C = typename Vec2<R>::T;
+  FieldSet<R> &F = c->fields<R>();
+  C *ins[3] = {F.ghat[0].p, F.ghat[1].p, F.ghat[2].p};
+  R *outs[3] = {F.grad[0].p, F.grad[1].p, F.grad[2].p};
   const double factor = (double)c->nx * c->ny * c->nz;
+  if (mass && c->fp32) {
+    fill_zero(c->mass64.p, sizeof(double), c->stream);
+    sum_fp64(c->N, F.phi.p, c->mass64.p, c->stream);
+  }
   if (c->fuse_spectral && fft_sep_grad_ok(c->plan)) {
-    fft_grad_sep(c->plan, c->phi.p, phihat_dst, ins[1], ins[2], outs[0], outs[1], outs[2], spec_const(c),
+    fft_grad_sep(c->plan, F.phi.p, phihat_dst, ins[1], ins[2], outs[0], outs[1], outs[2], spec_const(c),
                  c->stream);
     return;
   }
-  fft_r2c(c->plan, c->phi.p, phihat_dst, c->stream);
-  grad_multiply(c->plan, phihat_dst, c->ghat[0].p, c->ghat[1].p, c->ghat[2].p, spec_const(c), c->stream);
+  fft_r2c(c->plan, F.phi.p, phihat_dst, c->stream);
+  grad_multiply(c->plan, phihat_dst, F.ghat[0].p, F.ghat[1].p, F.ghat[2].p, spec_const(c), c->stream);
   fft_c2r_batch(c->plan, ins, outs, 3, factor, c->stream);
 }
+const double *origin_ptr(const phafd_hip_ctx *c) { return c->fp32 ? c->mass64.p : nullptr; }
 
 UpdateParams update_params(const phafd_hip_ctx *c, const FixDef &f) {
   UpdateParams u;
@@ -575,13 +612,19 @@ UpdateParams update_params(const phafd_hip_ctx *c, const FixDef &f) {
 }
 
 // ---------------------------------------------------------------- phases
-void phase_start(phafd_hip_ctx *c) {
+template <class R>
+void phase_start_t(phafd_hip_ctx *c) {
+  FieldSet<R> &F = c->fields<R>();
   for (auto &fp : c->fixes) {
     FixDef &f = *fp;
-    if (f.kind == FixDef::CONJ) compute_gradphi(c, c->phihat.p);
+    if (f.kind == FixDef::CONJ) compute_gradphi<R>(c, F.phihat.p, true);
     else if (f.kind == FixDef::GRADPHI && (!f.once || c->timestep == c->firststep))
-      compute_gradphi(c, c->muhat.p);
+      compute_gradphi<R>(c, F.muhat.p, false);
   }
+}
+void phase_start(phafd_hip_ctx *c) {
+  if (c->fp32) phase_start_t<float>(c);
+  else phase_start_t<double>(c);
 }
 
 void phase_neigh(phafd_hip_ctx *c) {
@@ -600,11 +643,23 @@ void phase_neigh(phafd_hip_ctx *c) {
 // emitted with their list-order keys, grouped per grid point by a counting
 // sort and summed in key order.  With the eager option the mu side walks the
 // stored list instead (the two are bit-identical).
+// typed grid-atom term buffers (the FP64 and FP32 forms have their own)
+template <class V4> DBuf<V4> &ga_terms(phafd_hip_ctx *c);
+template <> DBuf<double4> &ga_terms<double4>(phafd_hip_ctx *c) { return c->ga_ft; }
+template <> DBuf<float4> &ga_terms<float4>(phafd_hip_ctx *c) { return c->ga_ft32; }
+template <class R> DBuf<R> &ga_vals(phafd_hip_ctx *c, int which);
+template <> DBuf<double> &ga_vals<double>(phafd_hip_ctx *c, int which) { return which ? c->ga_sv : c->ga_pv; }
+template <> DBuf<float> &ga_vals<float>(phafd_hip_ctx *c, int which) { return which ? c->ga_sv32 : c->ga_pv32; }
+
 // first: mu has not been zeroed this step; the pass writes every point
 bool ga_uses_emit(const phafd_hip_ctx *c) { return !(c->eager_gridlist && c->grid_built); }
 
+template <class R>
 void grid_atom_pass(phafd_hip_ctx *c, PairDef &p, double *energy, bool first) {
-  GridView gv = c->gridview();
+  using V4 = typename std::conditional<std::is_same<R, float>::value, float4, double4>::type;
+  FieldSet<R> &Fs = c->fields<R>();
+  const GridView gv = c->gridview<double>();
+  const GridViewT<R> gvr = c->gridview<R>();
   GAParams ga = ga_params(p);
   const int ng = c->ngathered;
   const bool emit = ga_uses_emit(c);
@@ -620,31 +675,35 @@ void grid_atom_pass(phafd_hip_ctx *c, PairDef &p, double *energy, bool first) {
     p.cap_build = c->ncalls;
   }
   const int64_t cap = p.cap;
-  c->ga_pg.alloc(cap); c->ga_pq.alloc(cap); c->ga_ft.alloc(cap);
+  c->ga_pg.alloc(cap); c->ga_pq.alloc(cap);
+  DBuf<V4> &ft = ga_terms<V4>(c);
+  ft.alloc(cap);
   probe_begin(K_GA_FORCE, c->stream);
   launch_ga_count(ng, gv, ga, c->gx.p, order, ga.cutsq, c->ga_acnt.p, c->stream);
   scan_exclusive_i32(c->ga_acnt.p, c->ga_aoff.p, (size_t)ng, c->stream);
   const int *npairs = c->ga_aoff.p + ng;
   launch_ga_enum(ng, gv, ga, c->gx.p, order, c->ga_aoff.p, cap, c->ga_pg.p, c->ga_pq.p, c->flags.p + 3, c->stream);
   if (emit) {
-    c->ga_ppos.alloc(cap); c->ga_pv.alloc(cap); c->ga_skey.alloc(cap); c->ga_sv.alloc(cap);
+    c->ga_ppos.alloc(cap); c->ga_skey.alloc(cap);
+    ga_vals<R>(c, 0).alloc(cap); ga_vals<R>(c, 1).alloc(cap);
     c->ga_gcnt.alloc(c->N + 1);
     c->ga_goff.alloc(c->N + 2);
     fill_zero(c->ga_gcnt.p, sizeof(int) * c->N, c->stream);
   } else {
-    if (first) fill_zero(c->mu.p, sizeof(double) * c->N, c->stream);
+    if (first) fill_zero(Fs.mu.p, sizeof(R) * c->N, c->stream);
   }
-  launch_ga_pair(cap, npairs, gv, ga, c->gx.p, order, c->ga_pg.p, c->ga_pq.p, c->ga_ft.p, c->ga_pv.p, c->ga_ppos.p,
-                 c->ga_gcnt.p, emit, energy, c->stream);
-  launch_ga_fold(ng, order, c->ga_aoff.p, c->ga_ft.p, c->gF.p, c->stream);
+  launch_ga_pair(cap, npairs, gvr, ga, c->gx.p, order, c->ga_pg.p, c->ga_pq.p, ft.p, ga_vals<R>(c, 0).p,
+                 c->ga_ppos.p, c->ga_gcnt.p, emit, energy, c->stream);
+  launch_ga_fold(ng, order, c->ga_aoff.p, ft.p, c->gF.p, c->stream);
   probe_end(K_GA_FORCE, c->stream, 64.0 * ng);
   probe_begin(K_GA_DEPOSIT, c->stream);
   if (emit) {
     scan_exclusive_i32(c->ga_gcnt.p, c->ga_goff.p, c->N, c->stream);
-    launch_ga_place(cap, npairs, c->ga_pg.p, c->ga_ppos.p, c->ga_pq.p, c->ga_pv.p, c->ga_goff.p, c->ga_skey.p,
-                    c->ga_sv.p, c->stream);
-    launch_ga_sum(c->N, c->ga_goff.p, c->ga_skey.p, c->ga_sv.p, c->mu.p, first, c->stream);
+    launch_ga_place(cap, npairs, c->ga_pg.p, c->ga_ppos.p, c->ga_pq.p, ga_vals<R>(c, 0).p, c->ga_goff.p,
+                    c->ga_skey.p, ga_vals<R>(c, 1).p, c->stream);
+    launch_ga_sum(c->N, c->ga_goff.p, c->ga_skey.p, ga_vals<R>(c, 1).p, Fs.mu.p, first, c->stream);
   } else {
+    if (c->fp32) throw std::invalid_argument("the stored grid list walk is FP64 only");
     launch_ga_deposit(gv, ga, c->g_cnt.p, c->g_off.p, c->g_neigh.p, c->gx.p, c->nowned, nullptr, c->stream);
   }
   probe_end(K_GA_DEPOSIT, c->stream, 16.0 * c->N);
@@ -660,7 +719,7 @@ void phase_pair(phafd_hip_ctx *c, double *energy_ga = nullptr, double *energy_lj
                         return p->style != 2;
                       });
   if (!any_ga) {
-    fill_zero(c->mu.p, sizeof(double) * c->N, c->stream);
+    fill_zero(c->fp32 ? (void *)c->fs.mu.p : (void *)c->fd.mu.p, c->real_bytes() * c->N, c->stream);
     mu_pending = false;
   }
   launch_fill_d4(c->ngathered, c->gF.p, c->stream);
@@ -675,7 +734,8 @@ void phase_pair(phafd_hip_ctx *c, double *energy_ga = nullptr, double *energy_lj
                       c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, energy_lj, c->stream);
       probe_end(K_LJ, c->stream, 64.0 * c->ngathered + 4.0 * c->h_nent);
     } else {
-      grid_atom_pass(c, p, energy_ga, mu_pending);
+      if (c->fp32) grid_atom_pass<float>(c, p, energy_ga, mu_pending);
+      else grid_atom_pass<double>(c, p, energy_ga, mu_pending);
       mu_pending = false;
     }
   }
@@ -696,54 +756,68 @@ void phase_initial(phafd_hip_ctx *c) {
   for (auto &f : c->fixes) integrate_half(c, *f);
 }
 
-void phase_postforce(phafd_hip_ctx *c) {
+template <class R>
+void phase_postforce_t(phafd_hip_ctx *c) {
+  FieldSet<R> &F = c->fields<R>();
   for (auto &fp : c->fixes)
     if (fp->kind == FixDef::FH)
-      floryhuggins(c->N, c->phi.p, c->mu.p, fp->temp / fp->volFH, fp->chi, c->counters.p, c->stream);
+      floryhuggins(c->N, F.phi.p, F.mu.p, fp->temp / fp->volFH, fp->chi, c->counters.p, c->stream);
 }
 
-void phase_prefinal(phafd_hip_ctx *c, const FHPre *pre = nullptr) {
+template <class R>
+void phase_prefinal_t(phafd_hip_ctx *c, const FHPreT<R> *pre = nullptr) {
+  FieldSet<R> &F = c->fields<R>();
   for (auto &fp : c->fixes)
-    if (fp->kind == FixDef::CONJ) fft_r2c(c->plan, c->mu.p, c->muhat.p, c->stream, pre);
+    if (fp->kind == FixDef::CONJ) fft_r2c(c->plan, F.mu.p, F.muhat.p, c->stream, pre);
 }
 
-void phase_final(phafd_hip_ctx *c) {
+template <class R>
+void phase_final_t(phafd_hip_ctx *c) {
+  FieldSet<R> &F = c->fields<R>();
   for (auto &fp : c->fixes) {
     FixDef &f = *fp;
     if (f.kind == FixDef::CONJ) {
-      const double2 *nz = (f.noise_mode == PHAFD_NOISE_FIELD && c->noise_set) ? c->noise_k.p : nullptr;
-      spectral_update(c->plan, c->phihat.p, c->muhat.p, nz, update_params(c, f), c->stream);
+      const auto *nz = (f.noise_mode == PHAFD_NOISE_FIELD && c->noise_set) ? F.noise_k.p : nullptr;
+      spectral_update(c->plan, F.phihat.p, F.muhat.p, nz, update_params(c, f), origin_ptr(c), c->stream);
     } else {
       integrate_half(c, f);
     }
   }
 }
 
-void phase_postfinal(phafd_hip_ctx *c) {
+template <class R>
+void phase_postfinal_t(phafd_hip_ctx *c) {
+  FieldSet<R> &F = c->fields<R>();
   for (auto &fp : c->fixes)
-    if (fp->kind == FixDef::CONJ) fft_c2r(c->plan, c->phihat.p, c->phi.p, 0.0, c->stream);
+    if (fp->kind == FixDef::CONJ) fft_c2r(c->plan, F.phihat.p, F.phi.p, 0.0, c->stream);
 }
 
 // The same three hooks with the z pass of r2c(mu), the update and the z pass
 // of c2r(phihat) folded into one kernel; used by run_steps, where the hooks
 // always run back to back (one conjugate fix).
-void phase_prefinal_split(phafd_hip_ctx *c, const FHPre *pre) {
-  fft_r2c_xy(c->plan, c->mu.p, c->muhat.p, c->stream, pre);
+template <class R>
+void phase_prefinal_split(phafd_hip_ctx *c, const FHPreT<R> *pre) {
+  FieldSet<R> &F = c->fields<R>();
+  fft_r2c_xy(c->plan, F.mu.p, F.muhat.p, c->stream, pre);
 }
+template <class R>
 void phase_final_split(phafd_hip_ctx *c) {
+  FieldSet<R> &F = c->fields<R>();
   for (auto &fp : c->fixes) {
     FixDef &f = *fp;
     if (f.kind == FixDef::CONJ) {
-      const double2 *nz = (f.noise_mode == PHAFD_NOISE_FIELD && c->noise_set) ? c->noise_k.p : nullptr;
-      fft_z_update(c->plan, c->muhat.p, c->phihat.p, nz, update_params(c, f), c->stream);
+      const auto *nz = (f.noise_mode == PHAFD_NOISE_FIELD && c->noise_set) ? F.noise_k.p : nullptr;
+      fft_z_update(c->plan, F.muhat.p, F.phihat.p, nz, update_params(c, f), origin_ptr(c), c->stream);
     } else {
       integrate_half(c, f);
     }
   }
 }
+template <class R>
 void phase_postfinal_split(phafd_hip_ctx *c) {
-  double2 *in = c->phihat.p;
-  double *out = c->phi.p;
+  FieldSet<R> &F = c->fields<R>();
+  auto *in = F.phihat.p;
+  R *out = F.phi.p;
   fft_c2r_yx_batch(c->plan, &in, &out, 1, 0.0, c->stream);
 }
 
@@ -753,10 +827,22 @@ void run_phase(phafd_hip_ctx *c, int ph) {
     case PHAFD_PH_NEIGH: phase_neigh(c); break;
     case PHAFD_PH_PAIR: phase_pair(c); break;
     case PHAFD_PH_INITIAL: phase_initial(c); break;
-    case PHAFD_PH_POSTFORCE: phase_postforce(c); break;
-    case PHAFD_PH_PREFINAL: phase_prefinal(c); break;
-    case PHAFD_PH_FINAL: phase_final(c); break;
-    case PHAFD_PH_POSTFINAL: phase_postfinal(c); break;
+    case PHAFD_PH_POSTFORCE:
+      if (c->fp32) phase_postforce_t<float>(c);
+      else phase_postforce_t<double>(c);
+      break;
+    case PHAFD_PH_PREFINAL:
+      if (c->fp32) phase_prefinal_t<float>(c);
+      else phase_prefinal_t<double>(c);
+      break;
+    case PHAFD_PH_FINAL:
+      if (c->fp32) phase_final_t<float>(c);
+      else phase_final_t<double>(c);
+      break;
+    case PHAFD_PH_POSTFINAL:
+      if (c->fp32) phase_postfinal_t<float>(c);
+      else phase_postfinal_t<double>(c);
+      break;
     default: throw std::invalid_argument("bad phase");
   }
 }
@@ -766,7 +852,8 @@ void require_setup(phafd_hip_ctx *c) {
 }
 
 // FH fused into the r2c of mu when the two hooks are adjacent and alone
-bool can_fuse_fh(phafd_hip_ctx *c, FHPre &pre) {
+template <class R>
+bool can_fuse_fh(phafd_hip_ctx *c, FHPreT<R> &pre) {
   if (!c->fuse_fh) return false;
   int nconj = 0, nfh = 0;
   const FixDef *fh = nullptr;
@@ -776,17 +863,17 @@ bool can_fuse_fh(phafd_hip_ctx *c, FHPre &pre) {
   }
   if (nconj != 1 || nfh != 1) return false;
   pre.on = true;
-  pre.phi = c->phi.p;
-  pre.mu = c->mu.p;
+  pre.phi = c->fields<R>().phi.p;
+  pre.mu = c->fields<R>().mu.p;
   pre.tv = fh->temp / fh->volFH;
   pre.chi = fh->chi;
   pre.counters = c->counters.p;
   return true;
 }
 
-void run_steps(phafd_hip_ctx *c, int nsteps) {
-  require_setup(c);
-  FHPre pre;
+template <class R>
+void run_steps_t(phafd_hip_ctx *c, int nsteps) {
+  FHPreT<R> pre;
   const bool fused = can_fuse_fh(c, pre);
   int nconj = 0;
   for (auto &f : c->fixes) nconj += f->kind == FixDef::CONJ;
@@ -802,15 +889,15 @@ void run_steps(phafd_hip_ctx *c, int nsteps) {
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
     phase_initial(c);
     HIPCHK(hipEventRecord(c->ev[4], c->stream));
-    if (!fused) phase_postforce(c);
-    if (zsplit) phase_prefinal_split(c, fused ? &pre : nullptr);
-    else phase_prefinal(c, fused ? &pre : nullptr);
+    if (!fused) phase_postforce_t<R>(c);
+    if (zsplit) phase_prefinal_split<R>(c, fused ? &pre : nullptr);
+    else phase_prefinal_t<R>(c, fused ? &pre : nullptr);
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
-    if (zsplit) phase_final_split(c);
-    else phase_final(c);
+    if (zsplit) phase_final_split<R>(c);
+    else phase_final_t<R>(c);
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
-    if (zsplit) phase_postfinal_split(c);
-    else phase_postfinal(c);
+    if (zsplit) phase_postfinal_split<R>(c);
+    else phase_postfinal_t<R>(c);
     HIPCHK(hipEventRecord(c->ev[7], c->stream));
     // next step's rebuild decision (Neighbor::decide) + NaN check of
     // phi(0,0,0) (src/integrate.cpp:190-201), read back together
@@ -819,7 +906,7 @@ void run_steps(phafd_hip_ctx *c, int nsteps) {
       launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->flags.p, c->stream);
     }
     HIPCHK(hipMemcpyAsync(c->h_flags + 8, c->flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->phi.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->fields<R>().phi.p, sizeof(R), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipEventRecord(c->ev[8], c->stream));
     sync(c);
     probe_collect();
@@ -831,8 +918,8 @@ void run_steps(phafd_hip_ctx *c, int nsteps) {
     float tot = 0;
     HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[8]));
     c->tacc[7] += tot;
-    double phi0;
-    std::memcpy(&phi0, c->h_flags + 2, sizeof(double));
+    R phi0;
+    std::memcpy(&phi0, c->h_flags + 2, sizeof(R));
     if (std::isnan(phi0)) throw std::runtime_error("NAN encountered in phi.");
     if (c->h_flags[11]) throw std::runtime_error("grid-atom term capacity exceeded");
     if (c->neigh_set) {
@@ -840,6 +927,12 @@ void run_steps(phafd_hip_ctx *c, int nsteps) {
       c->pending_decision = c->h_flags[8] & 1;
     }
   }
+}
+
+void run_steps(phafd_hip_ctx *c, int nsteps) {
+  require_setup(c);
+  if (c->fp32) run_steps_t<float>(c, nsteps);
+  else run_steps_t<double>(c, nsteps);
 }
 
 // flatten group chunks (src/group.cpp:31-207) into member lists
@@ -866,20 +959,43 @@ void build_members(phafd_hip_ctx *c, FixDef &f) {
   }
 }
 
-void copy_field_in(phafd_hip_ctx *c, double *d, const double *h) {
-  HIPCHK(hipMemcpyAsync(d, h, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
-  sync(c);
-}
-
-double *field_ptr(phafd_hip_ctx *c, int which) {
+template <class R>
+R *field_ptr(phafd_hip_ctx *c, int which) {
+  FieldSet<R> &F = c->fields<R>();
   switch (which) {
-    case PHAFD_PHI: return c->phi.p;
-    case PHAFD_MU: return c->mu.p;
-    case PHAFD_GRADX: return c->grad[0].p;
-    case PHAFD_GRADY: return c->grad[1].p;
-    case PHAFD_GRADZ: return c->grad[2].p;
+    case PHAFD_PHI: return F.phi.p;
+    case PHAFD_MU: return F.mu.p;
+    case PHAFD_GRADX: return F.grad[0].p;
+    case PHAFD_GRADY: return F.grad[1].p;
+    case PHAFD_GRADZ: return F.grad[2].p;
   }
   throw std::invalid_argument("unknown field");
+}
+
+// the host side always holds FP64; the FP32 form rounds on the way in
+void field_in(phafd_hip_ctx *c, int which, const double *h) {
+  if (!c->fp32) {
+    HIPCHK(hipMemcpyAsync(field_ptr<double>(c, which), h, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
+  } else {
+    std::vector<float> t(c->N);
+    for (size_t i = 0; i < c->N; i++) t[i] = (float)h[i];
+    HIPCHK(hipMemcpyAsync(field_ptr<float>(c, which), t.data(), sizeof(float) * c->N, hipMemcpyHostToDevice,
+                          c->stream));
+    sync(c);
+  }
+  sync(c);
+}
+void field_out(phafd_hip_ctx *c, int which, double *h) {
+  if (!c->fp32) {
+    HIPCHK(hipMemcpyAsync(h, field_ptr<double>(c, which), sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+  } else {
+    std::vector<float> t(c->N);
+    HIPCHK(hipMemcpyAsync(t.data(), field_ptr<float>(c, which), sizeof(float) * c->N, hipMemcpyDeviceToHost,
+                          c->stream));
+    sync(c);
+    for (size_t i = 0; i < c->N; i++) h[i] = t[i];
+  }
 }
 
 std::vector<double4> to_d4(const double *xyz, int n) {
@@ -915,12 +1031,14 @@ int phafd_hip_create(const phafd_hip_desc *desc, phafd_hip_ctx **out) {
   auto c = std::make_unique<phafd_hip_ctx>();
   c->device = desc->device;
   int rc = guard(c.get(), [&] {
-    if (desc->precision != PHAFD_FP64) throw std::invalid_argument("only PHAFD_FP64 is built");
+    if (desc->precision != PHAFD_FP64 && desc->precision != PHAFD_FP32)
+      throw std::invalid_argument("precision must be PHAFD_FP64 or PHAFD_FP32");
+    c->fp32 = desc->precision == PHAFD_FP32;
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (desc->device < 0 || desc->device >= ndev) throw std::invalid_argument("no such HIP device");
     HIPCHK(hipSetDevice(desc->device));
-    c->plan.init(desc->nx, desc->ny, desc->nz);
+    c->plan.init(desc->nx, desc->ny, desc->nz, desc->precision == PHAFD_FP32);
     c->nx = desc->nx; c->ny = desc->ny; c->nz = desc->nz;
     for (int d = 0; d < 3; d++) {
       if (!(desc->period[d] > 0)) throw std::invalid_argument("need to set box dimensions.");
@@ -936,17 +1054,17 @@ int phafd_hip_create(const phafd_hip_desc *desc, phafd_hip_ctx **out) {
     c->N = c->plan.N();
     c->M = c->plan.M();
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    c->phi.alloc(c->N); c->mu.alloc(c->N);
-    for (int d = 0; d < 3; d++) { c->grad[d].alloc(c->N); c->ghat[d].alloc(c->M); }
-    c->phihat.alloc(c->M); c->muhat.alloc(c->M);
+    if (c->fp32) c->fs.alloc(c->N, c->M);
+    else c->fd.alloc(c->N, c->M);
+    c->mass64.alloc(1);
     c->counters.alloc(2);
     c->flags.alloc(4);
     c->esum.alloc(8);
     HIPCHK(hipHostMalloc((void **)&c->h_flags, 64, hipHostMallocDefault));
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
-    fill_zero(c->phi.p, sizeof(double) * c->N, c->stream);
-    fill_zero(c->mu.p, sizeof(double) * c->N, c->stream);
-    for (int d = 0; d < 3; d++) fill_zero(c->grad[d].p, sizeof(double) * c->N, c->stream);
+    for (int w = PHAFD_PHI; w <= PHAFD_GRADZ; w++)
+      fill_zero(c->fp32 ? (void *)field_ptr<float>(c.get(), w) : (void *)field_ptr<double>(c.get(), w),
+                c->real_bytes() * c->N, c->stream);
     fill_zero(c->counters.p, 16, c->stream);
     fill_zero(c->flags.p, 16, c->stream);
     sync(c.get());
@@ -975,25 +1093,34 @@ const char *phafd_hip_last_error(const phafd_hip_ctx *c) { return c ? c->err.c_s
 
 int phafd_hip_set_field(phafd_hip_ctx *c, int which, const double *h) {
   return guard(c, [&] {
-    copy_field_in(c, field_ptr(c, which), h);
+    field_in(c, which, h);
     return 0;
   });
 }
 int phafd_hip_get_field(phafd_hip_ctx *c, int which, double *h) {
   return guard(c, [&] {
-    HIPCHK(hipMemcpyAsync(h, field_ptr(c, which), sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
-    sync(c);
+    field_out(c, which, h);
     return 0;
   });
 }
 int phafd_hip_get_spectrum(phafd_hip_ctx *c, int which, double *h) {
   return guard(c, [&] {
-    const double2 *k = which == PHAFD_PHIHAT ? c->phihat.p : which == PHAFD_MUHAT ? c->muhat.p : nullptr;
-    if (!k) throw std::invalid_argument("unknown spectrum");
-    c->canon_tmp.alloc(c->M);
-    spec_k_to_canon(c->plan, k, c->canon_tmp.p, c->stream);
-    HIPCHK(hipMemcpyAsync(h, c->canon_tmp.p, sizeof(double2) * c->M, hipMemcpyDeviceToHost, c->stream));
-    sync(c);
+    if (which != PHAFD_PHIHAT && which != PHAFD_MUHAT) throw std::invalid_argument("unknown spectrum");
+    if (!c->fp32) {
+      FieldSet<double> &F = c->fd;
+      F.canon_tmp.alloc(c->M);
+      spec_k_to_canon(c->plan, which == PHAFD_PHIHAT ? F.phihat.p : F.muhat.p, F.canon_tmp.p, c->stream);
+      HIPCHK(hipMemcpyAsync(h, F.canon_tmp.p, sizeof(double2) * c->M, hipMemcpyDeviceToHost, c->stream));
+      sync(c);
+    } else {
+      FieldSet<float> &F = c->fs;
+      F.canon_tmp.alloc(c->M);
+      spec_k_to_canon(c->plan, which == PHAFD_PHIHAT ? F.phihat.p : F.muhat.p, F.canon_tmp.p, c->stream);
+      std::vector<float2> t(c->M);
+      HIPCHK(hipMemcpyAsync(t.data(), F.canon_tmp.p, sizeof(float2) * c->M, hipMemcpyDeviceToHost, c->stream));
+      sync(c);
+      for (size_t i = 0; i < c->M; i++) { h[2 * i] = t[i].x; h[2 * i + 1] = t[i].y; }
+    }
     return 0;
   });
 }
@@ -1118,10 +1245,22 @@ int phafd_hip_fix_conjugate(phafd_hip_ctx *c, int seed, double mobility, double 
 int phafd_hip_set_noise(phafd_hip_ctx *c, const double *xi) {
   return guard(c, [&] {
     if (!xi) { c->noise_set = false; return 0; }
-    c->noise_k.alloc(c->M);
-    c->canon_tmp.alloc(c->M);
-    HIPCHK(hipMemcpyAsync(c->canon_tmp.p, xi, sizeof(double2) * c->M, hipMemcpyHostToDevice, c->stream));
-    spec_canon_to_k(c->plan, c->canon_tmp.p, c->noise_k.p, c->stream);
+    if (!c->fp32) {
+      FieldSet<double> &F = c->fd;
+      F.noise_k.alloc(c->M);
+      F.canon_tmp.alloc(c->M);
+      HIPCHK(hipMemcpyAsync(F.canon_tmp.p, xi, sizeof(double2) * c->M, hipMemcpyHostToDevice, c->stream));
+      spec_canon_to_k(c->plan, F.canon_tmp.p, F.noise_k.p, c->stream);
+    } else {
+      FieldSet<float> &F = c->fs;
+      F.noise_k.alloc(c->M);
+      F.canon_tmp.alloc(c->M);
+      std::vector<float2> t(c->M);
+      for (size_t i = 0; i < c->M; i++) t[i] = make_float2((float)xi[2 * i], (float)xi[2 * i + 1]);
+      HIPCHK(hipMemcpyAsync(F.canon_tmp.p, t.data(), sizeof(float2) * c->M, hipMemcpyHostToDevice, c->stream));
+      spec_canon_to_k(c->plan, F.canon_tmp.p, F.noise_k.p, c->stream);
+      sync(c);
+    }
     sync(c);
     c->noise_set = true;
     return 0;
@@ -1327,13 +1466,21 @@ int phafd_hip_energy(phafd_hip_ctx *c, double *out) {
     for (auto &f : c->fixes) {
       if (f->kind == FixDef::FH) {
         // FH part only here; gradient part added by the CONJ fix below
-        reduce_energy_fh(c->N, c->phi.p, nullptr, nullptr, nullptr, f->temp / f->volFH, f->chi, 0.0,
-                         c->esum.p, c->stream);
+        if (c->fp32)
+          reduce_energy_fh(c->N, c->fs.phi.p, (const float *)nullptr, nullptr, nullptr, f->temp / f->volFH, f->chi,
+                           0.0, c->esum.p, c->stream);
+        else
+          reduce_energy_fh(c->N, c->fd.phi.p, (const double *)nullptr, nullptr, nullptr, f->temp / f->volFH,
+                           f->chi, 0.0, c->esum.p, c->stream);
       } else if (f->kind == FixDef::CONJ) {
         // square-gradient term: pass phi with chi = 0 would add FH terms, so
         // accumulate only the gradient sum into esum[1] then move it
-        reduce_energy_fh(c->N, c->phi.p, c->grad[0].p, c->grad[1].p, c->grad[2].p, 0.0, 0.0,
-                         f->temp / f->volFH * f->gamma, c->esum.p + 2, c->stream);
+        if (c->fp32)
+          reduce_energy_fh(c->N, c->fs.phi.p, c->fs.grad[0].p, c->fs.grad[1].p, c->fs.grad[2].p, 0.0, 0.0,
+                           f->temp / f->volFH * f->gamma, c->esum.p + 2, c->stream);
+        else
+          reduce_energy_fh(c->N, c->fd.phi.p, c->fd.grad[0].p, c->fd.grad[1].p, c->fd.grad[2].p, 0.0, 0.0,
+                           f->temp / f->volFH * f->gamma, c->esum.p + 2, c->stream);
       } else if (f->kind == FixDef::CHAIN) {
         ChainParams cp{f->r0, f->kbond, f->kbend, f->drag};
         launch_chain_energy(f->nmem, f->members.p, f->cstart.p, f->cend.p, c->ux.p, cp, c->esum.p + 6, c->stream);
@@ -1341,16 +1488,17 @@ int phafd_hip_energy(phafd_hip_ctx *c, double *out) {
     }
     // pair energies: recompute the pair pass into scratch copies of mu / gF
     if (c->neigh_set && c->lists_valid) {
-      DBuf<double> mu_save;
+      DBuf<double> mu_save;   // holds either precision (8 bytes per point is enough)
       DBuf<double4> gf_save, f_save;
       mu_save.alloc(c->N);
+      void *mu_p = c->fp32 ? (void *)c->fs.mu.p : (void *)c->fd.mu.p;
       gf_save.alloc((size_t)c->ngathered + 1);
       f_save.alloc((size_t)c->nowned + 1);
-      HIPCHK(hipMemcpyAsync(mu_save.p, c->mu.p, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(mu_save.p, mu_p, c->real_bytes() * c->N, hipMemcpyDeviceToDevice, c->stream));
       if (c->ngathered) HIPCHK(hipMemcpyAsync(gf_save.p, c->gF.p, sizeof(double4) * c->ngathered, hipMemcpyDeviceToDevice, c->stream));
       if (c->nowned) HIPCHK(hipMemcpyAsync(f_save.p, c->F.p, sizeof(double4) * c->nowned, hipMemcpyDeviceToDevice, c->stream));
       phase_pair(c, c->esum.p + 4, c->esum.p + 5);
-      HIPCHK(hipMemcpyAsync(c->mu.p, mu_save.p, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(mu_p, mu_save.p, c->real_bytes() * c->N, hipMemcpyDeviceToDevice, c->stream));
       if (c->ngathered) HIPCHK(hipMemcpyAsync(c->gF.p, gf_save.p, sizeof(double4) * c->ngathered, hipMemcpyDeviceToDevice, c->stream));
       if (c->nowned) HIPCHK(hipMemcpyAsync(c->F.p, f_save.p, sizeof(double4) * c->nowned, hipMemcpyDeviceToDevice, c->stream));
       sync(c);
@@ -1531,7 +1679,7 @@ int phafd_hip_spectral_update(int nx, int ny, int nz, const double *period, doub
     u.qx_scale = 2 * PI_REF / period[0];
     u.qky_scale = 2 * PI_REF / period[1];
     u.qkz_scale = 2 * PI_REF / period[2];
-    spectral_update(sc.plan, pk.p, mk.p, noise ? nk.p : nullptr, u, sc.s);
+    spectral_update(sc.plan, pk.p, mk.p, noise ? nk.p : nullptr, u, nullptr, sc.s);
     spec_k_to_canon(sc.plan, pk.p, cn.p, sc.s);
     HIPCHK(hipStreamSynchronize(sc.s));
     HIPCHK(hipMemcpy(phihat, cn.p, sizeof(double2) * M, hipMemcpyDeviceToHost));

@@ -75,81 +75,93 @@ void probe_collect();   // after the stream has been synchronised
 // ------------------------------------------------------------------- FFT
 // K layout of half spectra on the device: [Nx/2+1][Nz][Ny] (ky fastest), so
 // that the y pass reads contiguous rows and the z pass reads contiguous runs
-// of ky.  Canonical host layout is [Nz][Ny][Nx/2+1].
+// of ky.  Canonical host layout is [Nz][Ny][Nx/2+1].  Every transform exists
+// in FP64 (double / double2) and FP32 (float / float2), as overloads.
 struct FFTPlan {
   int nx = 0, ny = 0, nz = 0, hx = 0;
   DBuf<double2> tw_x_f, tw_x_b;   // length nx   (post/pre processing of the real pass)
   DBuf<double2> tw_h_f, tw_h_b;   // length nx/2 (complex FFT of the packed row)
   DBuf<double2> tw_y_f, tw_y_b;   // length ny
   DBuf<double2> tw_z_f, tw_z_b;   // length nz
-  void init(int nx, int ny, int nz);
+  DBuf<float2> tw32_x_f, tw32_x_b, tw32_h_f, tw32_h_b, tw32_y_f, tw32_y_b, tw32_z_f, tw32_z_b;
+  void init(int nx, int ny, int nz, bool fp32 = false);
   size_t M() const { return (size_t)hx * nz * ny; }
   size_t N() const { return (size_t)nx * ny * nz; }
 };
 
 // pre-op fused into the load of the r2c x pass
-struct FHPre {
+template <class R>
+struct FHPreT {
   bool on = false;
-  double *phi = nullptr;      // read, reflected values written back
-  double *mu = nullptr;       // read, mu + FH written back
+  R *phi = nullptr;           // read, reflected values written back
+  R *mu = nullptr;            // read, mu + FH written back
   double tv = 1, chi = 1;     // temp/volFH, chi
   unsigned long long *counters = nullptr;   // [nless, ngreat]
 };
+using FHPre = FHPreT<double>;
 
-void fft_r2c(const FFTPlan &p, const double *in, double2 *out, hipStream_t s,
-             const FHPre *pre = nullptr);
-// c2r destroys `in`; `divisor` != 0 divides the output (gradphi /= N)
-void fft_c2r(const FFTPlan &p, double2 *in, double *out, double divisor, hipStream_t s);
-// batched c2r over `batch` spectra (one launch per pass)
-void fft_c2r_batch(const FFTPlan &p, double2 *const *in, double *const *out, int batch,
-                   double divisor, hipStream_t s);
-
-// Split transforms for the fused z pass.  fft_r2c_xy does the x and y passes
-// of an r2c; fft_z_update finishes mu's transform in place, applies the
-// spectral update to phihat and leaves it z-inverted (finish with
-// fft_c2r_yx_batch).
 struct SpecConst;
 struct UpdateParams;
+#define PHAFD_FFT_DECL(RT, CT)                                                                         \
+  void fft_r2c(const FFTPlan &p, const RT *in, CT *out, hipStream_t s, const FHPreT<RT> *pre = nullptr); \
+  void fft_c2r(const FFTPlan &p, CT *in, RT *out, double divisor, hipStream_t s);                        \
+  void fft_c2r_batch(const FFTPlan &p, CT *const *in, RT *const *out, int batch, double divisor,         \
+                     hipStream_t s);                                                                     \
+  void fft_r2c_xy(const FFTPlan &p, const RT *in, CT *out, hipStream_t s, const FHPreT<RT> *pre = nullptr); \
+  void fft_grad_sep(const FFTPlan &p, const RT *phi, CT *ph, CT *gy_k, CT *gz_k, RT *gx, RT *gy, RT *gz, \
+                    SpecConst sc, hipStream_t s);                                                        \
+  void fft_z_update(const FFTPlan &p, CT *mu, CT *ph, const CT *noise, UpdateParams u,                   \
+                    const double *origin, hipStream_t s);                                                \
+  void fft_c2r_yx_batch(const FFTPlan &p, CT *const *in, RT *const *out, int batch, double divisor,      \
+                        hipStream_t s);                                                                  \
+  void spec_k_to_canon(const FFTPlan &p, const CT *k, CT *canon, hipStream_t s);                         \
+  void spec_canon_to_k(const FFTPlan &p, const CT *canon, CT *k, hipStream_t s);
+// fft_r2c / fft_c2r: whole transforms (c2r destroys `in`; divisor != 0
+// divides the output).  fft_c2r_batch: up to 4 spectra per launch.
+// fft_r2c_xy: the x and y passes of an r2c.  fft_grad_sep: the gradient of
+// phi by separable spectral derivatives (grids with fft_sep_grad_ok): gx from
+// one x-line kernel, gy from the y pass, gz from the z pass; ph receives
+// phihat; gy_k, gz_k are scratch spectra; equal in exact arithmetic to r2c,
+// grad multiply, three c2r / N.  fft_z_update: z pass of mu's r2c, the
+// spectral update of phihat and the z pass of its c2r in one kernel (grids
+// with fft_fused_z_ok); finish with fft_c2r_yx_batch.  origin != nullptr
+// (FP32 form): the k = 0 mode is taken from the FP64 sum there.
+PHAFD_FFT_DECL(double, double2)
+PHAFD_FFT_DECL(float, float2)
+#undef PHAFD_FFT_DECL
 bool fft_fused_z_ok(const FFTPlan &p);
-void fft_r2c_xy(const FFTPlan &p, const double *in, double2 *out, hipStream_t s,
-                const FHPre *pre = nullptr);
-// Gradient of phi by separable spectral derivatives (grids with
-// fft_sep_grad_ok): gx from one x-line kernel, gy from the y pass, gz from the
-// z pass; ph receives phihat (the full r2c of phi); gy_k, gz_k are scratch
-// spectra.  Equal in exact arithmetic to r2c, grad multiply, three c2r / N.
 bool fft_sep_grad_ok(const FFTPlan &p);
-void fft_grad_sep(const FFTPlan &p, const double *phi, double2 *ph, double2 *gy_k, double2 *gz_k,
-                  double *gx, double *gy, double *gz, SpecConst sc, hipStream_t s);
-void fft_z_update(const FFTPlan &p, double2 *mu, double2 *ph, const double2 *noise, UpdateParams u,
-                  hipStream_t s);
-void fft_c2r_yx_batch(const FFTPlan &p, double2 *const *in, double *const *out, int batch,
-                      double divisor, hipStream_t s);
-
-// K layout <-> canonical layout (host-visible order) conversions on device
-void spec_k_to_canon(const FFTPlan &p, const double2 *k, double2 *canon, hipStream_t s);
-void spec_canon_to_k(const FFTPlan &p, const double2 *canon, double2 *k, hipStream_t s);
 
 // ------------------------------------------------------------ field kernels
 struct SpecConst {
   double dqx, dqy_kz, dqz_ky;   // 2*3.141592/Lx, /Lz, /Ly (src/domain.cpp:206-208)
 };
-void grad_multiply(const FFTPlan &p, const double2 *phihat, double2 *gx, double2 *gy,
-                   double2 *gz, SpecConst c, hipStream_t s);
 struct UpdateParams {
   double mobility, temp, volFH, gamma, dt;
   double normalization, qx_scale, qky_scale, qkz_scale;
 };
-void spectral_update(const FFTPlan &p, double2 *phihat, const double2 *muhat,
-                     const double2 *noise_k, UpdateParams u, hipStream_t s);
-void spectral_update_edges(const FFTPlan &p, double2 *phihat, const double2 *muhat,
-                           const double2 *noise_k, UpdateParams u, hipStream_t s);
-void floryhuggins(size_t n, double *phi, double *mu, double tv, double chi,
-                  unsigned long long *counters, hipStream_t s);
+#define PHAFD_FIELD_DECL(RT, CT)                                                                     \
+  void grad_multiply(const FFTPlan &p, const CT *phihat, CT *gx, CT *gy, CT *gz, SpecConst c,        \
+                     hipStream_t s);                                                                 \
+  void spectral_update(const FFTPlan &p, CT *phihat, const CT *muhat, const CT *noise_k,             \
+                       UpdateParams u, const double *origin, hipStream_t s);                         \
+  void spectral_update_edges(const FFTPlan &p, CT *phihat, const CT *muhat, const CT *noise_k,       \
+                             UpdateParams u, const double *origin, hipStream_t s);                   \
+  void floryhuggins(size_t n, RT *phi, RT *mu, double tv, double chi, unsigned long long *counters,  \
+                    hipStream_t s);                                                                  \
+  void reduce_energy_fh(size_t n, const RT *phi, const RT *gx, const RT *gy, const RT *gz, double tv, \
+                        double chi, double tvg, double *out2, hipStream_t s);                        \
+  void sum_fp64(size_t n, const RT *x, double *out, hipStream_t s);
+// grad_multiply: the three i*q multiplies of the gradient.  spectral_update:
+// Conjugate::update as a pointwise map (origin != nullptr: the k = 0 mode is
+// set from the FP64 sum there times the normalisation); _edges: only the
+// planes kx = 0 and kx = Nx/2.  reduce_energy_fh: FH + square-gradient
+// energies summed in FP64 into out2[0..1].  sum_fp64: out[0] += sum of x in
+// FP64.
+PHAFD_FIELD_DECL(double, double2)
+PHAFD_FIELD_DECL(float, float2)
+#undef PHAFD_FIELD_DECL
 void fill_zero(void *p, size_t bytes, hipStream_t s);
-// nan check of phi[0] + rebuild decision reduced into flags[0..1]
-void reduce_energy_fh(size_t n, const double *phi, const double *gx, const double *gy,
-                      const double *gz, double tv, double chi, double tvg, double *out2,
-                      hipStream_t s);
 
 // --------------------------------------------------------------- scan utils
 // exclusive scan of n int32 values into int64 offsets; out has n+1 entries,

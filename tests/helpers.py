@@ -71,9 +71,10 @@ def make_oracle(w):
     return apply(s, w, True)
 
 
-def make_gpu(w):
+def make_gpu(w, precision=None):
     import phafd_amd as P
-    s = P.System(w["n"], w["n"], w["n"], [w["origin"]] * 3, [w["box"]] * 3, zghost=w["zghost"])
+    s = P.System(w["n"], w["n"], w["n"], [w["origin"]] * 3, [w["box"]] * 3, zghost=w["zghost"],
+                 precision=P.FP64 if precision is None else precision)
     return apply(s, w, False)
 
 

@@ -286,3 +286,54 @@ def test_dense_tile_mu_terms(ljish):
     assert rel_l2(a.get_field(P.MU), o.get_field(P.MU)) < 1e-10
     assert rel_l2(a.get_forces(), o.get_forces()) < 1e-10
     assert rel_l2(a.get_field(P.PHI), o.get_field(P.PHI)) < 1e-10
+
+
+# ---------------------------------------------------------------- FP32 form
+# FP32 FFT, fields and grid-atom terms (Kahan-compensated force folds and mu
+# sums); atoms, lists, lj/cut and chains stay FP64, and so do the k = 0 mode
+# and the energies.  Its oracle is the FP64 path (pinned above to the CPU
+# oracle); the bounds below are the measured FP32 error with headroom.
+def test_fp32_form_against_fp64():
+    w = workload(n=64, nchains=20, nbeads=50, seed=42, zghost=1, chi=3.0)
+    a = make_gpu(w, precision=P.FP32)
+    b = make_gpu(w)
+    phi0 = w["phi"].mean()
+    a.run(1)
+    b.run(1)
+    m1 = a.get_field(P.PHI).mean()
+    a.run(99)
+    b.run(99)
+    e_phi = rel_l2(a.get_field(P.PHI), b.get_field(P.PHI))
+    e_mu = rel_l2(a.get_field(P.MU), b.get_field(P.MU))
+    e_x = rel_l2(a.get_atoms()[0], b.get_atoms()[0])
+    e_f = rel_l2(a.get_forces(), b.get_forces())
+    ea, eb = np.array(a.energy()), np.array(b.energy())
+    print(f"fp32 vs fp64 after 100 steps: phi {e_phi:.2e} mu {e_mu:.2e} x {e_x:.2e} F {e_f:.2e} "
+          f"energies {np.abs(ea - eb) / np.maximum(np.abs(eb), 1e-30)}")
+    assert e_phi < 1e-5
+    assert e_mu < 1e-4
+    assert e_x < 1e-6
+    assert e_f < 1e-3
+    assert np.allclose(ea, eb, rtol=1e-4, atol=1e-6)
+    # mass: the k = 0 mode comes from the FP64 sum of phi, so the mean stays
+    # at the FP32 value nearest the initial mean (half an ulp of 0.5 away at
+    # most); what moves it is only the rounding of the c2r output around
+    # that mode (measured 3.7e-9 over 99 steps), below one FP32 ulp
+    m100 = a.get_field(P.PHI).mean()
+    assert abs(m1 - phi0) < 3.0e-8
+    assert abs(m100 - m1) < 2e-8
+    assert a.counts()["ncalls"] == b.counts()["ncalls"]
+
+
+@pytest.mark.parametrize("shape,period", [((16, 64, 32), [32.0, 96.0, 12.0]), ((32, 32, 32), [32.0, 32.0, 32.0])])
+def test_fp32_fields_vs_oracle(shape, period):
+    # fused and unfused FP32 spectral paths against the CPU oracle
+    nz, ny, nx = shape
+    o = _fields_only(O.System, shape, period, O.NOISE_OFF)
+    a = _fields_only(lambda *args: P.System(*args, precision=P.FP32), shape, period, P.NOISE_OFF)
+    o.run(20)
+    a.run(20)
+    errs = {f: rel_l2(a.get_field(f), o.get_field(f)) for f in (P.PHI, P.MU, P.GRADX, P.GRADY, P.GRADZ)}
+    print(f"fp32 fields vs oracle {shape}: {errs}")
+    assert errs[P.PHI] < 1e-5
+    assert all(e < 2e-4 for e in errs.values())
