@@ -36,7 +36,8 @@ def test_fft_golden(tag):
     assert rel_l2(P.fft_c2r(c, nx), GOLD[f"c2r_out_{tag}"]) < 1e-13
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 64), (32, 16, 128), (128, 8, 4), (4, 256, 512), (16, 1024, 8)])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (32, 16, 128), (128, 8, 4), (4, 256, 512), (16, 1024, 8),
+                                   (512, 8, 16), (8, 512, 16), (4, 8, 1024)])
 def test_fft_vs_oracle(shape):
     rng = np.random.default_rng(sum(shape))
     a = rng.standard_normal(shape)
@@ -325,7 +326,8 @@ def test_fp32_form_against_fp64():
     assert a.counts()["ncalls"] == b.counts()["ncalls"]
 
 
-@pytest.mark.parametrize("shape,period", [((16, 64, 32), [32.0, 96.0, 12.0]), ((32, 32, 32), [32.0, 32.0, 32.0])])
+@pytest.mark.parametrize("shape,period", [((16, 64, 32), [32.0, 96.0, 12.0]), ((32, 32, 32), [32.0, 32.0, 32.0]),
+                                          ((512, 16, 16), [16.0, 16.0, 512.0])])
 def test_fp32_fields_vs_oracle(shape, period):
     # fused and unfused FP32 spectral paths against the CPU oracle
     nz, ny, nx = shape
