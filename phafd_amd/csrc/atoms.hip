@@ -645,21 +645,32 @@ __global__ void k_ga_fold(int ng, const int *order, const int *aoff, const float
   }
 }
 
-// every term to its grid point's segment, keyed by the emitting atom's q
+// every term's key to its grid point's segment (in arbitrary order: the rank
+// came from an atomic)
+__global__ void k_ga_place(const int *npairs, const int *pg, const int *ppos, const int *pq, const int *goff,
+                           int *skey) {
+  const int np = *npairs;
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < np; w += gridDim.x * blockDim.x)
+    skey[goff[pg[w]] + ppos[w]] = pq[w];
+}
+
+// every term's value to its place in key (list) order inside its segment:
+// its rank is the number of smaller keys there (keys of a segment differ)
 template <class R>
-__global__ void k_ga_place(const int *npairs, const int *pg, const int *ppos, const int *pq, const R *pv,
-                           const int *goff, int *skey, R *sv) {
+__global__ void k_ga_order(const int *npairs, const int *pg, const int *pq, const R *pv, const int *goff,
+                           const int *skey, R *sv) {
   const int np = *npairs;
   for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < np; w += gridDim.x * blockDim.x) {
-    const int slot = goff[pg[w]] + ppos[w];
-    skey[slot] = pq[w];
-    sv[slot] = pv[w];
+    const int p = pg[w], key = pq[w];
+    const int s = goff[p], e = goff[p + 1];
+    int r = 0;
+    for (int j = s; j < e; j++) r += skey[j] < key ? 1 : 0;
+    sv[s + r] = pv[w];
   }
 }
 
-// mu(g) (+)= its terms in ascending key (list) order, four grid points per
-// item (a point holds few terms, so a selection over the group is cheap).
-// FP32 form: Kahan-compensated.
+// mu(g) (+)= its terms, already in key order, four grid points per item;
+// FP32 form: Kahan-compensated
 template <class R>
 struct MuAcc {   // FP64: plain running sum in order
   R m;
@@ -678,27 +689,7 @@ struct MuAcc<float> {
   __device__ __forceinline__ float get() const { return m; }
 };
 template <class R>
-__device__ __forceinline__ R ga_point_sum(R m0, int s, int n, const int *skey, const R *sv) {
-  MuAcc<R> acc;
-  acc.m = m0;
-  if (n == 1) {
-    acc.add(sv[s]);
-    return acc.get();
-  }
-  int last = -1;
-  for (int it = 0; it < n; it++) {
-    int best = 0x7fffffff, bi = s;
-    for (int q = s; q < s + n; q++) {
-      const int kq = skey[q];
-      if (kq > last && kq < best) { best = kq; bi = q; }
-    }
-    acc.add(sv[bi]);
-    last = best;
-  }
-  return acc.get();
-}
-template <class R>
-__global__ void k_ga_sum(size_t N, const int *goff, const int *skey, const R *sv, R *mu, bool first) {
+__global__ void k_ga_sum(size_t N, const int *goff, const R *sv, R *mu, bool first) {
   const size_t N4 = N / 4;
   for (size_t u = blockIdx.x * (size_t)blockDim.x + threadIdx.x; u < N4; u += (size_t)gridDim.x * blockDim.x) {
     const size_t p0 = 4 * u;
@@ -706,14 +697,13 @@ __global__ void k_ga_sum(size_t N, const int *goff, const int *skey, const R *sv
     const int o4 = goff[p0 + 4];
     const int st[5] = {o.x, o.y, o.z, o.w, o4};
     if (!first && o.x == o4) continue;
-    R m[4];
 #pragma unroll
-    for (int e = 0; e < 4; e++) m[e] = first ? R(0) : mu[p0 + e];
-#pragma unroll
-    for (int e = 0; e < 4; e++)
-      if (st[e + 1] > st[e]) m[e] = ga_point_sum(m[e], st[e], st[e + 1] - st[e], skey, sv);
-#pragma unroll
-    for (int e = 0; e < 4; e++) mu[p0 + e] = m[e];
+    for (int e = 0; e < 4; e++) {
+      MuAcc<R> acc;
+      acc.m = first ? R(0) : mu[p0 + e];
+      for (int j = st[e]; j < st[e + 1]; j++) acc.add(sv[j]);
+      mu[p0 + e] = acc.get();
+    }
   }
 }
 
@@ -1032,14 +1022,16 @@ void launch_ga_enum(int ng, const GridView &gv, const GAParams &ga, const double
   }                                                                                                           \
   void launch_ga_place(int64_t cap, const int *npairs, const int *pg, const int *ppos, const int *pq,         \
                        const RT *pv, const int *goff, int *skey, RT *sv, hipStream_t s) {                     \
-    hipLaunchKernelGGL(k_ga_place<RT>, dim3(nblk((size_t)cap)), dim3(TPB), 0, s, npairs, pg, ppos, pq, pv, goff, \
-                       skey, sv);                                                                            \
+    hipLaunchKernelGGL(k_ga_place, dim3(nblk((size_t)cap)), dim3(TPB), 0, s, npairs, pg, ppos, pq, goff, skey); \
+    HIPCHK(hipGetLastError());                                                                                \
+    hipLaunchKernelGGL(k_ga_order<RT>, dim3(nblk((size_t)cap)), dim3(TPB), 0, s, npairs, pg, pq, pv, goff, skey, sv); \
     HIPCHK(hipGetLastError());                                                                                \
   }                                                                                                           \
   void launch_ga_sum(size_t N, const int *goff, const int *skey, const RT *sv, RT *mu, bool first,           \
                      hipStream_t s) {                                                                         \
+    (void)skey;                                                                                               \
     if (N % 4) throw std::invalid_argument("grid size must be a multiple of 4");                              \
-    hipLaunchKernelGGL(k_ga_sum<RT>, dim3(nblk(N / 4)), dim3(TPB), 0, s, N, goff, skey, sv, mu, first);      \
+    hipLaunchKernelGGL(k_ga_sum<RT>, dim3(nblk(N / 4)), dim3(TPB), 0, s, N, goff, sv, mu, first);            \
     HIPCHK(hipGetLastError());                                                                                \
   }
 PHAFD_GA_DEF(double, double4)

@@ -81,9 +81,14 @@ void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const doubl
                      const double *cutsq, int *acnt, hipStream_t s);
 void launch_ga_enum(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
                     const int *aoff, int64_t cap, int *pg, int *pq, int *overflow, hipStream_t s);
-// per pair terms; the FP32 form (float fields, float4 terms) folds forces and
-// sums mu with Kahan compensation, energies stay FP64.  launch_ga_sum with
-// first: mu is written (terms summed from 0) instead of added to
+// launch_ga_enum / _pair / _fold: the atom side of the pass (pairs of each
+// atom in walk order at aoff[q]; per pair the mu record and the force term;
+// the force terms folded into gF in order); the FP32 form (float fields,
+// float4 terms) folds forces and sums mu with Kahan compensation, energies
+// stay FP64.  launch_ga_place puts
+// each term's key into its grid point's segment, then its value at its rank
+// in key order (sv); launch_ga_sum adds each segment in order (first: mu is
+// written, summed from 0, instead of added to)
 #define PHAFD_GA_DECL(RT, V4)                                                                                   \
   void launch_ga_pair(int64_t cap, const int *npairs, const GridViewT<RT> &gv, const GAParams &ga,             \
                       const double4 *gx, const int *order, const int *pg, const int *pq, V4 *ft, RT *pv,      \

@@ -117,9 +117,9 @@ struct phafd_hip_ctx {
   DBuf<double4> gx_build;             // gathered positions at the last build
   bool grid_built = false;            // grid list materialised for the last build
   bool eager_gridlist = false;        // build (and use) the stored grid list in the step
-  // grid-atom pairs: per-atom counts and offsets (bin order), per pair the
-  // grid point, atom position, rank, force term and mu term; per grid point
-  // counts and offsets; terms sorted by grid point (key, value)
+  // grid-atom pairs: per-atom counts and offsets (bin order); per pair the
+  // grid point, key (atom position), rank, force term and mu term; per grid
+  // point counts and offsets; the keys and values grouped by grid point
   DBuf<int> ga_acnt, ga_aoff, ga_pg, ga_pq, ga_ppos, ga_gcnt, ga_goff, ga_skey;
   DBuf<double4> ga_ft;
   DBuf<double> ga_pv, ga_sv;
