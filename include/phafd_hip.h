@@ -198,6 +198,31 @@ int phafd_hip_set_option(phafd_hip_ctx *ctx, int key, double value);
  * out3 = {total ms, launches, algorithmic bytes} accumulated so far. */
 int phafd_hip_probe(phafd_hip_ctx *ctx, int kernel_id, double *out3);
 
+/* ---- batched replicas (no counterpart in the reference, which runs one
+ * system per MPI job).  n independent systems of one grid advanced together:
+ * each member is a context configured through the calls above (fields,
+ * atoms, pairs, fixes, neighbor); members need one grid/conjugate/volfrac fix
+ * (shared mobility, temp, volFH, gamma; noise off) and one grid/floryhuggins
+ * fix (chi may differ per member).  The grid side of a step runs as one
+ * launch per pass over all members, captured in hipGraphs; the atom side per
+ * member on a few streams.  Each member's results equal, bit for bit, those
+ * of the same context run alone.  setup() also autotunes the members per
+ * batched launch and the number of member streams (choices that leave the
+ * arithmetic unchanged). */
+typedef struct phafd_hip_batch phafd_hip_batch;
+int phafd_hip_batch_create(const phafd_hip_desc *desc, int nmembers, phafd_hip_batch **out);
+/* member i (owned by the batch; do not destroy it) */
+int phafd_hip_batch_member(phafd_hip_batch *batch, int i, phafd_hip_ctx **member);
+int phafd_hip_batch_setup(phafd_hip_batch *batch, double dt);
+int phafd_hip_batch_run(phafd_hip_batch *batch, int nsteps);
+/* on = 0 launches the batched passes directly instead of replaying the graphs */
+int phafd_hip_batch_set_graphs(phafd_hip_batch *batch, int on);
+/* {members, members per launch, member streams, graphs on, gradient-block ms
+ *  for the four launch sizes tried, pair-pass ms for the four stream counts} */
+int phafd_hip_batch_info(phafd_hip_batch *batch, double *out12);
+int phafd_hip_batch_destroy(phafd_hip_batch *batch);
+const char *phafd_hip_batch_last_error(const phafd_hip_batch *batch);
+
 /* ---- stateless kernels (unit-level parity; allocate internally) */
 /* fftw_mpi_plan_dft_r2c_3d / _c2r_3d (src/grid.cpp:239-280), unnormalised;
  * c2r uses only Re() of the kx = 0 and kx = Nx/2 bins */

@@ -89,6 +89,14 @@ def lib():
         "phafd_hip_floryhuggins": ([ctypes.c_int64, _dp, _dp] + [ctypes.c_double] * 3 + [_lp, _lp], ctypes.c_int),
         "phafd_hip_spectral_update": ([ctypes.c_int] * 3 + [_dp, _dp, _dp] + [ctypes.c_double] * 5 + [_dp],
                                       ctypes.c_int),
+        "phafd_hip_batch_create": ([ctypes.POINTER(Desc), ctypes.c_int, ctypes.POINTER(V)], ctypes.c_int),
+        "phafd_hip_batch_member": ([V, ctypes.c_int, ctypes.POINTER(V)], ctypes.c_int),
+        "phafd_hip_batch_setup": ([V, ctypes.c_double], ctypes.c_int),
+        "phafd_hip_batch_run": ([V, ctypes.c_int], ctypes.c_int),
+        "phafd_hip_batch_set_graphs": ([V, ctypes.c_int], ctypes.c_int),
+        "phafd_hip_batch_info": ([V, _dp], ctypes.c_int),
+        "phafd_hip_batch_destroy": ([V], ctypes.c_int),
+        "phafd_hip_batch_last_error": ([V], ctypes.c_char_p),
         "phafd_make_unique_seed": ([ctypes.c_int], ctypes.c_int),
         "phafd_ic_noisy_constant": ([ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp],
                                     ctypes.c_int),
@@ -234,17 +242,22 @@ class System:
     """One whole-box subdomain on one GPU; method names follow the script
     commands of the reference (src/input.cpp:48-299)."""
 
-    def __init__(self, nx, ny, nz, boxlo, period, zghost=ZGHOST_REFERENCE, precision=FP64, device=0):
+    def __init__(self, nx, ny, nz, boxlo, period, zghost=ZGHOST_REFERENCE, precision=FP64, device=0,
+                 _member=None):
+        self.shape = (nz, ny, nx)
+        self.natoms = 0
+        if _member is not None:   # a batch member: the batch owns the context
+            self.h, self.owned = _member, False
+            return
         d = Desc(nx, ny, nz, _c3(boxlo), _c3(period), zghost, precision, device)
         h = ctypes.c_void_p()
         _check(lib().phafd_hip_create(ctypes.byref(d), ctypes.byref(h)))
-        self.h = h
-        self.shape = (nz, ny, nx)
-        self.natoms = 0
+        self.h, self.owned = h, True
 
     def close(self):
         if getattr(self, "h", None):
-            lib().phafd_hip_destroy(self.h)
+            if self.owned:
+                lib().phafd_hip_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -406,3 +419,58 @@ class System:
         self._c(lib().phafd_hip_timers(self.h, _d(o), 1 if reset else 0))
         keys = ["gradphi", "neighbor", "pair", "integrate", "fh_r2c", "update", "c2r_phi", "total"]
         return dict(zip(keys, o.tolist()))
+
+
+# ------------------------------------------------------------------ batches
+class Batch:
+    """Independent replicas of one grid advanced together (phafd_hip_batch_*);
+    member(i) is a System configured like any other."""
+
+    def __init__(self, nx, ny, nz, boxlo, period, nmembers, zghost=ZGHOST_REFERENCE, device=0):
+        d = Desc(nx, ny, nz, _c3(boxlo), _c3(period), zghost, FP64, device)
+        h = ctypes.c_void_p()
+        rc = lib().phafd_hip_batch_create(ctypes.byref(d), nmembers, ctypes.byref(h))
+        if rc < 0:
+            raise PhafdError(rc, (lib().phafd_hip_batch_last_error(None) or b"").decode())
+        self.h = h
+        self.members = []
+        for i in range(nmembers):
+            m = ctypes.c_void_p()
+            self._c(lib().phafd_hip_batch_member(self.h, i, ctypes.byref(m)))
+            self.members.append(System(nx, ny, nz, boxlo, period, _member=m))
+
+    def _c(self, rc):
+        if rc < 0:
+            raise PhafdError(rc, (lib().phafd_hip_batch_last_error(self.h) or b"").decode())
+        return rc
+
+    def member(self, i):
+        return self.members[i]
+
+    def setup(self, dt):
+        self._c(lib().phafd_hip_batch_setup(self.h, dt))
+
+    def run(self, nsteps):
+        self._c(lib().phafd_hip_batch_run(self.h, nsteps))
+
+    def set_graphs(self, on):
+        self._c(lib().phafd_hip_batch_set_graphs(self.h, 1 if on else 0))
+
+    def info(self):
+        o = np.zeros(12)
+        self._c(lib().phafd_hip_batch_info(self.h, _d(o)))
+        return {"members": int(o[0]), "members_per_launch": int(o[1]), "member_streams": int(o[2]),
+                "graphs": bool(o[3]), "tune_grad_ms": list(o[4:8]), "tune_pair_ms": list(o[8:12])}
+
+    def close(self):
+        if getattr(self, "h", None):
+            for m in self.members:
+                m.h = None
+            lib().phafd_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -2,6 +2,7 @@
 // the per-timestep hook sequence of Integrate::run (src/integrate.cpp:105-203)
 // and the C ABI declared in include/phafd_hip.h.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -77,6 +78,7 @@ struct phafd_hip_ctx {
   int zghost = 0;
   FFTPlan plan;
   size_t N = 0, M = 0;
+  bool own_stream = true;    // false for a batch member (the batch's stream)
   bool fp32 = false;         // PHAFD_FP32: fields, spectra and grid-atom terms in FP32
   FieldSet<double> fd;
   FieldSet<float> fs;
@@ -1022,15 +1024,19 @@ std::vector<T> download(phafd_hip_ctx *c, const T *d, size_t n) {
 
 }  // namespace
 
-// =================================================================== C ABI
-extern "C" {
+namespace {
 
-int phafd_hip_create(const phafd_hip_desc *desc, phafd_hip_ctx **out) {
-  if (!desc || !out) return PHAFD_E_ARG;
-  *out = nullptr;
-  auto c = std::make_unique<phafd_hip_ctx>();
-  c->device = desc->device;
-  int rc = guard(c.get(), [&] {
+// where a batch member's state lives: the batch's stream, its slab of fields
+// (member `index` at index * N reals / index * M spectra) and its FH counters
+struct MemberSlot {
+  hipStream_t stream;
+  FieldSet<double> *slab;
+  int index;
+  unsigned long long *counters;
+};
+
+// set up a context for one grid (slot: as member `index` of a batch)
+void init_ctx(phafd_hip_ctx *c, const phafd_hip_desc *desc, const MemberSlot *slot) {
     if (desc->precision != PHAFD_FP64 && desc->precision != PHAFD_FP32)
       throw std::invalid_argument("precision must be PHAFD_FP64 or PHAFD_FP32");
     c->fp32 = desc->precision == PHAFD_FP32;
@@ -1053,21 +1059,52 @@ int phafd_hip_create(const phafd_hip_desc *desc, phafd_hip_ctx **out) {
     c->zghost = desc->zghost;
     c->N = c->plan.N();
     c->M = c->plan.M();
-    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    if (c->fp32) c->fs.alloc(c->N, c->M);
-    else c->fd.alloc(c->N, c->M);
+    if (slot) {   // member of a batch: its stream, fields in the batch slab, counters in the batch's array
+      if (c->fp32) throw std::invalid_argument("batched replicas are FP64");
+      c->stream = slot->stream;
+      c->own_stream = false;
+      FieldSet<double> &F = c->fd;
+      const size_t r = (size_t)slot->index;
+      F.phi.wrap(slot->slab->phi.p + r * c->N, c->N);
+      F.mu.wrap(slot->slab->mu.p + r * c->N, c->N);
+      for (int d = 0; d < 3; d++) {
+        F.grad[d].wrap(slot->slab->grad[d].p + r * c->N, c->N);
+        F.ghat[d].wrap(slot->slab->ghat[d].p + r * c->M, c->M);
+      }
+      F.phihat.wrap(slot->slab->phihat.p + r * c->M, c->M);
+      F.muhat.wrap(slot->slab->muhat.p + r * c->M, c->M);
+      c->counters.wrap(slot->counters + 2 * r, 2);
+    } else {
+      HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      if (c->fp32) c->fs.alloc(c->N, c->M);
+      else c->fd.alloc(c->N, c->M);
+      c->counters.alloc(2);
+    }
     c->mass64.alloc(1);
-    c->counters.alloc(2);
     c->flags.alloc(4);
     c->esum.alloc(8);
     HIPCHK(hipHostMalloc((void **)&c->h_flags, 64, hipHostMallocDefault));
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
     for (int w = PHAFD_PHI; w <= PHAFD_GRADZ; w++)
-      fill_zero(c->fp32 ? (void *)field_ptr<float>(c.get(), w) : (void *)field_ptr<double>(c.get(), w),
+      fill_zero(c->fp32 ? (void *)field_ptr<float>(c, w) : (void *)field_ptr<double>(c, w),
                 c->real_bytes() * c->N, c->stream);
     fill_zero(c->counters.p, 16, c->stream);
     fill_zero(c->flags.p, 16, c->stream);
-    sync(c.get());
+  sync(c);
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+int phafd_hip_create(const phafd_hip_desc *desc, phafd_hip_ctx **out) {
+  if (!desc || !out) return PHAFD_E_ARG;
+  *out = nullptr;
+  auto c = std::make_unique<phafd_hip_ctx>();
+  c->device = desc->device;
+  int rc = guard(c.get(), [&] {
+    init_ctx(c.get(), desc, nullptr);
     return 0;
   });
   if (rc != 0) {
@@ -1084,7 +1121,7 @@ int phafd_hip_destroy(phafd_hip_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
 }
@@ -1686,5 +1723,365 @@ int phafd_hip_spectral_update(int nx, int ny, int nz, const double *period, doub
     return 0;
   });
 }
+
+}  // extern "C"
+
+// ================================================================ batches
+// Independent replicas of one grid advanced together.  The members are
+// ordinary contexts (configured through the calls above) whose fields live in
+// one slab; the grid side of a step runs as one launch per pass over all
+// members (captured in two hipGraphs), the atom side per member on a few
+// member streams.  Every member gives bit for bit what it gives run alone:
+// the batched passes run the same kernels on each member's slice.
+struct phafd_hip_batch {
+  std::string err;
+  int device = 0, n = 0;
+  size_t N = 0, M = 0;
+  hipStream_t stream = nullptr;                  // grid side
+  std::vector<hipStream_t> mstreams;             // atom side
+  FieldSet<double> slab;
+  DBuf<unsigned long long> counters;             // 2 per member
+  DBuf<double> chi_b;                            // chi per member
+  std::vector<std::unique_ptr<phafd_hip_ctx>> members;
+  hipEvent_t ev_grad = nullptr;
+  std::vector<hipEvent_t> ev_m;
+  hipGraph_t g_grad = nullptr, g_spec = nullptr;
+  hipGraphExec_t x_grad = nullptr, x_spec = nullptr;
+  bool setup_done = false, use_graphs = true, sep = false, zsplit = false;
+  int nstreams = 1, chunk = 1;
+  double tune_grad_ms[4] = {0, 0, 0, 0}, tune_pair_ms[4] = {0, 0, 0, 0};
+  UpdateParams u{};
+  SpecConst sc{};
+  double tv = 1;
+};
+
+namespace {
+
+template <class Fn>
+int bguard(phafd_hip_batch *b, Fn fn) {
+  try {
+    if (b) (void)hipSetDevice(b->device);
+    return fn();
+  } catch (const HipError &e) {
+    if (b) b->err = e.what();
+    return PHAFD_E_HIP;
+  } catch (const std::invalid_argument &e) {
+    if (b) b->err = e.what();
+    return PHAFD_E_ARG;
+  } catch (const std::exception &e) {
+    if (b) b->err = e.what();
+    std::string w = e.what();
+    if (w.find("NAN") != std::string::npos) return PHAFD_E_NAN;
+    return PHAFD_E_RUNTIME;
+  }
+}
+
+// gradient of every member (FixGridConjugate::start_of_step), chunk members
+// per launch
+void batch_grad(phafd_hip_batch *b, int chunk) {
+  const FFTPlan &p = b->members[0]->plan;
+  FieldSet<double> &S = b->slab;
+  for (int r0 = 0; r0 < b->n; r0 += chunk) {
+    Slab sl;
+    sl.n = std::min(chunk, b->n - r0);
+    sl.sr = b->N;
+    sl.sc = b->M;
+    const size_t oN = (size_t)r0 * b->N, oM = (size_t)r0 * b->M;
+    if (b->sep) {
+      fft_grad_sep(p, S.phi.p + oN, S.phihat.p + oM, S.ghat[1].p + oM, S.ghat[2].p + oM, S.grad[0].p + oN,
+                   S.grad[1].p + oN, S.grad[2].p + oN, b->sc, b->stream, sl);
+    } else {
+      fft_r2c(p, S.phi.p + oN, S.phihat.p + oM, b->stream, nullptr, sl);
+      grad_multiply(p, S.phihat.p + oM, S.ghat[0].p + oM, S.ghat[1].p + oM, S.ghat[2].p + oM, b->sc, b->stream, sl);
+      for (int d = 0; d < 3; d++)
+        fft_c2r_slab(p, S.ghat[d].p + oM, S.grad[d].p + oN, sl, (double)b->N, b->stream);
+    }
+  }
+}
+
+// Flory-Huggins fused into r2c(mu), the spectral update and c2r(phihat) of
+// every member (post_force .. post_final_integrate)
+void batch_spec(phafd_hip_batch *b, int chunk) {
+  const FFTPlan &p = b->members[0]->plan;
+  FieldSet<double> &S = b->slab;
+  for (int r0 = 0; r0 < b->n; r0 += chunk) {
+    Slab sl;
+    sl.n = std::min(chunk, b->n - r0);
+    sl.sr = b->N;
+    sl.sc = b->M;
+    const size_t oN = (size_t)r0 * b->N, oM = (size_t)r0 * b->M;
+    FHPre pre;
+    pre.on = true;
+    pre.phi = S.phi.p + oN;
+    pre.mu = S.mu.p + oN;
+    pre.tv = b->tv;
+    pre.counters = b->counters.p + 2 * r0;
+    pre.chi_b = b->chi_b.p + r0;
+    if (b->zsplit) {
+      fft_r2c_xy(p, S.mu.p + oN, S.muhat.p + oM, b->stream, &pre, sl);
+      fft_z_update(p, S.muhat.p + oM, S.phihat.p + oM, nullptr, b->u, nullptr, b->stream, sl);
+      fft_c2r_yx_slab(p, S.phihat.p + oM, S.phi.p + oN, sl, 0.0, b->stream);
+    } else {
+      fft_r2c(p, S.mu.p + oN, S.muhat.p + oM, b->stream, &pre, sl);
+      spectral_update(p, S.phihat.p + oM, S.muhat.p + oM, nullptr, b->u, nullptr, b->stream, sl);
+      fft_c2r_slab(p, S.phihat.p + oM, S.phi.p + oN, sl, 0.0, b->stream);
+    }
+  }
+}
+
+void batch_sync(phafd_hip_batch *b) {
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (auto s : b->mstreams) HIPCHK(hipStreamSynchronize(s));
+}
+
+void batch_assign_streams(phafd_hip_batch *b, int k) {
+  batch_sync(b);
+  for (int i = 0; i < b->n; i++) b->members[i]->stream = b->mstreams[i % k];
+  b->nstreams = k;
+}
+
+// time `fn` (enqueued work) over reps, after one warm call
+template <class Fn>
+double time_ms(phafd_hip_batch *b, int reps, Fn fn) {
+  fn();
+  batch_sync(b);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int r = 0; r < reps; r++) fn();
+  batch_sync(b);
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+}
+
+void destroy_graphs(phafd_hip_batch *b) {
+  if (b->x_grad) (void)hipGraphExecDestroy(b->x_grad);
+  if (b->x_spec) (void)hipGraphExecDestroy(b->x_spec);
+  if (b->g_grad) (void)hipGraphDestroy(b->g_grad);
+  if (b->g_spec) (void)hipGraphDestroy(b->g_spec);
+  b->x_grad = b->x_spec = nullptr;
+  b->g_grad = b->g_spec = nullptr;
+}
+
+template <class Fn>
+void capture(phafd_hip_batch *b, hipGraph_t *g, hipGraphExec_t *x, Fn fn) {
+  KernelProbe *saved = g_probe;   // no event records inside a graph
+  g_probe = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    fn();
+  } catch (...) {
+    hipGraph_t tmp;
+    (void)hipStreamEndCapture(b->stream, &tmp);
+    g_probe = saved;
+    throw;
+  }
+  HIPCHK(hipStreamEndCapture(b->stream, g));
+  g_probe = saved;
+  HIPCHK(hipGraphInstantiate(x, *g, nullptr, nullptr, 0));
+}
+
+void batch_steps(phafd_hip_batch *b, int nsteps) {
+  if (!b->setup_done) throw std::invalid_argument("call phafd_hip_batch_setup before running");
+  for (int it = 0; it < nsteps; it++) {
+    for (auto &m : b->members) m->timestep++;
+    if (b->use_graphs) HIPCHK(hipGraphLaunch(b->x_grad, b->stream));
+    else batch_grad(b, b->chunk);
+    HIPCHK(hipEventRecord(b->ev_grad, b->stream));
+    for (int i = 0; i < b->n; i++) {
+      phafd_hip_ctx *c = b->members[i].get();
+      HIPCHK(hipStreamWaitEvent(c->stream, b->ev_grad, 0));
+      phase_neigh(c);
+      phase_pair(c);
+      phase_initial(c);
+      HIPCHK(hipEventRecord(b->ev_m[i], c->stream));
+    }
+    for (int i = 0; i < b->n; i++) HIPCHK(hipStreamWaitEvent(b->stream, b->ev_m[i], 0));
+    if (b->use_graphs) HIPCHK(hipGraphLaunch(b->x_spec, b->stream));
+    else batch_spec(b, b->chunk);
+    for (int i = 0; i < b->n; i++) {
+      phafd_hip_ctx *c = b->members[i].get();
+      for (auto &f : c->fixes)
+        if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f);
+      if (c->neigh_set) {   // next step's rebuild decision (Neighbor::decide)
+        launch_fill_i32(1, c->flags.p, 0, c->stream);
+        launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->flags.p, c->stream);
+      }
+      HIPCHK(hipMemcpyAsync(c->h_flags + 8, c->flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->fd.phi.p, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    }
+    batch_sync(b);
+    for (int i = 0; i < b->n; i++) {   // the checks of src/integrate.cpp:190-201, per member
+      phafd_hip_ctx *c = b->members[i].get();
+      double phi0;
+      std::memcpy(&phi0, c->h_flags + 2, sizeof(double));
+      if (std::isnan(phi0)) throw std::runtime_error("NAN encountered in phi (member " + std::to_string(i) + ").");
+      if (c->h_flags[11]) throw std::runtime_error("grid-atom term capacity exceeded");
+      if (c->neigh_set) {
+        if (c->h_flags[8] & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
+        c->pending_decision = c->h_flags[8] & 1;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int phafd_hip_batch_create(const phafd_hip_desc *desc, int nmembers, phafd_hip_batch **out) {
+  if (!desc || !out) return PHAFD_E_ARG;
+  *out = nullptr;
+  auto b = std::make_unique<phafd_hip_batch>();
+  b->device = desc->device;
+  int rc = bguard(b.get(), [&] {
+    if (nmembers < 1) throw std::invalid_argument("a batch needs at least one member");
+    if (desc->precision != PHAFD_FP64) throw std::invalid_argument("batched replicas are FP64");
+    HIPCHK(hipSetDevice(desc->device));
+    b->n = nmembers;
+    b->N = (size_t)desc->nx * desc->ny * desc->nz;
+    b->M = (size_t)(desc->nx / 2 + 1) * desc->ny * desc->nz;
+    HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    b->slab.alloc(b->N * b->n, b->M * b->n);
+    b->counters.alloc(2 * (size_t)b->n);
+    fill_zero(b->counters.p, 16 * (size_t)b->n, b->stream);
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < b->n; i++) {
+      auto c = std::make_unique<phafd_hip_ctx>();
+      c->device = desc->device;
+      MemberSlot slot{b->stream, &b->slab, i, b->counters.p};
+      init_ctx(c.get(), desc, &slot);
+      b->members.push_back(std::move(c));
+    }
+    HIPCHK(hipEventCreateWithFlags(&b->ev_grad, hipEventDisableTiming));
+    b->ev_m.resize(b->n);
+    for (auto &e : b->ev_m) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return 0;
+  });
+  if (rc != 0) {
+    g_create_err = b->err;
+    return rc;
+  }
+  *out = b.release();
+  return 0;
+}
+
+int phafd_hip_batch_member(phafd_hip_batch *b, int i, phafd_hip_ctx **member) {
+  return bguard(b, [&] {
+    if (!member || i < 0 || i >= b->n) throw std::invalid_argument("no such member");
+    *member = b->members[i].get();
+    return 0;
+  });
+}
+
+int phafd_hip_batch_setup(phafd_hip_batch *b, double dt) {
+  return bguard(b, [&] {
+    std::vector<double> chi(b->n);
+    for (int i = 0; i < b->n; i++) {
+      phafd_hip_ctx *c = b->members[i].get();
+      int nconj = 0, nfh = 0;
+      const FixDef *cj = nullptr, *fh = nullptr;
+      for (auto &f : c->fixes) {
+        if (f->kind == FixDef::CONJ) { nconj++; cj = f.get(); }
+        if (f->kind == FixDef::FH) { nfh++; fh = f.get(); }
+        if (f->kind == FixDef::GRADPHI) throw std::invalid_argument("batch members take no grid/gradphi fix");
+      }
+      if (nconj != 1 || nfh != 1)
+        throw std::invalid_argument("each batch member needs one grid/conjugate/volfrac and one grid/floryhuggins fix");
+      if (cj->noise_mode != PHAFD_NOISE_OFF) throw std::invalid_argument("batch members run with noise off");
+      if (!c->fuse_fh || !c->fuse_spectral) throw std::invalid_argument("batch members keep the default options");
+      const int rcs = phafd_hip_setup(c, dt);
+      if (rcs != PHAFD_OK) throw std::runtime_error("member " + std::to_string(i) + ": " + c->err);
+      const UpdateParams u = update_params(c, *cj);
+      const double tv = fh->temp / fh->volFH;
+      if (i == 0) {
+        b->u = u;
+        b->tv = tv;
+        b->sc = spec_const(c);
+      } else if (std::memcmp(&u, &b->u, sizeof(u)) != 0 || tv != b->tv) {
+        throw std::invalid_argument("batch members share mobility, temp, volFH and gamma (chi may differ)");
+      }
+      chi[i] = fh->chi;
+    }
+    b->chi_b.alloc(b->n);
+    HIPCHK(hipMemcpy(b->chi_b.p, chi.data(), sizeof(double) * b->n, hipMemcpyHostToDevice));
+    phafd_hip_ctx *c0 = b->members[0].get();
+    b->sep = c0->fuse_spectral && fft_sep_grad_ok(c0->plan);
+    b->zsplit = spectral_fused(c0);
+    // autotune what does not change the arithmetic: members per batched
+    // launch (timed on the gradient block, which rewrites the same state) and
+    // member streams for the atom side (timed on the pair pass, likewise)
+    const int chunks[4] = {b->n, (b->n + 1) / 2, (b->n + 3) / 4, std::min(b->n, 8)};
+    double best = 1e30;
+    for (int k = 0; k < 4; k++) {
+      b->tune_grad_ms[k] = time_ms(b, 3, [&] { batch_grad(b, chunks[k]); });
+      if (b->tune_grad_ms[k] < best) { best = b->tune_grad_ms[k]; b->chunk = chunks[k]; }
+    }
+    const int maxs = std::min(b->n, 8);
+    while ((int)b->mstreams.size() < maxs) {
+      hipStream_t s;
+      HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      b->mstreams.push_back(s);
+    }
+    const int ks[4] = {1, std::min(2, maxs), std::min(4, maxs), maxs};
+    best = 1e30;
+    int bestk = 1;
+    for (int k = 0; k < 4; k++) {
+      batch_assign_streams(b, ks[k]);
+      b->tune_pair_ms[k] = time_ms(b, 2, [&] {
+        for (auto &m : b->members) phase_pair(m.get());
+      });
+      if (b->tune_pair_ms[k] < best) { best = b->tune_pair_ms[k]; bestk = ks[k]; }
+    }
+    batch_assign_streams(b, bestk);
+    batch_grad(b, b->chunk);   // the state the first step expects (start_of_step recomputes it anyway)
+    batch_sync(b);
+    destroy_graphs(b);
+    capture(b, &b->g_grad, &b->x_grad, [&] { batch_grad(b, b->chunk); });
+    capture(b, &b->g_spec, &b->x_spec, [&] { batch_spec(b, b->chunk); });
+    b->setup_done = true;
+    return 0;
+  });
+}
+
+int phafd_hip_batch_run(phafd_hip_batch *b, int nsteps) {
+  return bguard(b, [&] {
+    batch_steps(b, nsteps);
+    return 0;
+  });
+}
+
+int phafd_hip_batch_set_graphs(phafd_hip_batch *b, int on) {
+  return bguard(b, [&] {
+    b->use_graphs = on != 0;
+    return 0;
+  });
+}
+
+int phafd_hip_batch_info(phafd_hip_batch *b, double *out12) {
+  return bguard(b, [&] {
+    out12[0] = b->n;
+    out12[1] = b->chunk;
+    out12[2] = b->nstreams;
+    out12[3] = b->use_graphs ? 1 : 0;
+    for (int k = 0; k < 4; k++) out12[4 + k] = b->tune_grad_ms[k];
+    for (int k = 0; k < 4; k++) out12[8 + k] = b->tune_pair_ms[k];
+    return 0;
+  });
+}
+
+int phafd_hip_batch_destroy(phafd_hip_batch *b) {
+  if (!b) return 0;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto s : b->mstreams) (void)hipStreamSynchronize(s);
+  destroy_graphs(b);
+  for (auto &m : b->members) (void)phafd_hip_destroy(m.release());
+  if (b->ev_grad) (void)hipEventDestroy(b->ev_grad);
+  for (auto e : b->ev_m) if (e) (void)hipEventDestroy(e);
+  for (auto s : b->mstreams) (void)hipStreamDestroy(s);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+const char *phafd_hip_batch_last_error(const phafd_hip_batch *b) { return b ? b->err.c_str() : g_create_err.c_str(); }
 
 }  // extern "C"

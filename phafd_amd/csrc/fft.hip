@@ -122,8 +122,14 @@ __host__ __device__ __forceinline__ C mk2(R x, R y) { return make_float2(x, y); 
 
 // ------------------------------------------------------------ public API
 #define PHAFD_FFT_API(NS, RT, CT)                                                                          \
-  void fft_r2c(const FFTPlan &p, const RT *in, CT *out, hipStream_t s, const FHPreT<RT> *pre) {             \
-    NS::fft_r2c_p(p, in, out, s, pre);                                                                         \
+  void fft_r2c(const FFTPlan &p, const RT *in, CT *out, hipStream_t s, const FHPreT<RT> *pre, Slab sl) {    \
+    NS::fft_r2c_p(p, in, out, s, pre, sl);                                                                   \
+  }                                                                                                          \
+  void fft_c2r_slab(const FFTPlan &p, CT *in, RT *out, Slab sl, double divisor, hipStream_t s) {            \
+    NS::fft_c2r_slab_p(p, in, out, sl, (RT)divisor, s);                                                      \
+  }                                                                                                          \
+  void fft_c2r_yx_slab(const FFTPlan &p, CT *in, RT *out, Slab sl, double divisor, hipStream_t s) {         \
+    NS::fft_c2r_yx_slab_p(p, in, out, sl, (RT)divisor, s);                                                   \
   }                                                                                                          \
   void fft_c2r(const FFTPlan &p, CT *in, RT *out, double divisor, hipStream_t s) {                          \
     NS::fft_c2r_p(p, in, out, (RT)divisor, s);                                                                 \
@@ -132,16 +138,16 @@ __host__ __device__ __forceinline__ C mk2(R x, R y) { return make_float2(x, y); 
                      hipStream_t s) {                                                                        \
     NS::fft_c2r_batch_p(p, in, out, batch, (RT)divisor, s);                                                    \
   }                                                                                                          \
-  void fft_r2c_xy(const FFTPlan &p, const RT *in, CT *out, hipStream_t s, const FHPreT<RT> *pre) {          \
-    NS::fft_r2c_xy_p(p, in, out, s, pre);                                                                      \
+  void fft_r2c_xy(const FFTPlan &p, const RT *in, CT *out, hipStream_t s, const FHPreT<RT> *pre, Slab sl) { \
+    NS::fft_r2c_xy_p(p, in, out, s, pre, sl);                                                                \
   }                                                                                                          \
   void fft_grad_sep(const FFTPlan &p, const RT *phi, CT *ph, CT *gy_k, CT *gz_k, RT *gx, RT *gy, RT *gz,   \
-                    SpecConst sc, hipStream_t s) {                                                           \
-    NS::fft_grad_sep_p(p, phi, ph, gy_k, gz_k, gx, gy, gz, sc, s);                                            \
+                    SpecConst sc, hipStream_t s, Slab sl) {                                                  \
+    NS::fft_grad_sep_p(p, phi, ph, gy_k, gz_k, gx, gy, gz, sc, s, sl);                                       \
   }                                                                                                          \
   void fft_z_update(const FFTPlan &p, CT *mu, CT *ph, const CT *noise, UpdateParams u, const double *origin, \
-                    hipStream_t s) {                                                                         \
-    NS::fft_z_update_p(p, mu, ph, noise, u, origin, s);                                                       \
+                    hipStream_t s, Slab sl) {                                                                \
+    NS::fft_z_update_p(p, mu, ph, noise, u, origin, s, sl);                                                  \
   }                                                                                                          \
   void fft_c2r_yx_batch(const FFTPlan &p, CT *const *in, RT *const *out, int batch, double divisor,         \
                         hipStream_t s) {                                                                     \

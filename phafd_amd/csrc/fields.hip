@@ -20,13 +20,14 @@ __device__ __forceinline__ double flog(double x) { return log(x); }
 // K layout index = (kx*nz + kz)*ny + ky.
 template <class C>
 __global__ void k_grad_multiply(const C *__restrict__ ph, C *__restrict__ gx, C *__restrict__ gy,
-                                C *__restrict__ gz, int hx, int ny, int nz, SpecConst c) {
+                                C *__restrict__ gz, int hx, int ny, int nz, SpecConst c, Slab sl) {
   using R = typename C::value_type;
   const size_t M = (size_t)hx * nz * ny;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < M;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const int ky = (int)(i % ny);
-    const size_t t = i / ny;
+  for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < M * sl.n;
+       w += (size_t)gridDim.x * blockDim.x) {
+    const size_t loc = w % M, i = (w / M) * sl.sc + loc;
+    const int ky = (int)(loc % ny);
+    const size_t t = loc / ny;
     const int kz = (int)(t % nz);
     const int kx = (int)(t / nz);
     const C v = ph[i];
@@ -82,17 +83,23 @@ __device__ __forceinline__ void update_index(C *__restrict__ ph, const C *__rest
 
 template <class C>
 __global__ void k_update(C *__restrict__ ph, const C *__restrict__ mu, const C *__restrict__ noise, int hx,
-                         int ny, int nz, UpdateParams u, const double *origin) {
+                         int ny, int nz, UpdateParams u, const double *origin, Slab sl) {
   const size_t M = (size_t)hx * nz * ny;
-  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < M;
-       idx += (size_t)gridDim.x * blockDim.x)
-    update_index(ph, mu, noise, idx, hx, ny, nz, u, origin);
+  for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < M * sl.n;
+       w += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = w / M, off = r * sl.sc;
+    update_index(ph + off, mu + off, noise ? noise + off : nullptr, w % M, hx, ny, nz, u, origin ? origin + r : nullptr);
+  }
 }
 
 // only the planes kx = 0 and kx = Nx/2 (the bulk is fused into the z pass)
 template <class C>
 __global__ void k_update_edges(C *__restrict__ ph, const C *__restrict__ mu, const C *__restrict__ noise,
-                               int hx, int ny, int nz, UpdateParams u, const double *origin) {
+                               int hx, int ny, int nz, UpdateParams u, const double *origin, size_t sc) {
+  ph += blockIdx.y * sc;
+  mu += blockIdx.y * sc;
+  if (noise) noise += blockIdx.y * sc;
+  if (origin) origin += blockIdx.y;
   const size_t P = (size_t)nz * ny;
   for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < 2 * P;
        w += (size_t)gridDim.x * blockDim.x) {
@@ -104,7 +111,11 @@ __global__ void k_update_edges(C *__restrict__ ph, const C *__restrict__ mu, con
 // FixGridFloryHuggins::post_force, src/fixgrid_floryhuggins.cpp:65-85
 template <class R>
 __global__ void k_floryhuggins(size_t n, R *__restrict__ phi, R *__restrict__ mu, R tv, R chi,
-                               unsigned long long *counters) {
+                               unsigned long long *counters, size_t sr, const double *chi_b) {
+  phi += blockIdx.y * sr;
+  mu += blockIdx.y * sr;
+  counters += 2 * blockIdx.y;
+  if (chi_b) chi = (R)chi_b[blockIdx.y];
   __shared__ unsigned cnt[2];
   if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
   __syncthreads();
@@ -206,35 +217,38 @@ int grid_for(size_t n) {
 }
 
 template <class C>
-void grad_multiply_t(const FFTPlan &p, const C *phihat, C *gx, C *gy, C *gz, SpecConst c, hipStream_t s) {
+void grad_multiply_t(const FFTPlan &p, const C *phihat, C *gx, C *gy, C *gz, SpecConst c, hipStream_t s, Slab sl) {
+  if (sl.n > 1 && sl.sc < p.M()) throw std::invalid_argument("slab stride shorter than a spectrum");
   probe_begin(K_GRAD_MUL, s);
-  hipLaunchKernelGGL(k_grad_multiply<C>, dim3(grid_for(p.M())), dim3(TPB), 0, s, phihat, gx, gy, gz, p.hx, p.ny,
-                     p.nz, c);
-  probe_end(K_GRAD_MUL, s, 4.0 * sizeof(C) * p.M());
+  hipLaunchKernelGGL(k_grad_multiply<C>, dim3(grid_for(p.M() * sl.n)), dim3(TPB), 0, s, phihat, gx, gy, gz, p.hx,
+                     p.ny, p.nz, c, sl);
+  probe_end(K_GRAD_MUL, s, 4.0 * sizeof(C) * p.M() * sl.n);
   HIPCHK(hipGetLastError());
 }
 template <class C>
 void spectral_update_t(const FFTPlan &p, C *phihat, const C *muhat, const C *noise_k, UpdateParams u,
-                       const double *origin, hipStream_t s) {
+                       const double *origin, hipStream_t s, Slab sl) {
+  if (sl.n > 1 && sl.sc < p.M()) throw std::invalid_argument("slab stride shorter than a spectrum");
   probe_begin(K_UPDATE, s);
-  hipLaunchKernelGGL(k_update<C>, dim3(grid_for(p.M())), dim3(TPB), 0, s, phihat, muhat, noise_k, p.hx, p.ny,
-                     p.nz, u, origin);
-  probe_end(K_UPDATE, s, (noise_k ? 4.0 : 3.0) * sizeof(C) * p.M());
+  hipLaunchKernelGGL(k_update<C>, dim3(grid_for(p.M() * sl.n)), dim3(TPB), 0, s, phihat, muhat, noise_k, p.hx,
+                     p.ny, p.nz, u, origin, sl);
+  probe_end(K_UPDATE, s, (noise_k ? 4.0 : 3.0) * sizeof(C) * p.M() * sl.n);
   HIPCHK(hipGetLastError());
 }
 template <class C>
 void spectral_update_edges_t(const FFTPlan &p, C *phihat, const C *muhat, const C *noise_k, UpdateParams u,
-                             const double *origin, hipStream_t s) {
-  hipLaunchKernelGGL(k_update_edges<C>, dim3(grid_for(2 * (size_t)p.ny * p.nz)), dim3(TPB), 0, s, phihat, muhat,
-                     noise_k, p.hx, p.ny, p.nz, u, origin);
+                             const double *origin, hipStream_t s, Slab sl) {
+  hipLaunchKernelGGL(k_update_edges<C>, dim3(grid_for(2 * (size_t)p.ny * p.nz), sl.n), dim3(TPB), 0, s, phihat,
+                     muhat, noise_k, p.hx, p.ny, p.nz, u, origin, sl.sc);
   HIPCHK(hipGetLastError());
 }
 template <class R>
 void floryhuggins_t(size_t n, R *phi, R *mu, double tv, double chi, unsigned long long *counters,
-                    hipStream_t s) {
+                    hipStream_t s, Slab sl, const double *chi_b) {
   probe_begin(K_FH, s);
-  hipLaunchKernelGGL(k_floryhuggins<R>, dim3(grid_for(n)), dim3(TPB), 0, s, n, phi, mu, (R)tv, (R)chi, counters);
-  probe_end(K_FH, s, 3.0 * sizeof(R) * n);
+  hipLaunchKernelGGL(k_floryhuggins<R>, dim3(grid_for(n), sl.n), dim3(TPB), 0, s, n, phi, mu, (R)tv, (R)chi,
+                     counters, sl.sr, chi_b);
+  probe_end(K_FH, s, 3.0 * sizeof(R) * n * sl.n);
   HIPCHK(hipGetLastError());
 }
 template <class R>
@@ -253,20 +267,21 @@ void sum_fp64_t(size_t n, const R *x, double *out, hipStream_t s) {
 }  // namespace
 
 #define PHAFD_FIELD_DEF(RT, CT)                                                                              \
-  void grad_multiply(const FFTPlan &p, const CT *phihat, CT *gx, CT *gy, CT *gz, SpecConst c, hipStream_t s) { \
-    grad_multiply_t(p, phihat, gx, gy, gz, c, s);                                                            \
+  void grad_multiply(const FFTPlan &p, const CT *phihat, CT *gx, CT *gy, CT *gz, SpecConst c, hipStream_t s,  \
+                     Slab sl) {                                                                              \
+    grad_multiply_t(p, phihat, gx, gy, gz, c, s, sl);                                                        \
   }                                                                                                          \
   void spectral_update(const FFTPlan &p, CT *phihat, const CT *muhat, const CT *noise_k, UpdateParams u,    \
-                       const double *origin, hipStream_t s) {                                                \
-    spectral_update_t(p, phihat, muhat, noise_k, u, origin, s);                                              \
+                       const double *origin, hipStream_t s, Slab sl) {                                       \
+    spectral_update_t(p, phihat, muhat, noise_k, u, origin, s, sl);                                          \
   }                                                                                                          \
   void spectral_update_edges(const FFTPlan &p, CT *phihat, const CT *muhat, const CT *noise_k,              \
-                             UpdateParams u, const double *origin, hipStream_t s) {                          \
-    spectral_update_edges_t(p, phihat, muhat, noise_k, u, origin, s);                                        \
+                             UpdateParams u, const double *origin, hipStream_t s, Slab sl) {                 \
+    spectral_update_edges_t(p, phihat, muhat, noise_k, u, origin, s, sl);                                    \
   }                                                                                                          \
   void floryhuggins(size_t n, RT *phi, RT *mu, double tv, double chi, unsigned long long *counters,         \
-                    hipStream_t s) {                                                                         \
-    floryhuggins_t(n, phi, mu, tv, chi, counters, s);                                                        \
+                    hipStream_t s, Slab sl, const double *chi_b) {                                           \
+    floryhuggins_t(n, phi, mu, tv, chi, counters, s, sl, chi_b);                                             \
   }                                                                                                          \
   void reduce_energy_fh(size_t n, const RT *phi, const RT *gx, const RT *gy, const RT *gz, double tv,       \
                         double chi, double tvg, double *out2, hipStream_t s) {                               \

@@ -23,22 +23,31 @@ struct HipError : std::runtime_error {
 
 // ----------------------------------------------------------------- memory
 template <class T>
-struct DBuf {   // device buffer owned by the context
+struct DBuf {   // device buffer owned by the context (or a view into another's)
   T *p = nullptr;
   size_t n = 0;
+  bool own = true;
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
-  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), own(o.own) { o.p = nullptr; o.n = 0; }
   DBuf &operator=(DBuf &&o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    if (this != &o) { release(); p = o.p; n = o.n; own = o.own; o.p = nullptr; o.n = 0; }
     return *this;
   }
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && own) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    own = true;
+  }
+  // non-owning view of `count` elements at q (alloc() within it is a no-op)
+  void wrap(T *q, size_t count) {
+    release();
+    p = q;
+    n = count;
+    own = false;
   }
   void alloc(size_t count) {   // grow-only, contents not preserved
     if (count <= n && p) return;
@@ -89,6 +98,13 @@ struct FFTPlan {
   size_t N() const { return (size_t)nx * ny * nz; }
 };
 
+// A slab of n systems of one grid: real arrays sr, spectra sc elements apart
+// (replica r at base + r * stride); a single system is a slab of one.
+struct Slab {
+  int n = 1;
+  size_t sr = 0, sc = 0;
+};
+
 // pre-op fused into the load of the r2c x pass
 template <class R>
 struct FHPreT {
@@ -96,22 +112,27 @@ struct FHPreT {
   R *phi = nullptr;           // read, reflected values written back
   R *mu = nullptr;            // read, mu + FH written back
   double tv = 1, chi = 1;     // temp/volFH, chi
-  unsigned long long *counters = nullptr;   // [nless, ngreat]
+  unsigned long long *counters = nullptr;   // [nless, ngreat] (per replica in a slab)
+  const double *chi_b = nullptr;            // per-replica chi in a slab, or nullptr
 };
 using FHPre = FHPreT<double>;
 
 struct SpecConst;
 struct UpdateParams;
 #define PHAFD_FFT_DECL(RT, CT)                                                                         \
-  void fft_r2c(const FFTPlan &p, const RT *in, CT *out, hipStream_t s, const FHPreT<RT> *pre = nullptr); \
+  void fft_r2c(const FFTPlan &p, const RT *in, CT *out, hipStream_t s, const FHPreT<RT> *pre = nullptr, \
+               Slab sl = Slab());                                                                        \
+  void fft_c2r_slab(const FFTPlan &p, CT *in, RT *out, Slab sl, double divisor, hipStream_t s);          \
+  void fft_c2r_yx_slab(const FFTPlan &p, CT *in, RT *out, Slab sl, double divisor, hipStream_t s);       \
   void fft_c2r(const FFTPlan &p, CT *in, RT *out, double divisor, hipStream_t s);                        \
   void fft_c2r_batch(const FFTPlan &p, CT *const *in, RT *const *out, int batch, double divisor,         \
                      hipStream_t s);                                                                     \
-  void fft_r2c_xy(const FFTPlan &p, const RT *in, CT *out, hipStream_t s, const FHPreT<RT> *pre = nullptr); \
+  void fft_r2c_xy(const FFTPlan &p, const RT *in, CT *out, hipStream_t s, const FHPreT<RT> *pre = nullptr, \
+                  Slab sl = Slab());                                                                     \
   void fft_grad_sep(const FFTPlan &p, const RT *phi, CT *ph, CT *gy_k, CT *gz_k, RT *gx, RT *gy, RT *gz, \
-                    SpecConst sc, hipStream_t s);                                                        \
+                    SpecConst sc, hipStream_t s, Slab sl = Slab());                                      \
   void fft_z_update(const FFTPlan &p, CT *mu, CT *ph, const CT *noise, UpdateParams u,                   \
-                    const double *origin, hipStream_t s);                                                \
+                    const double *origin, hipStream_t s, Slab sl = Slab());                              \
   void fft_c2r_yx_batch(const FFTPlan &p, CT *const *in, RT *const *out, int batch, double divisor,      \
                         hipStream_t s);                                                                  \
   void spec_k_to_canon(const FFTPlan &p, const CT *k, CT *canon, hipStream_t s);                         \
@@ -142,13 +163,13 @@ struct UpdateParams {
 };
 #define PHAFD_FIELD_DECL(RT, CT)                                                                     \
   void grad_multiply(const FFTPlan &p, const CT *phihat, CT *gx, CT *gy, CT *gz, SpecConst c,        \
-                     hipStream_t s);                                                                 \
+                     hipStream_t s, Slab sl = Slab());                                               \
   void spectral_update(const FFTPlan &p, CT *phihat, const CT *muhat, const CT *noise_k,             \
-                       UpdateParams u, const double *origin, hipStream_t s);                         \
+                       UpdateParams u, const double *origin, hipStream_t s, Slab sl = Slab());       \
   void spectral_update_edges(const FFTPlan &p, CT *phihat, const CT *muhat, const CT *noise_k,       \
-                             UpdateParams u, const double *origin, hipStream_t s);                   \
+                             UpdateParams u, const double *origin, hipStream_t s, Slab sl = Slab()); \
   void floryhuggins(size_t n, RT *phi, RT *mu, double tv, double chi, unsigned long long *counters,  \
-                    hipStream_t s);                                                                  \
+                    hipStream_t s, Slab sl = Slab(), const double *chi_b = nullptr);                 \
   void reduce_energy_fh(size_t n, const RT *phi, const RT *gx, const RT *gy, const RT *gz, double tv, \
                         double chi, double tvg, double *out2, hipStream_t s);                        \
   void sum_fp64(size_t n, const RT *x, double *out, hipStream_t s);

@@ -39,7 +39,7 @@ def workload(n=32, box=None, nchains=6, nbeads=20, phi0=0.5, var=0.02, chi=3.0, 
                 skin=skin, dt=dt, zghost=zghost, starts=starts, ends=ends, ntypes=types, noise=noise)
 
 
-def apply(sysobj, w, is_oracle):
+def apply(sysobj, w, is_oracle, setup=True):
     s = sysobj
     s.set_field(0, w["phi"])
     s.set_atoms(w["xyz"], w["types"], tags=np.arange(len(w["xyz"]), dtype=np.int32), mols=w["mol"])
@@ -62,7 +62,8 @@ def apply(sysobj, w, is_oracle):
     if w["drag"]:
         s.fix_drag(w["starts"], w["ends"], 1.0)
     s.neighbor(w["skin"])
-    s.setup(w["dt"])
+    if setup:
+        s.setup(w["dt"])
     return s
 
 
