@@ -49,7 +49,13 @@ DESCR = {
     "c1": "plain FP64: 64^3 grid, 20x50-bead chains, gridatom/gaussian width 1 cut 3, chi 3, noise off",
     "c3": "512^3 grid, 20000x100-bead chains (2M beads), gridatom/LJish cut 2.5, lj/cut 2^(1/6), chi 2.5, "
           "noise off; FP64 and the FP32 form",
+    "c4": "batched replicas: 64 independent 128^3 systems, chi U[2,4], bead counts log-uniform 10k-200k in "
+          "chains of 50-200, phi0 mean U[0.3,0.5]; gridatom/gaussian width 1 cut 3, lj/cut 2^(1/6), noise off; "
+          "one batched launch per grid pass in hipGraphs, members per launch and atom streams autotuned",
 }
+# batched replicas (phafd_hip_batch_*): ranges the members are drawn from
+BATCH = dict(n=128, box=128.0, members=64, chi=(2.0, 4.0), beads=(1.0e4, 2.0e5), chain=(50, 200),
+             phi0=(0.3, 0.5), var=0.04, cut=3.0, nucwidth=1.0, ljish=False, zghost=1)
 
 
 def setup_inputs(wl):
@@ -60,7 +66,7 @@ def setup_inputs(wl):
     return phi, xyz, mol
 
 
-def configure(s, wl, phi, xyz, mol, is_oracle=False):
+def configure(s, wl, phi, xyz, mol, is_oracle=False, setup=True):
     nch, nb = wl["nchains"], wl["nbeads"]
     s.set_field(0, phi)
     s.set_atoms(xyz, np.zeros(len(xyz), np.int32), tags=np.arange(len(xyz), dtype=np.int32), mols=mol)
@@ -76,7 +82,27 @@ def configure(s, wl, phi, xyz, mol, is_oracle=False):
     st = (np.arange(nch) * nb).astype(np.int32)
     s.fix_chain(st, st + nb, 1.0, 100.0, 10.0, 1.0)
     s.neighbor(0.5)
-    s.setup(1e-3)
+    if setup:
+        s.setup(1e-3)
+
+
+def batch_members(bw, seed=2024):
+    """Per-member workloads of the batched-replica run (seeded draws)."""
+    import phafd_amd as P
+    rng = np.random.default_rng(seed)
+    n, box = bw["n"], bw["box"]
+    out = []
+    for i in range(bw["members"]):
+        chi = float(rng.uniform(*bw["chi"]))
+        target = float(np.exp(rng.uniform(np.log(bw["beads"][0]), np.log(bw["beads"][1]))))
+        nb = int(rng.integers(bw["chain"][0], bw["chain"][1] + 1))
+        nch = max(1, int(round(target / nb)))
+        phi0 = float(rng.uniform(*bw["phi0"]))
+        wl = dict(bw, nchains=nch, nbeads=nb, chi=chi, phi0=phi0)
+        phi = P.ic_noisy_constant((n, n, n), phi0, bw["var"], 1234 + i)
+        xyz, mol = P.ic_random_chains(nch, nb, 1.0, 1.0, [0, 0, 0], [box] * 3, 42 + i)
+        out.append((wl, phi, xyz, mol))
+    return out
 
 
 def cpu_baseline(wl, phi, xyz, mol, steps):
@@ -133,12 +159,97 @@ def read_traffic(kernel, workload):
     return None
 
 
+def bench_batch(args):
+    """Batched replicas: the batch timed over `steps`, then the same members
+    run one after another as solo systems (the state of every member is
+    compared bit for bit with its solo run), then a few eager steps of the
+    batch under the kernel probe for the roofline."""
+    import phafd_amd as P
+    bw = BATCH
+    n, box = bw["n"], bw["box"]
+    members = batch_members(bw)
+    b = P.Batch(n, n, n, [0, 0, 0], [box] * 3, len(members), zghost=bw["zghost"])
+    for i, (wl, phi, xyz, mol) in enumerate(members):
+        configure(b.member(i), wl, phi, xyz, mol, setup=False)
+    b.setup(1e-3)
+    info = b.info()
+    b.run(args.warmup)
+    t0 = time.perf_counter()
+    b.run(args.steps)
+    elapsed = time.perf_counter() - t0
+    got = []
+    for i in range(len(members)):
+        m = b.member(i)
+        got.append((m.get_field(P.PHI), m.get_atoms()[0], m.get_forces()))
+    b.set_graphs(False)
+    m0 = b.member(0)
+    m0.probe(args.probe)
+    b.run(5)
+    pms, plaunch, pbytes = m0.probe(None)
+    b.close()
+
+    solo = 0.0
+    same = 0
+    for i, (wl, phi, xyz, mol) in enumerate(members):
+        s = P.System(n, n, n, [0, 0, 0], [box] * 3, zghost=bw["zghost"])
+        configure(s, wl, phi, xyz, mol)
+        s.run(args.warmup)
+        t1 = time.perf_counter()
+        s.run(args.steps)
+        solo += time.perf_counter() - t1
+        ref = (s.get_field(P.PHI), s.get_atoms()[0], s.get_forces())
+        same += all(np.array_equal(a, r) for a, r in zip(got[i], ref))
+        s.close()
+
+    N = n ** 3 * len(members)
+    beads = [wl["nchains"] * wl["nbeads"] for wl, _, _, _ in members]
+    avg_ms = pms / max(plaunch, 1)
+    bytes_per_launch = pbytes / max(plaunch, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if plaunch else None
+    peak = 8.0e3
+    out = {
+        "metric": METRIC,
+        "value": N * args.steps / elapsed,
+        "unit": "grid-pt·steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": DESCR["c4"], "members": len(members), "grid": [n, n, n],
+                   "beads_total": int(sum(beads)), "beads_min": int(min(beads)), "beads_max": int(max(beads)),
+                   "members_per_launch": info["members_per_launch"], "member_streams": info["member_streams"],
+                   "autotune_grad_ms": info["tune_grad_ms"], "autotune_pair_ms": info["tune_pair_ms"]},
+        "beads_steps_per_s": sum(beads) * args.steps / elapsed,
+        "sequential_solo": {"value": N * args.steps / solo, "ms_per_step": 1e3 * solo / args.steps,
+                            "batch_speedup": solo / elapsed},
+        "members_bit_identical_to_solo": f"{same}/{len(members)}",
+        "roofline": {"bound": "hbm", "kernel": args.probe, "achieved": achieved, "peak": peak,
+                     "unit": "GB/s", "frac": (achieved / peak) if achieved else None, "traffic": None,
+                     "avg_launch_ms": avg_ms, "launches": plaunch,
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "note": "eager batched launches (graphs off) after the timed run"},
+    }
+    if not args.no_cpu_baseline:
+        wl, phi, xyz, mol = members[0]
+        t = cpu_baseline(wl, phi, xyz, mol, args.cpu_steps)
+        out["cpu_baseline"] = {"value": n ** 3 * args.cpu_steps / t, "unit": "grid-pt·steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{args.cpu_steps} full timesteps of member 0 ({beads[0]} beads) after "
+                                         f"setup, CPU FP64 oracle, one thread ({t:.1f} s)"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4"])
     ap.add_argument("--probe", default="c2r_x", help="kernel timed with HIP events for the roofline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (counter calibration) and exit")
@@ -154,6 +265,10 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, 5)
+        return
+
+    if args.workload == "c4":
+        bench_batch(args)
         return
 
     import phafd_amd as P

@@ -1121,7 +1121,10 @@ int phafd_hip_destroy(phafd_hip_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
-  if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream && c->own_stream) {
+    scan_forget_stream(c->stream);
+    (void)hipStreamDestroy(c->stream);
+  }
   delete c;
   return 0;
 }
@@ -1599,7 +1602,11 @@ struct Scratch {
     plan.init(nx, ny, nz);
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   }
-  ~Scratch() { if (s) (void)hipStreamDestroy(s); }
+  ~Scratch() {
+    if (!s) return;
+    scan_forget_stream(s);
+    (void)hipStreamDestroy(s);
+  }
 };
 template <class Fn>
 int sguard(Fn fn) {
@@ -1908,6 +1915,7 @@ void batch_steps(phafd_hip_batch *b, int nsteps) {
       HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->fd.phi.p, sizeof(double), hipMemcpyDeviceToHost, b->stream));
     }
     batch_sync(b);
+    probe_collect();
     for (int i = 0; i < b->n; i++) {   // the checks of src/integrate.cpp:190-201, per member
       phafd_hip_ctx *c = b->members[i].get();
       double phi0;
@@ -2076,8 +2084,14 @@ int phafd_hip_batch_destroy(phafd_hip_batch *b) {
   for (auto &m : b->members) (void)phafd_hip_destroy(m.release());
   if (b->ev_grad) (void)hipEventDestroy(b->ev_grad);
   for (auto e : b->ev_m) if (e) (void)hipEventDestroy(e);
-  for (auto s : b->mstreams) (void)hipStreamDestroy(s);
-  if (b->stream) (void)hipStreamDestroy(b->stream);
+  for (auto s : b->mstreams) {
+    scan_forget_stream(s);
+    (void)hipStreamDestroy(s);
+  }
+  if (b->stream) {
+    scan_forget_stream(b->stream);
+    (void)hipStreamDestroy(b->stream);
+  }
   delete b;
   return 0;
 }

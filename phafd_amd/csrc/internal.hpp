@@ -189,6 +189,7 @@ void fill_zero(void *p, size_t bytes, hipStream_t s);
 // out[n] = total
 void scan_exclusive_i32_to_i64(const int *in, int64_t *out, size_t n, hipStream_t s);
 void scan_exclusive_i32(const int *in, int *out, size_t n, hipStream_t s);   // out[n] = total
+void scan_forget_stream(hipStream_t s);   // frees the scan workspace of a stream about to be destroyed
 
 // ----------------------------------------------------------- atom kernels
 struct BinGeom {

@@ -2,6 +2,10 @@
 // offsets, counting sorts).  Three phases: per-block sums, scan of the block
 // sums (recursive), per-block scan plus block offset.  4096 items per block,
 // 16 consecutive items per thread.
+#include <map>
+#include <memory>
+#include <mutex>
+
 #include "internal.hpp"
 
 namespace phafd {
@@ -102,11 +106,25 @@ __global__ __launch_bounds__(ST) void k_scan_apply(const TI *__restrict__ in, si
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == ST - 1) out[n] = (TO)(b0 + total);
 }
 
-// workspace per recursion level (grow-only; one context stream at a time)
-DBuf<int64_t> g_ws_sum[4], g_ws_off[4];
+// Block sums and offsets per recursion level, one workspace per stream:
+// scans on one stream run in order and may share it, scans on different
+// streams (the members of a batch) must not.  Grow-only, after the stream
+// has drained, so no launch still queued reads a freed workspace.
+struct ScanWs {
+  DBuf<int64_t> sum[4], off[4];
+};
+std::mutex g_ws_mu;
+std::map<hipStream_t, std::unique_ptr<ScanWs>> g_ws;
+
+ScanWs &workspace(hipStream_t s) {
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  std::unique_ptr<ScanWs> &w = g_ws[s];
+  if (!w) w = std::make_unique<ScanWs>();
+  return *w;
+}
 
 template <class TI, class TO>
-void scan_impl(const TI *in, TO *out, size_t n, hipStream_t s, int level) {
+void scan_impl(const TI *in, TO *out, size_t n, hipStream_t s, int level, ScanWs &ws) {
   if (n == 0) {
     // out[0] = 0
     hipLaunchKernelGGL((k_scan_apply<TI, TO>), dim3(1), dim3(ST), 0, s, in, (size_t)0,
@@ -122,23 +140,34 @@ void scan_impl(const TI *in, TO *out, size_t n, hipStream_t s, int level) {
     return;
   }
   if (level >= 4) throw std::runtime_error("scan too large");
-  g_ws_sum[level].alloc(nb + 1);
-  g_ws_off[level].alloc(nb + 1);
-  hipLaunchKernelGGL((k_reduce<TI>), dim3((unsigned)nb), dim3(ST), 0, s, in, n, g_ws_sum[level].p);
+  if (ws.sum[level].n < nb + 1 || ws.off[level].n < nb + 1) HIPCHK(hipStreamSynchronize(s));   // before a regrow
+  ws.sum[level].alloc(nb + 1);
+  ws.off[level].alloc(nb + 1);
+  hipLaunchKernelGGL((k_reduce<TI>), dim3((unsigned)nb), dim3(ST), 0, s, in, n, ws.sum[level].p);
   HIPCHK(hipGetLastError());
-  scan_impl<int64_t, int64_t>(g_ws_sum[level].p, g_ws_off[level].p, nb, s, level + 1);
+  scan_impl<int64_t, int64_t>(ws.sum[level].p, ws.off[level].p, nb, s, level + 1, ws);
   hipLaunchKernelGGL((k_scan_apply<TI, TO>), dim3((unsigned)nb), dim3(ST), 0, s, in, n,
-                     (const int64_t *)g_ws_off[level].p, out);
+                     (const int64_t *)ws.off[level].p, out);
   HIPCHK(hipGetLastError());
 }
 
 }  // namespace
 
 void scan_exclusive_i32_to_i64(const int *in, int64_t *out, size_t n, hipStream_t s) {
-  scan_impl<int, int64_t>(in, out, n, s, 0);
+  scan_impl<int, int64_t>(in, out, n, s, 0, workspace(s));
+}
+void scan_forget_stream(hipStream_t s) {
+  std::unique_ptr<ScanWs> w;
+  {
+    std::lock_guard<std::mutex> lock(g_ws_mu);
+    auto it = g_ws.find(s);
+    if (it == g_ws.end()) return;
+    w = std::move(it->second);
+    g_ws.erase(it);
+  }
 }
 void scan_exclusive_i32(const int *in, int *out, size_t n, hipStream_t s) {
-  scan_impl<int, int>(in, out, n, s, 0);
+  scan_impl<int, int>(in, out, n, s, 0, workspace(s));
 }
 
 }  // namespace phafd
