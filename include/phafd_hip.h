@@ -223,6 +223,78 @@ int phafd_hip_batch_info(phafd_hip_batch *batch, double *out12);
 int phafd_hip_batch_destroy(phafd_hip_batch *batch);
 const char *phafd_hip_batch_last_error(const phafd_hip_batch *batch);
 
+/* ---- learned surrogate (no counterpart in the reference).  A three-level
+ * 3D convolutional U-Net (16/32/64 channels, 3^3 kernels, periodic; BF16
+ * weights and activations, FP32 accumulation on the MFMA units) correcting a
+ * spectral block integrator over K steps; trained on the GPU by the
+ * library's own forward and backward kernels.  Grids: nx % 64 == 0,
+ * ny % 32 == 0, nz % 4 == 0.  Network input: four channels per grid point,
+ * [4][Nz][Ny][Nx] floats: (phi0 - mean phi0) / norm[0], rho0 / norm[1],
+ * rho1 / norm[1], (mean phi0 - norm[2]) / norm[3]; rho0, rho1 being the bead
+ * density (phafd_hip_density) at the block's start and end; output:
+ * (phi after K steps - base) / norm[4], base from phafd_hip_surrogate_base.
+ * Parameters: per layer the PyTorch Conv3d weight [cout][cin][k][k][k] then
+ * the bias, layers e1 (4->16, k 3), e1b (16->16), e2 (16->32, at 1/2
+ * resolution), b (32->64, 1/4), q2 (64->32, k 1), d2 (32->32, 1/2; input
+ * nearest-upsampled q2 plus e2's output), q1 (32->16, k 1), d1 (16->16,
+ * full; input upsampled q1 plus e1b's output), head (16->1, k 1); relu after
+ * every k 3 layer; 2^3 average pooling down. */
+typedef struct phafd_unet phafd_unet;
+int phafd_unet_create(int device, unsigned long long seed, phafd_unet **out);
+int phafd_unet_destroy(phafd_unet *net);
+const char *phafd_unet_last_error(const phafd_unet *net);
+int phafd_unet_nparams(phafd_unet *net, int64_t *n);
+int phafd_unet_get_params(phafd_unet *net, float *params);
+int phafd_unet_set_params(phafd_unet *net, const float *params);   /* also resets the optimiser */
+/* norm5 = {phi scale, rho scale, mean reference, mean scale, output scale} */
+int phafd_unet_set_norm(phafd_unet *net, const double *norm5);
+int phafd_unet_get_norm(phafd_unet *net, double *norm5);
+/* out[Nz][Ny][Nx] of one input; the input is kept for phafd_unet_backward */
+int phafd_unet_forward(phafd_unet *net, int nx, int ny, int nz, const float *in4, float *out);
+/* adds the gradient of sum(dout * out) with respect to the parameters to
+ * the gradient buffer (read, and optionally zeroed, by phafd_unet_get_grads) */
+int phafd_unet_backward(phafd_unet *net, const float *dout);
+int phafd_unet_get_grads(phafd_unet *net, float *grads, int zero);
+/* training set resident on the device: nsamples inputs (in4 layout) and
+ * targets ([Nz][Ny][Nx]) */
+int phafd_unet_data_alloc(phafd_unet *net, int nx, int ny, int nz, int nsamples);
+int phafd_unet_data_put(phafd_unet *net, int i, const float *in4, const float *target);
+/* nsteps Adam steps (beta 0.9, 0.999) on minibatches of `batch` samples
+ * drawn from samples [0, ntrain); losses[k]: minibatch mean square error */
+int phafd_unet_train(phafd_unet *net, int nsteps, int batch, int ntrain, double lr, double *losses);
+int phafd_unet_eval(phafd_unet *net, int i0, int count, double *losses);
+
+/* bead density rho[Nz][Ny][Nx] of the context's atoms: the sum over atoms of
+ * the grid-atom kernel value e within its range (PairGridAtomGaussian /
+ * LJish, the factor of 2 (phi - phi_t) in the mu term,
+ * src/pair_gridatom_gaussian.cpp:95-101) */
+int phafd_hip_density(phafd_hip_ctx *ctx, float *rho);
+/* The plain update (src/conjugate_volfrac.cpp:89-133) per Fourier mode is
+ * phi <- G phi - c n with c = M q^2 dt, G = 1 - c (f'' + T/v gamma q^2),
+ * n = mu - f'' phi, mu = Flory-Huggins + 2 (phi - phi_t) rho,
+ * f'' = T/v (1/m + 1/(1-m) - 2 chi) + 2 mean(rho0), m = mean(phi0); over K
+ * steps phi_K = G^K phi_0 - c sum_k G^(K-1-k) n_k.
+ * base[Nz][Ny][Nx]: that sum with n held at its block-start value n0
+ * (density rho0): base = G^K phi0 - c (1 + G + ... + G^(K-1)) n0
+ * (exponential Euler over the block). */
+int phafd_hip_surrogate_base(phafd_hip_ctx *ctx, const double *phi0, const float *rho0, int K, double *base);
+/* the check of a surrogate block: out2 = {|R| / |phi1 - phi0|, mass drift},
+ * R = phi1 - G^K phi0 + c (A n0 + B n1) in Fourier space (n0 with rho0, n1
+ * at phi1 with rho1, A = sum_k G^(K-1-k) (1 - k/K), B = sum_k G^(K-1-k) k/K):
+ * the K-step equation with n linear across the block */
+int phafd_hip_block_residual(phafd_hip_ctx *ctx, const double *phi0, const double *phi1, const float *rho0,
+                             const float *rho1, int K, double *out2);
+/* nsteps (a multiple of K) in blocks of K: the atoms take K steps (lj/cut and
+ * chains every step, the grid-atom force of the block-start field held),
+ * then phi after K steps is predicted as the base plus the network's output
+ * times norm[4] (mean removed).  Accepted when the block residual <= tol and
+ * the mass drift <= 1e-12; otherwise the atoms are put back and the block
+ * runs as K plain steps.  stats8 = {blocks, accepted, fallbacks, max
+ * residual, last residual, max |mass drift|, ms in the network path, ms in
+ * total}.  FP64 contexts with one conjugate and one Flory-Huggins fix, noise
+ * off, at most one grid-atom style with one phi_t. */
+int phafd_hip_run_surrogate(phafd_hip_ctx *ctx, phafd_unet *net, int nsteps, int K, double tol, double *stats8);
+
 /* ---- stateless kernels (unit-level parity; allocate internally) */
 /* fftw_mpi_plan_dft_r2c_3d / _c2r_3d (src/grid.cpp:239-280), unnormalised;
  * c2r uses only Re() of the kx = 0 and kx = Nx/2 bins */

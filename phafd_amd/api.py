@@ -26,6 +26,7 @@ ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "NaN in phi", -4: "runtim
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _lp = ctypes.POINTER(ctypes.c_int64)
+_fp = ctypes.POINTER(ctypes.c_float)
 _lib = None
 
 
@@ -97,6 +98,25 @@ def lib():
         "phafd_hip_batch_info": ([V, _dp], ctypes.c_int),
         "phafd_hip_batch_destroy": ([V], ctypes.c_int),
         "phafd_hip_batch_last_error": ([V], ctypes.c_char_p),
+        "phafd_unet_create": ([ctypes.c_int, ctypes.c_ulonglong, ctypes.POINTER(V)], ctypes.c_int),
+        "phafd_unet_destroy": ([V], ctypes.c_int),
+        "phafd_unet_last_error": ([V], ctypes.c_char_p),
+        "phafd_unet_nparams": ([V, _lp], ctypes.c_int),
+        "phafd_unet_get_params": ([V, _fp], ctypes.c_int),
+        "phafd_unet_set_params": ([V, _fp], ctypes.c_int),
+        "phafd_unet_set_norm": ([V, _dp], ctypes.c_int),
+        "phafd_unet_get_norm": ([V, _dp], ctypes.c_int),
+        "phafd_unet_forward": ([V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp, _fp], ctypes.c_int),
+        "phafd_unet_backward": ([V, _fp], ctypes.c_int),
+        "phafd_unet_get_grads": ([V, _fp, ctypes.c_int], ctypes.c_int),
+        "phafd_unet_data_alloc": ([V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "phafd_unet_data_put": ([V, ctypes.c_int, _fp, _fp], ctypes.c_int),
+        "phafd_unet_train": ([V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp], ctypes.c_int),
+        "phafd_unet_eval": ([V, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
+        "phafd_hip_density": ([V, _fp], ctypes.c_int),
+        "phafd_hip_block_residual": ([V, _dp, _dp, _fp, _fp, ctypes.c_int, _dp], ctypes.c_int),
+        "phafd_hip_surrogate_base": ([V, _dp, _fp, ctypes.c_int, _dp], ctypes.c_int),
+        "phafd_hip_run_surrogate": ([V, V, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp], ctypes.c_int),
         "phafd_make_unique_seed": ([ctypes.c_int], ctypes.c_int),
         "phafd_ic_noisy_constant": ([ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp],
                                     ctypes.c_int),
@@ -130,6 +150,10 @@ def exported_symbols():
 
 def _d(a):
     return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _f(a):
+    return None if a is None else a.ctypes.data_as(_fp)
 
 
 def _i(a):
@@ -399,6 +423,40 @@ class System:
         self._c(lib().phafd_hip_energy(self.h, _d(o)))
         return o
 
+    def density(self):
+        """Bead density (sum of the grid-atom kernel values), float32 (Nz, Ny, Nx)."""
+        r = np.zeros(self.shape, dtype=np.float32)
+        self._c(lib().phafd_hip_density(self.h, _f(r)))
+        return r
+
+    def block_residual(self, phi0, phi1, rho0, rho1, K):
+        """(residual, mass drift) of phi1 as phi0 advanced by K steps, bead
+        densities rho0 and rho1 at the block's start and end."""
+        a = np.ascontiguousarray(phi0, dtype=np.float64)
+        b = np.ascontiguousarray(phi1, dtype=np.float64)
+        r0 = np.ascontiguousarray(rho0, dtype=np.float32)
+        r1 = np.ascontiguousarray(rho1, dtype=np.float32)
+        o = np.zeros(2)
+        self._c(lib().phafd_hip_block_residual(self.h, _d(a), _d(b), _f(r0), _f(r1), int(K), _d(o)))
+        return float(o[0]), float(o[1])
+
+    def surrogate_base(self, phi0, rho0, K):
+        """phi0 advanced K steps by exponential Euler (block-start density)."""
+        a = np.ascontiguousarray(phi0, dtype=np.float64)
+        r0 = np.ascontiguousarray(rho0, dtype=np.float32)
+        out = np.zeros(self.shape)
+        self._c(lib().phafd_hip_surrogate_base(self.h, _d(a), _f(r0), int(K), _d(out)))
+        return out
+
+    def run_surrogate(self, net, nsteps, K, tol):
+        """Blocks of K steps advanced by the network with the plain step as the
+        check and fallback; returns the block statistics."""
+        o = np.zeros(8)
+        self._c(lib().phafd_hip_run_surrogate(self.h, net.h, int(nsteps), int(K), float(tol), _d(o)))
+        keys = ["blocks", "accepted", "fallbacks", "max_residual", "last_residual", "max_mass_drift",
+                "ms_network_path", "ms_total"]
+        return dict(zip(keys, o.tolist()))
+
     def set_option(self, key, value):
         self._c(lib().phafd_hip_set_option(self.h, key, float(value)))
 
@@ -474,3 +532,116 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------- surrogate
+class UNet:
+    """The surrogate network (phafd_unet_*): parameters in PyTorch Conv3d
+    order per layer (LAYERS), training and inference on the GPU."""
+
+    # (name, cin, cout, kernel size, resolution level)
+    LAYERS = [("e1", 4, 16, 3, 1), ("e1b", 16, 16, 3, 1), ("e2", 16, 32, 3, 2), ("b", 32, 64, 3, 3),
+              ("q2", 64, 32, 1, 3), ("d2", 32, 32, 3, 2), ("q1", 32, 16, 1, 2), ("d1", 16, 16, 3, 1),
+              ("head", 16, 1, 1, 1)]
+
+    def __init__(self, seed=1, device=0):
+        h = ctypes.c_void_p()
+        rc = lib().phafd_unet_create(device, seed, ctypes.byref(h))
+        if rc < 0:
+            raise PhafdError(rc, (lib().phafd_unet_last_error(None) or b"").decode())
+        self.h = h
+        n = ctypes.c_int64()
+        self._c(lib().phafd_unet_nparams(self.h, ctypes.byref(n)))
+        self.nparams = int(n.value)
+
+    def _c(self, rc):
+        if rc < 0:
+            raise PhafdError(rc, (lib().phafd_unet_last_error(self.h) or b"").decode())
+        return rc
+
+    def get_params(self):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        self._c(lib().phafd_unet_get_params(self.h, _f(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.nparams:
+            raise ValueError(f"expected {self.nparams} parameters")
+        self._c(lib().phafd_unet_set_params(self.h, _f(p)))
+
+    def set_norm(self, norm5):
+        a = np.ascontiguousarray(norm5, dtype=np.float64)
+        self._c(lib().phafd_unet_set_norm(self.h, _d(a)))
+
+    def get_norm(self):
+        a = np.zeros(5)
+        self._c(lib().phafd_unet_get_norm(self.h, _d(a)))
+        return a
+
+    def forward(self, in4):
+        """in4: float32 (4, Nz, Ny, Nx) -> float32 (Nz, Ny, Nx)."""
+        x = np.ascontiguousarray(in4, dtype=np.float32)
+        _, nz, ny, nx = x.shape
+        out = np.zeros((nz, ny, nx), dtype=np.float32)
+        self._c(lib().phafd_unet_forward(self.h, nx, ny, nz, _f(x), _f(out)))
+        return out
+
+    def backward(self, dout):
+        d = np.ascontiguousarray(dout, dtype=np.float32)
+        self._c(lib().phafd_unet_backward(self.h, _f(d)))
+
+    def get_grads(self, zero=True):
+        g = np.zeros(self.nparams, dtype=np.float32)
+        self._c(lib().phafd_unet_get_grads(self.h, _f(g), 1 if zero else 0))
+        return g
+
+    def data_alloc(self, nx, ny, nz, nsamples):
+        self._c(lib().phafd_unet_data_alloc(self.h, nx, ny, nz, nsamples))
+
+    def data_put(self, i, in4, target):
+        x = np.ascontiguousarray(in4, dtype=np.float32)
+        y = np.ascontiguousarray(target, dtype=np.float32)
+        self._c(lib().phafd_unet_data_put(self.h, i, _f(x), _f(y)))
+
+    def train(self, nsteps, batch, ntrain, lr):
+        losses = np.zeros(nsteps)
+        self._c(lib().phafd_unet_train(self.h, nsteps, batch, ntrain, lr, _d(losses)))
+        return losses
+
+    def eval(self, i0, count):
+        losses = np.zeros(count)
+        self._c(lib().phafd_unet_eval(self.h, i0, count, _d(losses)))
+        return losses
+
+    def save(self, path):
+        np.savez(path, params=self.get_params(), norm=self.get_norm())
+
+    def load(self, path):
+        z = np.load(path)
+        self.set_params(z["params"])
+        self.set_norm(z["norm"])
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().phafd_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def surrogate_inputs(phi, rho0, rho1, norm):
+    """The network's four input channels (float32) from phi and the bead
+    densities at the block's start and end, as the library forms them on
+    the device."""
+    phi = np.asarray(phi, dtype=np.float64)
+    mean = phi.mean()
+    c0 = (phi - mean) / norm[0]
+    c1 = np.asarray(rho0, dtype=np.float64) / norm[1]
+    c2 = np.asarray(rho1, dtype=np.float64) / norm[1]
+    c3 = np.full(phi.shape, (mean - norm[2]) / norm[3])
+    return np.stack([c0, c1, c2, c3]).astype(np.float32)

@@ -993,6 +993,59 @@ void launch_ga_deposit(const GridView &gv, const GAParams &ga, const int *cnt, c
   hipLaunchKernelGGL(k_ga_deposit, dim3(nblk(N)), dim3(TPB), 0, s, gv, ga, cnt, off, neigh, gx, nowned, energy);
   HIPCHK(hipGetLastError());
 }
+// Bead density: rho(g) = sum over gathered atoms of the grid-atom kernel
+// value e at g (the factor of 2 (phi - phi_t) in the mu term), for points
+// within the pair cut, FP32 atomics (an input feature of the surrogate; the
+// order of the adds is free).
+__global__ void k_ga_density(int ng, GridView gv, GAParams ga, const double4 *gx, float *rho) {
+  GRID_STRIDE(q, ng) {
+    const double4 x = gx[q];
+    if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z))) continue;
+    const int t = (int)x.w;
+    const double cs = ga.cutsq[t];
+    const GABox b = ga_box(gv, x, cs);
+    for (int i = b.i0; i <= b.i1; i++) {
+      const double delz = (i * gv.dz + gv.lo[2]) - x.z;
+      for (int j = b.j0; j <= b.j1; j++) {
+        const double dely = (j * gv.dy + gv.lo[1]) - x.y;
+        int rk0, rk1;
+        if (!ga_row(gv, b, x.x, cs, dely * dely + delz * delz, rk0, rk1)) continue;
+        for (int k = rk0; k <= rk1; k++) {
+          const double delx = (k * gv.dx + gv.lo[0]) - x.x;
+          const double rsq = delx * delx + dely * dely + delz * delz;
+          if (!ga_in(ga, rsq, cs)) continue;
+          const double r = rsq / cs;
+          double e;
+          if (ga.style == 0) e = (exp(-r / 2.0) - ga.em05) * ga.e05 * ga.eps[t];
+          else e = exp(-1. / (1 - r) + 1) * ga.eps[t];
+          atomicAdd(&rho[((size_t)i * gv.ny + j) * gv.nx + k], (float)e);
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_add4(int n, const double4 *a, double4 *b) {
+  GRID_STRIDE(i, n) {
+    const double4 u = a[i];
+    double4 v = b[i];
+    v.x += u.x;
+    v.y += u.y;
+    v.z += u.z;
+    b[i] = v;
+  }
+}
+void launch_add4(int n, const double4 *a, double4 *b, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_add4, dim3(nblk(n)), dim3(TPB), 0, s, n, a, b);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_ga_density(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, float *rho, hipStream_t s) {
+  if (ng <= 0) return;
+  hipLaunchKernelGGL(k_ga_density, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, rho);
+  HIPCHK(hipGetLastError());
+}
 void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
                      const double *cutsq, int *acnt, hipStream_t s) {
   if (ng <= 0) return;

@@ -79,6 +79,8 @@ void launch_ga_deposit(const GridView &gv, const GAParams &ga, const int *cnt, c
 // of the pair arrays
 void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
                      const double *cutsq, int *acnt, hipStream_t s);
+// bead density (surrogate input): rho += kernel value e of every in-cut pair
+void launch_ga_density(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, float *rho, hipStream_t s);
 void launch_ga_enum(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
                     const int *aoff, int64_t cap, int *pg, int *pq, int *overflow, hipStream_t s);
 // launch_ga_enum / _pair / _fold: the atom side of the pass (pairs of each
@@ -115,5 +117,6 @@ void launch_chain_energy(int n, const int *members, const int *cstart, const int
                          const ChainParams &cp, double *energy, hipStream_t s);
 void launch_fill_i32(int n, int *p, int v, hipStream_t s);
 void launch_fill_d4(int n, double4 *p, hipStream_t s);
+void launch_add4(int n, const double4 *a, double4 *b, hipStream_t s);   // b.xyz += a.xyz
 
 }  // namespace phafd

@@ -11,6 +11,7 @@
 #include "../../include/phafd_hip.h"
 #include "atoms.hpp"
 #include "internal.hpp"
+#include "unet.hpp"
 
 using namespace phafd;
 
@@ -147,6 +148,14 @@ struct phafd_hip_ctx {
   double tacc[8] = {0};
   bool fuse_fh = true;
   bool fuse_spectral = true;
+  // surrogate blocks: bead densities at both ends, network input and
+  // output, the predicted phi, mu, the base, FP64 partial sums, frozen
+  // grid-atom forces, the atoms' block-start state
+  DBuf<float> sg_rho, sg_rho1, sg_out;
+  DBuf<uint16_t> sg_x0;
+  DBuf<double> sg_phi, sg_mu0, sg_mu1, sg_base, sg_red, sg_sum;
+  DBuf<double4> sg_fga, sg_x, sg_ux;
+  DBuf<int> sg_img;
 
   double dx() const { return period[0] / nx; }   // src/grid.cpp:187-201
   double dy() const { return period[1] / ny; }
@@ -2097,5 +2106,293 @@ int phafd_hip_batch_destroy(phafd_hip_batch *b) {
 }
 
 const char *phafd_hip_batch_last_error(const phafd_hip_batch *b) { return b ? b->err.c_str() : g_create_err.c_str(); }
+
+}  // extern "C"
+
+// ============================================================== surrogate
+// Blocks of K steps with the field advanced by the network.  The atoms go
+// first: K steps with lj/cut and the chains every step and the grid-atom
+// force of the block-start field held.  The bead density is deposited at
+// both ends of the block.  phi after K steps is then a base -- the plain
+// update with its linear part exact per Fourier mode and the rest of mu held
+// at its block-start value (exponential Euler) -- plus the network's
+// correction from phi0 and the two densities (mean removed, so the mass is
+// kept).  The check of the plain method's equation decides:
+// accepted, phi is replaced; rejected, the atoms are put back and the block
+// runs as K plain steps.
+namespace {
+
+struct SurrogateCfg {
+  PairDef *ga = nullptr;
+  const FixDef *cj = nullptr, *fh = nullptr;
+  double phit = 0;
+};
+constexpr int SG_NB = 512;   // blocks of the residual reduction
+
+SurrogateCfg surrogate_cfg(phafd_hip_ctx *c) {
+  require_setup(c);
+  if (c->fp32) throw std::invalid_argument("the surrogate runs with the FP64 form");
+  SurrogateCfg g;
+  int nconj = 0, nfh = 0, nga = 0;
+  for (auto &f : c->fixes) {
+    if (f->kind == FixDef::CONJ) { nconj++; g.cj = f.get(); }
+    if (f->kind == FixDef::FH) { nfh++; g.fh = f.get(); }
+    if (f->kind == FixDef::GRADPHI) throw std::invalid_argument("the surrogate takes no grid/gradphi fix");
+  }
+  if (nconj != 1 || nfh != 1)
+    throw std::invalid_argument("the surrogate needs one grid/conjugate/volfrac and one grid/floryhuggins fix");
+  if (g.cj->noise_mode != PHAFD_NOISE_OFF) throw std::invalid_argument("the surrogate runs with noise off");
+  for (auto &p : c->pairs)
+    if (p->style != 2) { nga++; g.ga = p.get(); }
+  if (nga > 1) throw std::invalid_argument("the surrogate takes at most one grid-atom pair style");
+  if (g.ga) {
+    g.phit = g.ga->phit.empty() ? 0.0 : g.ga->phit[0];
+    for (double v : g.ga->phit)
+      if (v != g.phit) throw std::invalid_argument("the surrogate needs one phi_t for all atom types");
+  }
+  const size_t N = c->N;
+  c->sg_rho.alloc(N); c->sg_rho1.alloc(N); c->sg_out.alloc(N); c->sg_x0.alloc(8 * N);
+  c->sg_phi.alloc(N); c->sg_mu0.alloc(N); c->sg_mu1.alloc(N); c->sg_base.alloc(N);
+  c->sg_red.alloc(2 * SG_NB + 2);
+  c->sg_sum.alloc(4);
+  const size_t na = (size_t)std::max(c->nowned, 1);
+  c->sg_fga.alloc(na); c->sg_x.alloc(na); c->sg_ux.alloc(na); c->sg_img.alloc(na);
+  return g;
+}
+
+// gathered positions current without consuming the prefetched rebuild decision
+void refresh_gathered(phafd_hip_ctx *c) {
+  if (!c->neigh_set) return;
+  if (c->pending_decision == 0) forward_comm(c);
+  else phase_neigh(c);
+}
+
+void deposit_density(phafd_hip_ctx *c, const SurrogateCfg &g, float *rho) {
+  fill_zero(rho, sizeof(float) * c->N, c->stream);
+  if (g.ga && c->neigh_set && c->lists_valid)
+    launch_ga_density(c->ngathered, c->gridview<double>(), ga_params(*g.ga), c->gx.p, rho, c->stream);
+}
+
+// sg_sum[0..1] = FP64 {sum phi0, sum rho0}
+void block_sums(phafd_hip_ctx *c, const double *phi0, const float *rho0) {
+  fill_zero(c->sg_sum.p, 4 * sizeof(double), c->stream);
+  sum_fp64(c->N, phi0, c->sg_sum.p, c->stream);
+  sum_fp64(c->N, rho0, c->sg_sum.p + 1, c->stream);
+}
+
+// base = phi0 after K steps by exponential Euler with the block-start
+// density rho0.  Leaves phihat = spectrum of phi0, muhat = spectrum of
+// mu(phi0, rho0) and sg_sum set; uses sg_mu0 and ghat[0].
+void surrogate_base(phafd_hip_ctx *c, const SurrogateCfg &g, const double *phi0, const float *rho0, double *base,
+                    int K) {
+  FieldSet<double> &F = c->fd;
+  const double tv = g.fh->temp / g.fh->volFH;
+  const UpdateParams u = update_params(c, *g.cj);
+  block_sums(c, phi0, rho0);
+  surrogate_mu(c->N, phi0, rho0, tv, g.fh->chi, g.phit, c->sg_mu0.p, c->stream);
+  fft_r2c(c->plan, phi0, F.phihat.p, c->stream);
+  fft_r2c(c->plan, c->sg_mu0.p, F.muhat.p, c->stream);
+  etd_block(c->plan, F.phihat.p, F.muhat.p, F.ghat[0].p, u, tv, g.fh->chi, c->sg_sum.p, K, c->stream);
+  fft_c2r(c->plan, F.ghat[0].p, base, (double)c->N, c->stream);
+}
+
+// {residual, relative mass drift} of phi1 as phi0 advanced by K steps with
+// densities rho0, rho1; needs phihat, muhat and sg_sum from surrogate_base
+// on the same phi0, rho0; uses sg_mu1, ghat[0..1]
+void block_check(phafd_hip_ctx *c, const SurrogateCfg &g, const double *phi0, const double *phi1,
+                 const float *rho1, int K, double out[2]) {
+  FieldSet<double> &F = c->fd;
+  const double tv = g.fh->temp / g.fh->volFH;
+  surrogate_mu(c->N, phi1, rho1, tv, g.fh->chi, g.phit, c->sg_mu1.p, c->stream);
+  fft_r2c(c->plan, phi1, F.ghat[0].p, c->stream);
+  fft_r2c(c->plan, c->sg_mu1.p, F.ghat[1].p, c->stream);
+  block_residual_partials(c->plan, F.phihat.p, F.ghat[0].p, F.muhat.p, F.ghat[1].p, update_params(c, *g.cj), tv,
+                          g.fh->chi, c->sg_sum.p, K, c->sg_red.p, SG_NB, c->stream);
+  double *ms = c->sg_red.p + 2 * SG_NB;
+  fill_zero(ms, 2 * sizeof(double), c->stream);
+  sum_fp64(c->N, phi0, ms, c->stream);
+  sum_fp64(c->N, phi1, ms + 1, c->stream);
+  std::vector<double> h(2 * SG_NB + 2);
+  HIPCHK(hipMemcpyAsync(h.data(), c->sg_red.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  double sr = 0, sd = 0;
+  for (int b = 0; b < SG_NB; b++) {
+    sr += h[2 * b];
+    sd += h[2 * b + 1];
+  }
+  out[0] = sd > 0 ? std::sqrt(sr / sd) : (sr > 0 ? INFINITY : 0.0);
+  out[1] = (h[2 * SG_NB + 1] - h[2 * SG_NB]) / h[2 * SG_NB];
+}
+
+// K atom steps (neighbour, lj/cut, chains every step) with the grid-atom
+// force of the current field computed once and held
+void atom_steps_frozen_field(phafd_hip_ctx *c, const SurrogateCfg &g, int K) {
+  phase_start_t<double>(c);   // grad phi of the block-start field
+  for (int s = 0; s < K; s++) {
+    c->timestep++;
+    phase_neigh(c);
+    const bool lists = c->neigh_set && c->lists_valid;
+    if (s == 0) {
+      launch_fill_d4(c->ngathered, c->gF.p, c->stream);
+      launch_fill_d4(c->nowned, c->F.p, c->stream);
+      if (g.ga && lists) {
+        grid_atom_pass<double>(c, *g.ga, nullptr, true);
+        reverse_comm(c);
+      }
+      launch_copy4(c->nowned, c->F.p, c->sg_fga.p, c->stream);
+    }
+    launch_fill_d4(c->ngathered, c->gF.p, c->stream);
+    launch_fill_d4(c->nowned, c->F.p, c->stream);
+    if (lists) {
+      for (auto &pp : c->pairs) {
+        if (pp->style != 2) continue;
+        LJParams lj{pp->lj1, pp->lj2, pp->ljcutsq};
+        launch_lj_force(c->ngathered, lj, c->gx.p, c->h_rowof.p, c->h_off.p, c->h_neigh.p, c->t_off.p,
+                        c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, nullptr, c->stream);
+      }
+      reverse_comm(c);
+    }
+    launch_add4(c->nowned, c->sg_fga.p, c->F.p, c->stream);
+    phase_initial(c);
+    for (auto &f : c->fixes)
+      if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f);
+    if (c->neigh_set) {   // next step's rebuild decision (Neighbor::decide)
+      launch_fill_i32(1, c->flags.p, 0, c->stream);
+      launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->flags.p, c->stream);
+    }
+    HIPCHK(hipMemcpyAsync(c->h_flags + 8, c->flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    if (c->h_flags[11]) throw std::runtime_error("grid-atom term capacity exceeded");
+    if (c->neigh_set) {
+      if (c->h_flags[8] & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
+      c->pending_decision = c->h_flags[8] & 1;
+    }
+  }
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// stats8: {blocks, accepted, fallbacks, max residual, last residual, max
+// |mass drift|, ms in the network path (densities, base, network, check),
+// ms in total}
+void surrogate_steps(phafd_hip_ctx *c, UNet &net, int nsteps, int K, double tol, double *stats) {
+  SurrogateCfg g = surrogate_cfg(c);
+  if (K < 1 || nsteps < 0 || nsteps % K) throw std::invalid_argument("nsteps must be a multiple of K >= 1");
+  if (net.device != c->device) throw std::invalid_argument("the network lives on another device");
+  if (!unet_shape_ok(c->nx, c->ny, c->nz))
+    throw std::invalid_argument("surrogate grid must have nx % 64 == 0, ny % 32 == 0, nz % 4 == 0");
+  net.ensure(c->nx, c->ny, c->nz, false);
+  double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const auto t_all = std::chrono::steady_clock::now();
+  const size_t N = c->N;
+  for (int blk = 0; blk < nsteps / K; blk++) {
+    auto t0 = std::chrono::steady_clock::now();
+    refresh_gathered(c);
+    deposit_density(c, g, c->sg_rho.p);
+    // the atoms' block-start state, for a fallback
+    const int64_t calls0 = c->ncalls, step0 = c->timestep;
+    const int pending0 = c->pending_decision;
+    launch_copy4(c->nowned, c->x.p, c->sg_x.p, c->stream);
+    launch_copy4(c->nowned, c->ux.p, c->sg_ux.p, c->stream);
+    HIPCHK(hipMemcpyAsync(c->sg_img.p, c->image.p, sizeof(int) * c->nowned, hipMemcpyDeviceToDevice, c->stream));
+    st[6] += ms_since(t0);
+    atom_steps_frozen_field(c, g, K);
+    t0 = std::chrono::steady_clock::now();
+    refresh_gathered(c);
+    deposit_density(c, g, c->sg_rho1.p);
+    const double *phi0 = c->fd.phi.p;
+    surrogate_base(c, g, phi0, c->sg_rho.p, c->sg_base.p, K);
+    unet_features(N, phi0, c->sg_rho.p, c->sg_rho1.p, c->sg_sum.p, net.norm, c->sg_x0.p, c->stream);
+    net.forward(c->sg_x0.p, c->sg_out.p, c->stream);
+    fill_zero(c->sg_sum.p + 2, sizeof(double), c->stream);
+    sum_fp64(N, c->sg_out.p, c->sg_sum.p + 2, c->stream);
+    unet_apply(N, c->sg_base.p, c->sg_out.p, c->sg_sum.p + 2, net.norm[4], c->sg_phi.p, c->stream);
+    double chk[2];
+    block_check(c, g, phi0, c->sg_phi.p, c->sg_rho1.p, K, chk);
+    st[6] += ms_since(t0);
+    st[0] += 1;
+    st[3] = std::max(st[3], chk[0]);
+    st[4] = chk[0];
+    st[5] = std::max(st[5], std::fabs(chk[1]));
+    if (chk[0] <= tol && std::fabs(chk[1]) <= 1e-12) {
+      HIPCHK(hipMemcpyAsync(c->fd.phi.p, c->sg_phi.p, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->fd.phi.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      sync(c);
+      double p0;
+      std::memcpy(&p0, c->h_flags + 2, sizeof(double));
+      if (std::isnan(p0)) throw std::runtime_error("NAN encountered in phi.");
+      st[1] += 1;
+    } else {
+      // put the atoms back; lists rebuilt in between are rebuilt again at
+      // the restored positions
+      launch_copy4(c->nowned, c->sg_x.p, c->x.p, c->stream);
+      launch_copy4(c->nowned, c->sg_ux.p, c->ux.p, c->stream);
+      HIPCHK(hipMemcpyAsync(c->image.p, c->sg_img.p, sizeof(int) * c->nowned, hipMemcpyDeviceToDevice, c->stream));
+      c->timestep = step0;
+      c->pending_decision = c->ncalls != calls0 ? 1 : pending0;
+      run_steps_t<double>(c, K);
+      st[2] += 1;
+    }
+  }
+  st[7] = ms_since(t_all);
+  if (stats) std::memcpy(stats, st, sizeof(st));
+}
+
+}  // namespace
+
+extern "C" {
+
+int phafd_hip_density(phafd_hip_ctx *c, float *rho) {
+  return guard(c, [&] {
+    const SurrogateCfg g = surrogate_cfg(c);
+    refresh_gathered(c);
+    deposit_density(c, g, c->sg_rho.p);
+    HIPCHK(hipMemcpyAsync(rho, c->sg_rho.p, sizeof(float) * c->N, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return 0;
+  });
+}
+
+int phafd_hip_surrogate_base(phafd_hip_ctx *c, const double *phi0, const float *rho0, int K, double *base) {
+  return guard(c, [&] {
+    const SurrogateCfg g = surrogate_cfg(c);
+    if (!phi0 || !rho0 || !base || K < 1) throw std::invalid_argument("bad arguments");
+    HIPCHK(hipMemcpyAsync(c->sg_base.p, phi0, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->sg_rho.p, rho0, sizeof(float) * c->N, hipMemcpyHostToDevice, c->stream));
+    DBuf<double> out;
+    out.alloc(c->N);
+    surrogate_base(c, g, c->sg_base.p, c->sg_rho.p, out.p, K);
+    HIPCHK(hipMemcpyAsync(base, out.p, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return 0;
+  });
+}
+
+int phafd_hip_block_residual(phafd_hip_ctx *c, const double *phi0, const double *phi1, const float *rho0,
+                             const float *rho1, int K, double *out2) {
+  return guard(c, [&] {
+    const SurrogateCfg g = surrogate_cfg(c);
+    if (!phi0 || !phi1 || !rho0 || !rho1 || !out2 || K < 1) throw std::invalid_argument("bad arguments");
+    DBuf<double> a, b, scratch;
+    a.alloc(c->N); b.alloc(c->N); scratch.alloc(c->N);
+    HIPCHK(hipMemcpyAsync(a.p, phi0, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b.p, phi1, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->sg_rho.p, rho0, sizeof(float) * c->N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->sg_rho1.p, rho1, sizeof(float) * c->N, hipMemcpyHostToDevice, c->stream));
+    surrogate_base(c, g, a.p, c->sg_rho.p, scratch.p, K);   // spectra of phi0, mu0 and the sums
+    block_check(c, g, a.p, b.p, c->sg_rho1.p, K, out2);
+    return 0;
+  });
+}
+
+int phafd_hip_run_surrogate(phafd_hip_ctx *c, phafd_unet *net, int nsteps, int K, double tol, double *stats8) {
+  return guard(c, [&] {
+    if (!net) throw std::invalid_argument("no network");
+    surrogate_steps(c, net->net, nsteps, K, tol, stats8);
+    return 0;
+  });
+}
 
 }  // extern "C"

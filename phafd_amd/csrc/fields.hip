@@ -304,4 +304,136 @@ void fill_zero(void *p, size_t bytes, hipStream_t s) {
   HIPCHK(hipGetLastError());
 }
 
+// ------------------------------------------------- surrogate block check
+// The plain update of one mode, phihat <- phihat - c (muhat + s phihat)
+// (src/conjugate_volfrac.cpp:89-133; c = mobility q^2 dt, s = temp/volFH
+// gamma q^2), split as mu = f2 phi + n(phi) with f2 = temp/volFH (1/m +
+// 1/(1-m) - 2 chi) + 2 mean(rho) (m = mean phi), is phihat <- G phihat -
+// c nhat with G = 1 - c (f2 + s); over K steps
+//   phi_K = G^K phi_0 - c sum_{k<K} G^(K-1-k) n_k.
+namespace {
+struct ModeK {
+  double c, G, GK;
+};
+__device__ __forceinline__ ModeK mode_k(size_t w, int ny, int nz, const UpdateParams &u, double f2, int K) {
+  const int i = (int)(w % ny);                    // ky
+  const int j = (int)((w / ny) % nz);             // kz
+  const int k = (int)(w / ((size_t)ny * nz));     // kx
+  const double qz = (i > ny / 2) ? -u.qky_scale * (ny - i) : u.qky_scale * i;
+  const double qy = (j > nz / 2) ? -u.qkz_scale * (nz - j) : u.qkz_scale * j;
+  const double qx = u.qx_scale * k;
+  const double q2 = qx * qx + qy * qy + qz * qz;
+  ModeK r;
+  r.c = u.mobility * q2 * u.dt;
+  r.G = 1.0 - r.c * (f2 + u.temp / u.volFH * u.gamma * q2);
+  r.GK = 1.0;
+  for (int n = 0; n < K; n++) r.GK *= r.G;
+  return r;
+}
+__device__ __forceinline__ double f2_of(const double *sums, double N, double tv, double chi) {
+  const double m = sums[0] / N, rbar = sums[1] / N;
+  return tv * (1.0 / m + 1.0 / (1.0 - m) - 2.0 * chi) + 2.0 * rbar;
+}
+
+// mu of the plain step at phi without the reflection: Flory-Huggins
+// (src/fixgrid_floryhuggins.cpp:71-83) plus the grid-atom term 2 (phi -
+// phi_t) rho for one atom type (rho = sum of the kernel values e)
+__global__ void k_mu_surrogate(size_t n, const double *__restrict__ phi, const float *__restrict__ rho, double tv,
+                               double chi, double phit, double *__restrict__ mu) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double p = phi[i];
+    mu[i] = tv * (log(p / (1 - p)) + chi * (1 - 2 * p)) + 2 * (p - phit) * (double)rho[i];
+  }
+}
+// K steps of one mode with the linear part exact and n = muhat - f2 phihat
+// held at its block-start value (exponential Euler over the block):
+//   out = G^K p0 - c (1 + G + ... + G^(K-1)) n0
+__global__ void k_etd(const double2 *__restrict__ p0, const double2 *__restrict__ m0, double2 *__restrict__ out,
+                      int ny, int nz, size_t M, UpdateParams u, double tv, double chi, const double *__restrict__ sums,
+                      double N, int K) {
+  const double f2 = f2_of(sums, N, tv, chi);
+  for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < M; w += (size_t)gridDim.x * blockDim.x) {
+    const ModeK q = mode_k(w, ny, nz, u, f2, K);
+    double S = 0.0, g = 1.0;
+    for (int n = 0; n < K; n++) {
+      S += g;
+      g *= q.G;
+    }
+    const double2 a = p0[w], ma = m0[w];
+    double2 r;
+    r.x = q.GK * a.x - q.c * S * (ma.x - f2 * a.x);
+    r.y = q.GK * a.y - q.c * S * (ma.y - f2 * a.y);
+    out[w] = r;
+  }
+}
+
+// per block: sums over the half spectrum (weights 1 on kx = 0, Nx/2, else 2)
+// of |R|^2 and |ph1 - ph0|^2 with R = ph1 - G^K ph0 + c (A n0 + B n1),
+// n = muhat - f2 phihat interpolated linearly across the block: A = sum_k
+// G^(K-1-k) (1 - k/K), B = sum_k G^(K-1-k) k/K
+__global__ __launch_bounds__(256) void k_block_residual(const double2 *__restrict__ ph0,
+                                                        const double2 *__restrict__ ph1,
+                                                        const double2 *__restrict__ mu0,
+                                                        const double2 *__restrict__ mu1, int nx, int ny, int nz,
+                                                        size_t M, UpdateParams u, double tv, double chi,
+                                                        const double *__restrict__ sums, double N, int K,
+                                                        double *__restrict__ part) {
+  __shared__ double r1[256], r2[256];
+  const double f2 = f2_of(sums, N, tv, chi);
+  double sr = 0, sd = 0;
+  for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < M; w += (size_t)gridDim.x * blockDim.x) {
+    const ModeK q = mode_k(w, ny, nz, u, f2, K);
+    double A = 0.0, B = 0.0, g = 1.0;
+    for (int k = K - 1; k >= 0; k--) {   // g = G^(K-1-k)
+      A += g * (1.0 - (double)k / K);
+      B += g * ((double)k / K);
+      g *= q.G;
+    }
+    const double2 a = ph0[w], b = ph1[w], m0 = mu0[w], m1 = mu1[w];
+    const double n0x = m0.x - f2 * a.x, n0y = m0.y - f2 * a.y;
+    const double n1x = m1.x - f2 * b.x, n1y = m1.y - f2 * b.y;
+    const double rx = b.x - q.GK * a.x + q.c * (A * n0x + B * n1x);
+    const double ry = b.y - q.GK * a.y + q.c * (A * n0y + B * n1y);
+    const double dx = b.x - a.x, dy = b.y - a.y;
+    const int kx = (int)(w / ((size_t)ny * nz));
+    const double wt = (kx == 0 || 2 * kx == nx) ? 1.0 : 2.0;
+    sr += wt * (rx * rx + ry * ry);
+    sd += wt * (dx * dx + dy * dy);
+  }
+  r1[threadIdx.x] = sr;
+  r2[threadIdx.x] = sd;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      r1[threadIdx.x] += r1[threadIdx.x + o];
+      r2[threadIdx.x] += r2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r1[0];
+    part[2 * blockIdx.x + 1] = r2[0];
+  }
+}
+}  // namespace
+
+void surrogate_mu(size_t n, const double *phi, const float *rho, double tv, double chi, double phit, double *mu,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_mu_surrogate, dim3(grid_for(n)), dim3(TPB), 0, s, n, phi, rho, tv, chi, phit, mu);
+  HIPCHK(hipGetLastError());
+}
+void etd_block(const FFTPlan &p, const double2 *p0, const double2 *m0, double2 *out, UpdateParams u, double tv,
+               double chi, const double *sums, int K, hipStream_t s) {
+  hipLaunchKernelGGL(k_etd, dim3(grid_for(p.M())), dim3(TPB), 0, s, p0, m0, out, p.ny, p.nz, p.M(), u, tv, chi, sums,
+                     (double)p.N(), K);
+  HIPCHK(hipGetLastError());
+}
+void block_residual_partials(const FFTPlan &p, const double2 *ph0, const double2 *ph1, const double2 *mu0,
+                             const double2 *mu1, UpdateParams u, double tv, double chi, const double *sums, int K,
+                             double *part, int nb, hipStream_t s) {
+  hipLaunchKernelGGL(k_block_residual, dim3(nb), dim3(TPB), 0, s, ph0, ph1, mu0, mu1, p.nx, p.ny, p.nz, p.M(), u,
+                     tv, chi, sums, (double)p.N(), K, part);
+  HIPCHK(hipGetLastError());
+}
+
 }  // namespace phafd

@@ -183,6 +183,20 @@ PHAFD_FIELD_DECL(double, double2)
 PHAFD_FIELD_DECL(float, float2)
 #undef PHAFD_FIELD_DECL
 void fill_zero(void *p, size_t bytes, hipStream_t s);
+// surrogate blocks (the plain update linearised about the mean: mu = f2 phi
+// + n, f2 = temp/volFH (1/m + 1/(1-m) - 2 chi) + 2 mean(rho); sums = FP64
+// {sum phi0, sum rho} on the device).  surrogate_mu: mu of the plain step at
+// phi (no reflection, one atom type); etd_block: K steps of the spectrum p0
+// (mu spectrum m0) with the linear part exact, n = muhat - f2 phihat held;
+// block_residual_partials: per-block sums {|R|^2, |ph1 - ph0|^2} of the
+// K-step residual with n interpolated linearly across the block.
+void surrogate_mu(size_t n, const double *phi, const float *rho, double tv, double chi, double phit, double *mu,
+                  hipStream_t s);
+void etd_block(const FFTPlan &p, const double2 *p0, const double2 *m0, double2 *out, UpdateParams u, double tv,
+               double chi, const double *sums, int K, hipStream_t s);
+void block_residual_partials(const FFTPlan &p, const double2 *ph0, const double2 *ph1, const double2 *mu0,
+                             const double2 *mu1, UpdateParams u, double tv, double chi, const double *sums, int K,
+                             double *part, int nb, hipStream_t s);
 
 // --------------------------------------------------------------- scan utils
 // exclusive scan of n int32 values into int64 offsets; out has n+1 entries,

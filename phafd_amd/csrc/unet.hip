@@ -1,0 +1,970 @@
+// unet.hip -- kernels and host code of the surrogate network (unet.hpp).
+//
+// Activations are channels-last ([z][y][x][c], x fastest) in bf16.  A 3^3
+// convolution is an implicit GEMM on the v_mfma_f32_16x16x32_bf16 units:
+// M = 16 consecutive x points of one row, N = 16 output channels, K = 32 =
+// four groups of 8 input channels, a group being (tap, 8 channels); periodic
+// wrap on every axis.  Each wave computes two rows (32 points) x all output
+// channels; weights are packed once per update in the exact lane order of the
+// B operand, so a wave reads them with one 16-byte load per lane.  The weight
+// gradient is the same GEMM with K = grid points (staged transposed in LDS);
+// the data gradient is a forward convolution with the transposed, flipped
+// kernel.  Every reduction runs in a fixed order: training is reproducible.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+
+#include "phafd_hip.h"
+#include "unet.hpp"
+
+namespace phafd {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int TPB = 256;
+constexpr int MT = 2;   // rows of 16 points per wave in the convolution
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// round to nearest even (the conversion PyTorch applies)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ bool bf_pos(uint16_t h) { return !(h & 0x8000u) && h != 0; }
+
+__device__ __forceinline__ int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+size_t nblk(size_t n) {
+  size_t b = (n + TPB - 1) / TPB;
+  return b < 1 ? 1 : (b > 65535u * 4 ? 65535u * 4 : b);
+}
+
+// ------------------------------------------------------------ convolution
+// Y[v][co] = act(bias[co] + sum_{tap, ci} W[co][ci][tap] X[v + off(tap)][ci])
+// W packed: [chunk][n-tile][lane][8] (B-operand order).  Grid: (nx/16) *
+// (ny/8) * nz blocks of 4 waves.
+template <int CIN, int COUT, int TAPS, bool RELU>
+__global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const u32x4 *__restrict__ W,
+                                              const float *__restrict__ bias, uint16_t *__restrict__ Y, int nx,
+                                              int ny, int nz) {
+  constexpr int CG = CIN / 8;
+  constexpr int G = TAPS * CG;
+  constexpr int NC = (G + 3) / 4;
+  constexpr int NT = COUT / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int xt = nx >> 4, yt = ny / (4 * MT);
+  int b = blockIdx.x;
+  const int bx = b % xt;
+  b /= xt;
+  const int by = b % yt;
+  const int z = b / yt;
+  const int x0 = bx * 16;
+  const int y0 = by * 4 * MT + wave * MT;
+  const int xi = x0 + (lane & 15);
+  const int kg = lane >> 4;
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int r = 0; r < MT; r++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+  for (int c = 0; c < NC; c++) {
+    const int g = 4 * c + kg;
+    bf16x8 a[MT];
+    if (g < G) {
+      const int tap = g / CG, cg = g % CG;
+      int dx = 0, dy = 0, dz = 0;
+      if (TAPS == 27) {
+        dz = tap / 9 - 1;
+        dy = (tap / 3) % 3 - 1;
+        dx = tap % 3 - 1;
+      }
+      const int xx = wrap1(xi + dx, nx);
+      const int zz = wrap1(z + dz, nz);
+#pragma unroll
+      for (int r = 0; r < MT; r++) {
+        const int yy = wrap1(y0 + r + dy, ny);
+        const size_t v = ((size_t)zz * ny + yy) * nx + xx;
+        a[r] = __builtin_bit_cast(bf16x8, X[v * CG + cg]);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < MT; r++) a[r] = __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u});
+    }
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const bf16x8 w = __builtin_bit_cast(bf16x8, W[((size_t)c * NT + t) * 64 + lane]);
+#pragma unroll
+      for (int r = 0; r < MT; r++) acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], w, acc[r][t], 0, 0, 0);
+    }
+  }
+  // D layout: lane holds rows 4*(lane/16) + j (points), column lane % 16 (channel)
+#pragma unroll
+  for (int r = 0; r < MT; r++) {
+    const size_t row = ((size_t)z * ny + (y0 + r)) * nx + x0;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int co = t * 16 + (lane & 15);
+      const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        float v = acc[r][t][j] + bb;
+        if (RELU) v = fmaxf(v, 0.f);
+        Y[(row + 4 * kg + j) * COUT + co] = f2bf(v);
+      }
+    }
+  }
+}
+
+// weight gradient partials: part[(tap * nb + blk) * 4 + wave][ci][co] =
+// sum over this wave's points v of X[v + off(tap)][ci] * D[v][co].  Grid
+// (nb, TAPS); each block takes groups of 256 points (64 per wave), stages X
+// (shifted) and D transposed in LDS, and runs K = 32-point MFMA steps.
+template <int CIN, int COUT, int TAPS>
+__global__ __launch_bounds__(256) void k_conv_dw(const uint16_t *__restrict__ X, const uint16_t *__restrict__ D,
+                                                 float *__restrict__ part, int nx, int ny, int nz) {
+  constexpr int MI = (CIN + 15) / 16;
+  constexpr int NT = COUT / 16;
+  constexpr int VS = 256 + 8;   // LDS row stride in points (16-byte aligned rows)
+  __shared__ __align__(16) uint16_t xs[CIN * VS];
+  __shared__ __align__(16) uint16_t ds[COUT * VS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tap = blockIdx.y;
+  int dx = 0, dy = 0, dz = 0;
+  if (TAPS == 27) {
+    dz = tap / 9 - 1;
+    dy = (tap / 3) % 3 - 1;
+    dx = tap % 3 - 1;
+  }
+  const size_t N = (size_t)nx * ny * nz;
+  const size_t ngroups = (N + 255) / 256;
+  f32x4 acc[MI][NT];
+#pragma unroll
+  for (int m = 0; m < MI; m++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (size_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const size_t v = grp * 256 + threadIdx.x;
+    if (v >= N) {   // past the last point: zero rows
+#pragma unroll
+      for (int c = 0; c < CIN; c++) xs[c * VS + threadIdx.x] = 0;
+#pragma unroll
+      for (int c = 0; c < COUT; c++) ds[c * VS + threadIdx.x] = 0;
+    } else {
+      const int x = (int)(v % nx);
+      const size_t t1 = v / nx;
+      const int y = (int)(t1 % ny), z = (int)(t1 / ny);
+      const size_t vs = ((size_t)wrap1(z + dz, nz) * ny + wrap1(y + dy, ny)) * nx + wrap1(x + dx, nx);
+      const u32x4 *xp = reinterpret_cast<const u32x4 *>(X + vs * CIN);
+#pragma unroll
+      for (int q = 0; q < CIN / 8; q++) {
+        const u32x4 w = xp[q];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          xs[(q * 8 + 2 * e) * VS + threadIdx.x] = (uint16_t)(w[e] & 0xffffu);
+          xs[(q * 8 + 2 * e + 1) * VS + threadIdx.x] = (uint16_t)(w[e] >> 16);
+        }
+      }
+      const u32x4 *dp = reinterpret_cast<const u32x4 *>(D + v * COUT);
+#pragma unroll
+      for (int q = 0; q < COUT / 8; q++) {
+        const u32x4 w = dp[q];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          ds[(q * 8 + 2 * e) * VS + threadIdx.x] = (uint16_t)(w[e] & 0xffffu);
+          ds[(q * 8 + 2 * e + 1) * VS + threadIdx.x] = (uint16_t)(w[e] >> 16);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      const int p0 = wave * 64 + s * 32 + 8 * (lane >> 4);
+      bf16x8 a[MI], bm[NT];
+#pragma unroll
+      for (int m = 0; m < MI; m++) {
+        const int ci = m * 16 + (lane & 15);
+        if (ci < CIN) a[m] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(&xs[ci * VS + p0]));
+        else a[m] = __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u});
+      }
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+        bm[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(&ds[(t * 16 + (lane & 15)) * VS + p0]));
+#pragma unroll
+      for (int m = 0; m < MI; m++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], bm[t], acc[m][t], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  float *o = part + (((size_t)tap * gridDim.x + blockIdx.x) * 4 + wave) * (CIN * COUT);
+#pragma unroll
+  for (int m = 0; m < MI; m++)
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int ci = m * 16 + 4 * (lane >> 4) + j;
+        if (ci < CIN) o[ci * COUT + t * 16 + (lane & 15)] = acc[m][t][j];
+      }
+}
+
+// G_w[co][ci][tap] += sum over the nb*4 partials (fixed order)
+__global__ void k_dw_reduce(const float *__restrict__ part, int nparts, int cin, int cout, int taps,
+                            float *__restrict__ gw) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = cin * cout * taps;
+  if (i >= total) return;
+  const int co = i % cout, ci = (i / cout) % cin, tap = i / (cout * cin);
+  const float *p = part + (size_t)tap * nparts * (cin * cout) + ci * cout + co;
+  float s = 0.f;
+  for (int k = 0; k < nparts; k++) s += p[(size_t)k * cin * cout];
+  gw[((size_t)co * cin + ci) * taps + tap] += s;
+}
+
+// column sums of D[N][C] per block: part[blk][C]
+template <int C>
+__global__ __launch_bounds__(256) void k_colsum(const uint16_t *__restrict__ D, size_t N, float *__restrict__ part) {
+  __shared__ float red[256];
+  constexpr int PER = 256 / C;   // points per block iteration
+  const int c = threadIdx.x % C, p = threadIdx.x / C;
+  float s = 0.f;
+  for (size_t v = (size_t)blockIdx.x * PER + p; v < N; v += (size_t)gridDim.x * PER) s += bf2f(D[v * C + c]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < C) {
+    float t = 0.f;
+    for (int k = 0; k < PER; k++) t += red[k * C + threadIdx.x];
+    part[(size_t)blockIdx.x * C + threadIdx.x] = t;
+  }
+}
+__global__ void k_colsum_reduce(const float *__restrict__ part, int nb, int C, float *__restrict__ g) {
+  const int c = threadIdx.x;
+  if (c >= C) return;
+  float s = 0.f;
+  for (int k = 0; k < nb; k++) s += part[(size_t)k * C + c];
+  g[c] += s;
+}
+
+// ----------------------------------------------------- pooling / upsampling
+// fine point of coarse point p, child k (dz, dy, dx bits)
+__device__ __forceinline__ size_t child(size_t p, int k, int cx, int cy) {
+  const int x = (int)(p % cx);
+  const size_t t = p / cx;
+  const int y = (int)(t % cy), z = (int)(t / cy);
+  const int fx = 2 * x + (k & 1), fy = 2 * y + ((k >> 1) & 1), fz = 2 * z + (k >> 2);
+  return ((size_t)fz * (2 * cy) + fy) * (2 * cx) + fx;
+}
+__device__ __forceinline__ size_t parent(size_t f, int fx_n, int fy_n) {
+  const int x = (int)(f % fx_n);
+  const size_t t = f / fx_n;
+  const int y = (int)(t % fy_n), z = (int)(t / fy_n);
+  return ((size_t)(z >> 1) * (fy_n >> 1) + (y >> 1)) * (fx_n >> 1) + (x >> 1);
+}
+
+// P[p][c] = mean of the 8 children (coarse dims cx, cy; Nc coarse points)
+__global__ void k_pool(const uint16_t *__restrict__ X, uint16_t *__restrict__ P, size_t Nc, int C, int cx, int cy) {
+  const size_t n = Nc * C;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t p = i / C;
+    const int c = (int)(i % C);
+    float s = 0.f;
+    for (int k = 0; k < 8; k++) s += bf2f(X[child(p, k, cx, cy) * C + c]);
+    P[i] = f2bf(s * 0.125f);
+  }
+}
+// dX[f][c] = dA[f][c] + dP[parent][c] / 8 (fine dims fx, fy)
+__global__ void k_pool_bwd_add(const uint16_t *__restrict__ dP, const uint16_t *__restrict__ dA,
+                               uint16_t *__restrict__ dX, size_t Nf, int C, int fx, int fy) {
+  const size_t n = Nf * C;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t f = i / C;
+    const int c = (int)(i % C);
+    dX[i] = f2bf(bf2f(dA[i]) + bf2f(dP[parent(f, fx, fy) * C + c]) * 0.125f);
+  }
+}
+// T[f][c] = S[f][c] + Q[parent][c]
+__global__ void k_upadd(const uint16_t *__restrict__ Q, const uint16_t *__restrict__ S, uint16_t *__restrict__ T,
+                        size_t Nf, int C, int fx, int fy) {
+  const size_t n = Nf * C;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t f = i / C;
+    const int c = (int)(i % C);
+    T[i] = f2bf(bf2f(S[i]) + bf2f(Q[parent(f, fx, fy) * C + c]));
+  }
+}
+// dQ[p][c] = sum of the 8 children of dT
+__global__ void k_downsum(const uint16_t *__restrict__ dT, uint16_t *__restrict__ dQ, size_t Nc, int C, int cx,
+                          int cy) {
+  const size_t n = Nc * C;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t p = i / C;
+    const int c = (int)(i % C);
+    float s = 0.f;
+    for (int k = 0; k < 8; k++) s += bf2f(dT[child(p, k, cx, cy) * C + c]);
+    dQ[i] = f2bf(s);
+  }
+}
+// D = D where A > 0, else 0 (gradient through relu, A its output)
+__global__ void k_mask(uint16_t *__restrict__ D, const uint16_t *__restrict__ A, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    if (!bf_pos(A[i])) D[i] = 0;
+}
+
+// ------------------------------------------------------------------ head
+// out[v] = b + sum_c w[c] A[v][c] (16 channels, FP32)
+__global__ void k_head(const u32x4 *__restrict__ A, const float *__restrict__ w, const float *__restrict__ b,
+                       float *__restrict__ out, size_t N) {
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < N; v += (size_t)gridDim.x * blockDim.x) {
+    float s = b[0];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const u32x4 u = A[v * 2 + q];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        s += w[q * 8 + 2 * e] * __uint_as_float(u[e] << 16);
+        s += w[q * 8 + 2 * e + 1] * __uint_as_float(u[e] & 0xffff0000u);
+      }
+    }
+    out[v] = s;
+  }
+}
+// dA[v][c] = (A[v][c] > 0) ? dout[v] w[c] : 0; part[blk][17] = (sum dout A[.][c], sum dout)
+__global__ __launch_bounds__(256) void k_head_bwd(const uint16_t *__restrict__ A, const float *__restrict__ w,
+                                                  const float *__restrict__ dout, uint16_t *__restrict__ dA, size_t N,
+                                                  float *__restrict__ part) {
+  __shared__ float red[256][17];
+  float s[17];
+#pragma unroll
+  for (int c = 0; c < 17; c++) s[c] = 0.f;
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < N; v += (size_t)gridDim.x * blockDim.x) {
+    const float d = dout[v];
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+      const uint16_t a = A[v * 16 + c];
+      dA[v * 16 + c] = bf_pos(a) ? f2bf(d * w[c]) : (uint16_t)0;
+      s[c] += d * bf2f(a);
+    }
+    s[16] += d;
+  }
+#pragma unroll
+  for (int c = 0; c < 17; c++) red[threadIdx.x][c] = s[c];
+  __syncthreads();
+  if (threadIdx.x < 17) {
+    float t = 0.f;
+    for (int k = 0; k < 256; k++) t += red[k][threadIdx.x];
+    part[(size_t)blockIdx.x * 17 + threadIdx.x] = t;
+  }
+}
+__global__ void k_head_reduce(const float *__restrict__ part, int nb, float *__restrict__ gw, float *__restrict__ gb) {
+  const int c = threadIdx.x;
+  if (c >= 17) return;
+  float s = 0.f;
+  for (int k = 0; k < nb; k++) s += part[(size_t)k * 17 + c];
+  if (c < 16) gw[c] += s;
+  else gb[0] += s;
+}
+
+// --------------------------------------------------------- loss, optimiser
+// dout = scale * 2 (out - y); part[blk] = sum (out - y)^2 (FP64)
+__global__ __launch_bounds__(256) void k_mse(const float *__restrict__ out, const float *__restrict__ y, size_t N,
+                                             float scale, float *__restrict__ dout, double *__restrict__ part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < N; v += (size_t)gridDim.x * blockDim.x) {
+    const float d = out[v] - y[v];
+    dout[v] = scale * 2.f * d;
+    s += (double)d * d;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_adam(float *__restrict__ p, float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                       size_t n, float lr_t, float b1, float b2, float eps) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr_t * mi / (sqrtf(vi) + eps);
+    g[i] = 0.f;
+  }
+}
+
+// packed B operand of a layer: out[((c * NT + t) * 64 + l) * 8 + j] = bf16 of
+// W[co][ci][tap] with g = 4c + l/16, tap = g / (CIN/8), ci = (g % (CIN/8)) * 8
+// + j, co = 16 t + l % 16; zero past the last group.  transposed: the weights
+// of the data gradient (CIN' = cout, COUT' = cin, tap' = taps - 1 - tap).
+__global__ void k_pack(const float *__restrict__ W, int cin, int cout, int taps, bool transposed,
+                       uint16_t *__restrict__ out) {
+  const int CIN = transposed ? cout : cin, COUT = transposed ? cin : cout;
+  const int CG = CIN / 8, G = taps * CG, NC = (G + 3) / 4, NT = COUT / 16;
+  const size_t total = (size_t)NC * NT * 64 * 8;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(i % 8);
+    const int l = (int)((i / 8) % 64);
+    const int t = (int)((i / 512) % NT);
+    const int c = (int)(i / (512 * (size_t)NT));
+    const int g = 4 * c + (l >> 4);
+    float w = 0.f;
+    if (g < G) {
+      const int tap = g / CG, ci = (g % CG) * 8 + j, co = t * 16 + (l & 15);
+      if (!transposed) w = W[((size_t)co * cin + ci) * taps + tap];
+      else w = W[((size_t)ci * cin + co) * taps + (taps - 1 - tap)];   // W[co'=ci][ci'=co]
+    }
+    out[i] = f2bf(w);
+  }
+}
+size_t packed_size(int cin, int cout, int taps) {
+  const int CG = cin / 8, G = taps * CG, NC = (G + 3) / 4, NT = cout / 16;
+  return (size_t)NC * NT * 64 * 8;
+}
+
+// ------------------------------------------------------ features / apply
+// channels: (phi - mean) / s0, rho0 / s1, rho1 / s1, (mean - mref) / s3
+__global__ void k_features(size_t N, const double *__restrict__ phi, const float *__restrict__ rho0,
+                           const float *__restrict__ rho1, const double *__restrict__ phisum, double s0, double s1,
+                           double mref, double s3, u32x4 *__restrict__ X0) {
+  const double mean = phisum[0] / (double)N;
+  const uint32_t c3 = f2bf((float)((mean - mref) / s3));
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < N; v += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t c0 = f2bf((float)((phi[v] - mean) / s0));
+    const uint32_t c1 = f2bf((float)(rho0[v] / s1));
+    const uint32_t c2 = f2bf((float)(rho1[v] / s1));
+    X0[v] = u32x4{c0 | (c1 << 16), c2 | (c3 << 16), 0u, 0u};
+  }
+}
+__global__ void k_features_ch(size_t N, const float *__restrict__ ch, u32x4 *__restrict__ X0) {
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < N; v += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t c0 = f2bf(ch[v]), c1 = f2bf(ch[N + v]), c2 = f2bf(ch[2 * N + v]), c3 = f2bf(ch[3 * N + v]);
+    X0[v] = u32x4{c0 | (c1 << 16), c2 | (c3 << 16), 0u, 0u};
+  }
+}
+__global__ void k_apply(size_t N, const double *__restrict__ phi, const float *__restrict__ out,
+                        const double *__restrict__ outsum, double scale, double *__restrict__ po) {
+  const double mean = outsum[0] / (double)N;
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < N; v += (size_t)gridDim.x * blockDim.x)
+    po[v] = phi[v] + scale * ((double)out[v] - mean);
+}
+
+// ------------------------------------------------------------ launchers
+template <int CIN, int COUT, int TAPS, bool RELU>
+void conv(const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y, int nx, int ny, int nz,
+          hipStream_t s) {
+  const unsigned blocks = (unsigned)((nx / 16) * (ny / (4 * MT)) * nz);
+  hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU>), dim3(blocks), dim3(256), 0, s,
+                     reinterpret_cast<const u32x4 *>(X), reinterpret_cast<const u32x4 *>(W), bias, Y, nx, ny, nz);
+  HIPCHK(hipGetLastError());
+}
+
+template <int CIN, int COUT, int TAPS>
+void conv_dw(const uint16_t *X, const uint16_t *D, float *part, float *gw, int nx, int ny, int nz, hipStream_t s) {
+  const size_t N = (size_t)nx * ny * nz;
+  const int ngroups = (int)((N + 255) / 256);
+  const int nb = ngroups < 256 ? ngroups : 256;
+  hipLaunchKernelGGL((k_conv_dw<CIN, COUT, TAPS>), dim3(nb, TAPS), dim3(256), 0, s, X, D, part, nx, ny, nz);
+  HIPCHK(hipGetLastError());
+  const int total = CIN * COUT * TAPS;
+  hipLaunchKernelGGL(k_dw_reduce, dim3((total + 255) / 256), dim3(256), 0, s, part, nb * 4, CIN, COUT, TAPS, gw);
+  HIPCHK(hipGetLastError());
+}
+
+template <int C>
+void colsum(const uint16_t *D, size_t N, float *part, float *g, hipStream_t s) {
+  constexpr int PER = 256 / C;
+  size_t nb = (N + PER - 1) / PER;
+  if (nb > 1024) nb = 1024;
+  hipLaunchKernelGGL((k_colsum<C>), dim3((unsigned)nb), dim3(256), 0, s, D, N, part);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_colsum_reduce, dim3(1), dim3(64), 0, s, part, (int)nb, C, g);
+  HIPCHK(hipGetLastError());
+}
+
+void ew_launch_check() { HIPCHK(hipGetLastError()); }
+
+}  // namespace
+
+bool unet_shape_ok(int nx, int ny, int nz) {
+  return nx >= 64 && ny >= 32 && nz >= 4 && nx % 64 == 0 && ny % 32 == 0 && nz % 4 == 0;
+}
+
+void UNet::init(int dev, uint64_t seed) {
+  device = dev;
+  HIPCHK(hipSetDevice(dev));
+  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  const int spec[9][5] = {{8, 16, 27, 1, 1},  {16, 16, 27, 1, 1}, {16, 32, 27, 2, 1},
+                          {32, 64, 27, 3, 1}, {64, 32, 1, 3, 0},  {32, 32, 27, 2, 1},
+                          {32, 16, 1, 2, 0},  {16, 16, 27, 1, 1}, {16, 1, 1, 1, 0}};
+  size_t off = 0, poff = 0;
+  for (auto &q : spec) {
+    UNetLayer l;
+    l.cin = q[0]; l.cout = q[1]; l.taps = q[2]; l.level = q[3]; l.relu = q[4] != 0;
+    l.w = off;
+    off += (size_t)l.cin * l.cout * l.taps;
+    l.b = off;
+    off += l.cout;
+    l.pf = l.pt = 0;
+    if (l.cout >= 16) {
+      l.pf = poff;
+      poff += packed_size(l.cin, l.cout, l.taps);
+      l.pt = poff;
+      if (l.cin >= 16) poff += packed_size(l.cout, l.cin, l.taps);
+    }
+    L.push_back(l);
+  }
+  nparam = off;
+  npack = poff;
+  P.alloc(nparam); G.alloc(nparam); Mo.alloc(nparam); Vo.alloc(nparam);
+  Wp.alloc(npack);
+  // He-uniform weights (fan in = cin_used * taps), zero biases; layer 0 uses
+  // 4 of its 8 input channels, the head starts at zero (the base unchanged)
+  std::vector<float> h(nparam, 0.f);
+  std::mt19937_64 gen(seed);
+  for (size_t li = 0; li < L.size(); li++) {
+    const UNetLayer &l = L[li];
+    const int cin_used = li == 0 ? UNET_IN : l.cin;
+    const double bound = li == 8 ? 0.0 : std::sqrt(6.0 / (cin_used * l.taps));
+    std::uniform_real_distribution<double> U(-bound, bound);
+    for (int co = 0; co < l.cout; co++)
+      for (int ci = 0; ci < l.cin; ci++)
+        for (int t = 0; t < l.taps; t++) {
+          const double w = (ci < cin_used && bound > 0) ? U(gen) : 0.0;
+          h[l.w + ((size_t)co * l.cin + ci) * l.taps + t] = (float)w;
+        }
+  }
+  HIPCHK(hipMemcpy(P.p, h.data(), sizeof(float) * nparam, hipMemcpyHostToDevice));
+  fill_zero(G.p, sizeof(float) * nparam, stream);
+  fill_zero(Mo.p, sizeof(float) * nparam, stream);
+  fill_zero(Vo.p, sizeof(float) * nparam, stream);
+  HIPCHK(hipStreamSynchronize(stream));
+  rng = seed ^ 0x5851f42d4c957f2dull;
+  packed = false;
+}
+
+void UNet::pack(hipStream_t s) {
+  for (const UNetLayer &l : L) {
+    if (l.cout < 16) continue;
+    hipLaunchKernelGGL(k_pack, dim3(64), dim3(256), 0, s, P.p + l.w, l.cin, l.cout, l.taps, false, Wp.p + l.pf);
+    if (l.cin >= 16)
+      hipLaunchKernelGGL(k_pack, dim3(64), dim3(256), 0, s, P.p + l.w, l.cin, l.cout, l.taps, true, Wp.p + l.pt);
+  }
+  HIPCHK(hipGetLastError());
+  packed = true;
+}
+
+void UNet::ensure(int nx, int ny, int nz, bool train) {
+  if (!unet_shape_ok(nx, ny, nz))
+    throw std::invalid_argument("surrogate grid must have nx % 64 == 0, ny % 32 == 0, nz % 4 == 0");
+  if (A.nx == nx && A.ny == ny && A.nz == nz && (A.train || !train)) return;
+  A = UNetActs();
+  A.nx = nx; A.ny = ny; A.nz = nz; A.train = train;
+  const size_t N1 = (size_t)nx * ny * nz, N2 = N1 / 8, N3 = N2 / 8;
+  A.ae1.alloc(N1 * 16); A.s1.alloc(N1 * 16); A.t1.alloc(N1 * 16); A.a1.alloc(N1 * 16);
+  A.p1.alloc(N2 * 16); A.s2.alloc(N2 * 32); A.t2.alloc(N2 * 32); A.a2.alloc(N2 * 32); A.q1.alloc(N2 * 16);
+  A.p2.alloc(N3 * 32); A.a3.alloc(N3 * 64); A.q2.alloc(N3 * 32);
+  A.out.alloc(N1);
+  A.red.alloc(4096);
+  if (train) {
+    A.g1a.alloc(N1 * 16); A.g1b.alloc(N1 * 16); A.g1c.alloc(N1 * 16);
+    A.g2a.alloc(N2 * 32); A.g2b.alloc(N2 * 32); A.g2c.alloc(N2 * 32);
+    A.g3a.alloc(N3 * 64); A.g3b.alloc(N3 * 64);
+    A.dout.alloc(N1);
+    // partials of the weight gradients: taps * 256 blocks * 4 waves * cin * cout
+    A.part.alloc((size_t)27 * 256 * 4 * 64 * 64);
+  }
+}
+
+void UNet::forward(const uint16_t *X0, float *out, hipStream_t s) {
+  if (!packed) pack(s);
+  const int n1x = A.nx, n1y = A.ny, n1z = A.nz;
+  const int n2x = n1x / 2, n2y = n1y / 2, n2z = n1z / 2, n3x = n1x / 4, n3y = n1y / 4, n3z = n1z / 4;
+  const size_t N1 = (size_t)n1x * n1y * n1z, N2 = N1 / 8, N3 = N2 / 8;
+  const uint16_t *W = Wp.p;
+  const float *p = P.p;
+  conv<8, 16, 27, true>(X0, W + L[0].pf, p + L[0].b, A.ae1.p, n1x, n1y, n1z, s);
+  conv<16, 16, 27, true>(A.ae1.p, W + L[1].pf, p + L[1].b, A.s1.p, n1x, n1y, n1z, s);
+  hipLaunchKernelGGL(k_pool, dim3(nblk(N2 * 16)), dim3(TPB), 0, s, A.s1.p, A.p1.p, N2, 16, n2x, n2y);
+  conv<16, 32, 27, true>(A.p1.p, W + L[2].pf, p + L[2].b, A.s2.p, n2x, n2y, n2z, s);
+  hipLaunchKernelGGL(k_pool, dim3(nblk(N3 * 32)), dim3(TPB), 0, s, A.s2.p, A.p2.p, N3, 32, n3x, n3y);
+  conv<32, 64, 27, true>(A.p2.p, W + L[3].pf, p + L[3].b, A.a3.p, n3x, n3y, n3z, s);
+  conv<64, 32, 1, false>(A.a3.p, W + L[4].pf, p + L[4].b, A.q2.p, n3x, n3y, n3z, s);
+  hipLaunchKernelGGL(k_upadd, dim3(nblk(N2 * 32)), dim3(TPB), 0, s, A.q2.p, A.s2.p, A.t2.p, N2, 32, n2x, n2y);
+  conv<32, 32, 27, true>(A.t2.p, W + L[5].pf, p + L[5].b, A.a2.p, n2x, n2y, n2z, s);
+  conv<32, 16, 1, false>(A.a2.p, W + L[6].pf, p + L[6].b, A.q1.p, n2x, n2y, n2z, s);
+  hipLaunchKernelGGL(k_upadd, dim3(nblk(N1 * 16)), dim3(TPB), 0, s, A.q1.p, A.s1.p, A.t1.p, N1, 16, n1x, n1y);
+  conv<16, 16, 27, true>(A.t1.p, W + L[7].pf, p + L[7].b, A.a1.p, n1x, n1y, n1z, s);
+  hipLaunchKernelGGL(k_head, dim3(nblk(N1)), dim3(TPB), 0, s, reinterpret_cast<const u32x4 *>(A.a1.p),
+                     p + L[8].w, p + L[8].b, out, N1);
+  ew_launch_check();
+  (void)n3z;
+}
+
+void UNet::backward(const uint16_t *X0, const float *dout, hipStream_t s) {
+  if (!A.train) throw std::logic_error("activations were not allocated for training");
+  const int n1x = A.nx, n1y = A.ny, n1z = A.nz;
+  const int n2x = n1x / 2, n2y = n1y / 2, n2z = n1z / 2, n3x = n1x / 4, n3y = n1y / 4, n3z = n1z / 4;
+  const size_t N1 = (size_t)n1x * n1y * n1z, N2 = N1 / 8, N3 = N2 / 8;
+  const uint16_t *W = Wp.p;
+  const float *p = P.p;
+  float *g = G.p, *part = A.part.p;
+  // head
+  {
+    const unsigned nb = 512;
+    hipLaunchKernelGGL(k_head_bwd, dim3(nb), dim3(TPB), 0, s, A.a1.p, p + L[8].w, dout, A.g1a.p, N1, part);
+    hipLaunchKernelGGL(k_head_reduce, dim3(1), dim3(64), 0, s, part, (int)nb, g + L[8].w, g + L[8].b);
+    ew_launch_check();
+  }
+  // d1 (input t1, output a1; g1a = its masked output gradient)
+  conv_dw<16, 16, 27>(A.t1.p, A.g1a.p, part, g + L[7].w, n1x, n1y, n1z, s);
+  colsum<16>(A.g1a.p, N1, part, g + L[7].b, s);
+  conv<16, 16, 27, false>(A.g1a.p, W + L[7].pt, nullptr, A.g1b.p, n1x, n1y, n1z, s);   // dT1
+  // q1: dQ1 = downsum(dT1)
+  hipLaunchKernelGGL(k_downsum, dim3(nblk(N2 * 16)), dim3(TPB), 0, s, A.g1b.p, A.g2c.p, N2, 16, n2x, n2y);
+  ew_launch_check();
+  conv_dw<32, 16, 1>(A.a2.p, A.g2c.p, part, g + L[6].w, n2x, n2y, n2z, s);
+  colsum<16>(A.g2c.p, N2, part, g + L[6].b, s);
+  conv<16, 32, 1, false>(A.g2c.p, W + L[6].pt, nullptr, A.g2a.p, n2x, n2y, n2z, s);   // dA2
+  hipLaunchKernelGGL(k_mask, dim3(nblk(N2 * 32)), dim3(TPB), 0, s, A.g2a.p, A.a2.p, N2 * 32);
+  ew_launch_check();
+  // d2
+  conv_dw<32, 32, 27>(A.t2.p, A.g2a.p, part, g + L[5].w, n2x, n2y, n2z, s);
+  colsum<32>(A.g2a.p, N2, part, g + L[5].b, s);
+  conv<32, 32, 27, false>(A.g2a.p, W + L[5].pt, nullptr, A.g2b.p, n2x, n2y, n2z, s);   // dT2
+  // q2: dQ2 = downsum(dT2)
+  hipLaunchKernelGGL(k_downsum, dim3(nblk(N3 * 32)), dim3(TPB), 0, s, A.g2b.p, A.g3b.p, N3, 32, n3x, n3y);
+  ew_launch_check();
+  conv_dw<64, 32, 1>(A.a3.p, A.g3b.p, part, g + L[4].w, n3x, n3y, n3z, s);
+  colsum<32>(A.g3b.p, N3, part, g + L[4].b, s);
+  conv<32, 64, 1, false>(A.g3b.p, W + L[4].pt, nullptr, A.g3a.p, n3x, n3y, n3z, s);   // dA3
+  hipLaunchKernelGGL(k_mask, dim3(nblk(N3 * 64)), dim3(TPB), 0, s, A.g3a.p, A.a3.p, N3 * 64);
+  ew_launch_check();
+  // b
+  conv_dw<32, 64, 27>(A.p2.p, A.g3a.p, part, g + L[3].w, n3x, n3y, n3z, s);
+  colsum<64>(A.g3a.p, N3, part, g + L[3].b, s);
+  conv<64, 32, 27, false>(A.g3a.p, W + L[3].pt, nullptr, A.g3b.p, n3x, n3y, n3z, s);   // dP2
+  // dS2 = dT2 + pool_bwd(dP2), through relu of e2
+  hipLaunchKernelGGL(k_pool_bwd_add, dim3(nblk(N2 * 32)), dim3(TPB), 0, s, A.g3b.p, A.g2b.p, A.g2a.p, N2, 32, n2x,
+                     n2y);
+  hipLaunchKernelGGL(k_mask, dim3(nblk(N2 * 32)), dim3(TPB), 0, s, A.g2a.p, A.s2.p, N2 * 32);
+  ew_launch_check();
+  // e2
+  conv_dw<16, 32, 27>(A.p1.p, A.g2a.p, part, g + L[2].w, n2x, n2y, n2z, s);
+  colsum<32>(A.g2a.p, N2, part, g + L[2].b, s);
+  conv<32, 16, 27, false>(A.g2a.p, W + L[2].pt, nullptr, A.g2c.p, n2x, n2y, n2z, s);   // dP1
+  // dS1 = dT1 + pool_bwd(dP1), through relu of e1b
+  hipLaunchKernelGGL(k_pool_bwd_add, dim3(nblk(N1 * 16)), dim3(TPB), 0, s, A.g2c.p, A.g1b.p, A.g1c.p, N1, 16, n1x,
+                     n1y);
+  hipLaunchKernelGGL(k_mask, dim3(nblk(N1 * 16)), dim3(TPB), 0, s, A.g1c.p, A.s1.p, N1 * 16);
+  ew_launch_check();
+  // e1b
+  conv_dw<16, 16, 27>(A.ae1.p, A.g1c.p, part, g + L[1].w, n1x, n1y, n1z, s);
+  colsum<16>(A.g1c.p, N1, part, g + L[1].b, s);
+  conv<16, 16, 27, false>(A.g1c.p, W + L[1].pt, nullptr, A.g1a.p, n1x, n1y, n1z, s);   // dAe1
+  hipLaunchKernelGGL(k_mask, dim3(nblk(N1 * 16)), dim3(TPB), 0, s, A.g1a.p, A.ae1.p, N1 * 16);
+  ew_launch_check();
+  // e1 (no input gradient)
+  conv_dw<8, 16, 27>(X0, A.g1a.p, part, g + L[0].w, n1x, n1y, n1z, s);
+  colsum<16>(A.g1a.p, N1, part, g + L[0].b, s);
+}
+
+double UNet::loss_grad(const float *target, double scale, hipStream_t s) {
+  const size_t N1 = (size_t)A.nx * A.ny * A.nz;
+  const unsigned nb = 1024;
+  hipLaunchKernelGGL(k_mse, dim3(nb), dim3(TPB), 0, s, A.out.p, target, N1, (float)scale, A.dout.p, A.red.p);
+  HIPCHK(hipGetLastError());
+  std::vector<double> h(nb);
+  HIPCHK(hipMemcpyAsync(h.data(), A.red.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double t = 0.0;
+  for (double x : h) t += x;
+  return t / (double)N1;
+}
+
+void UNet::adam(double lr, double b1, double b2, double eps, hipStream_t s) {
+  adam_t++;
+  const double lr_t = lr * std::sqrt(1.0 - std::pow(b2, (double)adam_t)) / (1.0 - std::pow(b1, (double)adam_t));
+  hipLaunchKernelGGL(k_adam, dim3(nblk(nparam)), dim3(TPB), 0, s, P.p, G.p, Mo.p, Vo.p, nparam, (float)lr_t,
+                     (float)b1, (float)b2, (float)eps);
+  HIPCHK(hipGetLastError());
+  pack(s);
+}
+
+void unet_features(size_t N, const double *phi, const float *rho0, const float *rho1, const double *phisum,
+                   const double *norm, uint16_t *X0, hipStream_t s) {
+  hipLaunchKernelGGL(k_features, dim3(nblk(N)), dim3(TPB), 0, s, N, phi, rho0, rho1, phisum, norm[0], norm[1],
+                     norm[2], norm[3], reinterpret_cast<u32x4 *>(X0));
+  HIPCHK(hipGetLastError());
+}
+void unet_features_from_channels(size_t N, const float *ch4, uint16_t *X0, hipStream_t s) {
+  hipLaunchKernelGGL(k_features_ch, dim3(nblk(N)), dim3(TPB), 0, s, N, ch4, reinterpret_cast<u32x4 *>(X0));
+  HIPCHK(hipGetLastError());
+}
+void unet_apply(size_t N, const double *phi, const float *out, const double *outsum, double scale, double *phi_out,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_apply, dim3(nblk(N)), dim3(TPB), 0, s, N, phi, out, outsum, scale, phi_out);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace phafd
+
+// ================================================================= C ABI
+namespace {
+thread_local std::string g_unet_err;
+
+template <class Fn>
+int uguard(phafd_unet *u, Fn fn) {
+  try {
+    if (u) (void)hipSetDevice(u->net.device);
+    return fn();
+  } catch (const phafd::HipError &e) {
+    (u ? u->net.err : g_unet_err) = e.what();
+    return PHAFD_E_HIP;
+  } catch (const std::invalid_argument &e) {
+    (u ? u->net.err : g_unet_err) = e.what();
+    return PHAFD_E_ARG;
+  } catch (const std::exception &e) {
+    (u ? u->net.err : g_unet_err) = e.what();
+    return PHAFD_E_RUNTIME;
+  }
+}
+
+// boundary parameter count: layer 0 takes UNET_IN input channels (8 stored)
+size_t boundary_count(const phafd::UNet &n) { return n.nparam - (size_t)16 * (8 - phafd::UNET_IN) * 27; }
+
+// boundary <-> stored parameter vectors
+void to_stored(const phafd::UNet &n, const float *in, std::vector<float> &out) {
+  out.assign(n.nparam, 0.f);
+  const phafd::UNetLayer &l0 = n.L[0];
+  size_t k = 0;
+  for (int co = 0; co < l0.cout; co++)
+    for (int ci = 0; ci < phafd::UNET_IN; ci++)
+      for (int t = 0; t < 27; t++) out[l0.w + ((size_t)co * l0.cin + ci) * 27 + t] = in[k++];
+  std::memcpy(out.data() + l0.b, in + k, sizeof(float) * (n.nparam - l0.b));
+}
+void to_boundary(const phafd::UNet &n, const std::vector<float> &st, float *out) {
+  const phafd::UNetLayer &l0 = n.L[0];
+  size_t k = 0;
+  for (int co = 0; co < l0.cout; co++)
+    for (int ci = 0; ci < phafd::UNET_IN; ci++)
+      for (int t = 0; t < 27; t++) out[k++] = st[l0.w + ((size_t)co * l0.cin + ci) * 27 + t];
+  std::memcpy(out + k, st.data() + l0.b, sizeof(float) * (n.nparam - l0.b));
+}
+
+uint64_t next_rand(uint64_t &s) {   // splitmix64
+  uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+}  // namespace
+
+extern "C" {
+
+int phafd_unet_create(int device, unsigned long long seed, phafd_unet **out) {
+  if (!out) return PHAFD_E_ARG;
+  *out = nullptr;
+  auto u = std::make_unique<phafd_unet>();
+  const int rc = uguard(nullptr, [&] {
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) throw std::invalid_argument("no such HIP device");
+    u->net.init(device, seed);
+    return 0;
+  });
+  if (rc != 0) return rc;
+  *out = u.release();
+  return 0;
+}
+
+int phafd_unet_destroy(phafd_unet *u) {
+  if (!u) return 0;
+  (void)hipSetDevice(u->net.device);
+  if (u->net.stream) {
+    (void)hipStreamSynchronize(u->net.stream);
+    phafd::scan_forget_stream(u->net.stream);
+    (void)hipStreamDestroy(u->net.stream);
+  }
+  delete u;
+  return 0;
+}
+
+const char *phafd_unet_last_error(const phafd_unet *u) { return u ? u->net.err.c_str() : g_unet_err.c_str(); }
+
+int phafd_unet_nparams(phafd_unet *u, int64_t *n) {
+  return uguard(u, [&] {
+    *n = (int64_t)boundary_count(u->net);
+    return 0;
+  });
+}
+
+int phafd_unet_get_params(phafd_unet *u, float *host) {
+  return uguard(u, [&] {
+    std::vector<float> st(u->net.nparam);
+    HIPCHK(hipMemcpy(st.data(), u->net.P.p, sizeof(float) * u->net.nparam, hipMemcpyDeviceToHost));
+    to_boundary(u->net, st, host);
+    return 0;
+  });
+}
+
+int phafd_unet_set_params(phafd_unet *u, const float *host) {
+  return uguard(u, [&] {
+    std::vector<float> st;
+    to_stored(u->net, host, st);
+    phafd::UNet &n = u->net;
+    HIPCHK(hipMemcpy(n.P.p, st.data(), sizeof(float) * n.nparam, hipMemcpyHostToDevice));
+    phafd::fill_zero(n.G.p, sizeof(float) * n.nparam, n.stream);
+    phafd::fill_zero(n.Mo.p, sizeof(float) * n.nparam, n.stream);
+    phafd::fill_zero(n.Vo.p, sizeof(float) * n.nparam, n.stream);
+    n.adam_t = 0;
+    n.pack(n.stream);
+    HIPCHK(hipStreamSynchronize(n.stream));
+    return 0;
+  });
+}
+
+int phafd_unet_get_grads(phafd_unet *u, float *host, int zero) {
+  return uguard(u, [&] {
+    phafd::UNet &n = u->net;
+    HIPCHK(hipStreamSynchronize(n.stream));
+    std::vector<float> st(n.nparam);
+    HIPCHK(hipMemcpy(st.data(), n.G.p, sizeof(float) * n.nparam, hipMemcpyDeviceToHost));
+    to_boundary(n, st, host);
+    if (zero) {
+      phafd::fill_zero(n.G.p, sizeof(float) * n.nparam, n.stream);
+      HIPCHK(hipStreamSynchronize(n.stream));
+    }
+    return 0;
+  });
+}
+
+int phafd_unet_set_norm(phafd_unet *u, const double *norm5) {
+  return uguard(u, [&] {
+    for (int k = 0; k < 5; k++) u->net.norm[k] = norm5[k];
+    return 0;
+  });
+}
+int phafd_unet_get_norm(phafd_unet *u, double *norm5) {
+  return uguard(u, [&] {
+    for (int k = 0; k < 5; k++) norm5[k] = u->net.norm[k];
+    return 0;
+  });
+}
+
+int phafd_unet_forward(phafd_unet *u, int nx, int ny, int nz, const float *in4, float *out) {
+  return uguard(u, [&] {
+    phafd::UNet &n = u->net;
+    n.ensure(nx, ny, nz, true);
+    const size_t N = (size_t)nx * ny * nz;
+    u->hin.alloc(4 * N);
+    u->x0.alloc(8 * N);
+    HIPCHK(hipMemcpyAsync(u->hin.p, in4, sizeof(float) * 4 * N, hipMemcpyHostToDevice, n.stream));
+    phafd::unet_features_from_channels(N, u->hin.p, u->x0.p, n.stream);
+    n.forward(u->x0.p, n.A.out.p, n.stream);
+    HIPCHK(hipMemcpyAsync(out, n.A.out.p, sizeof(float) * N, hipMemcpyDeviceToHost, n.stream));
+    HIPCHK(hipStreamSynchronize(n.stream));
+    return 0;
+  });
+}
+
+int phafd_unet_backward(phafd_unet *u, const float *dout) {
+  return uguard(u, [&] {
+    phafd::UNet &n = u->net;
+    if (!n.A.train || !u->x0.p) throw std::invalid_argument("call phafd_unet_forward first");
+    const size_t N = (size_t)n.A.nx * n.A.ny * n.A.nz;
+    HIPCHK(hipMemcpyAsync(n.A.dout.p, dout, sizeof(float) * N, hipMemcpyHostToDevice, n.stream));
+    n.backward(u->x0.p, n.A.dout.p, n.stream);
+    HIPCHK(hipStreamSynchronize(n.stream));
+    return 0;
+  });
+}
+
+int phafd_unet_data_alloc(phafd_unet *u, int nx, int ny, int nz, int nsamples) {
+  return uguard(u, [&] {
+    phafd::UNet &n = u->net;
+    if (nsamples < 1) throw std::invalid_argument("need at least one sample");
+    if (!phafd::unet_shape_ok(nx, ny, nz))
+      throw std::invalid_argument("surrogate grid must have nx % 64 == 0, ny % 32 == 0, nz % 4 == 0");
+    const size_t N = (size_t)nx * ny * nz;
+    n.dsX.release();
+    n.dsY.release();
+    n.dsX.alloc(N * 8 * nsamples);
+    n.dsY.alloc(N * nsamples);
+    n.ds_n = nsamples;
+    n.ds_nx = nx; n.ds_ny = ny; n.ds_nz = nz;
+    return 0;
+  });
+}
+
+int phafd_unet_data_put(phafd_unet *u, int i, const float *in4, const float *target) {
+  return uguard(u, [&] {
+    phafd::UNet &n = u->net;
+    if (i < 0 || i >= n.ds_n) throw std::invalid_argument("sample index out of range");
+    const size_t N = (size_t)n.ds_nx * n.ds_ny * n.ds_nz;
+    u->hin.alloc(4 * N);
+    HIPCHK(hipMemcpyAsync(u->hin.p, in4, sizeof(float) * 4 * N, hipMemcpyHostToDevice, n.stream));
+    phafd::unet_features_from_channels(N, u->hin.p, n.dsX.p + (size_t)i * N * 8, n.stream);
+    HIPCHK(hipMemcpyAsync(n.dsY.p + (size_t)i * N, target, sizeof(float) * N, hipMemcpyHostToDevice, n.stream));
+    HIPCHK(hipStreamSynchronize(n.stream));
+    return 0;
+  });
+}
+
+// nsteps Adam steps over minibatches of `batch` samples drawn from samples
+// [0, ntrain); losses[k] = mean square error of step k's minibatch
+int phafd_unet_train(phafd_unet *u, int nsteps, int batch, int ntrain, double lr, double *losses) {
+  return uguard(u, [&] {
+    phafd::UNet &n = u->net;
+    if (n.ds_n < 1) throw std::invalid_argument("no training data");
+    if (batch < 1 || ntrain < 1 || ntrain > n.ds_n) throw std::invalid_argument("bad batch or training range");
+    n.ensure(n.ds_nx, n.ds_ny, n.ds_nz, true);
+    const size_t N = (size_t)n.ds_nx * n.ds_ny * n.ds_nz;
+    for (int k = 0; k < nsteps; k++) {
+      double lsum = 0.0;
+      for (int b = 0; b < batch; b++) {
+        const int idx = (int)(next_rand(n.rng) % (uint64_t)ntrain);
+        const uint16_t *x = n.dsX.p + (size_t)idx * N * 8;
+        n.forward(x, n.A.out.p, n.stream);
+        lsum += n.loss_grad(n.dsY.p + (size_t)idx * N, 1.0 / ((double)N * batch), n.stream);
+        n.backward(x, n.A.dout.p, n.stream);
+      }
+      n.adam(lr, 0.9, 0.999, 1e-8, n.stream);
+      if (losses) losses[k] = lsum / batch;
+    }
+    HIPCHK(hipStreamSynchronize(n.stream));
+    return 0;
+  });
+}
+
+// mean square error of samples [i0, i0 + count) (no gradients)
+int phafd_unet_eval(phafd_unet *u, int i0, int count, double *losses) {
+  return uguard(u, [&] {
+    phafd::UNet &n = u->net;
+    if (i0 < 0 || count < 0 || i0 + count > n.ds_n) throw std::invalid_argument("sample range out of bounds");
+    n.ensure(n.ds_nx, n.ds_ny, n.ds_nz, true);
+    const size_t N = (size_t)n.ds_nx * n.ds_ny * n.ds_nz;
+    for (int k = 0; k < count; k++) {
+      const uint16_t *x = n.dsX.p + (size_t)(i0 + k) * N * 8;
+      n.forward(x, n.A.out.p, n.stream);
+      losses[k] = n.loss_grad(n.dsY.p + (size_t)(i0 + k) * N, 0.0, n.stream);
+    }
+    return 0;
+  });
+}
+
+}  // extern "C"

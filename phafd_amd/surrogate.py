@@ -1,0 +1,152 @@
+"""Training data and training of the surrogate network on the GPU.
+
+Trajectories of the plain solver at 64^3 (dx = 1, periodic ghosts along z)
+in three families -- the 64^3 reference case (phi0 0.5, chi 3, 20 x 50-bead
+chains, gridatom/gaussian cut 3), the 256^3 production case's bead density
+and composition at 64^3 (phi0 0.4, chi 2.5, 80 x 100 beads, gaussian cut 3)
+and the 512^3 case's coupling at 64^3 (LJish cut 2.5, width 2.5) -- are cut
+into blocks of K steps.  A sample is the network input (phi at the block
+start, the bead density at its start and end) and the target (phi after K
+steps - the exponential-Euler base) / norm[4].  The
+network is trained on the GPU (phafd_unet_train); the tolerance of the
+block check is set from the residuals of the plain method's own blocks.
+
+    python -m phafd_amd.surrogate --out weights.npz [--trajectories 20]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+FAMILIES = {
+    # name: (name, phi0, var, chi, nchains, nbeads, style, cut, width)
+    "reference64": ("reference64", 0.5, 0.02, 3.0, 20, 50, 0, 3.0, 1.0),
+    "production": ("production", 0.4, 0.04, 2.5, 80, 100, 0, 3.0, 1.0),
+    "ljish": ("ljish", 0.4, 0.04, 2.5, 80, 100, 1, 2.5, 2.5),
+}
+# trajectory mixes: the Gaussian-coupled network (the 64^3 reference case
+# and the production case's composition) and the LJish-coupled one
+MIXES = {
+    "gaussian": ["production", "reference64", "production", "production"],
+    "ljish": ["ljish"],
+}
+
+
+def make_system(P, n, fam, seed):
+    """A plain-solver system of one family (the same script commands as the
+    benchmark workloads)."""
+    _, phi0, var, chi, nch, nb, style, cut, width = fam
+    s = P.System(n, n, n, [0, 0, 0], [float(n)] * 3, zghost=P.ZGHOST_PERIODIC)
+    s.set_field(P.PHI, P.ic_noisy_constant((n, n, n), phi0, var, 1234 + seed))
+    xyz, mol = P.ic_random_chains(nch, nb, 1.0, 1.0, [0, 0, 0], [float(n)] * 3, 42 + seed)
+    s.set_atoms(xyz, np.zeros(len(xyz), np.int32), tags=np.arange(len(xyz), dtype=np.int32), mols=mol)
+    ip = s.pair_gridatom(style, cut)
+    s.pair_coeff(ip, 0, 1.0, 0.1, width)
+    s.pair_ljcut(1.122462048, 1.0, 1.0)
+    s.fix_conjugate(5678, 1.0, 1.0, 1.0, 1.0, P.NOISE_OFF)
+    s.fix_floryhuggins(1.0, 1.0, chi)
+    st = (np.arange(nch) * nb).astype(np.int32)
+    s.fix_chain(st, st + nb, 1.0, 100.0, 10.0, 1.0)
+    s.neighbor(0.5)
+    s.setup(1e-3)
+    return s
+
+
+def blocks(P, s, nblocks, K):
+    """(phi0, rho0, rho1, base, phi1, plain residual) of consecutive K-step
+    blocks (rho0, rho1: bead densities at the block's start and end)."""
+    rho1 = None
+    for _ in range(nblocks):
+        phi0 = s.get_field(P.PHI)
+        rho0 = s.density() if rho1 is None else rho1
+        s.run(K)
+        phi1 = s.get_field(P.PHI)
+        rho1 = s.density()
+        base = s.surrogate_base(phi0, rho0, K)
+        res, _ = s.block_residual(phi0, phi1, rho0, rho1, K)
+        yield phi0, rho0, rho1, base, phi1, res
+
+
+def build_dataset(P, net, mix="gaussian", n=64, ntraj=20, nblocks=100, K=10, log=print):
+    """Fill the network's device training set; returns per-sample metadata
+    (|phi1 - phi0|, |phi1 - base|, plain residual, trajectory) and the norm
+    used (its scales taken from the first trajectory's first block)."""
+    names = MIXES[mix]
+    fams = [FAMILIES[names[i % len(names)]] for i in range(ntraj)]
+    norm = None
+    meta = []
+    net.data_alloc(n, n, n, ntraj * nblocks)
+    k = 0
+    t0 = time.time()
+    for t, fam in enumerate(fams):
+        s = make_system(P, n, fam, seed=100 + t)
+        for phi0, rho0, rho1, base, phi1, res in blocks(P, s, nblocks, K):
+            d = phi1 - base
+            if norm is None:   # scales from the first block: O(1) inputs and targets
+                norm = np.array([max(float(np.std(phi0)), 1e-6), 1.0, 0.4, 0.1, max(float(np.std(d)), 1e-12)])
+                net.set_norm(norm)
+            x = P.surrogate_inputs(phi0, rho0, rho1, norm)
+            net.data_put(k, x, (d / norm[4]).astype(np.float32))
+            meta.append((float(np.linalg.norm(phi1 - phi0)), float(np.linalg.norm(d)), res, t))
+            k += 1
+        s.close()
+        log(f"trajectory {t + 1}/{ntraj} ({fam[0]}) done, {k} samples, {time.time() - t0:.1f} s")
+    return np.array(meta), norm
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--trajectories", type=int, default=20)
+    ap.add_argument("--blocks", type=int, default=100)
+    ap.add_argument("--K", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--holdout", type=int, default=2, help="trajectories kept for validation")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--mix", default="gaussian", choices=sorted(MIXES))
+    args = ap.parse_args(argv)
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import phafd_amd as P
+
+    def log(msg):
+        print(msg, flush=True)
+
+    net = P.UNet(seed=args.seed)
+    meta, norm = build_dataset(P, net, mix=args.mix, ntraj=args.trajectories, nblocks=args.blocks, K=args.K, log=log)
+    ntrain = (args.trajectories - args.holdout) * args.blocks
+    nval = len(meta) - ntrain
+    t0 = time.time()
+    chunks = [1.0, 0.7, 0.5, 0.3, 0.15, 0.05]
+    per = max(1, args.steps // len(chunks))
+    hist = []
+    for c in chunks:
+        losses = net.train(per, args.batch, ntrain, args.lr * c)
+        val = net.eval(ntrain, nval) if nval else np.zeros(0)
+        hist.append((float(losses.mean()), float(val.mean()) if nval else None))
+        log(f"lr {args.lr * c:.2e}: train mse {hist[-1][0]:.4e}, validation mse {hist[-1][1]}, "
+            f"{time.time() - t0:.1f} s")
+    # relative error of the predicted change on the validation blocks
+    val = net.eval(ntrain, nval) if nval else np.zeros(0)
+    N = 64 ** 3
+    err_pred = np.sqrt(val * N) * norm[4]
+    rel = err_pred / meta[ntrain:, 0] if nval else np.zeros(0)
+    base_only = meta[ntrain:, 1] / meta[ntrain:, 0] if nval else np.zeros(0)
+    tol = 1.5 * float(meta[:, 2].max())
+    summary = {"mix": args.mix, "norm": norm.tolist(), "tol": tol, "K": args.K, "train_samples": ntrain,
+               "validation_samples": nval,
+               "history": hist, "validation_rel_err_change_mean": float(rel.mean()) if nval else None,
+               "validation_rel_err_change_max": float(rel.max()) if nval else None,
+               "base_only_rel_err_change_mean": float(base_only.mean()) if nval else None,
+               "plain_residual_max": float(meta[:, 2].max()), "train_seconds": time.time() - t0}
+    np.savez(args.out, params=net.get_params(), norm=norm, tol=np.array([tol]), K=np.array([args.K]))
+    log(json.dumps(summary))
+    net.close()
+
+
+if __name__ == "__main__":
+    main()
