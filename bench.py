@@ -159,6 +159,57 @@ def read_traffic(kernel, workload):
     return None
 
 
+SURROGATE = os.path.join(ROOT, "phafd_amd", "data", "surrogate_gaussian64.npz")
+
+
+def surrogate_leg(args, wl, phi, xyz, mol, ref_phi, fp64_elapsed):
+    """The surrogate-accelerated form on the same workload: `warmup` plain
+    steps, then blocks of K steps (network + check, plain fallback) timed over
+    the largest multiple of K <= steps; phi compared with the FP64 run at the
+    same step.  Then both forms run `accuracy_steps` from the initial state
+    (untimed) for the long-horizon error."""
+    import phafd_amd as P
+    z = np.load(SURROGATE)
+    K, tol = int(z["K"][0]), float(z["tol"][0])
+    steps = (args.steps // K) * K
+    n = wl["n"]
+    net = P.UNet(seed=1)
+    net.load(SURROGATE)
+    s = P.System(n, n, n, [0, 0, 0], [wl["box"]] * 3, zghost=wl["zghost"])
+    configure(s, wl, phi, xyz, mol)
+    s.run(args.warmup)
+    s.run_surrogate(net, 0, K, tol)   # allocations outside the timed region
+    t0 = time.perf_counter()
+    st = s.run_surrogate(net, steps, K, tol)
+    el = time.perf_counter() - t0
+    got = s.get_field(P.PHI)
+    s.close()
+    out = {"value": n ** 3 * steps / el, "unit": "grid-pt·steps/s", "ms_per_step": 1e3 * el / steps,
+           "speedup_vs_fp64": (fp64_elapsed / args.steps) / (el / steps), "K": K, "tol": tol,
+           "blocks": int(st["blocks"]), "accepted": int(st["accepted"]), "fallbacks": int(st["fallbacks"]),
+           "max_residual": st["max_residual"], "ms_network_path_per_block": st["ms_network_path"] / max(st["blocks"], 1),
+           "form": "per block of K steps: atoms K steps with the block-start grid-atom force held, bead densities "
+                   "at both ends, exponential-Euler spectral base + BF16 U-Net correction, K-step residual check, "
+                   "plain fallback; FP64 fields"}
+    if steps == args.steps:
+        out["phi_rel_l2_vs_fp64"] = rel_err(got, ref_phi)
+    if args.accuracy_steps:
+        A = (args.accuracy_steps // K) * K
+        a = P.System(n, n, n, [0, 0, 0], [wl["box"]] * 3, zghost=wl["zghost"])
+        configure(a, wl, phi, xyz, mol)
+        a.run(A)
+        pa = a.get_field(P.PHI)
+        a.close()
+        b = P.System(n, n, n, [0, 0, 0], [wl["box"]] * 3, zghost=wl["zghost"])
+        configure(b, wl, phi, xyz, mol)
+        stb = b.run_surrogate(net, A, K, tol)
+        out[f"phi_rel_l2_vs_fp64_after_{A}_steps"] = rel_err(b.get_field(P.PHI), pa)
+        out[f"accepted_of_{A // K}_blocks"] = int(stb["accepted"])
+        b.close()
+    net.close()
+    return out
+
+
 def bench_batch(args):
     """Batched replicas: the batch timed over `steps`, then the same members
     run one after another as solo systems (the state of every member is
@@ -256,6 +307,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form leg")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="steps of the plain-vs-surrogate accuracy comparison (0: skip)")
     ap.add_argument("--timers", action="store_true", help="print per-hook GPU times to stderr")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -308,6 +362,10 @@ def main():
                 "rel_err_vs_fp64": err, "steps_compared": args.warmup + args.steps,
                 "form": "FP32 FFT + fields, Kahan-compensated grid-atom sums, FP64 k=0 mode, atoms and energies"}
 
+    sur = None
+    if not args.no_surrogate and not wl["ljish"] and os.path.exists(SURROGATE):
+        sur = surrogate_leg(args, wl, phi, xyz, mol, ref["phi"], elapsed)
+
     nbeads = wl["nchains"] * wl["nbeads"]
     ms_per_step = 1e3 * elapsed / args.steps
     peak = 8.0e3   # GB/s, MI355X HBM3E
@@ -338,6 +396,8 @@ def main():
     }
     if fp32 is not None:
         out["fp32"] = fp32
+    if sur is not None:
+        out["surrogate"] = sur
     if args.timers:
         print(json.dumps({"per_hook_ms_total": timers}), file=sys.stderr)
     if not args.no_cpu_baseline:

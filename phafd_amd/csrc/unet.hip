@@ -29,7 +29,6 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int TPB = 256;
-constexpr int MT = 2;   // rows of 16 points per wave in the convolution
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 // round to nearest even (the conversion PyTorch applies)
@@ -50,36 +49,60 @@ size_t nblk(size_t n) {
 
 // ------------------------------------------------------------ convolution
 // Y[v][co] = act(bias[co] + sum_{tap, ci} W[co][ci][tap] X[v + off(tap)][ci])
-// W packed: [chunk][n-tile][lane][8] (B-operand order).  Grid: (nx/16) *
-// (ny/8) * nz blocks of 4 waves.
-template <int CIN, int COUT, int TAPS, bool RELU>
+// W packed: [chunk][n-tile][lane][8] (B-operand order).  A block of 4 waves
+// computes a tile of 16 (x) by 4 * MTR (y) points at one z, MTR rows per
+// wave; its input halo (3 x (4 MTR + 2) x 18 points for a 3^3 kernel) is
+// staged in LDS once (periodic wrap applied there), so the K loop reads the
+// A operands from LDS with fixed offsets per tap.  Grid: (nx/16) *
+// (ny/(4 MTR)) * nz blocks.
+template <int CIN, int TAPS, int MTR>
+struct ConvTile {
+  static constexpr int CG = CIN / 8;
+  static constexpr int TY = 4 * MTR;
+  static constexpr int HX = TAPS == 27 ? 18 : 16;
+  static constexpr int HY = TAPS == 27 ? TY + 2 : TY;
+  static constexpr int HZ = TAPS == 27 ? 3 : 1;
+  static constexpr int ELEMS = HZ * HY * HX * CG;   // 16-byte entries
+};
+
+template <int CIN, int COUT, int TAPS, bool RELU, int MTR>
 __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const u32x4 *__restrict__ W,
                                               const float *__restrict__ bias, uint16_t *__restrict__ Y, int nx,
                                               int ny, int nz) {
-  constexpr int CG = CIN / 8;
+  using T = ConvTile<CIN, TAPS, MTR>;
+  constexpr int CG = T::CG;
   constexpr int G = TAPS * CG;
   constexpr int NC = (G + 3) / 4;
   constexpr int NT = COUT / 16;
+  __shared__ u32x4 tile[T::ELEMS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int xt = nx >> 4, yt = ny / (4 * MT);
+  const int xt = nx >> 4, yt = ny / T::TY;
   int b = blockIdx.x;
   const int bx = b % xt;
   b /= xt;
   const int by = b % yt;
   const int z = b / yt;
-  const int x0 = bx * 16;
-  const int y0 = by * 4 * MT + wave * MT;
-  const int xi = x0 + (lane & 15);
+  const int x0 = bx * 16, y0 = by * T::TY;
+  constexpr int H = TAPS == 27 ? 1 : 0;   // halo width
+  for (int t = threadIdx.x; t < T::ELEMS; t += 256) {
+    const int cg = t % CG;
+    const int v = t / CG;
+    const int hx = v % T::HX, hy = (v / T::HX) % T::HY, hz = v / (T::HX * T::HY);
+    const int xx = wrap1(x0 - H + hx, nx), yy = wrap1(y0 - H + hy, ny), zz = wrap1(z - H + hz, nz);
+    tile[t] = X[(((size_t)zz * ny + yy) * nx + xx) * CG + cg];
+  }
+  __syncthreads();
   const int kg = lane >> 4;
-  f32x4 acc[MT][NT];
+  const int lx = lane & 15;
+  f32x4 acc[MTR][NT];
 #pragma unroll
-  for (int r = 0; r < MT; r++)
+  for (int r = 0; r < MTR; r++)
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 2
   for (int c = 0; c < NC; c++) {
     const int g = 4 * c + kg;
-    bf16x8 a[MT];
+    bf16x8 a[MTR];
     if (g < G) {
       const int tap = g / CG, cg = g % CG;
       int dx = 0, dy = 0, dz = 0;
@@ -88,32 +111,27 @@ __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const
         dy = (tap / 3) % 3 - 1;
         dx = tap % 3 - 1;
       }
-      const int xx = wrap1(xi + dx, nx);
-      const int zz = wrap1(z + dz, nz);
+      const int base = (((dz + H) * T::HY + (wave * MTR + dy + H)) * T::HX + (lx + dx + H)) * CG + cg;
 #pragma unroll
-      for (int r = 0; r < MT; r++) {
-        const int yy = wrap1(y0 + r + dy, ny);
-        const size_t v = ((size_t)zz * ny + yy) * nx + xx;
-        a[r] = __builtin_bit_cast(bf16x8, X[v * CG + cg]);
-      }
+      for (int r = 0; r < MTR; r++) a[r] = __builtin_bit_cast(bf16x8, tile[base + r * T::HX * CG]);
     } else {
 #pragma unroll
-      for (int r = 0; r < MT; r++) a[r] = __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u});
+      for (int r = 0; r < MTR; r++) a[r] = __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u});
     }
 #pragma unroll
     for (int t = 0; t < NT; t++) {
       const bf16x8 w = __builtin_bit_cast(bf16x8, W[((size_t)c * NT + t) * 64 + lane]);
 #pragma unroll
-      for (int r = 0; r < MT; r++) acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], w, acc[r][t], 0, 0, 0);
+      for (int r = 0; r < MTR; r++) acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], w, acc[r][t], 0, 0, 0);
     }
   }
   // D layout: lane holds rows 4*(lane/16) + j (points), column lane % 16 (channel)
 #pragma unroll
-  for (int r = 0; r < MT; r++) {
-    const size_t row = ((size_t)z * ny + (y0 + r)) * nx + x0;
+  for (int r = 0; r < MTR; r++) {
+    const size_t row = ((size_t)z * ny + (y0 + wave * MTR + r)) * nx + x0;
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-      const int co = t * 16 + (lane & 15);
+      const int co = t * 16 + lx;
       const float bb = bias ? bias[co] : 0.f;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
@@ -271,47 +289,84 @@ __device__ __forceinline__ size_t parent(size_t f, int fx_n, int fy_n) {
   return ((size_t)(z >> 1) * (fy_n >> 1) + (y >> 1)) * (fx_n >> 1) + (x >> 1);
 }
 
-// P[p][c] = mean of the 8 children (coarse dims cx, cy; Nc coarse points)
-__global__ void k_pool(const uint16_t *__restrict__ X, uint16_t *__restrict__ P, size_t Nc, int C, int cx, int cy) {
-  const size_t n = Nc * C;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const size_t p = i / C;
-    const int c = (int)(i % C);
-    float s = 0.f;
-    for (int k = 0; k < 8; k++) s += bf2f(X[child(p, k, cx, cy) * C + c]);
-    P[i] = f2bf(s * 0.125f);
+// One thread per (point, group of 8 channels), 16-byte loads and stores;
+// C channels = C / 8 groups per point.
+__device__ __forceinline__ void unpack8(const u32x4 &u, float f[8]) {
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    f[2 * e] = __uint_as_float(u[e] << 16);
+    f[2 * e + 1] = __uint_as_float(u[e] & 0xffff0000u);
   }
 }
-// dX[f][c] = dA[f][c] + dP[parent][c] / 8 (fine dims fx, fy)
-__global__ void k_pool_bwd_add(const uint16_t *__restrict__ dP, const uint16_t *__restrict__ dA,
-                               uint16_t *__restrict__ dX, size_t Nf, int C, int fx, int fy) {
-  const size_t n = Nf * C;
+__device__ __forceinline__ u32x4 pack8(const float f[8]) {
+  u32x4 u;
+#pragma unroll
+  for (int e = 0; e < 4; e++) u[e] = (uint32_t)f2bf(f[2 * e]) | ((uint32_t)f2bf(f[2 * e + 1]) << 16);
+  return u;
+}
+
+// P[p] = mean of the 8 children (coarse dims cx, cy; Nc coarse points)
+__global__ void k_pool(const u32x4 *__restrict__ X, u32x4 *__restrict__ P, size_t Nc, int CG, int cx, int cy) {
+  const size_t n = Nc * CG;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const size_t f = i / C;
-    const int c = (int)(i % C);
-    dX[i] = f2bf(bf2f(dA[i]) + bf2f(dP[parent(f, fx, fy) * C + c]) * 0.125f);
+    const size_t p = i / CG;
+    const int g = (int)(i % CG);
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < 8; k++) {
+      float f[8];
+      unpack8(X[child(p, k, cx, cy) * CG + g], f);
+#pragma unroll
+      for (int e = 0; e < 8; e++) s[e] += f[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++) s[e] *= 0.125f;
+    P[i] = pack8(s);
   }
 }
-// T[f][c] = S[f][c] + Q[parent][c]
-__global__ void k_upadd(const uint16_t *__restrict__ Q, const uint16_t *__restrict__ S, uint16_t *__restrict__ T,
-                        size_t Nf, int C, int fx, int fy) {
-  const size_t n = Nf * C;
+// dX[f] = dA[f] + dP[parent] / 8 (fine dims fx, fy)
+__global__ void k_pool_bwd_add(const u32x4 *__restrict__ dP, const u32x4 *__restrict__ dA, u32x4 *__restrict__ dX,
+                               size_t Nf, int CG, int fx, int fy) {
+  const size_t n = Nf * CG;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const size_t f = i / C;
-    const int c = (int)(i % C);
-    T[i] = f2bf(bf2f(S[i]) + bf2f(Q[parent(f, fx, fy) * C + c]));
+    const size_t f = i / CG;
+    const int g = (int)(i % CG);
+    float a[8], q[8];
+    unpack8(dA[i], a);
+    unpack8(dP[parent(f, fx, fy) * CG + g], q);
+#pragma unroll
+    for (int e = 0; e < 8; e++) a[e] = a[e] + q[e] * 0.125f;
+    dX[i] = pack8(a);
   }
 }
-// dQ[p][c] = sum of the 8 children of dT
-__global__ void k_downsum(const uint16_t *__restrict__ dT, uint16_t *__restrict__ dQ, size_t Nc, int C, int cx,
-                          int cy) {
-  const size_t n = Nc * C;
+// T[f] = S[f] + Q[parent]
+__global__ void k_upadd(const u32x4 *__restrict__ Q, const u32x4 *__restrict__ S, u32x4 *__restrict__ T, size_t Nf,
+                        int CG, int fx, int fy) {
+  const size_t n = Nf * CG;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const size_t p = i / C;
-    const int c = (int)(i % C);
-    float s = 0.f;
-    for (int k = 0; k < 8; k++) s += bf2f(dT[child(p, k, cx, cy) * C + c]);
-    dQ[i] = f2bf(s);
+    const size_t f = i / CG;
+    const int g = (int)(i % CG);
+    float a[8], q[8];
+    unpack8(S[i], a);
+    unpack8(Q[parent(f, fx, fy) * CG + g], q);
+#pragma unroll
+    for (int e = 0; e < 8; e++) a[e] = a[e] + q[e];
+    T[i] = pack8(a);
+  }
+}
+// dQ[p] = sum of the 8 children of dT
+__global__ void k_downsum(const u32x4 *__restrict__ dT, u32x4 *__restrict__ dQ, size_t Nc, int CG, int cx, int cy) {
+  const size_t n = Nc * CG;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t p = i / CG;
+    const int g = (int)(i % CG);
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < 8; k++) {
+      float f[8];
+      unpack8(dT[child(p, k, cx, cy) * CG + g], f);
+#pragma unroll
+      for (int e = 0; e < 8; e++) s[e] += f[e];
+    }
+    dQ[i] = pack8(s);
   }
 }
 // D = D where A > 0, else 0 (gradient through relu, A its output)
@@ -464,12 +519,21 @@ __global__ void k_apply(size_t N, const double *__restrict__ phi, const float *_
 }
 
 // ------------------------------------------------------------ launchers
+// rows per wave: 4 when the rows of a level allow it and the halo tile fits
+// comfortably in LDS (64 input channels: 2)
 template <int CIN, int COUT, int TAPS, bool RELU>
 void conv(const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y, int nx, int ny, int nz,
           hipStream_t s) {
-  const unsigned blocks = (unsigned)((nx / 16) * (ny / (4 * MT)) * nz);
-  hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU>), dim3(blocks), dim3(256), 0, s,
-                     reinterpret_cast<const u32x4 *>(X), reinterpret_cast<const u32x4 *>(W), bias, Y, nx, ny, nz);
+  const bool four = CIN < 64 && ny % 16 == 0;
+  if (four) {
+    const unsigned blocks = (unsigned)((nx / 16) * (ny / 16) * nz);
+    hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 4>), dim3(blocks), dim3(256), 0, s,
+                       reinterpret_cast<const u32x4 *>(X), reinterpret_cast<const u32x4 *>(W), bias, Y, nx, ny, nz);
+  } else {
+    const unsigned blocks = (unsigned)((nx / 16) * (ny / 8) * nz);
+    hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 2>), dim3(blocks), dim3(256), 0, s,
+                       reinterpret_cast<const u32x4 *>(X), reinterpret_cast<const u32x4 *>(W), bias, Y, nx, ny, nz);
+  }
   HIPCHK(hipGetLastError());
 }
 
@@ -497,6 +561,7 @@ void colsum(const uint16_t *D, size_t N, float *part, float *g, hipStream_t s) {
 }
 
 void ew_launch_check() { HIPCHK(hipGetLastError()); }
+u32x4 *V(uint16_t *p) { return reinterpret_cast<u32x4 *>(p); }
 
 }  // namespace
 
@@ -599,15 +664,15 @@ void UNet::forward(const uint16_t *X0, float *out, hipStream_t s) {
   const float *p = P.p;
   conv<8, 16, 27, true>(X0, W + L[0].pf, p + L[0].b, A.ae1.p, n1x, n1y, n1z, s);
   conv<16, 16, 27, true>(A.ae1.p, W + L[1].pf, p + L[1].b, A.s1.p, n1x, n1y, n1z, s);
-  hipLaunchKernelGGL(k_pool, dim3(nblk(N2 * 16)), dim3(TPB), 0, s, A.s1.p, A.p1.p, N2, 16, n2x, n2y);
+  hipLaunchKernelGGL(k_pool, dim3(nblk(N2 * 2)), dim3(TPB), 0, s, V(A.s1.p), V(A.p1.p), N2, 2, n2x, n2y);
   conv<16, 32, 27, true>(A.p1.p, W + L[2].pf, p + L[2].b, A.s2.p, n2x, n2y, n2z, s);
-  hipLaunchKernelGGL(k_pool, dim3(nblk(N3 * 32)), dim3(TPB), 0, s, A.s2.p, A.p2.p, N3, 32, n3x, n3y);
+  hipLaunchKernelGGL(k_pool, dim3(nblk(N3 * 4)), dim3(TPB), 0, s, V(A.s2.p), V(A.p2.p), N3, 4, n3x, n3y);
   conv<32, 64, 27, true>(A.p2.p, W + L[3].pf, p + L[3].b, A.a3.p, n3x, n3y, n3z, s);
   conv<64, 32, 1, false>(A.a3.p, W + L[4].pf, p + L[4].b, A.q2.p, n3x, n3y, n3z, s);
-  hipLaunchKernelGGL(k_upadd, dim3(nblk(N2 * 32)), dim3(TPB), 0, s, A.q2.p, A.s2.p, A.t2.p, N2, 32, n2x, n2y);
+  hipLaunchKernelGGL(k_upadd, dim3(nblk(N2 * 4)), dim3(TPB), 0, s, V(A.q2.p), V(A.s2.p), V(A.t2.p), N2, 4, n2x, n2y);
   conv<32, 32, 27, true>(A.t2.p, W + L[5].pf, p + L[5].b, A.a2.p, n2x, n2y, n2z, s);
   conv<32, 16, 1, false>(A.a2.p, W + L[6].pf, p + L[6].b, A.q1.p, n2x, n2y, n2z, s);
-  hipLaunchKernelGGL(k_upadd, dim3(nblk(N1 * 16)), dim3(TPB), 0, s, A.q1.p, A.s1.p, A.t1.p, N1, 16, n1x, n1y);
+  hipLaunchKernelGGL(k_upadd, dim3(nblk(N1 * 2)), dim3(TPB), 0, s, V(A.q1.p), V(A.s1.p), V(A.t1.p), N1, 2, n1x, n1y);
   conv<16, 16, 27, true>(A.t1.p, W + L[7].pf, p + L[7].b, A.a1.p, n1x, n1y, n1z, s);
   hipLaunchKernelGGL(k_head, dim3(nblk(N1)), dim3(TPB), 0, s, reinterpret_cast<const u32x4 *>(A.a1.p),
                      p + L[8].w, p + L[8].b, out, N1);
@@ -635,7 +700,7 @@ void UNet::backward(const uint16_t *X0, const float *dout, hipStream_t s) {
   colsum<16>(A.g1a.p, N1, part, g + L[7].b, s);
   conv<16, 16, 27, false>(A.g1a.p, W + L[7].pt, nullptr, A.g1b.p, n1x, n1y, n1z, s);   // dT1
   // q1: dQ1 = downsum(dT1)
-  hipLaunchKernelGGL(k_downsum, dim3(nblk(N2 * 16)), dim3(TPB), 0, s, A.g1b.p, A.g2c.p, N2, 16, n2x, n2y);
+  hipLaunchKernelGGL(k_downsum, dim3(nblk(N2 * 2)), dim3(TPB), 0, s, V(A.g1b.p), V(A.g2c.p), N2, 2, n2x, n2y);
   ew_launch_check();
   conv_dw<32, 16, 1>(A.a2.p, A.g2c.p, part, g + L[6].w, n2x, n2y, n2z, s);
   colsum<16>(A.g2c.p, N2, part, g + L[6].b, s);
@@ -647,7 +712,7 @@ void UNet::backward(const uint16_t *X0, const float *dout, hipStream_t s) {
   colsum<32>(A.g2a.p, N2, part, g + L[5].b, s);
   conv<32, 32, 27, false>(A.g2a.p, W + L[5].pt, nullptr, A.g2b.p, n2x, n2y, n2z, s);   // dT2
   // q2: dQ2 = downsum(dT2)
-  hipLaunchKernelGGL(k_downsum, dim3(nblk(N3 * 32)), dim3(TPB), 0, s, A.g2b.p, A.g3b.p, N3, 32, n3x, n3y);
+  hipLaunchKernelGGL(k_downsum, dim3(nblk(N3 * 4)), dim3(TPB), 0, s, V(A.g2b.p), V(A.g3b.p), N3, 4, n3x, n3y);
   ew_launch_check();
   conv_dw<64, 32, 1>(A.a3.p, A.g3b.p, part, g + L[4].w, n3x, n3y, n3z, s);
   colsum<32>(A.g3b.p, N3, part, g + L[4].b, s);
@@ -659,8 +724,8 @@ void UNet::backward(const uint16_t *X0, const float *dout, hipStream_t s) {
   colsum<64>(A.g3a.p, N3, part, g + L[3].b, s);
   conv<64, 32, 27, false>(A.g3a.p, W + L[3].pt, nullptr, A.g3b.p, n3x, n3y, n3z, s);   // dP2
   // dS2 = dT2 + pool_bwd(dP2), through relu of e2
-  hipLaunchKernelGGL(k_pool_bwd_add, dim3(nblk(N2 * 32)), dim3(TPB), 0, s, A.g3b.p, A.g2b.p, A.g2a.p, N2, 32, n2x,
-                     n2y);
+  hipLaunchKernelGGL(k_pool_bwd_add, dim3(nblk(N2 * 4)), dim3(TPB), 0, s, V(A.g3b.p), V(A.g2b.p), V(A.g2a.p), N2, 4,
+                     n2x, n2y);
   hipLaunchKernelGGL(k_mask, dim3(nblk(N2 * 32)), dim3(TPB), 0, s, A.g2a.p, A.s2.p, N2 * 32);
   ew_launch_check();
   // e2
@@ -668,8 +733,8 @@ void UNet::backward(const uint16_t *X0, const float *dout, hipStream_t s) {
   colsum<32>(A.g2a.p, N2, part, g + L[2].b, s);
   conv<32, 16, 27, false>(A.g2a.p, W + L[2].pt, nullptr, A.g2c.p, n2x, n2y, n2z, s);   // dP1
   // dS1 = dT1 + pool_bwd(dP1), through relu of e1b
-  hipLaunchKernelGGL(k_pool_bwd_add, dim3(nblk(N1 * 16)), dim3(TPB), 0, s, A.g2c.p, A.g1b.p, A.g1c.p, N1, 16, n1x,
-                     n1y);
+  hipLaunchKernelGGL(k_pool_bwd_add, dim3(nblk(N1 * 2)), dim3(TPB), 0, s, V(A.g2c.p), V(A.g1b.p), V(A.g1c.p), N1, 2,
+                     n1x, n1y);
   hipLaunchKernelGGL(k_mask, dim3(nblk(N1 * 16)), dim3(TPB), 0, s, A.g1c.p, A.s1.p, N1 * 16);
   ew_launch_check();
   // e1b

@@ -9,7 +9,8 @@ into blocks of K steps.  A sample is the network input (phi at the block
 start, the bead density at its start and end) and the target (phi after K
 steps - the exponential-Euler base) / norm[4].  The
 network is trained on the GPU (phafd_unet_train); the tolerance of the
-block check is set from the residuals of the plain method's own blocks.
+block check is set above the residuals of the trained network's own
+predictions on the held-out trajectories.
 
     python -m phafd_amd.surrogate --out weights.npz [--trajectories 20]
 """
@@ -97,6 +98,25 @@ def build_dataset(P, net, mix="gaussian", n=64, ntraj=20, nblocks=100, K=10, log
     return np.array(meta), norm
 
 
+def calibrate(P, net, norm, mix, first_seed, ntraj, nblocks, K, n=64):
+    """Residuals of the network's own predictions on trajectories it was not
+    trained on (regenerated from their seeds) and the phi error per block;
+    the block check's tolerance is set above the largest residual, so that
+    it rejects what the network does not usually produce."""
+    names = MIXES[mix]
+    res, err = [], []
+    for t in range(first_seed, first_seed + ntraj):
+        s = make_system(P, n, FAMILIES[names[t % len(names)]], seed=100 + t)
+        for phi0, rho0, rho1, base, phi1, _ in blocks(P, s, nblocks, K):
+            out = net.forward(P.surrogate_inputs(phi0, rho0, rho1, norm)).astype(np.float64)
+            pred = base + norm[4] * (out - out.mean())
+            r, _ = s.block_residual(phi0, pred, rho0, rho1, K)
+            res.append(r)
+            err.append(float(np.linalg.norm(pred - phi1) / np.linalg.norm(phi1 - phi0)))
+        s.close()
+    return np.array(res), np.array(err)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
@@ -136,13 +156,17 @@ def main(argv=None):
     err_pred = np.sqrt(val * N) * norm[4]
     rel = err_pred / meta[ntrain:, 0] if nval else np.zeros(0)
     base_only = meta[ntrain:, 1] / meta[ntrain:, 0] if nval else np.zeros(0)
-    tol = 1.5 * float(meta[:, 2].max())
+    vres, verr = calibrate(P, net, norm, args.mix, args.trajectories - args.holdout, args.holdout, args.blocks,
+                           args.K)
+    tol = 1.25 * float(vres.max())
     summary = {"mix": args.mix, "norm": norm.tolist(), "tol": tol, "K": args.K, "train_samples": ntrain,
                "validation_samples": nval,
                "history": hist, "validation_rel_err_change_mean": float(rel.mean()) if nval else None,
                "validation_rel_err_change_max": float(rel.max()) if nval else None,
                "base_only_rel_err_change_mean": float(base_only.mean()) if nval else None,
-               "plain_residual_max": float(meta[:, 2].max()), "train_seconds": time.time() - t0}
+               "plain_residual_max": float(meta[:, 2].max()),
+               "network_residual_validation_max": float(vres.max()),
+               "network_residual_validation_mean": float(vres.mean()), "train_seconds": time.time() - t0}
     np.savez(args.out, params=net.get_params(), norm=norm, tol=np.array([tol]), K=np.array([args.K]))
     log(json.dumps(summary))
     net.close()
