@@ -179,14 +179,22 @@ def surrogate_leg(args, wl, phi, xyz, mol, ref_phi, fp64_elapsed):
     configure(s, wl, phi, xyz, mol)
     s.run(args.warmup)
     s.run_surrogate(net, 0, K, tol)   # allocations outside the timed region
+    s.probe("unet_conv")
     t0 = time.perf_counter()
     st = s.run_surrogate(net, steps, K, tol)
     el = time.perf_counter() - t0
+    pms, plaunch, pflops = s.probe(None)
     got = s.get_field(P.PHI)
     s.close()
+    conv_tflops = pflops / (pms * 1e-3) / 1e12 if plaunch else None
     out = {"value": n ** 3 * steps / el, "unit": "grid-pt·steps/s", "ms_per_step": 1e3 * el / steps,
            "speedup_vs_fp64": (fp64_elapsed / args.steps) / (el / steps), "K": K, "tol": tol,
            "blocks": int(st["blocks"]), "accepted": int(st["accepted"]), "fallbacks": int(st["fallbacks"]),
+           "roofline": {"bound": "mfma", "kernel": "k_conv<16,16,27> (3^3 conv 16->16, full resolution, BF16 MFMA)",
+                        "achieved": conv_tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                        "frac": conv_tflops / 2500.0 if conv_tflops else None,
+                        "avg_launch_ms": pms / max(plaunch, 1), "launches": plaunch,
+                        "algorithmic_flops_per_launch": pflops / max(plaunch, 1)},
            "max_residual": st["max_residual"], "ms_network_path_per_block": st["ms_network_path"] / max(st["blocks"], 1),
            "form": "per block of K steps: atoms K steps with the block-start grid-atom force held, bead densities "
                    "at both ends, exponential-Euler spectral base + BF16 U-Net correction, K-step residual check, "

@@ -2310,7 +2310,8 @@ void surrogate_steps(phafd_hip_ctx *c, UNet &net, int nsteps, int K, double tol,
     sum_fp64(N, c->sg_out.p, c->sg_sum.p + 2, c->stream);
     unet_apply(N, c->sg_base.p, c->sg_out.p, c->sg_sum.p + 2, net.norm[4], c->sg_phi.p, c->stream);
     double chk[2];
-    block_check(c, g, phi0, c->sg_phi.p, c->sg_rho1.p, K, chk);
+    block_check(c, g, phi0, c->sg_phi.p, c->sg_rho1.p, K, chk);   // ends with a synchronise
+    probe_collect();
     st[6] += ms_since(t0);
     st[0] += 1;
     st[3] = std::max(st[3], chk[0]);

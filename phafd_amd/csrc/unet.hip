@@ -50,11 +50,14 @@ size_t nblk(size_t n) {
 // ------------------------------------------------------------ convolution
 // Y[v][co] = act(bias[co] + sum_{tap, ci} W[co][ci][tap] X[v + off(tap)][ci])
 // W packed: [chunk][n-tile][lane][8] (B-operand order).  A block of 4 waves
-// computes a tile of 16 (x) by 4 * MTR (y) points at one z, MTR rows per
-// wave; its input halo (3 x (4 MTR + 2) x 18 points for a 3^3 kernel) is
-// staged in LDS once (periodic wrap applied there), so the K loop reads the
-// A operands from LDS with fixed offsets per tap.  Grid: (nx/16) *
-// (ny/(4 MTR)) * nz blocks.
+// computes tiles of 16 (x) by 4 * MTR (y) points, MTR rows per wave, for zb
+// consecutive z: its input planes (18 x (4 MTR + 2) halo points for a 3^3
+// kernel) sit in an LDS ring of three, one new plane loaded per z (periodic
+// wrap applied on load), so the K loop reads the A operands from LDS at
+// fixed offsets per tap.  A plane is stored group-major ([8-channel
+// group][y][x], 16 bytes per entry): the 16 lanes of a row read 256
+// contiguous bytes, free of bank conflicts.  Grid: (nx/16) * (ny/(4 MTR)) *
+// (nz/zb) blocks.
 template <int CIN, int TAPS, int MTR>
 struct ConvTile {
   static constexpr int CG = CIN / 8;
@@ -62,84 +65,143 @@ struct ConvTile {
   static constexpr int HX = TAPS == 27 ? 18 : 16;
   static constexpr int HY = TAPS == 27 ? TY + 2 : TY;
   static constexpr int HZ = TAPS == 27 ? 3 : 1;
-  static constexpr int ELEMS = HZ * HY * HX * CG;   // 16-byte entries
+  static constexpr int PLANE = HY * HX * CG;   // 16-byte entries per plane
 };
 
 template <int CIN, int COUT, int TAPS, bool RELU, int MTR>
 __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const u32x4 *__restrict__ W,
                                               const float *__restrict__ bias, uint16_t *__restrict__ Y, int nx,
-                                              int ny, int nz) {
+                                              int ny, int nz, int zb) {
   using T = ConvTile<CIN, TAPS, MTR>;
   constexpr int CG = T::CG;
   constexpr int G = TAPS * CG;
   constexpr int NC = (G + 3) / 4;
   constexpr int NT = COUT / 16;
-  __shared__ u32x4 tile[T::ELEMS];
+  constexpr int H = TAPS == 27 ? 1 : 0;   // halo width
+  constexpr int PER = (T::PLANE + 255) / 256;
+  __shared__ u32x4 tile[T::HZ * T::PLANE];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int xt = nx >> 4, yt = ny / T::TY;
   int b = blockIdx.x;
   const int bx = b % xt;
   b /= xt;
   const int by = b % yt;
-  const int z = b / yt;
+  const int z0 = (b / yt) * zb;
   const int x0 = bx * 16, y0 = by * T::TY;
-  constexpr int H = TAPS == 27 ? 1 : 0;   // halo width
-  for (int t = threadIdx.x; t < T::ELEMS; t += 256) {
-    const int cg = t % CG;
-    const int v = t / CG;
-    const int hx = v % T::HX, hy = (v / T::HX) % T::HY, hz = v / (T::HX * T::HY);
-    const int xx = wrap1(x0 - H + hx, nx), yy = wrap1(y0 - H + hy, ny), zz = wrap1(z - H + hz, nz);
-    tile[t] = X[(((size_t)zz * ny + yy) * nx + xx) * CG + cg];
-  }
-  __syncthreads();
   const int kg = lane >> 4;
   const int lx = lane & 15;
-  f32x4 acc[MTR][NT];
+  // this thread's share of a plane load: offset within a z plane of X (group
+  // fastest, coalesced) and the group-major LDS slot; the same every plane
+  int goff[PER], loff[PER];
 #pragma unroll
-  for (int r = 0; r < MTR; r++)
-#pragma unroll
-    for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-  for (int c = 0; c < NC; c++) {
-    const int g = 4 * c + kg;
-    bf16x8 a[MTR];
-    if (g < G) {
-      const int tap = g / CG, cg = g % CG;
-      int dx = 0, dy = 0, dz = 0;
-      if (TAPS == 27) {
-        dz = tap / 9 - 1;
-        dy = (tap / 3) % 3 - 1;
-        dx = tap % 3 - 1;
-      }
-      const int base = (((dz + H) * T::HY + (wave * MTR + dy + H)) * T::HX + (lx + dx + H)) * CG + cg;
-#pragma unroll
-      for (int r = 0; r < MTR; r++) a[r] = __builtin_bit_cast(bf16x8, tile[base + r * T::HX * CG]);
-    } else {
-#pragma unroll
-      for (int r = 0; r < MTR; r++) a[r] = __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u});
-    }
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-      const bf16x8 w = __builtin_bit_cast(bf16x8, W[((size_t)c * NT + t) * 64 + lane]);
-#pragma unroll
-      for (int r = 0; r < MTR; r++) acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], w, acc[r][t], 0, 0, 0);
-    }
+  for (int k = 0; k < PER; k++) {
+    const int t = threadIdx.x + 256 * k;
+    const int cg = t % CG, v = t / CG;
+    const int hx = v % T::HX, hy = v / T::HX;
+    const int xx = wrap1(x0 - H + hx, nx), yy = wrap1(y0 - H + hy, ny);
+    goff[k] = t < T::PLANE ? (yy * nx + xx) * CG + cg : -1;
+    loff[k] = (cg * T::HY + hy) * T::HX + hx;
   }
-  // D layout: lane holds rows 4*(lane/16) + j (points), column lane % 16 (channel)
+  const size_t zstride = (size_t)ny * nx * CG;
+  auto load_plane = [&](u32x4 *dst, int z) {
+    const u32x4 *Xz = X + (size_t)wrap1(z, nz) * zstride;
 #pragma unroll
-  for (int r = 0; r < MTR; r++) {
-    const size_t row = ((size_t)z * ny + (y0 + wave * MTR + r)) * nx + x0;
+    for (int k = 0; k < PER; k++)
+      if (goff[k] >= 0) dst[loff[k]] = Xz[goff[k]];
+  };
+  // this lane's A (points) offset for K chunk c within a ring plane and the
+  // plane it reads (dz + 1), packed as (dz + 1) << 20 | offset; -1 past the
+  // last group
+  auto chunk_code = [&](int c) -> int {
+    const int g = 4 * c + kg;
+    const int tap = g / CG, cg = g % CG;
+    int dx = 0, dy = 0, dzi = 0;
+    if (TAPS == 27) {
+      dzi = tap / 9;
+      dy = (tap / 3) % 3 - 1;
+      dx = tap % 3 - 1;
+    }
+    const int off = (cg * T::HY + wave * MTR + dy + H) * T::HX + (lx + dx + H);
+    return g < G ? (dzi << 20) | off : -1;
+  };
+  constexpr bool PRE = NC <= 16;   // codes kept in registers (fully unrolled K loop)
+  int code[PRE ? NC : 1];
+  if (PRE) {
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-      const int co = t * 16 + lx;
-      const float bb = bias ? bias[co] : 0.f;
+    for (int c = 0; c < (PRE ? NC : 1); c++) code[c] = chunk_code(c);
+  }
+  // the output channels of this lane: 4 * kg + j of each n-tile
+  float bb[NT][4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        float v = acc[r][t][j] + bb;
-        if (RELU) v = fmaxf(v, 0.f);
-        Y[(row + 4 * kg + j) * COUT + co] = f2bf(v);
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) bb[t][j] = bias ? bias[t * 16 + 4 * kg + j] : 0.f;
+  if (TAPS == 27) {
+    load_plane(tile, z0 - 1);
+    load_plane(tile + T::PLANE, z0);
+  }
+  for (int zi = 0; zi < zb; zi++) {
+    const int z = z0 + zi;
+    // plane z + d lives in ring slot (zi + 1 + d) % 3; a 1x1 kernel reads its
+    // one plane z from slot 0
+    if (TAPS == 27) load_plane(tile + ((zi + 2) % 3) * T::PLANE, z + 1);
+    else load_plane(tile, z);
+    __syncthreads();
+    const int sb0 = TAPS == 27 ? (zi % 3) * T::PLANE : 0;
+    const int sb1 = TAPS == 27 ? ((zi + 1) % 3) * T::PLANE : 0;
+    const int sb2 = TAPS == 27 ? ((zi + 2) % 3) * T::PLANE : 0;
+    f32x4 acc[MTR][NT];
+#pragma unroll
+    for (int r = 0; r < MTR; r++)
+#pragma unroll
+      for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    auto chunk = [&](int c, int cd) {
+      bf16x8 a[MTR];
+      if (cd >= 0) {
+        const int pz = cd >> 20;
+        const int base = (pz == 0 ? sb0 : (pz == 1 ? sb1 : sb2)) + (cd & 0xfffff);
+#pragma unroll
+        for (int r = 0; r < MTR; r++) a[r] = __builtin_bit_cast(bf16x8, tile[base + r * T::HX]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < MTR; r++) a[r] = __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u});
+      }
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const bf16x8 w = __builtin_bit_cast(bf16x8, W[((size_t)c * NT + t) * 64 + lane]);
+        // weights as the A operand (rows = output channels), points as B
+#pragma unroll
+        for (int r = 0; r < MTR; r++)
+          acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, a[r], acc[r][t], 0, 0, 0);
+      }
+    };
+    if (PRE) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) chunk(c, code[PRE ? c : 0]);
+    } else {
+#pragma unroll 2
+      for (int c = 0; c < NC; c++) chunk(c, chunk_code(c));
+    }
+    // D layout: lane holds rows 4 * kg + j (output channels of the n-tile),
+    // column lx (the point x0 + lx): 4 consecutive channels, one 8-byte store
+#pragma unroll
+    for (int r = 0; r < MTR; r++) {
+      const size_t pt = ((size_t)z * ny + (y0 + wave * MTR + r)) * nx + x0 + lx;
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        float v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          v[j] = acc[r][t][j] + bb[t][j];
+          if (RELU) v[j] = fmaxf(v[j], 0.f);
+        }
+        uint2 o;
+        o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+        o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+        *reinterpret_cast<uint2 *>(Y + pt * COUT + t * 16 + 4 * kg) = o;
       }
     }
+    __syncthreads();   // the ring slot read here is reloaded next
   }
 }
 
@@ -521,19 +583,30 @@ __global__ void k_apply(size_t N, const double *__restrict__ phi, const float *_
 // ------------------------------------------------------------ launchers
 // rows per wave: 4 when the rows of a level allow it and the halo tile fits
 // comfortably in LDS (64 input channels: 2)
+// z slices per block: the largest of 8, 4, 2, 1 dividing nz that leaves at
+// least 2048 blocks (8 per CU)
+int conv_zb(int tiles_xy, int nz) {
+  for (int zb = 8; zb > 1; zb >>= 1)
+    if (nz % zb == 0 && (size_t)tiles_xy * (nz / zb) >= 2048) return zb;
+  return 1;
+}
+
+// rows per wave: 4 when the rows of a level allow it and the halo planes fit
+// comfortably in LDS (64 input channels: 2)
 template <int CIN, int COUT, int TAPS, bool RELU>
 void conv(const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y, int nx, int ny, int nz,
           hipStream_t s) {
   const bool four = CIN < 64 && ny % 16 == 0;
-  if (four) {
-    const unsigned blocks = (unsigned)((nx / 16) * (ny / 16) * nz);
+  const int ty = four ? 16 : 8;
+  const int tiles = (nx / 16) * (ny / ty);
+  const int zb = conv_zb(tiles, nz);
+  const unsigned blocks = (unsigned)(tiles * (nz / zb));
+  if (four)
     hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 4>), dim3(blocks), dim3(256), 0, s,
-                       reinterpret_cast<const u32x4 *>(X), reinterpret_cast<const u32x4 *>(W), bias, Y, nx, ny, nz);
-  } else {
-    const unsigned blocks = (unsigned)((nx / 16) * (ny / 8) * nz);
+                       reinterpret_cast<const u32x4 *>(X), reinterpret_cast<const u32x4 *>(W), bias, Y, nx, ny, nz, zb);
+  else
     hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 2>), dim3(blocks), dim3(256), 0, s,
-                       reinterpret_cast<const u32x4 *>(X), reinterpret_cast<const u32x4 *>(W), bias, Y, nx, ny, nz);
-  }
+                       reinterpret_cast<const u32x4 *>(X), reinterpret_cast<const u32x4 *>(W), bias, Y, nx, ny, nz, zb);
   HIPCHK(hipGetLastError());
 }
 
@@ -663,7 +736,10 @@ void UNet::forward(const uint16_t *X0, float *out, hipStream_t s) {
   const uint16_t *W = Wp.p;
   const float *p = P.p;
   conv<8, 16, 27, true>(X0, W + L[0].pf, p + L[0].b, A.ae1.p, n1x, n1y, n1z, s);
+  const double conv_flops = 2.0 * 27 * 16 * 16 * (double)N1;   // 3^3 x 16 -> 16 at full resolution
+  probe_begin(K_UNET_CONV, s);
   conv<16, 16, 27, true>(A.ae1.p, W + L[1].pf, p + L[1].b, A.s1.p, n1x, n1y, n1z, s);
+  probe_end(K_UNET_CONV, s, conv_flops);
   hipLaunchKernelGGL(k_pool, dim3(nblk(N2 * 2)), dim3(TPB), 0, s, V(A.s1.p), V(A.p1.p), N2, 2, n2x, n2y);
   conv<16, 32, 27, true>(A.p1.p, W + L[2].pf, p + L[2].b, A.s2.p, n2x, n2y, n2z, s);
   hipLaunchKernelGGL(k_pool, dim3(nblk(N3 * 4)), dim3(TPB), 0, s, V(A.s2.p), V(A.p2.p), N3, 4, n3x, n3y);
@@ -673,7 +749,9 @@ void UNet::forward(const uint16_t *X0, float *out, hipStream_t s) {
   conv<32, 32, 27, true>(A.t2.p, W + L[5].pf, p + L[5].b, A.a2.p, n2x, n2y, n2z, s);
   conv<32, 16, 1, false>(A.a2.p, W + L[6].pf, p + L[6].b, A.q1.p, n2x, n2y, n2z, s);
   hipLaunchKernelGGL(k_upadd, dim3(nblk(N1 * 2)), dim3(TPB), 0, s, V(A.q1.p), V(A.s1.p), V(A.t1.p), N1, 2, n1x, n1y);
+  probe_begin(K_UNET_CONV, s);
   conv<16, 16, 27, true>(A.t1.p, W + L[7].pf, p + L[7].b, A.a1.p, n1x, n1y, n1z, s);
+  probe_end(K_UNET_CONV, s, conv_flops);
   hipLaunchKernelGGL(k_head, dim3(nblk(N1)), dim3(TPB), 0, s, reinterpret_cast<const u32x4 *>(A.a1.p),
                      p + L[8].w, p + L[8].b, out, N1);
   ew_launch_check();

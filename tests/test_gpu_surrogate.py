@@ -72,6 +72,20 @@ def test_forward_matches_torch_restatement():
     net.close()
 
 
+def test_forward_on_a_tiled_grid_is_the_tiled_forward():
+    """On a 256^3 grid holding a 64^3 input tiled 4 x 4 x 4, the output is
+    the 64^3 output tiled, bit for bit: every point sees the same inputs
+    through the same arithmetic, while the large grid takes the launch
+    shapes of large grids (several z slices per block at every level)."""
+    net = P.UNet(seed=3)
+    net.set_params(_random_params(net, 13))
+    x = np.random.default_rng(14).standard_normal((4, 64, 64, 64)).astype(np.float32)
+    small = net.forward(x)
+    big = net.forward(np.tile(x, (1, 4, 4, 4)))
+    assert np.array_equal(big, np.tile(small, (4, 4, 4)))
+    net.close()
+
+
 def test_parameter_gradients_match_torch_autograd():
     net = P.UNet(seed=3)
     p = _random_params(net, 4)
