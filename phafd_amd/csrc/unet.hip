@@ -591,12 +591,13 @@ int conv_zb(int tiles_xy, int nz) {
   return 1;
 }
 
-// rows per wave: 4 when the rows of a level allow it and the halo planes fit
-// comfortably in LDS (64 input channels: 2)
+// rows per wave: 4 when the rows of a level allow it, the halo planes fit
+// comfortably in LDS and the accumulators in registers (64 input or output
+// channels: 2)
 template <int CIN, int COUT, int TAPS, bool RELU>
 void conv(const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y, int nx, int ny, int nz,
           hipStream_t s) {
-  const bool four = CIN < 64 && ny % 16 == 0;
+  const bool four = CIN < 64 && COUT < 64 && ny % 16 == 0;
   const int ty = four ? 16 : 8;
   const int tiles = (nx / 16) * (ny / ty);
   const int zb = conv_zb(tiles, nz);
