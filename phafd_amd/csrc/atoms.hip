@@ -797,35 +797,40 @@ __device__ __forceinline__ void angle_t(const double4 &ua, const double4 &ub, co
     gb[c] = d1[c] * inv12 - cs * d2[c] / r2s;
   }
 }
-__global__ void k_chain_force(int n, const int *members, const int *cstart, const int *cend,
-                              const double4 *ux, ChainParams cp, double4 *fc) {
-  GRID_STRIDE(q, n) {
-    const int j = members[q];
-    const int s = cstart[q], e = cend[q];
-    const int b = j - s, nb = e - s;
-    const double4 *u = ux + s;
-    double acc[3] = {0, 0, 0}, t[3], ga[3], gb[3];
-    if (b >= 1) { bond_f(u[b - 1], u[b], cp.kbond, cp.r0, t); for (int c = 0; c < 3; c++) acc[c] += t[c]; }
-    if (b + 1 < nb) { bond_f(u[b + 1], u[b], cp.kbond, cp.r0, t); for (int c = 0; c < 3; c++) acc[c] += t[c]; }
-    if (b >= 2) { angle_t(u[b - 2], u[b - 1], u[b], ga, gb); for (int c = 0; c < 3; c++) acc[c] += cp.kbend * gb[c]; }
-    if (b >= 1 && b + 1 < nb) { angle_t(u[b - 1], u[b], u[b + 1], ga, gb); for (int c = 0; c < 3; c++) acc[c] += cp.kbend * (ga[c] - gb[c]); }
-    if (b + 2 < nb) { angle_t(u[b], u[b + 1], u[b + 2], ga, gb); for (int c = 0; c < 3; c++) acc[c] += -cp.kbend * ga[c]; }
-    fc[q] = make_double4(acc[0], acc[1], acc[2], 0.0);
-  }
+// bonded force on bead b of a chain of nb beads (unwrapped positions u)
+__device__ __forceinline__ void chain_bead_force(const double4 *u, int b, int nb, const ChainParams &cp,
+                                                 double acc[3]) {
+  double t[3], ga[3], gb[3];
+  acc[0] = acc[1] = acc[2] = 0;
+  if (b >= 1) { bond_f(u[b - 1], u[b], cp.kbond, cp.r0, t); for (int c = 0; c < 3; c++) acc[c] += t[c]; }
+  if (b + 1 < nb) { bond_f(u[b + 1], u[b], cp.kbond, cp.r0, t); for (int c = 0; c < 3; c++) acc[c] += t[c]; }
+  if (b >= 2) { angle_t(u[b - 2], u[b - 1], u[b], ga, gb); for (int c = 0; c < 3; c++) acc[c] += cp.kbend * gb[c]; }
+  if (b >= 1 && b + 1 < nb) { angle_t(u[b - 1], u[b], u[b + 1], ga, gb); for (int c = 0; c < 3; c++) acc[c] += cp.kbend * (ga[c] - gb[c]); }
+  if (b + 2 < nb) { angle_t(u[b], u[b + 1], u[b + 2], ga, gb); for (int c = 0; c < 3; c++) acc[c] += -cp.kbend * ga[c]; }
 }
-__global__ void k_chain_move(int n, const int *members, const double4 *fc, const double4 *F,
-                             double4 *ux, double4 *x, const int *image, ChainParams cp, double dt,
-                             Box bx) {
-  GRID_STRIDE(q, n) {
-    const int j = members[q];
-    const double4 f = F[j], c = fc[q];
-    double4 u = ux[j];
-    const double tx = f.x + c.x, ty = f.y + c.y, tz = f.z + c.z;
+
+// One half step of the chains, one chain per block: the chain's unwrapped
+// positions staged in LDS (all beads read before any moves), every bead's
+// bonded force from them, then x += drag (F + Fchain) dt / 2 on the
+// unwrapped position and Domain::map (src/domain.cpp:181-193) back to x.
+// Chains are contiguous atom ranges [chs[c], che[c]).
+__global__ void k_chain_step(const int *chs, const int *che, const double4 *F, double4 *ux, double4 *x,
+                             const int *image, ChainParams cp, double dt, Box bx) {
+  extern __shared__ double4 us[];
+  const int s = chs[blockIdx.x], nb = che[blockIdx.x] - s;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) us[b] = ux[s + b];
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    double acc[3];
+    chain_bead_force(us, b, nb, cp, acc);
+    const int j = s + b;
+    const double4 f = F[j];
+    double4 u = us[b];
+    const double tx = f.x + acc[0], ty = f.y + acc[1], tz = f.z + acc[2];
     u.x += cp.drag * tx * dt * 0.5;
     u.y += cp.drag * ty * dt * 0.5;
     u.z += cp.drag * tz * dt * 0.5;
     ux[j] = u;
-    // Domain::map, src/domain.cpp:181-193
     const int img = image[j];
     const int xb = (img & IMGMASK) - IMGMAX, yb = (img >> IMGBITS & IMGMASK) - IMGMAX,
               zb = (img >> IMG2BITS) - IMGMAX;
@@ -1104,17 +1109,12 @@ void launch_drag(int n, const int *members, double4 *x, const double4 *F, double
   hipLaunchKernelGGL(k_drag, dim3(nblk(n)), dim3(TPB), 0, s, n, members, x, F, drag, dt);
   HIPCHK(hipGetLastError());
 }
-void launch_chain_force(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,
-                        const ChainParams &cp, double4 *fc, hipStream_t s) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_chain_force, dim3(nblk(n)), dim3(TPB), 0, s, n, members, cstart, cend, ux, cp, fc);
-  HIPCHK(hipGetLastError());
-}
-void launch_chain_move(int n, const int *members, const double4 *fc, const double4 *F, double4 *ux,
-                       double4 *x, const int *image, const ChainParams &cp, double dt, const Box &b,
-                       hipStream_t s) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_chain_move, dim3(nblk(n)), dim3(TPB), 0, s, n, members, fc, F, ux, x, image, cp, dt, b);
+void launch_chain_step(int nchains, int maxlen, const int *chs, const int *che, const double4 *F, double4 *ux,
+                       double4 *x, const int *image, const ChainParams &cp, double dt, const Box &b, hipStream_t s) {
+  if (nchains <= 0 || maxlen <= 0) return;
+  const int threads = maxlen <= 64 ? 64 : (maxlen <= 128 ? 128 : 256);
+  hipLaunchKernelGGL(k_chain_step, dim3(nchains), dim3(threads), sizeof(double4) * maxlen, s, chs, che, F, ux, x,
+                     image, cp, dt, b);
   HIPCHK(hipGetLastError());
 }
 void launch_chain_energy(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,

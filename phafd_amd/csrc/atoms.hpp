@@ -108,11 +108,10 @@ void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *r
                      double4 *gF, double *energy, hipStream_t s);
 void launch_drag(int n, const int *members, double4 *x, const double4 *F, double drag, double dt,
                  hipStream_t s);
-void launch_chain_force(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,
-                        const ChainParams &cp, double4 *fc, hipStream_t s);
-void launch_chain_move(int n, const int *members, const double4 *fc, const double4 *F, double4 *ux,
-                       double4 *x, const int *image, const ChainParams &cp, double dt, const Box &b,
-                       hipStream_t s);
+// one half step of the chains (contiguous atom ranges [chs[c], che[c]),
+// at most maxlen beads, maxlen * 32 bytes of LDS per block)
+void launch_chain_step(int nchains, int maxlen, const int *chs, const int *che, const double4 *F, double4 *ux,
+                       double4 *x, const int *image, const ChainParams &cp, double dt, const Box &b, hipStream_t s);
 void launch_chain_energy(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,
                          const ChainParams &cp, double *energy, hipStream_t s);
 void launch_fill_i32(int n, int *p, int v, hipStream_t s);

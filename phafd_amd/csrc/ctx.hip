@@ -38,9 +38,9 @@ struct FixDef {
   int seed = 0, noise_mode = 0, once = 0;
   std::vector<int> start, end;
   double drag = 0, r0 = 1, kbond = 0, kbend = 0;
-  int nmem = 0;
+  int nmem = 0, nchain = 0, maxlen = 0;
   DBuf<int> members, cstart, cend;
-  DBuf<double4> fc;
+  DBuf<int> chs, che;   // chains (group chunks) as atom ranges [chs, che)
 };
 
 struct SwapDev {
@@ -758,8 +758,8 @@ void integrate_half(phafd_hip_ctx *c, FixDef &f) {
     launch_drag(f.nmem, f.members.p, c->x.p, c->F.p, f.drag, c->dt, c->stream);
   } else if (f.kind == FixDef::CHAIN) {
     ChainParams cp{f.r0, f.kbond, f.kbend, f.drag};
-    launch_chain_force(f.nmem, f.members.p, f.cstart.p, f.cend.p, c->ux.p, cp, f.fc.p, c->stream);
-    launch_chain_move(f.nmem, f.members.p, f.fc.p, c->F.p, c->ux.p, c->x.p, c->image.p, cp, c->dt, c->box(), c->stream);
+    launch_chain_step(f.nchain, f.maxlen, f.chs.p, f.che.p, c->F.p, c->ux.p, c->x.p, c->image.p, cp, c->dt,
+                      c->box(), c->stream);
   }
 }
 
@@ -948,25 +948,39 @@ void run_steps(phafd_hip_ctx *c, int nsteps) {
 
 // flatten group chunks (src/group.cpp:31-207) into member lists
 void build_members(phafd_hip_ctx *c, FixDef &f) {
-  std::vector<int> mem, cs, ce;
+  std::vector<int> mem, cs, ce, hs, he;
+  std::vector<char> seen(c->nowned, 0);
+  f.maxlen = 0;
   for (size_t k = 0; k < f.start.size(); k++) {
     if (f.start[k] < 0 || f.end[k] > c->nowned || f.end[k] < f.start[k])
       throw std::invalid_argument("fix group chunk out of range");
+    if (f.end[k] == f.start[k]) continue;
     for (int j = f.start[k]; j < f.end[k]; j++) {
+      if (seen[j] && f.kind == FixDef::CHAIN) throw std::invalid_argument("chain chunks overlap");
+      seen[j] = 1;
       mem.push_back(j);
       cs.push_back(f.start[k]);
       ce.push_back(f.end[k]);
     }
+    hs.push_back(f.start[k]);
+    he.push_back(f.end[k]);
+    f.maxlen = std::max(f.maxlen, f.end[k] - f.start[k]);
   }
+  if (f.kind == FixDef::CHAIN && f.maxlen > 2048)
+    throw std::invalid_argument("chains longer than 2048 beads are not supported");
   f.nmem = (int)mem.size();
+  f.nchain = (int)hs.size();
   f.members.alloc(mem.size() + 1);
   f.cstart.alloc(mem.size() + 1);
   f.cend.alloc(mem.size() + 1);
-  f.fc.alloc(mem.size() + 1);
+  f.chs.alloc(hs.size() + 1);
+  f.che.alloc(he.size() + 1);
   if (!mem.empty()) {
     HIPCHK(hipMemcpy(f.members.p, mem.data(), sizeof(int) * mem.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(f.cstart.p, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(f.cend.p, ce.data(), sizeof(int) * ce.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(f.chs.p, hs.data(), sizeof(int) * hs.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(f.che.p, he.data(), sizeof(int) * he.size(), hipMemcpyHostToDevice));
   }
 }
 

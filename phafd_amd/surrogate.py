@@ -12,7 +12,7 @@ network is trained on the GPU (phafd_unet_train); the tolerance of the
 block check is set above the residuals of the trained network's own
 predictions on the held-out trajectories.
 
-    python -m phafd_amd.surrogate --out weights.npz [--trajectories 20]
+    python -m phafd_amd.surrogate --out weights.npz [--K 40 --trajectories 60 --blocks 25]
 """
 import argparse
 import json
@@ -120,13 +120,14 @@ def calibrate(P, net, norm, mix, first_seed, ntraj, nblocks, K, n=64):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
-    ap.add_argument("--trajectories", type=int, default=20)
-    ap.add_argument("--blocks", type=int, default=100)
-    ap.add_argument("--K", type=int, default=10)
-    ap.add_argument("--steps", type=int, default=3000)
+    # defaults: the committed network (phafd_amd/data/surrogate_gaussian64.npz)
+    ap.add_argument("--trajectories", type=int, default=60)
+    ap.add_argument("--blocks", type=int, default=25)
+    ap.add_argument("--K", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=9000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--holdout", type=int, default=2, help="trajectories kept for validation")
+    ap.add_argument("--holdout", type=int, default=4, help="trajectories kept for validation")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mix", default="gaussian", choices=sorted(MIXES))
     args = ap.parse_args(argv)
@@ -152,7 +153,7 @@ def main(argv=None):
             f"{time.time() - t0:.1f} s")
     # relative error of the predicted change on the validation blocks
     val = net.eval(ntrain, nval) if nval else np.zeros(0)
-    N = 64 ** 3
+    N = 64 ** 3   # the training grid
     err_pred = np.sqrt(val * N) * norm[4]
     rel = err_pred / meta[ntrain:, 0] if nval else np.zeros(0)
     base_only = meta[ntrain:, 1] / meta[ntrain:, 0] if nval else np.zeros(0)
