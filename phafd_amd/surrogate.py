@@ -12,7 +12,7 @@ network is trained on the GPU (phafd_unet_train); the tolerance of the
 block check is set above the residuals of the trained network's own
 predictions on the held-out trajectories.
 
-    python -m phafd_amd.surrogate --out weights.npz [--K 40 --trajectories 60 --blocks 25]
+    python -m phafd_amd.surrogate --out weights.npz [--K 50 --trajectories 60 --blocks 20]
 """
 import argparse
 import json
@@ -122,8 +122,8 @@ def main(argv=None):
     ap.add_argument("--out", required=True)
     # defaults: the committed network (phafd_amd/data/surrogate_gaussian64.npz)
     ap.add_argument("--trajectories", type=int, default=60)
-    ap.add_argument("--blocks", type=int, default=25)
-    ap.add_argument("--K", type=int, default=40)
+    ap.add_argument("--blocks", type=int, default=20)
+    ap.add_argument("--K", type=int, default=50)
     ap.add_argument("--steps", type=int, default=9000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)

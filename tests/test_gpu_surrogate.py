@@ -200,7 +200,7 @@ def _np_block(phi0, phi1, rho0, rho1, K, w, phit=0.1, chi=3.0, tv=1.0):
     res = np.sqrt(np.sum(wt * np.abs(Rk) ** 2) / np.sum(wt * np.abs(p1 - p0) ** 2))
     drift = (phi1.sum() - phi0.sum()) / phi0.sum()
     S = sum(G ** k for k in range(K))
-    base = np.fft.irfftn(G ** K * p0 - c * S * (m0 - f2 * p0), s=phi0.shape)
+    base = np.fft.irfftn(G ** K * p0 - c * S * (m0 - f2 * p0), s=phi0.shape, axes=(0, 1, 2))
     return res, drift, base
 
 
