@@ -178,6 +178,22 @@ __global__ void k_reverse(int cnt, const int *send, int first, double4 *target, 
     target[j] = t;
   }
 }
+// the two image swaps of one dimension when no atom is in both send lists:
+// items [0, ca) fold swap a, [ca, ca + cb) swap b; every target is written
+// by one item, so the sums are those of two sequential launches
+__global__ void k_reverse2(int ca, const int *sa, int fa, int cb, const int *sb, int fb, double4 *gF) {
+  GRID_STRIDE(q, ca + cb) {
+    const bool ina = q < ca;
+    const int r = ina ? q : q - ca;
+    const int j = ina ? sa[r] : sb[r];
+    const double4 b = gF[(ina ? fa : fb) + r];
+    double4 t = gF[j];
+    t.x += b.x;
+    t.y += b.y;
+    t.z += b.z;
+    gF[j] = t;
+  }
+}
 
 // ---------------------------------------------------------------- bins
 // NBin::coord2bin, src/nbin.cpp:61-94
@@ -923,6 +939,11 @@ void launch_forward(int ng, const double4 *x, GatheredView g, hipStream_t s) {
 void launch_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF, hipStream_t s) {
   if (cnt <= 0) return;
   hipLaunchKernelGGL(k_reverse, dim3(nblk(cnt)), dim3(TPB), 0, s, cnt, send, first, target, gF);
+  HIPCHK(hipGetLastError());
+}
+void launch_reverse2(int ca, const int *sa, int fa, int cb, const int *sb, int fb, double4 *gF, hipStream_t s) {
+  if (ca + cb <= 0) return;
+  hipLaunchKernelGGL(k_reverse2, dim3(nblk(ca + cb)), dim3(TPB), 0, s, ca, sa, fa, cb, sb, fb, gF);
   HIPCHK(hipGetLastError());
 }
 void launch_atom2bin(int ng, const double4 *gx, const BinGeom &b, int *a2b, int *bincount, int *bad,

@@ -49,6 +49,10 @@ struct SwapDev {
   int first = 0;   // relative gathered index of the first received entry
   int count = 0;
   DBuf<int> send;  // sources: owned index (copy swap) or relative gathered index
+  // second image swap of a dimension: no atom can be sent by both swaps of
+  // the dimension (lo of this swap above hi of the first), so their reverse
+  // folds touch different targets and may share one launch
+  bool disjoint_pair = false;
 };
 
 }  // namespace
@@ -137,6 +141,7 @@ struct phafd_hip_ctx {
   DBuf<int> t_cnt, t_off, t_cur, t_pair, t_i;
   bool lists_valid = false;
   int pending_decision = -1;   // rebuild decision prefetched at the end of the last step
+  bool store_muhat = true;     // the fused z update writes muhat (false inside a run but its last step)
   // styles
   std::vector<std::unique_ptr<PairDef>> pairs;
   std::vector<std::unique_ptr<FixDef>> fixes;
@@ -233,7 +238,12 @@ void comm_setup(phafd_hip_ctx *c) {
       SwapDev &w = c->swaps.back();
       w.dim = dim;
       if (ineed == 0) { w.lo = -BIG; w.hi = c->boxlo[dim] + c->cutghost; w.shift = c->period[dim]; }
-      else { w.lo = c->boxhi[dim] - c->cutghost; w.hi = BIG; w.shift = -c->period[dim]; }
+      else {
+        w.lo = c->boxhi[dim] - c->cutghost;
+        w.hi = BIG;
+        w.shift = -c->period[dim];
+        w.disjoint_pair = w.lo > c->swaps[c->swaps.size() - 2].hi;
+      }
     }
 }
 
@@ -318,6 +328,12 @@ void forward_comm(phafd_hip_ctx *c) { launch_forward(c->ngathered, c->x.p, c->gv
 void reverse_comm(phafd_hip_ctx *c) {
   for (int iswap = (int)c->swaps.size() - 1; iswap >= 1; iswap--) {
     SwapDev &w = c->swaps[iswap];
+    if (w.disjoint_pair) {   // the dimension's two swaps in one launch (same sums)
+      SwapDev &v = c->swaps[iswap - 1];
+      launch_reverse2(w.count, w.send.p, w.first, v.count, v.send.p, v.first, c->gF.p, c->stream);
+      iswap--;
+      continue;
+    }
     launch_reverse(w.count, w.send.p, w.first, c->gF.p, c->gF.p, c->stream);
   }
   SwapDev &z = c->swaps[0];
@@ -818,7 +834,8 @@ void phase_final_split(phafd_hip_ctx *c) {
     FixDef &f = *fp;
     if (f.kind == FixDef::CONJ) {
       const auto *nz = (f.noise_mode == PHAFD_NOISE_FIELD && c->noise_set) ? F.noise_k.p : nullptr;
-      fft_z_update(c->plan, F.muhat.p, F.phihat.p, nz, update_params(c, f), origin_ptr(c), c->stream);
+      fft_z_update(c->plan, F.muhat.p, F.phihat.p, nz, update_params(c, f), origin_ptr(c), c->stream, Slab(),
+                   c->store_muhat);
     } else {
       integrate_half(c, f);
     }
@@ -891,6 +908,8 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
   const bool zsplit = nconj == 1 && spectral_fused(c);
   for (int it = 0; it < nsteps; it++) {
     c->timestep++;
+    // muhat is observable only after the run: its last step stores it
+    c->store_muhat = it == nsteps - 1;
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     phase_start(c);
     HIPCHK(hipEventRecord(c->ev[1], c->stream));

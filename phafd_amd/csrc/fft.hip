@@ -146,8 +146,8 @@ __host__ __device__ __forceinline__ C mk2(R x, R y) { return make_float2(x, y); 
     NS::fft_grad_sep_p(p, phi, ph, gy_k, gz_k, gx, gy, gz, sc, s, sl);                                       \
   }                                                                                                          \
   void fft_z_update(const FFTPlan &p, CT *mu, CT *ph, const CT *noise, UpdateParams u, const double *origin, \
-                    hipStream_t s, Slab sl) {                                                                \
-    NS::fft_z_update_p(p, mu, ph, noise, u, origin, s, sl);                                                  \
+                    hipStream_t s, Slab sl, bool store_mu) {                                                 \
+    NS::fft_z_update_p(p, mu, ph, noise, u, origin, s, sl, store_mu);                                        \
   }                                                                                                          \
   void fft_c2r_yx_batch(const FFTPlan &p, CT *const *in, RT *const *out, int batch, double divisor,         \
                         hipStream_t s) {                                                                     \

@@ -133,7 +133,7 @@ struct UpdateParams;
   void fft_grad_sep(const FFTPlan &p, const RT *phi, CT *ph, CT *gy_k, CT *gz_k, RT *gx, RT *gy, RT *gz, \
                     SpecConst sc, hipStream_t s, Slab sl = Slab());                                      \
   void fft_z_update(const FFTPlan &p, CT *mu, CT *ph, const CT *noise, UpdateParams u,                   \
-                    const double *origin, hipStream_t s, Slab sl = Slab());                              \
+                    const double *origin, hipStream_t s, Slab sl = Slab(), bool store_mu = true);        \
   void fft_c2r_yx_batch(const FFTPlan &p, CT *const *in, RT *const *out, int batch, double divisor,      \
                         hipStream_t s);                                                                  \
   void spec_k_to_canon(const FFTPlan &p, const CT *k, CT *canon, hipStream_t s);                         \
@@ -147,7 +147,9 @@ struct UpdateParams;
 // grad multiply, three c2r / N.  fft_z_update: z pass of mu's r2c, the
 // spectral update of phihat and the z pass of its c2r in one kernel (grids
 // with fft_fused_z_ok); finish with fft_c2r_yx_batch.  origin != nullptr
-// (FP32 form): the k = 0 mode is taken from the FP64 sum there.
+// (FP32 form): the k = 0 mode is taken from the FP64 sum there.  store_mu
+// false: muhat of the planes 1 <= kx < Nx/2 is left as the y pass wrote it
+// (for steps whose muhat nobody reads; the update itself is unchanged).
 PHAFD_FFT_DECL(double, double2)
 PHAFD_FFT_DECL(float, float2)
 #undef PHAFD_FFT_DECL
