@@ -79,7 +79,9 @@ __global__ void k_unmap(int n, const double4 *x, double4 *ux, const int *image, 
 }
 
 // Neighbor::check_distance, src/neighbor.cpp:79-97
-__global__ void k_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag) {
+// clear != nullptr: block 0 also zeroes *clear (the flag of the step after)
+__global__ void k_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag, int *clear) {
+  if (clear && blockIdx.x == 0 && threadIdx.x == 0) *clear = 0;
   int f = 0;
   GRID_STRIDE(i, n) {
     const double4 a = x[i], h = xh[i];
@@ -744,11 +746,11 @@ __device__ __forceinline__ bool lj_pair(const double4 &xi, const double4 &xj, co
 }
 __global__ void k_lj_force(int ng, LJParams lj, const double4 *gx, const int *rowof,
                            const int *hoff, const int *hneigh, const int *toff, const int *tpair,
-                           const int *ti, int nowned, double4 *gF, double *energy) {
+                           const int *ti, int nowned, double4 *gF, double *energy, int zero_init) {
   double esum = 0.0;
   GRID_STRIDE(a, ng) {
     const double4 xa = gx[a];
-    double4 F = gF[a];
+    double4 F = zero_init ? make_double4(0.0, 0.0, 0.0, 0.0) : gF[a];
     const int row = rowof[a];
     const int rs = row >= 0 ? hoff[row] : 0x7fffffff;
     const int re = row >= 0 ? hoff[row + 1] : 0x7fffffff;
@@ -898,9 +900,13 @@ void launch_unmap(int n, const double4 *x, double4 *ux, const int *image, const 
   hipLaunchKernelGGL(k_unmap, dim3(nblk(n)), dim3(TPB), 0, s, n, x, ux, image, b);
   HIPCHK(hipGetLastError());
 }
-void launch_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag, hipStream_t s) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_decide, dim3(nblk(n)), dim3(TPB), 0, s, n, x, xh, triggersq, flag);
+void launch_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag, int *clear,
+                   hipStream_t s) {
+  if (n <= 0) {
+    if (clear) launch_fill_i32(1, clear, 0, s);
+    return;
+  }
+  hipLaunchKernelGGL(k_decide, dim3(nblk(n)), dim3(TPB), 0, s, n, x, xh, triggersq, flag, clear);
   HIPCHK(hipGetLastError());
 }
 void launch_copy4(int n, const double4 *a, double4 *b, hipStream_t s) {
@@ -1021,9 +1027,12 @@ void launch_ga_deposit(const GridView &gv, const GAParams &ga, const int *cnt, c
 }
 // Bead density: rho(g) = sum over gathered atoms of the grid-atom kernel
 // value e at g (the factor of 2 (phi - phi_t) in the mu term), for points
-// within the pair cut, FP32 atomics (an input feature of the surrogate; the
-// order of the adds is free).
-__global__ void k_ga_density(int ng, GridView gv, GAParams ga, const double4 *gx, float *rho) {
+// within the pair cut (an input feature of the surrogate).  Each term is
+// rounded to a 64-bit fixed-point integer (e * scale) and added with integer
+// atomics, which are associative: the sum, and so rho, does not depend on
+// the order of the adds.
+__global__ void k_ga_density(int ng, GridView gv, GAParams ga, const double4 *gx, double scale,
+                             unsigned long long *acc) {
   GRID_STRIDE(q, ng) {
     const double4 x = gx[q];
     if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z))) continue;
@@ -1044,11 +1053,15 @@ __global__ void k_ga_density(int ng, GridView gv, GAParams ga, const double4 *gx
           double e;
           if (ga.style == 0) e = (exp(-r / 2.0) - ga.em05) * ga.e05 * ga.eps[t];
           else e = exp(-1. / (1 - r) + 1) * ga.eps[t];
-          atomicAdd(&rho[((size_t)i * gv.ny + j) * gv.nx + k], (float)e);
+          atomicAdd(&acc[((size_t)i * gv.ny + j) * gv.nx + k], (unsigned long long)llrint(e * scale));
         }
       }
     }
   }
+}
+
+__global__ void k_fixed_to_float(size_t n, const unsigned long long *acc, double inv_scale, float *rho) {
+  GRID_STRIDE(i, n) rho[i] = (float)((double)(long long)acc[i] * inv_scale);
 }
 
 __global__ void k_add4(int n, const double4 *a, double4 *b) {
@@ -1067,9 +1080,13 @@ void launch_add4(int n, const double4 *a, double4 *b, hipStream_t s) {
   HIPCHK(hipGetLastError());
 }
 
-void launch_ga_density(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, float *rho, hipStream_t s) {
-  if (ng <= 0) return;
-  hipLaunchKernelGGL(k_ga_density, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, rho);
+void launch_ga_density(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, double scale,
+                       unsigned long long *acc, size_t N, float *rho, hipStream_t s) {
+  if (ng > 0) {
+    hipLaunchKernelGGL(k_ga_density, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, scale, acc);
+    HIPCHK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_fixed_to_float, dim3(nblk(N)), dim3(TPB), 0, s, N, acc, 1.0 / scale, rho);
   HIPCHK(hipGetLastError());
 }
 void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
@@ -1118,10 +1135,10 @@ PHAFD_GA_DEF(float, float4)
 #undef PHAFD_GA_DEF
 void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,
                      const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
-                     double4 *gF, double *energy, hipStream_t s) {
+                     double4 *gF, double *energy, hipStream_t s, bool zero_init) {
   if (ng <= 0) return;
   hipLaunchKernelGGL(k_lj_force, dim3(nblk(ng)), dim3(TPB), 0, s, ng, lj, gx, rowof, hoff, hneigh, toff,
-                     tpair, ti, nowned, gF, energy);
+                     tpair, ti, nowned, gF, energy, zero_init ? 1 : 0);
   HIPCHK(hipGetLastError());
 }
 void launch_drag(int n, const int *members, double4 *x, const double4 *F, double drag, double dt,

@@ -44,7 +44,11 @@ struct ChainParams {
 
 void launch_pbc(int n, double4 *x, double4 *ux, int *image, const Box &b, hipStream_t s);
 void launch_unmap(int n, const double4 *x, double4 *ux, const int *image, const Box &b, hipStream_t s);
-void launch_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag, hipStream_t s);
+// Neighbor::decide: flag |= 1 if an atom moved more than sqrt(triggersq)
+// since the build, |= 2 for a non-finite position; clear (may be null) is
+// zeroed by the same launch
+void launch_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag, int *clear,
+                   hipStream_t s);
 void launch_copy4(int n, const double4 *a, double4 *b, hipStream_t s);
 void launch_copy_classify(int n, const double4 *x, double zstart, double zend, double lo, double hi,
                           int *cat, int *flag, hipStream_t s);
@@ -80,8 +84,11 @@ void launch_ga_deposit(const GridView &gv, const GAParams &ga, const int *cnt, c
 // of the pair arrays
 void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
                      const double *cutsq, int *acnt, hipStream_t s);
-// bead density (surrogate input): rho += kernel value e of every in-cut pair
-void launch_ga_density(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, float *rho, hipStream_t s);
+// bead density (surrogate input): rho = sum of the kernel value e of every
+// in-cut pair, accumulated in acc[N] (zeroed by the caller) as integers
+// e * scale and converted to FP32 (order-independent, so bit-reproducible)
+void launch_ga_density(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, double scale,
+                       unsigned long long *acc, size_t N, float *rho, hipStream_t s);
 void launch_ga_enum(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
                     const int *aoff, int64_t cap, int *pg, int *pq, int *overflow, hipStream_t s);
 // launch_ga_enum / _pair / _fold: the atom side of the pass (pairs of each
@@ -104,9 +111,11 @@ void launch_ga_enum(int ng, const GridView &gv, const GAParams &ga, const double
 PHAFD_GA_DECL(double, double4)
 PHAFD_GA_DECL(float, float4)
 #undef PHAFD_GA_DECL
+// lj/cut forces of every gathered atom added to gF (zero_init: written from
+// zero, the fill of gF left out when lj/cut is its only writer)
 void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,
                      const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
-                     double4 *gF, double *energy, hipStream_t s);
+                     double4 *gF, double *energy, hipStream_t s, bool zero_init = false);
 void launch_drag(int n, const int *members, double4 *x, const double4 *F, double drag, double dt,
                  hipStream_t s);
 // one half step of the chains (contiguous atom ranges [chs[c], che[c]),

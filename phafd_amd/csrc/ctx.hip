@@ -92,7 +92,7 @@ struct phafd_hip_ctx {
   DBuf<unsigned long long> counters;   // [nless, ngreat]
   DBuf<int> flags;                     // [0] rebuild, [1] overflow
   DBuf<double> esum;                   // energy accumulators
-  int *h_flags = nullptr;              // pinned: [rebuild, overflow, pad, pad] + phi0 as 2 ints
+  int *h_flags = nullptr;              // pinned, 16 ints: [2..3] phi(0,0,0), [4] scalar reads, [8..15] flags
   // owned atoms
   int nowned = 0, ntypes = 0;
   bool atoms_set = false;
@@ -157,6 +157,7 @@ struct phafd_hip_ctx {
   // output, the predicted phi, mu, the base, FP64 partial sums, frozen
   // grid-atom forces, the atoms' block-start state
   DBuf<float> sg_rho, sg_rho1, sg_out;
+  DBuf<unsigned long long> sg_acc;   // fixed-point density accumulator
   DBuf<uint16_t> sg_x0;
   DBuf<double> sg_phi, sg_mu0, sg_mu1, sg_base, sg_red, sg_sum;
   DBuf<double4> sg_fga, sg_x, sg_ux;
@@ -552,15 +553,42 @@ void neighbor_build(phafd_hip_ctx *c) {
   c->lists_valid = true;
 }
 
+// Device flags: [0] and [4] the rebuild decision (Neighbor::decide) made at
+// the end of a step, slot 4 * (timestep & 1); [1] list overflow; [2] atom
+// outside the bins; [3] grid-atom term capacity; [5] an immediate decide.
+// A step's decide clears the slot of the step after it, so no fill runs per
+// step; a run loop clears its first step's slot once.
+int decide_slot(const phafd_hip_ctx *c) { return 4 * (int)(c->timestep & 1); }
+void clear_decide_slot(phafd_hip_ctx *c, int64_t step) {
+  launch_fill_i32(1, c->flags.p + 4 * (int)(step & 1), 0, c->stream);
+}
+void enqueue_decide(phafd_hip_ctx *c) {
+  const int sl = decide_slot(c);
+  launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->flags.p + sl, c->flags.p + (4 - sl), c->stream);
+}
+// the step's flags to h_flags[8..15], read after the step's synchronise
+void enqueue_flags_readback(phafd_hip_ctx *c) {
+  HIPCHK(hipMemcpyAsync(c->h_flags + 8, c->flags.p, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+}
+// the checks after a step's synchronise (src/integrate.cpp:190-201)
+void check_step_flags(phafd_hip_ctx *c) {
+  if (c->h_flags[11]) throw std::runtime_error("grid-atom term capacity exceeded");
+  if (c->neigh_set) {
+    const int f = c->h_flags[8 + decide_slot(c)];
+    if (f & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
+    c->pending_decision = f & 1;
+  }
+}
+
 bool neighbor_decide(phafd_hip_ctx *c) {
   if (c->pending_decision >= 0) {
     bool r = c->pending_decision != 0;
     c->pending_decision = -1;
     return r;
   }
-  launch_fill_i32(1, c->flags.p, 0, c->stream);
-  launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->flags.p, c->stream);
-  const int f = read_int(c, c->flags.p);
+  launch_fill_i32(1, c->flags.p + 5, 0, c->stream);
+  launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->flags.p + 5, nullptr, c->stream);
+  const int f = read_int(c, c->flags.p + 5);
   if (f & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
   return (f & 1) != 0;
 }
@@ -906,6 +934,7 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
   int nconj = 0;
   for (auto &f : c->fixes) nconj += f->kind == FixDef::CONJ;
   const bool zsplit = nconj == 1 && spectral_fused(c);
+  if (c->neigh_set) clear_decide_slot(c, c->timestep + 1);
   for (int it = 0; it < nsteps; it++) {
     c->timestep++;
     // muhat is observable only after the run: its last step stores it
@@ -931,11 +960,8 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
     HIPCHK(hipEventRecord(c->ev[7], c->stream));
     // next step's rebuild decision (Neighbor::decide) + NaN check of
     // phi(0,0,0) (src/integrate.cpp:190-201), read back together
-    if (c->neigh_set) {
-      launch_fill_i32(1, c->flags.p, 0, c->stream);
-      launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->flags.p, c->stream);
-    }
-    HIPCHK(hipMemcpyAsync(c->h_flags + 8, c->flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (c->neigh_set) enqueue_decide(c);
+    enqueue_flags_readback(c);
     HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->fields<R>().phi.p, sizeof(R), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipEventRecord(c->ev[8], c->stream));
     sync(c);
@@ -951,11 +977,7 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
     R phi0;
     std::memcpy(&phi0, c->h_flags + 2, sizeof(R));
     if (std::isnan(phi0)) throw std::runtime_error("NAN encountered in phi.");
-    if (c->h_flags[11]) throw std::runtime_error("grid-atom term capacity exceeded");
-    if (c->neigh_set) {
-      if (c->h_flags[8] & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
-      c->pending_decision = c->h_flags[8] & 1;
-    }
+    check_step_flags(c);
   }
 }
 
@@ -1123,7 +1145,7 @@ void init_ctx(phafd_hip_ctx *c, const phafd_hip_desc *desc, const MemberSlot *sl
       c->counters.alloc(2);
     }
     c->mass64.alloc(1);
-    c->flags.alloc(4);
+    c->flags.alloc(8);
     c->esum.alloc(8);
     HIPCHK(hipHostMalloc((void **)&c->h_flags, 64, hipHostMallocDefault));
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
@@ -1929,6 +1951,8 @@ void capture(phafd_hip_batch *b, hipGraph_t *g, hipGraphExec_t *x, Fn fn) {
 
 void batch_steps(phafd_hip_batch *b, int nsteps) {
   if (!b->setup_done) throw std::invalid_argument("call phafd_hip_batch_setup before running");
+  for (auto &m : b->members)
+    if (m->neigh_set) clear_decide_slot(m.get(), m->timestep + 1);
   for (int it = 0; it < nsteps; it++) {
     for (auto &m : b->members) m->timestep++;
     if (b->use_graphs) HIPCHK(hipGraphLaunch(b->x_grad, b->stream));
@@ -1949,11 +1973,8 @@ void batch_steps(phafd_hip_batch *b, int nsteps) {
       phafd_hip_ctx *c = b->members[i].get();
       for (auto &f : c->fixes)
         if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f);
-      if (c->neigh_set) {   // next step's rebuild decision (Neighbor::decide)
-        launch_fill_i32(1, c->flags.p, 0, c->stream);
-        launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->flags.p, c->stream);
-      }
-      HIPCHK(hipMemcpyAsync(c->h_flags + 8, c->flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      if (c->neigh_set) enqueue_decide(c);   // next step's rebuild decision (Neighbor::decide)
+      enqueue_flags_readback(c);
       HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->fd.phi.p, sizeof(double), hipMemcpyDeviceToHost, b->stream));
     }
     batch_sync(b);
@@ -1963,11 +1984,7 @@ void batch_steps(phafd_hip_batch *b, int nsteps) {
       double phi0;
       std::memcpy(&phi0, c->h_flags + 2, sizeof(double));
       if (std::isnan(phi0)) throw std::runtime_error("NAN encountered in phi (member " + std::to_string(i) + ").");
-      if (c->h_flags[11]) throw std::runtime_error("grid-atom term capacity exceeded");
-      if (c->neigh_set) {
-        if (c->h_flags[8] & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
-        c->pending_decision = c->h_flags[8] & 1;
-      }
+      check_step_flags(c);
     }
   }
 }
@@ -2184,7 +2201,7 @@ SurrogateCfg surrogate_cfg(phafd_hip_ctx *c) {
       if (v != g.phit) throw std::invalid_argument("the surrogate needs one phi_t for all atom types");
   }
   const size_t N = c->N;
-  c->sg_rho.alloc(N); c->sg_rho1.alloc(N); c->sg_out.alloc(N); c->sg_x0.alloc(8 * N);
+  c->sg_rho.alloc(N); c->sg_rho1.alloc(N); c->sg_out.alloc(N); c->sg_x0.alloc(8 * N); c->sg_acc.alloc(N);
   c->sg_phi.alloc(N); c->sg_mu0.alloc(N); c->sg_mu1.alloc(N); c->sg_base.alloc(N);
   c->sg_red.alloc(2 * SG_NB + 2);
   c->sg_sum.alloc(4);
@@ -2200,10 +2217,19 @@ void refresh_gathered(phafd_hip_ctx *c) {
   else phase_neigh(c);
 }
 
+// 2^32 fixed-point units per max|eps| (a point would need ~2e9 beads at
+// full kernel weight to overflow; resolution ~2e-10 max|eps|)
 void deposit_density(phafd_hip_ctx *c, const SurrogateCfg &g, float *rho) {
-  fill_zero(rho, sizeof(float) * c->N, c->stream);
-  if (g.ga && c->neigh_set && c->lists_valid)
-    launch_ga_density(c->ngathered, c->gridview<double>(), ga_params(*g.ga), c->gx.p, rho, c->stream);
+  if (!(g.ga && c->neigh_set && c->lists_valid)) {
+    fill_zero(rho, sizeof(float) * c->N, c->stream);
+    return;
+  }
+  double emax = 0;
+  for (double e : g.ga->eps) emax = std::max(emax, std::fabs(e));
+  if (!(emax > 0)) emax = 1;
+  fill_zero(c->sg_acc.p, sizeof(unsigned long long) * c->N, c->stream);
+  launch_ga_density(c->ngathered, c->gridview<double>(), ga_params(*g.ga), c->gx.p, 4294967296.0 / emax,
+                    c->sg_acc.p, c->N, rho, c->stream);
 }
 
 // sg_sum[0..1] = FP64 {sum phi0, sum rho0}
@@ -2261,6 +2287,7 @@ void block_check(phafd_hip_ctx *c, const SurrogateCfg &g, const double *phi0, co
 // force of the current field computed once and held
 void atom_steps_frozen_field(phafd_hip_ctx *c, const SurrogateCfg &g, int K) {
   phase_start_t<double>(c);   // grad phi of the block-start field
+  if (c->neigh_set) clear_decide_slot(c, c->timestep + 1);
   for (int s = 0; s < K; s++) {
     c->timestep++;
     phase_neigh(c);
@@ -2274,32 +2301,28 @@ void atom_steps_frozen_field(phafd_hip_ctx *c, const SurrogateCfg &g, int K) {
       }
       launch_copy4(c->nowned, c->F.p, c->sg_fga.p, c->stream);
     }
-    launch_fill_d4(c->ngathered, c->gF.p, c->stream);
-    launch_fill_d4(c->nowned, c->F.p, c->stream);
+    // F starts as the held grid-atom force; lj/cut, the only pair force of
+    // these steps, writes gF from zero and the fold adds it: the same sums
+    // as zeroing both, folding and adding the held force after
+    launch_copy4(c->nowned, c->sg_fga.p, c->F.p, c->stream);
     if (lists) {
+      bool any = false;
       for (auto &pp : c->pairs) {
         if (pp->style != 2) continue;
         LJParams lj{pp->lj1, pp->lj2, pp->ljcutsq};
         launch_lj_force(c->ngathered, lj, c->gx.p, c->h_rowof.p, c->h_off.p, c->h_neigh.p, c->t_off.p,
-                        c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, nullptr, c->stream);
+                        c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, nullptr, c->stream, !any);
+        any = true;
       }
-      reverse_comm(c);
+      if (any) reverse_comm(c);
     }
-    launch_add4(c->nowned, c->sg_fga.p, c->F.p, c->stream);
     phase_initial(c);
     for (auto &f : c->fixes)
       if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f);
-    if (c->neigh_set) {   // next step's rebuild decision (Neighbor::decide)
-      launch_fill_i32(1, c->flags.p, 0, c->stream);
-      launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->flags.p, c->stream);
-    }
-    HIPCHK(hipMemcpyAsync(c->h_flags + 8, c->flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (c->neigh_set) enqueue_decide(c);   // next step's rebuild decision (Neighbor::decide)
+    enqueue_flags_readback(c);
     sync(c);
-    if (c->h_flags[11]) throw std::runtime_error("grid-atom term capacity exceeded");
-    if (c->neigh_set) {
-      if (c->h_flags[8] & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
-      c->pending_decision = c->h_flags[8] & 1;
-    }
+    check_step_flags(c);
   }
 }
 

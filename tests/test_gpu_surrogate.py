@@ -12,7 +12,9 @@
                        give the same losses bit for bit
   block check, base . residual, mass drift and the linearised K-step base
                        against numpy (rfftn) on the same inputs: 1e-9
-  bead density ....... against a periodic brute-force sum: 1e-5 (FP32 adds)
+  bead density ....... against a periodic brute-force sum: 1e-5 (FP32 output)
+  reproducibility .... two surrogate runs from one state: identical phi and
+                       positions (the density is summed in fixed point)
   fallback ........... a rejected block is K plain steps from the restored
                        atoms (1e-13: re-binned lists may reorder sums)
   gain ............... the committed trained network against the same block
@@ -277,6 +279,25 @@ def test_rejected_blocks_are_plain_steps():
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "phafd_amd", "data",
                        "surrogate_gaussian64.npz")
+
+
+def test_surrogate_runs_are_bit_reproducible():
+    # every kernel of a block sums in a fixed order (the bead density in
+    # 64-bit fixed point), so two runs from one state agree bit for bit
+    z = np.load(WEIGHTS)
+    K, tol = int(z["K"][0]), float(z["tol"][0])
+    out = []
+    for _ in range(2):
+        s = S.make_system(P, 64, S.FAMILIES["production"], seed=501)
+        net = P.UNet(seed=1)
+        net.load(WEIGHTS)
+        st = s.run_surrogate(net, 2 * K, K, tol)
+        out.append((s.get_field(P.PHI), s.get_atoms()[0], st["accepted"]))
+        net.close()
+        s.close()
+    assert out[0][2] == out[1][2] == 2
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
 
 
 def test_trained_network_gains_over_the_base_alone():
