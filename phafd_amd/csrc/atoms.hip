@@ -832,31 +832,48 @@ __device__ __forceinline__ void chain_bead_force(const double4 *u, int b, int nb
 // bonded force from them, then x += drag (F + Fchain) dt / 2 on the
 // unwrapped position and Domain::map (src/domain.cpp:181-193) back to x.
 // Chains are contiguous atom ranges [chs[c], che[c]).
+// halves = 2: both half steps of a timestep in one launch (F is the same for
+// both; the first half's positions stay in LDS, in the second buffer at
+// us + stride, and only the second half is written out)
 __global__ void k_chain_step(const int *chs, const int *che, const double4 *F, double4 *ux, double4 *x,
-                             const int *image, ChainParams cp, double dt, Box bx) {
+                             const int *image, ChainParams cp, double dt, Box bx, int halves, int stride) {
   extern __shared__ double4 us[];
   const int s = chs[blockIdx.x], nb = che[blockIdx.x] - s;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) us[b] = ux[s + b];
+  double4 *cur = us, *nxt = us + stride;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) cur[b] = ux[s + b];
   __syncthreads();
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    double acc[3];
-    chain_bead_force(us, b, nb, cp, acc);
-    const int j = s + b;
-    const double4 f = F[j];
-    double4 u = us[b];
-    const double tx = f.x + acc[0], ty = f.y + acc[1], tz = f.z + acc[2];
-    u.x += cp.drag * tx * dt * 0.5;
-    u.y += cp.drag * ty * dt * 0.5;
-    u.z += cp.drag * tz * dt * 0.5;
-    ux[j] = u;
-    const int img = image[j];
-    const int xb = (img & IMGMASK) - IMGMAX, yb = (img >> IMGBITS & IMGMASK) - IMGMAX,
-              zb = (img >> IMG2BITS) - IMGMAX;
-    double4 p = x[j];
-    p.x = u.x - xb * bx.period[0];
-    p.y = u.y - yb * bx.period[1];
-    p.z = u.z - zb * bx.period[2];
-    x[j] = p;
+  for (int h = 0; h < halves; h++) {
+    const bool last = h + 1 == halves;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+      double acc[3];
+      chain_bead_force(cur, b, nb, cp, acc);
+      const int j = s + b;
+      const double4 f = F[j];
+      double4 u = cur[b];
+      const double tx = f.x + acc[0], ty = f.y + acc[1], tz = f.z + acc[2];
+      u.x += cp.drag * tx * dt * 0.5;
+      u.y += cp.drag * ty * dt * 0.5;
+      u.z += cp.drag * tz * dt * 0.5;
+      if (!last) {
+        nxt[b] = u;
+        continue;
+      }
+      ux[j] = u;
+      const int img = image[j];
+      const int xb = (img & IMGMASK) - IMGMAX, yb = (img >> IMGBITS & IMGMASK) - IMGMAX,
+                zb = (img >> IMG2BITS) - IMGMAX;
+      double4 p = x[j];
+      p.x = u.x - xb * bx.period[0];
+      p.y = u.y - yb * bx.period[1];
+      p.z = u.z - zb * bx.period[2];
+      x[j] = p;
+    }
+    if (!last) {
+      __syncthreads();
+      double4 *t = cur;
+      cur = nxt;
+      nxt = t;
+    }
   }
 }
 __global__ void k_chain_energy(int n, const int *members, const int *cstart, const int *cend,
@@ -1148,11 +1165,13 @@ void launch_drag(int n, const int *members, double4 *x, const double4 *F, double
   HIPCHK(hipGetLastError());
 }
 void launch_chain_step(int nchains, int maxlen, const int *chs, const int *che, const double4 *F, double4 *ux,
-                       double4 *x, const int *image, const ChainParams &cp, double dt, const Box &b, hipStream_t s) {
+                       double4 *x, const int *image, const ChainParams &cp, double dt, const Box &b, hipStream_t s,
+                       int halves) {
   if (nchains <= 0 || maxlen <= 0) return;
+  if (halves != 1 && halves != 2) throw std::invalid_argument("chain step: halves must be 1 or 2");
   const int threads = maxlen <= 64 ? 64 : (maxlen <= 128 ? 128 : 256);
-  hipLaunchKernelGGL(k_chain_step, dim3(nchains), dim3(threads), sizeof(double4) * maxlen, s, chs, che, F, ux, x,
-                     image, cp, dt, b);
+  hipLaunchKernelGGL(k_chain_step, dim3(nchains), dim3(threads), sizeof(double4) * maxlen * halves, s, chs, che, F,
+                     ux, x, image, cp, dt, b, halves, maxlen);
   HIPCHK(hipGetLastError());
 }
 void launch_chain_energy(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,

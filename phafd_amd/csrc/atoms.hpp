@@ -118,10 +118,12 @@ void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *r
                      double4 *gF, double *energy, hipStream_t s, bool zero_init = false);
 void launch_drag(int n, const int *members, double4 *x, const double4 *F, double drag, double dt,
                  hipStream_t s);
-// one half step of the chains (contiguous atom ranges [chs[c], che[c]),
-// at most maxlen beads, maxlen * 32 bytes of LDS per block)
+// halves (1 or 2) half steps of the chains (contiguous atom ranges
+// [chs[c], che[c]), at most maxlen beads, halves * maxlen * 32 bytes of LDS
+// per block); 2: both halves of a timestep, with the same F
 void launch_chain_step(int nchains, int maxlen, const int *chs, const int *che, const double4 *F, double4 *ux,
-                       double4 *x, const int *image, const ChainParams &cp, double dt, const Box &b, hipStream_t s);
+                       double4 *x, const int *image, const ChainParams &cp, double dt, const Box &b, hipStream_t s,
+                       int halves = 1);
 void launch_chain_energy(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,
                          const ChainParams &cp, double *energy, hipStream_t s);
 void launch_fill_i32(int n, int *p, int v, hipStream_t s);

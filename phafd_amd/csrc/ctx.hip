@@ -811,6 +811,28 @@ void phase_initial(phafd_hip_ctx *c) {
   for (auto &f : c->fixes) integrate_half(c, *f);
 }
 
+// The chain fix when it is the only atom integrator (and its chains fit two
+// LDS buffers of 32 KB): then both half steps of a timestep run as one
+// launch at initial_integrate.  F is computed once per step by the pair pass
+// and the hooks between initial_integrate and final_integrate work on the
+// fields only, so nothing reads or moves the atoms in between and the
+// positions are those of two separate half steps.
+FixDef *fused_chain_fix(phafd_hip_ctx *c) {
+  FixDef *ch = nullptr;
+  int n = 0;
+  for (auto &f : c->fixes) {
+    if (f->kind == FixDef::CONJ || f->kind == FixDef::FH) continue;
+    n++;
+    if (f->kind == FixDef::CHAIN) ch = f.get();
+  }
+  return (n == 1 && ch && ch->maxlen <= 1024) ? ch : nullptr;
+}
+void integrate_both_halves(phafd_hip_ctx *c, FixDef &f) {
+  ChainParams cp{f.r0, f.kbond, f.kbend, f.drag};
+  launch_chain_step(f.nchain, f.maxlen, f.chs.p, f.che.p, c->F.p, c->ux.p, c->x.p, c->image.p, cp, c->dt, c->box(),
+                    c->stream, 2);
+}
+
 template <class R>
 void phase_postforce_t(phafd_hip_ctx *c) {
   FieldSet<R> &F = c->fields<R>();
@@ -827,14 +849,14 @@ void phase_prefinal_t(phafd_hip_ctx *c, const FHPreT<R> *pre = nullptr) {
 }
 
 template <class R>
-void phase_final_t(phafd_hip_ctx *c) {
+void phase_final_t(phafd_hip_ctx *c, bool atoms = true) {
   FieldSet<R> &F = c->fields<R>();
   for (auto &fp : c->fixes) {
     FixDef &f = *fp;
     if (f.kind == FixDef::CONJ) {
       const auto *nz = (f.noise_mode == PHAFD_NOISE_FIELD && c->noise_set) ? F.noise_k.p : nullptr;
       spectral_update(c->plan, F.phihat.p, F.muhat.p, nz, update_params(c, f), origin_ptr(c), c->stream);
-    } else {
+    } else if (atoms) {
       integrate_half(c, f);
     }
   }
@@ -856,7 +878,7 @@ void phase_prefinal_split(phafd_hip_ctx *c, const FHPreT<R> *pre) {
   fft_r2c_xy(c->plan, F.mu.p, F.muhat.p, c->stream, pre);
 }
 template <class R>
-void phase_final_split(phafd_hip_ctx *c) {
+void phase_final_split(phafd_hip_ctx *c, bool atoms = true) {
   FieldSet<R> &F = c->fields<R>();
   for (auto &fp : c->fixes) {
     FixDef &f = *fp;
@@ -864,7 +886,7 @@ void phase_final_split(phafd_hip_ctx *c) {
       const auto *nz = (f.noise_mode == PHAFD_NOISE_FIELD && c->noise_set) ? F.noise_k.p : nullptr;
       fft_z_update(c->plan, F.muhat.p, F.phihat.p, nz, update_params(c, f), origin_ptr(c), c->stream, Slab(),
                    c->store_muhat);
-    } else {
+    } else if (atoms) {
       integrate_half(c, f);
     }
   }
@@ -934,6 +956,7 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
   int nconj = 0;
   for (auto &f : c->fixes) nconj += f->kind == FixDef::CONJ;
   const bool zsplit = nconj == 1 && spectral_fused(c);
+  FixDef *fc = fused_chain_fix(c);
   if (c->neigh_set) clear_decide_slot(c, c->timestep + 1);
   for (int it = 0; it < nsteps; it++) {
     c->timestep++;
@@ -946,14 +969,15 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     phase_pair(c);
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
-    phase_initial(c);
+    if (fc) integrate_both_halves(c, *fc);
+    else phase_initial(c);
     HIPCHK(hipEventRecord(c->ev[4], c->stream));
     if (!fused) phase_postforce_t<R>(c);
     if (zsplit) phase_prefinal_split<R>(c, fused ? &pre : nullptr);
     else phase_prefinal_t<R>(c, fused ? &pre : nullptr);
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
-    if (zsplit) phase_final_split<R>(c);
-    else phase_final_t<R>(c);
+    if (zsplit) phase_final_split<R>(c, !fc);
+    else phase_final_t<R>(c, !fc);
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
     if (zsplit) phase_postfinal_split<R>(c);
     else phase_postfinal_t<R>(c);
@@ -1963,7 +1987,8 @@ void batch_steps(phafd_hip_batch *b, int nsteps) {
       HIPCHK(hipStreamWaitEvent(c->stream, b->ev_grad, 0));
       phase_neigh(c);
       phase_pair(c);
-      phase_initial(c);
+      if (FixDef *fc = fused_chain_fix(c)) integrate_both_halves(c, *fc);
+      else phase_initial(c);
       HIPCHK(hipEventRecord(b->ev_m[i], c->stream));
     }
     for (int i = 0; i < b->n; i++) HIPCHK(hipStreamWaitEvent(b->stream, b->ev_m[i], 0));
@@ -1971,8 +1996,9 @@ void batch_steps(phafd_hip_batch *b, int nsteps) {
     else batch_spec(b, b->chunk);
     for (int i = 0; i < b->n; i++) {
       phafd_hip_ctx *c = b->members[i].get();
-      for (auto &f : c->fixes)
-        if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f);
+      if (!fused_chain_fix(c))
+        for (auto &f : c->fixes)
+          if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f);
       if (c->neigh_set) enqueue_decide(c);   // next step's rebuild decision (Neighbor::decide)
       enqueue_flags_readback(c);
       HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->fd.phi.p, sizeof(double), hipMemcpyDeviceToHost, b->stream));
@@ -2316,9 +2342,13 @@ void atom_steps_frozen_field(phafd_hip_ctx *c, const SurrogateCfg &g, int K) {
       }
       if (any) reverse_comm(c);
     }
-    phase_initial(c);
-    for (auto &f : c->fixes)
-      if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f);
+    if (FixDef *fc = fused_chain_fix(c)) {
+      integrate_both_halves(c, *fc);
+    } else {
+      phase_initial(c);
+      for (auto &f : c->fixes)
+        if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f);
+    }
     if (c->neigh_set) enqueue_decide(c);   // next step's rebuild decision (Neighbor::decide)
     enqueue_flags_readback(c);
     sync(c);

@@ -154,13 +154,39 @@ __global__ void k_fill_zero_bytes(unsigned char *p, size_t n) {
     p[i] = 0;
 }
 
+// FP64 reductions in a fixed order, the same bits on every run: each block
+// writes its partial (a fixed grid-stride share, combined in a fixed tree),
+// then one block folds the partials in index order and adds them to out.
+constexpr int RED_BLOCKS = 2048;
+template <int W>
+__global__ void k_fold_partials(int nb, const double *__restrict__ part, double *out) {
+  __shared__ double ws[W][TPB / 64];
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    double a = 0;
+    for (int i = threadIdx.x; i < nb; i += TPB) a += part[(size_t)i * W + k];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) a += __shfl_down(a, d, 64);
+    if ((threadIdx.x & 63) == 0) ws[k][threadIdx.x >> 6] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      double t = 0;
+      for (int w = 0; w < TPB / 64; w++) t += ws[k][w];
+      out[k] += t;
+    }
+  }
+}
+
 // Flory-Huggins + square-gradient free energy (defined by this build):
 // sum of phi ln phi + (1-phi) ln(1-phi) + chi phi (1-phi) and |grad phi|^2,
-// evaluated and summed in FP64 (block partial sums, one atomicAdd per block).
+// evaluated and summed in FP64 (block partials part[2 * block + 0..1]).
 template <class R>
 __global__ void k_energy_fh(size_t n, const R *__restrict__ phi, const R *__restrict__ gx,
                             const R *__restrict__ gy, const R *__restrict__ gz, double tv, double chi,
-                            double tvg, double *out2) {
+                            double tvg, double *part) {
   __shared__ double s0[TPB], s1[TPB];
   double a = 0, b = 0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -185,14 +211,14 @@ __global__ void k_energy_fh(size_t n, const R *__restrict__ phi, const R *__rest
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    atomicAdd(&out2[0], tv * s0[0]);
-    atomicAdd(&out2[1], 0.5 * tvg * s1[0]);
+    part[2 * blockIdx.x] = tv * s0[0];
+    part[2 * blockIdx.x + 1] = 0.5 * tvg * s1[0];
   }
 }
 
-// out[0] += sum of x, accumulated in FP64
+// part[block] = the block's FP64 sum of its grid-stride share of x
 template <class R>
-__global__ void k_sum_fp64(size_t n, const R *__restrict__ x, double *out) {
+__global__ void k_sum_fp64(size_t n, const R *__restrict__ x, double *part) {
   __shared__ double ws[TPB / 64];
   double a = 0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -205,7 +231,7 @@ __global__ void k_sum_fp64(size_t n, const R *__restrict__ x, double *out) {
   if (threadIdx.x == 0) {
     double t = 0;
     for (int w = 0; w < TPB / 64; w++) t += ws[w];
-    atomicAdd(out, t);
+    part[blockIdx.x] = t;
   }
 }
 
@@ -254,13 +280,20 @@ void floryhuggins_t(size_t n, R *phi, R *mu, double tv, double chi, unsigned lon
 template <class R>
 void reduce_energy_fh_t(size_t n, const R *phi, const R *gx, const R *gy, const R *gz, double tv, double chi,
                         double tvg, double *out2, hipStream_t s) {
-  hipLaunchKernelGGL(k_energy_fh<R>, dim3(grid_for(n) > 1024 ? 1024 : grid_for(n)), dim3(TPB), 0, s, n, phi, gx,
-                     gy, gz, tv, chi, tvg, out2);
+  const int nb = grid_for(n) > 1024 ? 1024 : grid_for(n);
+  double *part = reduce_partials(s, 2 * (size_t)RED_BLOCKS);
+  hipLaunchKernelGGL(k_energy_fh<R>, dim3(nb), dim3(TPB), 0, s, n, phi, gx, gy, gz, tv, chi, tvg, part);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_fold_partials<2>, dim3(1), dim3(TPB), 0, s, nb, (const double *)part, out2);
   HIPCHK(hipGetLastError());
 }
 template <class R>
 void sum_fp64_t(size_t n, const R *x, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_fp64<R>, dim3(grid_for(n) > 2048 ? 2048 : grid_for(n)), dim3(TPB), 0, s, n, x, out);
+  const int nb = grid_for(n) > RED_BLOCKS ? RED_BLOCKS : grid_for(n);
+  double *part = reduce_partials(s, 2 * (size_t)RED_BLOCKS);
+  hipLaunchKernelGGL(k_sum_fp64<R>, dim3(nb), dim3(TPB), 0, s, n, x, part);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_fold_partials<1>, dim3(1), dim3(TPB), 0, s, nb, (const double *)part, out);
   HIPCHK(hipGetLastError());
 }
 

@@ -207,6 +207,9 @@ void block_residual_partials(const FFTPlan &p, const double2 *ph0, const double2
 void scan_exclusive_i32_to_i64(const int *in, int64_t *out, size_t n, hipStream_t s);
 void scan_exclusive_i32(const int *in, int *out, size_t n, hipStream_t s);   // out[n] = total
 void scan_forget_stream(hipStream_t s);   // frees the scan workspace of a stream about to be destroyed
+// per-stream space for n double partial sums of a reduction on stream s
+// (grown after draining s; kept until scan_forget_stream)
+double *reduce_partials(hipStream_t s, size_t n);
 
 // ----------------------------------------------------------- atom kernels
 struct BinGeom {

@@ -112,6 +112,7 @@ __global__ __launch_bounds__(ST) void k_scan_apply(const TI *__restrict__ in, si
 // has drained, so no launch still queued reads a freed workspace.
 struct ScanWs {
   DBuf<int64_t> sum[4], off[4];
+  DBuf<double> red;   // block partials of the fixed-order FP64 reductions
 };
 std::mutex g_ws_mu;
 std::map<hipStream_t, std::unique_ptr<ScanWs>> g_ws;
@@ -156,6 +157,15 @@ void scan_impl(const TI *in, TO *out, size_t n, hipStream_t s, int level, ScanWs
 void scan_exclusive_i32_to_i64(const int *in, int64_t *out, size_t n, hipStream_t s) {
   scan_impl<int, int64_t>(in, out, n, s, 0, workspace(s));
 }
+double *reduce_partials(hipStream_t s, size_t n) {
+  ScanWs &w = workspace(s);
+  if (w.red.n < n) {
+    HIPCHK(hipStreamSynchronize(s));   // before a regrow
+    w.red.alloc(n);
+  }
+  return w.red.p;
+}
+
 void scan_forget_stream(hipStream_t s) {
   std::unique_ptr<ScanWs> w;
   {
