@@ -12,7 +12,7 @@ network is trained on the GPU (phafd_unet_train); the tolerance of the
 block check is set above the residuals of the trained network's own
 predictions on the held-out trajectories.
 
-    python -m phafd_amd.surrogate --out weights.npz [--K 50 --trajectories 60 --blocks 20]
+    python -m phafd_amd.surrogate --out weights.npz [--K 100 --trajectories 100 --blocks 10]
 """
 import argparse
 import json
@@ -121,9 +121,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
     # defaults: the committed network (phafd_amd/data/surrogate_gaussian64.npz)
-    ap.add_argument("--trajectories", type=int, default=60)
-    ap.add_argument("--blocks", type=int, default=20)
-    ap.add_argument("--K", type=int, default=50)
+    ap.add_argument("--trajectories", type=int, default=100)
+    ap.add_argument("--blocks", type=int, default=10)
+    ap.add_argument("--K", type=int, default=100)
     ap.add_argument("--steps", type=int, default=9000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -142,11 +142,17 @@ def main(argv=None):
     ntrain = (args.trajectories - args.holdout) * args.blocks
     nval = len(meta) - ntrain
     t0 = time.time()
-    chunks = [1.0, 0.7, 0.5, 0.3, 0.15, 0.05]
-    per = max(1, args.steps // len(chunks))
+    # learning rate per chunk: a warmup over the first half chunk (Adam's first
+    # updates have full size; at full rate they can leave every ReLU of a layer
+    # inactive, after which the network predicts a constant), then decay
+    per = max(1, args.steps // 6)
+    w = max(1, per // 6)
+    schedule = [(w, 0.1), (w, 0.3), (w, 0.6), (per - 3 * w, 1.0)] + [(per, c) for c in (0.7, 0.5, 0.3, 0.15, 0.05)]
     hist = []
-    for c in chunks:
-        losses = net.train(per, args.batch, ntrain, args.lr * c)
+    for nst, c in schedule:
+        if nst <= 0:
+            continue
+        losses = net.train(nst, args.batch, ntrain, args.lr * c)
         val = net.eval(ntrain, nval) if nval else np.zeros(0)
         hist.append((float(losses.mean()), float(val.mean()) if nval else None))
         log(f"lr {args.lr * c:.2e}: train mse {hist[-1][0]:.4e}, validation mse {hist[-1][1]}, "
