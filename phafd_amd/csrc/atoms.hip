@@ -180,6 +180,22 @@ __global__ void k_reverse(int cnt, const int *send, int first, double4 *target, 
     target[j] = t;
   }
 }
+// the copy swap's fold when every owned atom is in its send list once:
+// target[send[q]] = init[send[q]] + gF[first + q] (init: the owned atoms'
+// force before the fold), the copy and the fold in one pass
+__global__ void k_reverse_init(int cnt, const int *send, int first, double4 *target, const double4 *init,
+                               const double4 *gF) {
+  GRID_STRIDE(q, cnt) {
+    const int j = send[q];
+    const double4 b = gF[first + q];
+    double4 t = init[j];
+    t.x += b.x;
+    t.y += b.y;
+    t.z += b.z;
+    target[j] = t;
+  }
+}
+
 // the two image swaps of one dimension when no atom is in both send lists:
 // items [0, ca) fold swap a, [ca, ca + cb) swap b; every target is written
 // by one item, so the sums are those of two sequential launches
@@ -962,6 +978,12 @@ void launch_forward(int ng, const double4 *x, GatheredView g, hipStream_t s) {
 void launch_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF, hipStream_t s) {
   if (cnt <= 0) return;
   hipLaunchKernelGGL(k_reverse, dim3(nblk(cnt)), dim3(TPB), 0, s, cnt, send, first, target, gF);
+  HIPCHK(hipGetLastError());
+}
+void launch_reverse_init(int cnt, const int *send, int first, double4 *target, const double4 *init,
+                         const double4 *gF, hipStream_t s) {
+  if (cnt <= 0) return;
+  hipLaunchKernelGGL(k_reverse_init, dim3(nblk(cnt)), dim3(TPB), 0, s, cnt, send, first, target, init, gF);
   HIPCHK(hipGetLastError());
 }
 void launch_reverse2(int ca, const int *sa, int fa, int cb, const int *sb, int fb, double4 *gF, hipStream_t s) {

@@ -59,6 +59,8 @@ void launch_img_scatter(int nscan, const int *flag, const int *pos, int dim, dou
                         int *send, GatheredView g, hipStream_t s);
 void launch_forward(int ng, const double4 *x, GatheredView g, hipStream_t s);
 void launch_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF, hipStream_t s);
+void launch_reverse_init(int cnt, const int *send, int first, double4 *target, const double4 *init,
+                         const double4 *gF, hipStream_t s);
 void launch_reverse2(int ca, const int *sa, int fa, int cb, const int *sb, int fb, double4 *gF, hipStream_t s);
 void launch_atom2bin(int ng, const double4 *gx, const BinGeom &b, int *a2b, int *bincount, int *bad,
                      hipStream_t s);

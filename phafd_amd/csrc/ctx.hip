@@ -326,6 +326,11 @@ void borders(phafd_hip_ctx *c) {
 
 void forward_comm(phafd_hip_ctx *c) { launch_forward(c->ngathered, c->x.p, c->gview(), c->stream); }
 
+// reverse_comm with the owned forces starting from init (instead of their
+// current values): when the copy swap sends every owned atom once, the copy
+// and the last fold are one pass; the sums are those of copy + reverse_comm
+void reverse_comm_from(phafd_hip_ctx *c, const double4 *init);
+
 void reverse_comm(phafd_hip_ctx *c) {
   for (int iswap = (int)c->swaps.size() - 1; iswap >= 1; iswap--) {
     SwapDev &w = c->swaps[iswap];
@@ -339,6 +344,26 @@ void reverse_comm(phafd_hip_ctx *c) {
   }
   SwapDev &z = c->swaps[0];
   launch_reverse(z.count, z.send.p, z.first, c->F.p, c->gF.p, c->stream);
+}
+
+void reverse_comm_from(phafd_hip_ctx *c, const double4 *init) {
+  SwapDev &z = c->swaps[0];
+  if (z.count != c->nowned) {
+    launch_copy4(c->nowned, init, c->F.p, c->stream);
+    reverse_comm(c);
+    return;
+  }
+  for (int iswap = (int)c->swaps.size() - 1; iswap >= 1; iswap--) {
+    SwapDev &w = c->swaps[iswap];
+    if (w.disjoint_pair) {
+      SwapDev &v = c->swaps[iswap - 1];
+      launch_reverse2(w.count, w.send.p, w.first, v.count, v.send.p, v.first, c->gF.p, c->stream);
+      iswap--;
+      continue;
+    }
+    launch_reverse(w.count, w.send.p, w.first, c->gF.p, c->gF.p, c->stream);
+  }
+  launch_reverse_init(z.count, z.send.p, z.first, c->F.p, init, c->gF.p, c->stream);
 }
 
 // ------------------------------------------------------------ neighbour
@@ -2330,9 +2355,8 @@ void atom_steps_frozen_field(phafd_hip_ctx *c, const SurrogateCfg &g, int K) {
     // F starts as the held grid-atom force; lj/cut, the only pair force of
     // these steps, writes gF from zero and the fold adds it: the same sums
     // as zeroing both, folding and adding the held force after
-    launch_copy4(c->nowned, c->sg_fga.p, c->F.p, c->stream);
+    bool any = false;
     if (lists) {
-      bool any = false;
       for (auto &pp : c->pairs) {
         if (pp->style != 2) continue;
         LJParams lj{pp->lj1, pp->lj2, pp->ljcutsq};
@@ -2340,8 +2364,9 @@ void atom_steps_frozen_field(phafd_hip_ctx *c, const SurrogateCfg &g, int K) {
                         c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, nullptr, c->stream, !any);
         any = true;
       }
-      if (any) reverse_comm(c);
     }
+    if (any) reverse_comm_from(c, c->sg_fga.p);
+    else launch_copy4(c->nowned, c->sg_fga.p, c->F.p, c->stream);
     if (FixDef *fc = fused_chain_fix(c)) {
       integrate_both_halves(c, *fc);
     } else {
