@@ -147,6 +147,51 @@ def copy_calibration(nbytes, reps):
     print(json.dumps({"copy_calibration_bytes": n * 8, "reps": reps}), flush=True)
 
 
+def device_copy_gbs(nbytes=1 << 30, reps=5):
+    """Rate of a device-to-device hipMemcpyAsync of 1 GiB (read + write bytes
+    per second, timed with HIP events), reported beside the roofline's
+    nominal peak as a measured streaming rate of this device.  Calls the HIP
+    runtime the library loaded (found in the process's mappings), not
+    torch's own copy of it."""
+    import ctypes
+    import phafd_amd as P
+    P.lib()
+    path = None
+    with open("/proc/self/maps") as f:
+        for line in f:
+            if "libamdhip64.so" in line:
+                path = line.split()[-1]
+                break
+    if path is None:
+        return None
+    hip = ctypes.CDLL(path)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(nbytes)
+    try:
+        if hip.hipMalloc(ctypes.byref(a), n) or hip.hipMalloc(ctypes.byref(b), n):
+            return None
+        if hip.hipEventCreate(ctypes.byref(e0)) or hip.hipEventCreate(ctypes.byref(e1)):
+            return None
+        hip.hipMemcpyAsync(b, a, n, 3, None)   # hipMemcpyDeviceToDevice, null stream
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(b, a, n, 3, None)
+        hip.hipEventRecord(e1, None)
+        if hip.hipEventSynchronize(e1):
+            return None
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9 if ms.value > 0 else None
+    finally:
+        for e in (e0, e1):
+            if e.value:
+                hip.hipEventDestroy(e)
+        for p in (a, b):
+            if p.value:
+                hip.hipFree(p)
+
+
 def read_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (profiles/<kernel>_<workload>_pmc.md), or None."""
@@ -378,6 +423,7 @@ def main():
     bytes_per_launch = pbytes / max(plaunch, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if plaunch else None
     traffic = read_traffic(args.probe, args.workload)
+    copy_gbs = device_copy_gbs()
     out = {
         "metric": METRIC,
         "value": N * args.steps / elapsed,
@@ -397,7 +443,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": args.probe, "achieved": achieved, "peak": peak,
                      "unit": "GB/s", "frac": (achieved / peak) if achieved else None,
                      "traffic": traffic, "avg_launch_ms": avg_ms, "launches": plaunch,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "hipmemcpy_d2d_GBs": copy_gbs,
+                     "frac_of_hipmemcpy_d2d": (achieved / copy_gbs) if (achieved and copy_gbs) else None},
     }
     if fp32 is not None:
         out["fp32"] = fp32

@@ -158,12 +158,20 @@ __global__ void k_img_scatter(int nscan, const int *flag, const int *pos, int di
 // CommBrick::forward_comm, src/comm_brick.cpp:190-224: every gathered entry
 // is its owner's position plus one periodic shift per coordinate (the other
 // swaps add 0.0, which is exact), so one pass reproduces the swap sequence.
-__global__ void k_forward(int ng, const double4 *x, GatheredView g) {
+// Local copies (label 1) carry a zero shift, so only ghosts read theirs; the
+// type in .w is the owner's (as the border swaps set it).
+__global__ void k_forward(int ng, const double4 *x, const int *type, GatheredView g) {
   GRID_STRIDE(i, ng) {
-    const double4 p = x[g.owner[i]];
-    const double4 s = g.shift[i];
-    const double w = g.x[i].w;
-    g.x[i] = make_double4(p.x + s.x, p.y + s.y, p.z + s.z, w);
+    const int o = g.owner[i];
+    const double4 p = x[o];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    if (g.label[i] != 1) {
+      const double4 s = g.shift[i];
+      sx = s.x;
+      sy = s.y;
+      sz = s.z;
+    }
+    g.x[i] = make_double4(p.x + sx, p.y + sy, p.z + sz, (double)type[o]);
   }
 }
 
@@ -970,9 +978,9 @@ void launch_img_scatter(int nscan, const int *flag, const int *pos, int dim, dou
   hipLaunchKernelGGL(k_img_scatter, dim3(nblk(nscan)), dim3(TPB), 0, s, nscan, flag, pos, dim, shift, first, send, g);
   HIPCHK(hipGetLastError());
 }
-void launch_forward(int ng, const double4 *x, GatheredView g, hipStream_t s) {
+void launch_forward(int ng, const double4 *x, const int *type, GatheredView g, hipStream_t s) {
   if (ng <= 0) return;
-  hipLaunchKernelGGL(k_forward, dim3(nblk(ng)), dim3(TPB), 0, s, ng, x, g);
+  hipLaunchKernelGGL(k_forward, dim3(nblk(ng)), dim3(TPB), 0, s, ng, x, type, g);
   HIPCHK(hipGetLastError());
 }
 void launch_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF, hipStream_t s) {

@@ -57,7 +57,7 @@ void launch_copy_scatter(int n, const double4 *x, const int *type, const int *ca
 void launch_img_flag(int nscan, const double4 *gx, int dim, double lo, double hi, int *flag, hipStream_t s);
 void launch_img_scatter(int nscan, const int *flag, const int *pos, int dim, double shift, int first,
                         int *send, GatheredView g, hipStream_t s);
-void launch_forward(int ng, const double4 *x, GatheredView g, hipStream_t s);
+void launch_forward(int ng, const double4 *x, const int *type, GatheredView g, hipStream_t s);
 void launch_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF, hipStream_t s);
 void launch_reverse_init(int cnt, const int *send, int first, double4 *target, const double4 *init,
                          const double4 *gF, hipStream_t s);

@@ -324,7 +324,7 @@ void borders(phafd_hip_ctx *c) {
   c->nghost = c->ngathered - c->nlocal;
 }
 
-void forward_comm(phafd_hip_ctx *c) { launch_forward(c->ngathered, c->x.p, c->gview(), c->stream); }
+void forward_comm(phafd_hip_ctx *c) { launch_forward(c->ngathered, c->x.p, c->type.p, c->gview(), c->stream); }
 
 // reverse_comm with the owned forces starting from init (instead of their
 // current values): when the copy swap sends every owned atom once, the copy
