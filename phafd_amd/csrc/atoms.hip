@@ -886,11 +886,9 @@ __global__ void k_chain_step(const int *chs, const int *che, const double4 *F, d
       const int img = image[j];
       const int xb = (img & IMGMASK) - IMGMAX, yb = (img >> IMGBITS & IMGMASK) - IMGMAX,
                 zb = (img >> IMG2BITS) - IMGMAX;
-      double4 p = x[j];
-      p.x = u.x - xb * bx.period[0];
-      p.y = u.y - yb * bx.period[1];
-      p.z = u.z - zb * bx.period[2];
-      x[j] = p;
+      // .w of x and ux is 0 for every owned atom (set at upload, kept by
+      // every kernel that moves atoms), so x is written without reading it
+      x[j] = make_double4(u.x - xb * bx.period[0], u.y - yb * bx.period[1], u.z - zb * bx.period[2], u.w);
     }
     if (!last) {
       __syncthreads();

@@ -160,6 +160,7 @@ struct phafd_hip_ctx {
   DBuf<unsigned long long> sg_acc;   // fixed-point density accumulator
   DBuf<uint16_t> sg_x0;
   DBuf<double> sg_phi, sg_mu0, sg_mu1, sg_base, sg_red, sg_sum;
+  DBuf<double2> sg_ph0, sg_mh0;   // spectra of phi and mu at the block start
   DBuf<double4> sg_fga, sg_x, sg_ux;
   DBuf<int> sg_img;
 
@@ -2254,6 +2255,7 @@ SurrogateCfg surrogate_cfg(phafd_hip_ctx *c) {
   const size_t N = c->N;
   c->sg_rho.alloc(N); c->sg_rho1.alloc(N); c->sg_out.alloc(N); c->sg_x0.alloc(8 * N); c->sg_acc.alloc(N);
   c->sg_phi.alloc(N); c->sg_mu0.alloc(N); c->sg_mu1.alloc(N); c->sg_base.alloc(N);
+  c->sg_ph0.alloc(c->M); c->sg_mh0.alloc(c->M);
   c->sg_red.alloc(2 * SG_NB + 2);
   c->sg_sum.alloc(4);
   const size_t na = (size_t)std::max(c->nowned, 1);
@@ -2291,24 +2293,29 @@ void block_sums(phafd_hip_ctx *c, const double *phi0, const float *rho0) {
 }
 
 // base = phi0 after K steps by exponential Euler with the block-start
-// density rho0.  Leaves phihat = spectrum of phi0, muhat = spectrum of
-// mu(phi0, rho0) and sg_sum set; uses sg_mu0 and ghat[0].
+// density rho0.  Leaves sg_ph0 = spectrum of phi0, sg_mh0 = spectrum of
+// mu0 = mu(phi0, rho0) (in sg_mu0) and sg_sum set; uses ghat[0].  carried:
+// sg_ph0, sg_mh0 and sg_mu0 already hold those of this phi0 and rho0 (the
+// previous block's check computed them for its accepted phi1 and rho1).
 void surrogate_base(phafd_hip_ctx *c, const SurrogateCfg &g, const double *phi0, const float *rho0, double *base,
-                    int K) {
+                    int K, bool carried = false) {
   FieldSet<double> &F = c->fd;
   const double tv = g.fh->temp / g.fh->volFH;
   const UpdateParams u = update_params(c, *g.cj);
   block_sums(c, phi0, rho0);
-  surrogate_mu(c->N, phi0, rho0, tv, g.fh->chi, g.phit, c->sg_mu0.p, c->stream);
-  fft_r2c(c->plan, phi0, F.phihat.p, c->stream);
-  fft_r2c(c->plan, c->sg_mu0.p, F.muhat.p, c->stream);
-  etd_block(c->plan, F.phihat.p, F.muhat.p, F.ghat[0].p, u, tv, g.fh->chi, c->sg_sum.p, K, c->stream);
+  if (!carried) {
+    surrogate_mu(c->N, phi0, rho0, tv, g.fh->chi, g.phit, c->sg_mu0.p, c->stream);
+    fft_r2c(c->plan, phi0, c->sg_ph0.p, c->stream);
+    fft_r2c(c->plan, c->sg_mu0.p, c->sg_mh0.p, c->stream);
+  }
+  etd_block(c->plan, c->sg_ph0.p, c->sg_mh0.p, F.ghat[0].p, u, tv, g.fh->chi, c->sg_sum.p, K, c->stream);
   fft_c2r(c->plan, F.ghat[0].p, base, (double)c->N, c->stream);
 }
 
 // {residual, relative mass drift} of phi1 as phi0 advanced by K steps with
-// densities rho0, rho1; needs phihat, muhat and sg_sum from surrogate_base
-// on the same phi0, rho0; uses sg_mu1, ghat[0..1]
+// densities rho0, rho1; needs sg_ph0, sg_mh0 and sg_sum from surrogate_base
+// on the same phi0, rho0; leaves mu1 in sg_mu1 and the spectra of phi1 and
+// mu1 in ghat[0], ghat[1]
 void block_check(phafd_hip_ctx *c, const SurrogateCfg &g, const double *phi0, const double *phi1,
                  const float *rho1, int K, double out[2]) {
   FieldSet<double> &F = c->fd;
@@ -2316,7 +2323,7 @@ void block_check(phafd_hip_ctx *c, const SurrogateCfg &g, const double *phi0, co
   surrogate_mu(c->N, phi1, rho1, tv, g.fh->chi, g.phit, c->sg_mu1.p, c->stream);
   fft_r2c(c->plan, phi1, F.ghat[0].p, c->stream);
   fft_r2c(c->plan, c->sg_mu1.p, F.ghat[1].p, c->stream);
-  block_residual_partials(c->plan, F.phihat.p, F.ghat[0].p, F.muhat.p, F.ghat[1].p, update_params(c, *g.cj), tv,
+  block_residual_partials(c->plan, c->sg_ph0.p, F.ghat[0].p, c->sg_mh0.p, F.ghat[1].p, update_params(c, *g.cj), tv,
                           g.fh->chi, c->sg_sum.p, K, c->sg_red.p, SG_NB, c->stream);
   double *ms = c->sg_red.p + 2 * SG_NB;
   fill_zero(ms, 2 * sizeof(double), c->stream);
@@ -2398,10 +2405,16 @@ void surrogate_steps(phafd_hip_ctx *c, UNet &net, int nsteps, int K, double tol,
   double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const auto t_all = std::chrono::steady_clock::now();
   const size_t N = c->N;
+  // After an accepted block the next one starts from exactly the checked
+  // phi1 and, with no list build in between, the same bead density: the
+  // check's mu1 and spectra are that block's mu0 and spectra.
+  bool carry = false;
+  int64_t carry_calls = -1;
   for (int blk = 0; blk < nsteps / K; blk++) {
     auto t0 = std::chrono::steady_clock::now();
     refresh_gathered(c);
-    deposit_density(c, g, c->sg_rho.p);
+    if (carry && c->ncalls != carry_calls) carry = false;
+    if (!carry) deposit_density(c, g, c->sg_rho.p);
     // the atoms' block-start state, for a fallback
     const int64_t calls0 = c->ncalls, step0 = c->timestep;
     const int pending0 = c->pending_decision;
@@ -2413,8 +2426,9 @@ void surrogate_steps(phafd_hip_ctx *c, UNet &net, int nsteps, int K, double tol,
     t0 = std::chrono::steady_clock::now();
     refresh_gathered(c);
     deposit_density(c, g, c->sg_rho1.p);
+    const int64_t calls1 = c->ncalls;
     const double *phi0 = c->fd.phi.p;
-    surrogate_base(c, g, phi0, c->sg_rho.p, c->sg_base.p, K);
+    surrogate_base(c, g, phi0, c->sg_rho.p, c->sg_base.p, K, carry);
     unet_features(N, phi0, c->sg_rho.p, c->sg_rho1.p, c->sg_sum.p, net.norm, c->sg_x0.p, c->stream);
     net.forward(c->sg_x0.p, c->sg_out.p, c->stream);
     fill_zero(c->sg_sum.p + 2, sizeof(double), c->stream);
@@ -2436,7 +2450,17 @@ void surrogate_steps(phafd_hip_ctx *c, UNet &net, int nsteps, int K, double tol,
       std::memcpy(&p0, c->h_flags + 2, sizeof(double));
       if (std::isnan(p0)) throw std::runtime_error("NAN encountered in phi.");
       st[1] += 1;
+      FieldSet<double> &F = c->fd;
+      carry = F.ghat[0].own && F.ghat[1].own;
+      if (carry) {
+        std::swap(c->sg_ph0, F.ghat[0]);
+        std::swap(c->sg_mh0, F.ghat[1]);
+        std::swap(c->sg_mu0, c->sg_mu1);
+        std::swap(c->sg_rho, c->sg_rho1);
+        carry_calls = calls1;
+      }
     } else {
+      carry = false;
       // put the atoms back; lists rebuilt in between are rebuilt again at
       // the restored positions
       launch_copy4(c->nowned, c->sg_x.p, c->x.p, c->stream);
