@@ -185,9 +185,14 @@ int phafd_hip_timers(phafd_hip_ctx *ctx, double *ms8, int reset);
  *    full neighbour cut at every build and walk it for lj/cut.  By default the
  *    step walks a pruned copy (pairs closer than lj cut + skin at the build, in
  *    the same relative order: bit-identical forces) and the full list is
- *    materialised from the last build when phafd_hip_halflist() asks. */
+ *    materialised from the last build when phafd_hip_halflist() asks.
+ *  PHAFD_OPT_ATOM_WINDOWS [1]: phafd_hip_run_surrogate runs the atom steps
+ *    of a block in speculative windows (one host read-back per window; the
+ *    steps after a rebuild decision are skipped on the device and redone
+ *    after the rebuild): the same steps, lists and sums as 0, which reads
+ *    the decision back after every step. */
 enum { PHAFD_OPT_EAGER_GRIDLIST = 0, PHAFD_OPT_FUSE_FH = 1, PHAFD_OPT_FUSE_SPECTRAL = 2,
-       PHAFD_OPT_EAGER_HALFLIST = 3 };
+       PHAFD_OPT_EAGER_HALFLIST = 3, PHAFD_OPT_ATOM_WINDOWS = 4 };
 int phafd_hip_set_option(phafd_hip_ctx *ctx, int key, double value);
 /* HIP-event timing of every launch of one kernel during run():
  * kernel_id >= 0 (re)starts the probe for that kernel (ids in the order

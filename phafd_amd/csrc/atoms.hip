@@ -22,6 +22,14 @@ int nblk(size_t n) {
 #define GRID_STRIDE(i, n) \
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)(n); i += (size_t)gridDim.x * blockDim.x)
 
+// a kernel of a speculative window's step that follows the step whose
+// rebuild decision fired (StepGate)
+__device__ __forceinline__ bool gate_off(const StepGate &g) {
+  if (!g.stop) return false;
+  const int t = *g.stop;
+  return t > 0 && g.step > t;
+}
+
 // ------------------------------------------------------------ domain
 // Domain::pbc + unmap, src/domain.cpp:102-178
 __global__ void k_pbc(int n, double4 *x, double4 *ux, int *image, Box b) {
@@ -80,7 +88,11 @@ __global__ void k_unmap(int n, const double4 *x, double4 *ux, const int *image, 
 
 // Neighbor::check_distance, src/neighbor.cpp:79-97
 // clear != nullptr: block 0 also zeroes *clear (the flag of the step after)
-__global__ void k_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag, int *clear) {
+// gate.stop non-null: a speculative window's step; a decision records its
+// step in *gate.stop (the first one wins)
+__global__ void k_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag, int *clear,
+                         StepGate gate) {
+  if (gate_off(gate)) return;
   if (clear && blockIdx.x == 0 && threadIdx.x == 0) *clear = 0;
   int f = 0;
   GRID_STRIDE(i, n) {
@@ -90,7 +102,10 @@ __global__ void k_decide(int n, const double4 *x, const double4 *xh, double trig
     if (rsq > triggersq) f |= 1;
     if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z)) f |= 2;   // reported as an error
   }
-  if (f) atomicOr(flag, f);
+  if (f) {
+    atomicOr(flag, f);
+    if (gate.stop) atomicCAS(gate.stop, 0, gate.step);
+  }
 }
 
 __global__ void k_copy4(int n, const double4 *a, double4 *b) {
@@ -160,7 +175,8 @@ __global__ void k_img_scatter(int nscan, const int *flag, const int *pos, int di
 // swaps add 0.0, which is exact), so one pass reproduces the swap sequence.
 // Local copies (label 1) carry a zero shift, so only ghosts read theirs; the
 // type in .w is the owner's (as the border swaps set it).
-__global__ void k_forward(int ng, const double4 *x, const int *type, GatheredView g) {
+__global__ void k_forward(int ng, const double4 *x, const int *type, GatheredView g, StepGate gate) {
+  if (gate_off(gate)) return;
   GRID_STRIDE(i, ng) {
     const int o = g.owner[i];
     const double4 p = x[o];
@@ -177,7 +193,8 @@ __global__ void k_forward(int ng, const double4 *x, const int *type, GatheredVie
 
 // CommBrick::reverse_comm for one swap, src/comm_brick.cpp:228-258 +
 // Atom::unpack_reverse, src/atom.cpp:255-269 (targets within a swap are distinct)
-__global__ void k_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF) {
+__global__ void k_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF, StepGate gate) {
+  if (gate_off(gate)) return;
   GRID_STRIDE(q, cnt) {
     const int j = send[q];
     const double4 b = gF[first + q];
@@ -192,7 +209,8 @@ __global__ void k_reverse(int cnt, const int *send, int first, double4 *target, 
 // target[send[q]] = init[send[q]] + gF[first + q] (init: the owned atoms'
 // force before the fold), the copy and the fold in one pass
 __global__ void k_reverse_init(int cnt, const int *send, int first, double4 *target, const double4 *init,
-                               const double4 *gF) {
+                               const double4 *gF, StepGate gate) {
+  if (gate_off(gate)) return;
   GRID_STRIDE(q, cnt) {
     const int j = send[q];
     const double4 b = gF[first + q];
@@ -207,7 +225,9 @@ __global__ void k_reverse_init(int cnt, const int *send, int first, double4 *tar
 // the two image swaps of one dimension when no atom is in both send lists:
 // items [0, ca) fold swap a, [ca, ca + cb) swap b; every target is written
 // by one item, so the sums are those of two sequential launches
-__global__ void k_reverse2(int ca, const int *sa, int fa, int cb, const int *sb, int fb, double4 *gF) {
+__global__ void k_reverse2(int ca, const int *sa, int fa, int cb, const int *sb, int fb, double4 *gF,
+                           StepGate gate) {
+  if (gate_off(gate)) return;
   GRID_STRIDE(q, ca + cb) {
     const bool ina = q < ca;
     const int r = ina ? q : q - ca;
@@ -770,7 +790,9 @@ __device__ __forceinline__ bool lj_pair(const double4 &xi, const double4 &xj, co
 }
 __global__ void k_lj_force(int ng, LJParams lj, const double4 *gx, const int *rowof,
                            const int *hoff, const int *hneigh, const int *toff, const int *tpair,
-                           const int *ti, int nowned, double4 *gF, double *energy, int zero_init) {
+                           const int *ti, int nowned, double4 *gF, double *energy, int zero_init,
+                           StepGate gate) {
+  if (gate_off(gate)) return;
   double esum = 0.0;
   GRID_STRIDE(a, ng) {
     const double4 xa = gx[a];
@@ -860,7 +882,9 @@ __device__ __forceinline__ void chain_bead_force(const double4 *u, int b, int nb
 // both; the first half's positions stay in LDS, in the second buffer at
 // us + stride, and only the second half is written out)
 __global__ void k_chain_step(const int *chs, const int *che, const double4 *F, double4 *ux, double4 *x,
-                             const int *image, ChainParams cp, double dt, Box bx, int halves, int stride) {
+                             const int *image, ChainParams cp, double dt, Box bx, int halves, int stride,
+                             StepGate gate) {
+  if (gate_off(gate)) return;   // uniform over the block: *gate.stop is fixed during this launch
   extern __shared__ double4 us[];
   const int s = chs[blockIdx.x], nb = che[blockIdx.x] - s;
   double4 *cur = us, *nxt = us + stride;
@@ -940,12 +964,12 @@ void launch_unmap(int n, const double4 *x, double4 *ux, const int *image, const 
   HIPCHK(hipGetLastError());
 }
 void launch_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag, int *clear,
-                   hipStream_t s) {
+                   hipStream_t s, StepGate gate) {
   if (n <= 0) {
     if (clear) launch_fill_i32(1, clear, 0, s);
     return;
   }
-  hipLaunchKernelGGL(k_decide, dim3(nblk(n)), dim3(TPB), 0, s, n, x, xh, triggersq, flag, clear);
+  hipLaunchKernelGGL(k_decide, dim3(nblk(n)), dim3(TPB), 0, s, n, x, xh, triggersq, flag, clear, gate);
   HIPCHK(hipGetLastError());
 }
 void launch_copy4(int n, const double4 *a, double4 *b, hipStream_t s) {
@@ -976,25 +1000,27 @@ void launch_img_scatter(int nscan, const int *flag, const int *pos, int dim, dou
   hipLaunchKernelGGL(k_img_scatter, dim3(nblk(nscan)), dim3(TPB), 0, s, nscan, flag, pos, dim, shift, first, send, g);
   HIPCHK(hipGetLastError());
 }
-void launch_forward(int ng, const double4 *x, const int *type, GatheredView g, hipStream_t s) {
+void launch_forward(int ng, const double4 *x, const int *type, GatheredView g, hipStream_t s, StepGate gate) {
   if (ng <= 0) return;
-  hipLaunchKernelGGL(k_forward, dim3(nblk(ng)), dim3(TPB), 0, s, ng, x, type, g);
+  hipLaunchKernelGGL(k_forward, dim3(nblk(ng)), dim3(TPB), 0, s, ng, x, type, g, gate);
   HIPCHK(hipGetLastError());
 }
-void launch_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF, hipStream_t s) {
+void launch_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF, hipStream_t s,
+                    StepGate gate) {
   if (cnt <= 0) return;
-  hipLaunchKernelGGL(k_reverse, dim3(nblk(cnt)), dim3(TPB), 0, s, cnt, send, first, target, gF);
+  hipLaunchKernelGGL(k_reverse, dim3(nblk(cnt)), dim3(TPB), 0, s, cnt, send, first, target, gF, gate);
   HIPCHK(hipGetLastError());
 }
 void launch_reverse_init(int cnt, const int *send, int first, double4 *target, const double4 *init,
-                         const double4 *gF, hipStream_t s) {
+                         const double4 *gF, hipStream_t s, StepGate gate) {
   if (cnt <= 0) return;
-  hipLaunchKernelGGL(k_reverse_init, dim3(nblk(cnt)), dim3(TPB), 0, s, cnt, send, first, target, init, gF);
+  hipLaunchKernelGGL(k_reverse_init, dim3(nblk(cnt)), dim3(TPB), 0, s, cnt, send, first, target, init, gF, gate);
   HIPCHK(hipGetLastError());
 }
-void launch_reverse2(int ca, const int *sa, int fa, int cb, const int *sb, int fb, double4 *gF, hipStream_t s) {
+void launch_reverse2(int ca, const int *sa, int fa, int cb, const int *sb, int fb, double4 *gF, hipStream_t s,
+                     StepGate gate) {
   if (ca + cb <= 0) return;
-  hipLaunchKernelGGL(k_reverse2, dim3(nblk(ca + cb)), dim3(TPB), 0, s, ca, sa, fa, cb, sb, fb, gF);
+  hipLaunchKernelGGL(k_reverse2, dim3(nblk(ca + cb)), dim3(TPB), 0, s, ca, sa, fa, cb, sb, fb, gF, gate);
   HIPCHK(hipGetLastError());
 }
 void launch_atom2bin(int ng, const double4 *gx, const BinGeom &b, int *a2b, int *bincount, int *bad,
@@ -1180,10 +1206,10 @@ PHAFD_GA_DEF(float, float4)
 #undef PHAFD_GA_DEF
 void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,
                      const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
-                     double4 *gF, double *energy, hipStream_t s, bool zero_init) {
+                     double4 *gF, double *energy, hipStream_t s, bool zero_init, StepGate gate) {
   if (ng <= 0) return;
   hipLaunchKernelGGL(k_lj_force, dim3(nblk(ng)), dim3(TPB), 0, s, ng, lj, gx, rowof, hoff, hneigh, toff,
-                     tpair, ti, nowned, gF, energy, zero_init ? 1 : 0);
+                     tpair, ti, nowned, gF, energy, zero_init ? 1 : 0, gate);
   HIPCHK(hipGetLastError());
 }
 void launch_drag(int n, const int *members, double4 *x, const double4 *F, double drag, double dt,
@@ -1194,12 +1220,12 @@ void launch_drag(int n, const int *members, double4 *x, const double4 *F, double
 }
 void launch_chain_step(int nchains, int maxlen, const int *chs, const int *che, const double4 *F, double4 *ux,
                        double4 *x, const int *image, const ChainParams &cp, double dt, const Box &b, hipStream_t s,
-                       int halves) {
+                       int halves, StepGate gate) {
   if (nchains <= 0 || maxlen <= 0) return;
   if (halves != 1 && halves != 2) throw std::invalid_argument("chain step: halves must be 1 or 2");
   const int threads = maxlen <= 64 ? 64 : (maxlen <= 128 ? 128 : 256);
   hipLaunchKernelGGL(k_chain_step, dim3(nchains), dim3(threads), sizeof(double4) * maxlen * halves, s, chs, che, F,
-                     ux, x, image, cp, dt, b, halves, maxlen);
+                     ux, x, image, cp, dt, b, halves, maxlen, gate);
   HIPCHK(hipGetLastError());
 }
 void launch_chain_energy(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,

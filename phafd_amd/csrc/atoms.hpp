@@ -34,6 +34,14 @@ struct GAParams {
   double em05, e05;   // exp(-0.5), exp(0.5) evaluated on the host
 };
 
+// Speculative windows of atom steps: stop (device) holds the 1-based step
+// of the window whose rebuild decision fired (0: none); the kernels of a
+// later step of the window return at once.  step 0 / stop null: not gated.
+struct StepGate {
+  int *stop = nullptr;
+  int step = 0;
+};
+
 struct LJParams {
   double lj1, lj2, cutsq;
 };
@@ -48,7 +56,7 @@ void launch_unmap(int n, const double4 *x, double4 *ux, const int *image, const 
 // since the build, |= 2 for a non-finite position; clear (may be null) is
 // zeroed by the same launch
 void launch_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag, int *clear,
-                   hipStream_t s);
+                   hipStream_t s, StepGate gate = StepGate());
 void launch_copy4(int n, const double4 *a, double4 *b, hipStream_t s);
 void launch_copy_classify(int n, const double4 *x, double zstart, double zend, double lo, double hi,
                           int *cat, int *flag, hipStream_t s);
@@ -57,11 +65,14 @@ void launch_copy_scatter(int n, const double4 *x, const int *type, const int *ca
 void launch_img_flag(int nscan, const double4 *gx, int dim, double lo, double hi, int *flag, hipStream_t s);
 void launch_img_scatter(int nscan, const int *flag, const int *pos, int dim, double shift, int first,
                         int *send, GatheredView g, hipStream_t s);
-void launch_forward(int ng, const double4 *x, const int *type, GatheredView g, hipStream_t s);
-void launch_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF, hipStream_t s);
+void launch_forward(int ng, const double4 *x, const int *type, GatheredView g, hipStream_t s,
+                    StepGate gate = StepGate());
+void launch_reverse(int cnt, const int *send, int first, double4 *target, const double4 *gF, hipStream_t s,
+                    StepGate gate = StepGate());
 void launch_reverse_init(int cnt, const int *send, int first, double4 *target, const double4 *init,
-                         const double4 *gF, hipStream_t s);
-void launch_reverse2(int ca, const int *sa, int fa, int cb, const int *sb, int fb, double4 *gF, hipStream_t s);
+                         const double4 *gF, hipStream_t s, StepGate gate = StepGate());
+void launch_reverse2(int ca, const int *sa, int fa, int cb, const int *sb, int fb, double4 *gF, hipStream_t s,
+                     StepGate gate = StepGate());
 void launch_atom2bin(int ng, const double4 *gx, const BinGeom &b, int *a2b, int *bincount, int *bad,
                      hipStream_t s);
 void launch_bin_place(int ng, const int *a2b, const int *binstart, int *cursor, int *binatoms, hipStream_t s);
@@ -117,7 +128,8 @@ PHAFD_GA_DECL(float, float4)
 // zero, the fill of gF left out when lj/cut is its only writer)
 void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,
                      const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
-                     double4 *gF, double *energy, hipStream_t s, bool zero_init = false);
+                     double4 *gF, double *energy, hipStream_t s, bool zero_init = false,
+                     StepGate gate = StepGate());
 void launch_drag(int n, const int *members, double4 *x, const double4 *F, double drag, double dt,
                  hipStream_t s);
 // halves (1 or 2) half steps of the chains (contiguous atom ranges
@@ -125,7 +137,7 @@ void launch_drag(int n, const int *members, double4 *x, const double4 *F, double
 // per block); 2: both halves of a timestep, with the same F
 void launch_chain_step(int nchains, int maxlen, const int *chs, const int *che, const double4 *F, double4 *ux,
                        double4 *x, const int *image, const ChainParams &cp, double dt, const Box &b, hipStream_t s,
-                       int halves = 1);
+                       int halves = 1, StepGate gate = StepGate());
 void launch_chain_energy(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,
                          const ChainParams &cp, double *energy, hipStream_t s);
 void launch_fill_i32(int n, int *p, int v, hipStream_t s);

@@ -92,7 +92,8 @@ struct phafd_hip_ctx {
   DBuf<unsigned long long> counters;   // [nless, ngreat]
   DBuf<int> flags;                     // [0] rebuild, [1] overflow
   DBuf<double> esum;                   // energy accumulators
-  int *h_flags = nullptr;              // pinned, 16 ints: [2..3] phi(0,0,0), [4] scalar reads, [8..15] flags
+  int *h_flags = nullptr;              // pinned, 16 ints: [2..3] phi(0,0,0), [4] scalar reads, [6..7] window
+                                       // stop step and flags, [8..15] flags
   // owned atoms
   int nowned = 0, ntypes = 0;
   bool atoms_set = false;
@@ -142,6 +143,7 @@ struct phafd_hip_ctx {
   bool lists_valid = false;
   int pending_decision = -1;   // rebuild decision prefetched at the end of the last step
   bool store_muhat = true;     // the fused z update writes muhat (false inside a run but its last step)
+  bool atom_windows = true;    // the surrogate's atom steps in speculative windows
   // styles
   std::vector<std::unique_ptr<PairDef>> pairs;
   std::vector<std::unique_ptr<FixDef>> fixes;
@@ -161,6 +163,7 @@ struct phafd_hip_ctx {
   DBuf<uint16_t> sg_x0;
   DBuf<double> sg_phi, sg_mu0, sg_mu1, sg_base, sg_red, sg_sum;
   DBuf<double2> sg_ph0, sg_mh0;   // spectra of phi and mu at the block start
+  DBuf<int> sg_win;               // speculative atom-step windows: [0] stop step, [1] OR of decide flags
   DBuf<double4> sg_fga, sg_x, sg_ux;
   DBuf<int> sg_img;
 
@@ -325,12 +328,14 @@ void borders(phafd_hip_ctx *c) {
   c->nghost = c->ngathered - c->nlocal;
 }
 
-void forward_comm(phafd_hip_ctx *c) { launch_forward(c->ngathered, c->x.p, c->type.p, c->gview(), c->stream); }
+void forward_comm(phafd_hip_ctx *c, StepGate gate = StepGate()) {
+  launch_forward(c->ngathered, c->x.p, c->type.p, c->gview(), c->stream, gate);
+}
 
 // reverse_comm with the owned forces starting from init (instead of their
 // current values): when the copy swap sends every owned atom once, the copy
 // and the last fold are one pass; the sums are those of copy + reverse_comm
-void reverse_comm_from(phafd_hip_ctx *c, const double4 *init);
+void reverse_comm_from(phafd_hip_ctx *c, const double4 *init, StepGate gate = StepGate());
 
 void reverse_comm(phafd_hip_ctx *c) {
   for (int iswap = (int)c->swaps.size() - 1; iswap >= 1; iswap--) {
@@ -347,9 +352,11 @@ void reverse_comm(phafd_hip_ctx *c) {
   launch_reverse(z.count, z.send.p, z.first, c->F.p, c->gF.p, c->stream);
 }
 
-void reverse_comm_from(phafd_hip_ctx *c, const double4 *init) {
+bool reverse_from_one_pass(const phafd_hip_ctx *c) { return c->swaps[0].count == c->nowned; }
+void reverse_comm_from(phafd_hip_ctx *c, const double4 *init, StepGate gate) {
   SwapDev &z = c->swaps[0];
-  if (z.count != c->nowned) {
+  if (!reverse_from_one_pass(c)) {
+    if (gate.stop) throw std::logic_error("gated fold needs the one-pass copy swap");
     launch_copy4(c->nowned, init, c->F.p, c->stream);
     reverse_comm(c);
     return;
@@ -358,13 +365,13 @@ void reverse_comm_from(phafd_hip_ctx *c, const double4 *init) {
     SwapDev &w = c->swaps[iswap];
     if (w.disjoint_pair) {
       SwapDev &v = c->swaps[iswap - 1];
-      launch_reverse2(w.count, w.send.p, w.first, v.count, v.send.p, v.first, c->gF.p, c->stream);
+      launch_reverse2(w.count, w.send.p, w.first, v.count, v.send.p, v.first, c->gF.p, c->stream, gate);
       iswap--;
       continue;
     }
-    launch_reverse(w.count, w.send.p, w.first, c->gF.p, c->gF.p, c->stream);
+    launch_reverse(w.count, w.send.p, w.first, c->gF.p, c->gF.p, c->stream, gate);
   }
-  launch_reverse_init(z.count, z.send.p, z.first, c->F.p, init, c->gF.p, c->stream);
+  launch_reverse_init(z.count, z.send.p, z.first, c->F.p, init, c->gF.p, c->stream, gate);
 }
 
 // ------------------------------------------------------------ neighbour
@@ -853,10 +860,10 @@ FixDef *fused_chain_fix(phafd_hip_ctx *c) {
   }
   return (n == 1 && ch && ch->maxlen <= 1024) ? ch : nullptr;
 }
-void integrate_both_halves(phafd_hip_ctx *c, FixDef &f) {
+void integrate_both_halves(phafd_hip_ctx *c, FixDef &f, StepGate gate = StepGate()) {
   ChainParams cp{f.r0, f.kbond, f.kbend, f.drag};
   launch_chain_step(f.nchain, f.maxlen, f.chs.p, f.che.p, c->F.p, c->ux.p, c->x.p, c->image.p, cp, c->dt, c->box(),
-                    c->stream, 2);
+                    c->stream, 2, gate);
 }
 
 template <class R>
@@ -1692,6 +1699,7 @@ int phafd_hip_set_option(phafd_hip_ctx *c, int key, double value) {
       case PHAFD_OPT_FUSE_FH: c->fuse_fh = value != 0; break;
       case PHAFD_OPT_FUSE_SPECTRAL: c->fuse_spectral = value != 0; break;
       case PHAFD_OPT_EAGER_HALFLIST: c->eager_halflist = value != 0; break;
+      case PHAFD_OPT_ATOM_WINDOWS: c->atom_windows = value != 0; break;
       default: throw std::invalid_argument("unknown option");
     }
     return 0;
@@ -2255,7 +2263,7 @@ SurrogateCfg surrogate_cfg(phafd_hip_ctx *c) {
   const size_t N = c->N;
   c->sg_rho.alloc(N); c->sg_rho1.alloc(N); c->sg_out.alloc(N); c->sg_x0.alloc(8 * N); c->sg_acc.alloc(N);
   c->sg_phi.alloc(N); c->sg_mu0.alloc(N); c->sg_mu1.alloc(N); c->sg_base.alloc(N);
-  c->sg_ph0.alloc(c->M); c->sg_mh0.alloc(c->M);
+  c->sg_ph0.alloc(c->M); c->sg_mh0.alloc(c->M); c->sg_win.alloc(2);
   c->sg_red.alloc(2 * SG_NB + 2);
   c->sg_sum.alloc(4);
   const size_t na = (size_t)std::max(c->nowned, 1);
@@ -2342,49 +2350,99 @@ void block_check(phafd_hip_ctx *c, const SurrogateCfg &g, const double *phi0, co
 }
 
 // K atom steps (neighbour, lj/cut, chains every step) with the grid-atom
-// force of the current field computed once and held
+// force of the current field computed once and held.
+//
+// Where every kernel of a step can be gated (the chains are the only
+// integrator, the copy swap folds in one pass, lj/cut is computed), steps
+// run in speculative windows of W: the first step takes the host's rebuild
+// decision, the later ones run as if no rebuild came, and each step's
+// decide records its step in a device word when a rebuild is due for the
+// next; the kernels of the steps after it then return at once.  One
+// read-back per window then resumes at the step after the decision, with the
+// rebuild: the same steps, lists and sums as deciding after every step.
 void atom_steps_frozen_field(phafd_hip_ctx *c, const SurrogateCfg &g, int K) {
   phase_start_t<double>(c);   // grad phi of the block-start field
   if (c->neigh_set) clear_decide_slot(c, c->timestep + 1);
-  for (int s = 0; s < K; s++) {
-    c->timestep++;
-    phase_neigh(c);
-    const bool lists = c->neigh_set && c->lists_valid;
-    if (s == 0) {
-      launch_fill_d4(c->ngathered, c->gF.p, c->stream);
-      launch_fill_d4(c->nowned, c->F.p, c->stream);
-      if (g.ga && lists) {
-        grid_atom_pass<double>(c, *g.ga, nullptr, true);
-        reverse_comm(c);
+  FixDef *fc = fused_chain_fix(c);
+  bool any_lj = false;
+  for (auto &pp : c->pairs) any_lj = any_lj || pp->style == 2;
+  bool slot_dirty = false;   // a window ran since the last single step
+  int since_build = 0, interval = 8;
+  int s = 0;
+  while (s < K) {
+    const bool can_window = c->atom_windows && s > 0 && c->neigh_set && c->lists_valid && fc && any_lj && reverse_from_one_pass(c) &&
+                            c->pending_decision == 0;
+    const int W = can_window ? std::min(K - s, std::max(2, std::min(16, interval - since_build))) : 1;
+    if (W > 1) launch_fill_i32(2, c->sg_win.p, 0, c->stream);
+    else if (slot_dirty && c->neigh_set) {
+      clear_decide_slot(c, c->timestep + 1);
+      slot_dirty = false;
+    }
+    const int64_t calls0 = c->ncalls;
+    for (int j = 1; j <= W; j++) {
+      const StepGate gate = W > 1 ? StepGate{c->sg_win.p, j} : StepGate{};
+      c->timestep++;
+      if (j == 1) phase_neigh(c);
+      else forward_comm(c, gate);
+      const bool lists = c->neigh_set && c->lists_valid;
+      if (s == 0) {
+        launch_fill_d4(c->ngathered, c->gF.p, c->stream);
+        launch_fill_d4(c->nowned, c->F.p, c->stream);
+        if (g.ga && lists) {
+          grid_atom_pass<double>(c, *g.ga, nullptr, true);
+          reverse_comm(c);
+        }
+        launch_copy4(c->nowned, c->F.p, c->sg_fga.p, c->stream);
       }
-      launch_copy4(c->nowned, c->F.p, c->sg_fga.p, c->stream);
-    }
-    // F starts as the held grid-atom force; lj/cut, the only pair force of
-    // these steps, writes gF from zero and the fold adds it: the same sums
-    // as zeroing both, folding and adding the held force after
-    bool any = false;
-    if (lists) {
-      for (auto &pp : c->pairs) {
-        if (pp->style != 2) continue;
-        LJParams lj{pp->lj1, pp->lj2, pp->ljcutsq};
-        launch_lj_force(c->ngathered, lj, c->gx.p, c->h_rowof.p, c->h_off.p, c->h_neigh.p, c->t_off.p,
-                        c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, nullptr, c->stream, !any);
-        any = true;
+      // F starts as the held grid-atom force; lj/cut, the only pair force of
+      // these steps, writes gF from zero and the fold adds it: the same sums
+      // as zeroing both, folding and adding the held force after
+      bool any = false;
+      if (lists) {
+        for (auto &pp : c->pairs) {
+          if (pp->style != 2) continue;
+          LJParams lj{pp->lj1, pp->lj2, pp->ljcutsq};
+          launch_lj_force(c->ngathered, lj, c->gx.p, c->h_rowof.p, c->h_off.p, c->h_neigh.p, c->t_off.p,
+                          c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, nullptr, c->stream, !any, gate);
+          any = true;
+        }
       }
+      if (any) reverse_comm_from(c, c->sg_fga.p, gate);
+      else launch_copy4(c->nowned, c->sg_fga.p, c->F.p, c->stream);
+      if (fc) {
+        integrate_both_halves(c, *fc, gate);
+      } else {
+        phase_initial(c);
+        for (auto &f : c->fixes)
+          if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f);
+      }
+      // next step's rebuild decision (Neighbor::decide)
+      if (W > 1) launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->sg_win.p + 1, nullptr, c->stream, gate);
+      else if (c->neigh_set) enqueue_decide(c);
     }
-    if (any) reverse_comm_from(c, c->sg_fga.p);
-    else launch_copy4(c->nowned, c->sg_fga.p, c->F.p, c->stream);
-    if (FixDef *fc = fused_chain_fix(c)) {
-      integrate_both_halves(c, *fc);
-    } else {
-      phase_initial(c);
-      for (auto &f : c->fixes)
-        if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f);
-    }
-    if (c->neigh_set) enqueue_decide(c);   // next step's rebuild decision (Neighbor::decide)
     enqueue_flags_readback(c);
+    if (W > 1)
+      HIPCHK(hipMemcpyAsync(c->h_flags + 6, c->sg_win.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     sync(c);
-    check_step_flags(c);
+    if (c->ncalls != calls0) {   // rebuilt at this window's first step
+      interval = std::max(2, since_build);
+      since_build = 0;
+    }
+    if (W == 1) {
+      check_step_flags(c);
+      since_build += 1;
+      s += 1;
+      continue;
+    }
+    if (c->h_flags[11]) throw std::runtime_error("grid-atom term capacity exceeded");
+    const int stop = c->h_flags[6], f = c->h_flags[7];
+    if (f & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
+    const int done = stop > 0 ? stop : W;
+    c->timestep -= W - done;
+    c->pending_decision = stop > 0 ? 1 : 0;
+    since_build += done;
+    s += done;
+    slot_dirty = true;
   }
 }
 

@@ -300,6 +300,27 @@ def test_surrogate_runs_are_bit_reproducible():
     assert np.array_equal(out[0][1], out[1][1])
 
 
+def test_atom_step_windows_match_single_steps():
+    # speculative windows of atom steps (one read-back per window, steps
+    # after a rebuild decision skipped and redone) against one read-back per
+    # step: the same trajectory bit for bit, over blocks with rebuilds
+    z = np.load(WEIGHTS)
+    K, tol = int(z["K"][0]), float(z["tol"][0])
+    out = []
+    for windows in (1, 0):
+        s = S.make_system(P, 64, S.FAMILIES["production"], seed=502)
+        s.set_option(P.OPT_ATOM_WINDOWS, windows)
+        net = P.UNet(seed=1)
+        net.load(WEIGHTS)
+        s.run_surrogate(net, 2 * K, K, tol)
+        out.append((s.get_field(P.PHI), s.get_atoms()[0], s.counts()["ncalls"]))
+        net.close()
+        s.close()
+    assert out[0][2] == out[1][2] and out[0][2] > 2   # the same rebuilds, several of them
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_trained_network_gains_over_the_base_alone():
     z = np.load(WEIGHTS)
     K, tol = int(z["K"][0]), float(z["tol"][0])
