@@ -345,8 +345,29 @@ void fill_zero(void *p, size_t bytes, hipStream_t s) {
 // c nhat with G = 1 - c (f2 + s); over K steps
 //   phi_K = G^K phi_0 - c sum_{k<K} G^(K-1-k) n_k.
 namespace {
+// A run of m consecutive powers G^j (j = 0 .. m-1): S = sum G^j, T = sum
+// j G^j, P = G^m.  Two runs concatenate (the second shifted by the first's
+// m) as S = S1 + P1 S2, T = T1 + P1 (T2 + m1 S2), P = P1 P2, so the K-term
+// sums of a block come from binary doubling in ~2 log2 K steps, with no
+// cancellation near G = 1 (unlike (1 - G^K) / (1 - G)).
+struct GeoRun {
+  double S, T, P, m;
+};
+__device__ __forceinline__ GeoRun geo_cat(const GeoRun &a, const GeoRun &b) {
+  return GeoRun{a.S + a.P * b.S, a.T + a.P * (b.T + a.m * b.S), a.P * b.P, a.m + b.m};
+}
+__device__ __forceinline__ GeoRun geo_run(double G, int K) {
+  GeoRun r{0.0, 0.0, 1.0, 0.0}, b{1.0, 0.0, G, 1.0};
+  while (K) {
+    if (K & 1) r = geo_cat(r, b);
+    K >>= 1;
+    if (K) b = geo_cat(b, b);
+  }
+  return r;
+}
+
 struct ModeK {
-  double c, G, GK;
+  double c, G, GK, S, T;   // S = sum_{j<K} G^j, T = sum_{j<K} j G^j
 };
 __device__ __forceinline__ ModeK mode_k(size_t w, int ny, int nz, const UpdateParams &u, double f2, int K) {
   const int i = (int)(w % ny);                    // ky
@@ -359,8 +380,10 @@ __device__ __forceinline__ ModeK mode_k(size_t w, int ny, int nz, const UpdatePa
   ModeK r;
   r.c = u.mobility * q2 * u.dt;
   r.G = 1.0 - r.c * (f2 + u.temp / u.volFH * u.gamma * q2);
-  r.GK = 1.0;
-  for (int n = 0; n < K; n++) r.GK *= r.G;
+  const GeoRun g = geo_run(r.G, K);
+  r.GK = g.P;
+  r.S = g.S;
+  r.T = g.T;
   return r;
 }
 __device__ __forceinline__ double f2_of(const double *sums, double N, double tv, double chi) {
@@ -387,11 +410,7 @@ __global__ void k_etd(const double2 *__restrict__ p0, const double2 *__restrict_
   const double f2 = f2_of(sums, N, tv, chi);
   for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < M; w += (size_t)gridDim.x * blockDim.x) {
     const ModeK q = mode_k(w, ny, nz, u, f2, K);
-    double S = 0.0, g = 1.0;
-    for (int n = 0; n < K; n++) {
-      S += g;
-      g *= q.G;
-    }
+    const double S = q.S;
     const double2 a = p0[w], ma = m0[w];
     double2 r;
     r.x = q.GK * a.x - q.c * S * (ma.x - f2 * a.x);
@@ -403,7 +422,8 @@ __global__ void k_etd(const double2 *__restrict__ p0, const double2 *__restrict_
 // per block: sums over the half spectrum (weights 1 on kx = 0, Nx/2, else 2)
 // of |R|^2 and |ph1 - ph0|^2 with R = ph1 - G^K ph0 + c (A n0 + B n1),
 // n = muhat - f2 phihat interpolated linearly across the block: A = sum_k
-// G^(K-1-k) (1 - k/K), B = sum_k G^(K-1-k) k/K
+// G^(K-1-k) (1 - k/K) = (T + S) / K, B = sum_k G^(K-1-k) k/K =
+// ((K - 1) S - T) / K (j = K-1-k; S, T the sums of ModeK)
 __global__ __launch_bounds__(256) void k_block_residual(const double2 *__restrict__ ph0,
                                                         const double2 *__restrict__ ph1,
                                                         const double2 *__restrict__ mu0,
@@ -416,12 +436,7 @@ __global__ __launch_bounds__(256) void k_block_residual(const double2 *__restric
   double sr = 0, sd = 0;
   for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < M; w += (size_t)gridDim.x * blockDim.x) {
     const ModeK q = mode_k(w, ny, nz, u, f2, K);
-    double A = 0.0, B = 0.0, g = 1.0;
-    for (int k = K - 1; k >= 0; k--) {   // g = G^(K-1-k)
-      A += g * (1.0 - (double)k / K);
-      B += g * ((double)k / K);
-      g *= q.G;
-    }
+    const double A = (q.T + q.S) / K, B = ((K - 1) * q.S - q.T) / K;
     const double2 a = ph0[w], b = ph1[w], m0 = mu0[w], m1 = mu1[w];
     const double n0x = m0.x - f2 * a.x, n0y = m0.y - f2 * a.y;
     const double n1x = m1.x - f2 * b.x, n1y = m1.y - f2 * b.y;
