@@ -115,8 +115,10 @@ __global__ void k_copy4(int n, const double4 *a, double4 *b) {
 // ------------------------------------------------------------- borders
 // copy swap classification, src/comm_brick.cpp:299-316 (one rank):
 // 1 = LOCAL copy, 2 = GHOST at z - Lz, 3 = GHOST at z + Lz, 0 = not sent
+// nloc (zeroed by the caller) counts the LOCAL (category 1) atoms
 __global__ void k_copy_classify(int n, const double4 *x, double zstart, double zend, double lo,
-                                double hi, int *cat, int *flag) {
+                                double hi, int *cat, int *flag, int *nloc) {
+  int local = 0;
   GRID_STRIDE(i, n) {
     const double z = x[i].z;
     int c = 0;
@@ -125,7 +127,11 @@ __global__ void k_copy_classify(int n, const double4 *x, double zstart, double z
     else if (z <= hi) c = 3;
     cat[i] = c;
     flag[i] = c != 0;
+    local += c == 1;
   }
+  // one add per wave (every lane of the block reaches this point)
+  for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
+  if ((threadIdx.x & 63) == 0 && local) atomicAdd(nloc, local);
 }
 __global__ void k_copy_scatter(int n, const double4 *x, const int *type, const int *cat,
                                const int *pos, double zprd, int *send, GatheredView g) {
@@ -978,9 +984,9 @@ void launch_copy4(int n, const double4 *a, double4 *b, hipStream_t s) {
   HIPCHK(hipGetLastError());
 }
 void launch_copy_classify(int n, const double4 *x, double zstart, double zend, double lo, double hi,
-                          int *cat, int *flag, hipStream_t s) {
+                          int *cat, int *flag, int *nloc, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_copy_classify, dim3(nblk(n)), dim3(TPB), 0, s, n, x, zstart, zend, lo, hi, cat, flag);
+  hipLaunchKernelGGL(k_copy_classify, dim3(nblk(n)), dim3(TPB), 0, s, n, x, zstart, zend, lo, hi, cat, flag, nloc);
   HIPCHK(hipGetLastError());
 }
 void launch_copy_scatter(int n, const double4 *x, const int *type, const int *cat, const int *pos,

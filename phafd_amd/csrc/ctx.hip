@@ -152,6 +152,7 @@ struct phafd_hip_ctx {
   bool setup_done = false;
   // timers
   hipEvent_t ev[9] = {};
+  hipEvent_t ev_pre[2] = {};   // a start_of_step enqueued ahead of the last read-back
   double tacc[8] = {0};
   bool fuse_fh = true;
   bool fuse_spectral = true;
@@ -214,6 +215,15 @@ int guard(phafd_hip_ctx *c, Fn fn) {
 }
 
 void sync(phafd_hip_ctx *c) { HIPCHK(hipStreamSynchronize(c->stream)); }
+
+// two device ints read back with one synchronise
+void read_int2(phafd_hip_ctx *c, const int *a, const int *b, int &va, int &vb) {
+  HIPCHK(hipMemcpyAsync(c->h_flags + 4, a, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_flags + 5, b, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  va = c->h_flags[4];
+  vb = c->h_flags[5];
+}
 
 int read_int(phafd_hip_ctx *c, const int *dptr) {
   HIPCHK(hipMemcpyAsync(c->h_flags + 4, dptr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -291,39 +301,45 @@ void borders(phafd_hip_ctx *c) {
   // copy swap
   SwapDev &z = c->swaps[0];
   const int zstart = (int)c->sublo[2], zend = (int)c->subhi[2];
-  launch_copy_classify(n, c->x.p, (double)zstart, (double)zend, z.lo, z.hi, c->tmp_cat.p, c->tmp_flag.p, c->stream);
-  const int n0 = compact_count(c, n);
+  // nlocal (the LOCAL entries of the copy swap) counted on the device in
+  // flags[6] and read back with the swap's count
+  launch_fill_i32(1, c->flags.p + 6, 0, c->stream);
+  launch_copy_classify(n, c->x.p, (double)zstart, (double)zend, z.lo, z.hi, c->tmp_cat.p, c->tmp_flag.p,
+                       c->flags.p + 6, c->stream);
+  scan_exclusive_i32(c->tmp_flag.p, c->tmp_pos.p, (size_t)n, c->stream);
+  int n0 = 0, nloc = 0;
+  read_int2(c, c->tmp_pos.p + n, c->flags.p + 6, n0, nloc);
   z.send.alloc(n0 + 1);
   launch_copy_scatter(n, c->x.p, c->type.p, c->tmp_cat.p, c->tmp_pos.p, c->period[2], z.send.p, c->gview(), c->stream);
   z.first = 0;
   z.count = n0;
   c->ngathered = n0;
-  // nlocal = LOCAL entries of the copy swap
-  {
-    std::vector<int> lab(n0);
-    if (n0) {
-      HIPCHK(hipMemcpyAsync(lab.data(), c->glabel.p, sizeof(int) * n0, hipMemcpyDeviceToHost, c->stream));
-      sync(c);
-    }
-    c->nlocal = (int)std::count(lab.begin(), lab.end(), 1);
-  }
-  // image swaps
+  c->nlocal = nloc;
+  // image swaps, a dimension at a time: both swaps of a dimension scan the
+  // same entries [0, nscan), so their flags and counts come back together
   size_t iswap = 1;
   while (iswap < c->swaps.size()) {
     const int nscan = c->ngathered;
-    for (int ineed = 0; ineed < 2; ineed++, iswap++) {
-      SwapDev &w = c->swaps[iswap];
-      c->tmp_flag.alloc((size_t)nscan + 16);
-      c->tmp_pos.alloc((size_t)nscan + 16);
-      launch_img_flag(nscan, c->gx.p, w.dim, w.lo, w.hi, c->tmp_flag.p, c->stream);
-      const int cnt = compact_count(c, nscan);
-      ensure_gathered(c, c->ngathered + cnt + 16);
-      w.send.alloc(cnt + 1);
-      w.first = c->ngathered;
-      w.count = cnt;
-      launch_img_scatter(nscan, c->tmp_flag.p, c->tmp_pos.p, w.dim, w.shift, w.first, w.send.p, c->gview(), c->stream);
-      c->ngathered += cnt;
+    const size_t stride = (size_t)nscan + 16;
+    c->tmp_flag.alloc(2 * stride);
+    c->tmp_pos.alloc(2 * stride);
+    SwapDev *w[2] = {&c->swaps[iswap], &c->swaps[iswap + 1]};
+    int cnt[2] = {0, 0};
+    for (int k = 0; k < 2; k++) {
+      launch_img_flag(nscan, c->gx.p, w[k]->dim, w[k]->lo, w[k]->hi, c->tmp_flag.p + k * stride, c->stream);
+      scan_exclusive_i32(c->tmp_flag.p + k * stride, c->tmp_pos.p + k * stride, (size_t)nscan, c->stream);
     }
+    read_int2(c, c->tmp_pos.p + nscan, c->tmp_pos.p + stride + nscan, cnt[0], cnt[1]);
+    ensure_gathered(c, c->ngathered + cnt[0] + cnt[1] + 16);
+    for (int k = 0; k < 2; k++) {
+      w[k]->send.alloc(cnt[k] + 1);
+      w[k]->first = c->ngathered;
+      w[k]->count = cnt[k];
+      launch_img_scatter(nscan, c->tmp_flag.p + k * stride, c->tmp_pos.p + k * stride, w[k]->dim, w[k]->shift,
+                         w[k]->first, w[k]->send.p, c->gview(), c->stream);
+      c->ngathered += cnt[k];
+    }
+    iswap += 2;
   }
   c->nghost = c->ngathered - c->nlocal;
 }
@@ -542,7 +558,8 @@ void build_halflist(phafd_hip_ctx *c, bool full) {
   c->tmp_flag.alloc((size_t)ng + 16);
   c->tmp_pos.alloc((size_t)ng + 16);
   launch_local_flag(ng, c->glabel.p, c->tmp_flag.p, c->stream);
-  const int nrows = compact_count(c, ng);
+  scan_exclusive_i32(c->tmp_flag.p, c->tmp_pos.p, (size_t)ng, c->stream);
+  const int nrows = c->nlocal;   // the LOCAL entries (label 1), counted at the borders
   c->h_nrows = nrows;
   c->h_ilist.alloc((size_t)nrows + 1);
   c->h_rowof.alloc((size_t)ng + 1);
@@ -603,11 +620,11 @@ void enqueue_decide(phafd_hip_ctx *c) {
 void enqueue_flags_readback(phafd_hip_ctx *c) {
   HIPCHK(hipMemcpyAsync(c->h_flags + 8, c->flags.p, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
 }
-// the checks after a step's synchronise (src/integrate.cpp:190-201)
-void check_step_flags(phafd_hip_ctx *c) {
+// the checks after step `step`'s synchronise (src/integrate.cpp:190-201)
+void check_step_flags(phafd_hip_ctx *c, int64_t step) {
   if (c->h_flags[11]) throw std::runtime_error("grid-atom term capacity exceeded");
   if (c->neigh_set) {
-    const int f = c->h_flags[8 + decide_slot(c)];
+    const int f = c->h_flags[8 + 4 * (int)(step & 1)];
     if (f & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
     c->pending_decision = f & 1;
   }
@@ -991,12 +1008,14 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
   const bool zsplit = nconj == 1 && spectral_fused(c);
   FixDef *fc = fused_chain_fix(c);
   if (c->neigh_set) clear_decide_slot(c, c->timestep + 1);
+  bool started = false;   // this step's start_of_step went in before the last read-back
   for (int it = 0; it < nsteps; it++) {
-    c->timestep++;
+    if (!started) c->timestep++;
+    const int64_t step = c->timestep;
     // muhat is observable only after the run: its last step stores it
     c->store_muhat = it == nsteps - 1;
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    phase_start(c);
+    if (!started) phase_start(c);
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     phase_neigh(c);
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
@@ -1021,6 +1040,16 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
     enqueue_flags_readback(c);
     HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->fields<R>().phi.p, sizeof(R), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipEventRecord(c->ev[8], c->stream));
+    // The next step's start_of_step (grad phi: the fields only, not the
+    // rebuild decision) goes in before the host waits for the read-back, so
+    // the device works through the host's turnaround.
+    started = it + 1 < nsteps;
+    if (started) {
+      c->timestep++;
+      HIPCHK(hipEventRecord(c->ev_pre[0], c->stream));
+      phase_start(c);
+      HIPCHK(hipEventRecord(c->ev_pre[1], c->stream));
+    }
     sync(c);
     probe_collect();
     for (int k = 0; k < 7; k++) {
@@ -1031,10 +1060,22 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
     float tot = 0;
     HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[8]));
     c->tacc[7] += tot;
+    if (started) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev_pre[0], c->ev_pre[1]));
+      c->tacc[0] += ms;
+      c->tacc[7] += ms;
+    }
     R phi0;
     std::memcpy(&phi0, c->h_flags + 2, sizeof(R));
+    if (std::isnan(phi0) && started) c->timestep = step;   // the next step did not happen
     if (std::isnan(phi0)) throw std::runtime_error("NAN encountered in phi.");
-    check_step_flags(c);
+    try {
+      check_step_flags(c, step);
+    } catch (...) {
+      if (started) c->timestep = step;
+      throw;
+    }
   }
 }
 
@@ -1206,6 +1247,7 @@ void init_ctx(phafd_hip_ctx *c, const phafd_hip_desc *desc, const MemberSlot *sl
     c->esum.alloc(8);
     HIPCHK(hipHostMalloc((void **)&c->h_flags, 64, hipHostMallocDefault));
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+    for (auto &e : c->ev_pre) HIPCHK(hipEventCreate(&e));
     for (int w = PHAFD_PHI; w <= PHAFD_GRADZ; w++)
       fill_zero(c->fp32 ? (void *)field_ptr<float>(c, w) : (void *)field_ptr<double>(c, w),
                 c->real_bytes() * c->N, c->stream);
@@ -1241,6 +1283,7 @@ int phafd_hip_destroy(phafd_hip_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+  for (auto &e : c->ev_pre) if (e) (void)hipEventDestroy(e);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->stream && c->own_stream) {
     scan_forget_stream(c->stream);
@@ -2044,7 +2087,7 @@ void batch_steps(phafd_hip_batch *b, int nsteps) {
       double phi0;
       std::memcpy(&phi0, c->h_flags + 2, sizeof(double));
       if (std::isnan(phi0)) throw std::runtime_error("NAN encountered in phi (member " + std::to_string(i) + ").");
-      check_step_flags(c);
+      check_step_flags(c, c->timestep);
     }
   }
 }
@@ -2429,7 +2472,7 @@ void atom_steps_frozen_field(phafd_hip_ctx *c, const SurrogateCfg &g, int K) {
       since_build = 0;
     }
     if (W == 1) {
-      check_step_flags(c);
+      check_step_flags(c, c->timestep);
       since_build += 1;
       s += 1;
       continue;
