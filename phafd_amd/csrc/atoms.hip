@@ -756,22 +756,43 @@ struct MuAcc<float> {
   }
   __device__ __forceinline__ float get() const { return m; }
 };
+// Four grid points per thread.  Their terms are contiguous in sv ([o.x,
+// o4)); they are loaded GS at a time (independent loads in flight) and
+// added to their point's sum in index order, the same order as one term at
+// a time.
 template <class R>
-__global__ void k_ga_sum(size_t N, const int *goff, const R *sv, R *mu, bool first) {
+__global__ void k_ga_sum(size_t N, const int *__restrict__ goff, const R *__restrict__ sv, R *__restrict__ mu,
+                         bool first) {
+  constexpr int GS = 8;
   const size_t N4 = N / 4;
   for (size_t u = blockIdx.x * (size_t)blockDim.x + threadIdx.x; u < N4; u += (size_t)gridDim.x * blockDim.x) {
     const size_t p0 = 4 * u;
     const int4 o = *reinterpret_cast<const int4 *>(goff + p0);
     const int o4 = goff[p0 + 4];
-    const int st[5] = {o.x, o.y, o.z, o.w, o4};
     if (!first && o.x == o4) continue;
+    MuAcc<R> a0, a1, a2, a3;
+    a0.m = first ? R(0) : mu[p0];
+    a1.m = first ? R(0) : mu[p0 + 1];
+    a2.m = first ? R(0) : mu[p0 + 2];
+    a3.m = first ? R(0) : mu[p0 + 3];
+    for (int j = o.x; j < o4; j += GS) {
+      R t[GS];
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      MuAcc<R> acc;
-      acc.m = first ? R(0) : mu[p0 + e];
-      for (int j = st[e]; j < st[e + 1]; j++) acc.add(sv[j]);
-      mu[p0 + e] = acc.get();
+      for (int k = 0; k < GS; k++) t[k] = j + k < o4 ? sv[j + k] : R(0);
+#pragma unroll
+      for (int k = 0; k < GS; k++) {
+        const int jj = j + k;
+        if (jj >= o4) break;
+        if (jj < o.y) a0.add(t[k]);
+        else if (jj < o.z) a1.add(t[k]);
+        else if (jj < o.w) a2.add(t[k]);
+        else a3.add(t[k]);
+      }
     }
+    mu[p0] = a0.get();
+    mu[p0 + 1] = a1.get();
+    mu[p0 + 2] = a2.get();
+    mu[p0 + 3] = a3.get();
   }
 }
 
