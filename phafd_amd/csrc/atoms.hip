@@ -115,9 +115,10 @@ __global__ void k_copy4(int n, const double4 *a, double4 *b) {
 // ------------------------------------------------------------- borders
 // copy swap classification, src/comm_brick.cpp:299-316 (one rank):
 // 1 = LOCAL copy, 2 = GHOST at z - Lz, 3 = GHOST at z + Lz, 0 = not sent
-// nloc (zeroed by the caller) counts the LOCAL (category 1) atoms
+// part[block] = the block's count of LOCAL (category 1) atoms
 __global__ void k_copy_classify(int n, const double4 *x, double zstart, double zend, double lo,
-                                double hi, int *cat, int *flag, int *nloc) {
+                                double hi, int *cat, int *flag, int *part) {
+  __shared__ int wsum[TPB / 64];
   int local = 0;
   GRID_STRIDE(i, n) {
     const double z = x[i].z;
@@ -129,9 +130,15 @@ __global__ void k_copy_classify(int n, const double4 *x, double zstart, double z
     flag[i] = c != 0;
     local += c == 1;
   }
-  // one add per wave (every lane of the block reaches this point)
+  // every lane of the block reaches this point
   for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
-  if ((threadIdx.x & 63) == 0 && local) atomicAdd(nloc, local);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < TPB / 64; w++) t += wsum[w];
+    part[blockIdx.x] = t;
+  }
 }
 __global__ void k_copy_scatter(int n, const double4 *x, const int *type, const int *cat,
                                const int *pos, double zprd, int *send, GatheredView g) {
@@ -1004,10 +1011,11 @@ void launch_copy4(int n, const double4 *a, double4 *b, hipStream_t s) {
   hipLaunchKernelGGL(k_copy4, dim3(nblk(n)), dim3(TPB), 0, s, n, a, b);
   HIPCHK(hipGetLastError());
 }
+int copy_classify_blocks(int n) { return nblk(n); }
 void launch_copy_classify(int n, const double4 *x, double zstart, double zend, double lo, double hi,
-                          int *cat, int *flag, int *nloc, hipStream_t s) {
+                          int *cat, int *flag, int *part, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_copy_classify, dim3(nblk(n)), dim3(TPB), 0, s, n, x, zstart, zend, lo, hi, cat, flag, nloc);
+  hipLaunchKernelGGL(k_copy_classify, dim3(nblk(n)), dim3(TPB), 0, s, n, x, zstart, zend, lo, hi, cat, flag, part);
   HIPCHK(hipGetLastError());
 }
 void launch_copy_scatter(int n, const double4 *x, const int *type, const int *cat, const int *pos,

@@ -58,9 +58,11 @@ void launch_unmap(int n, const double4 *x, double4 *ux, const int *image, const 
 void launch_decide(int n, const double4 *x, const double4 *xh, double triggersq, int *flag, int *clear,
                    hipStream_t s, StepGate gate = StepGate());
 void launch_copy4(int n, const double4 *a, double4 *b, hipStream_t s);
-// nloc (zeroed by the caller) receives the number of LOCAL atoms
+// part[b] receives block b's count of LOCAL atoms (copy_classify_blocks(n)
+// blocks)
+int copy_classify_blocks(int n);
 void launch_copy_classify(int n, const double4 *x, double zstart, double zend, double lo, double hi,
-                          int *cat, int *flag, int *nloc, hipStream_t s);
+                          int *cat, int *flag, int *part, hipStream_t s);
 void launch_copy_scatter(int n, const double4 *x, const int *type, const int *cat, const int *pos,
                          double zprd, int *send, GatheredView g, hipStream_t s);
 void launch_img_flag(int nscan, const double4 *gx, int dim, double lo, double hi, int *flag, hipStream_t s);

@@ -104,7 +104,7 @@ struct phafd_hip_ctx {
   int ngathered = 0, nlocal = 0, nghost = 0, gcap = 0;
   DBuf<double4> gx, gshift, gF;
   DBuf<int> gowner, glabel;
-  DBuf<int> tmp_flag, tmp_pos, tmp_cat;
+  DBuf<int> tmp_flag, tmp_pos, tmp_cat, tmp_part;
   std::vector<SwapDev> swaps;
   double cutghost = 0;
   // neighbour
@@ -301,14 +301,20 @@ void borders(phafd_hip_ctx *c) {
   // copy swap
   SwapDev &z = c->swaps[0];
   const int zstart = (int)c->sublo[2], zend = (int)c->subhi[2];
-  // nlocal (the LOCAL entries of the copy swap) counted on the device in
-  // flags[6] and read back with the swap's count
-  launch_fill_i32(1, c->flags.p + 6, 0, c->stream);
+  // nlocal (the LOCAL entries of the copy swap): per-block counts of the
+  // classification, read back with the swap's count and summed here
+  const int nbc = n > 0 ? copy_classify_blocks(n) : 0;
+  c->tmp_part.alloc((size_t)nbc + 1);
   launch_copy_classify(n, c->x.p, (double)zstart, (double)zend, z.lo, z.hi, c->tmp_cat.p, c->tmp_flag.p,
-                       c->flags.p + 6, c->stream);
+                       c->tmp_part.p, c->stream);
   scan_exclusive_i32(c->tmp_flag.p, c->tmp_pos.p, (size_t)n, c->stream);
-  int n0 = 0, nloc = 0;
-  read_int2(c, c->tmp_pos.p + n, c->flags.p + 6, n0, nloc);
+  std::vector<int> part((size_t)nbc + 1);
+  HIPCHK(hipMemcpyAsync(part.data(), c->tmp_pos.p + n, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (nbc) HIPCHK(hipMemcpyAsync(part.data() + 1, c->tmp_part.p, sizeof(int) * nbc, hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  const int n0 = part[0];
+  int nloc = 0;
+  for (int b = 1; b <= nbc; b++) nloc += part[b];
   z.send.alloc(n0 + 1);
   launch_copy_scatter(n, c->x.p, c->type.p, c->tmp_cat.p, c->tmp_pos.p, c->period[2], z.send.p, c->gview(), c->stream);
   z.first = 0;
