@@ -49,10 +49,16 @@ struct DBuf {   // device buffer owned by the context (or a view into another's)
     n = count;
     own = false;
   }
-  void alloc(size_t count) {   // grow-only, contents not preserved
+  // grow-only, contents not preserved.  A buffer that grows again (list and
+  // term buffers sized by each rebuild's counts) takes 25% headroom, so
+  // counts that creep up do not free and reallocate (hipFree synchronises
+  // the device) at every rebuild.
+  void alloc(size_t count) {
     if (count <= n && p) return;
+    const bool regrow = p != nullptr && own;
     release();
     size_t c = count ? count : 1;
+    if (regrow) c += c / 4;
     HIPCHK(hipMalloc(&p, c * sizeof(T)));
     n = c;
   }
