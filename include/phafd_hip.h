@@ -52,7 +52,8 @@ enum { PHAFD_ZGHOST_REFERENCE = 0, PHAFD_ZGHOST_PERIODIC = 1 };
 /* thermal noise of grid/conjugate/volfrac */
 enum { PHAFD_NOISE_OFF = 0, PHAFD_NOISE_FIELD = 2 };
 /* grid-atom pair styles */
-enum { PHAFD_PAIR_GAUSSIAN = 0, PHAFD_PAIR_LJISH = 1 };
+/* grid-atom pair styles of phafd_hip_pair_gridatom (lj/cut has its own call) */
+enum { PHAFD_PAIR_GAUSSIAN = 0, PHAFD_PAIR_LJISH = 1, PHAFD_PAIR_LJISH_LINEAR = 3 };
 enum { PHAFD_FP64 = 0, PHAFD_FP32 = 1 };
 /* hook phases of one timestep (src/integrate.cpp:105-203) */
 enum {
@@ -91,9 +92,12 @@ int phafd_hip_get_atoms(phafd_hip_ctx *ctx, double *xyz, double *uxyz, int *imag
 int phafd_hip_get_forces(phafd_hip_ctx *ctx, double *f);
 
 /* ---- pair styles, in script order (src/input.cpp:99-153).
- * pair gridatom/gaussian|gridatom/LJish <cut>: Pair::settings
- * (src/pair_gridatom_gaussian.cpp:123-140, src/pair_gridatom_lj_ish.cpp:184-204).
- * Returns the pair id (>= 0). */
+ * pair gridatom/gaussian|gridatom/LJish|gridatom/LJish/linear <cut>:
+ * Pair::settings (src/pair_gridatom_gaussian.cpp:123-140,
+ * src/pair_gridatom_lj_ish.cpp:184-204, src/pair_gridatom_lj_ish_linear.cpp:180-197).
+ * gridatom/LJish/linear deposits mu += e and pulls atoms with grad phi e dv,
+ * e = -eps exp(1 - 1/(1 - r^2/w^2)) (src/pair_gridatom_lj_ish_linear.cpp:46-179);
+ * its phi coefficient is unused.  Returns the pair id (>= 0). */
 int phafd_hip_pair_gridatom(phafd_hip_ctx *ctx, int style, double cut);
 /* pairCoeffs <type> epsilon e phi p nucwidth w (src/pair_gridatom_gaussian.cpp:143-181);
  * pass NaN to leave a value at its default */

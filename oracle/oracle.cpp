@@ -899,8 +899,9 @@ bool neighbor_decide(orc_sys *s) {
 }
 
 /* ------------------------------------------------------------ pairs --- */
-// PairGridAtomGaussian::compute (src/pair_gridatom_gaussian.cpp:45-120) and
-// PairGridAtomLJish::compute (src/pair_gridatom_lj_ish.cpp:44-181)
+// PairGridAtomGaussian::compute (src/pair_gridatom_gaussian.cpp:45-120),
+// PairGridAtomLJish::compute (src/pair_gridatom_lj_ish.cpp:44-181) and
+// PairGridAtomLJishLinear::compute (src/pair_gridatom_lj_ish_linear.cpp:46-179)
 void pair_gridatom(orc_sys *s, const PairStyle &p, double *energy) {
   const double Ox = s->sublo[0], Oy = s->sublo[1], Oz = s->sublo[2];
   const double dv = s->dx() * s->dy() * s->dz();
@@ -925,6 +926,15 @@ void pair_gridatom(orc_sys *s, const PairStyle &p, double *energy) {
       if (!in) continue;
       rsq /= p.cutsq[ajtype];
       double expfac;
+      if (p.style == ORC_PAIR_LJISH_LINEAR) {   // mu += e, F -= grad phi e dv; no energy
+        expfac = -1 * exp(-1. / (1 - rsq) + 1) * p.eps[ajtype];
+        s->mu[gridindex] += expfac;
+        expfac *= dv;
+        s->Fs[3 * aj + 0] -= s->grad[0][gridindex] * expfac;
+        s->Fs[3 * aj + 1] -= s->grad[1][gridindex] * expfac;
+        s->Fs[3 * aj + 2] -= s->grad[2][gridindex] * expfac;
+        continue;
+      }
       if (p.style == ORC_PAIR_GAUSSIAN)
         expfac = (exp(-rsq / 2.0) - exp(-0.5)) * exp(0.5) * p.eps[ajtype];
       else
@@ -1318,9 +1328,11 @@ int orc_get_forces(orc_sys *s, double *f) {
 }
 
 // Pair settings (src/pair_gridatom_gaussian.cpp:123-140,
-// src/pair_gridatom_lj_ish.cpp:184-204, src/pair_lj_cut.cpp:101-117)
+// src/pair_gridatom_lj_ish.cpp:184-204, src/pair_gridatom_lj_ish_linear.cpp:180-197,
+// src/pair_lj_cut.cpp:101-117)
 int orc_add_pair(orc_sys *s, int style, double a, double b, double c) {
   if (s->ntypes <= 0) { s->err = "Cannot create pair style before atoms are made."; return -1; }
+  if (style < ORC_PAIR_GAUSSIAN || style > ORC_PAIR_LJISH_LINEAR) { s->err = "Unrecognised pair style."; return -1; }
   PairStyle p;
   p.style = style;
   if (style == ORC_PAIR_LJCUT) {

@@ -17,7 +17,7 @@ PHI, MU, GRADX, GRADY, GRADZ = 0, 1, 2, 3, 4
 PHIHAT, MUHAT = 0, 1
 ZGHOST_REFERENCE, ZGHOST_PERIODIC = 0, 1
 NOISE_OFF, NOISE_FIELD = 0, 2
-PAIR_GAUSSIAN, PAIR_LJISH = 0, 1
+PAIR_GAUSSIAN, PAIR_LJISH, PAIR_LJISH_LINEAR = 0, 1, 3
 FP64, FP32 = 0, 1
 OPT_EAGER_GRIDLIST, OPT_FUSE_FH, OPT_FUSE_SPECTRAL, OPT_EAGER_HALFLIST, OPT_ATOM_WINDOWS = 0, 1, 2, 3, 4
 PH_START, PH_NEIGH, PH_PAIR, PH_INITIAL, PH_POSTFORCE, PH_PREFINAL, PH_FINAL, PH_POSTFINAL = range(8)

@@ -493,6 +493,10 @@ __global__ void k_ga_deposit(GridView gv, GAParams ga, const int *cnt, const int
       double expfac;
       if (ga.style == 0) expfac = (exp(-rsq / 2.0) - ga.em05) * ga.e05 * ga.eps[t];
       else expfac = exp(-1. / (1 - rsq) + 1) * ga.eps[t];
+      if (ga.style == GA_LJISH_LINEAR) {   // mu += -e (no phi factor, no energy)
+        m += -expfac;
+        continue;
+      }
       const double delphi = (ph - ga.phit[t]);
       m += 2 * delphi * expfac;
       if (energy) esum += delphi * delphi * expfac * gv.dv;
@@ -654,13 +658,25 @@ __global__ void k_ga_pair(const int *npairs, GridViewT<R> gv, GAParams ga, const
     R expfac;
     if (ga.style == 0) expfac = (fexp(-r / R(2)) - (R)ga.em05) * (R)ga.e05 * (R)ga.eps[t];
     else expfac = fexp(R(-1) / (1 - r) + 1) * (R)ga.eps[t];
-    const R delphi = (gv.phi[p] - (R)ga.phit[t]);
-    if (emit) {   // mu term, summed later in list order
-      pv[w] = 2 * delphi * expfac;
-      ppos[w] = atomicAdd(&gcnt[p], 1);
+    if (ga.style == GA_LJISH_LINEAR) {
+      // PairGridAtomLJishLinear::compute (src/pair_gridatom_lj_ish_linear.cpp:
+      // 93-160): e = -1 * bump * eps (the sign flip is exact); mu += e, the
+      // force weight is e dv, and no energy
+      expfac = -expfac;
+      if (emit) {
+        pv[w] = expfac;
+        ppos[w] = atomicAdd(&gcnt[p], 1);
+      }
+      expfac *= (R)gv.dv;
+    } else {
+      const R delphi = (gv.phi[p] - (R)ga.phit[t]);
+      if (emit) {   // mu term, summed later in list order
+        pv[w] = 2 * delphi * expfac;
+        ppos[w] = atomicAdd(&gcnt[p], 1);
+      }
+      if (energy) esum += (double)delphi * (double)delphi * (double)expfac * gv.dv;
+      expfac *= 2 * delphi * (R)gv.dv;
     }
-    if (energy) esum += (double)delphi * (double)delphi * (double)expfac * gv.dv;
-    expfac *= 2 * delphi * (R)gv.dv;
     typename Vec4<R>::T f;
     f.x = gv.gx[p] * expfac;
     f.y = gv.gy[p] * expfac;

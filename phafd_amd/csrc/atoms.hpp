@@ -28,8 +28,9 @@ struct GridViewT {
 };
 using GridView = GridViewT<double>;
 
+constexpr int GA_LJISH_LINEAR = 3;   // gridatom/LJish/linear (the public PHAFD_PAIR_LJISH_LINEAR)
 struct GAParams {
-  int style;        // 0 gaussian, 1 LJish
+  int style;        // 0 gaussian, 1 LJish, 3 LJish/linear
   const double *eps, *phit, *cutsq;
   double em05, e05;   // exp(-0.5), exp(0.5) evaluated on the host
 };

@@ -1396,7 +1396,8 @@ int phafd_hip_get_forces(phafd_hip_ctx *c, double *f) {
 int phafd_hip_pair_gridatom(phafd_hip_ctx *c, int style, double cut) {
   return guard(c, [&] {
     if (c->ntypes <= 0) throw std::runtime_error("Cannot create pair style before atoms are made.");
-    if (style != PHAFD_PAIR_GAUSSIAN && style != PHAFD_PAIR_LJISH) throw std::invalid_argument("Unrecognised pair style.");
+    if (style != PHAFD_PAIR_GAUSSIAN && style != PHAFD_PAIR_LJISH && style != PHAFD_PAIR_LJISH_LINEAR)
+      throw std::invalid_argument("Unrecognised pair style.");
     auto p = std::make_unique<PairDef>();
     p->style = style;
     p->maxcut = cut;
@@ -2304,6 +2305,8 @@ SurrogateCfg surrogate_cfg(phafd_hip_ctx *c) {
   for (auto &p : c->pairs)
     if (p->style != 2) { nga++; g.ga = p.get(); }
   if (nga > 1) throw std::invalid_argument("the surrogate takes at most one grid-atom pair style");
+  if (g.ga && g.ga->style == PHAFD_PAIR_LJISH_LINEAR)
+    throw std::invalid_argument("the surrogate's mu model needs a (phi - phi_t) coupling, not gridatom/LJish/linear");
   if (g.ga) {
     g.phit = g.ga->phit.empty() ? 0.0 : g.ga->phit[0];
     for (double v : g.ga->phit)

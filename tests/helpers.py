@@ -26,7 +26,7 @@ def chains(nchains, nbeads, box, seed, bond=1.0, minsep=0.9, origin=0.0):
 
 def workload(n=32, box=None, nchains=6, nbeads=20, phi0=0.5, var=0.02, chi=3.0, seed=7,
              gauss_cut=3.0, nucwidth=1.0, ljish=False, lj=True, chain=True, drag=False,
-             skin=0.5, dt=1e-3, zghost=0, origin=0.0, types=1, noise=0):
+             skin=0.5, dt=1e-3, zghost=0, origin=0.0, types=1, noise=0, eps=1.0):
     box = float(n if box is None else box)
     import phafd_amd as P
     phi = P.ic_noisy_constant((n, n, n), phi0, var, 1234 + seed)
@@ -36,20 +36,21 @@ def workload(n=32, box=None, nchains=6, nbeads=20, phi0=0.5, var=0.02, chi=3.0, 
     ends = starts + nbeads
     return dict(n=n, box=box, origin=origin, phi=phi, xyz=xyz, mol=mol, types=typ, chi=chi,
                 gauss_cut=gauss_cut, nucwidth=nucwidth, ljish=ljish, lj=lj, chain=chain, drag=drag,
-                skin=skin, dt=dt, zghost=zghost, starts=starts, ends=ends, ntypes=types, noise=noise)
+                skin=skin, dt=dt, zghost=zghost, starts=starts, ends=ends, ntypes=types, noise=noise, eps=eps)
 
 
 def apply(sysobj, w, is_oracle, setup=True):
     s = sysobj
     s.set_field(0, w["phi"])
     s.set_atoms(w["xyz"], w["types"], tags=np.arange(len(w["xyz"]), dtype=np.int32), mols=w["mol"])
-    style = 1 if w["ljish"] else 0
+    # ljish: False (gaussian), True (LJish) or "linear" (LJish/linear)
+    style = 3 if w["ljish"] == "linear" else (1 if w["ljish"] else 0)
     if is_oracle:
         ip = s.add_pair(style, w["gauss_cut"])
     else:
         ip = s.pair_gridatom(style, w["gauss_cut"])
     for t in range(w["ntypes"]):
-        s.pair_coeff(ip, t, 1.0 + 0.5 * t, 0.1, w["nucwidth"])
+        s.pair_coeff(ip, t, w["eps"] * (1.0 + 0.5 * t), 0.1, w["nucwidth"])
     if w["lj"]:
         if is_oracle:
             s.add_pair(O.PAIR_LJCUT, 1.122462048, 1.0, 1.0)

@@ -101,7 +101,7 @@ def test_setup_ghosts_bins_lists_bit_exact(zghost):
     assert _lists_equal(g.halflist(), o.halflist())
 
 
-@pytest.mark.parametrize("ljish", [False, True])
+@pytest.mark.parametrize("ljish", [False, True, "linear"])
 def test_one_step_phases(ljish):
     w = workload(n=32, nchains=10, nbeads=30, ljish=ljish, seed=4, nucwidth=2.0)
     g = make_gpu(w)
@@ -196,6 +196,31 @@ def test_list_free_deposit_matches_stored_list_bitwise():
     assert _lists_equal(a.gridlist(), b.gridlist())
 
 
+def test_linear_style_trajectory_and_stored_list():
+    # gridatom/LJish/linear (the style of the reference's example script,
+    # src/pair_gridatom_lj_ish_linear.cpp): 100 steps against the oracle with
+    # rebuilds, and the default deposit against the stored-list walk bitwise
+    # epsilon 0.2 keeps phi inside (0, 1) over the run (at 1.0 it crosses 1
+    # near step 90, where log(phi / (1 - phi)) amplifies rounding)
+    w = workload(n=32, nchains=12, nbeads=30, seed=12, ljish="linear", gauss_cut=2.5, nucwidth=2.5, zghost=1,
+                 eps=0.2)
+    g = make_gpu(w)
+    b = make_gpu(w)
+    b.set_option(P.OPT_EAGER_GRIDLIST, 1)
+    o = make_oracle(w)
+    for s in (g, b, o):
+        s.run(100)
+    assert rel_l2(g.get_field(P.PHI), o.get_field(P.PHI)) < 1e-10
+    assert rel_l2(g.get_field(P.MU), o.get_field(P.MU)) < 1e-10
+    assert rel_l2(g.get_atoms()[0], o.get_atoms()[0]) < 1e-10
+    assert rel_l2(g.get_forces(), o.get_forces()) < 1e-8
+    assert g.counts()["ncalls"] == o.counts()["ncalls"] > 1
+    assert np.allclose(g.energy(), o.energy(), rtol=1e-9, atol=1e-12)
+    assert np.array_equal(g.get_field(P.MU), b.get_field(P.MU))
+    assert np.array_equal(g.get_forces(), b.get_forces())
+    assert g.get_field(P.MU).min() < -0.2   # the linear deposit, -e per in-cut bead
+
+
 def _fields_only(cls, shape, period, noise, seed=0):
     nz, ny, nx = shape
     s = cls(nx, ny, nz, [0.0, 0.0, 0.0], period)
@@ -269,7 +294,7 @@ def test_pruned_lj_list_matches_full_list_bitwise():
     assert np.array_equal(a.get_atoms()[0], b.get_atoms()[0])
 
 
-@pytest.mark.parametrize("ljish", [False, True])
+@pytest.mark.parametrize("ljish", [False, True, "linear"])
 def test_dense_tile_mu_terms(ljish):
     # 400 atoms packed into a 5^3 corner (across the periodic boundary) with a
     # long grid-atom cut: some grid tiles hold more mu terms than the ordered

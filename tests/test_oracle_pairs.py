@@ -38,9 +38,12 @@ def brute(pos, phi, grads, style, w, eps=2.0, phit=0.1, n=8, L=8.0):
                 with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
                     e = np.exp(-1.0 / (1 - s) + 1) * eps
             e = np.where(m, e, 0.0)
-            mu += 2 * (phi - phit) * e
+            # the linear style (pair_gridatom_lj_ish_linear.cpp:46-179) deposits
+            # -e itself and weights the force by it; the others by 2 (phi - phit) e
+            wgt = -e if style == O.PAIR_LJISH_LINEAR else 2 * (phi - phit) * e
+            mu += wgt
             for c in range(3):
-                F[c] -= (grads[c] * e * 2 * (phi - phit) * (L / n) ** 3).sum()
+                F[c] -= (grads[c] * wgt * (L / n) ** 3).sum()
     return mu, F
 
 
@@ -49,7 +52,7 @@ def test_gridatom_deposit_and_force_bruteforce():
     g = np.arange(n) * L / n
     Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
     phi = 0.4 + 0.1 * np.sin(2 * np.pi * X / L) + 0.05 * np.cos(2 * np.pi * Y / L)
-    for style in (O.PAIR_GAUSSIAN, O.PAIR_LJISH):
+    for style in (O.PAIR_GAUSSIAN, O.PAIR_LJISH, O.PAIR_LJISH_LINEAR):
         s = one_atom([0.7, 7.6, 3.3], phi, style)
         grads = [s.get_field(O.GRADX), s.get_field(O.GRADY), s.get_field(O.GRADZ)]
         mu_ref, F_ref = brute([0.7, 7.6, 3.3], phi, grads, style, 1.5)
@@ -61,6 +64,19 @@ def test_gridatom_zero_force_for_constant_phi():
     s = one_atom([3.3, 4.1, 2.2], np.full((8, 8, 8), 0.37))
     assert np.abs(s.get_forces()).max() == 0.0
     assert s.get_field(O.MU).max() > 0
+    s = one_atom([3.3, 4.1, 2.2], np.full((8, 8, 8), 0.37), O.PAIR_LJISH_LINEAR)
+    assert np.abs(s.get_forces()).max() == 0.0
+    assert s.get_field(O.MU).min() < 0 and s.get_field(O.MU).max() == 0.0
+
+
+def test_linear_style_deposit_does_not_depend_on_phi():
+    # mu of gridatom/LJish/linear is -e at every in-cut point whatever phi is
+    # (no (phi - phi_t) factor), and its peak is -eps at the atom's grid point
+    rng = np.random.default_rng(5)
+    a = one_atom([3.0, 4.0, 2.0], 0.4 + 0.1 * rng.random((8, 8, 8)), O.PAIR_LJISH_LINEAR)
+    b = one_atom([3.0, 4.0, 2.0], np.full((8, 8, 8), 0.9), O.PAIR_LJISH_LINEAR)
+    assert np.array_equal(a.get_field(O.MU), b.get_field(O.MU))
+    assert a.get_field(O.MU)[2, 4, 3] == -2.0   # exp(1 - 1/(1 - 0)) eps = eps = 2
 
 
 def two_lj(p0, p1, L=16.0):
