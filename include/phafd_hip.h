@@ -105,6 +105,11 @@ int phafd_hip_pair_coeff(phafd_hip_ctx *ctx, int pair, int type, double epsilon,
                          double nucwidth);
 /* pair lj/cut <cut> <epsilon> <sigma> (src/pair_lj_cut.cpp:101-117) */
 int phafd_hip_pair_ljcut(phafd_hip_ctx *ctx, double cut, double epsilon, double sigma);
+/* pair harmonic/cut <cut> <springK> (src/pair_harmonic_cut.cpp:111-124): bead pairs
+ * closer than cut are pushed apart with fpair = -springK (r - cut) / r
+ * (src/pair_harmonic_cut.cpp:45-108), over the same half list as lj/cut;
+ * no energy.  Returns the pair id (>= 0). */
+int phafd_hip_pair_harmoniccut(phafd_hip_ctx *ctx, double cut, double springK);
 
 /* ---- fixes, in script order (src/input.cpp:175-230) */
 /* fix grid/conjugate/volfrac <name> <seed> mobility M temp T volFH v gamma g

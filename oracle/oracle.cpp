@@ -391,6 +391,8 @@ struct PairStyle {
   std::vector<double> eps, phit, cutsq;
   // lj/cut, src/pair_lj_cut.cpp:101-117
   double lj1 = 0, lj2 = 0, ljcutsq = 0;
+  // harmonic/cut, src/pair_harmonic_cut.cpp:111-124
+  double hk = 0, hcut = 0;
 };
 
 struct FixDef {
@@ -980,6 +982,37 @@ void pair_ljcut(orc_sys *s, const PairStyle &p, double *energy) {
   }
 }
 
+// PairHarmonicCut::compute, src/pair_harmonic_cut.cpp:45-108: a repulsive
+// spring k (cut - r) along the pair axis for r < cut; coincident atoms are
+// pushed apart along (0.001, 0.001, 0.001); no energy is tallied.
+void pair_harmonic(orc_sys *s, const PairStyle &p) {
+  for (size_t ii = 0; ii < s->h_ilist.size(); ii++) {
+    int i = s->h_ilist[ii];
+    double xtmp = s->xs[3 * i], ytmp = s->xs[3 * i + 1], ztmp = s->xs[3 * i + 2];
+    const int *jlist = &s->h_neigh[s->h_off[ii]];
+    int jnum = s->h_num[ii];
+    for (int jj = 0; jj < jnum; jj++) {
+      int j = jlist[jj];
+      double delx = xtmp - s->xs[3 * j];
+      double dely = ytmp - s->xs[3 * j + 1];
+      double delz = ztmp - s->xs[3 * j + 2];
+      double rij = sqrt(delx * delx + dely * dely + delz * delz);
+      if (!(rij < p.hcut)) continue;
+      if (rij == 0) {
+        delx = dely = delz = 0.001;
+        rij = sqrt(delx * delx + dely * dely + delz * delz);
+      }
+      double fpair = -1 * p.hk * (rij - p.hcut) / rij;
+      s->Fs[3 * i + 0] += delx * fpair;
+      s->Fs[3 * i + 1] += dely * fpair;
+      s->Fs[3 * i + 2] += delz * fpair;
+      s->Fs[3 * j + 0] -= delx * fpair;
+      s->Fs[3 * j + 1] -= dely * fpair;
+      s->Fs[3 * j + 2] -= delz * fpair;
+    }
+  }
+}
+
 /* ------------------------------------------------------- chain fix ---
  * Build-defined semiflexible chain (the reference's bead-rod dynamics live
  * in the absent beadrodpmer library, src/fixatom_semiflexible.cpp:144-191).
@@ -1096,6 +1129,7 @@ void phase_pair(orc_sys *s) {
   std::fill(s->Fs.begin(), s->Fs.end(), 0.0);
   for (auto &p : s->pairs) {
     if (p.style == ORC_PAIR_LJCUT) pair_ljcut(s, p, nullptr);
+    else if (p.style == ORC_PAIR_HARMONIC) pair_harmonic(s, p);
     else pair_gridatom(s, p, nullptr);
   }
   reverse_comm(s);
@@ -1329,13 +1363,18 @@ int orc_get_forces(orc_sys *s, double *f) {
 
 // Pair settings (src/pair_gridatom_gaussian.cpp:123-140,
 // src/pair_gridatom_lj_ish.cpp:184-204, src/pair_gridatom_lj_ish_linear.cpp:180-197,
-// src/pair_lj_cut.cpp:101-117)
+// src/pair_lj_cut.cpp:101-117, src/pair_harmonic_cut.cpp:111-124)
 int orc_add_pair(orc_sys *s, int style, double a, double b, double c) {
   if (s->ntypes <= 0) { s->err = "Cannot create pair style before atoms are made."; return -1; }
-  if (style < ORC_PAIR_GAUSSIAN || style > ORC_PAIR_LJISH_LINEAR) { s->err = "Unrecognised pair style."; return -1; }
+  if (style < ORC_PAIR_GAUSSIAN || style > ORC_PAIR_HARMONIC) { s->err = "Unrecognised pair style."; return -1; }
   PairStyle p;
   p.style = style;
-  if (style == ORC_PAIR_LJCUT) {
+  if (style == ORC_PAIR_HARMONIC) {   // a = cut, b = springK
+    p.maxcut = a;
+    p.hcut = a;
+    p.hk = b;
+    s->need_half = true;
+  } else if (style == ORC_PAIR_LJCUT) {
     p.maxcut = a;
     p.lj1 = 48 * b * pow(c, 12.0);
     p.lj2 = 24 * b * pow(c, 6.0);
@@ -1540,6 +1579,7 @@ int orc_energy(orc_sys *s, double *out) {
     std::vector<double> mu_save = s->mu, f_save = s->Fs;
     for (auto &p : s->pairs) {
       if (p.style == ORC_PAIR_LJCUT) pair_ljcut(s, p, &elj);
+      else if (p.style == ORC_PAIR_HARMONIC) continue;
       else pair_gridatom(s, p, &ega);
     }
     s->mu.swap(mu_save);

@@ -30,7 +30,8 @@ typedef struct orc_sys orc_sys;
 enum { ORC_PHI = 0, ORC_MU = 1, ORC_GRADX = 2, ORC_GRADY = 3, ORC_GRADZ = 4 };
 enum { ORC_ZGHOST_REFERENCE = 0, ORC_ZGHOST_PERIODIC = 1 };
 enum { ORC_NOISE_OFF = 0, ORC_NOISE_MT19937 = 1, ORC_NOISE_FIELD = 2 };
-enum { ORC_PAIR_GAUSSIAN = 0, ORC_PAIR_LJISH = 1, ORC_PAIR_LJCUT = 2, ORC_PAIR_LJISH_LINEAR = 3 };
+enum { ORC_PAIR_GAUSSIAN = 0, ORC_PAIR_LJISH = 1, ORC_PAIR_LJCUT = 2, ORC_PAIR_LJISH_LINEAR = 3,
+       ORC_PAIR_HARMONIC = 4 };
 
 /* ---- stateless kernels (unit-level parity) ---- */
 int orc_fft_r2c(int nx, int ny, int nz, const double *in, double *out);

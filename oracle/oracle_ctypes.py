@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 PHI, MU, GRADX, GRADY, GRADZ = 0, 1, 2, 3, 4
 ZGHOST_REFERENCE, ZGHOST_PERIODIC = 0, 1
 NOISE_OFF, NOISE_MT19937, NOISE_FIELD = 0, 1, 2
-PAIR_GAUSSIAN, PAIR_LJISH, PAIR_LJCUT, PAIR_LJISH_LINEAR = 0, 1, 2, 3
+PAIR_GAUSSIAN, PAIR_LJISH, PAIR_LJCUT, PAIR_LJISH_LINEAR, PAIR_HARMONIC = 0, 1, 2, 3, 4
 PH_START, PH_NEIGH, PH_PAIR, PH_INITIAL, PH_POSTFORCE, PH_PREFINAL, PH_FINAL, PH_POSTFINAL = range(8)
 
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -65,6 +65,7 @@ def lib():
         "phafd_hip_pair_gridatom": ([V, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "phafd_hip_pair_coeff": ([V, ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 3, ctypes.c_int),
         "phafd_hip_pair_ljcut": ([V] + [ctypes.c_double] * 3, ctypes.c_int),
+        "phafd_hip_pair_harmoniccut": ([V] + [ctypes.c_double] * 2, ctypes.c_int),
         "phafd_hip_fix_conjugate": ([V, ctypes.c_int] + [ctypes.c_double] * 4 + [ctypes.c_int], ctypes.c_int),
         "phafd_hip_set_noise": ([V, _dp], ctypes.c_int),
         "phafd_hip_fix_floryhuggins": ([V] + [ctypes.c_double] * 3, ctypes.c_int),
@@ -337,6 +338,9 @@ class System:
 
     def pair_ljcut(self, cut, eps, sigma):
         return self._c(lib().phafd_hip_pair_ljcut(self.h, cut, eps, sigma))
+
+    def pair_harmoniccut(self, cut, spring_k):
+        return self._c(lib().phafd_hip_pair_harmoniccut(self.h, cut, spring_k))
 
     def fix_conjugate(self, seed, mobility=1.0, temp=1.0, volFH=1.0, gamma=1.0, noise=NOISE_OFF):
         self._c(lib().phafd_hip_fix_conjugate(self.h, seed, mobility, temp, volFH, gamma, noise))

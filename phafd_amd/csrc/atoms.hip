@@ -838,10 +838,46 @@ __device__ __forceinline__ bool lj_pair(const double4 &xi, const double4 &xj, co
   e = r6inv * (lj.lj1 / 12.0 * r6inv - lj.lj2 / 6.0);
   return true;
 }
-__global__ void k_lj_force(int ng, LJParams lj, const double4 *gx, const int *rowof,
-                           const int *hoff, const int *hneigh, const int *toff, const int *tpair,
-                           const int *ti, int nowned, double4 *gF, double *energy, int zero_init,
-                           StepGate gate) {
+// PairHarmonicCut::compute, src/pair_harmonic_cut.cpp:45-108: fpair =
+// -k (r - cut) / r inside the cut; coincident atoms are pushed apart along
+// (0.001, 0.001, 0.001) as the reference does; no energy
+__device__ __forceinline__ bool harm_pair(const double4 &xi, const double4 &xj, const HarmParams &h, double &fx,
+                                          double &fy, double &fz, double &e) {
+  double delx = xi.x - xj.x, dely = xi.y - xj.y, delz = xi.z - xj.z;
+  double rij = sqrt(delx * delx + dely * dely + delz * delz);
+  if (!(rij < h.cut)) return false;
+  if (rij == 0) {
+    delx = dely = delz = 0.001;
+    rij = sqrt(delx * delx + dely * dely + delz * delz);
+  }
+  const double fpair = -1 * h.k * (rij - h.cut) / rij;
+  fx = delx * fpair;
+  fy = dely * fpair;
+  fz = delz * fpair;
+  e = 0.0;
+  return true;
+}
+struct LJLaw {
+  LJParams p;
+  __device__ bool operator()(const double4 &a, const double4 &b, double &fx, double &fy, double &fz,
+                             double &e) const {
+    return lj_pair(a, b, p, fx, fy, fz, e);
+  }
+};
+struct HarmLaw {
+  HarmParams p;
+  __device__ bool operator()(const double4 &a, const double4 &b, double &fx, double &fy, double &fz,
+                             double &e) const {
+    return harm_pair(a, b, p, fx, fy, fz, e);
+  }
+};
+// one bead-bead pair style over the pruned half list, gathered per atom in
+// the reference's slot order (the law: lj/cut or harmonic/cut)
+template <class Law>
+__global__ void k_pair_force(int ng, Law law, const double4 *gx, const int *rowof,
+                             const int *hoff, const int *hneigh, const int *toff, const int *tpair,
+                             const int *ti, int nowned, double4 *gF, double *energy, int zero_init,
+                             StepGate gate) {
   if (gate_off(gate)) return;
   double esum = 0.0;
   GRID_STRIDE(a, ng) {
@@ -854,18 +890,18 @@ __global__ void k_lj_force(int ng, LJParams lj, const double4 *gx, const int *ro
     const int q1 = toff[a + 1];
     double fx, fy, fz, e;
     for (; q < q1 && tpair[q] < rs; q++) {
-      if (lj_pair(gx[ti[q]], xa, lj, fx, fy, fz, e)) { F.x -= fx; F.y -= fy; F.z -= fz; }
+      if (law(gx[ti[q]], xa, fx, fy, fz, e)) { F.x -= fx; F.y -= fy; F.z -= fz; }
     }
     if (row >= 0) {
       for (int s = rs; s < re; s++) {
-        if (lj_pair(xa, gx[hneigh[s] - nowned], lj, fx, fy, fz, e)) {
+        if (law(xa, gx[hneigh[s] - nowned], fx, fy, fz, e)) {
           F.x += fx; F.y += fy; F.z += fz;
           if (energy) esum += e;
         }
       }
     }
     for (; q < q1; q++) {
-      if (lj_pair(gx[ti[q]], xa, lj, fx, fy, fz, e)) { F.x -= fx; F.y -= fy; F.z -= fz; }
+      if (law(gx[ti[q]], xa, fx, fy, fz, e)) { F.x -= fx; F.y -= fy; F.z -= fz; }
     }
     gF[a] = F;
   }
@@ -1259,8 +1295,16 @@ void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *r
                      const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
                      double4 *gF, double *energy, hipStream_t s, bool zero_init, StepGate gate) {
   if (ng <= 0) return;
-  hipLaunchKernelGGL(k_lj_force, dim3(nblk(ng)), dim3(TPB), 0, s, ng, lj, gx, rowof, hoff, hneigh, toff,
-                     tpair, ti, nowned, gF, energy, zero_init ? 1 : 0, gate);
+  hipLaunchKernelGGL(k_pair_force<LJLaw>, dim3(nblk(ng)), dim3(TPB), 0, s, ng, LJLaw{lj}, gx, rowof, hoff, hneigh,
+                     toff, tpair, ti, nowned, gF, energy, zero_init ? 1 : 0, gate);
+  HIPCHK(hipGetLastError());
+}
+void launch_harmonic_force(int ng, const HarmParams &h, const double4 *gx, const int *rowof, const int *hoff,
+                           const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
+                           double4 *gF, hipStream_t s, bool zero_init, StepGate gate) {
+  if (ng <= 0) return;
+  hipLaunchKernelGGL(k_pair_force<HarmLaw>, dim3(nblk(ng)), dim3(TPB), 0, s, ng, HarmLaw{h}, gx, rowof, hoff,
+                     hneigh, toff, tpair, ti, nowned, gF, (double *)nullptr, zero_init ? 1 : 0, gate);
   HIPCHK(hipGetLastError());
 }
 void launch_drag(int n, const int *members, double4 *x, const double4 *F, double drag, double dt,

@@ -46,6 +46,9 @@ struct StepGate {
 struct LJParams {
   double lj1, lj2, cutsq;
 };
+struct HarmParams {   // pair harmonic/cut <cut> <k> (src/pair_harmonic_cut.cpp:111-124)
+  double k, cut;
+};
 
 struct ChainParams {
   double r0, kbond, kbend, drag;
@@ -134,6 +137,10 @@ void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *r
                      const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
                      double4 *gF, double *energy, hipStream_t s, bool zero_init = false,
                      StepGate gate = StepGate());
+// harmonic/cut forces over the same list and in the same order (no energy)
+void launch_harmonic_force(int ng, const HarmParams &h, const double4 *gx, const int *rowof, const int *hoff,
+                           const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,
+                           double4 *gF, hipStream_t s, bool zero_init = false, StepGate gate = StepGate());
 void launch_drag(int n, const int *members, double4 *x, const double4 *F, double drag, double dt,
                  hipStream_t s);
 // halves (1 or 2) half steps of the chains (contiguous atom ranges

@@ -23,14 +23,18 @@ const double BIG = 1e20;          // src/comm_brick.cpp:8
 const double SMALL = 1.0e-6;      // src/nbin_standard.cpp:25
 
 struct PairDef {
-  int style;   // 0 gaussian, 1 LJish, 2 lj/cut
+  int style;   // 0 gaussian, 1 LJish, 3 LJish/linear (grid-atom); 2 lj/cut, 4 harmonic/cut (bead pairs)
   double maxcut = 0;
   std::vector<double> eps, phit, cutsq;
   DBuf<double> d_eps, d_phit, d_cutsq, d_cutsq_bound;
   int64_t cap_build = -1;   // ncalls of the build the pair capacity was sized at
   int64_t cap = 0;
   double lj1 = 0, lj2 = 0, ljcutsq = 0;
+  double hk = 0, hcut = 0;   // harmonic/cut spring constant and cut
 };
+// lj/cut and harmonic/cut act between beads over the half list; the other
+// styles are grid-atom couplings
+bool bead_pair(const PairDef &p) { return p.style == 2 || p.style == 4; }
 
 struct FixDef {
   enum Kind { CONJ, FH, GRADPHI, DRAG, CHAIN } kind;
@@ -484,8 +488,10 @@ void create_stencils(phafd_hip_ctx *c) {
           if (bin_distance(i, j, k) < cutneighsq)
             c->stencil_half.push_back(k * b.mbiny * b.mbinx + j * b.mbinx + i);
   double ljcut = 0.0;
-  for (auto &p : c->pairs)
+  for (auto &p : c->pairs) {
     if (p->style == 2) ljcut = std::max(ljcut, std::sqrt(p->ljcutsq));
+    if (p->style == 4) ljcut = std::max(ljcut, p->hcut);
+  }
   const double prune = ljcut + c->skin;
   c->prune_sq = prune < c->cutneigh ? prune * prune : cutneighsq;
   c->stencil_lj.clear();
@@ -827,7 +833,7 @@ void phase_pair(phafd_hip_ctx *c, double *energy_ga = nullptr, double *energy_lj
   bool mu_pending = true;
   const bool any_ga = c->neigh_set && c->lists_valid &&
                       std::any_of(c->pairs.begin(), c->pairs.end(), [](const std::unique_ptr<PairDef> &p) {
-                        return p->style != 2;
+                        return !bead_pair(*p);
                       });
   if (!any_ga) {
     fill_zero(c->fp32 ? (void *)c->fs.mu.p : (void *)c->fd.mu.p, c->real_bytes() * c->N, c->stream);
@@ -844,6 +850,9 @@ void phase_pair(phafd_hip_ctx *c, double *energy_ga = nullptr, double *energy_lj
       launch_lj_force(c->ngathered, lj, c->gx.p, c->h_rowof.p, c->h_off.p, c->h_neigh.p, c->t_off.p,
                       c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, energy_lj, c->stream);
       probe_end(K_LJ, c->stream, 64.0 * c->ngathered + 4.0 * c->h_nent);
+    } else if (p.style == 4) {
+      launch_harmonic_force(c->ngathered, HarmParams{p.hk, p.hcut}, c->gx.p, c->h_rowof.p, c->h_off.p, c->h_neigh.p,
+                            c->t_off.p, c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, c->stream);
     } else {
       if (c->fp32) grid_atom_pass<float>(c, p, energy_ga, mu_pending);
       else grid_atom_pass<double>(c, p, energy_ga, mu_pending);
@@ -1413,7 +1422,8 @@ int phafd_hip_pair_coeff(phafd_hip_ctx *c, int ipair, int type, double eps, doub
   return guard(c, [&] {
     if (ipair < 0 || ipair >= (int)c->pairs.size()) throw std::invalid_argument("no such pair");
     PairDef &p = *c->pairs[ipair];
-    if (p.style == 2) return 0;   // PairLJCut::coeff is empty (src/pair_lj_cut.cpp:119-125)
+    if (bead_pair(p)) return 0;   // PairLJCut / PairHarmonicCut::coeff are empty (src/pair_lj_cut.cpp:119-125,
+                                  // src/pair_harmonic_cut.cpp:127-130)
     if (type < 0 || type >= (int)p.eps.size()) throw std::runtime_error("Pair grid/atom atom type out of range.");
     if (!std::isnan(eps)) p.eps[type] = eps;
     if (!std::isnan(phit)) p.phit[type] = phit;
@@ -1433,6 +1443,20 @@ int phafd_hip_pair_ljcut(phafd_hip_ctx *c, double cut, double eps, double sigma)
     p->lj1 = 48 * eps * pow(sigma, 12.0);
     p->lj2 = 24 * eps * pow(sigma, 6.0);
     p->ljcutsq = cut * cut;
+    c->pairs.push_back(std::move(p));
+    c->need_half = true;
+    return (int)c->pairs.size() - 1;
+  });
+}
+
+int phafd_hip_pair_harmoniccut(phafd_hip_ctx *c, double cut, double springK) {
+  return guard(c, [&] {
+    if (c->ntypes <= 0) throw std::runtime_error("Cannot create pair style before atoms are made.");
+    auto p = std::make_unique<PairDef>();
+    p->style = 4;
+    p->maxcut = cut;
+    p->hcut = cut;
+    p->hk = springK;
     c->pairs.push_back(std::move(p));
     c->need_half = true;
     return (int)c->pairs.size() - 1;
@@ -1534,7 +1558,7 @@ int phafd_hip_neighbor(phafd_hip_ctx *c, double skin) {
     setup_bins(c);
     create_stencils(c);
     comm_setup(c);
-    for (auto &p : c->pairs) if (p->style != 2) upload_pair(*p, c->skin);
+    for (auto &p : c->pairs) if (!bead_pair(*p)) upload_pair(*p, c->skin);
     c->neigh_set = true;
     return 0;
   });
@@ -1544,7 +1568,7 @@ int phafd_hip_setup(phafd_hip_ctx *c, double dt) {
   return guard(c, [&] {
     if (dt <= 0.0) throw std::runtime_error("Cannot run simulation without setting dt > 0.");
     c->dt = dt;
-    for (auto &p : c->pairs) if (p->style != 2) upload_pair(*p, c->skin);
+    for (auto &p : c->pairs) if (!bead_pair(*p)) upload_pair(*p, c->skin);
     if (c->neigh_set && c->nowned > 0) {
       launch_pbc(c->nowned, c->x.p, c->ux.p, c->image.p, c->box(), c->stream);
       borders(c);
@@ -2303,7 +2327,7 @@ SurrogateCfg surrogate_cfg(phafd_hip_ctx *c) {
     throw std::invalid_argument("the surrogate needs one grid/conjugate/volfrac and one grid/floryhuggins fix");
   if (g.cj->noise_mode != PHAFD_NOISE_OFF) throw std::invalid_argument("the surrogate runs with noise off");
   for (auto &p : c->pairs)
-    if (p->style != 2) { nga++; g.ga = p.get(); }
+    if (!bead_pair(*p)) { nga++; g.ga = p.get(); }
   if (nga > 1) throw std::invalid_argument("the surrogate takes at most one grid-atom pair style");
   if (g.ga && g.ga->style == PHAFD_PAIR_LJISH_LINEAR)
     throw std::invalid_argument("the surrogate's mu model needs a (phi - phi_t) coupling, not gridatom/LJish/linear");
@@ -2417,7 +2441,7 @@ void atom_steps_frozen_field(phafd_hip_ctx *c, const SurrogateCfg &g, int K) {
   if (c->neigh_set) clear_decide_slot(c, c->timestep + 1);
   FixDef *fc = fused_chain_fix(c);
   bool any_lj = false;
-  for (auto &pp : c->pairs) any_lj = any_lj || pp->style == 2;
+  for (auto &pp : c->pairs) any_lj = any_lj || bead_pair(*pp);
   bool slot_dirty = false;   // a window ran since the last single step
   int since_build = 0, interval = 8;
   int s = 0;
@@ -2452,10 +2476,16 @@ void atom_steps_frozen_field(phafd_hip_ctx *c, const SurrogateCfg &g, int K) {
       bool any = false;
       if (lists) {
         for (auto &pp : c->pairs) {
-          if (pp->style != 2) continue;
-          LJParams lj{pp->lj1, pp->lj2, pp->ljcutsq};
-          launch_lj_force(c->ngathered, lj, c->gx.p, c->h_rowof.p, c->h_off.p, c->h_neigh.p, c->t_off.p,
-                          c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, nullptr, c->stream, !any, gate);
+          if (!bead_pair(*pp)) continue;
+          if (pp->style == 2) {
+            LJParams lj{pp->lj1, pp->lj2, pp->ljcutsq};
+            launch_lj_force(c->ngathered, lj, c->gx.p, c->h_rowof.p, c->h_off.p, c->h_neigh.p, c->t_off.p,
+                            c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, nullptr, c->stream, !any, gate);
+          } else {
+            launch_harmonic_force(c->ngathered, HarmParams{pp->hk, pp->hcut}, c->gx.p, c->h_rowof.p, c->h_off.p,
+                                  c->h_neigh.p, c->t_off.p, c->t_pair.p, c->t_i.p, c->nowned, c->gF.p, c->stream,
+                                  !any, gate);
+          }
           any = true;
         }
       }

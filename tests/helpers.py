@@ -26,7 +26,7 @@ def chains(nchains, nbeads, box, seed, bond=1.0, minsep=0.9, origin=0.0):
 
 def workload(n=32, box=None, nchains=6, nbeads=20, phi0=0.5, var=0.02, chi=3.0, seed=7,
              gauss_cut=3.0, nucwidth=1.0, ljish=False, lj=True, chain=True, drag=False,
-             skin=0.5, dt=1e-3, zghost=0, origin=0.0, types=1, noise=0, eps=1.0):
+             skin=0.5, dt=1e-3, zghost=0, origin=0.0, types=1, noise=0, eps=1.0, harmonic=None):
     box = float(n if box is None else box)
     import phafd_amd as P
     phi = P.ic_noisy_constant((n, n, n), phi0, var, 1234 + seed)
@@ -36,7 +36,8 @@ def workload(n=32, box=None, nchains=6, nbeads=20, phi0=0.5, var=0.02, chi=3.0, 
     ends = starts + nbeads
     return dict(n=n, box=box, origin=origin, phi=phi, xyz=xyz, mol=mol, types=typ, chi=chi,
                 gauss_cut=gauss_cut, nucwidth=nucwidth, ljish=ljish, lj=lj, chain=chain, drag=drag,
-                skin=skin, dt=dt, zghost=zghost, starts=starts, ends=ends, ntypes=types, noise=noise, eps=eps)
+                skin=skin, dt=dt, zghost=zghost, starts=starts, ends=ends, ntypes=types, noise=noise, eps=eps,
+                harmonic=harmonic)
 
 
 def apply(sysobj, w, is_oracle, setup=True):
@@ -56,6 +57,11 @@ def apply(sysobj, w, is_oracle, setup=True):
             s.add_pair(O.PAIR_LJCUT, 1.122462048, 1.0, 1.0)
         else:
             s.pair_ljcut(1.122462048, 1.0, 1.0)
+    if w["harmonic"]:   # (cut, spring constant) of pair harmonic/cut
+        if is_oracle:
+            s.add_pair(O.PAIR_HARMONIC, *w["harmonic"])
+        else:
+            s.pair_harmoniccut(*w["harmonic"])
     s.fix_conjugate(5678, 1.0, 1.0, 1.0, 1.0, w["noise"])
     s.fix_floryhuggins(1.0, 1.0, w["chi"])
     if w["chain"]:

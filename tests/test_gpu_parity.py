@@ -8,6 +8,8 @@ Tolerances (relative L2 unless noted):
   grid and half lists ... bit-exact (integer / index results)
   mu, forces, one step .. 1e-10   (the stated FP64 tolerance)
   200-step trajectory ... 1e-10 for phi, positions
+  bead pair styles ...... lj/cut and harmonic/cut at the same 1e-10; the
+                          pruned list against the full half list bitwise
 """
 import os
 
@@ -219,6 +221,34 @@ def test_linear_style_trajectory_and_stored_list():
     assert np.array_equal(g.get_field(P.MU), b.get_field(P.MU))
     assert np.array_equal(g.get_forces(), b.get_forces())
     assert g.get_field(P.MU).min() < -0.2   # the linear deposit, -e per in-cut bead
+
+
+@pytest.mark.parametrize("lj", [True, False])
+def test_harmonic_cut_one_step_and_trajectory(lj):
+    # pair harmonic/cut (src/pair_harmonic_cut.cpp) beside lj/cut or alone:
+    # the pair phase against the oracle, 100 steps with rebuilds, and the
+    # pruned list (cut = the larger of the two bead cuts) against the full
+    # NPairHalfBinNewton list bitwise
+    w = workload(n=32, nchains=12, nbeads=30, seed=14, zghost=1, lj=lj, harmonic=(1.3, 40.0))
+    g = make_gpu(w)
+    o = make_oracle(w)
+    for ph in range(P.PH_PAIR + 1):
+        g.phase(ph)
+        o.phase(ph)
+    assert rel_l2(g.get_forces(), o.get_forces()) < 1e-10
+    g = make_gpu(w)
+    o = make_oracle(w)
+    full = make_gpu(w)
+    full.set_option(P.OPT_EAGER_HALFLIST, 1)
+    for s in (g, o, full):
+        s.run(100)
+    assert rel_l2(g.get_field(P.PHI), o.get_field(P.PHI)) < 1e-10
+    assert rel_l2(g.get_atoms()[0], o.get_atoms()[0]) < 1e-10
+    assert rel_l2(g.get_forces(), o.get_forces()) < 1e-8
+    assert g.counts()["ncalls"] == o.counts()["ncalls"] > 1
+    assert np.allclose(g.energy(), o.energy(), rtol=1e-9, atol=1e-12)
+    assert np.array_equal(g.get_atoms()[0], full.get_atoms()[0])
+    assert np.array_equal(g.get_forces(), full.get_forces())
 
 
 def _fields_only(cls, shape, period, noise, seed=0):

@@ -15,6 +15,9 @@
   bead density ....... against a periodic brute-force sum: 1e-5 (FP32 output)
   reproducibility .... two surrogate runs from one state: identical phi and
                        positions (the density is summed in fixed point)
+  atom windows ....... speculative windows of atom steps against one step per
+                       read-back, bit for bit (with lj/cut, and with
+                       harmonic/cut as the only bead pair)
   fallback ........... a rejected block is K plain steps from the restored
                        atoms (1e-13: re-binned lists may reorder sums)
   gain ............... the committed trained network against the same block
@@ -319,6 +322,42 @@ def test_atom_step_windows_match_single_steps():
     assert out[0][2] == out[1][2] and out[0][2] > 2   # the same rebuilds, several of them
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_surrogate_atom_steps_with_harmonic_cut():
+    # the surrogate's atom steps with pair harmonic/cut as the only bead pair
+    # (it then writes the pair force from zero): windows against single steps
+    # bit for bit, and the block trajectory stays within the acceptance test
+    z = np.load(WEIGHTS)
+    K, tol = int(z["K"][0]), float(z["tol"][0])
+    _, phi0, var, chi, nch, nb, style, cut, width = S.FAMILIES["production"]
+    n = 64
+    out = []
+    for windows in (1, 0):
+        s = P.System(n, n, n, [0, 0, 0], [float(n)] * 3, zghost=P.ZGHOST_PERIODIC)
+        s.set_field(P.PHI, P.ic_noisy_constant((n, n, n), phi0, var, 1234 + 503))
+        xyz, mol = P.ic_random_chains(nch, nb, 1.0, 1.0, [0, 0, 0], [float(n)] * 3, 42 + 503)
+        s.set_atoms(xyz, np.zeros(len(xyz), np.int32), tags=np.arange(len(xyz), dtype=np.int32), mols=mol)
+        ip = s.pair_gridatom(style, cut)
+        s.pair_coeff(ip, 0, 1.0, 0.1, width)
+        s.pair_harmoniccut(1.2, 30.0)
+        s.fix_conjugate(5678, 1.0, 1.0, 1.0, 1.0, P.NOISE_OFF)
+        s.fix_floryhuggins(1.0, 1.0, chi)
+        st = (np.arange(nch) * nb).astype(np.int32)
+        s.fix_chain(st, st + nb, 1.0, 100.0, 10.0, 1.0)
+        s.neighbor(0.5)
+        s.setup(1e-3)
+        s.set_option(P.OPT_ATOM_WINDOWS, windows)
+        net = P.UNet(seed=1)
+        net.load(WEIGHTS)
+        s.run_surrogate(net, 2 * K, K, tol)
+        out.append((s.get_field(P.PHI), s.get_atoms()[0], s.counts()["ncalls"]))
+        net.close()
+        s.close()
+    assert out[0][2] == out[1][2] and out[0][2] > 1
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.isfinite(out[0][1]).all()
 
 
 def test_trained_network_gains_over_the_base_alone():

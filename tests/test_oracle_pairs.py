@@ -79,11 +79,12 @@ def test_linear_style_deposit_does_not_depend_on_phi():
     assert a.get_field(O.MU)[2, 4, 3] == -2.0   # exp(1 - 1/(1 - 0)) eps = eps = 2
 
 
-def two_lj(p0, p1, L=16.0):
+def two_lj(p0, p1, L=16.0, pairs=((O.PAIR_LJCUT, 1.5, 1.0, 1.0),)):
     s = O.System(16, 16, 16, [0, 0, 0], [L] * 3)
     s.set_field(O.PHI, np.full((16, 16, 16), 0.5))
     s.set_atoms(np.array([p0, p1]), np.zeros(2, np.int32))
-    s.add_pair(O.PAIR_LJCUT, 1.5, 1.0, 1.0)
+    for pr in pairs:
+        s.add_pair(*pr)
     s.neighbor(0.5)
     s.setup(1e-3)
     s.phase(O.PH_START)
@@ -110,6 +111,59 @@ def test_lj_across_x_boundary_and_not_z():
     assert np.allclose(F[0], -F[1], atol=1e-15)
     F = two_lj([5.0, 5.0, 0.3], [5.0, 5.0, 15.5])    # no z images on one rank
     assert np.abs(F).max() == 0.0
+
+
+HARM = (O.PAIR_HARMONIC, 1.3, 40.0)
+
+
+def harmonic_force(d, cut=1.3, k=40.0):
+    # U = k/2 (cut - r)^2 for r < cut, so F_i = -dU/dx_i = k (cut - r) d / r
+    r = np.linalg.norm(d)
+    return k * (cut - r) * d / r if r < cut else np.zeros(3)
+
+
+def test_harmonic_two_body_and_newton():
+    for p1 in ([6.1, 5.2, 4.9], [5.3, 5.0, 5.4], [5.0, 6.29, 5.0]):
+        F = two_lj([5.0, 5.0, 5.0], p1, pairs=(HARM,))
+        d = np.array([5.0, 5.0, 5.0]) - np.array(p1)
+        assert np.allclose(F[0], harmonic_force(d), rtol=1e-13, atol=0)
+        assert np.array_equal(F[0], -F[1])
+    # the force is repulsive: atom 0 is pushed away from atom 1
+    F = two_lj([5.0, 5.0, 5.0], [5.5, 5.0, 5.0], pairs=(HARM,))
+    assert F[0][0] < 0 and F[0][1] == 0 and F[0][2] == 0
+
+
+def test_harmonic_force_is_minus_energy_gradient():
+    p0, p1 = np.array([5.0, 5.0, 5.0]), np.array([5.7, 5.4, 4.8])
+    F = two_lj(p0, p1, pairs=(HARM,))
+
+    def energy(x0):
+        r = np.linalg.norm(x0 - p1)
+        return 0.5 * 40.0 * (1.3 - r) ** 2 if r < 1.3 else 0.0
+    h = 1e-6
+    g = np.array([(energy(p0 + h * e) - energy(p0 - h * e)) / (2 * h) for e in np.eye(3)])
+    assert np.allclose(F[0], -g, rtol=1e-7)
+
+
+def test_harmonic_cut_is_strict_and_coincident_atoms_separate():
+    F = two_lj([5.0, 5.0, 5.0], [6.5, 5.0, 5.0], pairs=((O.PAIR_HARMONIC, 1.5, 40.0),))
+    assert np.abs(F).max() == 0.0                      # r == cut: no force
+    F = two_lj([5.0, 5.0, 5.0], [6.6, 5.0, 5.0], pairs=(HARM,))
+    assert np.abs(F).max() == 0.0
+    # coincident atoms: pushed apart along (1, 1, 1) as if 0.001 apart on each axis
+    F = two_lj([5.0, 5.0, 5.0], [5.0, 5.0, 5.0], pairs=(HARM,))
+    assert np.allclose(F[0], harmonic_force(np.full(3, 0.001)), rtol=1e-13)
+    assert np.array_equal(F[0], -F[1])
+
+
+def test_harmonic_across_x_boundary_and_sum_with_lj():
+    F = two_lj([0.3, 5.0, 5.0], [15.5, 5.0, 5.0], pairs=(HARM,))
+    assert np.allclose(F[0], harmonic_force(np.array([0.8, 0.0, 0.0])), rtol=1e-12)
+    F = two_lj([5.0, 5.0, 0.3], [5.0, 5.0, 15.5], pairs=(HARM,))   # no z images on one rank
+    assert np.abs(F).max() == 0.0
+    both = two_lj([5.0, 5.0, 5.0], [6.1, 5.2, 4.9], pairs=((O.PAIR_LJCUT, 1.5, 1.0, 1.0), HARM))
+    d = np.array([5.0, 5.0, 5.0]) - np.array([6.1, 5.2, 4.9])
+    assert np.allclose(both[0], lj_force(d) + harmonic_force(d), rtol=1e-13)
 
 
 def test_chain_force_is_minus_energy_gradient():
