@@ -259,6 +259,44 @@ def surrogate_leg(args, wl, phi, xyz, mol):
                     "plain fallback; FP64 fields"}
 
 
+def dump_leg(args, wl, phi, xyz, mol, plain_ms):
+    """The workload with the reference example's dumps (grid: phi, chempot,
+    gradphi_x/y/z; atom/owned: x ux ix F ID molID type) every K steps,
+    written synchronously and asynchronously (the snapshot in HBM, the copy
+    to the host and the file write overlapping the next steps)."""
+    import shutil
+    import tempfile
+    import phafd_amd as P
+    from phafd_amd.dump import Dump, run_with_dumps
+    n = wl["n"]
+    res = {"every": args.dump_every, "steps": args.steps, "ms_per_step_no_dumps": plain_ms}
+    for mode in ("sync", "async"):
+        d = tempfile.mkdtemp(prefix="phafd_dump_", dir=args.dump_dir)
+        try:
+            s = P.System(n, n, n, [0, 0, 0], [wl["box"]] * 3, zghost=wl["zghost"])
+            configure(s, wl, phi, xyz, mol)
+            s.run(args.warmup)
+            grid = Dump(s, "realgrid", "grid", os.path.join(d, "field"), args.dump_every,
+                        ["phi", "chempot", "gradphi_x", "gradphi_y", "gradphi_z"], async_=(mode == "async"))
+            atoms = Dump(s, "atoms", "atom/owned", os.path.join(d, "atom"), args.dump_every,
+                         ["x", "ux", "ix", "F", "ID", "molID", "type"], async_=(mode == "async"))
+            for dm in (grid, atoms):
+                dm.write_collection_header()
+            t0 = time.perf_counter()
+            run_with_dumps(s, args.steps, [grid, atoms], 1e-3, first_step=args.warmup)
+            s.dump_wait()
+            el = time.perf_counter() - t0
+            for dm in (grid, atoms):
+                dm.write_collection_footer()
+            files = [os.path.join(d, f) for f in os.listdir(d)]
+            res[f"ms_per_step_{mode}"] = 1e3 * el / args.steps
+            res["bytes_written"] = sum(os.path.getsize(f) for f in files)
+            s.close()
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    return res
+
+
 def bench_batch(args):
     """Batched replicas: the batch timed over `steps`, then the same members
     run one after another as solo systems (the state of every member is
@@ -361,6 +399,9 @@ def main():
                     help="steps of the timed plain-vs-surrogate comparison (0: --steps); rounded up to a multiple "
                          "of the network's K")
     ap.add_argument("--timers", action="store_true", help="print per-hook GPU times to stderr")
+    ap.add_argument("--dump-every", type=int, default=0, metavar="K",
+                    help="also time the run with grid and atom dumps every K steps (sync and async)")
+    ap.add_argument("--dump-dir", default=None, help="directory for the dump leg's files (default: system temp)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -451,6 +492,8 @@ def main():
         out["fp32"] = fp32
     if sur is not None:
         out["surrogate"] = sur
+    if args.dump_every > 0:
+        out["dumps"] = dump_leg(args, wl, phi, xyz, mol, ms_per_step)
     if args.timers:
         print(json.dumps({"per_hook_ms_total": timers}), file=sys.stderr)
     if not args.no_cpu_baseline:

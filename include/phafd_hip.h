@@ -86,10 +86,33 @@ int phafd_hip_get_spectrum(phafd_hip_ctx *ctx, int which, double *host);
  *      (src/atom.cpp:26-110,140-252, src/read_atoms.cpp:58-97) */
 int phafd_hip_set_atoms(phafd_hip_ctx *ctx, int n, const double *xyz, const int *type,
                         const int *tag, const int *mol);
+/* Atom::images of the owned atoms as read by read_dump (src/read_dump.cpp:
+ * 240-244), in Domain's packed image bits; after phafd_hip_set_atoms, which
+ * sets every image to zero (src/atom.cpp:245), and before phafd_hip_setup */
+int phafd_hip_set_images(phafd_hip_ctx *ctx, const int *image);
 /* Atom::xs, uxs, images (src/atom.hpp:38-47); any pointer may be NULL */
 int phafd_hip_get_atoms(phafd_hip_ctx *ctx, double *xyz, double *uxyz, int *image);
 /* Atom::Fs of owned atoms after reverse_comm (src/comm_brick.cpp:228-258) */
 int phafd_hip_get_forces(phafd_hip_ctx *ctx, double *f);
+
+/* ---- dumps and restart (formats and attribute names of include/phafd_host.h).
+ * dump <name> grid <file> ... attributes n phi chempot gradphi_x ...:
+ * Dump::write_grid_timestep (src/dump.cpp:681-780) of the current fields.
+ * dump <name> atom/owned <file> ... attributes n x ux ix F ID molID type
+ * [decimalplaces p] [fmt lf|e]: Dump::write_atom_timestep (src/dump.cpp:
+ * 360-418).  The arrays are snapshotted in device memory on the step stream,
+ * so with async != 0 the call returns at once and later steps may run while
+ * the copy to the host and the file write proceed; the next dump call, a
+ * read_dump or phafd_hip_dump_wait() waits for it and reports its error. */
+int phafd_hip_dump_grid(phafd_hip_ctx *ctx, const char *path, int nattr, const char *const *names, int async);
+int phafd_hip_dump_atoms(phafd_hip_ctx *ctx, const char *path, int nattr, const char *const *names, int precision,
+                         int fmt, int async);
+int phafd_hip_dump_wait(phafd_hip_ctx *ctx);
+/* grid_populate read_dump grid <file> attributes n phi ... [no_padding]:
+ * ReadDump::process_grid_attributes (src/read_dump.cpp:103-195) into the
+ * context's fields (phi must be among the attributes) */
+int phafd_hip_read_dump_grid(phafd_hip_ctx *ctx, const char *path, int nattr, const char *const *names,
+                             int no_padding);
 
 /* ---- pair styles, in script order (src/input.cpp:99-153).
  * pair gridatom/gaussian|gridatom/LJish|gridatom/LJish/linear <cut>:

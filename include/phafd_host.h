@@ -39,6 +39,42 @@ int phafd_ic_random_chains(int nchains, int nbeads, double bond, double minsep,
                            const double *boxlo, const double *period, uint64_t seed,
                            double *xyz, int *mol);
 
+/* ---- dump files (the reference's VTK formats, one subdomain) ----------
+ * Attribute names are the reference's: grid phi, chempot, gradphi_x,
+ * gradphi_y, gradphi_z; atoms x, F, ux (three doubles per atom) and ix, ID,
+ * molID, type, label (one int per atom).  On error the functions return a
+ * negative value and phafd_host_last_error() gives the message. */
+enum { PHAFD_VTK_FMT_LF = 0, PHAFD_VTK_FMT_E = 1 };   /* dump ... fmt lf|e */
+enum { PHAFD_VTK_PVD_HEADER = 0, PHAFD_VTK_PVD_DATASET = 1, PHAFD_VTK_PVD_FOOTER = 2 };
+const char *phafd_host_last_error(void);
+/* Dump::write_grid_timestep (src/dump.cpp:681-780): ImageData with the
+ * arrays ([Nz][Ny][Nx] doubles each) appended raw after 32-bit byte lengths;
+ * one rank, so the extent is 0..Nz-1 and each length is Nx*Ny*Nz doubles */
+int phafd_vtk_write_grid(const char *path, int nx, int ny, int nz, const double *origin, const double *spacing,
+                         int nattr, const char *const *names, const double *const *arrays);
+/* Dump::write_grid_timestep_pvti (src/dump.cpp:782-855) */
+int phafd_vtk_write_pgrid(const char *path, int nx, int ny, int nz, const double *origin, const double *spacing,
+                          int nattr, const char *const *names, const char *source);
+/* Dump::write_atom_timestep, atom/owned (src/dump.cpp:360-418, 978-1118);
+ * precision <= 0: the stream default; fmt PHAFD_VTK_FMT_* */
+int phafd_vtk_write_atoms(const char *path, int n, int nattr, const char *const *names, const void *const *arrays,
+                          int precision, int fmt);
+/* Dump::write_atom_timestep_pvtp (src/dump.cpp:420-477) */
+int phafd_vtk_write_patoms(const char *path, int nattr, const char *const *names, const char *source);
+/* Dump::write_collection_header / _middle / _footer (src/dump.cpp:226-300):
+ * part PHAFD_VTK_PVD_*; time and file are used by the data-set line */
+int phafd_vtk_collection(const char *path, int part, double time, const char *file);
+/* ReadDump grid (src/read_dump.cpp:103-195, 300-338): out[k] (Nx*Ny*Nz
+ * doubles, may be null to skip) receives requested attribute k; phi must be
+ * requested */
+int phafd_vtk_read_grid(const char *path, int nx, int ny, int nz, int nattr, const char *const *names,
+                        double *const *out, int no_padding);
+/* ReadDump atoms (src/read_dump.cpp:199-296): out[k] receives requested
+ * attribute k for natoms atoms; x, type, ID (and molID, ix when molecule)
+ * must be requested; returns natoms */
+int phafd_vtk_read_atoms(const char *path, int natoms, int nattr, const char *const *names, int molecule,
+                         void *const *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,6 +1,6 @@
 """Build recipe for the CPU FP64 oracle (test infrastructure only).
 
-Compiles oracle/oracle.cpp into oracle/liboracle.so with g++ -O3
+Compiles oracle/oracle.cpp and oracle/oracle_vtk.cpp into oracle/liboracle.so with g++ -O3
 -ffp-contract=off (no FMA contraction, so the operation order written in the
 restatement is the one executed).  The reference itself cannot be built here:
 it needs FFTW3-MPI, Eigen3, fftw_arr and beadrodpmer, none of which exist in
@@ -15,10 +15,10 @@ LIB = os.path.join(HERE, "liboracle.so")
 
 
 def build(verbose=False):
-    src = os.path.join(HERE, "oracle.cpp")
+    srcs = [os.path.join(HERE, f) for f in ("oracle.cpp", "oracle_vtk.cpp")]
     tmp = LIB + ".tmp"
     cmd = ["g++", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
-           "-fPIC", "-shared", "-o", tmp, src]
+           "-fPIC", "-shared", "-o", tmp] + srcs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

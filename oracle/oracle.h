@@ -98,6 +98,18 @@ enum {
 };
 int orc_phase(orc_sys *, int phase);
 
+/* ---- dump writers (oracle_vtk.cpp): the reference's std::ostream output
+ * of src/dump.cpp for one rank; arguments as phafd_vtk_write_* ---- */
+const char *orc_vtk_last_error(void);
+int orc_vtk_write_grid(const char *path, int nx, int ny, int nz, const double *origin, const double *spacing,
+                       int nattr, const char *const *names, const double *const *arrays);
+int orc_vtk_write_pgrid(const char *path, int nx, int ny, int nz, const double *origin, const double *spacing,
+                        int nattr, const char *const *names, const char *source);
+int orc_vtk_write_atoms(const char *path, int n, int nattr, const char *const *names, const void *const *arrays,
+                        int precision, int fmt);
+int orc_vtk_write_patoms(const char *path, int nattr, const char *const *names, const char *source);
+int orc_vtk_collection(const char *path, int part, double time, const char *file);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,15 @@ def lib():
         L.orc_mt19937_uniform.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
         L.orc_make_unique_seed.argtypes = [ctypes.c_int]
         L.orc_mt19937_raw.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
+        _cp, _pp = ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p)
+        _vpp = ctypes.POINTER(ctypes.c_void_p)
+        L.orc_vtk_last_error.restype = ctypes.c_char_p
+        _geo = [_cp] + [ctypes.c_int] * 3 + [_dp, _dp, ctypes.c_int, _pp]
+        L.orc_vtk_write_grid.argtypes = _geo + [ctypes.POINTER(_dp)]
+        L.orc_vtk_write_pgrid.argtypes = _geo + [_cp]
+        L.orc_vtk_write_atoms.argtypes = [_cp, ctypes.c_int, ctypes.c_int, _pp, _vpp, ctypes.c_int, ctypes.c_int]
+        L.orc_vtk_write_patoms.argtypes = [_cp, ctypes.c_int, _pp, _cp]
+        L.orc_vtk_collection.argtypes = [_cp, ctypes.c_int, ctypes.c_double, _cp]
         _lib = L
     return _lib
 
@@ -309,3 +318,44 @@ class System:
         o = np.zeros(4)
         self._chk(lib().orc_energy(self.h, dptr(o)))
         return o
+
+
+# ------------------------------------------------------------ dump writers
+def _names(names):
+    return (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+
+
+def _vtk(rc):
+    if rc < 0:
+        raise RuntimeError(lib().orc_vtk_last_error().decode())
+
+
+def vtk_write_grid(path, arrays, names, origin, spacing):
+    """Dump::write_grid_timestep restated with std::ostream (one rank)."""
+    arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    nz, ny, nx = arrays[0].shape
+    ptrs = (_dp * len(arrays))(*[a.ctypes.data_as(_dp) for a in arrays])
+    _vtk(lib().orc_vtk_write_grid(path.encode(), nx, ny, nz, (ctypes.c_double * 3)(*origin),
+                                  (ctypes.c_double * 3)(*spacing), len(names), _names(names), ptrs))
+
+
+def vtk_write_pgrid(path, shape, names, origin, spacing, source):
+    nz, ny, nx = shape
+    _vtk(lib().orc_vtk_write_pgrid(path.encode(), nx, ny, nz, (ctypes.c_double * 3)(*origin),
+                                   (ctypes.c_double * 3)(*spacing), len(names), _names(names), source.encode()))
+
+
+def vtk_write_atoms(path, arrays, names, precision=-1, fmt=0):
+    """arrays: per name, (n, 3) float64 for x/F/ux, (n,) int32 otherwise."""
+    conv = [np.ascontiguousarray(a, dtype=np.float64 if a.ndim == 2 else np.int32) for a in arrays]
+    n = len(conv[0])
+    ptrs = (ctypes.c_void_p * len(conv))(*[a.ctypes.data for a in conv])
+    _vtk(lib().orc_vtk_write_atoms(path.encode(), n, len(names), _names(names), ptrs, precision, fmt))
+
+
+def vtk_write_patoms(path, names, source):
+    _vtk(lib().orc_vtk_write_patoms(path.encode(), len(names), _names(names), source.encode()))
+
+
+def vtk_collection(path, part, time=0.0, file=""):
+    _vtk(lib().orc_vtk_collection(path.encode(), part, time, file.encode()))

@@ -42,6 +42,11 @@ class Desc(ctypes.Structure):
                 ("zghost", ctypes.c_int), ("precision", ctypes.c_int), ("device", ctypes.c_int)]
 
 
+_cp, _pp = ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p)
+_vpp = ctypes.POINTER(ctypes.c_void_p)
+_GEO = [_cp] + [ctypes.c_int] * 3 + [_dp, _dp, ctypes.c_int, _pp]
+
+
 def lib():
     """Load libphafd_hip.so (raises if it has not been built)."""
     global _lib
@@ -61,6 +66,7 @@ def lib():
         "phafd_hip_get_spectrum": ([V, ctypes.c_int, _dp], ctypes.c_int),
         "phafd_hip_set_atoms": ([V, ctypes.c_int, _dp, _ip, _ip, _ip], ctypes.c_int),
         "phafd_hip_get_atoms": ([V, _dp, _dp, _ip], ctypes.c_int),
+        "phafd_hip_set_images": ([V, _ip], ctypes.c_int),
         "phafd_hip_get_forces": ([V, _dp], ctypes.c_int),
         "phafd_hip_pair_gridatom": ([V, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "phafd_hip_pair_coeff": ([V, ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 3, ctypes.c_int),
@@ -129,6 +135,20 @@ def lib():
         "phafd_noise_next": ([V] + [ctypes.c_int] * 3 + [_dp] + [ctypes.c_double] * 3 + [_dp], ctypes.c_int),
         "phafd_ic_random_chains": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
                                     ctypes.c_uint64, _dp, _ip], ctypes.c_int),
+        "phafd_host_last_error": ([], ctypes.c_char_p),
+        "phafd_vtk_write_grid": (_GEO + [ctypes.POINTER(_dp)], ctypes.c_int),
+        "phafd_vtk_write_pgrid": (_GEO + [_cp], ctypes.c_int),
+        "phafd_vtk_write_atoms": ([_cp, ctypes.c_int, ctypes.c_int, _pp, _vpp, ctypes.c_int, ctypes.c_int],
+                                  ctypes.c_int),
+        "phafd_vtk_write_patoms": ([_cp, ctypes.c_int, _pp, _cp], ctypes.c_int),
+        "phafd_vtk_collection": ([_cp, ctypes.c_int, ctypes.c_double, _cp], ctypes.c_int),
+        "phafd_vtk_read_grid": ([_cp] + [ctypes.c_int] * 4 + [_pp, ctypes.POINTER(_dp), ctypes.c_int],
+                                ctypes.c_int),
+        "phafd_vtk_read_atoms": ([_cp, ctypes.c_int, ctypes.c_int, _pp, ctypes.c_int, _vpp], ctypes.c_int),
+        "phafd_hip_dump_grid": ([V, _cp, ctypes.c_int, _pp, ctypes.c_int], ctypes.c_int),
+        "phafd_hip_dump_atoms": ([V, _cp, ctypes.c_int, _pp] + [ctypes.c_int] * 3, ctypes.c_int),
+        "phafd_hip_dump_wait": ([V], ctypes.c_int),
+        "phafd_hip_read_dump_grid": ([V, _cp, ctypes.c_int, _pp, ctypes.c_int], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -192,6 +212,88 @@ def ic_random_chains(nchains, nbeads, bond, minsep, boxlo, period, seed):
     pe = np.ascontiguousarray(period, dtype=np.float64)
     _check(lib().phafd_ic_random_chains(nchains, nbeads, bond, minsep, _d(lo), _d(pe), seed, _d(xyz), _i(mol)))
     return xyz, mol
+
+
+# ------------------------------------------------------------ dump files
+VTK_FMT_LF, VTK_FMT_E = 0, 1
+PVD_HEADER, PVD_DATASET, PVD_FOOTER = 0, 1, 2
+GRID_ATTRIBUTES = ("phi", "chempot", "gradphi_x", "gradphi_y", "gradphi_z")
+ATOM_VEC3 = ("x", "F", "ux")
+ATOM_INT = ("ix", "ID", "molID", "type", "label")
+
+
+def _names(names):
+    if isinstance(names, str):
+        names = [names]
+    return len(names), (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+
+
+def _host(rc):
+    if rc < 0:
+        raise PhafdError(rc, (lib().phafd_host_last_error() or b"").decode())
+    return rc
+
+
+def _fmt(fmt):
+    if fmt in ("lf", VTK_FMT_LF):
+        return VTK_FMT_LF
+    if fmt in ("e", VTK_FMT_E):
+        return VTK_FMT_E
+    raise ValueError("Dump fmt must be either lf or e.")
+
+
+def vtk_write_grid(path, arrays, names, origin, spacing):
+    """Dump::write_grid_timestep of host arrays ([Nz][Ny][Nx] each)."""
+    arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    nz, ny, nx = arrays[0].shape
+    n, nm = _names(names)
+    ptrs = (_dp * len(arrays))(*[_d(a) for a in arrays])
+    _host(lib().phafd_vtk_write_grid(str(path).encode(), nx, ny, nz, _c3(origin), _c3(spacing), n, nm, ptrs))
+
+
+def vtk_write_pgrid(path, shape, names, origin, spacing, source):
+    nz, ny, nx = shape
+    n, nm = _names(names)
+    _host(lib().phafd_vtk_write_pgrid(str(path).encode(), nx, ny, nz, _c3(origin), _c3(spacing), n, nm,
+                                      source.encode()))
+
+
+def vtk_write_atoms(path, arrays, names, precision=-1, fmt="lf"):
+    """Dump::write_atom_timestep (atom/owned); arrays per name: (n, 3) float64
+    for x, F, ux and (n,) int32 for ix, ID, molID, type, label."""
+    conv = [np.ascontiguousarray(a, dtype=np.float64 if w in ATOM_VEC3 else np.int32)
+            for a, w in zip(arrays, names)]
+    n, nm = _names(names)
+    ptrs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in conv])
+    _host(lib().phafd_vtk_write_atoms(str(path).encode(), len(conv[0]), n, nm, ptrs, precision, _fmt(fmt)))
+
+
+def vtk_write_patoms(path, names, source):
+    n, nm = _names(names)
+    _host(lib().phafd_vtk_write_patoms(str(path).encode(), n, nm, source.encode()))
+
+
+def vtk_collection(path, part, time=0.0, file=""):
+    _host(lib().phafd_vtk_collection(str(path).encode(), part, time, file.encode()))
+
+
+def vtk_read_grid(path, shape, names, no_padding=False):
+    """ReadDump grid: the requested attributes of a .vti file, in request order."""
+    n, nm = _names(names)
+    out = [np.empty(shape) for _ in range(n)]
+    ptrs = (_dp * n)(*[_d(a) for a in out])
+    nz, ny, nx = shape
+    _host(lib().phafd_vtk_read_grid(str(path).encode(), nx, ny, nz, n, nm, ptrs, int(no_padding)))
+    return out
+
+
+def read_dump_atoms(path, natoms, names, molecule=True):
+    """ReadDump atoms (atom_populate read_dump): {name: array} of natoms atoms."""
+    n, nm = _names(names)
+    out = {w: (np.empty((natoms, 3)) if w in ATOM_VEC3 else np.empty(natoms, np.int32)) for w in names}
+    ptrs = (ctypes.c_void_p * n)(*[out[w].ctypes.data for w in names])
+    _host(lib().phafd_vtk_read_atoms(str(path).encode(), natoms, n, nm, int(molecule), ptrs))
+    return out
 
 
 class NoiseGen:
@@ -270,6 +372,8 @@ class System:
     def __init__(self, nx, ny, nz, boxlo, period, zghost=ZGHOST_REFERENCE, precision=FP64, device=0,
                  _member=None):
         self.shape = (nz, ny, nx)
+        self.boxlo = [float(v) for v in boxlo]
+        self.period = [float(v) for v in period]
         self.natoms = 0
         if _member is not None:   # a batch member: the batch owns the context
             self.h, self.owned = _member, False
@@ -318,6 +422,10 @@ class System:
         self.natoms = n
         self._c(lib().phafd_hip_set_atoms(self.h, n, _d(xyz), _i(t), _i(tg), _i(ml)))
 
+    def set_images(self, image):
+        a = np.ascontiguousarray(image, dtype=np.int32)
+        self._c(lib().phafd_hip_set_images(self.h, _i(a)))
+
     def get_atoms(self):
         x = np.zeros((self.natoms, 3))
         u = np.zeros((self.natoms, 3))
@@ -329,6 +437,25 @@ class System:
         f = np.zeros((self.natoms, 3))
         self._c(lib().phafd_hip_get_forces(self.h, _d(f)))
         return f
+
+    def dump_grid(self, path, names, async_=False):
+        """dump ... grid: the current fields to a .vti file (async_: the write
+        overlaps later steps; dump_wait() or the next dump reports errors)."""
+        n, nm = _names(names)
+        self._c(lib().phafd_hip_dump_grid(self.h, str(path).encode(), n, nm, int(async_)))
+
+    def dump_atoms(self, path, names, precision=-1, fmt="lf", async_=False):
+        """dump ... atom/owned: the owned atoms to a .vtp file."""
+        n, nm = _names(names)
+        self._c(lib().phafd_hip_dump_atoms(self.h, str(path).encode(), n, nm, precision, _fmt(fmt), int(async_)))
+
+    def dump_wait(self):
+        self._c(lib().phafd_hip_dump_wait(self.h))
+
+    def read_dump_grid(self, path, names, no_padding=False):
+        """grid_populate read_dump: fields from a .vti file."""
+        n, nm = _names(names)
+        self._c(lib().phafd_hip_read_dump_grid(self.h, str(path).encode(), n, nm, int(no_padding)))
 
     def pair_gridatom(self, style, cut):
         return self._c(lib().phafd_hip_pair_gridatom(self.h, style, cut))

@@ -2,13 +2,16 @@
 // the per-timestep hook sequence of Integrate::run (src/integrate.cpp:105-203)
 // and the C ABI declared in include/phafd_hip.h.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <set>
+#include <thread>
 
 #include "../../include/phafd_hip.h"
+#include "../../include/phafd_host.h"
 #include "atoms.hpp"
 #include "internal.hpp"
 #include "unet.hpp"
@@ -57,6 +60,19 @@ struct SwapDev {
   // the dimension (lo of this swap above hi of the first), so their reverse
   // folds touch different targets and may share one launch
   bool disjoint_pair = false;
+};
+
+// a dump in flight: the fields or atom arrays snapshotted into HBM on the
+// step stream, copied to pinned host memory on the dump stream, written to
+// disk by the writer thread (while the steps go on when asynchronous)
+struct DumpJob {
+  DBuf<double> stage;
+  char *host = nullptr;
+  size_t host_bytes = 0;
+  hipStream_t cs = nullptr;
+  hipEvent_t snap = nullptr, done = nullptr;
+  std::thread th;
+  std::string err;
 };
 
 }  // namespace
@@ -171,6 +187,7 @@ struct phafd_hip_ctx {
   DBuf<int> sg_win;               // speculative atom-step windows: [0] stop step, [1] OR of decide flags
   DBuf<double4> sg_fga, sg_x, sg_ux;
   DBuf<int> sg_img;
+  DumpJob dump;
 
   double dx() const { return period[0] / nx; }   // src/grid.cpp:187-201
   double dy() const { return period[1] / ny; }
@@ -1300,6 +1317,12 @@ int phafd_hip_destroy(phafd_hip_ctx *c) {
   for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
   for (auto &e : c->ev_pre) if (e) (void)hipEventDestroy(e);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
+  if (c->dump.th.joinable()) c->dump.th.join();
+  if (c->dump.cs) (void)hipStreamSynchronize(c->dump.cs);
+  if (c->dump.host) (void)hipHostFree(c->dump.host);
+  if (c->dump.snap) (void)hipEventDestroy(c->dump.snap);
+  if (c->dump.done) (void)hipEventDestroy(c->dump.done);
+  if (c->dump.cs) (void)hipStreamDestroy(c->dump.cs);
   if (c->stream && c->own_stream) {
     scan_forget_stream(c->stream);
     (void)hipStreamDestroy(c->stream);
@@ -1384,6 +1407,16 @@ int phafd_hip_set_atoms(phafd_hip_ctx *c, int n, const double *xyz, const int *t
     return 0;
   });
 }
+int phafd_hip_set_images(phafd_hip_ctx *c, const int *image) {
+  return guard(c, [&] {
+    if (!c->atoms_set) throw std::runtime_error("set the atoms before their images");
+    if (c->nowned && !image) throw std::invalid_argument("null image array");
+    if (c->nowned) HIPCHK(hipMemcpy(c->image.p, image, sizeof(int) * c->nowned, hipMemcpyHostToDevice));
+    c->setup_done = false;
+    c->lists_valid = false;
+    return 0;
+  });
+}
 int phafd_hip_get_atoms(phafd_hip_ctx *c, double *xyz, double *uxyz, int *image) {
   return guard(c, [&] {
     if (xyz) from_d4(download(c, c->x.p, c->nowned), xyz);
@@ -1398,6 +1431,218 @@ int phafd_hip_get_atoms(phafd_hip_ctx *c, double *xyz, double *uxyz, int *image)
 int phafd_hip_get_forces(phafd_hip_ctx *c, double *f) {
   return guard(c, [&] {
     from_d4(download(c, c->F.p, c->nowned), f);
+    return 0;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+__global__ void k_widen(size_t n, const float *__restrict__ a, double *__restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+// grid attributes of the dump and read_dump commands (src/dump.cpp:941-955,
+// src/read_dump.cpp:170-185)
+int grid_attribute(const std::string &w) {
+  if (w == "phi") return PHAFD_PHI;
+  if (w == "chempot") return PHAFD_MU;
+  if (w == "gradphi_x") return PHAFD_GRADX;
+  if (w == "gradphi_y") return PHAFD_GRADY;
+  if (w == "gradphi_z") return PHAFD_GRADZ;
+  throw std::runtime_error("Dump error: Attribute does not exist.");
+}
+
+std::vector<std::string> attribute_list(int nattr, const char *const *names, const char *what) {
+  if (nattr <= 0 || !names) throw std::invalid_argument(std::string("Need attributes to ") + what + " dump files.");
+  std::vector<std::string> v;
+  for (int k = 0; k < nattr; k++) {
+    if (!names[k]) throw std::invalid_argument("null attribute name");
+    v.emplace_back(names[k]);
+  }
+  return v;
+}
+
+// the previous dump's writer finished; its error, if any, is this call's
+void dump_wait(phafd_hip_ctx *c) {
+  DumpJob &d = c->dump;
+  if (d.th.joinable()) d.th.join();
+  if (!d.err.empty()) {
+    std::string e;
+    e.swap(d.err);
+    throw std::runtime_error(e);
+  }
+}
+
+void dump_prepare(phafd_hip_ctx *c, size_t bytes) {
+  DumpJob &d = c->dump;
+  if (!d.cs) {
+    HIPCHK(hipStreamCreateWithFlags(&d.cs, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&d.snap, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&d.done, hipEventDisableTiming));
+  }
+  if (d.host_bytes < bytes) {
+    if (d.host) HIPCHK(hipHostFree(d.host));
+    d.host = nullptr;
+    d.host_bytes = 0;
+    HIPCHK(hipHostMalloc((void **)&d.host, bytes, hipHostMallocDefault));
+    d.host_bytes = bytes;
+  }
+  d.stage.alloc((bytes + sizeof(double) - 1) / sizeof(double));
+}
+
+// stage -> pinned host on the dump stream, after what the step stream has
+// enqueued so far (the snapshot); then the write, on the writer thread or here
+template <class W>
+void dump_launch(phafd_hip_ctx *c, size_t bytes, bool async, W &&write) {
+  DumpJob &d = c->dump;
+  HIPCHK(hipEventRecord(d.snap, c->stream));
+  HIPCHK(hipStreamWaitEvent(d.cs, d.snap, 0));
+  HIPCHK(hipMemcpyAsync(d.host, d.stage.p, bytes, hipMemcpyDeviceToHost, d.cs));
+  HIPCHK(hipEventRecord(d.done, d.cs));
+  auto job = [&d, w = std::forward<W>(write)]() mutable {
+    try {
+      HIPCHK(hipEventSynchronize(d.done));
+      w(static_cast<const char *>(d.host));
+    } catch (const std::exception &e) {
+      d.err = e.what();
+    }
+  };
+  if (async) {
+    d.th = std::thread(std::move(job));
+  } else {
+    job();
+    dump_wait(c);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int phafd_hip_dump_grid(phafd_hip_ctx *c, const char *path, int nattr, const char *const *names, int async) {
+  return guard(c, [&] {
+    dump_wait(c);
+    if (!path) throw std::invalid_argument("null file name");
+    auto at = attribute_list(nattr, names, "write");
+    std::vector<int> ids;
+    for (auto &w : at) ids.push_back(grid_attribute(w));
+    const size_t N = c->N, bytes = (size_t)nattr * N * sizeof(double);
+    dump_prepare(c, bytes);
+    for (int k = 0; k < nattr; k++) {
+      double *dst = c->dump.stage.p + (size_t)k * N;
+      if (!c->fp32) {
+        HIPCHK(hipMemcpyAsync(dst, field_ptr<double>(c, ids[k]), N * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
+      } else {
+        const int blocks = (int)std::min<size_t>((N + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c->stream, N, field_ptr<float>(c, ids[k]), dst);
+        HIPCHK(hipGetLastError());
+      }
+    }
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    const std::array<double, 3> origin{c->boxlo[0], c->boxlo[1], c->boxlo[2]};
+    const std::array<double, 3> spacing{c->dx(), c->dy(), c->dz()};
+    dump_launch(c, bytes, async != 0, [=, file = std::string(path)](const char *host) {
+      std::vector<const char *> nm;
+      std::vector<const double *> arr;
+      for (int k = 0; k < nattr; k++) {
+        nm.push_back(at[k].c_str());
+        arr.push_back(reinterpret_cast<const double *>(host) + (size_t)k * N);
+      }
+      if (phafd_vtk_write_grid(file.c_str(), nx, ny, nz, origin.data(), spacing.data(), nattr, nm.data(),
+                               arr.data()) < 0)
+        throw std::runtime_error(phafd_host_last_error());
+    });
+    return 0;
+  });
+}
+
+int phafd_hip_dump_atoms(phafd_hip_ctx *c, const char *path, int nattr, const char *const *names, int precision,
+                         int fmt, int async) {
+  return guard(c, [&] {
+    dump_wait(c);
+    if (!path) throw std::invalid_argument("null file name");
+    if (fmt != PHAFD_VTK_FMT_LF && fmt != PHAFD_VTK_FMT_E)
+      throw std::invalid_argument("Dump fmt must be either lf or e.");
+    auto at = attribute_list(nattr, names, "write");
+    bool has_x = false;
+    for (auto &w : at) {
+      if (w != "x" && w != "F" && w != "ux" && w != "ix" && w != "ID" && w != "molID" && w != "type" && w != "label")
+        throw std::runtime_error("Invalid per atom quantity in dump file.");
+      has_x |= (w == "x");
+    }
+    if (!has_x) throw std::runtime_error("Atom dump must contain positions.");
+    // device arrays in the stage: x, ux, F (double4 each), image (int)
+    const size_t n = (size_t)c->nowned, v4 = n * sizeof(double4);
+    const size_t bytes = 3 * v4 + n * sizeof(int) + 1;
+    dump_prepare(c, bytes);
+    char *st = reinterpret_cast<char *>(c->dump.stage.p);
+    if (n) {
+      HIPCHK(hipMemcpyAsync(st, c->x.p, v4, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(st + v4, c->ux.p, v4, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(st + 2 * v4, c->F.p, v4, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(st + 3 * v4, c->image.p, n * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    }
+    dump_launch(c, bytes, async != 0,
+                [=, file = std::string(path), tag = c->h_tag, mol = c->h_mol, type = c->h_type](const char *host) {
+                  auto xyz = [&](int which) {
+                    std::vector<double> v(3 * n);
+                    const double4 *s = reinterpret_cast<const double4 *>(host + which * v4);
+                    for (size_t i = 0; i < n; i++) {
+                      v[3 * i] = s[i].x; v[3 * i + 1] = s[i].y; v[3 * i + 2] = s[i].z;
+                    }
+                    return v;
+                  };
+                  std::vector<std::vector<double>> vec;
+                  std::vector<int> img(n), label(n, 0);   // every owned atom is OWNED (0)
+                  if (n) std::memcpy(img.data(), host + 3 * v4, n * sizeof(int));
+                  std::vector<const char *> nm;
+                  std::vector<const void *> arr;
+                  vec.reserve(nattr);
+                  for (auto &w : at) {
+                    nm.push_back(w.c_str());
+                    if (w == "x") { vec.push_back(xyz(0)); arr.push_back(vec.back().data()); }
+                    else if (w == "ux") { vec.push_back(xyz(1)); arr.push_back(vec.back().data()); }
+                    else if (w == "F") { vec.push_back(xyz(2)); arr.push_back(vec.back().data()); }
+                    else if (w == "ix") arr.push_back(img.data());
+                    else if (w == "ID") arr.push_back(tag.data());
+                    else if (w == "molID") arr.push_back(mol.data());
+                    else if (w == "type") arr.push_back(type.data());
+                    else arr.push_back(label.data());
+                  }
+                  if (phafd_vtk_write_atoms(file.c_str(), (int)n, nattr, nm.data(), arr.data(), precision, fmt) < 0)
+                    throw std::runtime_error(phafd_host_last_error());
+                });
+    return 0;
+  });
+}
+
+int phafd_hip_dump_wait(phafd_hip_ctx *c) {
+  return guard(c, [&] {
+    dump_wait(c);
+    return 0;
+  });
+}
+
+int phafd_hip_read_dump_grid(phafd_hip_ctx *c, const char *path, int nattr, const char *const *names,
+                             int no_padding) {
+  return guard(c, [&] {
+    dump_wait(c);
+    auto at = attribute_list(nattr, names, "read");
+    std::vector<int> ids;
+    for (auto &w : at) {
+      if (w.rfind("c_", 0) == 0 || w.rfind("f_", 0) == 0) throw std::runtime_error("Cannot read computes from dump");
+      ids.push_back(grid_attribute(w));
+    }
+    std::vector<std::vector<double>> buf(nattr, std::vector<double>(c->N));
+    std::vector<double *> out;
+    for (auto &b : buf) out.push_back(b.data());
+    if (phafd_vtk_read_grid(path, c->nx, c->ny, c->nz, nattr, names, out.data(), no_padding) < 0)
+      throw std::runtime_error(phafd_host_last_error());
+    for (int k = 0; k < nattr; k++) field_in(c, ids[k], buf[k].data());
     return 0;
   });
 }
