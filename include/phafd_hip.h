@@ -95,6 +95,14 @@ int phafd_hip_get_atoms(phafd_hip_ctx *ctx, double *xyz, double *uxyz, int *imag
 /* Atom::Fs of owned atoms after reverse_comm (src/comm_brick.cpp:228-258) */
 int phafd_hip_get_forces(phafd_hip_ctx *ctx, double *f);
 
+/* compute <id> pair <pairstyle>: ComputePair::end_of_step (src/compute_pair.cpp:
+ * 43-85).  f[nowned][3] receives the forces due to pair style ipair (the index
+ * its creation returned) alone, computed afresh on the current gathered
+ * positions and fields, as at the end of a step; the owned forces are left as
+ * they were.  Like the reference's pair compute, a grid-atom style adds its
+ * chempot deposit to mu once more (mu is zeroed again by the next step). */
+int phafd_hip_compute_pair(phafd_hip_ctx *ctx, int ipair, double *f);
+
 /* ---- dumps and restart (formats and attribute names of include/phafd_host.h).
  * dump <name> grid <file> ... attributes n phi chempot gradphi_x ...:
  * Dump::write_grid_timestep (src/dump.cpp:681-780) of the current fields.
@@ -102,8 +110,9 @@ int phafd_hip_get_forces(phafd_hip_ctx *ctx, double *f);
  * [decimalplaces p] [fmt lf|e]: Dump::write_atom_timestep (src/dump.cpp:
  * 360-418).  The arrays are snapshotted in device memory on the step stream,
  * so with async != 0 the call returns at once and later steps may run while
- * the copy to the host and the file write proceed; the next dump call, a
- * read_dump or phafd_hip_dump_wait() waits for it and reports its error. */
+ * the copy to the host and the file write proceed (a grid and an atom dump
+ * may be in flight together); the next dump of the same kind, a read_dump
+ * or phafd_hip_dump_wait() waits for it and reports its error. */
 int phafd_hip_dump_grid(phafd_hip_ctx *ctx, const char *path, int nattr, const char *const *names, int async);
 int phafd_hip_dump_atoms(phafd_hip_ctx *ctx, const char *path, int nattr, const char *const *names, int precision,
                          int fmt, int async);

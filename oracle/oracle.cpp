@@ -1499,6 +1499,27 @@ int orc_phase(orc_sys *s, int phase) {
     run_phase(s, phase);
   });
 }
+// ComputePair::end_of_step (src/compute_pair.cpp:43-85): the forces due to
+// pair style ipair alone, from a fresh compute on the current gathered
+// positions and fields plus reverse_comm; the owned forces are then put
+// back.  A grid-atom style's compute adds its deposit to mu again, as the
+// reference's pair compute does (mu is not zeroed first).
+int orc_compute_pair(orc_sys *s, int ipair, double *f) {
+  return guard(s, [&] {
+    if (ipair < 0 || ipair >= (int)s->pairs.size())
+      throw std::invalid_argument("Cannot compute pair style, it hasn't been created!");
+    const size_t no = 3 * (size_t)s->nowned;
+    std::vector<double> keep(s->Fs.begin(), s->Fs.begin() + no);
+    std::fill(s->Fs.begin(), s->Fs.end(), 0.0);
+    const PairStyle &p = s->pairs[ipair];
+    if (p.style == ORC_PAIR_LJCUT) pair_ljcut(s, p, nullptr);
+    else if (p.style == ORC_PAIR_HARMONIC) pair_harmonic(s, p);
+    else pair_gridatom(s, p, nullptr);
+    reverse_comm(s);
+    std::memcpy(f, s->Fs.data(), sizeof(double) * no);
+    std::copy(keep.begin(), keep.end(), s->Fs.begin());
+  });
+}
 int orc_counts(orc_sys *s, int64_t *o) {
   o[0] = s->nowned; o[1] = s->ngathered; o[2] = s->nlocal; o[3] = s->nghost;
   o[4] = s->ncalls; o[5] = s->nless; o[6] = s->ngreat; o[7] = s->list_overflow ? 1 : 0;

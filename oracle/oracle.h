@@ -97,6 +97,8 @@ enum {
   ORC_PH_POSTFINAL = 7 /* c2r phi                                   */
 };
 int orc_phase(orc_sys *, int phase);
+/* ComputePair::end_of_step: owned forces of pair style ipair alone */
+int orc_compute_pair(orc_sys *, int ipair, double *f);
 
 /* ---- dump writers (oracle_vtk.cpp): the reference's std::ostream output
  * of src/dump.cpp for one rank; arguments as phafd_vtk_write_* ---- */

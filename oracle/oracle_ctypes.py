@@ -51,6 +51,7 @@ def lib():
         L.orc_setup.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_phase.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_compute_pair.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.orc_counts.argtypes = [ctypes.c_void_p, _lp]
         for name in ("orc_gridlist", "orc_halflist"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_int64, _ip, _ip, _ip, _lp, _lp]
@@ -226,6 +227,11 @@ class System:
     def get_forces(self):
         f = np.zeros((self.natoms, 3))
         lib().orc_get_forces(self.h, dptr(f))
+        return f
+
+    def compute_pair(self, ipair):
+        f = np.zeros((self.natoms, 3))
+        self._chk(lib().orc_compute_pair(self.h, ipair, dptr(f)))
         return f
 
     def add_pair(self, style, a, b=0.0, c=0.0):

@@ -32,7 +32,9 @@ int vguard(F &&f) {
   }
 }
 
-bool is_vec3(const std::string &w) { return w == "x" || w == "F" || w == "ux"; }
+// x, F, ux (Matrix3Xd) and a per-atom compute of three components: the
+// same header text and value layout (src/dump.cpp:978-1118)
+bool is_vec3(const std::string &w) { return w == "x" || w == "F" || w == "ux" || w.rfind("c_", 0) == 0; }
 }  // namespace
 
 extern "C" {

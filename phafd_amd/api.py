@@ -148,6 +148,7 @@ def lib():
         "phafd_hip_dump_grid": ([V, _cp, ctypes.c_int, _pp, ctypes.c_int], ctypes.c_int),
         "phafd_hip_dump_atoms": ([V, _cp, ctypes.c_int, _pp] + [ctypes.c_int] * 3, ctypes.c_int),
         "phafd_hip_dump_wait": ([V], ctypes.c_int),
+        "phafd_hip_compute_pair": ([V, ctypes.c_int, _dp], ctypes.c_int),
         "phafd_hip_read_dump_grid": ([V, _cp, ctypes.c_int, _pp, ctypes.c_int], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
@@ -260,8 +261,9 @@ def vtk_write_pgrid(path, shape, names, origin, spacing, source):
 
 def vtk_write_atoms(path, arrays, names, precision=-1, fmt="lf"):
     """Dump::write_atom_timestep (atom/owned); arrays per name: (n, 3) float64
-    for x, F, ux and (n,) int32 for ix, ID, molID, type, label."""
-    conv = [np.ascontiguousarray(a, dtype=np.float64 if w in ATOM_VEC3 else np.int32)
+    for x, F, ux and a c_<id> compute pair, (n,) int32 for ix, ID, molID,
+    type, label."""
+    conv = [np.ascontiguousarray(a, dtype=np.float64 if (w in ATOM_VEC3 or w.startswith("c_")) else np.int32)
             for a, w in zip(arrays, names)]
     n, nm = _names(names)
     ptrs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in conv])
@@ -421,6 +423,8 @@ class System:
         ml = None if mols is None else np.ascontiguousarray(mols, dtype=np.int32)
         self.natoms = n
         self._c(lib().phafd_hip_set_atoms(self.h, n, _d(xyz), _i(t), _i(tg), _i(ml)))
+        self.atom_ints = {"type": t, "ID": np.arange(n, dtype=np.int32) if tg is None else tg,
+                          "molID": np.zeros(n, np.int32) if ml is None else ml}
 
     def set_images(self, image):
         a = np.ascontiguousarray(image, dtype=np.int32)
@@ -436,6 +440,12 @@ class System:
     def get_forces(self):
         f = np.zeros((self.natoms, 3))
         self._c(lib().phafd_hip_get_forces(self.h, _d(f)))
+        return f
+
+    def compute_pair(self, ipair):
+        """compute <id> pair <style>: owned forces of pair style ipair alone."""
+        f = np.zeros((self.natoms, 3))
+        self._c(lib().phafd_hip_compute_pair(self.h, ipair, _d(f)))
         return f
 
     def dump_grid(self, path, names, async_=False):

@@ -187,7 +187,8 @@ struct phafd_hip_ctx {
   DBuf<int> sg_win;               // speculative atom-step windows: [0] stop step, [1] OR of decide flags
   DBuf<double4> sg_fga, sg_x, sg_ux;
   DBuf<int> sg_img;
-  DumpJob dump;
+  DumpJob dump[2];   // [0] grid, [1] atoms: the two kinds write concurrently
+  DBuf<double4> cp_keep;   // owned forces kept across a compute pair
 
   double dx() const { return period[0] / nx; }   // src/grid.cpp:187-201
   double dy() const { return period[1] / ny; }
@@ -1317,12 +1318,14 @@ int phafd_hip_destroy(phafd_hip_ctx *c) {
   for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
   for (auto &e : c->ev_pre) if (e) (void)hipEventDestroy(e);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
-  if (c->dump.th.joinable()) c->dump.th.join();
-  if (c->dump.cs) (void)hipStreamSynchronize(c->dump.cs);
-  if (c->dump.host) (void)hipHostFree(c->dump.host);
-  if (c->dump.snap) (void)hipEventDestroy(c->dump.snap);
-  if (c->dump.done) (void)hipEventDestroy(c->dump.done);
-  if (c->dump.cs) (void)hipStreamDestroy(c->dump.cs);
+  for (auto &d : c->dump) {
+    if (d.th.joinable()) d.th.join();
+    if (d.cs) (void)hipStreamSynchronize(d.cs);
+    if (d.host) (void)hipHostFree(d.host);
+    if (d.snap) (void)hipEventDestroy(d.snap);
+    if (d.done) (void)hipEventDestroy(d.done);
+    if (d.cs) (void)hipStreamDestroy(d.cs);
+  }
   if (c->stream && c->own_stream) {
     scan_forget_stream(c->stream);
     (void)hipStreamDestroy(c->stream);
@@ -1465,9 +1468,9 @@ std::vector<std::string> attribute_list(int nattr, const char *const *names, con
   return v;
 }
 
-// the previous dump's writer finished; its error, if any, is this call's
-void dump_wait(phafd_hip_ctx *c) {
-  DumpJob &d = c->dump;
+// the previous dump of the job's kind finished; its error, if any, is this
+// call's
+void dump_wait_job(DumpJob &d) {
   if (d.th.joinable()) d.th.join();
   if (!d.err.empty()) {
     std::string e;
@@ -1475,9 +1478,19 @@ void dump_wait(phafd_hip_ctx *c) {
     throw std::runtime_error(e);
   }
 }
+void dump_wait(phafd_hip_ctx *c) {
+  std::string err;
+  for (auto &d : c->dump) {
+    try {
+      dump_wait_job(d);
+    } catch (const std::exception &e) {
+      if (err.empty()) err = e.what();
+    }
+  }
+  if (!err.empty()) throw std::runtime_error(err);
+}
 
-void dump_prepare(phafd_hip_ctx *c, size_t bytes) {
-  DumpJob &d = c->dump;
+void dump_prepare(DumpJob &d, size_t bytes) {
   if (!d.cs) {
     HIPCHK(hipStreamCreateWithFlags(&d.cs, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&d.snap, hipEventDisableTiming));
@@ -1496,8 +1509,7 @@ void dump_prepare(phafd_hip_ctx *c, size_t bytes) {
 // stage -> pinned host on the dump stream, after what the step stream has
 // enqueued so far (the snapshot); then the write, on the writer thread or here
 template <class W>
-void dump_launch(phafd_hip_ctx *c, size_t bytes, bool async, W &&write) {
-  DumpJob &d = c->dump;
+void dump_launch(phafd_hip_ctx *c, DumpJob &d, size_t bytes, bool async, W &&write) {
   HIPCHK(hipEventRecord(d.snap, c->stream));
   HIPCHK(hipStreamWaitEvent(d.cs, d.snap, 0));
   HIPCHK(hipMemcpyAsync(d.host, d.stage.p, bytes, hipMemcpyDeviceToHost, d.cs));
@@ -1514,7 +1526,7 @@ void dump_launch(phafd_hip_ctx *c, size_t bytes, bool async, W &&write) {
     d.th = std::thread(std::move(job));
   } else {
     job();
-    dump_wait(c);
+    dump_wait_job(d);
   }
 }
 
@@ -1524,15 +1536,16 @@ extern "C" {
 
 int phafd_hip_dump_grid(phafd_hip_ctx *c, const char *path, int nattr, const char *const *names, int async) {
   return guard(c, [&] {
-    dump_wait(c);
+    DumpJob &job = c->dump[0];
+    dump_wait_job(job);
     if (!path) throw std::invalid_argument("null file name");
     auto at = attribute_list(nattr, names, "write");
     std::vector<int> ids;
     for (auto &w : at) ids.push_back(grid_attribute(w));
     const size_t N = c->N, bytes = (size_t)nattr * N * sizeof(double);
-    dump_prepare(c, bytes);
+    dump_prepare(job, bytes);
     for (int k = 0; k < nattr; k++) {
-      double *dst = c->dump.stage.p + (size_t)k * N;
+      double *dst = job.stage.p + (size_t)k * N;
       if (!c->fp32) {
         HIPCHK(hipMemcpyAsync(dst, field_ptr<double>(c, ids[k]), N * sizeof(double), hipMemcpyDeviceToDevice,
                               c->stream));
@@ -1545,7 +1558,7 @@ int phafd_hip_dump_grid(phafd_hip_ctx *c, const char *path, int nattr, const cha
     const int nx = c->nx, ny = c->ny, nz = c->nz;
     const std::array<double, 3> origin{c->boxlo[0], c->boxlo[1], c->boxlo[2]};
     const std::array<double, 3> spacing{c->dx(), c->dy(), c->dz()};
-    dump_launch(c, bytes, async != 0, [=, file = std::string(path)](const char *host) {
+    dump_launch(c, job, bytes, async != 0, [=, file = std::string(path)](const char *host) {
       std::vector<const char *> nm;
       std::vector<const double *> arr;
       for (int k = 0; k < nattr; k++) {
@@ -1563,13 +1576,16 @@ int phafd_hip_dump_grid(phafd_hip_ctx *c, const char *path, int nattr, const cha
 int phafd_hip_dump_atoms(phafd_hip_ctx *c, const char *path, int nattr, const char *const *names, int precision,
                          int fmt, int async) {
   return guard(c, [&] {
-    dump_wait(c);
+    DumpJob &job = c->dump[1];
+    dump_wait_job(job);
     if (!path) throw std::invalid_argument("null file name");
     if (fmt != PHAFD_VTK_FMT_LF && fmt != PHAFD_VTK_FMT_E)
       throw std::invalid_argument("Dump fmt must be either lf or e.");
     auto at = attribute_list(nattr, names, "write");
     bool has_x = false;
     for (auto &w : at) {
+      if (w.rfind("c_", 0) == 0)
+        throw std::runtime_error("compute attributes are written with phafd_vtk_write_atoms from the compute's array");
       if (w != "x" && w != "F" && w != "ux" && w != "ix" && w != "ID" && w != "molID" && w != "type" && w != "label")
         throw std::runtime_error("Invalid per atom quantity in dump file.");
       has_x |= (w == "x");
@@ -1578,15 +1594,15 @@ int phafd_hip_dump_atoms(phafd_hip_ctx *c, const char *path, int nattr, const ch
     // device arrays in the stage: x, ux, F (double4 each), image (int)
     const size_t n = (size_t)c->nowned, v4 = n * sizeof(double4);
     const size_t bytes = 3 * v4 + n * sizeof(int) + 1;
-    dump_prepare(c, bytes);
-    char *st = reinterpret_cast<char *>(c->dump.stage.p);
+    dump_prepare(job, bytes);
+    char *st = reinterpret_cast<char *>(job.stage.p);
     if (n) {
       HIPCHK(hipMemcpyAsync(st, c->x.p, v4, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(st + v4, c->ux.p, v4, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(st + 2 * v4, c->F.p, v4, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(st + 3 * v4, c->image.p, n * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
     }
-    dump_launch(c, bytes, async != 0,
+    dump_launch(c, job, bytes, async != 0,
                 [=, file = std::string(path), tag = c->h_tag, mol = c->h_mol, type = c->h_type](const char *host) {
                   auto xyz = [&](int which) {
                     std::vector<double> v(3 * n);
@@ -1616,6 +1632,40 @@ int phafd_hip_dump_atoms(phafd_hip_ctx *c, const char *path, int nattr, const ch
                   if (phafd_vtk_write_atoms(file.c_str(), (int)n, nattr, nm.data(), arr.data(), precision, fmt) < 0)
                     throw std::runtime_error(phafd_host_last_error());
                 });
+    return 0;
+  });
+}
+
+int phafd_hip_compute_pair(phafd_hip_ctx *c, int ipair, double *f) {
+  return guard(c, [&] {
+    if (ipair < 0 || ipair >= (int)c->pairs.size())
+      throw std::invalid_argument("Cannot compute pair style, it hasn't been created!");
+    if (!c->setup_done) throw std::runtime_error("compute pair needs a set-up system");
+    if (!f) throw std::invalid_argument("null force array");
+    PairDef &p = *c->pairs[ipair];
+    const int n = c->nowned;
+    c->cp_keep.alloc(n + 1);
+    launch_copy4(n, c->F.p, c->cp_keep.p, c->stream);
+    launch_fill_d4(c->ngathered, c->gF.p, c->stream);
+    launch_fill_d4(n, c->F.p, c->stream);
+    if (c->neigh_set && c->lists_valid) {
+      if (p.style == 2) {
+        LJParams lj{p.lj1, p.lj2, p.ljcutsq};
+        launch_lj_force(c->ngathered, lj, c->gx.p, c->h_rowof.p, c->h_off.p, c->h_neigh.p, c->t_off.p,
+                        c->t_pair.p, c->t_i.p, n, c->gF.p, nullptr, c->stream);
+      } else if (p.style == 4) {
+        launch_harmonic_force(c->ngathered, HarmParams{p.hk, p.hcut}, c->gx.p, c->h_rowof.p, c->h_off.p,
+                              c->h_neigh.p, c->t_off.p, c->t_pair.p, c->t_i.p, n, c->gF.p, c->stream);
+      } else if (c->fp32) {
+        grid_atom_pass<float>(c, p, nullptr, false);   // adds its deposit to mu
+      } else {
+        grid_atom_pass<double>(c, p, nullptr, false);
+      }
+    }
+    reverse_comm(c);
+    from_d4(download(c, c->F.p, n), f);
+    launch_copy4(n, c->cp_keep.p, c->F.p, c->stream);
+    sync(c);
     return 0;
   });
 }

@@ -11,12 +11,14 @@ data, <base>_<step>.pvti|pvtp for the header, <base>.pvd for the collection.
 """
 import os
 
+import numpy as np
+
 from . import api as A
 
 
 class Dump:
     def __init__(self, system, name, dump_type, base_name, every, attributes, decimalplaces=None, fmt="lf",
-                 async_=True):
+                 async_=True, computes=None):
         if dump_type.startswith("atom"):
             which = dump_type[5:]
             if which != "owned":
@@ -32,6 +34,14 @@ class Dump:
             raise ValueError("Duplicate attribute values in dump file.")
         if decimalplaces is not None and decimalplaces <= 0:
             raise ValueError("Dump precision must be greater than or equal to zero.")
+        # c_<id> attributes: compute pair arrays, {id: pair index}
+        self.computes = dict(computes or {})
+        for w in attributes:
+            if w.startswith("c_") and w[2:] not in self.computes:
+                raise ValueError("Cannot write dump, compute ID " + w[2:] + "doesn't exist.")
+            if w.startswith("c_") and self.kind != "atom":
+                raise ValueError("Cannot write dump, compute ID " + w[2:] + "is not a per_grid quantity.")
+        self._computed = None
         self.system, self.name, self.every = system, name, int(every)
         self.attributes = list(attributes)
         self.precision = -1 if decimalplaces is None else int(decimalplaces)
@@ -60,12 +70,37 @@ class Dump:
         A.vtk_collection(self.collection_name, A.PVD_DATASET, timestep * dt,
                          f"{self.nopath_base_name}_{timestep}.p{self.fext}")
         if self.kind == "atom":
-            s.dump_atoms(instance, self.attributes, self.precision, self.fmt, self.async_)
+            if self._computed is not None or any(w.startswith("c_") for w in self.attributes):
+                self._atoms_with_computes(instance)
+            else:
+                s.dump_atoms(instance, self.attributes, self.precision, self.fmt, self.async_)
             A.vtk_write_patoms(pinstance, self.attributes, source)
         else:
             s.dump_grid(instance, self.attributes, self.async_)
             spacing = [p / n for p, n in zip(s.period, s.shape[::-1])]
             A.vtk_write_pgrid(pinstance, s.shape, self.attributes, s.boxlo, spacing, source)
+
+
+    def end_of_step(self):
+        """The computes this dump holds (ComputePair::end_of_step), evaluated
+        before any dump of the step is written, as the reference's step runs
+        its computes' end_of_step before its dumps (src/integrate.cpp:172-190):
+        a grid-atom compute's second chempot deposit is then in a chempot
+        dump of the same step."""
+        self._computed = {w: self.system.compute_pair(self.computes[w[2:]])
+                          for w in self.attributes if w.startswith("c_")} or None
+
+    def _atoms_with_computes(self, instance):
+        """An atom dump holding compute pair arrays, written from the host."""
+        s = self.system
+        if self._computed is None:
+            self.end_of_step()
+        x, ux, img = s.get_atoms()
+        vals = {"x": x, "ux": ux, "ix": img, "F": s.get_forces(), "label": np.zeros(len(x), np.int32)}
+        vals.update(s.atom_ints)
+        vals.update(self._computed)
+        self._computed = None
+        A.vtk_write_atoms(instance, [vals[w] for w in self.attributes], self.attributes, self.precision, self.fmt)
 
 
 def run_with_dumps(system, nsteps, dumps, dt, first_step=0):
@@ -78,7 +113,9 @@ def run_with_dumps(system, nsteps, dumps, dt, first_step=0):
         nxt = min([end] + [((step // d.every) + 1) * d.every for d in dumps])
         system.run(nxt - step)
         step = nxt
-        for d in dumps:
-            if d.due(step):
-                d.write_collection_middle(step, dt)
+        due = [d for d in dumps if d.due(step)]
+        for d in due:
+            d.end_of_step()
+        for d in due:
+            d.write_collection_middle(step, dt)
     return step

@@ -100,12 +100,14 @@ struct File {
 };
 
 enum Kind { VEC3, INT1 };
-// the per-atom quantities of the atom dump (src/dump.cpp:955-975)
+// the per-atom quantities of the atom dump (src/dump.cpp:955-975); c_<id>:
+// a per-atom compute of three components (compute pair), written as the
+// reference writes a compute's array (src/dump.cpp:1044-1118)
 Kind atom_kind(const std::string &w) {
   if (w == "x" || w == "F" || w == "ux") return VEC3;
   if (w == "ix" || w == "ID" || w == "molID" || w == "type" || w == "label") return INT1;
-  if (w.rfind("c_", 0) == 0 || w.rfind("f_", 0) == 0)
-    throw std::invalid_argument("compute and fix attributes are not dumped here: " + w);
+  if (w.rfind("c_", 0) == 0 && w.size() > 2) return VEC3;
+  if (w.rfind("f_", 0) == 0) throw std::invalid_argument("fix attributes are not dumped here: " + w);
   throw std::invalid_argument("Invalid per atom quantity in dump file.");
 }
 

@@ -155,7 +155,8 @@ def test_product_grid_writer_matches_oracle_bytes(tmp_path, shape, origin, spaci
 @pytest.mark.parametrize("precision,fmt", [(-1, 0), (-1, 1), (1, 0), (3, 0), (12, 0), (4, 1), (17, 1), (25, 0)])
 def test_product_atom_writer_matches_oracle_bytes(tmp_path, precision, fmt):
     a = _atoms(300, 4)
-    for names in (["x", "ux", "ix", "F", "ID", "molID", "type"], ["ID", "x", "label"]):
+    a["c_droppolyforces"] = a["F"][::-1] * 0.37
+    for names in (["x", "ux", "ix", "F", "c_droppolyforces", "ID", "molID", "type"], ["ID", "x", "label"]):
         o, g = str(tmp_path / "o.vtp"), str(tmp_path / "g.vtp")
         O.vtk_write_atoms(o, [a[w] for w in names], names, precision, fmt)
         P.vtk_write_atoms(g, [a[w] for w in names], names, precision, "e" if fmt else "lf")
@@ -194,7 +195,7 @@ def test_writer_errors():
     with pytest.raises(P.PhafdError, match="Invalid per atom quantity"):
         P.vtk_write_atoms("/tmp/none.vtp", [np.zeros((2, 3)), np.zeros(2, np.int32)], ["x", "velocity"])
     with pytest.raises(P.PhafdError, match="not dumped here"):
-        P.vtk_write_atoms("/tmp/none.vtp", [np.zeros((2, 3)), np.zeros(2, np.int32)], ["x", "c_pair"])
+        P.vtk_write_atoms("/tmp/none.vtp", [np.zeros((2, 3)), np.zeros(2, np.int32)], ["x", "f_ave"])
     with pytest.raises(P.PhafdError, match="Cannot open file"):
         P.vtk_write_grid("/nonexistent_dir/x.vti", [np.zeros((2, 2, 2))], ["phi"], [0] * 3, [1] * 3)
 

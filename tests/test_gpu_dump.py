@@ -79,7 +79,7 @@ def test_async_dumps_overlap_steps_without_changing_them(tmp_path):
     s.dump_atoms(sync_a, ATOM_NAMES, 17, "e")
     s.dump_grid(async_g, GRID_NAMES, async_=True)
     s.run(10)                        # runs while the copy and the write proceed
-    s.dump_atoms(async_a, ATOM_NAMES, 17, "e", async_=True)   # waits for the grid dump first
+    s.dump_atoms(async_a, ATOM_NAMES, 17, "e", async_=True)   # beside the grid dump if still in flight
     s.run(10)
     s.dump_wait()
     t.run(20)
@@ -155,3 +155,34 @@ def test_dump_command_over_a_run(tmp_path, async_):
     ref = str(tmp_path / "ref.vti")
     _host_grid(s, ref, w)
     assert _read(ref) == _read(f"{base}_p0_30.vti")
+
+
+def test_example_dumps_with_compute_pair_against_oracle(tmp_path):
+    # the reference example's dump lines: an atom/owned dump holding
+    # c_droppolyforces (compute pair of the gridatom/LJish/linear style) and a
+    # grid dump of phi and chempot at the same step; the compute runs before
+    # the dumps, so the dumped chempot carries its second deposit
+    from tests.helpers import make_oracle
+    w = workload(n=32, nchains=8, nbeads=20, seed=25, ljish="linear", eps=0.2)
+    s, o = make_gpu(w), make_oracle(w)
+    base = str(tmp_path / "run")
+    attrs = ["x", "ux", "ix", "F", "c_droppolyforces", "ID", "molID", "type"]
+    grid = Dump(s, "realgrid", "grid", base + "field", 10, ["phi", "chempot"], async_=True)
+    atoms = Dump(s, "atoms", "atom/owned", base + "atom", 10, attrs, decimalplaces=17, fmt="e",
+                 computes={"droppolyforces": 0})
+    for d in (grid, atoms):
+        d.write_collection_header()
+    run_with_dumps(s, 20, [grid, atoms], w["dt"])
+    s.dump_wait()
+    o.run(20)
+    f_o = o.compute_pair(0)
+    piece = ET.parse(f"{base}atom_p0_20.vtp").getroot().find("PolyData/Piece")
+    arrays = {e.get("Name"): e for e in piece.iter("DataArray")}
+    f_g = np.array(arrays["c_droppolyforces"].text.split(), dtype=np.float64).reshape(-1, 3)
+    assert arrays["c_droppolyforces"].get("NumberOfComponents") == "3"
+    assert rel_l2(f_g, f_o) < 1e-10
+    x_g = np.array(arrays["x"].text.split(), dtype=np.float64).reshape(-1, 3)
+    assert rel_l2(x_g, o.get_atoms()[0]) < 1e-10
+    phi, mu = P.vtk_read_grid(f"{base}field_p0_20.vti", s.shape, ["phi", "chempot"])
+    assert rel_l2(phi, o.get_field(P.PHI)) < 1e-10
+    assert rel_l2(mu, o.get_field(P.MU)) < 1e-10

@@ -251,6 +251,31 @@ def test_harmonic_cut_one_step_and_trajectory(lj):
     assert np.array_equal(g.get_forces(), full.get_forces())
 
 
+@pytest.mark.parametrize("ljish", [False, "linear"])
+def test_compute_pair_end_of_step(ljish):
+    # ComputePair after a run (the end of its last step): the forces of each
+    # pair style alone, from the step's gathered positions and the current
+    # fields, against the oracle; the owned forces stay as they were and the
+    # grid-atom style's deposit lands on mu once more
+    w = workload(n=32, nchains=10, nbeads=25, seed=33, zghost=1, ljish=ljish, harmonic=(1.3, 40.0), eps=0.2)
+    g = make_gpu(w)
+    o = make_oracle(w)
+    g.run(25)
+    o.run(25)
+    F = g.get_forces()
+    for ip in range(3):   # grid-atom, lj/cut, harmonic/cut
+        fg, fo = g.compute_pair(ip), o.compute_pair(ip)
+        assert rel_l2(fg, fo) < 1e-10, ip
+        assert np.abs(fg).max() > 0
+    assert np.array_equal(g.get_forces(), F)
+    assert rel_l2(g.get_field(P.MU), o.get_field(P.MU)) < 1e-10
+    g.run(5)
+    o.run(5)
+    assert rel_l2(g.get_field(P.PHI), o.get_field(P.PHI)) < 1e-10
+    with pytest.raises(P.PhafdError):
+        g.compute_pair(3)
+
+
 def _fields_only(cls, shape, period, noise, seed=0):
     nz, ny, nx = shape
     s = cls(nx, ny, nz, [0.0, 0.0, 0.0], period)

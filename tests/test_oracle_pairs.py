@@ -1,5 +1,6 @@
 """Pins of the oracle's pair styles and chain fix (CPU only)."""
 import numpy as np
+import pytest
 
 from oracle import oracle_ctypes as O
 
@@ -195,3 +196,39 @@ def test_chain_force_is_minus_energy_gradient():
             dn = xyz.copy(); dn[b, c] -= h
             g = (energy(up) - energy(dn)) / (2 * h)
             assert abs(fchain[b, c] + g) < 1e-5 * max(1.0, abs(g))
+
+
+def test_compute_pair_splits_the_pair_forces():
+    # ComputePair (src/compute_pair.cpp:43-85) right after the pair phase sees
+    # the phase's inputs: the per-style forces add up to the phase's force
+    # (superposition), the owned forces are left untouched, and the grid-atom
+    # style's second compute deposits onto mu once more
+    from tests.helpers import workload, make_oracle
+    w = workload(n=32, nchains=8, nbeads=20, seed=31, harmonic=(1.3, 40.0))
+    o = make_oracle(w)
+    o.run(5)
+    for ph in (O.PH_START, O.PH_NEIGH, O.PH_PAIR):
+        o.phase(ph)
+    F, mu = o.get_forces(), o.get_field(O.MU)
+    parts = [o.compute_pair(i) for i in range(3)]   # gridatom, lj/cut, harmonic/cut
+    assert np.array_equal(o.get_forces(), F)
+    assert np.allclose(sum(parts), F, rtol=0, atol=1e-13 * np.abs(F).max())
+    assert all(np.abs(p).max() > 0 for p in parts)
+    assert np.allclose(o.get_field(O.MU), 2 * mu, rtol=1e-13, atol=1e-15)
+    with pytest.raises(RuntimeError):
+        o.compute_pair(3)
+
+
+def test_compute_pair_two_body_closed_form():
+    s = O.System(16, 16, 16, [0, 0, 0], [16.0] * 3)
+    s.set_field(O.PHI, np.full((16, 16, 16), 0.5))
+    s.set_atoms(np.array([[5.0, 5.0, 5.0], [6.1, 5.2, 4.9]]), np.zeros(2, np.int32))
+    s.add_pair(O.PAIR_LJCUT, 1.5, 1.0, 1.0)
+    s.add_pair(*HARM)
+    s.neighbor(0.5)
+    s.setup(1e-3)
+    for ph in (O.PH_START, O.PH_NEIGH, O.PH_PAIR):
+        s.phase(ph)
+    d = np.array([5.0, 5.0, 5.0]) - np.array([6.1, 5.2, 4.9])
+    assert np.allclose(s.compute_pair(0)[0], lj_force(d), rtol=1e-13)
+    assert np.allclose(s.compute_pair(1)[0], harmonic_force(d), rtol=1e-13)
