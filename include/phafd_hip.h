@@ -103,6 +103,24 @@ int phafd_hip_get_forces(phafd_hip_ctx *ctx, double *f);
  * chempot deposit to mu once more (mu is zeroed again by the next step). */
 int phafd_hip_compute_pair(phafd_hip_ctx *ctx, int ipair, double *f);
 
+/* ---- Fourier-space computes.
+ * phafd_hip_fourier_capture(on): the last step of each later phafd_hip_run
+ * keeps ft_phi and ft_chempot as they are at the in_fourier hook
+ * (src/integrate.cpp:156-160: after their r2c, before the update).
+ * compute <id> complex ft_phi|ft_chempot real|imag|modulus|norm:
+ * ComputeComplex::in_fourier (src/compute_complex.cpp:54-91) of the kept
+ * spectrum, scaled by 1/(dx dy dz) (squared for norm); out holds
+ * Ny*Nz*(Nx/2+1) values in the reference's ft array order [ky][kz][kx].
+ * fix <id> grid/ave <every> <repeat> <freq> c_<id>: FixGridAve
+ * (src/fixgrid_ave.cpp:92-215) of such a compute, accumulated on the device
+ * in one of 8 slots: first starts the window from zero, last divides by
+ * repeat; out (may be NULL) receives the slot after the call. */
+enum { PHAFD_CPX_REAL = 0, PHAFD_CPX_IMAG = 1, PHAFD_CPX_MODULUS = 2, PHAFD_CPX_NORM = 3 };
+int phafd_hip_fourier_capture(phafd_hip_ctx *ctx, int on);
+int phafd_hip_compute_complex(phafd_hip_ctx *ctx, int which, int quant, double *out);
+int phafd_hip_grid_ave(phafd_hip_ctx *ctx, int slot, int which, int quant, int first, int last, int repeat,
+                       double *out);
+
 /* ---- dumps and restart (formats and attribute names of include/phafd_host.h).
  * dump <name> grid <file> ... attributes n phi chempot gradphi_x ...:
  * Dump::write_grid_timestep (src/dump.cpp:681-780) of the current fields.

@@ -1520,6 +1520,29 @@ int orc_compute_pair(orc_sys *s, int ipair, double *f) {
     std::copy(keep.begin(), keep.end(), s->Fs.begin());
   });
 }
+// ComputeComplex::in_fourier (src/compute_complex.cpp:54-91), called between
+// ORC_PH_PREFINAL and ORC_PH_FINAL: ft_phi (the start-of-step r2c of phi,
+// which the reference transforms again at pre_final_integrate from the same
+// phi) or ft_chempot, in the reference's ft array order: its arrays are
+// built (Nx, Nz, Ny) (src/grid.cpp:214-224), so (i, j, k) = (ky, kz, kx)
+int orc_compute_complex(orc_sys *s, int which, int quant, double *out) {
+  return guard(s, [&] {
+    const std::vector<cplx> &a = which == 0 ? s->ftphi : s->ftmu;
+    if (which < 0 || which > 1 || a.size() != s->M) throw std::runtime_error("spectrum not available");
+    const int hx = s->nx / 2 + 1;
+    const double prefac = 1. / (s->dx() * s->dy() * s->dz());
+    size_t count = 0;
+    for (int i = 0; i < s->ny; i++)
+      for (int j = 0; j < s->nz; j++)
+        for (int k = 0; k < hx; k++) {
+          const cplx z = a[((size_t)j * s->ny + i) * hx + k];
+          if (quant == 2) out[count++] = std::abs(z) * prefac;
+          else if (quant == 3) out[count++] = std::norm(z) * prefac * prefac;
+          else if (quant == 0) out[count++] = z.real() * prefac;
+          else out[count++] = z.imag() * prefac;
+        }
+  });
+}
 int orc_counts(orc_sys *s, int64_t *o) {
   o[0] = s->nowned; o[1] = s->ngathered; o[2] = s->nlocal; o[3] = s->nghost;
   o[4] = s->ncalls; o[5] = s->nless; o[6] = s->ngreat; o[7] = s->list_overflow ? 1 : 0;

@@ -97,6 +97,10 @@ enum {
   ORC_PH_POSTFINAL = 7 /* c2r phi                                   */
 };
 int orc_phase(orc_sys *, int phase);
+/* ComputeComplex::in_fourier between ORC_PH_PREFINAL and ORC_PH_FINAL: which
+ * 0 ft_phi, 1 ft_chempot; quant 0 real, 1 imag, 2 modulus, 3 norm; out in
+ * the reference's ft order [ky][kz][kx] */
+int orc_compute_complex(orc_sys *, int which, int quant, double *out);
 /* ComputePair::end_of_step: owned forces of pair style ipair alone */
 int orc_compute_pair(orc_sys *, int ipair, double *f);
 

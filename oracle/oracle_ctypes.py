@@ -52,6 +52,7 @@ def lib():
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_phase.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_compute_pair.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_compute_complex.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_counts.argtypes = [ctypes.c_void_p, _lp]
         for name in ("orc_gridlist", "orc_halflist"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_int64, _ip, _ip, _ip, _lp, _lp]
@@ -228,6 +229,13 @@ class System:
         f = np.zeros((self.natoms, 3))
         lib().orc_get_forces(self.h, dptr(f))
         return f
+
+    def compute_complex(self, which, quant):
+        """(Ny, Nz, Nx/2+1) array; between PH_PREFINAL and PH_FINAL."""
+        nz, ny, nx = self.shape
+        out = np.zeros((ny, nz, nx // 2 + 1))
+        self._chk(lib().orc_compute_complex(self.h, which, quant, dptr(out)))
+        return out
 
     def compute_pair(self, ipair):
         f = np.zeros((self.natoms, 3))

@@ -149,6 +149,9 @@ def lib():
         "phafd_hip_dump_atoms": ([V, _cp, ctypes.c_int, _pp] + [ctypes.c_int] * 3, ctypes.c_int),
         "phafd_hip_dump_wait": ([V], ctypes.c_int),
         "phafd_hip_compute_pair": ([V, ctypes.c_int, _dp], ctypes.c_int),
+        "phafd_hip_fourier_capture": ([V, ctypes.c_int], ctypes.c_int),
+        "phafd_hip_compute_complex": ([V, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
+        "phafd_hip_grid_ave": ([V] + [ctypes.c_int] * 6 + [_dp], ctypes.c_int),
         "phafd_hip_read_dump_grid": ([V, _cp, ctypes.c_int, _pp, ctypes.c_int], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
@@ -216,6 +219,7 @@ def ic_random_chains(nchains, nbeads, bond, minsep, boxlo, period, seed):
 
 
 # ------------------------------------------------------------ dump files
+CPX_REAL, CPX_IMAG, CPX_MODULUS, CPX_NORM = 0, 1, 2, 3
 VTK_FMT_LF, VTK_FMT_E = 0, 1
 PVD_HEADER, PVD_DATASET, PVD_FOOTER = 0, 1, 2
 GRID_ATTRIBUTES = ("phi", "chempot", "gradphi_x", "gradphi_y", "gradphi_z")
@@ -447,6 +451,26 @@ class System:
         f = np.zeros((self.natoms, 3))
         self._c(lib().phafd_hip_compute_pair(self.h, ipair, _d(f)))
         return f
+
+    def fourier_capture(self, on=True):
+        """Keep ft_phi / ft_chempot of the last step of each later run."""
+        self._c(lib().phafd_hip_fourier_capture(self.h, int(on)))
+
+    def _spec_shape(self):
+        nz, ny, nx = self.shape
+        return (ny, nz, nx // 2 + 1)
+
+    def compute_complex(self, which, quant):
+        """compute complex ft_phi|ft_chempot <quant>: (Ny, Nz, Nx/2+1), the
+        reference's ft array order."""
+        out = np.zeros(self._spec_shape())
+        self._c(lib().phafd_hip_compute_complex(self.h, which, quant, _d(out)))
+        return out
+
+    def grid_ave(self, slot, which, quant, first, last, repeat, want=False):
+        out = np.zeros(self._spec_shape()) if want else None
+        self._c(lib().phafd_hip_grid_ave(self.h, slot, which, quant, int(first), int(last), repeat, _d(out)))
+        return out
 
     def dump_grid(self, path, names, async_=False):
         """dump ... grid: the current fields to a .vti file (async_: the write

@@ -163,6 +163,12 @@ struct phafd_hip_ctx {
   bool lists_valid = false;
   int pending_decision = -1;   // rebuild decision prefetched at the end of the last step
   bool store_muhat = true;     // the fused z update writes muhat (false inside a run but its last step)
+  // in_fourier: the last step of a run copies ft_phi (before the update) and
+  // ft_chempot (K layout, FP64) for compute complex / fix grid/ave
+  bool fourier_capture = false;
+  bool cap_valid = false;
+  DBuf<double2> cap_hat[2];
+  DBuf<double> cc_out, ave[8];
   bool atom_windows = true;    // the surrogate's atom steps in speculative windows
   // styles
   std::vector<std::unique_ptr<PairDef>> pairs;
@@ -1032,6 +1038,21 @@ bool can_fuse_fh(phafd_hip_ctx *c, FHPreT<R> &pre) {
   return true;
 }
 
+__global__ void k_widen_c(size_t n, const float2 *__restrict__ a, double2 *__restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = make_double2(a[i].x, a[i].y);
+}
+void capture_spectrum(phafd_hip_ctx *c, int k, const double2 *src) {
+  c->cap_hat[k].alloc(c->M);
+  HIPCHK(hipMemcpyAsync(c->cap_hat[k].p, src, sizeof(double2) * c->M, hipMemcpyDeviceToDevice, c->stream));
+}
+void capture_spectrum(phafd_hip_ctx *c, int k, const float2 *src) {
+  c->cap_hat[k].alloc(c->M);
+  const int blocks = (int)std::min<size_t>((c->M + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_widen_c, dim3(blocks), dim3(256), 0, c->stream, c->M, src, c->cap_hat[k].p);
+  HIPCHK(hipGetLastError());
+}
+
 template <class R>
 void run_steps_t(phafd_hip_ctx *c, int nsteps) {
   FHPreT<R> pre;
@@ -1061,11 +1082,17 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
     if (zsplit) phase_prefinal_split<R>(c, fused ? &pre : nullptr);
     else phase_prefinal_t<R>(c, fused ? &pre : nullptr);
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
+    const bool capture = c->fourier_capture && it == nsteps - 1;
+    if (capture) capture_spectrum(c, 0, c->fields<R>().phihat.p);   // in_fourier: ft_phi before the update
     if (zsplit) phase_final_split<R>(c, !fc);
     else phase_final_t<R>(c, !fc);
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
     if (zsplit) phase_postfinal_split<R>(c);
     else phase_postfinal_t<R>(c);
+    if (capture) {   // ft_chempot: the last step of a run stores the complete muhat
+      capture_spectrum(c, 1, c->fields<R>().muhat.p);
+      c->cap_valid = true;
+    }
     HIPCHK(hipEventRecord(c->ev[7], c->stream));
     // next step's rebuild decision (Neighbor::decide) + NaN check of
     // phi(0,0,0) (src/integrate.cpp:190-201), read back together
@@ -1665,6 +1692,104 @@ int phafd_hip_compute_pair(phafd_hip_ctx *c, int ipair, double *f) {
     reverse_comm(c);
     from_d4(download(c, c->F.p, n), f);
     launch_copy4(n, c->cp_keep.p, c->F.p, c->stream);
+    sync(c);
+    return 0;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+// ComputeComplex::in_fourier (src/compute_complex.cpp:54-91) on a captured K
+// layout spectrum [kx][kz][ky]: out in the reference's ft array order
+// [ky][kz][kx] (its arrays are built (Nx, Nz, Ny), src/grid.cpp:214-224).
+// One block per (32 ky x 32 kx) tile of one kz plane, transposed in LDS so
+// that both the reads (ky runs) and the writes (kx runs) are coalesced.
+__global__ void __launch_bounds__(256) k_complex_out(const double2 *__restrict__ hat, int hx, int ny, int nz,
+                                                     int quant, double prefac, double *__restrict__ out) {
+  __shared__ double2 tile[32][33];
+  const int ky0 = blockIdx.x * 32, kx0 = blockIdx.y * 32, kz = blockIdx.z;
+  const int tx = threadIdx.x, ty = threadIdx.y;   // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int kx = kx0 + r, ky = ky0 + tx;
+    if (kx < hx && ky < ny) tile[r][tx] = hat[((size_t)kx * nz + kz) * ny + ky];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int ky = ky0 + r, kx = kx0 + tx;
+    if (kx < hx && ky < ny) {
+      const double2 z = tile[tx][r];
+      double v;
+      if (quant == PHAFD_CPX_MODULUS) v = hypot(z.x, z.y) * prefac;
+      else if (quant == PHAFD_CPX_NORM) v = (z.x * z.x + z.y * z.y) * prefac * prefac;
+      else if (quant == PHAFD_CPX_REAL) v = z.x * prefac;
+      else v = z.y * prefac;
+      out[((size_t)ky * nz + kz) * hx + kx] = v;
+    }
+  }
+}
+
+// FixGridAve::end_of_step (src/fixgrid_ave.cpp:170-215): the first sample of
+// a window starts from zero, the last divides by repeat
+__global__ void k_grid_ave(size_t n, const double *__restrict__ x, double *__restrict__ acc, int first, int last,
+                           double repeat) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double a = first ? 0.0 : acc[i];
+    a += x[i];
+    if (last) a /= repeat;
+    acc[i] = a;
+  }
+}
+
+void complex_to(phafd_hip_ctx *c, int which, int quant, double *dst) {
+  if (which != PHAFD_PHIHAT && which != PHAFD_MUHAT) throw std::invalid_argument("Invalid array to compute in compute modulus.");
+  if (quant < PHAFD_CPX_REAL || quant > PHAFD_CPX_NORM) throw std::invalid_argument("Invalid argument in compute complex.");
+  if (!c->cap_valid) throw std::runtime_error("no captured spectra: enable phafd_hip_fourier_capture before the run");
+  if (c->nz > 65535) throw std::invalid_argument("compute complex: Nz above the launch's grid limit");
+  const int hx = c->nx / 2 + 1;
+  const double prefac = 1. / (c->dx() * c->dy() * c->dz());
+  dim3 grid((c->ny + 31) / 32, (hx + 31) / 32, c->nz);
+  hipLaunchKernelGGL(k_complex_out, grid, dim3(32, 8), 0, c->stream, c->cap_hat[which].p, hx, c->ny, c->nz, quant,
+                     prefac, dst);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+int phafd_hip_fourier_capture(phafd_hip_ctx *c, int on) {
+  return guard(c, [&] {
+    c->fourier_capture = on != 0;
+    return 0;
+  });
+}
+
+int phafd_hip_compute_complex(phafd_hip_ctx *c, int which, int quant, double *out) {
+  return guard(c, [&] {
+    if (!out) throw std::invalid_argument("null output");
+    c->cc_out.alloc(c->M);
+    complex_to(c, which, quant, c->cc_out.p);
+    HIPCHK(hipMemcpyAsync(out, c->cc_out.p, sizeof(double) * c->M, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return 0;
+  });
+}
+
+int phafd_hip_grid_ave(phafd_hip_ctx *c, int slot, int which, int quant, int first, int last, int repeat,
+                       double *out) {
+  return guard(c, [&] {
+    if (slot < 0 || slot >= 8) throw std::invalid_argument("grid/ave slot out of range (0..7)");
+    if (repeat <= 0) throw std::invalid_argument("Every, repeat, and freq must all be >= 0.");
+    c->cc_out.alloc(c->M);
+    c->ave[slot].alloc(c->M);
+    complex_to(c, which, quant, c->cc_out.p);
+    const int blocks = (int)std::min<size_t>((c->M + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_grid_ave, dim3(blocks), dim3(256), 0, c->stream, c->M, c->cc_out.p, c->ave[slot].p,
+                       first != 0, last != 0, (double)repeat);
+    HIPCHK(hipGetLastError());
+    if (out) HIPCHK(hipMemcpyAsync(out, c->ave[slot].p, sizeof(double) * c->M, hipMemcpyDeviceToHost, c->stream));
     sync(c);
     return 0;
   });
