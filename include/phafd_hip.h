@@ -136,7 +136,7 @@ int phafd_hip_dump_atoms(phafd_hip_ctx *ctx, const char *path, int nattr, const 
                          int fmt, int async);
 int phafd_hip_dump_wait(phafd_hip_ctx *ctx);
 /* grid_populate read_dump grid <file> attributes n phi ... [no_padding]:
- * ReadDump::process_grid_attributes (src/read_dump.cpp:103-195) into the
+ * ReadDump::process_grid_attributes (src/read_dump.cpp:90-185) into the
  * context's fields (phi must be among the attributes) */
 int phafd_hip_read_dump_grid(phafd_hip_ctx *ctx, const char *path, int nattr, const char *const *names,
                              int no_padding);

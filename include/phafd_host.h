@@ -55,21 +55,21 @@ int phafd_vtk_write_grid(const char *path, int nx, int ny, int nz, const double 
 /* Dump::write_grid_timestep_pvti (src/dump.cpp:782-855) */
 int phafd_vtk_write_pgrid(const char *path, int nx, int ny, int nz, const double *origin, const double *spacing,
                           int nattr, const char *const *names, const char *source);
-/* Dump::write_atom_timestep, atom/owned (src/dump.cpp:360-418, 978-1118);
+/* Dump::write_atom_timestep, atom/owned (src/dump.cpp:348-418, 1014-1133);
  * precision <= 0: the stream default; fmt PHAFD_VTK_FMT_* */
 int phafd_vtk_write_atoms(const char *path, int n, int nattr, const char *const *names, const void *const *arrays,
                           int precision, int fmt);
-/* Dump::write_atom_timestep_pvtp (src/dump.cpp:420-477) */
+/* Dump::write_atom_timestep_pvtp (src/dump.cpp:422-483) */
 int phafd_vtk_write_patoms(const char *path, int nattr, const char *const *names, const char *source);
-/* Dump::write_collection_header / _middle / _footer (src/dump.cpp:226-300):
+/* Dump::write_collection_header / _middle / _footer (src/dump.cpp:235-313):
  * part PHAFD_VTK_PVD_*; time and file are used by the data-set line */
 int phafd_vtk_collection(const char *path, int part, double time, const char *file);
-/* ReadDump grid (src/read_dump.cpp:103-195, 300-338): out[k] (Nx*Ny*Nz
+/* ReadDump grid (src/read_dump.cpp:90-185, 307-360): out[k] (Nx*Ny*Nz
  * doubles, may be null to skip) receives requested attribute k; phi must be
  * requested */
 int phafd_vtk_read_grid(const char *path, int nx, int ny, int nz, int nattr, const char *const *names,
                         double *const *out, int no_padding);
-/* ReadDump atoms (src/read_dump.cpp:199-296): out[k] receives requested
+/* ReadDump atoms (src/read_dump.cpp:188-305): out[k] receives requested
  * attribute k for natoms atoms; x, type, ID (and molID, ix when molecule)
  * must be requested; returns natoms */
 int phafd_vtk_read_atoms(const char *path, int natoms, int nattr, const char *const *names, int molecule,

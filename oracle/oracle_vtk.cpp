@@ -8,7 +8,7 @@
 // One-rank grid files: the byte length written before each array is the
 // size ReadDump requires of a one-rank file (Nx*Ny*Nz doubles,
 // src/read_dump.cpp:318-338); the reference's writer sizes it for a
-// neighbour plane (src/dump.cpp:724-728) that a single rank does not have,
+// neighbour plane (src/dump.cpp:730-733) that a single rank does not have,
 // and its plane exchange addresses rank 1 (src/dump.cpp:1155-1158).
 #include <cstdint>
 #include <fstream>
@@ -33,7 +33,7 @@ int vguard(F &&f) {
 }
 
 // x, F, ux (Matrix3Xd) and a per-atom compute of three components: the
-// same header text and value layout (src/dump.cpp:978-1118)
+// same header text and value layout (src/dump.cpp:1014-1133)
 bool is_vec3(const std::string &w) { return w == "x" || w == "F" || w == "ux" || w.rfind("c_", 0) == 0; }
 }  // namespace
 
@@ -102,7 +102,7 @@ int orc_vtk_write_pgrid(const char *path, int nx, int ny, int nz, const double *
 }
 
 // Dump::write_atom_timestep (atom/owned) with write_ascii_data,
-// src/dump.cpp:340-418, 978-1118
+// src/dump.cpp:348-418, 1014-1133
 int orc_vtk_write_atoms(const char *path, int n, int nattr, const char *const *names, const void *const *arrays,
                         int precision, int fmt) {
   return vguard([&] {
@@ -146,7 +146,7 @@ int orc_vtk_write_atoms(const char *path, int n, int nattr, const char *const *n
   });
 }
 
-// Dump::write_atom_timestep_pvtp, src/dump.cpp:420-477
+// Dump::write_atom_timestep_pvtp, src/dump.cpp:422-483
 int orc_vtk_write_patoms(const char *path, int nattr, const char *const *names, const char *source) {
   return vguard([&] {
     std::ofstream f(path, std::ios::out);
@@ -175,7 +175,7 @@ int orc_vtk_write_patoms(const char *path, int nattr, const char *const *names, 
   });
 }
 
-// Dump::write_collection_header / _middle / _footer, src/dump.cpp:226-300
+// Dump::write_collection_header / _middle / _footer, src/dump.cpp:235-313
 int orc_vtk_collection(const char *path, int part, double time, const char *file) {
   return vguard([&] {
     if (part == 0) {

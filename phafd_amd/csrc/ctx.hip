@@ -1474,7 +1474,7 @@ __global__ void k_widen(size_t n, const float *__restrict__ a, double *__restric
     b[i] = a[i];
 }
 
-// grid attributes of the dump and read_dump commands (src/dump.cpp:941-955,
+// grid attributes of the dump and read_dump commands (src/dump.cpp:969-983,
 // src/read_dump.cpp:170-185)
 int grid_attribute(const std::string &w) {
   if (w == "phi") return PHAFD_PHI;

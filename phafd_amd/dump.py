@@ -1,4 +1,4 @@
-"""The dump command over the C ABI (Dump, src/dump.cpp:28-300): one file per
+"""The dump command over the C ABI (Dump, src/dump.cpp:28-313): one file per
 dumped timestep, its parallel header naming the one piece, and the .pvd
 collection listing the timesteps.
 
@@ -6,7 +6,7 @@ collection listing the timesteps.
     dump <name> atom/owned <base> <every> attributes n x ux ix F ID molID type
          [decimalplaces p] [fmt lf|e]
 
-File names follow src/dump.cpp:302-318: <base>_p0_<step>.vti|vtp for the
+File names follow src/dump.cpp:315-334: <base>_p0_<step>.vti|vtp for the
 data, <base>_<step>.pvti|pvtp for the header, <base>.pvd for the collection.
 """
 import os

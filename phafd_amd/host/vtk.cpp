@@ -38,7 +38,7 @@ int guard(F &&f) {
   }
 }
 
-// the stream format of the dump (src/dump.cpp:349-355): fixed / scientific
+// the stream format of the dump (src/dump.cpp:365-371): fixed / scientific
 // with the given precision, scientific with 6 digits, or the default %.6g
 struct NumFmt {
   std::chars_format f = std::chars_format::general;
@@ -100,9 +100,9 @@ struct File {
 };
 
 enum Kind { VEC3, INT1 };
-// the per-atom quantities of the atom dump (src/dump.cpp:955-975); c_<id>:
+// the per-atom quantities of the atom dump (src/dump.cpp:985-1004); c_<id>:
 // a per-atom compute of three components (compute pair), written as the
-// reference writes a compute's array (src/dump.cpp:1044-1118)
+// reference writes a compute's array (src/dump.cpp:872-930, 1061-1133)
 Kind atom_kind(const std::string &w) {
   if (w == "x" || w == "F" || w == "ux") return VEC3;
   if (w == "ix" || w == "ID" || w == "molID" || w == "type" || w == "label") return INT1;
@@ -197,8 +197,8 @@ int phafd_vtk_write_pgrid(const char *path, int nx, int ny, int nz, const double
   });
 }
 
-// Dump::write_atom_timestep for atom/owned (src/dump.cpp:360-418, the ASCII
-// arrays of src/dump.cpp:978-1118): positions first, then the other
+// Dump::write_atom_timestep for atom/owned (src/dump.cpp:348-418, the ASCII
+// arrays of src/dump.cpp:1014-1133): positions first, then the other
 // attributes in the given order
 int phafd_vtk_write_atoms(const char *path, int n, int nattr, const char *const *names, const void *const *arrays,
                           int precision, int fmt) {
@@ -283,7 +283,7 @@ int phafd_vtk_write_atoms(const char *path, int n, int nattr, const char *const 
   });
 }
 
-// Dump::write_atom_timestep_pvtp (src/dump.cpp:420-477), one piece
+// Dump::write_atom_timestep_pvtp (src/dump.cpp:422-483), one piece
 int phafd_vtk_write_patoms(const char *path, int nattr, const char *const *names, const char *source) {
   return guard([&] {
     auto at = name_list(nattr, names);
@@ -309,7 +309,7 @@ int phafd_vtk_write_patoms(const char *path, int nattr, const char *const *names
   });
 }
 
-// Dump::write_collection_header / _middle / _footer (src/dump.cpp:226-300)
+// Dump::write_collection_header / _middle / _footer (src/dump.cpp:235-313)
 int phafd_vtk_collection(const char *path, int part, double time, const char *file) {
   return guard([&] {
     Out o;
