@@ -121,6 +121,30 @@ int phafd_hip_compute_complex(phafd_hip_ctx *ctx, int which, int quant, double *
 int phafd_hip_grid_ave(phafd_hip_ctx *ctx, int slot, int which, int quant, int first, int last, int repeat,
                        double *out);
 
+/* compute <id> grid/clusters phi <threshold> lt|gt: ComputeGridClusters::
+ * end_of_step (src/compute_grid_clusters.cpp:62-168) on one subdomain, the
+ * box periodic in x, y and z: out[Nz][Ny][Nx] = the smallest id
+ * i + (j*Nx + k)*Ny ((i, j, k) = (z, y, x)) of the point's 6-connected
+ * cluster of points with phi < threshold (lt) or > threshold (gt), -1
+ * elsewhere */
+enum { PHAFD_CLUSTER_LT = 0, PHAFD_CLUSTER_GT = 1 };
+int phafd_hip_compute_grid_clusters(phafd_hip_ctx *ctx, double threshold, int condition, double *out);
+
+/* run_until_touching <nsteps> <touch_point> [continue] [delay]:
+ * Integrate::run_until_touching (src/integrate.cpp:221-415).  After every
+ * step the distance between the last and the first owned atom (positions as
+ * held, no minimum image) is compared with touch_point.  occurred[] receives
+ * the steps the reference reports as "cyclisation occured" (a touch at least
+ * delay_step steps after the previous one), ended[] the steps of "cyclisation
+ * interval ended"; at most cap of each are stored, the counts are complete.
+ * Without continue_running the run stops after the first touching step and
+ * the call returns 1 (0 when the steps ran out); steps_run counts the steps
+ * done.  The reference's extra computes and dumps at the stop are the
+ * caller's. */
+int phafd_hip_run_until_touching(phafd_hip_ctx *ctx, int nsteps, double touch_point, int continue_running,
+                                 int delay_step, int cap, int64_t *occurred, int *n_occurred, int64_t *ended,
+                                 int *n_ended, int *steps_run);
+
 /* ---- dumps and restart (formats and attribute names of include/phafd_host.h).
  * dump <name> grid <file> ... attributes n phi chempot gradphi_x ...:
  * Dump::write_grid_timestep (src/dump.cpp:681-780) of the current fields.

@@ -1543,6 +1543,78 @@ int orc_compute_complex(orc_sys *s, int which, int quant, double *out) {
         }
   });
 }
+// ComputeGridClusters::end_of_step (src/compute_grid_clusters.cpp:62-168)
+// with one rank: the reference takes its z neighbours from lefts / rights,
+// the end planes of the neighbouring ranks; with the box on one rank those
+// are this rank's own planes Nz-1 and 0, so z is periodic like x and y.
+// Points beyond the threshold start with id i + (j*Nx + k)*Ny and the
+// in-place min sweeps run until one changes nothing.
+int orc_compute_grid_clusters(orc_sys *s, double threshold, int condition, double *out) {
+  return guard(s, [&] {
+    const int Nz = s->nz, Ny = s->ny, Nx = s->nx;
+    std::vector<int> a((size_t)Nx * Ny * Nz);
+    auto at = [&](int i, int j, int k) -> int & { return a[(size_t)k + ((size_t)i * Ny + j) * Nx]; };
+    for (int i = 0; i < Nz; i++)
+      for (int j = 0; j < Ny; j++)
+        for (int k = 0; k < Nx; k++) {
+          const double v = s->phi[(size_t)k + ((size_t)i * Ny + j) * Nx];
+          const bool in = condition == 0 ? v < threshold : v > threshold;
+          at(i, j, k) = in ? i + (j * Nx + k) * Ny : -1;
+        }
+    bool done = false;
+    while (!done) {
+      done = true;
+      for (int i = 0; i < Nz; i++)
+        for (int j = 0; j < Ny; j++)
+          for (int k = 0; k < Nx; k++) {
+            if (at(i, j, k) == -1) continue;
+            int *nb[6] = {&at(i == 0 ? Nz - 1 : i - 1, j, k), &at(i == Nz - 1 ? 0 : i + 1, j, k),
+                          &at(i, j == 0 ? Ny - 1 : j - 1, k), &at(i, j == Ny - 1 ? 0 : j + 1, k),
+                          &at(i, j, k == 0 ? Nx - 1 : k - 1), &at(i, j, k == Nx - 1 ? 0 : k + 1)};
+            for (int *c : nb) {
+              if (*c == -1 || *c == at(i, j, k)) continue;
+              at(i, j, k) = *c = std::min(at(i, j, k), *c);
+              done = false;
+            }
+          }
+    }
+    for (size_t p = 0; p < a.size(); p++) out[p] = a[p];
+  });
+}
+// Integrate::run_until_touching (src/integrate.cpp:221-415): full steps;
+// after each, |x[nowned-1] - x[0]| (held positions) against touch_point,
+// with the reference's since_last_cyc bookkeeping; returns 1 when it stopped
+int orc_run_until_touching(orc_sys *s, int nsteps, double touch_point, int continue_running, int delay_step,
+                           int cap, int64_t *occurred, int *n_occurred, int64_t *ended, int *n_ended,
+                           int *steps_run) {
+  int rc = 0, nocc = 0, nend = 0, done = 0;
+  int since = delay_step;
+  bool breakflag = false;
+  int g = guard(s, [&] {
+    if (s->nowned < 1) throw std::runtime_error("run_until_touching needs atoms");
+    for (int it = 0; it < nsteps; it++) {
+      s->timestep++;
+      for (int ph = ORC_PH_START; ph <= ORC_PH_POSTFINAL; ph++) run_phase(s, ph);
+      if (std::isnan(s->phi[0])) throw std::runtime_error("NAN encountered in phi.");
+      done = it + 1;
+      const double *a = &s->xs[0], *b = &s->xs[3 * (size_t)(s->nowned - 1)];
+      const double dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+      if (std::sqrt(dx * dx + dy * dy + dz * dz) < touch_point) {
+        if (since >= delay_step) { if (nocc < cap && occurred) occurred[nocc] = s->timestep; nocc++; }
+        since = 0;
+        breakflag = true;
+      } else {
+        if (breakflag && since == delay_step) { if (nend < cap && ended) ended[nend] = s->timestep - delay_step; nend++; }
+        since += 1;
+      }
+      if (!continue_running && breakflag) { rc = 1; break; }
+    }
+  });
+  if (n_occurred) *n_occurred = nocc;
+  if (n_ended) *n_ended = nend;
+  if (steps_run) *steps_run = done;
+  return g < 0 ? g : rc;
+}
 int orc_counts(orc_sys *s, int64_t *o) {
   o[0] = s->nowned; o[1] = s->ngathered; o[2] = s->nlocal; o[3] = s->nghost;
   o[4] = s->ncalls; o[5] = s->nless; o[6] = s->ngreat; o[7] = s->list_overflow ? 1 : 0;

@@ -101,6 +101,13 @@ int orc_phase(orc_sys *, int phase);
  * 0 ft_phi, 1 ft_chempot; quant 0 real, 1 imag, 2 modulus, 3 norm; out in
  * the reference's ft order [ky][kz][kx] */
 int orc_compute_complex(orc_sys *, int which, int quant, double *out);
+/* ComputeGridClusters::end_of_step, one rank (periodic z): condition 0 lt,
+ * 1 gt; out[Nz][Ny][Nx] cluster ids, -1 outside */
+int orc_compute_grid_clusters(orc_sys *, double threshold, int condition, double *out);
+/* Integrate::run_until_touching; arguments as phafd_hip_run_until_touching */
+int orc_run_until_touching(orc_sys *, int nsteps, double touch_point, int continue_running, int delay_step,
+                           int cap, int64_t *occurred, int *n_occurred, int64_t *ended, int *n_ended,
+                           int *steps_run);
 /* ComputePair::end_of_step: owned forces of pair style ipair alone */
 int orc_compute_pair(orc_sys *, int ipair, double *f);
 

@@ -53,6 +53,9 @@ def lib():
         L.orc_phase.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_compute_pair.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.orc_compute_complex.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_compute_grid_clusters.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_run_until_touching.argtypes = ([ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_int, _lp, _ip, _lp, _ip, _ip])
         L.orc_counts.argtypes = [ctypes.c_void_p, _lp]
         for name in ("orc_gridlist", "orc_halflist"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_int64, _ip, _ip, _ip, _lp, _lp]
@@ -235,6 +238,21 @@ class System:
         nz, ny, nx = self.shape
         out = np.zeros((ny, nz, nx // 2 + 1))
         self._chk(lib().orc_compute_complex(self.h, which, quant, dptr(out)))
+        return out
+
+    def run_until_touching(self, nsteps, touch_point, continue_running=False, delay_step=0, cap=1000):
+        occ, end = np.zeros(cap, np.int64), np.zeros(cap, np.int64)
+        no, ne, done = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        rc = self._chk(lib().orc_run_until_touching(
+            self.h, nsteps, touch_point, int(continue_running), delay_step, cap,
+            occ.ctypes.data_as(_lp), ctypes.byref(no), end.ctypes.data_as(_lp), ctypes.byref(ne),
+            ctypes.byref(done)))
+        return {"stopped": rc == 1, "steps": done.value, "occurred": occ[:no.value].tolist(),
+                "ended": end[:ne.value].tolist()}
+
+    def compute_grid_clusters(self, threshold, condition=0):
+        out = np.zeros(self.shape)
+        self._chk(lib().orc_compute_grid_clusters(self.h, threshold, condition, dptr(out)))
         return out
 
     def compute_pair(self, ipair):

@@ -150,6 +150,9 @@ def lib():
         "phafd_hip_dump_wait": ([V], ctypes.c_int),
         "phafd_hip_compute_pair": ([V, ctypes.c_int, _dp], ctypes.c_int),
         "phafd_hip_fourier_capture": ([V, ctypes.c_int], ctypes.c_int),
+        "phafd_hip_compute_grid_clusters": ([V, ctypes.c_double, ctypes.c_int, _dp], ctypes.c_int),
+        "phafd_hip_run_until_touching": ([V, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          _lp, _ip, _lp, _ip, _ip], ctypes.c_int),
         "phafd_hip_compute_complex": ([V, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
         "phafd_hip_grid_ave": ([V] + [ctypes.c_int] * 6 + [_dp], ctypes.c_int),
         "phafd_hip_read_dump_grid": ([V, _cp, ctypes.c_int, _pp, ctypes.c_int], ctypes.c_int),
@@ -220,6 +223,7 @@ def ic_random_chains(nchains, nbeads, bond, minsep, boxlo, period, seed):
 
 # ------------------------------------------------------------ dump files
 CPX_REAL, CPX_IMAG, CPX_MODULUS, CPX_NORM = 0, 1, 2, 3
+CLUSTER_LT, CLUSTER_GT = 0, 1
 VTK_FMT_LF, VTK_FMT_E = 0, 1
 PVD_HEADER, PVD_DATASET, PVD_FOOTER = 0, 1, 2
 GRID_ATTRIBUTES = ("phi", "chempot", "gradphi_x", "gradphi_y", "gradphi_z")
@@ -451,6 +455,24 @@ class System:
         f = np.zeros((self.natoms, 3))
         self._c(lib().phafd_hip_compute_pair(self.h, ipair, _d(f)))
         return f
+
+    def run_until_touching(self, nsteps, touch_point, continue_running=False, delay_step=0, cap=1000):
+        """run_until_touching: {"stopped", "steps", "occurred", "ended"}."""
+        occ, end = np.zeros(cap, np.int64), np.zeros(cap, np.int64)
+        no, ne, done = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        rc = self._c(lib().phafd_hip_run_until_touching(
+            self.h, nsteps, touch_point, int(continue_running), delay_step, cap,
+            occ.ctypes.data_as(_lp), ctypes.byref(no), end.ctypes.data_as(_lp), ctypes.byref(ne),
+            ctypes.byref(done)))
+        return {"stopped": rc == 1, "steps": done.value, "occurred": occ[:min(no.value, cap)].tolist(),
+                "ended": end[:min(ne.value, cap)].tolist()}
+
+    def compute_grid_clusters(self, threshold, condition="lt"):
+        """compute grid/clusters phi <threshold> lt|gt: (Nz, Ny, Nx) labels."""
+        cond = {"lt": CLUSTER_LT, "gt": CLUSTER_GT}.get(condition, condition)
+        out = np.zeros(self.shape)
+        self._c(lib().phafd_hip_compute_grid_clusters(self.h, threshold, cond, _d(out)))
+        return out
 
     def fourier_capture(self, on=True):
         """Keep ft_phi / ft_chempot of the last step of each later run."""

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -65,6 +66,16 @@ struct SwapDev {
 // a dump in flight: the fields or atom arrays snapshotted into HBM on the
 // step stream, copied to pinned host memory on the dump stream, written to
 // disk by the writer thread (while the steps go on when asynchronous)
+// Integrate::run_until_touching state (src/integrate.cpp:221-415)
+struct Touch {
+  double touch_point = 0;
+  bool cont = false;
+  int delay = 0, since = 0;
+  bool breakflag = false, stopped = false;
+  int steps = 0;
+  std::vector<int64_t> occurred, ended;
+};
+
 struct DumpJob {
   DBuf<double> stage;
   char *host = nullptr;
@@ -169,6 +180,10 @@ struct phafd_hip_ctx {
   bool cap_valid = false;
   DBuf<double2> cap_hat[2];
   DBuf<double> cc_out, ave[8];
+  DBuf<int> cl_parent, cl_minid;   // compute grid/clusters work arrays
+  struct Touch *touch = nullptr;    // run_until_touching in progress
+  double *h_ends = nullptr;         // pinned: x of the first and last owned atom (2 double4)
+  DBuf<double> cl_out;
   bool atom_windows = true;    // the surrogate's atom steps in speculative windows
   // styles
   std::vector<std::unique_ptr<PairDef>> pairs;
@@ -1038,6 +1053,31 @@ bool can_fuse_fh(phafd_hip_ctx *c, FHPreT<R> &pre) {
   return true;
 }
 
+// the check of Integrate::run_until_touching after a step
+// (src/integrate.cpp:355-385): the distance between the last and the first
+// owned atom (positions as held, no minimum image) against touch_point;
+// true when the run stops here
+bool touch_check(phafd_hip_ctx *c, int64_t step, int done) {
+  Touch &t = *c->touch;
+  const double *a = c->h_ends, *b = c->h_ends + 4;
+  const double dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+  const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+  t.steps = done;
+  if (dist < t.touch_point) {
+    if (t.since >= t.delay) t.occurred.push_back(step);   // "cyclisation occured at step"
+    t.since = 0;
+    t.breakflag = true;
+  } else {
+    if (t.breakflag && t.since == t.delay) t.ended.push_back(step - t.delay);   // "interval ended at step"
+    t.since += 1;
+  }
+  if (!t.cont && t.breakflag) {
+    t.stopped = true;
+    return true;
+  }
+  return false;
+}
+
 __global__ void k_widen_c(size_t n, const float2 *__restrict__ a, double2 *__restrict__ b) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     b[i] = make_double2(a[i].x, a[i].y);
@@ -1099,11 +1139,16 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
     if (c->neigh_set) enqueue_decide(c);
     enqueue_flags_readback(c);
     HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->fields<R>().phi.p, sizeof(R), hipMemcpyDeviceToHost, c->stream));
+    if (c->touch) {   // the chain ends' positions after the step, read back with the flags
+      HIPCHK(hipMemcpyAsync(c->h_ends, c->x.p, sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(c->h_ends + 4, c->x.p + (c->nowned - 1), sizeof(double4), hipMemcpyDeviceToHost,
+                            c->stream));
+    }
     HIPCHK(hipEventRecord(c->ev[8], c->stream));
     // The next step's start_of_step (grad phi: the fields only, not the
     // rebuild decision) goes in before the host waits for the read-back, so
     // the device works through the host's turnaround.
-    started = it + 1 < nsteps;
+    started = it + 1 < nsteps && !c->touch;   // a touching run may stop after any step
     if (started) {
       c->timestep++;
       HIPCHK(hipEventRecord(c->ev_pre[0], c->stream));
@@ -1136,6 +1181,7 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
       if (started) c->timestep = step;
       throw;
     }
+    if (c->touch && touch_check(c, step, it + 1)) break;
   }
 }
 
@@ -1345,6 +1391,7 @@ int phafd_hip_destroy(phafd_hip_ctx *c) {
   for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
   for (auto &e : c->ev_pre) if (e) (void)hipEventDestroy(e);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
+  if (c->h_ends) (void)hipHostFree(c->h_ends);
   for (auto &d : c->dump) {
     if (d.th.joinable()) d.th.join();
     if (d.cs) (void)hipStreamSynchronize(d.cs);
@@ -1795,6 +1842,28 @@ int phafd_hip_grid_ave(phafd_hip_ctx *c, int slot, int which, int quant, int fir
   });
 }
 
+int phafd_hip_compute_grid_clusters(phafd_hip_ctx *c, double threshold, int condition, double *out) {
+  return guard(c, [&] {
+    if (condition != PHAFD_CLUSTER_LT && condition != PHAFD_CLUSTER_GT)
+      throw std::invalid_argument("Must specify lt or gt for compute grid cluster.");
+    if (!out) throw std::invalid_argument("null output");
+    if (c->N > (size_t)INT_MAX) throw std::invalid_argument("grid too large for int cluster ids");
+    c->cl_parent.alloc(c->N);
+    c->cl_minid.alloc(c->N);
+    c->cl_out.alloc(c->N);
+    const bool gt = condition == PHAFD_CLUSTER_GT;
+    if (c->fp32)
+      launch_grid_clusters<float>(c->nx, c->ny, c->nz, c->fs.phi.p, threshold, gt, c->cl_parent.p, c->cl_minid.p,
+                                  c->cl_out.p, c->stream);
+    else
+      launch_grid_clusters<double>(c->nx, c->ny, c->nz, c->fd.phi.p, threshold, gt, c->cl_parent.p,
+                                   c->cl_minid.p, c->cl_out.p, c->stream);
+    HIPCHK(hipMemcpyAsync(out, c->cl_out.p, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return 0;
+  });
+}
+
 int phafd_hip_dump_wait(phafd_hip_ctx *c) {
   return guard(c, [&] {
     dump_wait(c);
@@ -2009,6 +2078,35 @@ int phafd_hip_run(phafd_hip_ctx *c, int nsteps) {
   return guard(c, [&] {
     run_steps(c, nsteps);
     return 0;
+  });
+}
+
+int phafd_hip_run_until_touching(phafd_hip_ctx *c, int nsteps, double touch_point, int continue_running,
+                                 int delay_step, int cap, int64_t *occurred, int *n_occurred, int64_t *ended,
+                                 int *n_ended, int *steps_run) {
+  return guard(c, [&] {
+    require_setup(c);
+    if (c->nowned < 1) throw std::runtime_error("run_until_touching needs atoms");
+    if (!c->h_ends) HIPCHK(hipHostMalloc((void **)&c->h_ends, 8 * sizeof(double), hipHostMallocDefault));
+    Touch t;
+    t.touch_point = touch_point;
+    t.cont = continue_running != 0;
+    t.delay = delay_step;
+    t.since = delay_step;   // since_last_cyc starts at delay_step
+    c->touch = &t;
+    try {
+      run_steps(c, nsteps);
+    } catch (...) {
+      c->touch = nullptr;
+      throw;
+    }
+    c->touch = nullptr;
+    if (n_occurred) *n_occurred = (int)t.occurred.size();
+    if (n_ended) *n_ended = (int)t.ended.size();
+    if (steps_run) *steps_run = t.steps;
+    for (int i = 0; i < (int)t.occurred.size() && i < cap && occurred; i++) occurred[i] = t.occurred[i];
+    for (int i = 0; i < (int)t.ended.size() && i < cap && ended; i++) ended[i] = t.ended[i];
+    return t.stopped ? 1 : 0;
   });
 }
 

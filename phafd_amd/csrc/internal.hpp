@@ -217,6 +217,13 @@ void scan_forget_stream(hipStream_t s);   // frees the scan workspace of a strea
 // (grown after draining s; kept until scan_forget_stream)
 double *reduce_partials(hipStream_t s, size_t n);
 
+// compute grid/clusters (clusters.hip): out[p] = the smallest reference id
+// i + (j*Nx + k)*Ny of p's periodic 6-connected component of points with
+// phi < thr (gt: phi > thr), -1 outside; parent and minid hold Nx*Ny*Nz ints
+template <class R>
+void launch_grid_clusters(int nx, int ny, int nz, const R *phi, double thr, bool gt, int *parent, int *minid,
+                          double *out, hipStream_t s);
+
 // ----------------------------------------------------------- atom kernels
 struct BinGeom {
   int nbinx, nbiny, nbinz, mbinx, mbiny, mbinz, mbinxlo, mbinylo, mbinzlo, mbins;
