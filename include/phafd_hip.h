@@ -122,7 +122,7 @@ int phafd_hip_grid_ave(phafd_hip_ctx *ctx, int slot, int which, int quant, int f
                        double *out);
 
 /* compute <id> grid/clusters phi <threshold> lt|gt: ComputeGridClusters::
- * end_of_step (src/compute_grid_clusters.cpp:62-168) on one subdomain, the
+ * end_of_step (src/compute_grid_clusters.cpp:66-228) on one subdomain, the
  * box periodic in x, y and z: out[Nz][Ny][Nx] = the smallest id
  * i + (j*Nx + k)*Ny ((i, j, k) = (z, y, x)) of the point's 6-connected
  * cluster of points with phi < threshold (lt) or > threshold (gt), -1

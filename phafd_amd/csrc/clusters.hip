@@ -1,5 +1,5 @@
 // clusters.hip -- compute grid/clusters (ComputeGridClusters::end_of_step,
-// src/compute_grid_clusters.cpp:62-168) on one subdomain holding the box.
+// src/compute_grid_clusters.cpp:66-228) on one subdomain holding the box.
 //
 // The reference marks every grid point beyond the threshold with the id
 // i + (j*Nx + k)*Ny ((i, j, k) = (z, y, x)) and sweeps min-propagation over
