@@ -1543,7 +1543,7 @@ int orc_compute_complex(orc_sys *s, int which, int quant, double *out) {
         }
   });
 }
-// ComputeGridClusters::end_of_step (src/compute_grid_clusters.cpp:62-168)
+// ComputeGridClusters::end_of_step (src/compute_grid_clusters.cpp:66-228)
 // with one rank: the reference takes its z neighbours from lefts / rights,
 // the end planes of the neighbouring ranks; with the box on one rank those
 // are this rank's own planes Nz-1 and 0, so z is periodic like x and y.
