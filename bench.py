@@ -302,6 +302,37 @@ def dump_leg(args, wl, phi, xyz, mol, plain_ms):
     return res
 
 
+def diagnostics_leg(args, wl, phi, xyz, mol):
+    """The computes beside the step on the same workload: grid clusters of
+    phi (union-find), compute complex of ft_phi at in_fourier, and
+    run_until_touching's per-step cost."""
+    import phafd_amd as P
+    n = wl["n"]
+    s = P.System(n, n, n, [0, 0, 0], [wl["box"]] * 3, zghost=wl["zghost"])
+    configure(s, wl, phi, xyz, mol)
+    s.run(args.warmup)
+    out = {}
+    lab = s.compute_grid_clusters(wl["phi0"], "lt")   # allocates the work arrays
+    t = time.perf_counter()
+    for _ in range(5):
+        lab = s.compute_grid_clusters(wl["phi0"], "lt")
+    out["grid_clusters_ms_incl_d2h"] = 1e3 * (time.perf_counter() - t) / 5
+    out["clusters"] = int(len(np.unique(lab[lab >= 0])))
+    s.fourier_capture(True)
+    s.run(1)
+    s.compute_complex(P.PHIHAT, P.CPX_NORM)
+    t = time.perf_counter()
+    for _ in range(5):
+        s.compute_complex(P.PHIHAT, P.CPX_NORM)
+    out["compute_complex_ms_incl_d2h"] = 1e3 * (time.perf_counter() - t) / 5
+    s.fourier_capture(False)
+    t = time.perf_counter()
+    r = s.run_until_touching(args.steps, 0.0, continue_running=True)
+    out["run_until_touching_ms_per_step"] = 1e3 * (time.perf_counter() - t) / r["steps"]
+    s.close()
+    return out
+
+
 def bench_batch(args):
     """Batched replicas: the batch timed over `steps`, then the same members
     run one after another as solo systems (the state of every member is
@@ -407,6 +438,8 @@ def main():
     ap.add_argument("--dump-every", type=int, default=0, metavar="K",
                     help="also time the run with grid and atom dumps every K steps (sync and async)")
     ap.add_argument("--dump-dir", default=None, help="directory for the dump leg's files (default: system temp)")
+    ap.add_argument("--diagnostics", action="store_true",
+                    help="also time grid clusters, compute complex and run_until_touching on the workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -499,6 +532,8 @@ def main():
         out["surrogate"] = sur
     if args.dump_every > 0:
         out["dumps"] = dump_leg(args, wl, phi, xyz, mol, ms_per_step)
+    if args.diagnostics:
+        out["diagnostics"] = diagnostics_leg(args, wl, phi, xyz, mol)
     if args.timers:
         print(json.dumps({"per_hook_ms_total": timers}), file=sys.stderr)
     if not args.no_cpu_baseline:

@@ -6,17 +6,17 @@
 // the 6 neighbours (periodic) until nothing changes, so a point ends with the
 // smallest id of its connected component.  Here the components come from a
 // concurrent union-find instead of sweeps (their number grows with the
-// cluster's diameter):
+// cluster's diameter), ordered by the key (id, index):
 //   init     parent[p] = p inside a cluster, -1 outside
 //   merge    every inside point unites with its +x, +y, +z neighbours; a
-//            root is linked under a smaller index with atomicCAS, so parent
-//            pointers only decrease and every find ends
+//            root is linked under the root of smaller key with atomicCAS, so
+//            keys strictly decrease along parent pointers, every find ends,
+//            and a root holds the smallest key of its tree
 //   flatten  parent[p] = root
-//   min id   atomicMin of the reference's id into the root's slot
-//   label    out[p] = that minimum (as a double, -1 outside)
-// The result is the component minimum whatever order the atomics take.
-#include <climits>
-
+//   label    out[p] = id(root) (as a double, -1 outside)
+// The root of a component is its member of smallest key, so the label is the
+// component's smallest id (ids collide when Nz > Ny; the index breaks ties)
+// whatever order the atomics take.
 #include "internal.hpp"
 
 namespace phafd {
@@ -26,18 +26,30 @@ __device__ __forceinline__ int ld(const int *p) { return __hip_atomic_load(p, __
 
 __device__ int uf_find(int *parent, int p) {
   int q = ld(parent + p);
-  while (q != p) {   // parent[p] <= p always: the chain strictly decreases
+  while (q != p) {   // keys strictly decrease along parent pointers
     p = q;
     q = ld(parent + p);
   }
   return p;
 }
 
-__device__ void uf_union(int *parent, int a, int b) {
+struct Grid3 {
+  int nx, ny, nz;
+  // the reference's id i + (j*Nx + k)*Ny of flat point p = (i*Ny + j)*Nx + k
+  __device__ int id(int p) const {
+    const int k = p % nx, j = (p / nx) % ny, i = p / (nx * ny);
+    return i + (j * nx + k) * ny;
+  }
+  __device__ unsigned long long key(int p) const {
+    return ((unsigned long long)(unsigned)id(p) << 32) | (unsigned)p;
+  }
+};
+
+__device__ void uf_union(const Grid3 &g, int *parent, int a, int b) {
   a = uf_find(parent, a);
   b = uf_find(parent, b);
   while (a != b) {
-    if (a < b) { const int t = a; a = b; b = t; }
+    if (g.key(a) < g.key(b)) { const int t = a; a = b; b = t; }   // link the larger key under the smaller
     const int old = atomicCAS(parent + a, a, b);
     if (old == a) return;
     a = uf_find(parent, old);
@@ -54,7 +66,8 @@ __global__ void k_cl_init(size_t n, const R *__restrict__ phi, double thr, int g
   }
 }
 
-__global__ void k_cl_merge(int nx, int ny, int nz, int *parent) {
+__global__ void k_cl_merge(Grid3 g, int *parent) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t n = (size_t)nx * ny * nz;
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
     if (ld(parent + p) < 0) continue;
@@ -63,32 +76,20 @@ __global__ void k_cl_merge(int nx, int ny, int nz, int *parent) {
                          p + (size_t)((y + 1 == ny ? 0 : y + 1) - y) * nx,
                          p + (size_t)((z + 1 == nz ? 0 : z + 1) - z) * nx * ny};
     for (int d = 0; d < 3; d++)
-      if (q[d] != p && ld(parent + q[d]) >= 0) uf_union(parent, (int)p, (int)q[d]);
+      if (q[d] != p && ld(parent + q[d]) >= 0) uf_union(g, parent, (int)p, (int)q[d]);
   }
 }
 
-__global__ void k_cl_flatten(size_t n, int *parent, int *__restrict__ minid) {
-  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-    minid[p] = INT_MAX;
+__global__ void k_cl_flatten(size_t n, int *parent) {
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x)
     if (ld(parent + p) >= 0) parent[p] = uf_find(parent, (int)p);
-  }
 }
 
-__global__ void k_cl_minid(int nx, int ny, int nz, const int *__restrict__ parent, int *minid) {
-  const size_t n = (size_t)nx * ny * nz;
+__global__ void k_cl_label(Grid3 g, const int *__restrict__ parent, double *__restrict__ out) {
+  const size_t n = (size_t)g.nx * g.ny * g.nz;
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
     const int r = parent[p];
-    if (r < 0) continue;
-    const int k = (int)(p % nx), j = (int)((p / nx) % ny), i = (int)(p / ((size_t)nx * ny));
-    atomicMin(minid + r, i + (j * nx + k) * ny);
-  }
-}
-
-__global__ void k_cl_label(size_t n, const int *__restrict__ parent, const int *__restrict__ minid,
-                           double *__restrict__ out) {
-  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-    const int r = parent[p];
-    out[p] = r < 0 ? -1.0 : (double)minid[r];
+    out[p] = r < 0 ? -1.0 : (double)g.id(r);
   }
 }
 
@@ -97,20 +98,19 @@ int grid_blocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 16384)
 }  // namespace
 
 template <class R>
-void launch_grid_clusters(int nx, int ny, int nz, const R *phi, double thr, bool gt, int *parent, int *minid,
-                          double *out, hipStream_t s) {
+void launch_grid_clusters(int nx, int ny, int nz, const R *phi, double thr, bool gt, int *parent, double *out,
+                          hipStream_t s) {
   const size_t n = (size_t)nx * ny * nz;
   const int b = grid_blocks(n);
+  const Grid3 g{nx, ny, nz};
   hipLaunchKernelGGL(k_cl_init<R>, dim3(b), dim3(256), 0, s, n, phi, thr, gt ? 1 : 0, parent);
-  hipLaunchKernelGGL(k_cl_merge, dim3(b), dim3(256), 0, s, nx, ny, nz, parent);
-  hipLaunchKernelGGL(k_cl_flatten, dim3(b), dim3(256), 0, s, n, parent, minid);
-  hipLaunchKernelGGL(k_cl_minid, dim3(b), dim3(256), 0, s, nx, ny, nz, parent, minid);
-  hipLaunchKernelGGL(k_cl_label, dim3(b), dim3(256), 0, s, n, parent, minid, out);
+  hipLaunchKernelGGL(k_cl_merge, dim3(b), dim3(256), 0, s, g, parent);
+  hipLaunchKernelGGL(k_cl_flatten, dim3(b), dim3(256), 0, s, n, parent);
+  hipLaunchKernelGGL(k_cl_label, dim3(b), dim3(256), 0, s, g, parent, out);
   HIPCHK(hipGetLastError());
 }
-template void launch_grid_clusters<double>(int, int, int, const double *, double, bool, int *, int *, double *,
+template void launch_grid_clusters<double>(int, int, int, const double *, double, bool, int *, double *,
                                            hipStream_t);
-template void launch_grid_clusters<float>(int, int, int, const float *, double, bool, int *, int *, double *,
-                                          hipStream_t);
+template void launch_grid_clusters<float>(int, int, int, const float *, double, bool, int *, double *, hipStream_t);
 
 }  // namespace phafd

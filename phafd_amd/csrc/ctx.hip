@@ -180,7 +180,7 @@ struct phafd_hip_ctx {
   bool cap_valid = false;
   DBuf<double2> cap_hat[2];
   DBuf<double> cc_out, ave[8];
-  DBuf<int> cl_parent, cl_minid;   // compute grid/clusters work arrays
+  DBuf<int> cl_parent;   // compute grid/clusters union-find
   struct Touch *touch = nullptr;    // run_until_touching in progress
   double *h_ends = nullptr;         // pinned: x of the first and last owned atom (2 double4)
   DBuf<double> cl_out;
@@ -1849,15 +1849,14 @@ int phafd_hip_compute_grid_clusters(phafd_hip_ctx *c, double threshold, int cond
     if (!out) throw std::invalid_argument("null output");
     if (c->N > (size_t)INT_MAX) throw std::invalid_argument("grid too large for int cluster ids");
     c->cl_parent.alloc(c->N);
-    c->cl_minid.alloc(c->N);
     c->cl_out.alloc(c->N);
     const bool gt = condition == PHAFD_CLUSTER_GT;
     if (c->fp32)
-      launch_grid_clusters<float>(c->nx, c->ny, c->nz, c->fs.phi.p, threshold, gt, c->cl_parent.p, c->cl_minid.p,
-                                  c->cl_out.p, c->stream);
+      launch_grid_clusters<float>(c->nx, c->ny, c->nz, c->fs.phi.p, threshold, gt, c->cl_parent.p, c->cl_out.p,
+                                  c->stream);
     else
-      launch_grid_clusters<double>(c->nx, c->ny, c->nz, c->fd.phi.p, threshold, gt, c->cl_parent.p,
-                                   c->cl_minid.p, c->cl_out.p, c->stream);
+      launch_grid_clusters<double>(c->nx, c->ny, c->nz, c->fd.phi.p, threshold, gt, c->cl_parent.p, c->cl_out.p,
+                                   c->stream);
     HIPCHK(hipMemcpyAsync(out, c->cl_out.p, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
     sync(c);
     return 0;

@@ -219,10 +219,10 @@ double *reduce_partials(hipStream_t s, size_t n);
 
 // compute grid/clusters (clusters.hip): out[p] = the smallest reference id
 // i + (j*Nx + k)*Ny of p's periodic 6-connected component of points with
-// phi < thr (gt: phi > thr), -1 outside; parent and minid hold Nx*Ny*Nz ints
+// phi < thr (gt: phi > thr), -1 outside; parent holds Nx*Ny*Nz ints
 template <class R>
-void launch_grid_clusters(int nx, int ny, int nz, const R *phi, double thr, bool gt, int *parent, int *minid,
-                          double *out, hipStream_t s);
+void launch_grid_clusters(int nx, int ny, int nz, const R *phi, double thr, bool gt, int *parent, double *out,
+                          hipStream_t s);
 
 // ----------------------------------------------------------- atom kernels
 struct BinGeom {
