@@ -103,19 +103,29 @@ class Dump:
         A.vtk_write_atoms(instance, [vals[w] for w in self.attributes], self.attributes, self.precision, self.fmt)
 
 
-def run_with_dumps(system, nsteps, dumps, dt, first_step=0):
-    """Integrate::run with dumps: step, and after every step write the dumps
-    that are due (src/integrate.cpp:180-190); the runs between dumps are
-    single calls, so the steps are not synchronised for the dumps."""
+def run_with_dumps(system, nsteps, dumps, dt, first_step=0, aves=()):
+    """Integrate::run with dumps and fix grid/ave: the runs stop at every step
+    a dump or an average needs (src/integrate.cpp:105-203).  At such a step
+    the computes run first (ComputePair::end_of_step), then the averages take
+    their samples (the run's last step kept its in_fourier spectra for them),
+    then the dumps are written (src/integrate.cpp:172-190).  The runs between
+    stops are single calls, so the steps are not synchronised for them."""
     step = first_step
     end = first_step + nsteps
     while step < end:
-        nxt = min([end] + [((step // d.every) + 1) * d.every for d in dumps])
+        nxt = min([end] + [((step // d.every) + 1) * d.every for d in dumps] + [a.next_sample(step) for a in aves])
+        sampling = [a for a in aves if a.samples_at(nxt)]
+        if aves:
+            system.fourier_capture(bool(sampling))
         system.run(nxt - step)
         step = nxt
         due = [d for d in dumps if d.due(step)]
         for d in due:
             d.end_of_step()
+        for a in sampling:
+            a.end_of_step(step)
         for d in due:
             d.write_collection_middle(step, dt)
+    if aves:
+        system.fourier_capture(False)
     return step

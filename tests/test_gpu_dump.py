@@ -186,3 +186,26 @@ def test_example_dumps_with_compute_pair_against_oracle(tmp_path):
     phi, mu = P.vtk_read_grid(f"{base}field_p0_20.vti", s.shape, ["phi", "chempot"])
     assert rel_l2(phi, o.get_field(P.PHI)) < 1e-10
     assert rel_l2(mu, o.get_field(P.MU)) < 1e-10
+
+
+def test_dumps_and_grid_ave_in_one_run(tmp_path):
+    # a grid dump every 5 steps beside fix grid/ave every 2 repeat 3 freq 10:
+    # the average equals the one of a run with the average alone, the dumped
+    # phi that of a run with the dumps alone
+    from phafd_amd.computes import ComputeComplex, FixGridAve, run_with_aves
+    w = workload(n=32, nchains=6, nbeads=20, seed=26)
+    a, b, c = make_gpu(w), make_gpu(w), make_gpu(w)
+    ave_a = FixGridAve(a, "ave", 2, 3, 10, ComputeComplex(a, "sq", "ft_phi", "modulus"))
+    grid_a = Dump(a, "g", "grid", str(tmp_path / "a"), 5, ["phi"], async_=True)
+    grid_a.write_collection_header()
+    run_with_dumps(a, 20, [grid_a], w["dt"], aves=[ave_a])
+    a.dump_wait()
+    ave_b = FixGridAve(b, "ave", 2, 3, 10, ComputeComplex(b, "sq", "ft_phi", "modulus"))
+    run_with_aves(b, 20, [ave_b])
+    assert np.array_equal(ave_a.array, ave_b.array)
+    grid_c = Dump(c, "g", "grid", str(tmp_path / "c"), 5, ["phi"])
+    grid_c.write_collection_header()
+    run_with_dumps(c, 20, [grid_c], w["dt"])
+    c.dump_wait()   # dumps are asynchronous by default: the last one may still be writing
+    for t in (5, 10, 15, 20):
+        assert _read(str(tmp_path / f"a_p0_{t}.vti")) == _read(str(tmp_path / f"c_p0_{t}.vti"))
