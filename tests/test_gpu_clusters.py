@@ -97,3 +97,15 @@ def test_run_until_touching_matches_oracle():
         o = make_oracle(w).run_until_touching(60, tp, continue_running=cont, delay_step=delay)
         assert g == o, (cont, delay)
     assert g["stopped"] and g["steps"] < 60
+
+
+def test_clusters_fp32_form():
+    # FP32 fields: the labels of the float field (read back widened, exactly)
+    # equal the oracle's on the same values
+    phi = 0.5 + 0.1 * np.random.default_rng(53).standard_normal((16, 32, 32))
+    nz, ny, nx = phi.shape
+    s = P.System(nx, ny, nz, [0, 0, 0], [float(nx), float(ny), float(nz)], precision=P.FP32)
+    s.set_field(P.PHI, phi)
+    got = s.compute_grid_clusters(0.5, "gt")
+    ref = _oracle_field(s.get_field(P.PHI)).compute_grid_clusters(0.5, 1)
+    assert np.array_equal(got, ref)

@@ -88,3 +88,18 @@ def test_grid_ave_windows_against_oracle():
     for a, b in zip(windows, got):
         assert rel_l2(a, b) < 1e-10
     assert rel_l2(g.get_field(P.PHI), o.get_field(P.PHI)) < 1e-10
+
+
+def test_compute_complex_fp32_form():
+    # the FP32 form keeps float spectra: the capture widens them.  Against the
+    # oracle at 2e-4: the non-mean modes carry FP32 rounding relative to the
+    # fluctuation, not to phi (imag, which has no mean mode, measures 6e-5;
+    # the FP32 form's grad phi is at 3e-5 for the same reason)
+    w = workload(n=64, nchains=8, nbeads=20, seed=45, zghost=1)
+    g, o = make_gpu(w, P.FP32), make_oracle(w)
+    g.fourier_capture(True)
+    g.run(6)
+    o.run(5)
+    ref = _oracle_step_with(o, lambda: {q: o.compute_complex(0, q) for q in range(4)})
+    for q, r in ref.items():
+        assert rel_l2(g.compute_complex(P.PHIHAT, q), r) < 2e-4, q
