@@ -296,6 +296,14 @@ def dump_leg(args, wl, phi, xyz, mol, plain_ms):
             files = [os.path.join(d, f) for f in os.listdir(d)]
             res[f"ms_per_step_{mode}"] = 1e3 * el / args.steps
             res["bytes_written"] = sum(os.path.getsize(f) for f in files)
+            if mode == "sync":   # one dump of each kind alone, buffers allocated
+                t1 = time.perf_counter()
+                s.dump_grid(os.path.join(d, "one.vti"), grid.attributes)
+                t2 = time.perf_counter()
+                s.dump_atoms(os.path.join(d, "one.vtp"), atoms.attributes)
+                t3 = time.perf_counter()
+                res["grid_dump_ms"] = 1e3 * (t2 - t1)
+                res["atom_dump_ms"] = 1e3 * (t3 - t2)
             s.close()
         finally:
             shutil.rmtree(d, ignore_errors=True)
