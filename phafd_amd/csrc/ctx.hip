@@ -1617,6 +1617,11 @@ int phafd_hip_dump_grid(phafd_hip_ctx *c, const char *path, int nattr, const cha
     std::vector<int> ids;
     for (auto &w : at) ids.push_back(grid_attribute(w));
     const size_t N = c->N, bytes = (size_t)nattr * N * sizeof(double);
+    // the format's limits, checked before anything is enqueued (the writer
+    // checks them again): a 32-bit byte length and int offsets
+    if (N * sizeof(double) > (size_t)UINT32_MAX) throw std::invalid_argument("grid too large for a 32-bit VTK byte length");
+    if ((size_t)(nattr - 1) * (N * sizeof(double) + sizeof(unsigned)) > (size_t)INT_MAX)
+      throw std::invalid_argument("appended offsets exceed the format's int range");
     dump_prepare(job, bytes);
     for (int k = 0; k < nattr; k++) {
       double *dst = job.stage.p + (size_t)k * N;
