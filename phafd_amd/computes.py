@@ -85,6 +85,8 @@ def run_with_aves(system, nsteps, aves, first_step=0):
     """Steps with grid/ave sampling: runs stop at sample steps, whose last step
     keeps its spectra (the capture is off for the other runs)."""
     step, end = first_step, first_step + nsteps
+    for a in aves:   # FixGridAve::setup at the start of every run (src/fixgrid_ave.cpp:99-125)
+        a.setup(first_step)
     while step < end:
         nxt = min([end] + [a.next_sample(step) for a in aves])
         due = [a for a in aves if a.samples_at(nxt)]

@@ -112,6 +112,8 @@ def run_with_dumps(system, nsteps, dumps, dt, first_step=0, aves=()):
     stops are single calls, so the steps are not synchronised for them."""
     step = first_step
     end = first_step + nsteps
+    for a in aves:   # FixGridAve::setup at the start of every run (src/fixgrid_ave.cpp:99-125)
+        a.setup(first_step)
     while step < end:
         nxt = min([end] + [((step // d.every) + 1) * d.every for d in dumps] + [a.next_sample(step) for a in aves])
         sampling = [a for a in aves if a.samples_at(nxt)]

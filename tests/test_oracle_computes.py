@@ -47,3 +47,42 @@ def test_grid_ave_schedule():
     assert save_steps(2, 3, 10, 2) == [6, 8, 10, 16, 18, 20]
     assert save_steps(5, 1, 5, 3) == [5, 10, 15]
     assert save_steps(1, 4, 4, 1) == [1, 2, 3, 4]
+
+
+class _FakeSystem:
+    """Records the run driver's calls (no GPU): run lengths, the capture flag
+    of each run, and the grid/ave samples."""
+
+    def __init__(self):
+        self.runs, self.capture, self.samples = [], False, []
+        self.shape = (4, 4, 4)
+
+    def fourier_capture(self, on=True):
+        self.capture = bool(on)
+
+    def run(self, n):
+        self.runs.append((n, self.capture))
+
+    def grid_ave(self, slot, which, quant, first, last, repeat, want=False):
+        self.samples.append((first, last))
+        return np.zeros(1) if want else None
+
+    def dump_wait(self):
+        pass
+
+
+def test_run_driver_stops_and_captures_at_samples():
+    from phafd_amd.computes import ComputeComplex, FixGridAve
+    from phafd_amd.dump import run_with_dumps
+    s = _FakeSystem()
+    ave = FixGridAve(s, "ave", 2, 3, 10, ComputeComplex(s, "sq", "ft_phi", "norm"))
+    end = run_with_dumps(s, 25, [], 1e-3, first_step=100, aves=[ave])
+    assert end == 125
+    # a run restarts the schedule: samples at 106 108 110, 116 118 120
+    steps, t = [], 100
+    for n, cap in s.runs:
+        t += n
+        steps.append((t, cap))
+    assert steps == [(106, True), (108, True), (110, True), (116, True), (118, True), (120, True), (125, False)]
+    assert s.samples == [(True, False), (False, False), (False, True)] * 2
+    assert s.capture is False
