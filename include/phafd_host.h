@@ -61,6 +61,12 @@ int phafd_vtk_write_atoms(const char *path, int n, int nattr, const char *const 
                           int precision, int fmt);
 /* Dump::write_atom_timestep_pvtp (src/dump.cpp:422-483) */
 int phafd_vtk_write_patoms(const char *path, int nattr, const char *const *names, const char *source);
+/* dump <name> lammpsatom <file> <every> attributes n ...: Dump::write_lammps_dump
+ * (src/dump.cpp:485-679), one timestep appended to path (create or truncate
+ * the file when the dump is set up); arrays as phafd_vtk_write_atoms,
+ * boxlo/boxhi the box bounds */
+int phafd_lammps_write_step(const char *path, int64_t timestep, int n, const double *boxlo, const double *boxhi,
+                            int nattr, const char *const *names, const void *const *arrays, int precision, int fmt);
 /* Dump::write_collection_header / _middle / _footer (src/dump.cpp:235-313):
  * part PHAFD_VTK_PVD_*; time and file are used by the data-set line */
 int phafd_vtk_collection(const char *path, int part, double time, const char *file);

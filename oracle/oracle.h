@@ -122,6 +122,8 @@ int orc_vtk_write_atoms(const char *path, int n, int nattr, const char *const *n
                         int precision, int fmt);
 int orc_vtk_write_patoms(const char *path, int nattr, const char *const *names, const char *source);
 int orc_vtk_collection(const char *path, int part, double time, const char *file);
+int orc_lammps_write_step(const char *path, int64_t timestep, int n, const double *boxlo, const double *boxhi,
+                          int nattr, const char *const *names, const void *const *arrays, int precision, int fmt);
 
 #ifdef __cplusplus
 }

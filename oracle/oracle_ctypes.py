@@ -81,6 +81,8 @@ def lib():
         L.orc_vtk_write_atoms.argtypes = [_cp, ctypes.c_int, ctypes.c_int, _pp, _vpp, ctypes.c_int, ctypes.c_int]
         L.orc_vtk_write_patoms.argtypes = [_cp, ctypes.c_int, _pp, _cp]
         L.orc_vtk_collection.argtypes = [_cp, ctypes.c_int, ctypes.c_double, _cp]
+        L.orc_lammps_write_step.argtypes = [_cp, ctypes.c_int64, ctypes.c_int, _dp, _dp, ctypes.c_int, _pp, _vpp,
+                                            ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -391,3 +393,10 @@ def vtk_write_patoms(path, names, source):
 
 def vtk_collection(path, part, time=0.0, file=""):
     _vtk(lib().orc_vtk_collection(path.encode(), part, time, file.encode()))
+
+
+def lammps_write_step(path, timestep, arrays, names, boxlo, boxhi, precision=-1, fmt=0):
+    conv = [np.ascontiguousarray(a, dtype=np.float64 if a.ndim == 2 else np.int32) for a in arrays]
+    ptrs = (ctypes.c_void_p * len(conv))(*[a.ctypes.data for a in conv])
+    _vtk(lib().orc_lammps_write_step(path.encode(), timestep, len(conv[0]), (ctypes.c_double * 3)(*boxlo),
+                                     (ctypes.c_double * 3)(*boxhi), len(names), _names(names), ptrs, precision, fmt))

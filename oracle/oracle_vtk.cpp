@@ -175,6 +175,47 @@ int orc_vtk_write_patoms(const char *path, int nattr, const char *const *names, 
   });
 }
 
+// Dump::write_lammps_dump, src/dump.cpp:485-679 (owned atoms, appended)
+int orc_lammps_write_step(const char *path, int64_t timestep, int n, const double *boxlo, const double *boxhi,
+                          int nattr, const char *const *names, const void *const *arrays, int precision, int fmt) {
+  return vguard([&] {
+    std::fstream f(path, std::ios::out | std::ios::app);
+    if (!f.is_open()) throw std::runtime_error("Cannot open file");
+    if (precision > 0 && fmt == 0) f << std::fixed << std::setprecision(precision);
+    else if (precision > 0 && fmt == 1) f << std::scientific << std::setprecision(precision);
+    else if (fmt == 1) f << std::scientific;
+    f << "ITEM: TIMESTEP" << std::endl << timestep << std::endl;
+    f << "ITEM: NUMBER OF ATOMS" << std::endl << n << std::endl;
+    f << "ITEM: BOX BOUNDS pp pp pp" << std::endl
+      << boxlo[0] << " " << boxhi[0] << std::endl
+      << boxlo[1] << " " << boxhi[1] << std::endl
+      << boxlo[2] << " " << boxhi[2] << std::endl;
+    f << "ITEM: ATOMS ";
+    for (int k = 0; k < nattr; k++) {
+      std::string w = names[k];
+      if (w == "x") f << "x y z ";
+      else if (w == "F") f << "Fx Fy Fz ";
+      else if (w == "ux") f << "ux uy uz ";
+      else if (is_vec3(w))
+        for (int c = 0; c < 3; c++) f << w << "_" << c << " ";
+      else f << w << " ";
+    }
+    f << std::endl;
+    for (int i = 0; i < n; i++) {
+      for (int k = 0; k < nattr; k++) {
+        if (is_vec3(names[k])) {
+          const double *v = (const double *)arrays[k] + 3 * (size_t)i;
+          f << v[0] << " " << v[1] << " " << v[2] << " ";
+        } else {
+          f << ((const int *)arrays[k])[i] << " ";
+        }
+      }
+      f << std::endl;
+    }
+    return 0;
+  });
+}
+
 // Dump::write_collection_header / _middle / _footer, src/dump.cpp:235-313
 int orc_vtk_collection(const char *path, int part, double time, const char *file) {
   return vguard([&] {

@@ -142,6 +142,8 @@ def lib():
                                   ctypes.c_int),
         "phafd_vtk_write_patoms": ([_cp, ctypes.c_int, _pp, _cp], ctypes.c_int),
         "phafd_vtk_collection": ([_cp, ctypes.c_int, ctypes.c_double, _cp], ctypes.c_int),
+        "phafd_lammps_write_step": ([_cp, ctypes.c_int64, ctypes.c_int, _dp, _dp, ctypes.c_int, _pp, _vpp,
+                                     ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "phafd_vtk_read_grid": ([_cp] + [ctypes.c_int] * 4 + [_pp, ctypes.POINTER(_dp), ctypes.c_int],
                                 ctypes.c_int),
         "phafd_vtk_read_atoms": ([_cp, ctypes.c_int, ctypes.c_int, _pp, ctypes.c_int, _vpp], ctypes.c_int),
@@ -276,6 +278,16 @@ def vtk_write_atoms(path, arrays, names, precision=-1, fmt="lf"):
     n, nm = _names(names)
     ptrs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in conv])
     _host(lib().phafd_vtk_write_atoms(str(path).encode(), len(conv[0]), n, nm, ptrs, precision, _fmt(fmt)))
+
+
+def lammps_write_step(path, timestep, arrays, names, boxlo, boxhi, precision=-1, fmt="lf"):
+    """Dump::write_lammps_dump: one timestep appended to a LAMMPS-style file."""
+    conv = [np.ascontiguousarray(a, dtype=np.float64 if (w in ATOM_VEC3 or w.startswith("c_")) else np.int32)
+            for a, w in zip(arrays, names)]
+    n, nm = _names(names)
+    ptrs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in conv])
+    _host(lib().phafd_lammps_write_step(str(path).encode(), int(timestep), len(conv[0]), _c3(boxlo), _c3(boxhi),
+                                        n, nm, ptrs, precision, _fmt(fmt)))
 
 
 def vtk_write_patoms(path, names, source):

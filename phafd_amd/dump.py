@@ -5,6 +5,7 @@ collection listing the timesteps.
     dump <name> grid <base> <every> attributes n phi chempot ...
     dump <name> atom/owned <base> <every> attributes n x ux ix F ID molID type
          [decimalplaces p] [fmt lf|e]
+    dump <name> lammpsatom <base> <every> attributes n ...   (one text file)
 
 File names follow src/dump.cpp:315-334: <base>_p0_<step>.vti|vtp for the
 data, <base>_<step>.pvti|pvtp for the header, <base>.pvd for the collection.
@@ -19,7 +20,9 @@ from . import api as A
 class Dump:
     def __init__(self, system, name, dump_type, base_name, every, attributes, decimalplaces=None, fmt="lf",
                  async_=True, computes=None):
-        if dump_type.startswith("atom"):
+        if dump_type == "lammpsatom":
+            self.kind, self.fext = "lammps", "lmmpstrj"
+        elif dump_type.startswith("atom"):
             which = dump_type[5:]
             if which != "owned":
                 raise ValueError("only atom/owned dumps are written here (one subdomain holds every atom)")
@@ -39,7 +42,7 @@ class Dump:
         for w in attributes:
             if w.startswith("c_") and w[2:] not in self.computes:
                 raise ValueError("Cannot write dump, compute ID " + w[2:] + "doesn't exist.")
-            if w.startswith("c_") and self.kind != "atom":
+            if w.startswith("c_") and self.kind == "grid":
                 raise ValueError("Cannot write dump, compute ID " + w[2:] + "is not a per_grid quantity.")
         self._computed = None
         self.system, self.name, self.every = system, name, int(every)
@@ -52,11 +55,17 @@ class Dump:
         self.collection_name = base_name + ".pvd"
 
     def write_collection_header(self):
+        """The collection header; a lammpsatom dump instead creates its one
+        file (Dump::setup, src/dump.cpp:204-207, 334-346)."""
+        if self.kind == "lammps":
+            open(f"{self.base_name}_p0.{self.fext}", "w").close()
+            return
         A.vtk_collection(self.collection_name, A.PVD_HEADER)
 
     def write_collection_footer(self):
         self.system.dump_wait()
-        A.vtk_collection(self.collection_name, A.PVD_FOOTER)
+        if self.kind != "lammps":
+            A.vtk_collection(self.collection_name, A.PVD_FOOTER)
 
     def due(self, timestep):
         return timestep % self.every == 0
@@ -64,6 +73,12 @@ class Dump:
     def write_collection_middle(self, timestep, dt):
         """The data-set line, then this timestep's file and its header."""
         s = self.system
+        if self.kind == "lammps":   # no collection line, one file for every step
+            vals = self._atom_values()
+            top = [lo + p for lo, p in zip(s.boxlo, s.period)]
+            A.lammps_write_step(f"{self.base_name}_p0.{self.fext}", timestep, [vals[w] for w in self.attributes],
+                                self.attributes, s.boxlo, top, self.precision, self.fmt)
+            return
         instance = f"{self.base_name}_p0_{timestep}.{self.fext}"
         pinstance = f"{self.base_name}_{timestep}.p{self.fext}"
         source = f"{self.nopath_base_name}_p0_{timestep}.{self.fext}"
@@ -90,16 +105,21 @@ class Dump:
         self._computed = {w: self.system.compute_pair(self.computes[w[2:]])
                           for w in self.attributes if w.startswith("c_")} or None
 
-    def _atoms_with_computes(self, instance):
-        """An atom dump holding compute pair arrays, written from the host."""
+    def _atom_values(self):
+        """Every atom attribute of this step from the device, computes included."""
         s = self.system
         if self._computed is None:
             self.end_of_step()
         x, ux, img = s.get_atoms()
         vals = {"x": x, "ux": ux, "ix": img, "F": s.get_forces(), "label": np.zeros(len(x), np.int32)}
         vals.update(s.atom_ints)
-        vals.update(self._computed)
+        vals.update(self._computed or {})
         self._computed = None
+        return vals
+
+    def _atoms_with_computes(self, instance):
+        """An atom dump holding compute pair arrays, written from the host."""
+        vals = self._atom_values()
         A.vtk_write_atoms(instance, [vals[w] for w in self.attributes], self.attributes, self.precision, self.fmt)
 
 

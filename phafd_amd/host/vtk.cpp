@@ -309,6 +309,55 @@ int phafd_vtk_write_patoms(const char *path, int nattr, const char *const *names
   });
 }
 
+// Dump::write_lammps_dump (src/dump.cpp:485-679): one timestep appended to
+// a LAMMPS-style text file (the dump setup truncates it, src/dump.cpp:333-346);
+// the box bounds are printed with the same stream flags as the values
+int phafd_lammps_write_step(const char *path, int64_t timestep, int n, const double *boxlo, const double *boxhi,
+                            int nattr, const char *const *names, const void *const *arrays, int precision, int fmt) {
+  return guard([&] {
+    auto at = name_list(nattr, names);
+    if (n < 0 || !arrays || !boxlo || !boxhi) throw std::invalid_argument("bad arguments");
+    if (fmt != PHAFD_VTK_FMT_LF && fmt != PHAFD_VTK_FMT_E)
+      throw std::invalid_argument("Dump fmt must be either lf or e.");
+    std::vector<Kind> kind;
+    for (int k = 0; k < nattr; k++) {
+      kind.push_back(atom_kind(at[k]));
+      if (n > 0 && !arrays[k]) throw std::invalid_argument("null array for " + at[k]);
+    }
+    Out o;
+    o.nf = num_fmt(precision, fmt);
+    o.s.reserve((size_t)n * 16 * (3 * nattr) + 1024);
+    o << "ITEM: TIMESTEP\n" << (long long)timestep << "\n" << "ITEM: NUMBER OF ATOMS\n" << n << "\n"
+      << "ITEM: BOX BOUNDS pp pp pp\n";
+    for (int d = 0; d < 3; d++) o << boxlo[d] << " " << boxhi[d] << "\n";
+    o << "ITEM: ATOMS ";
+    for (int k = 0; k < nattr; k++) {
+      const std::string &w = at[k];
+      if (w == "x") o << "x y z ";
+      else if (w == "F") o << "Fx Fy Fz ";
+      else if (w == "ux") o << "ux uy uz ";
+      else if (kind[k] == VEC3) o << w << "_0 " << w << "_1 " << w << "_2 ";   // a compute of 3 components
+      else o << w << " ";
+    }
+    o << "\n";
+    for (int i = 0; i < n; i++) {
+      for (int k = 0; k < nattr; k++) {
+        if (kind[k] == VEC3) {
+          const double *v = static_cast<const double *>(arrays[k]) + 3 * (size_t)i;
+          o << v[0] << " " << v[1] << " " << v[2] << " ";
+        } else {
+          o << static_cast<const int *>(arrays[k])[i] << " ";
+        }
+      }
+      o << "\n";
+    }
+    File f(path, "ab");
+    f.put(o.s);
+    f.close();
+    return 0;
+  });
+}
+
 // Dump::write_collection_header / _middle / _footer (src/dump.cpp:235-313)
 int phafd_vtk_collection(const char *path, int part, double time, const char *file) {
   return guard([&] {

@@ -243,3 +243,25 @@ def test_atom_reader_round_trip_and_required_attributes(tmp_path):
     assert P.read_dump_atoms(p, 50, ["x", "ID", "type"], molecule=False)["ID"][-1] == 49
     with pytest.raises(P.PhafdError, match="Only found attributes"):
         P.read_dump_atoms(p, 50, ["x", "ID", "type", "label"], molecule=False)
+
+
+@pytest.mark.parametrize("precision,fmt", [(-1, 0), (-1, 1), (4, 0), (17, 1)])
+def test_lammps_dump_matches_oracle_and_reads_back(tmp_path, precision, fmt):
+    a = _atoms(200, 9)
+    a["c_pf"] = a["F"] * -0.5
+    names = ["x", "F", "c_pf", "ID", "molID", "type", "ix"]
+    lo, hi = [-128.0, -128.0, -128.0], [128.0, 128.0, 128.0]
+    o, g = str(tmp_path / "o.lmmpstrj"), str(tmp_path / "g.lmmpstrj")
+    for t in (0, 100):   # two steps appended to one file
+        O.lammps_write_step(o, t, [a[w] for w in names], names, lo, hi, precision, fmt)
+        P.lammps_write_step(g, t, [a[w] for w in names], names, lo, hi, precision, "e" if fmt else "lf")
+    assert _read(o) == _read(g)
+    lines = _read(o).decode().split("\n")
+    f = _py_fmt(precision, fmt)
+    assert lines[:4] == ["ITEM: TIMESTEP", "0", "ITEM: NUMBER OF ATOMS", "200"]
+    assert lines[5] == f"{f(-128.0)} {f(128.0)}"
+    assert lines[8] == "ITEM: ATOMS x y z Fx Fy Fz c_pf_0 c_pf_1 c_pf_2 ID molID type ix "
+    row = lines[9].split()
+    assert row[:3] == [f(v) for v in a["x"][0]] and row[6:9] == [f(v) for v in a["c_pf"][0]]
+    assert row[9:] == [str(a[w][0]) for w in ("ID", "molID", "type", "ix")]
+    assert lines[9 + 200] == "ITEM: TIMESTEP" and lines[10 + 200] == "100"

@@ -209,3 +209,27 @@ def test_dumps_and_grid_ave_in_one_run(tmp_path):
     c.dump_wait()   # dumps are asynchronous by default: the last one may still be writing
     for t in (5, 10, 15, 20):
         assert _read(str(tmp_path / f"a_p0_{t}.vti")) == _read(str(tmp_path / f"c_p0_{t}.vti"))
+
+
+def test_lammpsatom_dump_over_a_run(tmp_path):
+    # dump <name> lammpsatom: one text file, a block per dumped step, values
+    # against the oracle's state at the same steps
+    from tests.helpers import make_oracle
+    w = workload(n=32, nchains=6, nbeads=20, seed=27, ljish="linear", eps=0.2)
+    s, o = make_gpu(w), make_oracle(w)
+    attrs = ["x", "F", "c_pf", "ID", "type"]
+    d = Dump(s, "lmp", "lammpsatom", str(tmp_path / "traj"), 10, attrs, decimalplaces=17, fmt="e",
+             computes={"pf": 0})
+    d.write_collection_header()
+    run_with_dumps(s, 20, [d], w["dt"])
+    d.write_collection_footer()
+    text = open(str(tmp_path / "traj_p0.lmmpstrj")).read().split("ITEM: TIMESTEP\n")[1:]
+    assert [blk.split("\n")[0] for blk in text] == ["10", "20"]
+    n = len(w["xyz"])
+    o.run(20)
+    rows = np.array([r.split() for r in text[1].split("\n")[8:8 + n]], dtype=np.float64)
+    assert rel_l2(rows[:, 0:3], o.get_atoms()[0]) < 1e-10
+    assert rel_l2(rows[:, 3:6], o.get_forces()) < 1e-8
+    assert rel_l2(rows[:, 6:9], o.compute_pair(0)) < 1e-10
+    assert np.array_equal(rows[:, 9], np.arange(n)) and np.array_equal(rows[:, 10], w["types"])
+    assert not (tmp_path / "traj.pvd").exists()
