@@ -6,6 +6,7 @@ collection listing the timesteps.
     dump <name> atom/owned <base> <every> attributes n x ux ix F ID molID type
          [decimalplaces p] [fmt lf|e]
     dump <name> lammpsatom <base> <every> attributes n ...   (one text file)
+    dump <name> ftgrid <base> <every> attributes n c_<id> f_<id> ...   (compute complex, fix grid/ave)
 
 File names follow src/dump.cpp:315-334: <base>_p0_<step>.vti|vtp for the
 data, <base>_<step>.pvti|pvtp for the header, <base>.pvd for the collection.
@@ -29,6 +30,8 @@ class Dump:
             self.kind, self.fext = "atom", "vtp"
         elif dump_type == "grid":
             self.kind, self.fext = "grid", "vti"
+        elif dump_type == "ftgrid":
+            self.kind, self.fext = "ftgrid", "vti"
         else:
             raise ValueError("Invalid dump type (must be atom or grid or ftgrid).")
         if not attributes:
@@ -40,10 +43,14 @@ class Dump:
         # c_<id> attributes: compute pair arrays, {id: pair index}
         self.computes = dict(computes or {})
         for w in attributes:
+            if self.kind == "ftgrid" and w.startswith("f_") and w[2:] in self.computes:
+                continue   # a fix grid/ave over a compute complex
             if w.startswith("c_") and w[2:] not in self.computes:
                 raise ValueError("Cannot write dump, compute ID " + w[2:] + "doesn't exist.")
             if w.startswith("c_") and self.kind == "grid":
                 raise ValueError("Cannot write dump, compute ID " + w[2:] + "is not a per_grid quantity.")
+            if self.kind == "ftgrid" and not (w.startswith("c_") or w.startswith("f_")):
+                raise ValueError("Dump error: Attribute does not exist.")
         self._computed = None
         self.system, self.name, self.every = system, name, int(every)
         self.attributes = list(attributes)
@@ -70,9 +77,20 @@ class Dump:
     def due(self, timestep):
         return timestep % self.every == 0
 
+    def needs_capture(self, timestep):
+        """An ftgrid dump of a compute complex needs the in_fourier spectra of
+        the step it is written at."""
+        from .computes import ComputeComplex
+        return self.kind == "ftgrid" and self.due(timestep) and any(
+            isinstance(c, ComputeComplex) for c in self.computes.values())
+
     def write_collection_middle(self, timestep, dt):
         """The data-set line, then this timestep's file and its header."""
+        self.dt = dt
         s = self.system
+        if self.kind == "ftgrid":
+            self._write_ftgrid(timestep)
+            return
         if self.kind == "lammps":   # no collection line, one file for every step
             vals = self._atom_values()
             top = [lo + p for lo, p in zip(s.boxlo, s.period)]
@@ -96,12 +114,38 @@ class Dump:
             A.vtk_write_pgrid(pinstance, s.shape, self.attributes, s.boxlo, spacing, source)
 
 
+    def _write_ftgrid(self, timestep):
+        """Dump::write_grid_timestep for ftgrid (src/dump.cpp:703-723): the
+        per-mode compute arrays in the ft order [ky][kz][kx], origin 0 and the
+        spacings dqx, dqy = 2pi/Lz, dqz = 2pi/Ly of the reference's Domain
+        (src/domain.cpp:206-208, pi as 3.141592).  One rank: each array holds
+        exactly its Ny*Nz*(Nx/2+1) values (the reference's one-rank writer
+        announces one plane more than the compute holds)."""
+        from .computes import ComputeComplex
+        s = self.system
+        instance = f"{self.base_name}_p0_{timestep}.{self.fext}"
+        pinstance = f"{self.base_name}_{timestep}.p{self.fext}"
+        source = f"{self.nopath_base_name}_p0_{timestep}.{self.fext}"
+        A.vtk_collection(self.collection_name, A.PVD_DATASET, timestep * self.dt,
+                         f"{self.nopath_base_name}_{timestep}.p{self.fext}")
+        arrays = []
+        for w in self.attributes:
+            c = self.computes[w[2:]]
+            arrays.append(c.array() if isinstance(c, ComputeComplex) else c.array)
+        lx, ly, lz = s.period
+        dq = [2 * 3.141592 / lx, 2 * 3.141592 / lz, 2 * 3.141592 / ly]
+        A.vtk_write_grid(instance, arrays, self.attributes, [0.0, 0.0, 0.0], dq)
+        A.vtk_write_pgrid(pinstance, arrays[0].shape, self.attributes, [0.0, 0.0, 0.0], dq, source)
+
     def end_of_step(self):
         """The computes this dump holds (ComputePair::end_of_step), evaluated
         before any dump of the step is written, as the reference's step runs
         its computes' end_of_step before its dumps (src/integrate.cpp:172-190):
         a grid-atom compute's second chempot deposit is then in a chempot
         dump of the same step."""
+        if self.kind == "ftgrid":   # its computes are read when the file is written
+            self._computed = None
+            return
         self._computed = {w: self.system.compute_pair(self.computes[w[2:]])
                           for w in self.attributes if w.startswith("c_")} or None
 
@@ -137,8 +181,9 @@ def run_with_dumps(system, nsteps, dumps, dt, first_step=0, aves=()):
     while step < end:
         nxt = min([end] + [((step // d.every) + 1) * d.every for d in dumps] + [a.next_sample(step) for a in aves])
         sampling = [a for a in aves if a.samples_at(nxt)]
-        if aves:
-            system.fourier_capture(bool(sampling))
+        spectra = bool(sampling) or any(d.needs_capture(nxt) for d in dumps)
+        if aves or any(d.kind == "ftgrid" for d in dumps):
+            system.fourier_capture(spectra)
         system.run(nxt - step)
         step = nxt
         due = [d for d in dumps if d.due(step)]
@@ -148,6 +193,6 @@ def run_with_dumps(system, nsteps, dumps, dt, first_step=0, aves=()):
             a.end_of_step(step)
         for d in due:
             d.write_collection_middle(step, dt)
-    if aves:
+    if aves or any(d.kind == "ftgrid" for d in dumps):
         system.fourier_capture(False)
     return step

@@ -21,6 +21,7 @@ import pytest
 
 import phafd_amd as P
 from phafd_amd.dump import Dump, run_with_dumps
+from oracle import oracle_ctypes as O
 from tests.helpers import workload, make_gpu, apply, rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -233,3 +234,35 @@ def test_lammpsatom_dump_over_a_run(tmp_path):
     assert rel_l2(rows[:, 6:9], o.compute_pair(0)) < 1e-10
     assert np.array_equal(rows[:, 9], np.arange(n)) and np.array_equal(rows[:, 10], w["types"])
     assert not (tmp_path / "traj.pvd").exists()
+
+
+def test_ftgrid_dump_of_computes(tmp_path):
+    # dump <name> ftgrid: a compute complex (its step's in_fourier spectrum)
+    # and a fix grid/ave over it, in the ft order with the reference's dq
+    # spacings; the compute against the oracle at the dumped step
+    import re
+    import struct
+    from tests.helpers import make_oracle
+    from phafd_amd.computes import ComputeComplex, FixGridAve
+    w = workload(n=32, nchains=6, nbeads=20, seed=28, box=40.0)
+    s, o = make_gpu(w), make_oracle(w)
+    sq = ComputeComplex(s, "sq", "ft_phi", "norm")
+    ave = FixGridAve(s, "sqave", 2, 3, 10, sq)
+    d = Dump(s, "ft", "ftgrid", str(tmp_path / "sq"), 10, ["c_sq", "f_sqave"], computes={"sq": sq, "sqave": ave})
+    d.write_collection_header()
+    run_with_dumps(s, 20, [d], w["dt"], aves=[ave])
+    d.write_collection_footer()
+    raw = open(str(tmp_path / "sq_p0_20.vti"), "rb").read()
+    head, _, tail = raw.partition(b'<AppendedData encoding="raw">\n_')
+    dq = [2 * 3.141592 / 40.0] * 3
+    assert f'Spacing="{dq[0]:.6g} {dq[1]:.6g} {dq[2]:.6g} "'.encode() in head
+    (nb,) = struct.unpack_from("<I", tail, 0)
+    got = np.frombuffer(tail, "<f8", nb // 8, 4).reshape(32, 32, 17)
+    o.run(19)
+    for ph in (O.PH_START, O.PH_NEIGH, O.PH_PAIR, O.PH_INITIAL, O.PH_POSTFORCE, O.PH_PREFINAL):
+        o.phase(ph)
+    assert rel_l2(got, o.compute_complex(0, 3)) < 1e-10
+    (nb2,) = struct.unpack_from("<I", tail, 4 + nb)
+    got_ave = np.frombuffer(tail, "<f8", nb2 // 8, 8 + nb).reshape(32, 32, 17)
+    assert np.array_equal(got_ave, ave.array)
+    assert re.search(rb'DataArray Name="f_sqave"', head)
