@@ -29,6 +29,23 @@ class ComputeComplex:
         return self.system.compute_complex(self.which, self.quant)
 
 
+class ComputeGridClusters:
+    """compute <id> grid/clusters phi <threshold> lt|gt
+    (src/compute_grid_clusters.cpp:22-60): a per-grid compute, evaluated on
+    the current phi when read."""
+
+    def __init__(self, system, name, arrname, threshold, condition):
+        if arrname != "phi":
+            raise ValueError("Must use phi as array for clustering.")
+        if condition not in ("lt", "gt"):
+            raise ValueError("Must specify lt or gt for compute grid cluster.")
+        self.system, self.name = system, name
+        self.threshold, self.condition = float(threshold), condition
+
+    def array(self):
+        return self.system.compute_grid_clusters(self.threshold, self.condition)
+
+
 def save_steps(every, repeat, freq, nwindows):
     """The step counts (steps since the fix's setup) at which grid/ave samples,
     over nwindows windows (src/fixgrid_ave.cpp:113-140, 170-185)."""

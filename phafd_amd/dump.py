@@ -47,7 +47,7 @@ class Dump:
                 continue   # a fix grid/ave over a compute complex
             if w.startswith("c_") and w[2:] not in self.computes:
                 raise ValueError("Cannot write dump, compute ID " + w[2:] + "doesn't exist.")
-            if w.startswith("c_") and self.kind == "grid":
+            if w.startswith("c_") and self.kind == "grid" and not hasattr(self.computes[w[2:]], "threshold"):
                 raise ValueError("Cannot write dump, compute ID " + w[2:] + "is not a per_grid quantity.")
             if self.kind == "ftgrid" and not (w.startswith("c_") or w.startswith("f_")):
                 raise ValueError("Dump error: Attribute does not exist.")
@@ -109,8 +109,19 @@ class Dump:
                 s.dump_atoms(instance, self.attributes, self.precision, self.fmt, self.async_)
             A.vtk_write_patoms(pinstance, self.attributes, source)
         else:
-            s.dump_grid(instance, self.attributes, self.async_)
             spacing = [p / n for p, n in zip(s.period, s.shape[::-1])]
+            if any(w.startswith("c_") for w in self.attributes):
+                # with per-grid computes the file is written from host arrays
+                if self._computed is None:
+                    self.end_of_step()
+                fields = {"phi": A.PHI, "chempot": A.MU, "gradphi_x": A.GRADX, "gradphi_y": A.GRADY,
+                          "gradphi_z": A.GRADZ}
+                arrays = [self._computed[w] if w.startswith("c_") else s.get_field(fields[w])
+                          for w in self.attributes]
+                self._computed = None
+                A.vtk_write_grid(instance, arrays, self.attributes, s.boxlo, spacing)
+            else:
+                s.dump_grid(instance, self.attributes, self.async_)
             A.vtk_write_pgrid(pinstance, s.shape, self.attributes, s.boxlo, spacing, source)
 
 
@@ -145,6 +156,9 @@ class Dump:
         dump of the same step."""
         if self.kind == "ftgrid":   # its computes are read when the file is written
             self._computed = None
+            return
+        if self.kind == "grid":     # per-grid computes (grid clusters) of this step
+            self._computed = {w: self.computes[w[2:]].array() for w in self.attributes if w.startswith("c_")} or None
             return
         self._computed = {w: self.system.compute_pair(self.computes[w[2:]])
                           for w in self.attributes if w.startswith("c_")} or None

@@ -266,3 +266,25 @@ def test_ftgrid_dump_of_computes(tmp_path):
     got_ave = np.frombuffer(tail, "<f8", nb2 // 8, 8 + nb).reshape(32, 32, 17)
     assert np.array_equal(got_ave, ave.array)
     assert re.search(rb'DataArray Name="f_sqave"', head)
+
+
+def test_grid_dump_with_cluster_labels(tmp_path):
+    # a grid dump holding phi and a grid/clusters compute (the labels of the
+    # dumped step's phi), read back with the format's own layout
+    import struct
+    from phafd_amd.computes import ComputeGridClusters
+    w = workload(n=32, nchains=6, nbeads=20, seed=29, chi=3.0, var=0.05)
+    s = make_gpu(w)
+    cl = ComputeGridClusters(s, "clusters", "phi", 0.5, "lt")
+    d = Dump(s, "g", "grid", str(tmp_path / "cl"), 10, ["phi", "c_clusters"], computes={"clusters": cl})
+    d.write_collection_header()
+    run_with_dumps(s, 20, [d], w["dt"])
+    d.write_collection_footer()
+    raw = open(str(tmp_path / "cl_p0_20.vti"), "rb").read()
+    tail = raw.partition(b'<AppendedData encoding="raw">\n_')[2]
+    (nb,) = struct.unpack_from("<I", tail, 0)
+    phi = np.frombuffer(tail, "<f8", nb // 8, 4).reshape(32, 32, 32)
+    lab = np.frombuffer(tail, "<f8", nb // 8, 8 + nb).reshape(32, 32, 32)
+    assert np.array_equal(phi, s.get_field(P.PHI))
+    assert np.array_equal(lab, s.compute_grid_clusters(0.5, "lt"))
+    assert np.array_equal(lab >= 0, phi < 0.5)
