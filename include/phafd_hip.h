@@ -49,8 +49,10 @@ enum { PHAFD_PHI = 0, PHAFD_MU = 1, PHAFD_GRADX = 2, PHAFD_GRADY = 3, PHAFD_GRAD
 enum { PHAFD_PHIHAT = 0, PHAFD_MUHAT = 1 };
 /* ghost images along z on one GPU (single-rank CommBrick keeps none) */
 enum { PHAFD_ZGHOST_REFERENCE = 0, PHAFD_ZGHOST_PERIODIC = 1 };
-/* thermal noise of grid/conjugate/volfrac */
-enum { PHAFD_NOISE_OFF = 0, PHAFD_NOISE_FIELD = 2 };
+/* thermal noise of grid/conjugate/volfrac: off, generated on the device
+ * (PHAFD_NOISE_DEVICE, see phafd_hip_noise_field), or one host field per
+ * update (PHAFD_NOISE_FIELD, see phafd_hip_set_noise) */
+enum { PHAFD_NOISE_OFF = 0, PHAFD_NOISE_DEVICE = 1, PHAFD_NOISE_FIELD = 2 };
 /* grid-atom pair styles */
 /* grid-atom pair styles of phafd_hip_pair_gridatom (lj/cut has its own call) */
 enum { PHAFD_PAIR_GAUSSIAN = 0, PHAFD_PAIR_LJISH = 1, PHAFD_PAIR_LJISH_LINEAR = 3 };
@@ -79,6 +81,13 @@ const char *phafd_hip_last_error(const phafd_hip_ctx *ctx);
 /* ---- fields: Grid public arrays phi, chempot, gradphi (src/grid.hpp:28-36) */
 int phafd_hip_set_field(phafd_hip_ctx *ctx, int which, const double *host);
 int phafd_hip_get_field(phafd_hip_ctx *ctx, int which, double *host);
+/* the same with rows ld doubles apart (ld >= Nx): row (z, y) of Nx values at
+ * host + (z*Ny + y)*ld.  The reference's real fftw arrays pad each x row to
+ * 2*(Nx/2+1) doubles ("real fftw arrays aren't contiguous", src/dump.cpp:
+ * 1181-1187, src/read_dump.cpp:344-350), so Grid::phi->data() binds with
+ * ld = 2*(Nx/2+1); padding entries are neither read nor written. */
+int phafd_hip_set_field_ld(phafd_hip_ctx *ctx, int which, const double *host, int64_t ld);
+int phafd_hip_get_field_ld(phafd_hip_ctx *ctx, int which, double *host, int64_t ld);
 /* ft_phi / ft_chempot (src/grid.hpp:34-35), returned in the canonical layout */
 int phafd_hip_get_spectrum(phafd_hip_ctx *ctx, int which, double *host);
 
@@ -188,14 +197,22 @@ int phafd_hip_pair_harmoniccut(phafd_hip_ctx *ctx, double cut, double springK);
 /* ---- fixes, in script order (src/input.cpp:175-230) */
 /* fix grid/conjugate/volfrac <name> <seed> mobility M temp T volFH v gamma g
  * (src/fixgrid_conjugate.cpp:21-45, src/conjugate_volfrac.cpp:29-69).
- * noise_mode: PHAFD_NOISE_OFF (new switch: the reference always adds noise)
- * or PHAFD_NOISE_FIELD (per-mode host field, see phafd_hip_set_noise). */
+ * noise_mode: PHAFD_NOISE_OFF (new switch: the reference always adds noise),
+ * PHAFD_NOISE_DEVICE (fresh noise at every update, drawn on the device:
+ * phafd_hip_noise_field(..., make_unique_seed(seed), step, ...) is exactly
+ * what the update of timestep `step` adds) or PHAFD_NOISE_FIELD (one host
+ * field per update, see phafd_hip_set_noise). */
 int phafd_hip_fix_conjugate(phafd_hip_ctx *ctx, int seed, double mobility, double temp,
                             double volFH, double gamma, int noise_mode);
-/* host noise field xi in the canonical spectrum layout, used by the next
- * spectral updates (the value ConjugateVolFrac adds per mode,
- * src/conjugate_volfrac.cpp:95-96,118); NULL clears it */
+/* PHAFD_NOISE_FIELD: queue a host noise field xi (canonical spectrum layout:
+ * the value ConjugateVolFrac adds per mode, src/conjugate_volfrac.cpp:
+ * 95-96,118, e.g. from phafd_noise_next in the reference's mt19937 order).
+ * Each spectral update consumes one queued field, oldest first, so a
+ * phafd_hip_run(n) needs n fields queued (else PHAFD_E_ARG, nothing run).
+ * NULL clears the queue. */
 int phafd_hip_set_noise(phafd_hip_ctx *ctx, const double *xi);
+/* number of queued, unconsumed noise fields */
+int phafd_hip_noise_queued(phafd_hip_ctx *ctx, int64_t *n);
 /* fix grid/floryhuggins <name> temp T volFH v chi c (src/fixgrid_floryhuggins.cpp:22-53) */
 int phafd_hip_fix_floryhuggins(phafd_hip_ctx *ctx, double temp, double volFH, double chi);
 /* fix grid/gradphi <name> once|every (src/fixgrid_gradphi.cpp:21-33) */
@@ -273,9 +290,14 @@ int phafd_hip_timers(phafd_hip_ctx *ctx, double *ms8, int reset);
  *    of a block in speculative windows (one host read-back per window; the
  *    steps after a rebuild decision are skipped on the device and redone
  *    after the rebuild): the same steps, lists and sums as 0, which reads
- *    the decision back after every step. */
+ *    the decision back after every step.
+ *  PHAFD_OPT_SEMI_IMPLICIT [0]: the k-space update takes the linear
+ *    gradient term implicitly, phihat <- (phihat - c muhat) / (1 + c s) / N
+ *    + xi with c = M q^2 dt, s = T/v gamma q^2 (the semi-implicit form kept
+ *    in a comment at src/conjugate.cpp:356); 0 is the reference's explicit
+ *    update (src/conjugate_volfrac.cpp:89-133), the drop-in. */
 enum { PHAFD_OPT_EAGER_GRIDLIST = 0, PHAFD_OPT_FUSE_FH = 1, PHAFD_OPT_FUSE_SPECTRAL = 2,
-       PHAFD_OPT_EAGER_HALFLIST = 3, PHAFD_OPT_ATOM_WINDOWS = 4 };
+       PHAFD_OPT_EAGER_HALFLIST = 3, PHAFD_OPT_ATOM_WINDOWS = 4, PHAFD_OPT_SEMI_IMPLICIT = 5 };
 int phafd_hip_set_option(phafd_hip_ctx *ctx, int key, double value);
 /* HIP-event timing of every launch of one kernel during run():
  * kernel_id >= 0 (re)starts the probe for that kernel (ids in the order
@@ -396,6 +418,19 @@ int phafd_hip_gradphi(int nx, int ny, int nz, const double *period, const double
 /* FixGridFloryHuggins::post_force (src/fixgrid_floryhuggins.cpp:65-85) */
 int phafd_hip_floryhuggins(int64_t n, double *phi, double *mu, double temp, double volFH,
                            double chi, int64_t *nless, int64_t *ngreat);
+/* The thermal noise the update of timestep `step` adds in
+ * PHAFD_NOISE_DEVICE mode, xi[Nz][Ny][Nx/2+1][2] (canonical layout), for
+ * ConjugateVolFrac's amplitudes (src/conjugate_volfrac.cpp:72-133) and its
+ * Hermitian rules (src/conjugate.cpp:163-327): a mode updated with
+ * complex_update gets sqrt(12 T M / L^3) |q| sqrt(dt) (U1 + i U2), a
+ * self-conjugate mode sqrt(24 T M / L^3) |q| sqrt(dt) U1, the mirror of an
+ * updated mode on the kx = 0, Nx/2 planes the conjugate of its noise, the
+ * origin 0.  U1, U2 = the top 53 bits of words 0 and 1 of
+ * Philox4x64-10(counter {m, step, 0, 0}, key {key0, 0x5048414644}) as
+ * doubles in [-1/2, 1/2), m = (kx*Nz + kz)*Ny + ky of the updated mode;
+ * key0 = make_unique_seed(seed) of the fix. */
+int phafd_hip_noise_field(int nx, int ny, int nz, const double *period, double mobility, double temp,
+                          double dt, uint64_t key0, int64_t step, double *xi);
 /* Conjugate::update + ConjugateVolFrac (src/conjugate.cpp:163-327,
  * src/conjugate_volfrac.cpp:72-144) on a canonical-layout spectrum */
 int phafd_hip_spectral_update(int nx, int ny, int nz, const double *period, double *phihat,

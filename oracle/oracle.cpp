@@ -235,6 +235,7 @@ struct SpectralParams {
   double period[3];
   double mobility, temp, volFH, gamma, dt;
   int noise_mode;
+  int semi;              // the opt-in semi-implicit form (src/conjugate.cpp:356, a comment there)
   const double *noise;   // canonical layout (re,im) when noise_mode == FIELD
   std::mt19937 *gen;     // when noise_mode == MT19937
   std::vector<double> *record;   // optional: record draws per mode (noise field generation)
@@ -278,6 +279,18 @@ struct SpectralUpdater {
     double u = real_dist(*p.gen);
     return u;
   }
+  // the bracket of src/conjugate_volfrac.cpp:106-109 (explicit Euler), or
+  // with p.semi the linear gradient term implicit: (phi - c mu) / (1 + c s),
+  // c = M q^2 dt, s = T/v gamma q^2, the denominator of the formula left in
+  // a comment at src/conjugate.cpp:356
+  cplx step_mode(cplx ph, cplx m, double q2) const {
+    if (p.semi) {
+      const double c = p.mobility * q2 * p.dt;
+      const double sg = p.temp / p.volFH * p.gamma * q2;
+      return (ph - c * m) / (1.0 + c * sg);
+    }
+    return ph - p.mobility * q2 * p.dt * (m + p.temp / p.volFH * p.gamma * q2 * ph);
+  }
   // src/conjugate_volfrac.cpp:89-113
   void complex_update(int i, int j, int k) {
     double qx, q2 = q2_of(i, j, k, qx);
@@ -291,9 +304,7 @@ struct SpectralUpdater {
     } else if (p.noise_mode == ORC_NOISE_FIELD) {
       noise = cplx(p.noise[2 * id], p.noise[2 * id + 1]);
     }
-    phi[id] = (phi[id] - p.mobility * q2 * p.dt *
-                             (mu[id] + p.temp / p.volFH * p.gamma * q2 * phi[id])) *
-                  normalization + noise;
+    phi[id] = step_mode(phi[id], mu[id], q2) * normalization + noise;
   }
   // src/conjugate_volfrac.cpp:116-133
   void real_update(int i, int j, int k) {
@@ -306,9 +317,7 @@ struct SpectralUpdater {
     } else if (p.noise_mode == ORC_NOISE_FIELD) {
       noise = cplx(p.noise[2 * id], p.noise[2 * id + 1]);
     }
-    phi[id] = (phi[id] - p.mobility * q2 * p.dt *
-                             (mu[id] + p.temp / p.volFH * p.gamma * q2 * phi[id])) *
-                  normalization + noise;
+    phi[id] = step_mode(phi[id], mu[id], q2) * normalization + noise;
   }
   // src/conjugate_volfrac.cpp:135-141
   void origin_update() { phi[idx(0, 0, 0)] *= normalization; }
@@ -428,7 +437,8 @@ struct orc_sys {
   size_t N, M;
   std::vector<double> phi, mu, grad[3];
   std::vector<cplx> ftphi, ftmu;
-  std::vector<double> noisefield;
+  std::vector<double> noisefield;   // consumed by one update
+  int semi = 0;
   // atoms (src/atom.hpp)
   int nowned = 0, ngathered = 0, nlocal = 0, nghost = 0, ntypes = 0;
   std::vector<double> xs, uxs, Fs;
@@ -1165,11 +1175,13 @@ void phase_final(orc_sys *s) {
       for (int d = 0; d < 3; d++) p.period[d] = s->period[d];
       p.mobility = f.mobility; p.temp = f.temp; p.volFH = f.volFH; p.gamma = f.gamma; p.dt = s->dt;
       p.noise_mode = f.noise_mode;
+      p.semi = s->semi;
       p.noise = s->noisefield.empty() ? nullptr : s->noisefield.data();
       if (p.noise_mode == ORC_NOISE_FIELD && !p.noise) p.noise_mode = ORC_NOISE_OFF;
       p.gen = &f.gen;
       SpectralUpdater up(p);
       up.update(s->ftphi.data(), s->ftmu.data());
+      if (f.noise_mode == ORC_NOISE_FIELD) s->noisefield.clear();   // one field per update
     } else if (f.kind == FixDef::DRAG) {
       drag_halfstep(s, f);
     } else if (f.kind == FixDef::CHAIN) {
@@ -1414,6 +1426,10 @@ int orc_add_fix_conjugate(orc_sys *s, int seed, double mobility, double temp, do
   f.noise_mode = noise_mode;
   f.gen.seed(f.seed);
   s->fixes.push_back(f);
+  return 0;
+}
+int orc_set_semi_implicit(orc_sys *s, int on) {
+  s->semi = on ? 1 : 0;
   return 0;
 }
 int orc_set_noise(orc_sys *s, const double *noise) {

@@ -67,6 +67,7 @@ int orc_pair_coeff(orc_sys *, int ipair, int type, double eps, double phit,
 int orc_add_fix_conjugate(orc_sys *, int seed, double mobility, double temp,
                           double volFH, double gamma, int noise_mode);
 int orc_set_noise(orc_sys *, const double *noise);
+int orc_set_semi_implicit(orc_sys *, int on);
 int orc_add_fix_floryhuggins(orc_sys *, double temp, double volFH, double chi);
 int orc_add_fix_gradphi(orc_sys *, int once);
 int orc_add_fix_drag(orc_sys *, int nchunk, const int *start, const int *end,

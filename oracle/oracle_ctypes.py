@@ -43,6 +43,7 @@ def lib():
         L.orc_pair_coeff.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 3
         L.orc_add_fix_conjugate.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 4 + [ctypes.c_int]
         L.orc_set_noise.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_set_semi_implicit.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_add_fix_floryhuggins.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3
         L.orc_add_fix_gradphi.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_add_fix_drag.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip, ctypes.c_double]
@@ -277,6 +278,9 @@ class System:
         else:
             a = np.ascontiguousarray(xi, dtype=np.complex128)
             lib().orc_set_noise(self.h, a.ctypes.data_as(_dp))
+
+    def set_semi_implicit(self, on=True):
+        lib().orc_set_semi_implicit(self.h, int(on))
 
     def fix_floryhuggins(self, temp=1.0, volFH=1.0, chi=1.0):
         self._chk(lib().orc_add_fix_floryhuggins(self.h, temp, volFH, chi))

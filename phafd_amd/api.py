@@ -16,10 +16,11 @@ LIB_PATH = os.path.join(HERE, "libphafd_hip.so")
 PHI, MU, GRADX, GRADY, GRADZ = 0, 1, 2, 3, 4
 PHIHAT, MUHAT = 0, 1
 ZGHOST_REFERENCE, ZGHOST_PERIODIC = 0, 1
-NOISE_OFF, NOISE_FIELD = 0, 2
+NOISE_OFF, NOISE_DEVICE, NOISE_FIELD = 0, 1, 2
 PAIR_GAUSSIAN, PAIR_LJISH, PAIR_LJISH_LINEAR = 0, 1, 3
 FP64, FP32 = 0, 1
-OPT_EAGER_GRIDLIST, OPT_FUSE_FH, OPT_FUSE_SPECTRAL, OPT_EAGER_HALFLIST, OPT_ATOM_WINDOWS = 0, 1, 2, 3, 4
+(OPT_EAGER_GRIDLIST, OPT_FUSE_FH, OPT_FUSE_SPECTRAL, OPT_EAGER_HALFLIST, OPT_ATOM_WINDOWS,
+ OPT_SEMI_IMPLICIT) = 0, 1, 2, 3, 4, 5
 PH_START, PH_NEIGH, PH_PAIR, PH_INITIAL, PH_POSTFORCE, PH_PREFINAL, PH_FINAL, PH_POSTFINAL = range(8)
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "NaN in phi", -4: "runtime error", -5: "list overflow"}
 
@@ -64,6 +65,11 @@ def lib():
         "phafd_hip_set_field": ([V, ctypes.c_int, _dp], ctypes.c_int),
         "phafd_hip_get_field": ([V, ctypes.c_int, _dp], ctypes.c_int),
         "phafd_hip_get_spectrum": ([V, ctypes.c_int, _dp], ctypes.c_int),
+        "phafd_hip_set_field_ld": ([V, ctypes.c_int, _dp, ctypes.c_int64], ctypes.c_int),
+        "phafd_hip_get_field_ld": ([V, ctypes.c_int, _dp, ctypes.c_int64], ctypes.c_int),
+        "phafd_hip_noise_queued": ([V, _lp], ctypes.c_int),
+        "phafd_hip_noise_field": ([ctypes.c_int] * 3 + [_dp] + [ctypes.c_double] * 3 + [ctypes.c_uint64,
+                                  ctypes.c_int64, _dp], ctypes.c_int),
         "phafd_hip_set_atoms": ([V, ctypes.c_int, _dp, _ip, _ip, _ip], ctypes.c_int),
         "phafd_hip_get_atoms": ([V, _dp, _dp, _ip], ctypes.c_int),
         "phafd_hip_set_images": ([V, _ip], ctypes.c_int),
@@ -374,6 +380,17 @@ def floryhuggins(phi, mu, temp, volFH, chi):
     return phi, mu, a.value, b.value
 
 
+def noise_field(shape, period, mobility, temp, dt, key0, step):
+    """The thermal noise the update of timestep `step` adds in NOISE_DEVICE
+    mode (key0 = make_unique_seed(seed) of the fix): (Nz, Ny, Nx/2+1) complex."""
+    nz, ny, nx = shape
+    out = np.zeros((nz, ny, nx // 2 + 1), dtype=np.complex128)
+    pe = np.ascontiguousarray(period, dtype=np.float64)
+    _check(lib().phafd_hip_noise_field(nx, ny, nz, _d(pe), mobility, temp, dt, int(key0), int(step),
+                                       out.ctypes.data_as(_dp)))
+    return out
+
+
 def spectral_update(phihat, muhat, nx, period, mobility, temp, volFH, gamma, dt, noise=None):
     nz, ny, hx = phihat.shape
     ph = np.array(phihat, dtype=np.complex128, order="C")
@@ -428,6 +445,26 @@ class System:
         a = np.zeros(self.shape)
         self._c(lib().phafd_hip_get_field(self.h, which, _d(a)))
         return a
+
+    def set_field_ld(self, which, buf, ld):
+        """set_field from a host buffer whose x rows are ld doubles apart."""
+        a = np.ascontiguousarray(buf, dtype=np.float64)
+        nz, ny, nx = self.shape
+        if a.size < ((nz * ny - 1) * ld + nx):
+            raise ValueError("buffer too small for the padded rows")
+        self._c(lib().phafd_hip_set_field_ld(self.h, which, _d(a), ld))
+
+    def get_field_ld(self, which, buf, ld):
+        """get_field into a host buffer (modified in place) whose x rows are ld doubles apart."""
+        nz, ny, nx = self.shape
+        if not (buf.flags["C_CONTIGUOUS"] and buf.dtype == np.float64) or buf.size < ((nz * ny - 1) * ld + nx):
+            raise ValueError("need a contiguous float64 buffer holding the padded rows")
+        self._c(lib().phafd_hip_get_field_ld(self.h, which, _d(buf), ld))
+
+    def noise_queued(self):
+        n = ctypes.c_int64()
+        self._c(lib().phafd_hip_noise_queued(self.h, ctypes.byref(n)))
+        return n.value
 
     def get_spectrum(self, which):
         nz, ny, nx = self.shape

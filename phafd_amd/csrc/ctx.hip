@@ -44,6 +44,7 @@ struct FixDef {
   enum Kind { CONJ, FH, GRADPHI, DRAG, CHAIN } kind;
   double mobility = 1, temp = 1, volFH = 1, gamma = 1, chi = 1;
   int seed = 0, noise_mode = 0, once = 0;
+  unsigned long long noise_key = 0;   // device noise: make_unique_seed(seed)
   std::vector<int> start, end;
   double drag = 0, r0 = 1, kbond = 0, kbend = 0;
   int nmem = 0, nchain = 0, maxlen = 0;
@@ -97,7 +98,10 @@ template <class R>
 struct FieldSet {
   using C = typename Vec2<R>::T;
   DBuf<R> phi, mu, grad[3];
-  DBuf<C> phihat, muhat, ghat[3], noise_k, canon_tmp;
+  DBuf<C> phihat, muhat, ghat[3], canon_tmp;
+  // host noise fields queued by phafd_hip_set_noise, K layout; each spectral
+  // update consumes the one at noise_head
+  std::vector<DBuf<C>> noise_q;
   void alloc(size_t N, size_t M) {
     phi.alloc(N); mu.alloc(N);
     for (int d = 0; d < 3; d++) { grad[d].alloc(N); ghat[d].alloc(M); }
@@ -119,7 +123,8 @@ struct phafd_hip_ctx {
   FieldSet<double> fd;
   FieldSet<float> fs;
   DBuf<double> mass64;       // FP32 form: sum of phi in FP64, the k = 0 mode
-  bool noise_set = false;
+  size_t noise_head = 0;      // next unconsumed entry of the noise queue
+  bool semi_implicit = false; // PHAFD_OPT_SEMI_IMPLICIT
   DBuf<unsigned long long> counters;   // [nless, ngreat]
   DBuf<int> flags;                     // [0] rebuild, [1] overflow
   DBuf<double> esum;                   // energy accumulators
@@ -758,13 +763,50 @@ void compute_gradphi(phafd_hip_ctx *c, typename Vec2<R>::T *phihat_dst, bool mas
 const double *origin_ptr(const phafd_hip_ctx *c) { return c->fp32 ? c->mass64.p : nullptr; }
 
 UpdateParams update_params(const phafd_hip_ctx *c, const FixDef &f) {
-  UpdateParams u;
+  UpdateParams u{};
   u.mobility = f.mobility; u.temp = f.temp; u.volFH = f.volFH; u.gamma = f.gamma; u.dt = c->dt;
   u.normalization = 1.0 / ((double)c->nx * c->nz * c->ny);   // src/conjugate_volfrac.cpp:78
   u.qx_scale = 2 * PI_REF / c->period[0];
   u.qky_scale = 2 * PI_REF / c->period[1];
   u.qkz_scale = 2 * PI_REF / c->period[2];
+  u.semi = c->semi_implicit ? 1 : 0;
+  if (f.noise_mode == PHAFD_NOISE_DEVICE) {
+    // ConjugateVolFrac::reset_dt prefactors, src/conjugate_volfrac.cpp:72-86
+    const double invL3 = 1.0 / (c->period[0] * c->period[1] * c->period[2]);
+    u.noise_dev = 1;
+    u.key0 = f.noise_key;
+    u.key1 = 0x5048414644ull;   // fixed second key word of this generator
+    u.step = (unsigned long long)c->timestep;
+    u.pref_c = std::sqrt(12 * f.temp * f.mobility * invL3);
+    u.pref_r = std::sqrt(24 * f.temp * f.mobility * invL3);
+    u.sqrtdt = std::sqrt(c->dt);
+  }
   return u;
+}
+
+// The host noise field of this update in PHAFD_NOISE_FIELD mode: the oldest
+// unconsumed phafd_hip_set_noise field, consumed here (the reference draws
+// fresh noise at every update, src/conjugate_volfrac.cpp:95-96,118).
+template <class R>
+const typename Vec2<R>::T *take_noise(phafd_hip_ctx *c, const FixDef &f) {
+  if (f.noise_mode != PHAFD_NOISE_FIELD) return nullptr;
+  FieldSet<R> &F = c->fields<R>();
+  if (c->noise_head >= F.noise_q.size())
+    throw std::invalid_argument("grid/conjugate/volfrac in noise field mode: every update needs its own "
+                                "phafd_hip_set_noise field and none is queued");
+  return F.noise_q[c->noise_head++].p;
+}
+size_t noise_queued(const phafd_hip_ctx *c) {
+  const size_t n = c->fp32 ? c->fs.noise_q.size() : c->fd.noise_q.size();
+  return n - c->noise_head;
+}
+// a run of nsteps with a noise-field conjugate fix needs nsteps fields queued
+void require_noise_fields(const phafd_hip_ctx *c, int64_t nsteps) {
+  for (auto &f : c->fixes)
+    if (f->kind == FixDef::CONJ && f->noise_mode == PHAFD_NOISE_FIELD && (int64_t)noise_queued(c) < nsteps)
+      throw std::invalid_argument("grid/conjugate/volfrac in noise field mode: a run of " + std::to_string(nsteps) +
+                                  " steps needs as many phafd_hip_set_noise fields, " +
+                                  std::to_string(noise_queued(c)) + " queued");
 }
 
 // ---------------------------------------------------------------- phases
@@ -958,7 +1000,7 @@ void phase_final_t(phafd_hip_ctx *c, bool atoms = true) {
   for (auto &fp : c->fixes) {
     FixDef &f = *fp;
     if (f.kind == FixDef::CONJ) {
-      const auto *nz = (f.noise_mode == PHAFD_NOISE_FIELD && c->noise_set) ? F.noise_k.p : nullptr;
+      const auto *nz = take_noise<R>(c, f);
       spectral_update(c->plan, F.phihat.p, F.muhat.p, nz, update_params(c, f), origin_ptr(c), c->stream);
     } else if (atoms) {
       integrate_half(c, f);
@@ -987,7 +1029,7 @@ void phase_final_split(phafd_hip_ctx *c, bool atoms = true) {
   for (auto &fp : c->fixes) {
     FixDef &f = *fp;
     if (f.kind == FixDef::CONJ) {
-      const auto *nz = (f.noise_mode == PHAFD_NOISE_FIELD && c->noise_set) ? F.noise_k.p : nullptr;
+      const auto *nz = take_noise<R>(c, f);
       fft_z_update(c->plan, F.muhat.p, F.phihat.p, nz, update_params(c, f), origin_ptr(c), c->stream, Slab(),
                    c->store_muhat);
     } else if (atoms) {
@@ -1419,6 +1461,27 @@ int phafd_hip_set_field(phafd_hip_ctx *c, int which, const double *h) {
 int phafd_hip_get_field(phafd_hip_ctx *c, int which, double *h) {
   return guard(c, [&] {
     field_out(c, which, h);
+    return 0;
+  });
+}
+// rows ld doubles apart on the host (the reference's padded fftw rows)
+int phafd_hip_set_field_ld(phafd_hip_ctx *c, int which, const double *h, int64_t ld) {
+  return guard(c, [&] {
+    if (!h || ld < c->nx) throw std::invalid_argument("set_field_ld: need a buffer and ld >= Nx");
+    const size_t rows = (size_t)c->ny * c->nz;
+    std::vector<double> t(c->N);
+    for (size_t r = 0; r < rows; r++) std::memcpy(&t[r * c->nx], h + r * (size_t)ld, sizeof(double) * c->nx);
+    field_in(c, which, t.data());
+    return 0;
+  });
+}
+int phafd_hip_get_field_ld(phafd_hip_ctx *c, int which, double *h, int64_t ld) {
+  return guard(c, [&] {
+    if (!h || ld < c->nx) throw std::invalid_argument("get_field_ld: need a buffer and ld >= Nx");
+    const size_t rows = (size_t)c->ny * c->nz;
+    std::vector<double> t(c->N);
+    field_out(c, which, t.data());
+    for (size_t r = 0; r < rows; r++) std::memcpy(h + r * (size_t)ld, &t[r * c->nx], sizeof(double) * c->nx);
     return 0;
   });
 }
@@ -1959,38 +2022,62 @@ int phafd_hip_pair_harmoniccut(phafd_hip_ctx *c, double cut, double springK) {
 int phafd_hip_fix_conjugate(phafd_hip_ctx *c, int seed, double mobility, double temp, double volFH,
                             double gamma, int noise_mode) {
   return guard(c, [&] {
-    if (noise_mode != PHAFD_NOISE_OFF && noise_mode != PHAFD_NOISE_FIELD)
-      throw std::invalid_argument("noise mode must be off or field");
+    if (noise_mode != PHAFD_NOISE_OFF && noise_mode != PHAFD_NOISE_FIELD && noise_mode != PHAFD_NOISE_DEVICE)
+      throw std::invalid_argument("noise mode must be off, field or device");
     auto f = std::make_unique<FixDef>();
     f->kind = FixDef::CONJ;
     f->seed = seed;
+    f->noise_key = (unsigned long long)(unsigned)phafd_make_unique_seed(seed);
     f->mobility = mobility; f->temp = temp; f->volFH = volFH; f->gamma = gamma;
     f->noise_mode = noise_mode;
     c->fixes.push_back(std::move(f));
     return 0;
   });
 }
+}  // extern "C"
+namespace {
+// queue one host field; a consumed entry's device buffer is reused
+template <class R>
+void noise_push(phafd_hip_ctx *c, const double *xi) {
+  using C = typename Vec2<R>::T;
+  FieldSet<R> &F = c->fields<R>();
+  DBuf<C> buf;
+  if (c->noise_head > 0) {
+    buf = std::move(F.noise_q.front());
+    F.noise_q.erase(F.noise_q.begin());
+    c->noise_head--;
+  }
+  buf.alloc(c->M);
+  F.canon_tmp.alloc(c->M);
+  std::vector<C> t(c->M);
+  for (size_t i = 0; i < c->M; i++) {
+    t[i].x = (R)xi[2 * i];
+    t[i].y = (R)xi[2 * i + 1];
+  }
+  HIPCHK(hipMemcpyAsync(F.canon_tmp.p, t.data(), sizeof(C) * c->M, hipMemcpyHostToDevice, c->stream));
+  spec_canon_to_k(c->plan, F.canon_tmp.p, buf.p, c->stream);
+  sync(c);
+  F.noise_q.push_back(std::move(buf));
+}
+}  // namespace
+extern "C" {
 int phafd_hip_set_noise(phafd_hip_ctx *c, const double *xi) {
   return guard(c, [&] {
-    if (!xi) { c->noise_set = false; return 0; }
-    if (!c->fp32) {
-      FieldSet<double> &F = c->fd;
-      F.noise_k.alloc(c->M);
-      F.canon_tmp.alloc(c->M);
-      HIPCHK(hipMemcpyAsync(F.canon_tmp.p, xi, sizeof(double2) * c->M, hipMemcpyHostToDevice, c->stream));
-      spec_canon_to_k(c->plan, F.canon_tmp.p, F.noise_k.p, c->stream);
-    } else {
-      FieldSet<float> &F = c->fs;
-      F.noise_k.alloc(c->M);
-      F.canon_tmp.alloc(c->M);
-      std::vector<float2> t(c->M);
-      for (size_t i = 0; i < c->M; i++) t[i] = make_float2((float)xi[2 * i], (float)xi[2 * i + 1]);
-      HIPCHK(hipMemcpyAsync(F.canon_tmp.p, t.data(), sizeof(float2) * c->M, hipMemcpyHostToDevice, c->stream));
-      spec_canon_to_k(c->plan, F.canon_tmp.p, F.noise_k.p, c->stream);
-      sync(c);
+    sync(c);   // no update still reads a queued field
+    if (!xi) {
+      c->fd.noise_q.clear();
+      c->fs.noise_q.clear();
+      c->noise_head = 0;
+      return 0;
     }
-    sync(c);
-    c->noise_set = true;
+    if (c->fp32) noise_push<float>(c, xi);
+    else noise_push<double>(c, xi);
+    return 0;
+  });
+}
+int phafd_hip_noise_queued(phafd_hip_ctx *c, int64_t *n) {
+  return guard(c, [&] {
+    if (n) *n = (int64_t)noise_queued(c);
     return 0;
   });
 }
@@ -2080,6 +2167,7 @@ int phafd_hip_setup(phafd_hip_ctx *c, double dt) {
 
 int phafd_hip_run(phafd_hip_ctx *c, int nsteps) {
   return guard(c, [&] {
+    require_noise_fields(c, nsteps);
     run_steps(c, nsteps);
     return 0;
   });
@@ -2091,6 +2179,7 @@ int phafd_hip_run_until_touching(phafd_hip_ctx *c, int nsteps, double touch_poin
   return guard(c, [&] {
     require_setup(c);
     if (c->nowned < 1) throw std::runtime_error("run_until_touching needs atoms");
+    require_noise_fields(c, nsteps);
     if (!c->h_ends) HIPCHK(hipHostMalloc((void **)&c->h_ends, 8 * sizeof(double), hipHostMallocDefault));
     Touch t;
     t.touch_point = touch_point;
@@ -2296,6 +2385,7 @@ int phafd_hip_set_option(phafd_hip_ctx *c, int key, double value) {
       case PHAFD_OPT_FUSE_SPECTRAL: c->fuse_spectral = value != 0; break;
       case PHAFD_OPT_EAGER_HALFLIST: c->eager_halflist = value != 0; break;
       case PHAFD_OPT_ATOM_WINDOWS: c->atom_windows = value != 0; break;
+      case PHAFD_OPT_SEMI_IMPLICIT: c->semi_implicit = value != 0; break;
       default: throw std::invalid_argument("unknown option");
     }
     return 0;
@@ -2341,6 +2431,35 @@ int sguard(Fn fn) {
 }  // namespace
 
 extern "C" {
+
+int phafd_hip_noise_field(int nx, int ny, int nz, const double *period, double mobility, double temp, double dt,
+                          uint64_t key0, int64_t step, double *xi) {
+  return sguard([&] {
+    if (!period || !xi || nx < 2 || ny < 2 || nz < 2 || (nx & 1)) throw std::invalid_argument("noise_field: bad args");
+    const int hx = nx / 2 + 1;
+    const size_t M = (size_t)hx * ny * nz;
+    UpdateParams u{};
+    u.dt = dt;
+    u.qx_scale = 2 * PI_REF / period[0];
+    u.qky_scale = 2 * PI_REF / period[1];
+    u.qkz_scale = 2 * PI_REF / period[2];
+    const double invL3 = 1.0 / (period[0] * period[1] * period[2]);
+    u.noise_dev = 1;
+    u.key0 = key0;
+    u.key1 = 0x5048414644ull;
+    u.step = (unsigned long long)step;
+    u.pref_c = std::sqrt(12 * temp * mobility * invL3);
+    u.pref_r = std::sqrt(24 * temp * mobility * invL3);
+    u.sqrtdt = std::sqrt(dt);
+    Scratch sc(nx, ny, nz);
+    DBuf<double2> out;
+    out.alloc(M);
+    noise_field(hx, ny, nz, u, out.p, sc.s);
+    HIPCHK(hipStreamSynchronize(sc.s));
+    HIPCHK(hipMemcpy(xi, out.p, sizeof(double2) * M, hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
 
 int phafd_hip_fft_r2c(int nx, int ny, int nz, const double *in, double *out) {
   return sguard([&] {
@@ -2435,7 +2554,7 @@ int phafd_hip_spectral_update(int nx, int ny, int nz, const double *period, doub
       HIPCHK(hipMemcpy(cn.p, noise, sizeof(double2) * M, hipMemcpyHostToDevice));
       spec_canon_to_k(sc.plan, cn.p, nk.p, sc.s);
     }
-    UpdateParams u;
+    UpdateParams u{};
     u.mobility = mobility; u.temp = temp; u.volFH = volFH; u.gamma = gamma; u.dt = dt;
     u.normalization = 1.0 / ((double)nx * nz * ny);
     u.qx_scale = 2 * PI_REF / period[0];

@@ -53,9 +53,8 @@ __device__ __forceinline__ void update_index(C *__restrict__ ph, const C *__rest
   const size_t t = idx / ny;
   const int j = (int)(t % nz);     // kz
   const int k = (int)(t / nz);     // kx
-  const C nz0 = cpx<C>(R(0), R(0));
   if (k != 0 && k != H) {
-    ph[idx] = mode_update(ph[idx], mu[idx], noise ? noise[idx] : nz0, i, j, k, A, B, u);
+    ph[idx] = mode_update(ph[idx], mu[idx], mode_noise(noise, idx, u, i, j, k, A, B, false), i, j, k, A, B, u);
     return;
   }
   const bool self = (i == 0 || i == A / 2) && (j == 0 || j == B / 2);
@@ -67,15 +66,15 @@ __device__ __forceinline__ void update_index(C *__restrict__ ph, const C *__rest
         const C v = ph[idx];
         ph[idx] = cpx<C>(v.x * (R)u.normalization, v.y * (R)u.normalization);
       }
-    } else {
-      ph[idx] = mode_update(ph[idx], mu[idx], noise ? noise[idx] : nz0, i, j, k, A, B, u);
+    } else {   // real_update
+      ph[idx] = mode_update(ph[idx], mu[idx], mode_noise(noise, idx, u, i, j, k, A, B, true), i, j, k, A, B, u);
     }
     return;
   }
   const bool primary = (j >= 1 && j < B / 2) || (j == 0 && i >= 1 && i < A / 2) ||
                        (j == B / 2 && i > A / 2);
   if (!primary) return;
-  const C r = mode_update(ph[idx], mu[idx], noise ? noise[idx] : nz0, i, j, k, A, B, u);
+  const C r = mode_update(ph[idx], mu[idx], mode_noise(noise, idx, u, i, j, k, A, B, false), i, j, k, A, B, u);
   ph[idx] = r;
   const int im = (A - i) % A, jm = (B - j) % B;
   ph[((size_t)k * nz + jm) * ny + im] = cpx<C>(r.x, -r.y);
@@ -105,6 +104,39 @@ __global__ void k_update_edges(C *__restrict__ ph, const C *__restrict__ mu, con
        w += (size_t)gridDim.x * blockDim.x) {
     const size_t idx = w < P ? w : (size_t)(hx - 1) * P + (w - P);
     update_index(ph, mu, noise, idx, hx, ny, nz, u, origin);
+  }
+}
+
+// What the update (update_index) adds as noise at K-layout index idx with
+// the device generator: the mode's own draw where it is updated, the
+// conjugate of its primary's draw on a mirror, nothing at the origin.
+__device__ double2 update_noise_at(const UpdateParams &u, size_t idx, int hx, int ny, int nz) {
+  const int A = ny, B = nz, H = hx - 1;
+  const int i = (int)(idx % ny);
+  const size_t t = idx / ny;
+  const int j = (int)(t % nz);
+  const int k = (int)(t / nz);
+  if (k != 0 && k != H) return device_noise(u, idx, i, j, k, A, B, false);
+  const bool self = (i == 0 || i == A / 2) && (j == 0 || j == B / 2);
+  if (self) {
+    if (k == 0 && i == 0 && j == 0) return make_double2(0.0, 0.0);
+    return device_noise(u, idx, i, j, k, A, B, true);
+  }
+  const bool primary = (j >= 1 && j < B / 2) || (j == 0 && i >= 1 && i < A / 2) || (j == B / 2 && i > A / 2);
+  if (primary) return device_noise(u, idx, i, j, k, A, B, false);
+  const int im = (A - i) % A, jm = (B - j) % B;
+  const double2 v = device_noise(u, ((size_t)k * nz + jm) * ny + im, im, jm, k, A, B, false);
+  return make_double2(v.x, -v.y);
+}
+// out[kz][ky][kx] (canonical layout)
+__global__ void k_noise_field(double2 *__restrict__ out, int hx, int ny, int nz, UpdateParams u) {
+  const size_t M = (size_t)hx * ny * nz;
+  for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < M; w += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(w % hx);
+    const size_t t = w / hx;
+    const int ky = (int)(t % ny);
+    const int kz = (int)(t / ny);
+    out[w] = update_noise_at(u, ((size_t)kx * nz + kz) * ny + ky, hx, ny, nz);
   }
 }
 
@@ -324,6 +356,12 @@ void sum_fp64_t(size_t n, const R *x, double *out, hipStream_t s) {
 PHAFD_FIELD_DEF(double, double2)
 PHAFD_FIELD_DEF(float, float2)
 #undef PHAFD_FIELD_DEF
+
+void noise_field(int hx, int ny, int nz, UpdateParams u, double2 *out_canon, hipStream_t s) {
+  hipLaunchKernelGGL(k_noise_field, dim3(grid_for((size_t)hx * ny * nz)), dim3(TPB), 0, s, out_canon, hx, ny, nz,
+                     u);
+  HIPCHK(hipGetLastError());
+}
 
 void fill_zero(void *p, size_t bytes, hipStream_t s) {
   if (!bytes) return;

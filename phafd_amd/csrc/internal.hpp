@@ -169,6 +169,15 @@ struct SpecConst {
 struct UpdateParams {
   double mobility, temp, volFH, gamma, dt;
   double normalization, qx_scale, qky_scale, qkz_scale;
+  // semi != 0: the linear gradient term taken implicitly (opt-in scheme)
+  int semi;
+  // noise_dev != 0: thermal noise generated per mode on the device from a
+  // counter-based generator (Philox4x64-10 keyed by (key0, key1), counter
+  // (K-layout mode index, step, 0, 0)); pref_c, pref_r: the complex and real
+  // prefactors sqrt(12 T M / L^3), sqrt(24 T M / L^3); sqrtdt = sqrt(dt)
+  int noise_dev;
+  unsigned long long key0, key1, step;
+  double pref_c, pref_r, sqrtdt;
 };
 #define PHAFD_FIELD_DECL(RT, CT)                                                                     \
   void grad_multiply(const FFTPlan &p, const CT *phihat, CT *gx, CT *gy, CT *gz, SpecConst c,        \
@@ -192,6 +201,9 @@ PHAFD_FIELD_DECL(double, double2)
 PHAFD_FIELD_DECL(float, float2)
 #undef PHAFD_FIELD_DECL
 void fill_zero(void *p, size_t bytes, hipStream_t s);
+// the device noise of one update (u.noise_dev set) in the canonical layout
+// [Nz][Ny][Nx/2+1], as the update of that step adds it
+void noise_field(int hx, int ny, int nz, UpdateParams u, double2 *out_canon, hipStream_t s);
 // surrogate blocks (the plain update linearised about the mean: mu = f2 phi
 // + n, f2 = temp/volFH (1/m + 1/(1-m) - 2 chi) + 2 mean(rho); sums = FP64
 // {sum phi0, sum rho} on the device).  surrogate_mu: mu of the plain step at

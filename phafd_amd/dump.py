@@ -190,6 +190,14 @@ def run_with_dumps(system, nsteps, dumps, dt, first_step=0, aves=()):
     stops are single calls, so the steps are not synchronised for them."""
     step = first_step
     end = first_step + nsteps
+    # FixGridAve::setup refuses a dump of the fix whose every differs from
+    # the fix's freq (src/fixgrid_ave.cpp:101-107)
+    for d in dumps:
+        for w in d.attributes:
+            fix = d.computes.get(w[2:]) if w.startswith("f_") else None
+            if hasattr(fix, "freq") and d.every != fix.freq:
+                raise ValueError("Using fixgrid/ave with frequency that doesn't match dump frequency - "
+                                 "waste of time.")
     for a in aves:   # FixGridAve::setup at the start of every run (src/fixgrid_ave.cpp:99-125)
         a.setup(first_step)
     while step < end:

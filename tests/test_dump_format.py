@@ -265,3 +265,15 @@ def test_lammps_dump_matches_oracle_and_reads_back(tmp_path, precision, fmt):
     assert row[:3] == [f(v) for v in a["x"][0]] and row[6:9] == [f(v) for v in a["c_pf"][0]]
     assert row[9:] == [str(a[w][0]) for w in ("ID", "molID", "type", "ix")]
     assert lines[9 + 200] == "ITEM: TIMESTEP" and lines[10 + 200] == "100"
+
+
+def test_grid_ave_dump_frequency_must_match():
+    # FixGridAve::setup refuses a dump of the fix whose every differs from
+    # the fix's freq (src/fixgrid_ave.cpp:101-107); no GPU work happens first
+    from phafd_amd.computes import ComputeComplex, FixGridAve
+    from phafd_amd.dump import Dump, run_with_dumps
+    sq = ComputeComplex(None, "sq", "ft_phi", "norm")
+    ave = FixGridAve(None, "sqave", 2, 5, 10, sq)
+    bad = Dump(None, "ft", "ftgrid", "/nonexistent/sq", 20, ["c_sq", "f_sqave"], computes={"sq": sq, "sqave": ave})
+    with pytest.raises(ValueError, match="doesn't match dump frequency"):
+        run_with_dumps(None, 100, [bad], 1e-3, aves=[ave])

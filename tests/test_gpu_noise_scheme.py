@@ -1,0 +1,258 @@
+"""Thermal noise, the opt-in semi-implicit update, padded-row field I/O and
+atom/drag on the GPU, through the C ABI.
+
+Noise (src/conjugate_volfrac.cpp:72-133, src/conjugate.cpp:163-327):
+  * host fields (PHAFD_NOISE_FIELD) are queued and each update consumes one;
+    a run of n steps needs n queued fields;
+  * the device generator (PHAFD_NOISE_DEVICE) is pinned bit for bit against
+    numpy.random.Philox (the same Philox4x64-10 block function) and the
+    reference's amplitudes and Hermitian rules, then by its moments against
+    the closed-form variance, and a noisy run matches the oracle fed the same
+    per-step fields (1e-12).
+Semi-implicit (the formula left in a comment at src/conjugate.cpp:356):
+  closed-form decay of single Fourier modes with mu = 0 (1e-12), and the
+  oracle's restatement over a full workload (1e-10).
+"""
+import numpy as np
+import pytest
+
+import phafd_amd as P
+from oracle import oracle_ctypes as O
+from tests.helpers import workload, make_gpu, make_oracle, rel_l2
+
+pytestmark = pytest.mark.gpu
+PI_REF = 3.141592
+KEY1 = 0x5048414644
+MASK = (1 << 64) - 1
+
+
+def _philox_words(ctr0, step, key0):
+    """Words 0, 1 of Philox4x64-10 at counter (ctr0, step, 0, 0), key
+    (key0, KEY1), from numpy's generator (which increments its counter before
+    each block)."""
+    c0, c1 = (ctr0 - 1) & MASK, step - (1 if ctr0 == 0 else 0)
+    bg = np.random.Philox(counter=[c0, c1, 0, 0], key=[key0, KEY1])
+    w = bg.random_raw(2)
+    return int(w[0]), int(w[1])
+
+
+def _unit(w):
+    return float(w >> 11) * 2.0 ** -53 - 0.5
+
+
+def _expected_noise(shape, period, mobility, temp, dt, key0, step):
+    """The noise the update adds per mode, restated in Python floats (IEEE
+    double, no fused multiply-add: the library's operation order)."""
+    nz, ny, nx = shape
+    hx = nx // 2 + 1
+    A, B, H = ny, nz, hx - 1
+    qxs, qky, qkz = 2 * PI_REF / period[0], 2 * PI_REF / period[1], 2 * PI_REF / period[2]
+    invL3 = 1.0 / (period[0] * period[1] * period[2])
+    pc, pr, sdt = np.sqrt(12 * temp * mobility * invL3), np.sqrt(24 * temp * mobility * invL3), np.sqrt(dt)
+
+    def draw(i, j, k, real):
+        qz = -qky * (A - i) if i > A // 2 else qky * i
+        qy = -qkz * (B - j) if j > B // 2 else qkz * j
+        qx = qxs * k
+        q2 = qx * qx + qy * qy + qz * qz
+        w0, w1 = _philox_words((k * nz + j) * ny + i, step, key0)
+        if real:
+            return complex(pr * np.sqrt(q2) * sdt * _unit(w0), 0.0)
+        a = pc * np.sqrt(q2) * sdt
+        return complex(a * _unit(w0), a * _unit(w1))
+
+    out = np.zeros((nz, ny, hx), complex)
+    for j in range(nz):          # kz
+        for i in range(ny):      # ky
+            for k in range(hx):  # kx
+                if k not in (0, H):
+                    v = draw(i, j, k, False)
+                elif (i in (0, A // 2)) and (j in (0, B // 2)):
+                    v = 0j if (k == 0 and i == 0 and j == 0) else draw(i, j, k, True)
+                elif (1 <= j < B // 2) or (j == 0 and 1 <= i < A // 2) or (j == B // 2 and i > A // 2):
+                    v = draw(i, j, k, False)
+                else:
+                    v = draw((A - i) % A, (B - j) % B, k, False).conjugate()
+                out[j, i, k] = v
+    return out
+
+
+@pytest.mark.parametrize("shape,period", [((8, 8, 8), [8.0, 8.0, 8.0]),
+                                          ((4, 16, 8), [7.0, 20.0, 5.0])])
+def test_device_noise_matches_philox_restatement(shape, period):
+    key0 = P.make_unique_seed(5678)
+    for step in (1, 2, 1000):
+        got = P.noise_field(shape, period, 1.3, 0.7, 1e-3, key0, step)
+        ref = _expected_noise(shape, period, 1.3, 0.7, 1e-3, key0, step)
+        assert np.array_equal(got, ref), step
+
+
+def test_device_noise_moments_and_symmetry():
+    shape, L = (128, 128, 128), [128.0] * 3
+    nz, ny, nx = shape
+    key0 = P.make_unique_seed(5678)
+    T, M, dt = 1.0, 1.0, 1e-3
+    xi1 = P.noise_field(shape, L, M, T, dt, key0, 1)
+    xi2 = P.noise_field(shape, L, M, T, dt, key0, 2)
+    hx = nx // 2 + 1
+    kz = np.fft.fftfreq(nz, 1.0 / nz)[:, None, None]
+    ky = np.fft.fftfreq(ny, 1.0 / ny)[None, :, None]
+    kx = np.arange(hx)[None, None, :]
+    q = np.sqrt((2 * PI_REF / L[0] * kx) ** 2 + (2 * PI_REF / L[1] * ky) ** 2 + (2 * PI_REF / L[2] * kz) ** 2)
+    amp = np.sqrt(12 * T * M / np.prod(L)) * q * np.sqrt(dt)
+    bulk = (slice(None), slice(None), slice(1, hx - 1))
+    u = np.concatenate([(xi1[bulk] / amp[bulk]).real.ravel(), (xi1[bulk] / amp[bulk]).imag.ravel()])
+    n = u.size
+    # U(-1/2, 1/2): range, mean 0 (sd sqrt(1/12/n)), E[U^2] = 1/12 (sd sqrt(1/180/n))
+    assert u.min() >= -0.5 and u.max() < 0.5
+    assert abs(u.mean()) < 5 * np.sqrt(1 / 12 / n)
+    assert abs((u * u).mean() - 1 / 12) < 5 * np.sqrt(1 / 180 / n)
+    # fresh draws per step: no correlation between steps, nor between parts
+    u2 = (xi2[bulk] / amp[bulk]).real.ravel()
+    m = u2.size
+    assert abs(np.corrcoef(u[:m], u2)[0, 1]) < 5 / np.sqrt(m)
+    assert abs(np.corrcoef(u[:m], u[m:])[0, 1]) < 5 / np.sqrt(m)
+    # Hermitian rules of the kx = 0 and Nx/2 planes: the mirror (-ky, -kz)
+    # holds the conjugate, self-conjugate modes are real, the origin is 0
+    for k in (0, hx - 1):
+        pl = xi1[:, :, k]
+        mir = np.conj(pl[(-np.arange(nz)) % nz][:, (-np.arange(ny)) % ny])
+        assert np.array_equal(pl, mir)
+    for j in (0, nz // 2):
+        for i in (0, ny // 2):
+            for k in (0, hx - 1):
+                assert xi1[j, i, k].imag == 0.0
+    assert xi1[0, 0, 0] == 0
+    # self-conjugate modes: sqrt(24 T M / L^3) |q| sqrt(dt) U, |U| < 1/2
+    rp = np.sqrt(24 * T * M / np.prod(L)) * q[nz // 2, 0, 0] * np.sqrt(dt)
+    assert abs(xi1[nz // 2, 0, 0].real) <= 0.5 * rp
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_device_noise_run_matches_oracle(fused):
+    w = workload(n=32, nchains=5, nbeads=20, seed=6, noise=P.NOISE_DEVICE)
+    g = make_gpu(w)
+    if not fused:
+        g.set_option(P.OPT_FUSE_SPECTRAL, 0)
+    w2 = dict(w)
+    w2["noise"] = O.NOISE_FIELD
+    o = make_oracle(w2)
+    key0 = P.make_unique_seed(5678)
+    g.run(3)
+    for step in (1, 2, 3):
+        o.set_noise(P.noise_field((32, 32, 32), [32.0] * 3, 1.0, 1.0, 1e-3, key0, step))
+        o.run(1)
+    assert rel_l2(g.get_field(P.PHI), o.get_field(P.PHI)) < 1e-12
+    assert rel_l2(g.get_atoms()[0], o.get_atoms()[0]) < 1e-12
+
+
+def test_noise_fields_are_consumed_once():
+    w = workload(n=32, nchains=5, nbeads=20, seed=6, noise=P.NOISE_FIELD)
+    g = make_gpu(w)
+    w2 = dict(w)
+    w2["noise"] = O.NOISE_MT19937
+    o = make_oracle(w2)
+    gen = P.NoiseGen(5678)
+    xi = [gen.next(32, 32, 32, [32.0] * 3, 1.0, 1.0, 1e-3) for _ in range(3)]
+    g.set_noise(xi[0])
+    assert g.noise_queued() == 1
+    with pytest.raises(P.PhafdError) as e:   # one field cannot serve two updates
+        g.run(2)
+    assert e.value.code == -1
+    assert g.noise_queued() == 1             # nothing ran
+    g.set_noise(xi[1])
+    g.run(2)
+    assert g.noise_queued() == 0
+    with pytest.raises(P.PhafdError):
+        g.run(1)
+    g.set_noise(xi[2])
+    g.run(1)
+    o.run(3)
+    assert rel_l2(g.get_field(P.PHI), o.get_field(P.PHI)) < 1e-12
+    g.set_noise(xi[0])
+    g.set_noise(None)
+    assert g.noise_queued() == 0
+
+
+def _single_modes(shape, amps):
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    (a1, n1), (a2, n2), (a3, n3) = amps
+    return (0.5 + a1 * np.cos(2 * np.pi * n1 * x / nx) + a2 * np.sin(2 * np.pi * n2 * y / ny)
+            + a3 * np.cos(2 * np.pi * n3 * z / nz))
+
+
+@pytest.mark.parametrize("semi,dt,K", [(False, 1e-3, 200), (True, 0.1, 100)])
+def test_single_mode_closed_form(semi, dt, K):
+    # mu = 0 (no Flory-Huggins, no atoms): each Fourier mode decays by the
+    # closed-form factor per step, 1 - c s (explicit) or 1 / (1 + c s)
+    # (semi-implicit), c s = M q^2 dt (T/v) gamma q^2, q from 2*3.141592/L
+    shape, L = (32, 16, 64), [64.0, 16.0, 32.0]
+    nz, ny, nx = shape
+    amps = [(0.01, 3), (0.02, 2), (0.015, 5)]
+    Mb, T, v, gam = 1.3, 0.9, 1.1, 0.8
+    s = P.System(nx, ny, nz, [0.0] * 3, L)
+    s.set_field(P.PHI, _single_modes(shape, amps))
+    z = np.zeros(0, np.int32)
+    s.set_atoms(np.zeros((0, 3)), z, tags=z, mols=z)
+    s.fix_conjugate(5678, Mb, T, v, gam, P.NOISE_OFF)
+    s.setup(dt)
+    s.set_option(P.OPT_SEMI_IMPLICIT, 1 if semi else 0)
+    s.run(K)
+    fac = []
+    for (a, n), (Lax, Nax) in zip(amps, [(L[0], nx), (L[1], ny), (L[2], nz)]):
+        q = 2 * PI_REF / Lax * n
+        cs = Mb * q * q * dt * (T / v * gam * q * q)
+        f = 1.0 / (1.0 + cs) if semi else 1.0 - cs
+        fac.append((a * f ** K, n))
+    expect = _single_modes(shape, fac)
+    got = s.get_field(P.PHI)
+    assert np.max(np.abs(got - expect)) < 1e-12
+    # the modes did decay by more than the tolerance
+    assert np.max(np.abs(got - _single_modes(shape, amps))) > 1e-4
+
+
+@pytest.mark.parametrize("zghost", [0, 1])
+def test_semi_implicit_workload_vs_oracle(zghost):
+    w = workload(n=32, nchains=8, nbeads=25, seed=13, zghost=zghost)
+    g = make_gpu(w)
+    o = make_oracle(w)
+    g.set_option(P.OPT_SEMI_IMPLICIT, 1)
+    o.set_semi_implicit(True)
+    g.run(60)
+    o.run(60)
+    assert rel_l2(g.get_field(P.PHI), o.get_field(P.PHI)) < 1e-10
+    assert rel_l2(g.get_atoms()[0], o.get_atoms()[0]) < 1e-10
+
+
+def test_padded_rows_round_trip():
+    # the reference's real fftw arrays pad each x row to 2*(Nx/2+1) doubles
+    nz, ny, nx = 16, 8, 32
+    ld = 2 * (nx // 2 + 1)
+    s = P.System(nx, ny, nz, [0.0] * 3, [32.0, 8.0, 16.0])
+    phi = P.ic_noisy_constant((nz, ny, nx), 0.4, 0.04, 77)
+    padded = np.full((nz, ny, ld), np.nan)
+    padded[:, :, :nx] = phi
+    s.set_field_ld(P.PHI, padded, ld)
+    assert np.array_equal(s.get_field(P.PHI), phi)
+    out = np.full((nz, ny, ld), -7.0)
+    s.get_field_ld(P.PHI, out, ld)
+    assert np.array_equal(out[:, :, :nx], phi)
+    assert np.all(out[:, :, nx:] == -7.0)   # padding untouched
+    with pytest.raises(P.PhafdError):
+        s.set_field_ld(P.PHI, padded, nx - 1)
+
+
+@pytest.mark.parametrize("zghost", [0, 1])
+@pytest.mark.parametrize("chain", [False, True])
+def test_drag_trajectory_200_steps(zghost, chain):
+    # fix atom/drag (src/fixatom_drag.cpp:43-66), alone and beside the chain fix
+    w = workload(n=64, nchains=20, nbeads=50, seed=42, zghost=zghost, chi=3.0, drag=True, chain=chain)
+    g = make_gpu(w)
+    o = make_oracle(w)
+    g.run(200)
+    o.run(200)
+    assert rel_l2(g.get_field(P.PHI), o.get_field(P.PHI)) < 1e-10
+    assert rel_l2(g.get_atoms()[0], o.get_atoms()[0]) < 1e-10
+    assert rel_l2(g.get_atoms()[1], o.get_atoms()[1]) < 1e-10
+    assert g.counts()["ncalls"] == o.counts()["ncalls"]

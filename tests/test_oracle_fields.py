@@ -98,3 +98,40 @@ def test_mt19937_standard_kat():
     # uniform_real_distribution(-0.5, 0.5) = (lo + hi*2^32)/2^64 - 0.5 (libstdc++)
     u = O.mt19937_uniform(5489, 1)[0]
     assert u == (3499211612 + 581869302 * 2.0 ** 32) / 2.0 ** 64 - 0.5
+
+
+def _modes(shape, amps):
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    (a1, n1), (a2, n2), (a3, n3) = amps
+    return (0.5 + a1 * np.cos(2 * np.pi * n1 * x / nx) + a2 * np.sin(2 * np.pi * n2 * y / ny)
+            + a3 * np.cos(2 * np.pi * n3 * z / nz))
+
+
+def test_semi_implicit_and_explicit_closed_form_decay():
+    # mu = 0 (no Flory-Huggins, no atoms): K steps scale each Fourier mode by
+    # (1 - c s)^K (explicit, src/conjugate_volfrac.cpp:106-109) or
+    # (1 + c s)^-K (semi-implicit, src/conjugate.cpp:356), c s = M q^2 dt T/v
+    # gamma q^2 with q = 2*3.141592 n / L; a dropped factor, the wrong axis
+    # scale or the wrong sign fails the 1e-12 bar
+    shape, L = (16, 8, 32), [32.0, 8.0, 16.0]
+    nz, ny, nx = shape
+    amps = [(0.01, 3), (0.02, 1), (0.015, 2)]
+    Mb, T, v, gam = 1.3, 0.9, 1.1, 0.8
+    for semi, dt, K in ((False, 1e-3, 50), (True, 0.1, 40)):
+        s = O.System(nx, ny, nz, [0.0] * 3, L)
+        s.set_field(0, _modes(shape, amps))
+        z = np.zeros(0, np.int32)
+        s.set_atoms(np.zeros((0, 3)), z, tags=z, mols=z)
+        s.fix_conjugate(5678, Mb, T, v, gam, O.NOISE_OFF)
+        s.setup(dt)
+        s.set_semi_implicit(semi)
+        s.run(K)
+        fac = []
+        for (a, n), Lax in zip(amps, L):
+            q = 2 * 3.141592 / Lax * n
+            cs = Mb * q * q * dt * (T / v * gam * q * q)
+            fac.append((a * (1.0 / (1.0 + cs) if semi else 1.0 - cs) ** K, n))
+        got = s.get_field(0)
+        assert np.max(np.abs(got - _modes(shape, fac))) < 1e-12
+        assert np.max(np.abs(got - _modes(shape, amps))) > 1e-5
