@@ -318,10 +318,10 @@ __global__ void k_bin_sort(int nbins, const int *binstart, int *binatoms, const 
 // NPairGridBin::build, src/npair_grid_bin.cpp:74-125: for every grid point in
 // (z, y, x) order, walk the full stencil in order and each bin in its list
 // order; keep atoms with rsq <= cutneighsq.  Two passes: count, then fill.
-template <bool FILL>
+template <bool FILL, class OffT>
 __global__ void k_grid_list(GridView gv, BinGeom b, const int *stencil, int nst,
                             const int *binstart, const int *binatoms, const double4 *gx,
-                            double cutneighsq, int nowned, int *cnt, const int64_t *off,
+                            double cutneighsq, int nowned, int *cnt, const OffT *off,
                             int *neigh, int *overflow) {
   const size_t N = (size_t)gv.nx * gv.ny * gv.nz;
   GRID_STRIDE(p, N) {
@@ -333,7 +333,7 @@ __global__ void k_grid_list(GridView gv, BinGeom b, const int *stencil, int nst,
     const double xtmp = i * gv.dx + gv.lo[0];
     const int ibin = coord2bin(b, xtmp, ytmp, ztmp);
     int n = 0;
-    int64_t w = FILL ? off[p] : 0;
+    OffT w = FILL ? off[p] : 0;
     for (int m = 0; m < nst; m++) {
       const int bb = ibin + stencil[m];
       const int s = binstart[bb], e = binstart[bb + 1];
@@ -517,13 +517,17 @@ __device__ __forceinline__ int bin1(double c, double lo, double hi, double inv, 
 struct GABox {
   int i0, i1, j0, j1, k0, k1;
 };
-__device__ __forceinline__ GABox ga_box(const GridView &gv, const double4 &x, double cs) {
+template <class GV>
+__device__ __forceinline__ GABox ga_box(const GV &gv, const double4 &x, double cs) {
+  // products with the inverse spacings, widened by a margin far above their
+  // rounding: the box only bounds the walk, the exact test decides
   const double w = sqrt(cs);
   const double mg = 1e-9;
+  const double iz = 1.0 / gv.dz, iy = 1.0 / gv.dy, ix = 1.0 / gv.dx;
   GABox b;
-  b.i0 = (int)ceil((x.z - w - gv.lo[2]) / gv.dz - mg); b.i1 = (int)floor((x.z + w - gv.lo[2]) / gv.dz + mg);
-  b.j0 = (int)ceil((x.y - w - gv.lo[1]) / gv.dy - mg); b.j1 = (int)floor((x.y + w - gv.lo[1]) / gv.dy + mg);
-  b.k0 = (int)ceil((x.x - w - gv.lo[0]) / gv.dx - mg); b.k1 = (int)floor((x.x + w - gv.lo[0]) / gv.dx + mg);
+  b.i0 = (int)ceil((x.z - w - gv.lo[2]) * iz - mg); b.i1 = (int)floor((x.z + w - gv.lo[2]) * iz + mg);
+  b.j0 = (int)ceil((x.y - w - gv.lo[1]) * iy - mg); b.j1 = (int)floor((x.y + w - gv.lo[1]) * iy + mg);
+  b.k0 = (int)ceil((x.x - w - gv.lo[0]) * ix - mg); b.k1 = (int)floor((x.x + w - gv.lo[0]) * ix + mg);
   b.i0 = b.i0 < 0 ? 0 : b.i0; b.j0 = b.j0 < 0 ? 0 : b.j0; b.k0 = b.k0 < 0 ? 0 : b.k0;
   b.i1 = b.i1 > gv.nz - 1 ? gv.nz - 1 : b.i1;
   b.j1 = b.j1 > gv.ny - 1 ? gv.ny - 1 : b.j1;
@@ -536,7 +540,8 @@ __device__ __forceinline__ bool ga_in(const GAParams &ga, double rsq, double cs)
 // x range of one (z, y) row of the box that can be in range: the sphere's
 // chord widened by one grid point each side (the exact test stays in the
 // loop, so the widening only has to cover the rounding of this estimate)
-__device__ __forceinline__ bool ga_row(const GridView &gv, const GABox &b, double xx, double cs, double dyz,
+template <class GV>
+__device__ __forceinline__ bool ga_row(const GV &gv, const GABox &b, double xx, double cs, double dyz,
                                        int &k0, int &k1) {
   const double rem = cs - dyz;
   if (rem < -1e-9 * cs) return false;
@@ -550,142 +555,28 @@ __device__ __forceinline__ bool ga_row(const GridView &gv, const GABox &b, doubl
 
 // Grid-atom pass (PairGridAtomGaussian / PairGridAtomLJish::compute,
 // src/pair_gridatom_gaussian.cpp:69-118, src/pair_gridatom_lj_ish.cpp:72-178).
-// The reference adds to F(aj) once per listed grid point, in ascending grid
-// index, and to mu(g) once per listed atom, in list order.  Every grid point
-// with rsq <= cutsq(type) is in the atom's list (the list cut is cutsq + skin
-// and no atom has moved more than skin/2 since the build), so walking the
-// in-box grid points around the atom in (z, y, x) order finds the same pairs
-// in the force's order.  A grid point's list walks the stencil bins in
-// (dz, dy, dx) lexicographic order (src/nstencil_full_bin_3d.cpp:30-36) and
-// each bin in its list order (src/nbin_standard.cpp:181-229); bins are
-// numbered z-major, so that order is ascending position q in the bin-sorted
-// atom array: the key mu terms are summed by.
-//
-// Pipeline: per atom (bin order) count then enumerate its in-range points;
-// per pair the expensive part (exp, field gathers, the mu term, the force
-// term, the pair's rank among its grid point's terms); per atom the force
-// fold in pair order; per pair the placement by grid point; per grid point
-// the sum in key order.  `order` is the bin-sorted atom array.
-__global__ void k_ga_count(int ng, GridView gv, GAParams ga, const double4 *gx, const int *order,
-                           const double *cutsq, int *acnt) {
-  GRID_STRIDE(q, ng) {
-    const double4 x = gx[order[q]];
-    int n = 0;
-    if (isfinite(x.x) && isfinite(x.y) && isfinite(x.z)) {
-      const double cs = cutsq[(int)x.w];
-      const GABox b = ga_box(gv, x, cs);
-      for (int i = b.i0; i <= b.i1; i++) {
-        const double delz = (i * gv.dz + gv.lo[2]) - x.z;
-        for (int j = b.j0; j <= b.j1; j++) {
-          const double dely = (j * gv.dy + gv.lo[1]) - x.y;
-          int rk0, rk1;
-          if (!ga_row(gv, b, x.x, cs, dely * dely + delz * delz, rk0, rk1)) continue;
-          for (int k = rk0; k <= rk1; k++) {
-            const double delx = (k * gv.dx + gv.lo[0]) - x.x;
-            const double rsq = delx * delx + dely * dely + delz * delz;
-            n += ga_in(ga, rsq, cs) ? 1 : 0;
-          }
-        }
-      }
-    }
-    acnt[q] = n;
-  }
-}
-
-// in-range grid point and atom position of every pair, atom by atom
-__global__ void k_ga_enum(int ng, GridView gv, GAParams ga, const double4 *gx, const int *order, const int *aoff,
-                          int64_t cap, int *pg, int *pq, int *overflow) {
-  GRID_STRIDE(q, ng) {
-    const double4 x = gx[order[q]];
-    if (!isfinite(x.x) || !isfinite(x.y) || !isfinite(x.z)) continue;
-    const double cs = ga.cutsq[(int)x.w];
-    const GABox b = ga_box(gv, x, cs);
-    int w = aoff[q];
-    if (aoff[q + 1] > cap) {
-      atomicOr(overflow, 1);
-      continue;
-    }
-    for (int i = b.i0; i <= b.i1; i++) {
-      const double delz = (i * gv.dz + gv.lo[2]) - x.z;
-      for (int j = b.j0; j <= b.j1; j++) {
-        const double dely = (j * gv.dy + gv.lo[1]) - x.y;
-        int rk0, rk1;
-        if (!ga_row(gv, b, x.x, cs, dely * dely + delz * delz, rk0, rk1)) continue;
-        for (int k = rk0; k <= rk1; k++) {
-          const double delx = (k * gv.dx + gv.lo[0]) - x.x;
-          const double rsq = delx * delx + dely * dely + delz * delz;
-          if (!ga_in(ga, rsq, cs)) continue;
-          pg[w] = (int)(((size_t)i * gv.ny + j) * gv.nx + k);
-          pq[w] = (int)q;
-          w++;
-        }
-      }
-    }
-  }
-}
-
-__device__ __forceinline__ float fexp(float x) { return expf(x); }
-__device__ __forceinline__ double fexp(double x) { return exp(x); }
+// The reference walks its grid list: for every grid point the listed atoms in
+// list order, adding to mu(g) and to F(aj) per in-cut pair.  F(aj) thus
+// receives its terms in ascending grid index, and mu(g) its terms in list
+// order: the stencil bins in (dz, dy, dx) lexicographic order
+// (src/nstencil_full_bin_3d.cpp:30-36), each bin in its list order
+// (src/nbin_standard.cpp:181-229).  Bins are numbered z-major, so that is
+// ascending position q in the bin-sorted atom array.  Every atom within the
+// pair cut of a point is in its list: the list cut is cut + skin and no atom
+// moves more than skin/2 between builds.  So here, at each build, the pairs
+// within (cut + skin/2) are listed twice: per atom in ascending grid index
+// and per grid point in ascending q.  Each step:
+//   k_ga_force: one thread per atom walks its pairs, keeps those in the cut
+//     now, subtracts their force terms in order and stores each one's mu term;
+//   k_ga_list_mu: one thread per grid point adds its pairs' mu terms in q
+//     order.
+// The exponentials are evaluated once per pair, by the atom side, with every
+// lane busy; the sums are those of the reference.
 template <class R> struct Vec4;
 template <> struct Vec4<double> { using T = double4; };
 template <> struct Vec4<float> { using T = float4; };
-
-// one pair: the loop body of the reference's compute, with the same
-// arithmetic (rsq rebuilt from the grid indices exactly as the walk made it).
-// FP32 form: rsq in FP64 from the FP64 positions, everything after it in
-// FP32; the energy term in FP64.
-template <class R>
-__global__ void k_ga_pair(const int *npairs, GridViewT<R> gv, GAParams ga, const double4 *gx, const int *order,
-                          const int *pg, const int *pq, typename Vec4<R>::T *ft, R *pv, int *ppos, int *gcnt,
-                          bool emit, double *energy) {
-  const int np = *npairs;
-  double esum = 0.0;
-  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < np; w += gridDim.x * blockDim.x) {
-    const int p = pg[w];
-    const double4 x = gx[order[pq[w]]];
-    const int t = (int)x.w;
-    const double cs = ga.cutsq[t];
-    const int k = p % gv.nx, j = (p / gv.nx) % gv.ny, i = p / (gv.nx * gv.ny);
-    const double ztmp = i * gv.dz + gv.lo[2];
-    const double delz = ztmp - x.z;
-    const double ytmp = j * gv.dy + gv.lo[1];
-    const double dely = ytmp - x.y;
-    const double xtmp = k * gv.dx + gv.lo[0];
-    const double delx = xtmp - x.x;
-    double rsq = delx * delx + dely * dely + delz * delz;
-    rsq /= cs;
-    const R r = (R)rsq;
-    R expfac;
-    if (ga.style == 0) expfac = (fexp(-r / R(2)) - (R)ga.em05) * (R)ga.e05 * (R)ga.eps[t];
-    else expfac = fexp(R(-1) / (1 - r) + 1) * (R)ga.eps[t];
-    if (ga.style == GA_LJISH_LINEAR) {
-      // PairGridAtomLJishLinear::compute (src/pair_gridatom_lj_ish_linear.cpp:
-      // 93-160): e = -1 * bump * eps (the sign flip is exact); mu += e, the
-      // force weight is e dv, and no energy
-      expfac = -expfac;
-      if (emit) {
-        pv[w] = expfac;
-        ppos[w] = atomicAdd(&gcnt[p], 1);
-      }
-      expfac *= (R)gv.dv;
-    } else {
-      const R delphi = (gv.phi[p] - (R)ga.phit[t]);
-      if (emit) {   // mu term, summed later in list order
-        pv[w] = 2 * delphi * expfac;
-        ppos[w] = atomicAdd(&gcnt[p], 1);
-      }
-      if (energy) esum += (double)delphi * (double)delphi * (double)expfac * gv.dv;
-      expfac *= 2 * delphi * (R)gv.dv;
-    }
-    typename Vec4<R>::T f;
-    f.x = gv.gx[p] * expfac;
-    f.y = gv.gy[p] * expfac;
-    f.z = gv.gz[p] * expfac;
-    f.w = R(0);
-    ft[w] = f;
-  }
-  if (energy && esum != 0.0) atomicAdd(energy, esum);
-}
+__device__ __forceinline__ float fexp(float x) { return expf(x); }
+__device__ __forceinline__ double fexp(double x) { return exp(x); }
 
 // Kahan-compensated running sum (FP32 form)
 struct Kahan {
@@ -697,73 +588,9 @@ struct Kahan {
     s = t;
   }
 };
-
-// F(a) -= its pair terms in ascending grid index, as the reference adds them
-// (FP64); FP32 form: the terms summed with Kahan compensation, then taken
-// from F in FP64
-__global__ void k_ga_fold(int ng, const int *order, const int *aoff, const double4 *ft, double4 *gF) {
-  GRID_STRIDE(q, ng) {
-    const int a = order[q];
-    const int s = aoff[q], e = aoff[q + 1];
-    if (s == e) continue;
-    double4 F = gF[a];
-    for (int w = s; w < e; w++) {
-      const double4 f = ft[w];
-      F.x -= f.x;
-      F.y -= f.y;
-      F.z -= f.z;
-    }
-    gF[a] = F;
-  }
-}
-__global__ void k_ga_fold(int ng, const int *order, const int *aoff, const float4 *ft, double4 *gF) {
-  GRID_STRIDE(q, ng) {
-    const int a = order[q];
-    const int s = aoff[q], e = aoff[q + 1];
-    if (s == e) continue;
-    Kahan kx, ky, kz;
-    for (int w = s; w < e; w++) {
-      const float4 f = ft[w];
-      kx.add(f.x);
-      ky.add(f.y);
-      kz.add(f.z);
-    }
-    double4 F = gF[a];
-    F.x -= (double)kx.s;
-    F.y -= (double)ky.s;
-    F.z -= (double)kz.s;
-    gF[a] = F;
-  }
-}
-
-// every term's key to its grid point's segment (in arbitrary order: the rank
-// came from an atomic)
-__global__ void k_ga_place(const int *npairs, const int *pg, const int *ppos, const int *pq, const int *goff,
-                           int *skey) {
-  const int np = *npairs;
-  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < np; w += gridDim.x * blockDim.x)
-    skey[goff[pg[w]] + ppos[w]] = pq[w];
-}
-
-// every term's value to its place in key (list) order inside its segment:
-// its rank is the number of smaller keys there (keys of a segment differ)
+// running sum of a point's mu terms: FP64 plain, in order; FP32 Kahan
 template <class R>
-__global__ void k_ga_order(const int *npairs, const int *pg, const int *pq, const R *pv, const int *goff,
-                           const int *skey, R *sv) {
-  const int np = *npairs;
-  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < np; w += gridDim.x * blockDim.x) {
-    const int p = pg[w], key = pq[w];
-    const int s = goff[p], e = goff[p + 1];
-    int r = 0;
-    for (int j = s; j < e; j++) r += skey[j] < key ? 1 : 0;
-    sv[s + r] = pv[w];
-  }
-}
-
-// mu(g) (+)= its terms, already in key order, four grid points per item;
-// FP32 form: Kahan-compensated
-template <class R>
-struct MuAcc {   // FP64: plain running sum in order
+struct MuAcc {
   R m;
   __device__ __forceinline__ void add(R v) { m += v; }
   __device__ __forceinline__ R get() const { return m; }
@@ -779,43 +606,299 @@ struct MuAcc<float> {
   }
   __device__ __forceinline__ float get() const { return m; }
 };
-// Four grid points per thread.  Their terms are contiguous in sv ([o.x,
-// o4)); they are loaded GS at a time (independent loads in flight) and
-// added to their point's sum in index order, the same order as one term at
-// a time.
+// running force of one atom: FP64 subtracts each term from F in order; FP32
+// sums the terms with Kahan compensation and subtracts the sum once
 template <class R>
-__global__ void k_ga_sum(size_t N, const int *__restrict__ goff, const R *__restrict__ sv, R *__restrict__ mu,
-                         bool first) {
-  constexpr int GS = 8;
-  const size_t N4 = N / 4;
-  for (size_t u = blockIdx.x * (size_t)blockDim.x + threadIdx.x; u < N4; u += (size_t)gridDim.x * blockDim.x) {
-    const size_t p0 = 4 * u;
-    const int4 o = *reinterpret_cast<const int4 *>(goff + p0);
-    const int o4 = goff[p0 + 4];
-    if (!first && o.x == o4) continue;
-    MuAcc<R> a0, a1, a2, a3;
-    a0.m = first ? R(0) : mu[p0];
-    a1.m = first ? R(0) : mu[p0 + 1];
-    a2.m = first ? R(0) : mu[p0 + 2];
-    a3.m = first ? R(0) : mu[p0 + 3];
-    for (int j = o.x; j < o4; j += GS) {
-      R t[GS];
-#pragma unroll
-      for (int k = 0; k < GS; k++) t[k] = j + k < o4 ? sv[j + k] : R(0);
-#pragma unroll
-      for (int k = 0; k < GS; k++) {
-        const int jj = j + k;
-        if (jj >= o4) break;
-        if (jj < o.y) a0.add(t[k]);
-        else if (jj < o.z) a1.add(t[k]);
-        else if (jj < o.w) a2.add(t[k]);
-        else a3.add(t[k]);
+struct ForceAcc {
+  double4 F;
+  __device__ __forceinline__ void sub(double fx, double fy, double fz) { F.x -= fx; F.y -= fy; F.z -= fz; }
+  __device__ __forceinline__ double4 get() const { return F; }
+};
+template <>
+struct ForceAcc<float> {
+  double4 F;
+  Kahan kx, ky, kz;
+  __device__ __forceinline__ void sub(float fx, float fy, float fz) { kx.add(fx); ky.add(fy); kz.add(fz); }
+  __device__ __forceinline__ double4 get() const {
+    double4 r = F;
+    r.x -= (double)kx.s;
+    r.y -= (double)ky.s;
+    r.z -= (double)kz.s;
+    return r;
+  }
+};
+
+// the kernel value e of one in-cut pair (rsq already divided by cutsq): the
+// expressions of the reference's loops; FP32 form: rsq in FP64 from the FP64
+// positions, e in FP32
+template <class R>
+__device__ __forceinline__ R ga_expfac(const GAParams &ga, double rsq_over_cs, int t) {
+  const R r = (R)rsq_over_cs;
+  if (ga.style == 0) return (fexp(-r / R(2)) - (R)ga.em05) * (R)ga.e05 * (R)ga.eps[t];
+  return fexp(R(-1) / (1 - r) + 1) * (R)ga.eps[t];
+}
+
+// Blocks are dispatched to the 8 XCDs round robin (block b to XCD b % 8),
+// each XCD with its own L2.  This maps them so that each XCD takes one
+// contiguous range of logical blocks (spatially neighbouring work, sharing
+// field and atom lines in its L2); a bijection of [0, G).
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G) {
+  const unsigned x = b % 8u, per = G / 8u, rem = G % 8u;
+  const unsigned first = x < rem ? x * (per + 1u) : rem * (per + 1u) + (x - rem) * per;
+  return first + b / 8u;
+}
+
+// ---- the pair lists of a build.  am: for each atom (bin order q) the grid
+// points within reach (largest pair cut + skin/2) of its build position, in
+// ascending grid index; pm: the same pairs grouped by grid point, in
+// ascending q within a point (the reference's list order).  Every pair in
+// the pair cut at any step before the next build is among them (no atom
+// moves more than skin/2 in between).
+__global__ void k_ga_am_walk(int ng, GridView gv, const double4 *__restrict__ gxb, const int *__restrict__ order,
+                             double reachsq, int *__restrict__ cnt, const int *__restrict__ off,
+                             int2 *__restrict__ ps, int *__restrict__ amq) {
+  GRID_STRIDE(q, ng) {
+    const double4 x = gxb[order[q]];
+    int n = 0;
+    if (isfinite(x.x) && isfinite(x.y) && isfinite(x.z)) {
+      const GABox b = ga_box(gv, x, reachsq);
+      int w = off ? off[q] : 0;
+      for (int i = b.i0; i <= b.i1; i++) {
+        const double delz = (i * gv.dz + gv.lo[2]) - x.z;
+        for (int j = b.j0; j <= b.j1; j++) {
+          const double dely = (j * gv.dy + gv.lo[1]) - x.y;
+          for (int k = b.k0; k <= b.k1; k++) {
+            const double delx = (k * gv.dx + gv.lo[0]) - x.x;
+            if (delx * delx + dely * dely + delz * delz <= reachsq) {
+              if (off) {
+                ps[w].x = (int)(((size_t)i * gv.ny + j) * gv.nx + k);
+                amq[w] = (int)q;
+                w++;
+              }
+              n++;
+            }
+          }
+        }
       }
     }
-    mu[p0] = a0.get();
-    mu[p0 + 1] = a1.get();
-    mu[p0 + 2] = a2.get();
-    mu[p0 + 3] = a3.get();
+    if (!off) cnt[q] = n;
+  }
+}
+// per pair: its rank among its point's pairs (arbitrary), and the point count
+__global__ void k_ga_pm_count(int nent, const int2 *__restrict__ ps, int *__restrict__ pcnt, int *__restrict__ rank) {
+  GRID_STRIDE(e, nent) rank[e] = atomicAdd(&pcnt[ps[e].x], 1);
+}
+// per pair: its place in its point's segment, carrying (q, entry)
+__global__ void k_ga_pm_place(int nent, const int2 *__restrict__ ps, const int *__restrict__ rank,
+                              const int *__restrict__ amq, const int *__restrict__ poff, int *__restrict__ key,
+                              int *__restrict__ ent) {
+  GRID_STRIDE(e, nent) {
+    const int slot = poff[ps[e].x] + rank[e];
+    key[slot] = amq[e];
+    ent[slot] = e;
+  }
+}
+// each point's segment into ascending q, then ps[entry].y = its slot.
+// GM_U points per lane, 64 apart; segments of up to 8 are sorted in
+// registers (their loads issued together), longer ones in memory.
+constexpr int GM_U = 8;
+__global__ __launch_bounds__(256) void k_ga_pm_sort(size_t N, const int *__restrict__ poff, int *__restrict__ key,
+                                                    int *__restrict__ ent, int2 *__restrict__ ps) {
+  const size_t wbase = ((size_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * (64 * GM_U);
+  const int lane = threadIdx.x & 63;
+  int s[GM_U], n[GM_U];
+#pragma unroll
+  for (int u = 0; u < GM_U; u++) {
+    const size_t p = wbase + lane + 64 * u;
+    s[u] = p < N ? poff[p] : 0;
+    n[u] = p < N ? poff[p + 1] - s[u] : 0;
+  }
+#pragma unroll
+  for (int u = 0; u < GM_U; u++) {
+    const int a = s[u], m = n[u];
+    if (m == 0) continue;
+    if (m == 1) {
+      ps[ent[a]].y = a;
+    } else if (m == 2) {
+      const int k0 = key[a], k1 = key[a + 1], e0 = ent[a], e1 = ent[a + 1];
+      const bool sw = k0 > k1;
+      ps[sw ? e1 : e0].y = a;
+      ps[sw ? e0 : e1].y = a + 1;
+    } else if (m <= 8) {
+      int kk[8], ee[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        kk[i] = i < m ? key[a + i] : 0x7fffffff;
+        ee[i] = i < m ? ent[a + i] : 0;
+      }
+      // insertion sort over the fixed 8 (keys of a segment differ)
+#pragma unroll
+      for (int i = 1; i < 8; i++) {
+#pragma unroll
+        for (int j = i; j > 0; j--) {
+          const bool sw = kk[j - 1] > kk[j];
+          const int tk = sw ? kk[j - 1] : kk[j], te = sw ? ee[j - 1] : ee[j];
+          kk[j - 1] = sw ? kk[j] : kk[j - 1];
+          ee[j - 1] = sw ? ee[j] : ee[j - 1];
+          kk[j] = tk;
+          ee[j] = te;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        if (i < m) {
+          key[a + i] = kk[i];
+          ent[a + i] = ee[i];
+          ps[ee[i]].y = a + i;
+        }
+      }
+    } else {
+      for (int i = a + 1; i < a + m; i++) {
+        const int kv = key[i], ev = ent[i];
+        int j = i - 1;
+        while (j >= a && key[j] > kv) {
+          key[j + 1] = key[j];
+          ent[j + 1] = ent[j];
+          j--;
+        }
+        key[j + 1] = kv;
+        ent[j + 1] = ev;
+      }
+      for (int i = a; i < a + m; i++) ps[ent[i]].y = i;
+    }
+  }
+}
+
+// The step's grid-atom pass, atom side: one thread per atom (bin order q)
+// walks its build pairs in ascending grid index, keeps those in the pair
+// cut now (the reference's test), and subtracts their force terms in that
+// order.  The mu term of each pair it keeps goes to its grid point's slot,
+// mterm[ps[entry].y] (NaN for a pair out of the cut).  Entries are taken
+// GA_B at a time: the tests and field loads first, then the terms in
+// order, so that up to GA_B pairs' loads are in flight.  Grid sides are
+// powers of two (lx = log2 nx, lxy = log2 (nx ny)).
+constexpr int GA_B = 8;
+template <class R>
+__global__ __launch_bounds__(256) void k_ga_force(int ng, GridViewT<R> gv, GAParams ga,
+                                                  const double4 *__restrict__ gx, const int *__restrict__ order,
+                                                  const int *__restrict__ aoff, const int2 *__restrict__ ps,
+                                                  R *__restrict__ mterm,
+                                                  double4 *__restrict__ gF, double *energy, int lx, int lxy) {
+  double esum = 0.0;
+  const size_t bl = xcd_block(blockIdx.x, gridDim.x);
+  const int mx = gv.nx - 1, my = gv.ny - 1;
+  for (size_t q = bl * blockDim.x + threadIdx.x; q < (size_t)ng; q += (size_t)gridDim.x * blockDim.x) {
+    const int s = aoff[q], e = aoff[q + 1];
+    const int a = order[q];
+    if (s == e) continue;
+    const double4 x = gx[a];
+    const double4 F0 = gF[a];
+    const int t = (int)x.w;
+    const double cs = ga.cutsq[t];
+    const R phit = (R)ga.phit[t];
+    ForceAcc<R> acc;
+    acc.F = F0;
+    bool any = false;
+    for (int c0 = s; c0 < e; c0 += GA_B) {
+      int pp[GA_B], so[GA_B];
+#pragma unroll
+      for (int u = 0; u < GA_B; u++) {
+        const int2 v = c0 + u < e ? ps[c0 + u] : make_int2(0, 0);
+        pp[u] = v.x;
+        so[u] = v.y;
+      }
+      bool in[GA_B];
+      double rs[GA_B];
+      R ph[GA_B], fx[GA_B], fy[GA_B], fz[GA_B];
+#pragma unroll
+      for (int u = 0; u < GA_B; u++) {
+        const int p = pp[u];
+        const int k = p & mx, j = (p >> lx) & my, i = p >> lxy;
+        const double delz = (i * gv.dz + gv.lo[2]) - x.z;
+        const double dely = (j * gv.dy + gv.lo[1]) - x.y;
+        const double delx = (k * gv.dx + gv.lo[0]) - x.x;
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        in[u] = c0 + u < e && ga_in(ga, rsq, cs);   // NaN positions: false
+        rs[u] = rsq;
+        ph[u] = fx[u] = fy[u] = fz[u] = R(0);
+        if (in[u]) {
+          ph[u] = gv.phi[p];
+          fx[u] = gv.gx[p];
+          fy[u] = gv.gy[p];
+          fz[u] = gv.gz[p];
+        } else if (c0 + u < e) {
+          mterm[so[u]] = __builtin_nan("");
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < GA_B; u++) {
+        if (!in[u]) continue;
+        R expfac = ga_expfac<R>(ga, rs[u] / cs, t);
+        if (ga.style == GA_LJISH_LINEAR) {
+          // PairGridAtomLJishLinear::compute (src/pair_gridatom_lj_ish_linear.cpp:
+          // 93-160): e = -1 * bump * eps, mu += e, force weight e dv, no energy
+          expfac = -expfac;
+          mterm[so[u]] = expfac;
+          expfac *= (R)gv.dv;
+        } else {
+          const R delphi = (ph[u] - phit);
+          mterm[so[u]] = 2 * delphi * expfac;
+          if (energy) esum += (double)delphi * (double)delphi * (double)expfac * gv.dv;
+          expfac *= 2 * delphi * (R)gv.dv;
+        }
+        acc.sub(fx[u] * expfac, fy[u] * expfac, fz[u] * expfac);
+        any = true;
+      }
+    }
+    if (any) gF[a] = acc.get();
+  }
+  if (energy && esum != 0.0) atomicAdd(energy, esum);
+}
+
+// The step's grid-atom pass, grid side: mu(g) (first: from zero, else added
+// to) += the mu terms of its pairs, held in its segment in the reference's
+// list order, skipping the pairs out of the cut (NaN).  A wave takes
+// 64 * GM_U consecutive points, GM_U per lane 64 apart, and issues the loads
+// of its points (offsets, then up to four terms each) before using them.
+template <class R>
+__global__ __launch_bounds__(256) void k_ga_list_mu(size_t N, R *__restrict__ mu, const int *__restrict__ poff,
+                                                    const R *__restrict__ mterm, int first) {
+  const size_t wbase = ((size_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * (64 * GM_U);
+  const int lane = threadIdx.x & 63;
+  int s[GM_U], n[GM_U];
+  R m[GM_U];
+#pragma unroll
+  for (int u = 0; u < GM_U; u++) {
+    const size_t p = wbase + lane + 64 * u;
+    s[u] = p <= N ? poff[p] : 0;   // poff[N] is the total
+    m[u] = (first || p >= N) ? R(0) : mu[p];
+  }
+  // a point's end is the next point's start: the next lane's, or for lane
+  // 63 the next row's lane 0 (one more load after the last row)
+  const size_t plast = wbase + 64 * GM_U;
+  const int tail = (lane == 63 && plast <= N) ? poff[plast] : 0;
+#pragma unroll
+  for (int u = 0; u < GM_U; u++) {
+    const int nxt = __shfl(s[u], (lane + 1) & 63, 64);
+    const int row0 = u + 1 < GM_U ? __shfl(s[u + 1 < GM_U ? u + 1 : u], 0, 64) : tail;
+    const size_t p = wbase + lane + 64 * u;
+    n[u] = p < N ? (lane == 63 ? row0 : nxt) - s[u] : 0;
+  }
+  R v0[GM_U];
+#pragma unroll
+  for (int u = 0; u < GM_U; u++) v0[u] = n[u] > 0 ? mterm[s[u]] : R(0);
+#pragma unroll
+  for (int u = 0; u < GM_U; u++) {
+    const size_t p = wbase + lane + 64 * u;
+    if (p >= N) continue;
+    MuAcc<R> acc;
+    acc.m = m[u];
+    if (n[u] > 0 && !isnan(v0[u])) acc.add(v0[u]);
+    for (int c = s[u] + 1; c < s[u] + n[u]; c++) {
+      const R x = mterm[c];
+      if (!isnan(x)) acc.add(x);
+    }
+    mu[p] = acc.get();
   }
 }
 
@@ -1125,17 +1208,30 @@ void launch_bin_sort(int nbins, const int *binstart, int *binatoms, const int *l
   hipLaunchKernelGGL(k_bin_sort, dim3(nblk(nbins)), dim3(TPB), 0, s, nbins, binstart, binatoms, label);
   HIPCHK(hipGetLastError());
 }
+template <class OffT>
+void launch_grid_list_t(bool fill, const GridView &gv, const BinGeom &b, const int *stencil, int nst,
+                        const int *binstart, const int *binatoms, const double4 *gx, double cutneighsq, int nowned,
+                        int *cnt, const OffT *off, int *neigh, int *overflow, hipStream_t s) {
+  const size_t N = (size_t)gv.nx * gv.ny * gv.nz;
+  if (fill)
+    hipLaunchKernelGGL((k_grid_list<true, OffT>), dim3(nblk(N)), dim3(TPB), 0, s, gv, b, stencil, nst, binstart,
+                       binatoms, gx, cutneighsq, nowned, cnt, off, neigh, overflow);
+  else
+    hipLaunchKernelGGL((k_grid_list<false, OffT>), dim3(nblk(N)), dim3(TPB), 0, s, gv, b, stencil, nst, binstart,
+                       binatoms, gx, cutneighsq, nowned, cnt, off, neigh, overflow);
+  HIPCHK(hipGetLastError());
+}
 void launch_grid_list(bool fill, const GridView &gv, const BinGeom &b, const int *stencil, int nst,
                       const int *binstart, const int *binatoms, const double4 *gx, double cutneighsq,
                       int nowned, int *cnt, const int64_t *off, int *neigh, int *overflow, hipStream_t s) {
-  const size_t N = (size_t)gv.nx * gv.ny * gv.nz;
-  if (fill)
-    hipLaunchKernelGGL((k_grid_list<true>), dim3(nblk(N)), dim3(TPB), 0, s, gv, b, stencil, nst, binstart,
-                       binatoms, gx, cutneighsq, nowned, cnt, off, neigh, overflow);
-  else
-    hipLaunchKernelGGL((k_grid_list<false>), dim3(nblk(N)), dim3(TPB), 0, s, gv, b, stencil, nst, binstart,
-                       binatoms, gx, cutneighsq, nowned, cnt, off, neigh, overflow);
-  HIPCHK(hipGetLastError());
+  launch_grid_list_t(fill, gv, b, stencil, nst, binstart, binatoms, gx, cutneighsq, nowned, cnt, off, neigh, overflow,
+                     s);
+}
+void launch_grid_list(bool fill, const GridView &gv, const BinGeom &b, const int *stencil, int nst,
+                      const int *binstart, const int *binatoms, const double4 *gx, double cutneighsq,
+                      int nowned, int *cnt, const int *off, int *neigh, int *overflow, hipStream_t s) {
+  launch_grid_list_t(fill, gv, b, stencil, nst, binstart, binatoms, gx, cutneighsq, nowned, cnt, off, neigh, overflow,
+                     s);
 }
 void launch_local_flag(int ng, const int *label, int *flag, hipStream_t s) {
   if (ng <= 0) return;
@@ -1247,49 +1343,50 @@ void launch_ga_density(int ng, const GridView &gv, const GAParams &ga, const dou
   hipLaunchKernelGGL(k_fixed_to_float, dim3(nblk(N)), dim3(TPB), 0, s, N, acc, 1.0 / scale, rho);
   HIPCHK(hipGetLastError());
 }
-void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
-                     const double *cutsq, int *acnt, hipStream_t s) {
+void launch_ga_am_walk(int ng, const GridView &gv, const double4 *gxb, const int *order, double reachsq, int *cnt,
+                       const int *off, int2 *ps, int *amq, hipStream_t s) {
   if (ng <= 0) return;
-  hipLaunchKernelGGL(k_ga_count, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, order, cutsq, acnt);
+  hipLaunchKernelGGL(k_ga_am_walk, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, gxb, order, reachsq, cnt, off, ps, amq);
   HIPCHK(hipGetLastError());
 }
-void launch_ga_enum(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
-                    const int *aoff, int64_t cap, int *pg, int *pq, int *overflow, hipStream_t s) {
-  if (ng <= 0) return;
-  hipLaunchKernelGGL(k_ga_enum, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, order, aoff, cap, pg, pq,
-                     overflow);
+void launch_ga_pm_count(int nent, const int2 *ps, int *pcnt, int *rank, hipStream_t s) {
+  if (nent <= 0) return;
+  hipLaunchKernelGGL(k_ga_pm_count, dim3(nblk(nent)), dim3(TPB), 0, s, nent, ps, pcnt, rank);
   HIPCHK(hipGetLastError());
 }
-#define PHAFD_GA_DEF(RT, V4)                                                                                    \
-  void launch_ga_pair(int64_t cap, const int *npairs, const GridViewT<RT> &gv, const GAParams &ga,             \
-                      const double4 *gx, const int *order, const int *pg, const int *pq, V4 *ft, RT *pv,      \
-                      int *ppos, int *gcnt, bool emit, double *energy, hipStream_t s) {                       \
-    hipLaunchKernelGGL(k_ga_pair<RT>, dim3(nblk((size_t)cap)), dim3(TPB), 0, s, npairs, gv, ga, gx, order, pg, pq, \
-                       ft, pv, ppos, gcnt, emit, energy);                                                    \
-    HIPCHK(hipGetLastError());                                                                                \
-  }                                                                                                           \
-  void launch_ga_fold(int ng, const int *order, const int *aoff, const V4 *ft, double4 *gF, hipStream_t s) {  \
+void launch_ga_pm_place(int nent, int2 *ps, const int *rank, const int *amq, const int *poff, int *key, int *ent,
+                        size_t N, hipStream_t s) {
+  if (nent <= 0) return;
+  hipLaunchKernelGGL(k_ga_pm_place, dim3(nblk(nent)), dim3(TPB), 0, s, nent, ps, rank, amq, poff, key, ent);
+  HIPCHK(hipGetLastError());
+  const size_t per_block = 4 * 64 * GM_U;
+  hipLaunchKernelGGL(k_ga_pm_sort, dim3((unsigned)((N + per_block - 1) / per_block)), dim3(256), 0, s, N, poff, key,
+                     ent, ps);
+  HIPCHK(hipGetLastError());
+}
+static int ilog2(int v) {
+  int l = 0;
+  while ((1 << l) < v) l++;
+  if ((1 << l) != v) throw std::invalid_argument("grid sides must be powers of two");
+  return l;
+}
+#define PHAFD_GA_DEF(RT)                                                                                        \
+  void launch_ga_force(int ng, const GridViewT<RT> &gv, const GAParams &ga, const double4 *gx, const int *order, \
+                       const int *aoff, const int2 *ps, RT *mterm, double4 *gF, double *energy, hipStream_t s) { \
     if (ng <= 0) return;                                                                                      \
-    const V4 *f = ft;                                                                                         \
-    hipLaunchKernelGGL(k_ga_fold, dim3(nblk(ng)), dim3(TPB), 0, s, ng, order, aoff, f, gF);                   \
+    const int lx = ilog2(gv.nx), lxy = lx + ilog2(gv.ny);                                                     \
+    hipLaunchKernelGGL(k_ga_force<RT>, dim3(nblk(ng)), dim3(TPB), 0, s, ng, gv, ga, gx, order, aoff, ps,      \
+                       mterm, gF, energy, lx, lxy);                                                           \
     HIPCHK(hipGetLastError());                                                                                \
   }                                                                                                           \
-  void launch_ga_place(int64_t cap, const int *npairs, const int *pg, const int *ppos, const int *pq,         \
-                       const RT *pv, const int *goff, int *skey, RT *sv, hipStream_t s) {                     \
-    hipLaunchKernelGGL(k_ga_place, dim3(nblk((size_t)cap)), dim3(TPB), 0, s, npairs, pg, ppos, pq, goff, skey); \
-    HIPCHK(hipGetLastError());                                                                                \
-    hipLaunchKernelGGL(k_ga_order<RT>, dim3(nblk((size_t)cap)), dim3(TPB), 0, s, npairs, pg, pq, pv, goff, skey, sv); \
-    HIPCHK(hipGetLastError());                                                                                \
-  }                                                                                                           \
-  void launch_ga_sum(size_t N, const int *goff, const int *skey, const RT *sv, RT *mu, bool first,           \
-                     hipStream_t s) {                                                                         \
-    (void)skey;                                                                                               \
-    if (N % 4) throw std::invalid_argument("grid size must be a multiple of 4");                              \
-    hipLaunchKernelGGL(k_ga_sum<RT>, dim3(nblk(N / 4)), dim3(TPB), 0, s, N, goff, sv, mu, first);            \
+  void launch_ga_list_mu(size_t N, RT *mu, const int *poff, const RT *mterm, bool first, hipStream_t s) {     \
+    const size_t per_block = 4 * 64 * GM_U;                                                                   \
+    hipLaunchKernelGGL(k_ga_list_mu<RT>, dim3((unsigned)((N + per_block - 1) / per_block)), dim3(256), 0, s, N, \
+                       mu, poff, mterm, first ? 1 : 0);                                                       \
     HIPCHK(hipGetLastError());                                                                                \
   }
-PHAFD_GA_DEF(double, double4)
-PHAFD_GA_DEF(float, float4)
+PHAFD_GA_DEF(double)
+PHAFD_GA_DEF(float)
 #undef PHAFD_GA_DEF
 void launch_lj_force(int ng, const LJParams &lj, const double4 *gx, const int *rowof, const int *hoff,
                      const int *hneigh, const int *toff, const int *tpair, const int *ti, int nowned,

@@ -29,8 +29,10 @@ struct GridViewT {
 using GridView = GridViewT<double>;
 
 constexpr int GA_LJISH_LINEAR = 3;   // gridatom/LJish/linear (the public PHAFD_PAIR_LJISH_LINEAR)
+constexpr int GA_MAXTYPES = 64;      // atom types of a grid-atom style
 struct GAParams {
   int style;        // 0 gaussian, 1 LJish, 3 LJish/linear
+  int ntypes;       // <= GA_MAXTYPES
   const double *eps, *phit, *cutsq;
   double em05, e05;   // exp(-0.5), exp(0.5) evaluated on the host
 };
@@ -87,6 +89,10 @@ void launch_bin_sort(int nbins, const int *binstart, int *binatoms, const int *l
 void launch_grid_list(bool fill, const GridView &gv, const BinGeom &b, const int *stencil, int nst,
                       const int *binstart, const int *binatoms, const double4 *gx, double cutneighsq,
                       int nowned, int *cnt, const int64_t *off, int *neigh, int *overflow, hipStream_t s);
+// the same with 32-bit offsets (the pruned grid-atom list)
+void launch_grid_list(bool fill, const GridView &gv, const BinGeom &b, const int *stencil, int nst,
+                      const int *binstart, const int *binatoms, const double4 *gx, double cutneighsq,
+                      int nowned, int *cnt, const int *off, int *neigh, int *overflow, hipStream_t s);
 void launch_local_flag(int ng, const int *label, int *flag, hipStream_t s);
 void launch_local_scatter(int ng, const int *flag, const int *pos, int *ilist, int *rowof, hipStream_t s);
 void launch_half_list(bool fill, int nrows, const int *ilist, const int *a2b, const int *binstart,
@@ -99,37 +105,36 @@ void launch_t_place(int nrows, const int *ilist, const int *off, const int *neig
 void launch_t_sort(int ng, const int *toff, int *tpair, int *ti, hipStream_t s);
 void launch_ga_deposit(const GridView &gv, const GAParams &ga, const int *cnt, const int64_t *off,
                        const int *neigh, const double4 *gx, int nowned, double *energy, hipStream_t s);
-// grid-atom pass; order = bin-sorted atom array, npairs = device pointer to
-// the number of pairs (the last entry of the scanned counts), cap = capacity
-// of the pair arrays
-void launch_ga_count(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
-                     const double *cutsq, int *acnt, hipStream_t s);
 // bead density (surrogate input): rho = sum of the kernel value e of every
 // in-cut pair, accumulated in acc[N] (zeroed by the caller) as integers
 // e * scale and converted to FP32 (order-independent, so bit-reproducible)
 void launch_ga_density(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, double scale,
                        unsigned long long *acc, size_t N, float *rho, hipStream_t s);
-void launch_ga_enum(int ng, const GridView &gv, const GAParams &ga, const double4 *gx, const int *order,
-                    const int *aoff, int64_t cap, int *pg, int *pq, int *overflow, hipStream_t s);
-// launch_ga_enum / _pair / _fold: the atom side of the pass (pairs of each
-// atom in walk order at aoff[q]; per pair the mu record and the force term;
-// the force terms folded into gF in order); the FP32 form (float fields,
-// float4 terms) folds forces and sums mu with Kahan compensation, energies
-// stay FP64.  launch_ga_place puts
-// each term's key into its grid point's segment, then its value at its rank
-// in key order (sv); launch_ga_sum adds each segment in order (first: mu is
-// written, summed from 0, instead of added to)
-#define PHAFD_GA_DECL(RT, V4)                                                                                   \
-  void launch_ga_pair(int64_t cap, const int *npairs, const GridViewT<RT> &gv, const GAParams &ga,             \
-                      const double4 *gx, const int *order, const int *pg, const int *pq, V4 *ft, RT *pv,      \
-                      int *ppos, int *gcnt, bool emit, double *energy, hipStream_t s);                        \
-  void launch_ga_fold(int ng, const int *order, const int *aoff, const V4 *ft, double4 *gF, hipStream_t s);   \
-  void launch_ga_place(int64_t cap, const int *npairs, const int *pg, const int *ppos, const int *pq,         \
-                       const RT *pv, const int *goff, int *skey, RT *sv, hipStream_t s);                      \
-  void launch_ga_sum(size_t N, const int *goff, const int *skey, const RT *sv, RT *mu, bool first,           \
-                     hipStream_t s);
-PHAFD_GA_DECL(double, double4)
-PHAFD_GA_DECL(float, float4)
+// The grid-atom pairs of a build (from the build positions gxb; order =
+// the bin-sorted atom array): launch_ga_am_walk with off == null counts
+// each atom's grid points within sqrt(reachsq) into cnt[q], with off (the
+// scanned counts) writes them to ps[].x in ascending grid index, with their
+// atom's q in amq (the am list); launch_ga_pm_count / _place group the am
+// entries by grid point in ascending q (pcnt zeroed by the caller, poff its
+// scan; key, ent scratch): ps[entry].y = its slot in the point's segment.
+void launch_ga_am_walk(int ng, const GridView &gv, const double4 *gxb, const int *order, double reachsq, int *cnt,
+                       const int *off, int2 *ps, int *amq, hipStream_t s);
+void launch_ga_pm_count(int nent, const int2 *ps, int *pcnt, int *rank, hipStream_t s);
+void launch_ga_pm_place(int nent, int2 *ps, const int *rank, const int *amq, const int *poff, int *key, int *ent,
+                        size_t N, hipStream_t s);
+// One grid-atom pair style (PairGridAtomGaussian / LJish / LJish/linear
+// compute) over the pairs of the last build.  launch_ga_force: gF(a) -= the
+// atom's terms in ascending grid index (FP32 form: Kahan-summed FP32 terms
+// subtracted once); mterm[ps[entry].y] = the pair's mu term, NaN out of the
+// cut; energy (may be null) += the coupling energy.
+// launch_ga_list_mu: mu(g) (first: from zero, else added to) += the terms
+// of its segment in order (ascending q, the reference's order).
+#define PHAFD_GA_DECL(RT)                                                                                       \
+  void launch_ga_force(int ng, const GridViewT<RT> &gv, const GAParams &ga, const double4 *gx, const int *order, \
+                       const int *aoff, const int2 *ps, RT *mterm, double4 *gF, double *energy, hipStream_t s);  \
+  void launch_ga_list_mu(size_t N, RT *mu, const int *poff, const RT *mterm, bool first, hipStream_t s);
+PHAFD_GA_DECL(double)
+PHAFD_GA_DECL(float)
 #undef PHAFD_GA_DECL
 // lj/cut forces of every gathered atom added to gF (zero_init: written from
 // zero, the fill of gF left out when lj/cut is its only writer)

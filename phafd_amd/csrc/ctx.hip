@@ -30,9 +30,7 @@ struct PairDef {
   int style;   // 0 gaussian, 1 LJish, 3 LJish/linear (grid-atom); 2 lj/cut, 4 harmonic/cut (bead pairs)
   double maxcut = 0;
   std::vector<double> eps, phit, cutsq;
-  DBuf<double> d_eps, d_phit, d_cutsq, d_cutsq_bound;
-  int64_t cap_build = -1;   // ncalls of the build the pair capacity was sized at
-  int64_t cap = 0;
+  DBuf<double> d_eps, d_phit, d_cutsq;
   double lj1 = 0, lj2 = 0, ljcutsq = 0;
   double hk = 0, hcut = 0;   // harmonic/cut spring constant and cut
 };
@@ -161,17 +159,20 @@ struct phafd_hip_ctx {
   DBuf<double4> gx_build;             // gathered positions at the last build
   bool grid_built = false;            // grid list materialised for the last build
   bool eager_gridlist = false;        // build (and use) the stored grid list in the step
-  // grid-atom pairs: per-atom counts and offsets (bin order); per pair the
-  // grid point, key (atom position), rank, force term and mu term; per grid
-  // point counts and offsets; the keys and values grouped by grid point
-  DBuf<int> ga_acnt, ga_aoff, ga_pg, ga_pq, ga_ppos, ga_gcnt, ga_goff, ga_skey;
-  DBuf<double4> ga_ft;
-  DBuf<double> ga_pv, ga_sv;
-  DBuf<float4> ga_ft32;
-  DBuf<float> ga_pv32, ga_sv32;
   DBuf<int> a2b, bincount, binstart, bincursor, binatoms;
   DBuf<int> g_cnt, g_neigh;
   DBuf<int64_t> g_off;
+  // The grid-atom pairs of the last build: every (atom, grid point) within
+  // (largest grid-atom cut + skin/2) at the build, per atom in bin order
+  // (ga_aoff / ga_ps.x, ascending grid index) and grouped by grid point in
+  // the reference's list order, ascending q (ga_poff; ga_ps.y: the
+  // atom-major entry's slot); ga_mterm: each slot's mu term of the current
+  // pass (NaN out of the cut)
+  DBuf<int> ga_acnt, ga_aoff, ga_amq, ga_rank, ga_pcnt, ga_poff, ga_key, ga_ent;
+  DBuf<int2> ga_ps;
+  DBuf<double> ga_mterm;
+  DBuf<float> ga_mterm32;
+  int64_t ga_nent = 0, ga_calls = -1;   // entries; the build (ncalls) they belong to
   int64_t g_nent = 0;
   DBuf<int> h_ilist, h_rowof, h_cnt, h_off, h_neigh;
   int h_nrows = 0, h_nent = 0;
@@ -584,7 +585,7 @@ void build_gridlist(phafd_hip_ctx *c) {
   c->g_off.alloc(c->N + 1);
   probe_begin(K_GRID_COUNT, c->stream);
   launch_grid_list(false, gv, c->bg, c->d_stencil_full.p, (int)c->stencil_full.size(), c->binstart.p,
-                   c->binatoms.p, gxb, cutneighsq, c->nowned, c->g_cnt.p, nullptr, nullptr,
+                   c->binatoms.p, gxb, cutneighsq, c->nowned, c->g_cnt.p, (const int64_t *)nullptr, nullptr,
                    c->flags.p + 1, c->stream);
   probe_end(K_GRID_COUNT, c->stream, 4.0 * c->N);
   scan_exclusive_i32_to_i64(c->g_cnt.p, c->g_off.p, c->N, c->stream);
@@ -647,6 +648,50 @@ void build_halflist(phafd_hip_ctx *c, bool full) {
 }
 
 // Neighbor::build, src/neighbor.cpp:99-125
+// the largest grid-atom cut + skin/2, with a margin far above rounding
+double ga_reach_of(const phafd_hip_ctx *c) {
+  double r = 0;
+  for (auto &p : c->pairs) {
+    if (bead_pair(*p)) continue;
+    for (double cs : p->cutsq) r = std::max(r, std::sqrt(cs));
+  }
+  return r > 0 ? (r + 0.5 * c->skin) * (1 + 1e-9) + 1e-9 : 0.0;
+}
+// the grid-atom pairs of the build: the atom-major walk (count, scan,
+// fill), then the grouping by grid point (count with ranks, scan, place,
+// order by q)
+void build_ga_pairs(phafd_hip_ctx *c) {
+  const double reach = ga_reach_of(c);
+  if (reach <= 0) return;
+  const int ng = c->ngathered;
+  GridView gv = c->gridview<double>();
+  c->ga_acnt.alloc((size_t)ng + 1);
+  c->ga_aoff.alloc((size_t)ng + 2);
+  launch_ga_am_walk(ng, gv, c->gx_build.p, c->binatoms.p, reach * reach, c->ga_acnt.p, nullptr, nullptr, nullptr,
+                    c->stream);
+  scan_exclusive_i32(c->ga_acnt.p, c->ga_aoff.p, (size_t)ng, c->stream);
+  const int nent = read_int(c, c->ga_aoff.p + ng);
+  if (nent < 0) throw std::runtime_error("grid-atom pairs of a build exceed 2^31");
+  c->ga_nent = nent;
+  c->ga_ps.alloc((size_t)nent + 1);
+  c->ga_amq.alloc((size_t)nent + 1);
+  launch_ga_am_walk(ng, gv, c->gx_build.p, c->binatoms.p, reach * reach, nullptr, c->ga_aoff.p, c->ga_ps.p,
+                    c->ga_amq.p, c->stream);
+  c->ga_pcnt.alloc(c->N + 1);
+  c->ga_poff.alloc(c->N + 2);
+  c->ga_rank.alloc((size_t)nent + 1);
+  c->ga_key.alloc((size_t)nent + 1);
+  c->ga_ent.alloc((size_t)nent + 1);
+  fill_zero(c->ga_pcnt.p, sizeof(int) * c->N, c->stream);
+  launch_ga_pm_count(nent, c->ga_ps.p, c->ga_pcnt.p, c->ga_rank.p, c->stream);
+  scan_exclusive_i32(c->ga_pcnt.p, c->ga_poff.p, c->N, c->stream);
+  launch_ga_pm_place(nent, c->ga_ps.p, c->ga_rank.p, c->ga_amq.p, c->ga_poff.p, c->ga_key.p, c->ga_ent.p, c->N,
+                     c->stream);
+  if (c->fp32) c->ga_mterm32.alloc((size_t)nent + 1);
+  else c->ga_mterm.alloc((size_t)nent + 1);
+  c->ga_calls = c->ncalls;
+}
+
 void neighbor_build(phafd_hip_ctx *c) {
   c->ncalls++;
   launch_copy4(c->nowned, c->x.p, c->xhold.p, c->stream);
@@ -655,13 +700,14 @@ void neighbor_build(phafd_hip_ctx *c) {
   launch_copy4(c->ngathered, c->gx.p, c->gx_build.p, c->stream);
   c->grid_built = false;
   if (c->need_full && c->eager_gridlist) build_gridlist(c);
+  if (c->need_full) build_ga_pairs(c);
   if (c->need_half) build_halflist(c, c->eager_halflist);
   c->lists_valid = true;
 }
 
 // Device flags: [0] and [4] the rebuild decision (Neighbor::decide) made at
 // the end of a step, slot 4 * (timestep & 1); [1] list overflow; [2] atom
-// outside the bins; [3] grid-atom term capacity; [5] an immediate decide.
+// outside the bins; [5] an immediate decide.
 // A step's decide clears the slot of the step after it, so no fill runs per
 // step; a run loop clears its first step's slot once.
 int decide_slot(const phafd_hip_ctx *c) { return 4 * (int)(c->timestep & 1); }
@@ -678,7 +724,6 @@ void enqueue_flags_readback(phafd_hip_ctx *c) {
 }
 // the checks after step `step`'s synchronise (src/integrate.cpp:190-201)
 void check_step_flags(phafd_hip_ctx *c, int64_t step) {
-  if (c->h_flags[11]) throw std::runtime_error("grid-atom term capacity exceeded");
   if (c->neigh_set) {
     const int f = c->h_flags[8 + 4 * (int)(step & 1)];
     if (f & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
@@ -701,23 +746,17 @@ bool neighbor_decide(phafd_hip_ctx *c) {
 
 void upload_pair(PairDef &p, double skin) {
   int nt = (int)p.eps.size();
-  p.d_eps.alloc(nt); p.d_phit.alloc(nt); p.d_cutsq.alloc(nt); p.d_cutsq_bound.alloc(nt);
+  p.d_eps.alloc(nt); p.d_phit.alloc(nt); p.d_cutsq.alloc(nt);
   HIPCHK(hipMemcpy(p.d_eps.p, p.eps.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p.d_phit.p, p.phit.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p.d_cutsq.p, p.cutsq.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
-  // (cut + skin/2 + margin)^2: holds every pair any step before the next
-  // build can have (no atom moves more than skin/2 in between)
-  std::vector<double> b(nt);
-  for (int t = 0; t < nt; t++) {
-    const double r = std::sqrt(p.cutsq[t]) + 0.5 * skin;
-    b[t] = r * r * (1 + 1e-9) + 1e-12;
-  }
-  HIPCHK(hipMemcpy(p.d_cutsq_bound.p, b.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
 }
 
 GAParams ga_params(const PairDef &p) {
   GAParams g;
   g.style = p.style;
+  g.ntypes = (int)p.eps.size();
+  if (g.ntypes > GA_MAXTYPES) throw std::invalid_argument("grid-atom styles take at most 64 atom types");
   g.eps = p.d_eps.p; g.phit = p.d_phit.p; g.cutsq = p.d_cutsq.p;
   g.em05 = exp(-0.5);
   g.e05 = exp(0.5);
@@ -837,74 +876,36 @@ void phase_neigh(phafd_hip_ctx *c) {
 }
 
 // One grid-atom pair style (PairGridAtomGaussian / PairGridAtomLJish
-// compute): atom-centric force terms in reference order, and the mu terms
-// emitted with their list-order keys, grouped per grid point by a counting
-// sort and summed in key order.  With the eager option the mu side walks the
-// stored list instead (the two are bit-identical).
-// typed grid-atom term buffers (the FP64 and FP32 forms have their own)
-template <class V4> DBuf<V4> &ga_terms(phafd_hip_ctx *c);
-template <> DBuf<double4> &ga_terms<double4>(phafd_hip_ctx *c) { return c->ga_ft; }
-template <> DBuf<float4> &ga_terms<float4>(phafd_hip_ctx *c) { return c->ga_ft32; }
-template <class R> DBuf<R> &ga_vals(phafd_hip_ctx *c, int which);
-template <> DBuf<double> &ga_vals<double>(phafd_hip_ctx *c, int which) { return which ? c->ga_sv : c->ga_pv; }
-template <> DBuf<float> &ga_vals<float>(phafd_hip_ctx *c, int which) { return which ? c->ga_sv32 : c->ga_pv32; }
-
-// first: mu has not been zeroed this step; the pass writes every point
-bool ga_uses_emit(const phafd_hip_ctx *c) { return !(c->eager_gridlist && c->grid_built); }
-
+// compute) over the pairs of the last build: the atom-side walk (forces in
+// ascending grid index, each kept pair's mu term), then the grid-side sum
+// of the terms in the reference's list order (or, with the eager option,
+// the walk of the stored NPairGridBin list; the two are bit-identical).
+// first: mu has not been zeroed this step; the pass writes every point.
+template <class R> DBuf<R> &ga_mterm(phafd_hip_ctx *c);
+template <> DBuf<double> &ga_mterm<double>(phafd_hip_ctx *c) { return c->ga_mterm; }
+template <> DBuf<float> &ga_mterm<float>(phafd_hip_ctx *c) { return c->ga_mterm32; }
 template <class R>
 void grid_atom_pass(phafd_hip_ctx *c, PairDef &p, double *energy, bool first) {
-  using V4 = typename std::conditional<std::is_same<R, float>::value, float4, double4>::type;
   FieldSet<R> &Fs = c->fields<R>();
-  const GridView gv = c->gridview<double>();
   const GridViewT<R> gvr = c->gridview<R>();
   GAParams ga = ga_params(p);
-  const int ng = c->ngathered;
-  const bool emit = ga_uses_emit(c);
-  const int *order = c->binatoms.p;
-  c->ga_acnt.alloc((size_t)ng + 1);
-  c->ga_aoff.alloc((size_t)ng + 2);
-  if (p.cap_build != c->ncalls) {
-    // pair capacity for the steps until the next build: the pairs within
-    // cut + skin/2 of the build positions, counted once (one read-back per build)
-    launch_ga_count(ng, gv, ga, c->gx.p, order, p.d_cutsq_bound.p, c->ga_acnt.p, c->stream);
-    scan_exclusive_i32(c->ga_acnt.p, c->ga_aoff.p, (size_t)ng, c->stream);
-    p.cap = std::max(1, read_int(c, c->ga_aoff.p + ng));
-    p.cap_build = c->ncalls;
-  }
-  const int64_t cap = p.cap;
-  c->ga_pg.alloc(cap); c->ga_pq.alloc(cap);
-  DBuf<V4> &ft = ga_terms<V4>(c);
-  ft.alloc(cap);
+  if (c->ga_calls != c->ncalls) build_ga_pairs(c);
+  DBuf<R> &mt = ga_mterm<R>(c);
+  mt.alloc((size_t)c->ga_nent + 1);
   probe_begin(K_GA_FORCE, c->stream);
-  launch_ga_count(ng, gv, ga, c->gx.p, order, ga.cutsq, c->ga_acnt.p, c->stream);
-  scan_exclusive_i32(c->ga_acnt.p, c->ga_aoff.p, (size_t)ng, c->stream);
-  const int *npairs = c->ga_aoff.p + ng;
-  launch_ga_enum(ng, gv, ga, c->gx.p, order, c->ga_aoff.p, cap, c->ga_pg.p, c->ga_pq.p, c->flags.p + 3, c->stream);
-  if (emit) {
-    c->ga_ppos.alloc(cap); c->ga_skey.alloc(cap);
-    ga_vals<R>(c, 0).alloc(cap); ga_vals<R>(c, 1).alloc(cap);
-    c->ga_gcnt.alloc(c->N + 1);
-    c->ga_goff.alloc(c->N + 2);
-    fill_zero(c->ga_gcnt.p, sizeof(int) * c->N, c->stream);
-  } else {
-    if (first) fill_zero(Fs.mu.p, sizeof(R) * c->N, c->stream);
-  }
-  launch_ga_pair(cap, npairs, gvr, ga, c->gx.p, order, c->ga_pg.p, c->ga_pq.p, ft.p, ga_vals<R>(c, 0).p,
-                 c->ga_ppos.p, c->ga_gcnt.p, emit, energy, c->stream);
-  launch_ga_fold(ng, order, c->ga_aoff.p, ft.p, c->gF.p, c->stream);
-  probe_end(K_GA_FORCE, c->stream, 64.0 * ng);
+  launch_ga_force(c->ngathered, gvr, ga, c->gx.p, c->binatoms.p, c->ga_aoff.p, c->ga_ps.p, mt.p, c->gF.p, energy,
+                  c->stream);
+  probe_end(K_GA_FORCE, c->stream, 64.0 * c->ngathered);
   probe_begin(K_GA_DEPOSIT, c->stream);
-  if (emit) {
-    scan_exclusive_i32(c->ga_gcnt.p, c->ga_goff.p, c->N, c->stream);
-    launch_ga_place(cap, npairs, c->ga_pg.p, c->ga_ppos.p, c->ga_pq.p, ga_vals<R>(c, 0).p, c->ga_goff.p,
-                    c->ga_skey.p, ga_vals<R>(c, 1).p, c->stream);
-    launch_ga_sum(c->N, c->ga_goff.p, c->ga_skey.p, ga_vals<R>(c, 1).p, Fs.mu.p, first, c->stream);
+  if (!(c->eager_gridlist && c->grid_built)) {
+    launch_ga_list_mu(c->N, Fs.mu.p, c->ga_poff.p, mt.p, first, c->stream);
   } else {
     if (c->fp32) throw std::invalid_argument("the stored grid list walk is FP64 only");
-    launch_ga_deposit(gv, ga, c->g_cnt.p, c->g_off.p, c->g_neigh.p, c->gx.p, c->nowned, nullptr, c->stream);
+    if (first) fill_zero(Fs.mu.p, sizeof(R) * c->N, c->stream);
+    launch_ga_deposit(c->gridview<double>(), ga, c->g_cnt.p, c->g_off.p, c->g_neigh.p, c->gx.p, c->nowned, nullptr,
+                      c->stream);
   }
-  probe_end(K_GA_DEPOSIT, c->stream, 16.0 * c->N);
+  probe_end(K_GA_DEPOSIT, c->stream, 2.0 * sizeof(R) * c->N + (first ? 0.0 : sizeof(R) * c->N));
 }
 
 // Integrate::run pair block, src/integrate.cpp:130-146 (one pass, finding F4)
@@ -3157,8 +3158,7 @@ void atom_steps_frozen_field(phafd_hip_ctx *c, const SurrogateCfg &g, int K) {
       s += 1;
       continue;
     }
-    if (c->h_flags[11]) throw std::runtime_error("grid-atom term capacity exceeded");
-    const int stop = c->h_flags[6], f = c->h_flags[7];
+      const int stop = c->h_flags[6], f = c->h_flags[7];
     if (f & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
     const int done = stop > 0 ? stop : W;
     c->timestep -= W - done;

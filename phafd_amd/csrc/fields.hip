@@ -43,7 +43,7 @@ __global__ void k_grad_multiply(const C *__restrict__ ph, C *__restrict__ gx, C 
 // and conj() into their mirror (-ky, -kz); mirror threads do nothing; the
 // eight self-conjugate modes use real_update; the origin is only rescaled
 // (or, with `origin`, set to the FP64 sum there times the normalisation).
-template <class C>
+template <bool DEV, class C>
 __device__ __forceinline__ void update_index(C *__restrict__ ph, const C *__restrict__ mu,
                                              const C *__restrict__ noise, size_t idx, int hx, int ny,
                                              int nz, const UpdateParams &u, const double *origin) {
@@ -54,7 +54,7 @@ __device__ __forceinline__ void update_index(C *__restrict__ ph, const C *__rest
   const int j = (int)(t % nz);     // kz
   const int k = (int)(t / nz);     // kx
   if (k != 0 && k != H) {
-    ph[idx] = mode_update(ph[idx], mu[idx], mode_noise(noise, idx, u, i, j, k, A, B, false), i, j, k, A, B, u);
+    ph[idx] = mode_update(ph[idx], mu[idx], mode_noise<DEV>(noise, idx, u, i, j, k, A, B, false), i, j, k, A, B, u);
     return;
   }
   const bool self = (i == 0 || i == A / 2) && (j == 0 || j == B / 2);
@@ -67,32 +67,33 @@ __device__ __forceinline__ void update_index(C *__restrict__ ph, const C *__rest
         ph[idx] = cpx<C>(v.x * (R)u.normalization, v.y * (R)u.normalization);
       }
     } else {   // real_update
-      ph[idx] = mode_update(ph[idx], mu[idx], mode_noise(noise, idx, u, i, j, k, A, B, true), i, j, k, A, B, u);
+      ph[idx] = mode_update(ph[idx], mu[idx], mode_noise<DEV>(noise, idx, u, i, j, k, A, B, true), i, j, k, A, B, u);
     }
     return;
   }
   const bool primary = (j >= 1 && j < B / 2) || (j == 0 && i >= 1 && i < A / 2) ||
                        (j == B / 2 && i > A / 2);
   if (!primary) return;
-  const C r = mode_update(ph[idx], mu[idx], mode_noise(noise, idx, u, i, j, k, A, B, false), i, j, k, A, B, u);
+  const C r = mode_update(ph[idx], mu[idx], mode_noise<DEV>(noise, idx, u, i, j, k, A, B, false), i, j, k, A, B, u);
   ph[idx] = r;
   const int im = (A - i) % A, jm = (B - j) % B;
   ph[((size_t)k * nz + jm) * ny + im] = cpx<C>(r.x, -r.y);
 }
 
-template <class C>
+template <class C, bool DEV>
 __global__ void k_update(C *__restrict__ ph, const C *__restrict__ mu, const C *__restrict__ noise, int hx,
                          int ny, int nz, UpdateParams u, const double *origin, Slab sl) {
   const size_t M = (size_t)hx * nz * ny;
   for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < M * sl.n;
        w += (size_t)gridDim.x * blockDim.x) {
     const size_t r = w / M, off = r * sl.sc;
-    update_index(ph + off, mu + off, noise ? noise + off : nullptr, w % M, hx, ny, nz, u, origin ? origin + r : nullptr);
+    update_index<DEV>(ph + off, mu + off, noise ? noise + off : nullptr, w % M, hx, ny, nz, u,
+                      origin ? origin + r : nullptr);
   }
 }
 
 // only the planes kx = 0 and kx = Nx/2 (the bulk is fused into the z pass)
-template <class C>
+template <class C, bool DEV>
 __global__ void k_update_edges(C *__restrict__ ph, const C *__restrict__ mu, const C *__restrict__ noise,
                                int hx, int ny, int nz, UpdateParams u, const double *origin, size_t sc) {
   ph += blockIdx.y * sc;
@@ -103,7 +104,7 @@ __global__ void k_update_edges(C *__restrict__ ph, const C *__restrict__ mu, con
   for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < 2 * P;
        w += (size_t)gridDim.x * blockDim.x) {
     const size_t idx = w < P ? w : (size_t)(hx - 1) * P + (w - P);
-    update_index(ph, mu, noise, idx, hx, ny, nz, u, origin);
+    update_index<DEV>(ph, mu, noise, idx, hx, ny, nz, u, origin);
   }
 }
 
@@ -288,16 +289,24 @@ void spectral_update_t(const FFTPlan &p, C *phihat, const C *muhat, const C *noi
                        const double *origin, hipStream_t s, Slab sl) {
   if (sl.n > 1 && sl.sc < p.M()) throw std::invalid_argument("slab stride shorter than a spectrum");
   probe_begin(K_UPDATE, s);
-  hipLaunchKernelGGL(k_update<C>, dim3(grid_for(p.M() * sl.n)), dim3(TPB), 0, s, phihat, muhat, noise_k, p.hx,
-                     p.ny, p.nz, u, origin, sl);
+  if (u.noise_dev)
+    hipLaunchKernelGGL((k_update<C, true>), dim3(grid_for(p.M() * sl.n)), dim3(TPB), 0, s, phihat, muhat, noise_k,
+                       p.hx, p.ny, p.nz, u, origin, sl);
+  else
+    hipLaunchKernelGGL((k_update<C, false>), dim3(grid_for(p.M() * sl.n)), dim3(TPB), 0, s, phihat, muhat, noise_k,
+                       p.hx, p.ny, p.nz, u, origin, sl);
   probe_end(K_UPDATE, s, (noise_k ? 4.0 : 3.0) * sizeof(C) * p.M() * sl.n);
   HIPCHK(hipGetLastError());
 }
 template <class C>
 void spectral_update_edges_t(const FFTPlan &p, C *phihat, const C *muhat, const C *noise_k, UpdateParams u,
                              const double *origin, hipStream_t s, Slab sl) {
-  hipLaunchKernelGGL(k_update_edges<C>, dim3(grid_for(2 * (size_t)p.ny * p.nz), sl.n), dim3(TPB), 0, s, phihat,
-                     muhat, noise_k, p.hx, p.ny, p.nz, u, origin, sl.sc);
+  if (u.noise_dev)
+    hipLaunchKernelGGL((k_update_edges<C, true>), dim3(grid_for(2 * (size_t)p.ny * p.nz), sl.n), dim3(TPB), 0, s,
+                       phihat, muhat, noise_k, p.hx, p.ny, p.nz, u, origin, sl.sc);
+  else
+    hipLaunchKernelGGL((k_update_edges<C, false>), dim3(grid_for(2 * (size_t)p.ny * p.nz), sl.n), dim3(TPB), 0, s,
+                       phihat, muhat, noise_k, p.hx, p.ny, p.nz, u, origin, sl.sc);
   HIPCHK(hipGetLastError());
 }
 template <class R>

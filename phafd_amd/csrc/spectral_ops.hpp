@@ -102,16 +102,18 @@ __device__ __forceinline__ double2 device_noise(const UpdateParams &u, size_t ki
   const double a = u.pref_c * sqrt(q2) * u.sqrtdt;
   return make_double2(a * unit_centered(w[0]), a * unit_centered(w[1]));
 }
-// the noise of one updated mode: a host field, the device generator, or none
-template <class C>
+// the noise of one updated mode: the device generator (DEV, a template
+// argument so that kernels without it carry none of its code), a host field,
+// or none
+template <bool DEV, class C>
 __device__ __forceinline__ C mode_noise(const C *__restrict__ noise, size_t idx, const UpdateParams &u, int i,
                                         int j, int k, int A, int B, bool real) {
   using R = typename C::value_type;
-  if (noise) return noise[idx];
-  if (u.noise_dev) {
+  if (DEV) {
     const double2 v = device_noise(u, idx, i, j, k, A, B, real);
     return cpx<C>((R)v.x, (R)v.y);
   }
+  if (noise) return noise[idx];
   return cpx<C>(R(0), R(0));
 }
 

@@ -64,10 +64,13 @@ def apply(sysobj, w, is_oracle, setup=True):
             s.pair_harmoniccut(*w["harmonic"])
     s.fix_conjugate(5678, 1.0, 1.0, 1.0, 1.0, w["noise"])
     s.fix_floryhuggins(1.0, 1.0, w["chi"])
+    # chain_range / drag_range: the chains (index range) each fix acts on
     if w["chain"]:
-        s.fix_chain(w["starts"], w["ends"], 1.0, 100.0, 10.0, 1.0)
+        a, b = w.get("chain_range", (0, len(w["starts"])))
+        s.fix_chain(w["starts"][a:b], w["ends"][a:b], 1.0, 100.0, 10.0, 1.0)
     if w["drag"]:
-        s.fix_drag(w["starts"], w["ends"], 1.0)
+        a, b = w.get("drag_range", (0, len(w["starts"])))
+        s.fix_drag(w["starts"][a:b], w["ends"][a:b], 1.0)
     s.neighbor(w["skin"])
     if setup:
         s.setup(w["dt"])

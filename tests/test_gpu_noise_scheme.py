@@ -214,7 +214,7 @@ def test_single_mode_closed_form(semi, dt, K):
 
 @pytest.mark.parametrize("zghost", [0, 1])
 def test_semi_implicit_workload_vs_oracle(zghost):
-    w = workload(n=32, nchains=8, nbeads=25, seed=13, zghost=zghost)
+    w = workload(n=64, nchains=20, nbeads=50, seed=42, zghost=zghost, chi=3.0)
     g = make_gpu(w)
     o = make_oracle(w)
     g.set_option(P.OPT_SEMI_IMPLICIT, 1)
@@ -246,8 +246,11 @@ def test_padded_rows_round_trip():
 @pytest.mark.parametrize("zghost", [0, 1])
 @pytest.mark.parametrize("chain", [False, True])
 def test_drag_trajectory_200_steps(zghost, chain):
-    # fix atom/drag (src/fixatom_drag.cpp:43-66), alone and beside the chain fix
+    # fix atom/drag (src/fixatom_drag.cpp:43-66) on every chain, or on half
+    # of them beside the chain fix on the other half
     w = workload(n=64, nchains=20, nbeads=50, seed=42, zghost=zghost, chi=3.0, drag=True, chain=chain)
+    if chain:
+        w["drag_range"], w["chain_range"] = (0, 10), (10, 20)
     g = make_gpu(w)
     o = make_oracle(w)
     g.run(200)

@@ -352,8 +352,8 @@ def test_pruned_lj_list_matches_full_list_bitwise():
 @pytest.mark.parametrize("ljish", [False, True, "linear"])
 def test_dense_tile_mu_terms(ljish):
     # 400 atoms packed into a 5^3 corner (across the periodic boundary) with a
-    # long grid-atom cut: some grid tiles hold more mu terms than the ordered
-    # sum keeps in LDS and take the in-memory selection instead
+    # long grid-atom cut: the tiles of the mu deposit there stage more atoms
+    # than fit in LDS at once and take them in several rounds
     w = workload(n=32, nchains=40, nbeads=10, seed=21, lj=False, chain=False, gauss_cut=3.0,
                  nucwidth=3.0, ljish=ljish, zghost=1)
     rng = np.random.default_rng(5)
