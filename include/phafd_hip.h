@@ -295,9 +295,17 @@ int phafd_hip_timers(phafd_hip_ctx *ctx, double *ms8, int reset);
  *    gradient term implicitly, phihat <- (phihat - c muhat) / (1 + c s) / N
  *    + xi with c = M q^2 dt, s = T/v gamma q^2 (the semi-implicit form kept
  *    in a comment at src/conjugate.cpp:356); 0 is the reference's explicit
- *    update (src/conjugate_volfrac.cpp:89-133), the drop-in. */
+ *    update (src/conjugate_volfrac.cpp:89-133), the drop-in.
+ *  PHAFD_OPT_NEXT_GRADIENT [1]: inside phafd_hip_run, a step that another
+ *    step follows also makes the next step's grad phi from its updated
+ *    spectrum (the next r2c of phi is N times that spectrum in exact
+ *    arithmetic, src/fixgrid_conjugate.cpp:62-154), so the next
+ *    start_of_step transforms nothing; results agree with 0 to rounding.
+ *    The last step of a run (and every step of run_until_touching) leaves
+ *    the fields as the reference does. */
 enum { PHAFD_OPT_EAGER_GRIDLIST = 0, PHAFD_OPT_FUSE_FH = 1, PHAFD_OPT_FUSE_SPECTRAL = 2,
-       PHAFD_OPT_EAGER_HALFLIST = 3, PHAFD_OPT_ATOM_WINDOWS = 4, PHAFD_OPT_SEMI_IMPLICIT = 5 };
+       PHAFD_OPT_EAGER_HALFLIST = 3, PHAFD_OPT_ATOM_WINDOWS = 4, PHAFD_OPT_SEMI_IMPLICIT = 5,
+       PHAFD_OPT_NEXT_GRADIENT = 6 };
 int phafd_hip_set_option(phafd_hip_ctx *ctx, int key, double value);
 /* HIP-event timing of every launch of one kernel during run():
  * kernel_id >= 0 (re)starts the probe for that kernel (ids in the order
@@ -318,7 +326,8 @@ int phafd_hip_probe(phafd_hip_ctx *ctx, int kernel_id, double *out3);
  * fix (chi may differ per member).  The grid side of a step runs as one
  * launch per pass over all members, captured in hipGraphs; the atom side per
  * member on a few streams.  Each member's results equal, bit for bit, those
- * of the same context run alone.  setup() also autotunes the members per
+ * of the same context run alone with PHAFD_OPT_NEXT_GRADIENT 0 (the batch
+ * takes every step's gradient from phi).  setup() also autotunes the members per
  * batched launch and the number of member streams (choices that leave the
  * arithmetic unchanged). */
 typedef struct phafd_hip_batch phafd_hip_batch;

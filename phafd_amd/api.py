@@ -20,7 +20,7 @@ NOISE_OFF, NOISE_DEVICE, NOISE_FIELD = 0, 1, 2
 PAIR_GAUSSIAN, PAIR_LJISH, PAIR_LJISH_LINEAR = 0, 1, 3
 FP64, FP32 = 0, 1
 (OPT_EAGER_GRIDLIST, OPT_FUSE_FH, OPT_FUSE_SPECTRAL, OPT_EAGER_HALFLIST, OPT_ATOM_WINDOWS,
- OPT_SEMI_IMPLICIT) = 0, 1, 2, 3, 4, 5
+ OPT_SEMI_IMPLICIT, OPT_NEXT_GRADIENT) = 0, 1, 2, 3, 4, 5, 6
 PH_START, PH_NEIGH, PH_PAIR, PH_INITIAL, PH_POSTFORCE, PH_PREFINAL, PH_FINAL, PH_POSTFINAL = range(8)
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "NaN in phi", -4: "runtime error", -5: "list overflow"}
 

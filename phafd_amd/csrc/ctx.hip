@@ -203,6 +203,10 @@ struct phafd_hip_ctx {
   double tacc[8] = {0};
   bool fuse_fh = true;
   bool fuse_spectral = true;
+  // PHAFD_OPT_NEXT_GRADIENT: a step that another step of the same run
+  // follows leaves phi, grad phi and phihat as the next start_of_step makes
+  // them (grad_ready), and that start_of_step then has nothing to transform
+  bool next_grad = true, grad_ready = false;
   // surrogate blocks: bead densities at both ends, network input and
   // output, the predicted phi, mu, the base, FP64 partial sums, frozen
   // grid-atom forces, the atoms' block-start state
@@ -852,9 +856,16 @@ void require_noise_fields(const phafd_hip_ctx *c, int64_t nsteps) {
 template <class R>
 void phase_start_t(phafd_hip_ctx *c) {
   FieldSet<R> &F = c->fields<R>();
+  const bool ready = c->grad_ready;
+  c->grad_ready = false;
   for (auto &fp : c->fixes) {
     FixDef &f = *fp;
-    if (f.kind == FixDef::CONJ) compute_gradphi<R>(c, F.phihat.p, true);
+    if (f.kind == FixDef::CONJ && ready) {
+      if (c->fp32) {   // the k = 0 mode of the FP32 update: the FP64 sum of phi
+        fill_zero(c->mass64.p, sizeof(double), c->stream);
+        sum_fp64(c->N, F.phi.p, c->mass64.p, c->stream);
+      }
+    } else if (f.kind == FixDef::CONJ) compute_gradphi<R>(c, F.phihat.p, true);
     else if (f.kind == FixDef::GRADPHI && (!f.once || c->timestep == c->firststep))
       compute_gradphi<R>(c, F.muhat.p, false);
   }
@@ -1025,11 +1036,15 @@ void phase_prefinal_split(phafd_hip_ctx *c, const FHPreT<R> *pre) {
   fft_r2c_xy(c->plan, F.mu.p, F.muhat.p, c->stream, pre);
 }
 template <class R>
-void phase_final_split(phafd_hip_ctx *c, bool atoms = true) {
+void phase_final_split(phafd_hip_ctx *c, bool atoms = true, bool next = false) {
   FieldSet<R> &F = c->fields<R>();
   for (auto &fp : c->fixes) {
     FixDef &f = *fp;
-    if (f.kind == FixDef::CONJ) {
+    if (f.kind == FixDef::CONJ && next) {
+      const auto *nz = take_noise<R>(c, f);
+      fft_z_update_next(c->plan, F.muhat.p, F.phihat.p, F.ghat[0].p, F.ghat[2].p, nz, update_params(c, f),
+                        origin_ptr(c), spec_const(c), c->stream, c->store_muhat);
+    } else if (f.kind == FixDef::CONJ) {
       const auto *nz = take_noise<R>(c, f);
       fft_z_update(c->plan, F.muhat.p, F.phihat.p, nz, update_params(c, f), origin_ptr(c), c->stream, Slab(),
                    c->store_muhat);
@@ -1039,8 +1054,14 @@ void phase_final_split(phafd_hip_ctx *c, bool atoms = true) {
   }
 }
 template <class R>
-void phase_postfinal_split(phafd_hip_ctx *c) {
+void phase_postfinal_split(phafd_hip_ctx *c, bool next = false) {
   FieldSet<R> &F = c->fields<R>();
+  if (next) {
+    fft_c2r_next(c->plan, F.ghat[0].p, F.ghat[1].p, F.ghat[2].p, F.phi.p, F.grad[0].p, F.grad[1].p, F.grad[2].p,
+                 spec_const(c), c->stream);
+    c->grad_ready = true;
+    return;
+  }
   auto *in = F.phihat.p;
   R *out = F.phi.p;
   fft_c2r_yx_batch(c->plan, &in, &out, 1, 0.0, c->stream);
@@ -1143,6 +1164,12 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
   int nconj = 0;
   for (auto &f : c->fixes) nconj += f->kind == FixDef::CONJ;
   const bool zsplit = nconj == 1 && spectral_fused(c);
+  int ngrad = 0;
+  for (auto &f : c->fixes) ngrad += f->kind == FixDef::GRADPHI;
+  // a run that may stop after any step (run_until_touching) keeps every
+  // step's fields as the reference leaves them
+  const bool next_ok = zsplit && c->next_grad && !c->touch && ngrad == 0 && fft_sep_grad_ok(c->plan);
+  c->grad_ready = false;
   FixDef *fc = fused_chain_fix(c);
   if (c->neigh_set) clear_decide_slot(c, c->timestep + 1);
   bool started = false;   // this step's start_of_step went in before the last read-back
@@ -1167,10 +1194,11 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
     const bool capture = c->fourier_capture && it == nsteps - 1;
     if (capture) capture_spectrum(c, 0, c->fields<R>().phihat.p);   // in_fourier: ft_phi before the update
-    if (zsplit) phase_final_split<R>(c, !fc);
+    const bool next = next_ok && it + 1 < nsteps;
+    if (zsplit) phase_final_split<R>(c, !fc, next);
     else phase_final_t<R>(c, !fc);
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
-    if (zsplit) phase_postfinal_split<R>(c);
+    if (zsplit) phase_postfinal_split<R>(c, next);
     else phase_postfinal_t<R>(c);
     if (capture) {   // ft_chempot: the last step of a run stores the complete muhat
       capture_spectrum(c, 1, c->fields<R>().muhat.p);
@@ -1217,11 +1245,15 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
     R phi0;
     std::memcpy(&phi0, c->h_flags + 2, sizeof(R));
     if (std::isnan(phi0) && started) c->timestep = step;   // the next step did not happen
-    if (std::isnan(phi0)) throw std::runtime_error("NAN encountered in phi.");
+    if (std::isnan(phi0)) {
+      c->grad_ready = false;
+      throw std::runtime_error("NAN encountered in phi.");
+    }
     try {
       check_step_flags(c, step);
     } catch (...) {
       if (started) c->timestep = step;
+      c->grad_ready = false;
       throw;
     }
     if (c->touch && touch_check(c, step, it + 1)) break;
@@ -2387,6 +2419,7 @@ int phafd_hip_set_option(phafd_hip_ctx *c, int key, double value) {
       case PHAFD_OPT_EAGER_HALFLIST: c->eager_halflist = value != 0; break;
       case PHAFD_OPT_ATOM_WINDOWS: c->atom_windows = value != 0; break;
       case PHAFD_OPT_SEMI_IMPLICIT: c->semi_implicit = value != 0; break;
+      case PHAFD_OPT_NEXT_GRADIENT: c->next_grad = value != 0; break;
       default: throw std::invalid_argument("unknown option");
     }
     return 0;

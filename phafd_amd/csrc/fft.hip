@@ -153,6 +153,14 @@ __host__ __device__ __forceinline__ C mk2(R x, R y) { return make_float2(x, y); 
                         hipStream_t s) {                                                                     \
     NS::fft_c2r_yx_batch_p(p, in, out, batch, (RT)divisor, s);                                                 \
   }                                                                                                          \
+  void fft_z_update_next(const FFTPlan &p, CT *mu, CT *ph, CT *za, CT *zg, const CT *noise, UpdateParams u,  \
+                         const double *origin, SpecConst sc, hipStream_t s, bool store_mu) {                \
+    NS::fft_z_update_next_p(p, mu, ph, za, zg, noise, u, origin, sc, s, store_mu);                           \
+  }                                                                                                          \
+  void fft_c2r_next(const FFTPlan &p, CT *za, CT *gy_k, CT *zg, RT *phi, RT *gx, RT *gy, RT *gz, SpecConst sc, \
+                    hipStream_t s) {                                                                         \
+    NS::fft_c2r_next_p(p, za, gy_k, zg, phi, gx, gy, gz, sc, s);                                             \
+  }                                                                                                          \
   void spec_k_to_canon(const FFTPlan &p, const CT *k, CT *canon, hipStream_t s) {                           \
     NS::spec_k_to_canon_p(p, k, canon, s);                                                                     \
   }                                                                                                          \

@@ -142,6 +142,10 @@ struct UpdateParams;
                     const double *origin, hipStream_t s, Slab sl = Slab(), bool store_mu = true);        \
   void fft_c2r_yx_batch(const FFTPlan &p, CT *const *in, RT *const *out, int batch, double divisor,      \
                         hipStream_t s);                                                                  \
+  void fft_z_update_next(const FFTPlan &p, CT *mu, CT *ph, CT *za, CT *zg, const CT *noise, UpdateParams u,  \
+                         const double *origin, SpecConst sc, hipStream_t s, bool store_mu);                \
+  void fft_c2r_next(const FFTPlan &p, CT *za, CT *gy_k, CT *zg, RT *phi, RT *gx, RT *gy, RT *gz, SpecConst sc, \
+                    hipStream_t s);                                                                         \
   void spec_k_to_canon(const FFTPlan &p, const CT *k, CT *canon, hipStream_t s);                         \
   void spec_canon_to_k(const FFTPlan &p, const CT *canon, CT *k, hipStream_t s);
 // fft_r2c / fft_c2r: whole transforms (c2r destroys `in`; divisor != 0
@@ -156,6 +160,12 @@ struct UpdateParams;
 // (FP32 form): the k = 0 mode is taken from the FP64 sum there.  store_mu
 // false: muhat of the planes 1 <= kx < Nx/2 is left as the y pass wrote it
 // (for steps whose muhat nobody reads; the update itself is unchanged).
+// fft_z_update_next / fft_c2r_next (grids with fft_sep_grad_ok, one
+// system): the update of a step that another step of the run follows, and
+// the c2r of phi together with the next step's gradient: phihat keeps N
+// times the updated spectrum (the next step's r2c of phi, up to rounding),
+// za and zg hold the z passes of phi and of its z gradient, gy_k is scratch;
+// phi and grad phi come out as the next step's start_of_step would make them.
 PHAFD_FFT_DECL(double, double2)
 PHAFD_FFT_DECL(float, float2)
 #undef PHAFD_FFT_DECL

@@ -3,7 +3,8 @@
 A batch advances independent systems of one grid together: the grid passes
 run once over all members (a slab of fields, captured in hipGraphs), the atom
 side per member.  The arithmetic of a member is the arithmetic of the same
-system run alone, so the bar is bit-identity with the solo run (phi,
+system run alone (each step's gradient from phi: PHAFD_OPT_NEXT_GRADIENT
+off), so the bar is bit-identity with the solo run (phi,
 positions, images, forces, list counters) on every path the grid can take:
   64^3 -- separable gradient and the fused z pass of the update;
   32^3 -- whole r2c / c2r transforms and the pointwise kernels.
@@ -57,6 +58,7 @@ def test_members_match_solo_runs_bitwise(n):
     b.run(12)
     for i, w in enumerate(ws):
         s = make_gpu(w)
+        s.set_option(P.OPT_NEXT_GRADIENT, 0)   # the batch takes every step's gradient from phi
         s.run(12)
         _assert_same(_state(b.member(i)), _state(s), f"member {i}")
     b.close()

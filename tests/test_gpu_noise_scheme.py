@@ -259,3 +259,57 @@ def test_drag_trajectory_200_steps(zghost, chain):
     assert rel_l2(g.get_atoms()[0], o.get_atoms()[0]) < 1e-10
     assert rel_l2(g.get_atoms()[1], o.get_atoms()[1]) < 1e-10
     assert g.counts()["ncalls"] == o.counts()["ncalls"]
+
+
+# ------------------------------------------------- next step's gradient
+# Inside one run, a step followed by another makes the next step's grad phi
+# from its updated spectrum instead of a forward transform of phi
+# (PHAFD_OPT_NEXT_GRADIENT); the last step of a run leaves the fields as the
+# reference does.  Against the same run without it (rounding only) and the
+# oracle stepping one step per call.
+@pytest.mark.parametrize("shape,period", [((16, 64, 32), [32.0, 96.0, 12.0]),
+                                          ((256, 16, 8), [8.0, 16.0, 256.0]),
+                                          ((64, 64, 64), [64.0, 64.0, 64.0])])
+def test_next_step_gradient_from_spectrum(shape, period):
+    from tests.test_gpu_parity import _fields_only
+    nz, ny, nx = shape
+    a = _fields_only(P.System, shape, period, P.NOISE_FIELD)
+    b = _fields_only(P.System, shape, period, P.NOISE_FIELD)
+    b.set_option(P.OPT_NEXT_GRADIENT, 0)
+    o = _fields_only(O.System, shape, period, O.NOISE_MT19937)
+    gen = P.NoiseGen(5678)
+    xi = [gen.next(nx, ny, nz, period, 1.0, 1.0, 1e-3) for _ in range(6)]
+    for x in xi:
+        a.set_noise(x)
+        b.set_noise(x)
+    a.run(6)
+    b.run(6)
+    o.run(6)
+    # rounding only, to the 1e-12 of the FFT and gradient checks
+    eab = {f: rel_l2(a.get_field(f), b.get_field(f)) for f in (P.PHI, P.MU, P.GRADX, P.GRADY, P.GRADZ)}
+    eao = {f: rel_l2(a.get_field(f), o.get_field(f)) for f in (P.PHI, P.MU, P.GRADX, P.GRADY, P.GRADZ)}
+    ebo = {f: rel_l2(b.get_field(f), o.get_field(f)) for f in (P.PHI, P.MU, P.GRADX, P.GRADY, P.GRADZ)}
+    msg = f"next vs plain {eab}, next vs oracle {eao}, plain vs oracle {ebo}"
+    print(msg)
+    # phi and grad phi to 1e-12; mu, a near-cancelling sum of log terms at
+    # phi ~ 0.5, to the 1e-10 FP64 bar (the path without the option is
+    # 1.8e-11 from the oracle on the 64^3 case after six steps); and no
+    # worse than three times the plain path's own distance from the oracle
+    for f in (P.PHI, P.GRADX, P.GRADY, P.GRADZ):
+        assert eab[f] < 1e-12 and eao[f] < 1e-12, msg
+    assert eab[P.MU] < 1e-10 and eao[P.MU] < 1e-10, msg
+    assert all(eao[f] <= 3 * ebo[f] + 1e-13 for f in eao), msg
+    assert rel_l2(a.get_spectrum(P.MUHAT), b.get_spectrum(P.MUHAT)) < 1e-10
+
+
+@pytest.mark.parametrize("zghost", [0, 1])
+def test_next_step_gradient_trajectory(zghost):
+    w = workload(n=64, nchains=20, nbeads=50, seed=42, zghost=zghost, chi=3.0)
+    g = make_gpu(w)
+    o = make_oracle(w)
+    g.run(100)
+    o.run(100)
+    for f in (P.PHI, P.MU, P.GRADX, P.GRADY, P.GRADZ):
+        assert rel_l2(g.get_field(f), o.get_field(f)) < 1e-10, f
+    assert rel_l2(g.get_atoms()[0], o.get_atoms()[0]) < 1e-10
+    assert g.counts()["ncalls"] == o.counts()["ncalls"]

@@ -318,6 +318,7 @@ def test_fused_spectral_probe_counts():
     # the fused kernels are the ones that run on a 64^3 grid
     w = workload(n=64, nchains=4, nbeads=10, seed=9)
     g = make_gpu(w)
+    g.set_option(P.OPT_NEXT_GRADIENT, 0)
     g.probe("z_update")
     g.run(3)
     ms, launches, nbytes = g.probe(None)
@@ -327,6 +328,17 @@ def test_fused_spectral_probe_counts():
     g.probe("z_grad")
     g.run(2)
     assert g.probe(None)[1] == 2
+    # with the next step's gradient made by the update, only a run's first
+    # step transforms phi forward; its update also writes the z passes of
+    # phi and of the z gradient (+32 B per mode) on every step but the last
+    g.set_option(P.OPT_NEXT_GRADIENT, 1)
+    g.probe("z_grad")
+    g.run(3)
+    assert g.probe(None)[1] == 1
+    g.probe("z_update")
+    g.run(3)
+    ms, launches, nbytes = g.probe(None)
+    assert launches == 3 and nbytes == (2 * 80.0 + 64.0) * 31 * 64 * 64
 
 
 def test_pruned_lj_list_matches_full_list_bitwise():
