@@ -446,6 +446,25 @@ int phafd_hip_spectral_update(int nx, int ny, int nz, const double *period, doub
                               const double *muhat, double mobility, double temp, double volFH,
                               double gamma, double dt, const double *noise);
 
+/* ---- no device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The gradient of
+ * phafd_hip_gradphi, computed by the same kernels, on caller-owned DEVICE
+ * arrays phi, gx, gy, gz (double[Nz][Ny][Nx]), enqueued on `stream` (a
+ * hipStream_t; NULL: the null stream) and not waited for.  The
+ * intermediates (two half spectra) and the FFT twiddles live in a static
+ * device array of PHAFD_NOALLOC_SCRATCH_BYTES that each GPU holds until the
+ * process ends; grids with (Nx/2+1)*Ny*Nz above
+ * PHAFD_GRADPHI_NOALLOC_MAX_MODES (128^3 fits, 256x128x128 does not) are
+ * rejected with PHAFD_E_ARG.  The first call on a GPU sets that GPU up
+ * (symbol lookup, a 64 KiB twiddle copy); later calls only launch kernels.
+ * All calls on one GPU share its array: calls that overlap on one GPU (other
+ * streams or host threads) race.  Errors: phafd_hip_last_error(NULL). */
+#define PHAFD_NOALLOC_SCRATCH_BYTES (40u * 1024u * 1024u)
+#define PHAFD_GRADPHI_NOALLOC_MAX_MODES 1308672
+int phafd_hip_gradphi_noalloc(int nx, int ny, int nz, const double *period, const double *phi,
+                              double *gx, double *gy, double *gz, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

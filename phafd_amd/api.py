@@ -100,6 +100,7 @@ def lib():
         "phafd_hip_fft_r2c": ([ctypes.c_int] * 3 + [_dp, _dp], ctypes.c_int),
         "phafd_hip_fft_c2r": ([ctypes.c_int] * 3 + [_dp, _dp], ctypes.c_int),
         "phafd_hip_gradphi": ([ctypes.c_int] * 3 + [_dp] * 5, ctypes.c_int),
+        "phafd_hip_gradphi_noalloc": ([ctypes.c_int] * 3 + [_dp] + [V] * 5, ctypes.c_int),
         "phafd_hip_floryhuggins": ([ctypes.c_int64, _dp, _dp] + [ctypes.c_double] * 3 + [_lp, _lp], ctypes.c_int),
         "phafd_hip_spectral_update": ([ctypes.c_int] * 3 + [_dp, _dp, _dp] + [ctypes.c_double] * 5 + [_dp],
                                       ctypes.c_int),

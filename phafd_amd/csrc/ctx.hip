@@ -1440,6 +1440,11 @@ void init_ctx(phafd_hip_ctx *c, const phafd_hip_desc *desc, const MemberSlot *sl
 }  // namespace
 
 // =================================================================== C ABI
+namespace phafd {
+// the message phafd_hip_last_error(NULL) reports on this thread
+void set_stateless_error(const std::string &msg) { g_create_err = msg; }
+}  // namespace phafd
+
 extern "C" {
 
 int phafd_hip_create(const phafd_hip_desc *desc, phafd_hip_ctx **out) {

@@ -55,18 +55,24 @@ void probe_collect() {
   p->bytes.clear();
 }
 
-namespace {
 // exp(-+2 pi i m / L) for m < L, evaluated in long double and rounded
+template <class CV>
+void twiddle_host(CV *f, CV *b, int L) {
+  for (int m = 0; m < L; m++) {
+    long double a = 2.0L * 3.14159265358979323846264338327950288L * m / L;
+    f[m].x = (decltype(f[m].x))cosl(a);
+    f[m].y = (decltype(f[m].y))-sinl(a);
+    b[m].x = (decltype(b[m].x))cosl(a);
+    b[m].y = (decltype(b[m].y))sinl(a);
+  }
+}
+template void twiddle_host<double2>(double2 *, double2 *, int);
+
+namespace {
 template <class CV>
 void make_table(DBuf<CV> &f, DBuf<CV> &b, int L) {
   std::vector<CV> hf(L), hb(L);
-  for (int m = 0; m < L; m++) {
-    long double a = 2.0L * 3.14159265358979323846264338327950288L * m / L;
-    hf[m].x = (decltype(hf[m].x))cosl(a);
-    hf[m].y = (decltype(hf[m].y))-sinl(a);
-    hb[m].x = (decltype(hb[m].x))cosl(a);
-    hb[m].y = (decltype(hb[m].y))sinl(a);
-  }
+  twiddle_host(hf.data(), hb.data(), L);
   f.alloc(L);
   b.alloc(L);
   HIPCHK(hipMemcpy(f.p, hf.data(), sizeof(CV) * L, hipMemcpyHostToDevice));
@@ -74,7 +80,7 @@ void make_table(DBuf<CV> &f, DBuf<CV> &b, int L) {
 }
 }  // namespace
 
-void FFTPlan::init(int nx_, int ny_, int nz_, bool fp32) {
+void FFTPlan::set_shape(int nx_, int ny_, int nz_) {
   auto ok = [](int n) { return n >= 4 && n <= 1024 && (n & (n - 1)) == 0; };
   if (!ok(nx_) || !ok(ny_) || !ok(nz_))
     throw std::invalid_argument("grid dimensions must be powers of two in [4, 1024]");
@@ -82,6 +88,10 @@ void FFTPlan::init(int nx_, int ny_, int nz_, bool fp32) {
   ny = ny_;
   nz = nz_;
   hx = nx / 2 + 1;
+}
+
+void FFTPlan::init(int nx_, int ny_, int nz_, bool fp32) {
+  set_shape(nx_, ny_, nz_);
   make_table(tw_x_f, tw_x_b, nx);
   make_table(tw_h_f, tw_h_b, nx / 2);
   make_table(tw_y_f, tw_y_b, ny);

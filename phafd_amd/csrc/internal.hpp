@@ -93,6 +93,11 @@ void probe_collect();   // after the stream has been synchronised
 // that the y pass reads contiguous rows and the z pass reads contiguous runs
 // of ky.  Canonical host layout is [Nz][Ny][Nx/2+1].  Every transform exists
 // in FP64 (double / double2) and FP32 (float / float2), as overloads.
+// exp(-2 pi i m / L) into f[m], exp(+2 pi i m / L) into b[m], m < L
+// (long double evaluation rounded to the element type): the FFT twiddles
+template <class CV>
+void twiddle_host(CV *f, CV *b, int L);
+
 struct FFTPlan {
   int nx = 0, ny = 0, nz = 0, hx = 0;
   DBuf<double2> tw_x_f, tw_x_b;   // length nx   (post/pre processing of the real pass)
@@ -101,6 +106,8 @@ struct FFTPlan {
   DBuf<double2> tw_z_f, tw_z_b;   // length nz
   DBuf<float2> tw32_x_f, tw32_x_b, tw32_h_f, tw32_h_b, tw32_y_f, tw32_y_b, tw32_z_f, tw32_z_b;
   void init(int nx, int ny, int nz, bool fp32 = false);
+  // the sizes alone (validated); the tables are left to the caller (wrap)
+  void set_shape(int nx, int ny, int nz);
   size_t M() const { return (size_t)hx * nz * ny; }
   size_t N() const { return (size_t)nx * ny * nz; }
 };
