@@ -105,9 +105,35 @@ def batch_members(bw, seed=2024):
     return out
 
 
+def host_cpu():
+    """Model name and logical CPU counts of the host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def cpu_threads():
+    """Threads given to the CPU oracle: OMP_NUM_THREADS if set, else the
+    CPUs this process may run on."""
+    v = os.environ.get("OMP_NUM_THREADS", "")
+    return int(v) if v.isdigit() and int(v) > 0 else len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(wl, phi, xyz, mol, steps):
-    """The CPU oracle (single thread) on a bounded sample of the same workload."""
+    """The CPU oracle on a bounded sample of the same workload: `steps` full
+    timesteps after setup.  Its FFT lines and pointwise loops run on
+    cpu_threads() OpenMP threads; the pair pass, lists and update traversal
+    run on one.  Returns (seconds, threads)."""
     from oracle import oracle_ctypes as O
+    th = cpu_threads()
+    O.set_threads(th)
     n = wl["n"]
     s = O.System(n, n, n, [0, 0, 0], [wl["box"]] * 3, zghost=wl["zghost"])
     configure(s, wl, phi, xyz, mol, is_oracle=True)
@@ -115,7 +141,23 @@ def cpu_baseline(wl, phi, xyz, mol, steps):
     s.run(steps)
     dt = time.perf_counter() - t0
     s.close()
-    return dt
+    return dt, th
+
+
+def cpu_baseline_record(value, t, th, steps, what):
+    return dict({"value": value, "unit": "grid-pt·steps/s", "cores": th, "kind": "port",
+                 "sample": f"{steps} full timesteps of {what} after setup, CPU FP64 oracle, {th} OpenMP threads for "
+                           f"the FFT lines and pointwise loops, one for the rest ({t:.1f} s)"}, **host_cpu())
+
+
+def bead_error(u, uref, u0):
+    """Unwrapped bead positions u against the plain FP64 run's uref (both
+    from u0): RMS and max distance, and the RMS displacement of the plain
+    run over the same steps as the scale (lengths in the box's units, bead
+    diameter 1)."""
+    d = np.linalg.norm(u - uref, axis=1)
+    return {"rms": float(np.sqrt(np.mean(d * d))), "max": float(d.max()),
+            "rms_displacement_fp64": float(np.sqrt(np.mean(np.sum((uref - u0) ** 2, axis=1))))}
 
 
 def snapshot(s, P):
@@ -225,10 +267,12 @@ def surrogate_leg(args, wl, phi, xyz, mol):
         return s
 
     a = system()
+    u0 = a.get_atoms()[1]
     t0 = time.perf_counter()
     a.run(A)
     plain = time.perf_counter() - t0
     ref = a.get_field(P.PHI)
+    uref = a.get_atoms()[1]
     a.close()
     net = P.UNet(seed=1)
     net.load(SURROGATE)
@@ -240,6 +284,7 @@ def surrogate_leg(args, wl, phi, xyz, mol):
     el = time.perf_counter() - t0
     pms, plaunch, pflops = s.probe(None)
     got = s.get_field(P.PHI)
+    beads = bead_error(s.get_atoms()[1], uref, u0)
     s.close()
     net.close()
     conv_tflops = pflops / (pms * 1e-3) / 1e12 if plaunch else None
@@ -247,7 +292,7 @@ def surrogate_leg(args, wl, phi, xyz, mol):
             "plain_fp64_ms_per_step_same_run": 1e3 * plain / A, "speedup_vs_fp64": plain / el,
             f"phi_rel_l2_vs_fp64_after_{A}_steps": rel_err(got, ref), "K": K, "tol": tol,
             "blocks": int(st["blocks"]), "accepted": int(st["accepted"]), "fallbacks": int(st["fallbacks"]),
-            "max_residual": st["max_residual"],
+            "max_residual": st["max_residual"], f"bead_position_err_vs_fp64_after_{A}_steps": beads,
             "ms_network_path_per_block": st["ms_network_path"] / max(st["blocks"], 1),
             "roofline": {"bound": "mfma", "kernel": "k_conv<16,16,27> (3^3 conv 16->16, full resolution, BF16 MFMA)",
                          "achieved": conv_tflops, "peak": 2500.0, "unit": "TFLOP/s",
@@ -425,11 +470,9 @@ def bench_batch(args):
     }
     if not args.no_cpu_baseline:
         wl, phi, xyz, mol = members[0]
-        t = cpu_baseline(wl, phi, xyz, mol, args.cpu_steps)
-        out["cpu_baseline"] = {"value": n ** 3 * args.cpu_steps / t, "unit": "grid-pt·steps/s", "cores": 1,
-                               "kind": "port",
-                               "sample": f"{args.cpu_steps} full timesteps of member 0 ({beads[0]} beads) after "
-                                         f"setup, CPU FP64 oracle, one thread ({t:.1f} s)"}
+        t, th = cpu_baseline(wl, phi, xyz, mol, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline_record(n ** 3 * args.cpu_steps / t, t, th, args.cpu_steps,
+                                                  f"member 0 ({beads[0]} beads)")
     print(json.dumps(out), flush=True)
 
 
@@ -442,7 +485,7 @@ def main():
     ap.add_argument("--probe", default="c2r_x", help="kernel timed with HIP events for the roofline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (counter calibration) and exit")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
@@ -552,11 +595,8 @@ def main():
     if args.timers:
         print(json.dumps({"per_hook_ms_total": timers}), file=sys.stderr)
     if not args.no_cpu_baseline:
-        t = cpu_baseline(wl, phi, xyz, mol, args.cpu_steps)
-        out["cpu_baseline"] = {"value": N * args.cpu_steps / t, "unit": "grid-pt·steps/s", "cores": 1,
-                               "kind": "port",
-                               "sample": f"{args.cpu_steps} full timesteps of the same workload after setup, "
-                                         f"CPU FP64 oracle, one thread ({t:.1f} s)"}
+        t, th = cpu_baseline(wl, phi, xyz, mol, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline_record(N * args.cpu_steps / t, t, th, args.cpu_steps, "the same workload")
     print(json.dumps(out), flush=True)
 
 

@@ -2,7 +2,8 @@
 
 Compiles oracle/oracle.cpp and oracle/oracle_vtk.cpp into oracle/liboracle.so with g++ -O3
 -ffp-contract=off (no FMA contraction, so the operation order written in the
-restatement is the one executed).  The reference itself cannot be built here:
+restatement is the one executed) and -fopenmp (FFT lines and pointwise loops
+across threads; no reduction depends on the thread count).  The reference itself cannot be built here:
 it needs FFTW3-MPI, Eigen3, fftw_arr and beadrodpmer, none of which exist in
 this image, so there is no oracle/_ref build.
 """
@@ -17,7 +18,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 def build(verbose=False):
     srcs = [os.path.join(HERE, f) for f in ("oracle.cpp", "oracle_vtk.cpp")]
     tmp = LIB + ".tmp"
-    cmd = ["g++", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+    cmd = ["g++", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
            "-fPIC", "-shared", "-o", tmp] + srcs
     if verbose:
         print(" ".join(cmd), flush=True)

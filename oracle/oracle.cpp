@@ -43,6 +43,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstring>
+#include <omp.h>
 #include <random>
 #include <set>
 #include <stdexcept>
@@ -108,27 +109,33 @@ struct FFT1 {
 void fft3_r2c(int nx, int ny, int nz, const double *in, cplx *out) {
   FFT1 fx(nx), fy(ny), fz(nz);
   const int hx = nx / 2 + 1;
-  std::vector<cplx> line(std::max(nx, std::max(ny, nz)));
-  for (int z = 0; z < nz; z++)
-    for (int y = 0; y < ny; y++) {
-      const double *r = in + ((size_t)z * ny + y) * nx;
+  const int L = std::max(nx, std::max(ny, nz));
+#pragma omp parallel
+  {
+    std::vector<cplx> line(L);
+#pragma omp for schedule(static)
+    for (int zy = 0; zy < nz * ny; zy++) {
+      const double *r = in + (size_t)zy * nx;
       for (int x = 0; x < nx; x++) line[x] = cplx(r[x], 0.0);
       fx.run(line.data(), -1);
-      cplx *o = out + ((size_t)z * ny + y) * hx;
+      cplx *o = out + (size_t)zy * hx;
       for (int k = 0; k < hx; k++) o[k] = line[k];
     }
-  for (int z = 0; z < nz; z++)
-    for (int k = 0; k < hx; k++) {
+#pragma omp for schedule(static)
+    for (int zk = 0; zk < nz * hx; zk++) {
+      const int z = zk / hx, k = zk % hx;
       for (int y = 0; y < ny; y++) line[y] = out[((size_t)z * ny + y) * hx + k];
       fy.run(line.data(), -1);
       for (int y = 0; y < ny; y++) out[((size_t)z * ny + y) * hx + k] = line[y];
     }
-  for (int y = 0; y < ny; y++)
-    for (int k = 0; k < hx; k++) {
+#pragma omp for schedule(static)
+    for (int yk = 0; yk < ny * hx; yk++) {
+      const int y = yk / hx, k = yk % hx;
       for (int z = 0; z < nz; z++) line[z] = out[((size_t)z * ny + y) * hx + k];
       fz.run(line.data(), -1);
       for (int z = 0; z < nz; z++) out[((size_t)z * ny + y) * hx + k] = line[z];
     }
+  }
 }
 
 /* 3-D c2r (unnormalised) with the half-complex synthesis rule: inverse c2c
@@ -141,22 +148,27 @@ void fft3_c2r(int nx, int ny, int nz, const cplx *in, double *out) {
   FFT1 fx(nx), fy(ny), fz(nz);
   const int hx = nx / 2 + 1;
   std::vector<cplx> w(in, in + (size_t)nz * ny * hx);
-  std::vector<cplx> line(std::max(nx, std::max(ny, nz)));
-  for (int y = 0; y < ny; y++)
-    for (int k = 0; k < hx; k++) {
+  const int L = std::max(nx, std::max(ny, nz));
+#pragma omp parallel
+  {
+    std::vector<cplx> line(L);
+#pragma omp for schedule(static)
+    for (int yk = 0; yk < ny * hx; yk++) {
+      const int y = yk / hx, k = yk % hx;
       for (int z = 0; z < nz; z++) line[z] = w[((size_t)z * ny + y) * hx + k];
       fz.run(line.data(), +1);
       for (int z = 0; z < nz; z++) w[((size_t)z * ny + y) * hx + k] = line[z];
     }
-  for (int z = 0; z < nz; z++)
-    for (int k = 0; k < hx; k++) {
+#pragma omp for schedule(static)
+    for (int zk = 0; zk < nz * hx; zk++) {
+      const int z = zk / hx, k = zk % hx;
       for (int y = 0; y < ny; y++) line[y] = w[((size_t)z * ny + y) * hx + k];
       fy.run(line.data(), +1);
       for (int y = 0; y < ny; y++) w[((size_t)z * ny + y) * hx + k] = line[y];
     }
-  for (int z = 0; z < nz; z++)
-    for (int y = 0; y < ny; y++) {
-      const cplx *c = w.data() + ((size_t)z * ny + y) * hx;
+#pragma omp for schedule(static)
+    for (int zy = 0; zy < nz * ny; zy++) {
+      const cplx *c = w.data() + (size_t)zy * hx;
       line[0] = cplx(c[0].real(), 0.0);
       for (int k = 1; k < nx / 2; k++) {
         line[k] = c[k];
@@ -164,9 +176,10 @@ void fft3_c2r(int nx, int ny, int nz, const cplx *in, double *out) {
       }
       if (nx > 1) line[nx / 2] = cplx(c[nx / 2].real(), 0.0);
       fx.run(line.data(), +1);
-      double *r = out + ((size_t)z * ny + y) * nx;
+      double *r = out + (size_t)zy * nx;
       for (int x = 0; x < nx; x++) r[x] = line[x].real();
     }
+  }
 }
 
 /* signed wavenumber index: i > N/2 -> i-N (Nyquist stays positive),
@@ -188,6 +201,7 @@ void gradphi_impl(int nx, int ny, int nz, const double period[3], const double *
   const cplx idqx(0, 2 * PI_REF / period[0]);
   const cplx idqy(0, 2 * PI_REF / period[2]);
   const cplx idqz(0, 2 * PI_REF / period[1]);
+#pragma omp parallel for schedule(static)
   for (int ky = 0; ky < ny; ky++) {
     double l = (double)signed_k(ky, ny);
     for (int kz = 0; kz < nz; kz++) {
@@ -206,6 +220,7 @@ void gradphi_impl(int nx, int ny, int nz, const double period[3], const double *
   fft3_c2r(nx, ny, nz, g2.data(), gy);   // swapped target, src/grid.cpp:276-279
   double factor = (double)nx * ny * nz;  // src/fixgrid_conjugate.cpp:111
   const size_t N = (size_t)nx * ny * nz;
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; i++) { gx[i] /= factor; gy[i] /= factor; gz[i] /= factor; }
   if (phihat_out) phihat_out->swap(ph);
 }
@@ -213,16 +228,20 @@ void gradphi_impl(int nx, int ny, int nz, const double period[3], const double *
 /* FixGridFloryHuggins::post_force, src/fixgrid_floryhuggins.cpp:65-85 */
 void floryhuggins_impl(int64_t n, double *phi, double *mu, double temp, double volFH,
                        double chi, int64_t &nless, int64_t &ngreat) {
+  int64_t nl = 0, ng = 0;
+#pragma omp parallel for schedule(static) reduction(+ : nl, ng)
   for (int64_t i = 0; i < n; i++) {
     if (phi[i] <= 0) {
       phi[i] = -phi[i];
-      nless += 1;
+      nl += 1;
     } else if (phi[i] >= 1) {
       phi[i] = 1 - (phi[i] - 1);
-      ngreat += 1;
+      ng += 1;
     }
     mu[i] += temp / volFH * (log(phi[i] / (1 - phi[i])) + chi * (1 - 2 * phi[i]));
   }
+  nless += nl;
+  ngreat += ng;
 }
 
 /* Conjugate::update single-rank traversal, src/conjugate.cpp:163-327, with
@@ -1284,6 +1303,12 @@ int orc_mt19937_uniform(int seed, int n, double *out) {
   return 0;
 }
 int orc_make_unique_seed(int seed) { return make_unique_seed(seed); }
+int orc_set_threads(int n) {
+  if (n < 1) return -1;
+  omp_set_num_threads(n);
+  return 0;
+}
+int orc_threads() { return omp_get_max_threads(); }
 int orc_mt19937_raw(int seed, int n, uint32_t *out) {
   std::mt19937 gen;
   gen.seed(seed);

@@ -48,6 +48,10 @@ int orc_noise_field(int nx, int ny, int nz, const double period[3], int seed,
                     double mobility, double temp, double dt, double *noise);
 int orc_mt19937_uniform(int seed, int n, double *out);
 int orc_make_unique_seed(int seed);
+/* OpenMP threads of the FFT lines and pointwise loops (results do not
+ * depend on the count: every line and point is computed alone) */
+int orc_set_threads(int n);
+int orc_threads(void);
 int orc_mt19937_raw(int seed, int n, uint32_t *out);
 
 /* ---- whole-system state machine (mirrors Integrate::setup / run) ---- */

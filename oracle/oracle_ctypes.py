@@ -72,6 +72,7 @@ def lib():
         L.orc_noise_field.argtypes = [ctypes.c_int] * 3 + [_dp, ctypes.c_int] + [ctypes.c_double] * 3 + [_dp]
         L.orc_mt19937_uniform.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
         L.orc_make_unique_seed.argtypes = [ctypes.c_int]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_mt19937_raw.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
         _cp, _pp = ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p)
         _vpp = ctypes.POINTER(ctypes.c_void_p)
@@ -177,6 +178,16 @@ def mt19937_raw(seed, n):
 
 def make_unique_seed(seed):
     return lib().orc_make_unique_seed(seed)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's FFT lines and pointwise loops."""
+    if lib().orc_set_threads(int(n)) != 0:
+        raise ValueError("thread count must be >= 1")
+
+
+def threads():
+    return int(lib().orc_threads())
 
 
 # ------------------------------------------------------------ system
