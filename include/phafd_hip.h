@@ -229,6 +229,27 @@ int phafd_hip_fix_drag(phafd_hip_ctx *ctx, int nchunk, const int *start, const i
  * overdamped with the same two half steps as atom/drag. */
 int phafd_hip_fix_chain(phafd_hip_ctx *ctx, int nchunk, const int *start, const int *end,
                         double r0, double kbond, double kbend, double drag);
+/* fix atom/semiflexible/{notether,singletether,doubletether} <group> <seed>
+ * bondlength b zeta_para zpar zeta_perp zperp bending kb temp T
+ * (src/fixatom_semiflexible.cpp:27-191; tether 0, 1, 2), one polymer per
+ * group chunk, per-polymer seeds drawn as the reference draws them
+ * (uniform_int_distribution<int> on mt19937(make_unique_seed(seed))).  The
+ * reference delegates the dynamics to beadrodpmer, which is not available;
+ * this is the published bead-rod Brownian dynamics of Montesi, Morse &
+ * Pasquali (J. Chem. Phys. 122, 084903, 2005): rigid rods, bending
+ * kb (1 - u_j.u_{j+1}), anisotropic bead friction about the bead tangent,
+ * metric pseudo-potential (kT/2) ln det(G H G^T), midpoint predictor
+ * (initial_integrate) and Newton corrector of the rod tensions
+ * (final_integrate, at most 50 iterations; past that the step returns
+ * PHAFD_E_RUNTIME "Failed to maintain polymer bond-length constraints.").
+ * Tethered beads (bead 0; bead 0 and N-1) do not move.  Brownian draws:
+ * Philox4x64-10 keyed by the polymer seed (uniform, unit variance).
+ * phafd_hip_setup checks every bond against b (|delta| > 1e-3 is the
+ * reference's error).  At most 700 beads per polymer; plain runs only (not
+ * batched members or the surrogate). */
+int phafd_hip_fix_semiflexible(phafd_hip_ctx *ctx, int tether, int nchunk, const int *start, const int *end,
+                               int seed, double bondlength, double zeta_para, double zeta_perp, double bending,
+                               double temp);
 
 /* ---- neighbor <skin>: Neighbor::setup + CommBrick::setup
  *      (src/neighbor.cpp:39-68, src/comm_brick.cpp:96-187) */

@@ -424,7 +424,7 @@ struct PairStyle {
 };
 
 struct FixDef {
-  enum { CONJ, FH, GRADPHI, DRAG, CHAIN } kind;
+  enum { CONJ, FH, GRADPHI, DRAG, CHAIN, SEMIFLEX } kind;
   // conjugate
   int seed = 0, noise_mode = 0;
   double mobility = 1, temp = 1, volFH = 1, gamma = 1, chi = 1;
@@ -434,6 +434,12 @@ struct FixDef {
   // drag / chain
   std::vector<int> start, end;
   double drag = 0, r0 = 1, kbond = 0, kbend = 0;
+  // atom/semiflexible/*: one polymer per group chunk
+  int tether = 0;
+  double bondlength = 1, zpar = 1, zperp = 1, bending = 0, brtemp = 0;
+  std::vector<int> pseed;            // per-polymer seeds
+  std::vector<double> R0, FB, T;     // per atom: start-of-step positions, Brownian force, rod tensions
+  int64_t brstep = 0;                // first_step calls
 };
 
 struct Swap {
@@ -1126,6 +1132,316 @@ void drag_halfstep(orc_sys *s, const FixDef &f) {
       for (int q = 0; q < 3; q++) s->xs[3 * j + q] += f.drag * s->Fs[3 * j + q] * s->dt * 0.5;
 }
 
+/* --------------------------------------------------- bead-rod chains ---
+ * fix atom/semiflexible/{notether,singletether,doubletether}
+ * (src/fixatom_semiflexible.cpp:27-191).  The reference hands the dynamics
+ * to beadrodpmer (absent here, no version pinned), so this restates a
+ * published bead-rod Brownian dynamics instead (Montesi, Morse & Pasquali,
+ * J. Chem. Phys. 122, 084903 (2005)):
+ *   - rigid rods of length b between consecutive beads, tension T_j;
+ *   - bending U = kb sum_j (1 - u_j.u_{j+1});
+ *   - the metric pseudo-potential (kT/2) ln det G, G_jj = w_j + w_{j+1},
+ *     G_{j,j+1} = -w_{j+1} u_j.u_{j+1}, under which the rigid chain samples
+ *     independent bond orientations weighted by exp(-U/kT) (Fixman);
+ *   - bead mobility H_i = w_i [t t / zeta_para + (I - t t) / zeta_perp]
+ *     about the bead tangent t (w_i = 0 for a tethered bead: bead 0 for
+ *     singletether, beads 0 and N-1 for doubletether);
+ *   - Brownian force sqrt(2 kT / dt) [sqrt(zeta_para) t t + sqrt(zeta_perp)
+ *     (I - t t)] xi at the step's start, xi of unit variance;
+ *   - first_step: the tensions that keep every rod length constant to first
+ *     order, then the half step R* = R0 + dt/2 V(R0);
+ *   - second_step: R1 = R0 + dt V(R*), every mobility, direction and force
+ *     taken at R*, with the tensions solved by Newton iteration on
+ *     |R1_{j+1} - R1_j|^2 = b^2 (tridiagonal Jacobian), at most itermax
+ *     iterations (src/fixatom_semiflexible.cpp:160-187).
+ * xi: uniform deviates (w >> 11) 2^-53 - 1/2 scaled by sqrt(12), words 0-2
+ * of Philox4x64-10(counter {bead in polymer, step, 0, 0}, key {polymer seed,
+ * 0x4245414452}).  Parity with beadrodpmer itself: unpinned.              */
+constexpr uint64_t BR_KEY1 = 0x4245414452ull;
+constexpr double BR_TOL = 1e-12;   // |R1_{j+1} - R1_j|^2 - b^2 within BR_TOL b^2
+
+// Philox4x64-10 (Salmon, Moraes, Dror & Shaw, SC'11): ten rounds of the
+// 64-bit multiply-xor round with Weyl key increments
+void philox4x64_10_host(uint64_t c[4], uint64_t k0, uint64_t k1) {
+  const uint64_t M0 = 0xD2E7470EE14C6C93ull, M1 = 0xCA5A826395121157ull;
+  const uint64_t W0 = 0x9E3779B97F4A7C15ull, W1 = 0xBB67AE8584CAA73Bull;
+  for (int r = 0; r < 10; r++) {
+    const unsigned __int128 p0 = (unsigned __int128)M0 * c[0];
+    const unsigned __int128 p1 = (unsigned __int128)M1 * c[2];
+    const uint64_t hi0 = (uint64_t)(p0 >> 64), lo0 = (uint64_t)p0;
+    const uint64_t hi1 = (uint64_t)(p1 >> 64), lo1 = (uint64_t)p1;
+    const uint64_t n0 = hi1 ^ c[1] ^ k0, n1 = lo1, n2 = hi0 ^ c[3] ^ k1, n3 = lo0;
+    c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+    k0 += W0;
+    k1 += W1;
+  }
+}
+
+struct BRParams {
+  double b, zpar, zperp, kb, temp, dt;
+  int tether;
+};
+
+inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// chain geometry at positions R (3N): unit bonds u (3m), inverse lengths il,
+// bead tangents t (3N)
+struct BRGeom {
+  int N = 0, m = 0;
+  std::vector<double> u, il, t, w;
+  void compute(const double *R, int n, int tether) {
+    N = n;
+    m = n - 1;
+    u.assign(3 * (size_t)m, 0.0);
+    il.assign(m, 0.0);
+    t.assign(3 * (size_t)N, 0.0);
+    w.assign(N, 1.0);
+    if (tether >= 1) w[0] = 0.0;
+    if (tether >= 2) w[N - 1] = 0.0;
+    for (int j = 0; j < m; j++) {
+      double d[3];
+      for (int q = 0; q < 3; q++) d[q] = R[3 * (j + 1) + q] - R[3 * j + q];
+      const double l = sqrt(dot3(d, d));
+      il[j] = 1.0 / l;
+      for (int q = 0; q < 3; q++) u[3 * j + q] = d[q] * il[j];
+    }
+    for (int i = 0; i < N; i++) {
+      double *ti = &t[3 * i];
+      if (m == 0) continue;   // a lone bead: isotropic, zeta_perp
+      if (i == 0 || i == N - 1) {
+        const double *uu = &u[3 * (i == 0 ? 0 : m - 1)];
+        for (int q = 0; q < 3; q++) ti[q] = uu[q];
+        continue;
+      }
+      double sv[3];
+      for (int q = 0; q < 3; q++) sv[q] = u[3 * (i - 1) + q] + u[3 * i + q];
+      const double n2 = sqrt(dot3(sv, sv));
+      if (n2 > 1e-12) for (int q = 0; q < 3; q++) ti[q] = sv[q] / n2;
+      else for (int q = 0; q < 3; q++) ti[q] = u[3 * i + q];
+    }
+  }
+  // out = H_i v
+  void mob(const BRParams &p, int i, const double *v, double *out) const {
+    if (w[i] == 0.0) { out[0] = out[1] = out[2] = 0.0; return; }
+    const double *ti = &t[3 * i];
+    const double tv = dot3(ti, v);
+    const double a = 1.0 / p.zperp, c = 1.0 / p.zpar - 1.0 / p.zperp;
+    for (int q = 0; q < 3; q++) out[q] = v[q] * a + c * tv * ti[q];
+  }
+};
+
+// F (3N) += bending and metric pseudo-potential forces.  The metric is the
+// rod-tension matrix A = G H G^T (G: rows grad |R_{j+1} - R_j|), i.e.
+// A_jj = u_j.(H_j + H_{j+1}) u_j, A_{j,j+1} = -u_j.H_{j+1} u_{j+1}; with the
+// bead tangents of BRGeom every entry is a function of the bond cosines
+// c_j = u_j.u_{j+1} only ((t.u)^2 = (1 + c)/2 at an interior bead, 1 at an
+// end), so U_ps = (kT/2) ln det A acts through the cosines like the bending
+// energy: F = sum_j (kb - (kT/2) d ln det A / d c_j) grad c_j.
+void br_bend_ps(const BRParams &p, const BRGeom &g, double *F) {
+  const int m = g.m, N = g.N;
+  if (m < 2) return;
+  const double ip = 1.0 / p.zperp, kh = 1.0 / p.zpar - 1.0 / p.zperp;
+  std::vector<double> c(m - 1), a(m), off(m, 0.0), al(m), be(m);
+  for (int j = 0; j + 1 < m; j++) c[j] = dot3(&g.u[3 * j], &g.u[3 * (j + 1)]);
+  for (int j = 0; j < m; j++) {
+    const double aL = j == 0 ? 1.0 : 0.5 * (1.0 + c[j - 1]);       // (t_j.u_j)^2
+    const double aR = j + 1 == N - 1 ? 1.0 : 0.5 * (1.0 + c[j]);   // (t_{j+1}.u_j)^2
+    a[j] = (g.w[j] + g.w[j + 1]) * ip + kh * (g.w[j] * aL + g.w[j + 1] * aR);
+    if (j + 1 < m) off[j] = -g.w[j + 1] * (c[j] * ip + kh * 0.5 * (1.0 + c[j]));
+  }
+  auto safe = [](double x) { return fabs(x) > 1e-300; };
+  al[0] = a[0];
+  for (int j = 1; j < m; j++) al[j] = safe(al[j - 1]) ? a[j] - off[j - 1] * off[j - 1] / al[j - 1] : a[j];
+  be[m - 1] = a[m - 1];
+  for (int j = m - 2; j >= 0; j--) be[j] = safe(be[j + 1]) ? a[j] - off[j] * off[j] / be[j + 1] : a[j];
+  // diagonal of A^-1: 1 / (al_j + be_j - a_j)
+  std::vector<double> dinv(m);
+  for (int j = 0; j < m; j++) {
+    const double den = al[j] + be[j] - a[j];
+    dinv[j] = safe(den) ? 1.0 / den : 0.0;
+  }
+  for (int j = 0; j + 1 < m; j++) {
+    double e = p.kb;
+    if (p.temp != 0.0) {
+      // (A^-1)_{j,j+1} = -off_j / be_{j+1} (A^-1)_jj
+      const double oinv = safe(be[j + 1]) ? -off[j] / be[j + 1] * dinv[j] : 0.0;
+      const double w1 = g.w[j + 1];
+      const double dlog = 2.0 * oinv * (-w1 * (ip + 0.5 * kh)) + 0.5 * kh * w1 * (dinv[j] + dinv[j + 1]);
+      e -= 0.5 * p.temp * dlog;
+    }
+    const double *u0 = &g.u[3 * j], *u1 = &g.u[3 * (j + 1)];
+    for (int q = 0; q < 3; q++) {
+      const double pj = g.il[j] * (u1[q] - c[j] * u0[q]);
+      const double qj = g.il[j + 1] * (u0[q] - c[j] * u1[q]);
+      F[3 * j + q] -= e * pj;
+      F[3 * (j + 1) + q] += e * (pj - qj);
+      F[3 * (j + 2) + q] += e * qj;
+    }
+  }
+}
+
+// Thomas algorithm: lo_j x_{j-1} + di_j x_j + up_j x_{j+1} = r_j (x in r);
+// a vanishing pivot leaves its row's unknown 0
+void br_thomas(int m, const double *lo, const double *di, const double *up, double *r) {
+  std::vector<double> cp(m);
+  for (int j = 0; j < m; j++) {
+    const double den = di[j] - (j > 0 ? lo[j] * cp[j - 1] : 0.0);
+    if (fabs(den) > 1e-300) {
+      cp[j] = up[j] / den;
+      r[j] = (r[j] - (j > 0 ? lo[j] * r[j - 1] : 0.0)) / den;
+    } else {
+      cp[j] = 0.0;
+      r[j] = 0.0;
+    }
+  }
+  for (int j = m - 2; j >= 0; j--) r[j] -= cp[j] * r[j + 1];
+}
+
+// tensions T (m) that keep u_j.(V_{j+1} - V_j) = 0 for the forces F
+void br_tension_linear(const BRParams &p, const BRGeom &g, const double *F, double *T) {
+  const int m = g.m;
+  std::vector<double> lo(m, 0.0), di(m), up(m, 0.0);
+  for (int j = 0; j < m; j++) {
+    const double *uj = &g.u[3 * j];
+    double h0[3], h1[3], f0[3], f1[3];
+    g.mob(p, j, uj, h0);
+    g.mob(p, j + 1, uj, h1);
+    di[j] = -(dot3(uj, h1) + dot3(uj, h0));
+    if (j > 0) {
+      double hl[3];
+      g.mob(p, j, &g.u[3 * (j - 1)], hl);
+      lo[j] = dot3(uj, hl);
+    }
+    if (j + 1 < m) {
+      double hu[3];
+      g.mob(p, j + 1, &g.u[3 * (j + 1)], hu);
+      up[j] = dot3(uj, hu);
+    }
+    g.mob(p, j, &F[3 * j], f0);
+    g.mob(p, j + 1, &F[3 * (j + 1)], f1);
+    double df[3] = {f1[0] - f0[0], f1[1] - f0[1], f1[2] - f0[2]};
+    T[j] = -dot3(uj, df);
+  }
+  br_thomas(m, lo.data(), di.data(), up.data(), T);
+}
+
+// total force (without tension) of the bead-rod step: external F + Brownian
+// FB + bending and pseudo-potential at the geometry g
+std::vector<double> br_forces(const BRParams &p, const BRGeom &g, const double *Fext, const double *FB) {
+  std::vector<double> F(3 * (size_t)g.N);
+  for (size_t k = 0; k < F.size(); k++) F[k] = Fext[k] + FB[k];
+  br_bend_ps(p, g, F.data());
+  return F;
+}
+
+// Brownian forces FB (3N) at the tangents of g
+void br_noise(const BRParams &p, const BRGeom &g, uint64_t seed, int64_t step, double *FB) {
+  const double amp = p.temp > 0 ? sqrt(2.0 * p.temp / p.dt) : 0.0;
+  const double s12 = sqrt(12.0), sp = sqrt(p.zpar), sq = sqrt(p.zperp);
+  for (int i = 0; i < g.N; i++) {
+    double xi[3] = {0.0, 0.0, 0.0};
+    if (amp != 0.0) {
+      uint64_t c[4] = {(uint64_t)i, (uint64_t)step, 0, 0};
+      philox4x64_10_host(c, seed, BR_KEY1);
+      for (int q = 0; q < 3; q++) xi[q] = s12 * ((double)(c[q] >> 11) * 0x1.0p-53 - 0.5);
+    }
+    const double *t = &g.t[3 * i];
+    const double tx = dot3(t, xi);
+    for (int q = 0; q < 3; q++) FB[3 * i + q] = amp * (sp * tx * t[q] + sq * (xi[q] - tx * t[q]));
+  }
+}
+
+// first_step of one polymer of N beads: R (3N) holds R0 on entry and R* on
+// return; R0, FB (3N) and T (N-1) are kept for second_step
+void br_first_step(const BRParams &p, uint64_t seed, int64_t step, int N, double *R, const double *Fext, double *R0,
+                   double *FB, double *T) {
+  std::copy(R, R + 3 * (size_t)N, R0);
+  BRGeom g;
+  g.compute(R0, N, p.tether);
+  br_noise(p, g, seed, step, FB);
+  std::vector<double> F = br_forces(p, g, Fext, FB);
+  if (g.m > 0) br_tension_linear(p, g, F.data(), T);
+  for (int i = 0; i < N; i++) {
+    double tot[3], v[3];
+    for (int q = 0; q < 3; q++) {
+      tot[q] = F[3 * i + q];
+      if (i < g.m) tot[q] += T[i] * g.u[3 * i + q];
+      if (i > 0) tot[q] -= T[i - 1] * g.u[3 * (i - 1) + q];
+    }
+    g.mob(p, i, tot, v);
+    for (int q = 0; q < 3; q++) R[3 * i + q] = R0[3 * i + q] + 0.5 * p.dt * v[q];
+  }
+}
+
+// second_step of one polymer: R holds R* on entry and R1 on return; T (the
+// first step's tensions on entry) is the Newton start.  Returns the Newton
+// iterations taken, itermax + 1 when the rods were not brought to length.
+int br_second_step(const BRParams &p, int N, double *R, const double *Fext, const double *R0, const double *FB,
+                   double *T, int itermax) {
+  BRGeom g;
+  g.compute(R, N, p.tether);
+  std::vector<double> F = br_forces(p, g, Fext, FB);
+  const int m = g.m;
+  std::vector<double> P(3 * (size_t)N), hA(3 * (size_t)m), hB(3 * (size_t)m), D(3 * (size_t)m);
+  std::vector<double> r(m), lo(m), di(m), up(m);
+  for (int i = 0; i < N; i++) {
+    double v[3];
+    g.mob(p, i, &F[3 * i], v);
+    for (int q = 0; q < 3; q++) P[3 * i + q] = R0[3 * i + q] + p.dt * v[q];
+  }
+  for (int j = 0; j < m; j++) {
+    g.mob(p, j, &g.u[3 * j], &hA[3 * j]);
+    g.mob(p, j + 1, &g.u[3 * j], &hB[3 * j]);
+  }
+  int it = 0;
+  const double tol = BR_TOL * p.b * p.b;
+  for (;; it++) {
+    double gmax = 0.0;
+    for (int j = 0; j < m; j++) {
+      for (int q = 0; q < 3; q++) {
+        double d = P[3 * (j + 1) + q] - P[3 * j + q];
+        double tq = -T[j] * hB[3 * j + q] - T[j] * hA[3 * j + q];
+        if (j + 1 < m) tq += T[j + 1] * hA[3 * (j + 1) + q];
+        if (j > 0) tq += T[j - 1] * hB[3 * (j - 1) + q];
+        D[3 * j + q] = d + p.dt * tq;
+      }
+      r[j] = dot3(&D[3 * j], &D[3 * j]) - p.b * p.b;
+      gmax = std::max(gmax, fabs(r[j]));
+    }
+    if (gmax <= tol) break;
+    if (it == itermax) {
+      it = itermax + 1;
+      break;
+    }
+    for (int j = 0; j < m; j++) {
+      const double *Dj = &D[3 * j];
+      double s2[3] = {hA[3 * j] + hB[3 * j], hA[3 * j + 1] + hB[3 * j + 1], hA[3 * j + 2] + hB[3 * j + 2]};
+      di[j] = -2.0 * p.dt * dot3(Dj, s2);
+      lo[j] = j > 0 ? 2.0 * p.dt * dot3(Dj, &hB[3 * (j - 1)]) : 0.0;
+      up[j] = j + 1 < m ? 2.0 * p.dt * dot3(Dj, &hA[3 * (j + 1)]) : 0.0;
+    }
+    br_thomas(m, lo.data(), di.data(), up.data(), r.data());
+    for (int j = 0; j < m; j++) T[j] -= r[j];
+  }
+  for (int i = 0; i < N; i++)
+    for (int q = 0; q < 3; q++) {
+      double v = P[3 * i + q];
+      if (i < m) v += p.dt * T[i] * hA[3 * i + q];
+      if (i > 0) v -= p.dt * T[i - 1] * hB[3 * (i - 1) + q];
+      R[3 * i + q] = v;
+    }
+  return it;
+}
+
+// bending energy kb sum (1 - u_j.u_{j+1}) of one polymer
+double br_energy(const double *R, int N, double kb) {
+  BRGeom g;
+  g.compute(R, N, 0);
+  double e = 0.0;
+  for (int j = 0; j + 1 < g.m; j++) e += kb * (1.0 - dot3(&g.u[3 * j], &g.u[3 * (j + 1)]));
+  return e;
+}
+
 /* ----------------------------------------------------------- phases --- */
 void phase_start(orc_sys *s) {
   for (auto &f : s->fixes) {
@@ -1164,10 +1480,57 @@ void phase_pair(orc_sys *s) {
   reverse_comm(s);
 }
 
+BRParams br_params(const orc_sys *s, const FixDef &f) {
+  return BRParams{f.bondlength, f.zpar, f.zperp, f.bending, f.brtemp, s->dt, f.tether};
+}
+
+// FixAtomSemiFlexible::initial_integrate, src/fixatom_semiflexible.cpp:144-156
+void semiflex_first(orc_sys *s, FixDef &f) {
+  const BRParams p = br_params(s, f);
+  f.brstep++;
+  for (size_t c = 0; c < f.start.size(); c++) {
+    const int st = f.start[c], nb = f.end[c] - f.start[c];
+    if (nb <= 0) continue;
+    const size_t o = 3 * (size_t)st;
+    br_first_step(p, (uint64_t)(uint32_t)f.pseed[c], f.brstep, nb, &s->uxs[o], &s->Fs[o], &f.R0[o], &f.FB[o],
+                  &f.T[st]);
+    for (int j = st; j < st + nb; j++) map(s, &s->uxs[3 * j], s->images[j], &s->xs[3 * j]);
+  }
+}
+
+// FixAtomSemiFlexible::final_integrate, src/fixatom_semiflexible.cpp:158-191
+void semiflex_second(orc_sys *s, FixDef &f) {
+  const BRParams p = br_params(s, f);
+  const int itermax = 50;
+  int errflag = 0;
+  for (size_t c = 0; c < f.start.size(); c++) {
+    const int st = f.start[c], nb = f.end[c] - f.start[c];
+    if (nb <= 0) continue;
+    const size_t o = 3 * (size_t)st;
+    const int it = br_second_step(p, nb, &s->uxs[o], &s->Fs[o], &f.R0[o], &f.FB[o], &f.T[st], itermax);
+    for (int j = st; j < st + nb; j++) map(s, &s->uxs[3 * j], s->images[j], &s->xs[3 * j]);
+    if (it > itermax) errflag = 1;
+  }
+  if (errflag) throw std::runtime_error("Failed to maintain polymer bond-length constraints.");
+}
+
+// FixAtomSemiFlexible::check_bonds, src/fixatom_semiflexible.cpp:114-140
+void semiflex_check_bonds(const orc_sys *s, const FixDef &f) {
+  for (size_t c = 0; c < f.start.size(); c++)
+    for (int mu = f.start[c] + 1; mu < f.end[c]; mu++) {
+      const double *a = &s->uxs[3 * (size_t)mu], *b = &s->uxs[3 * (size_t)(mu - 1)];
+      const double d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+      if (fabs(sqrt(dot3(d, d)) - f.bondlength) > 1e-3)
+        throw std::runtime_error("Adjacent atom displacements do not match the FixAtomSemiFlexible "
+                                 "bondlength specified.");
+    }
+}
+
 void phase_integrate(orc_sys *s) {
   for (auto &f : s->fixes) {
     if (f.kind == FixDef::DRAG) drag_halfstep(s, f);
     else if (f.kind == FixDef::CHAIN) chain_halfstep(s, f);
+    else if (f.kind == FixDef::SEMIFLEX) semiflex_first(s, f);
   }
 }
 
@@ -1205,6 +1568,8 @@ void phase_final(orc_sys *s) {
       drag_halfstep(s, f);
     } else if (f.kind == FixDef::CHAIN) {
       chain_halfstep(s, f);
+    } else if (f.kind == FixDef::SEMIFLEX) {
+      semiflex_second(s, f);
     }
   }
 }
@@ -1495,6 +1860,62 @@ int orc_add_fix_chain(orc_sys *s, int nchunk, const int *start, const int *end, 
   s->fixes.push_back(f);
   return 0;
 }
+// fix atom/semiflexible/{notether,singletether,doubletether} <group> <seed>
+// bondlength b zeta_para zpar zeta_perp zperp bending kb temp T over group
+// chunks (one polymer each); per-polymer seeds as the reference draws them
+// (src/fixatom_semiflexible.cpp:44-80): uniform_int_distribution<int> on
+// mt19937(make_unique_seed(seed))
+int orc_add_fix_semiflexible(orc_sys *s, int tether, int nchunk, const int *start, const int *end, int seed,
+                             double bondlength, double zpar, double zperp, double bending, double temp) {
+  return guard(s, [&] {
+    if (tether < 0 || tether > 2) throw std::invalid_argument("tether must be 0, 1 or 2");
+    if (!(bondlength > 0) || !(zpar > 0) || !(zperp > 0) || temp < 0)
+      throw std::invalid_argument("bondlength, zeta_para, zeta_perp must be positive, temp non-negative");
+    FixDef f;
+    f.kind = FixDef::SEMIFLEX;
+    f.tether = tether;
+    f.start.assign(start, start + nchunk);
+    f.end.assign(end, end + nchunk);
+    f.bondlength = bondlength; f.zpar = zpar; f.zperp = zperp; f.bending = bending; f.brtemp = temp;
+    std::mt19937 gen;
+    std::uniform_int_distribution<int> integer_dist;
+    gen.seed(make_unique_seed(seed));
+    for (int c = 0; c < nchunk; c++) f.pseed.push_back(integer_dist(gen));
+    f.R0.assign(3 * (size_t)s->nowned, 0.0);
+    f.FB.assign(3 * (size_t)s->nowned, 0.0);
+    f.T.assign((size_t)s->nowned, 0.0);
+    s->fixes.push_back(f);
+  });
+}
+int orc_semiflexible_seeds(orc_sys *s, int ifix, int *out) {
+  return guard(s, [&] {
+    int k = 0;
+    for (auto &f : s->fixes)
+      if (f.kind == FixDef::SEMIFLEX && k++ == ifix) {
+        std::copy(f.pseed.begin(), f.pseed.end(), out);
+        return;
+      }
+    throw std::invalid_argument("no such semiflexible fix");
+  });
+}
+// one bead-rod timestep (first_step, then second_step with the same
+// external forces) of one polymer; *iters = the corrector's iterations
+int orc_beadrod_step(int tether, int N, double *R, const double *Fext, uint64_t seed, int64_t step, double b,
+                     double zpar, double zperp, double bending, double temp, double dt, int itermax, int *iters) {
+  return guard(nullptr, [&] {
+    if (N < 1 || tether < 0 || tether > 2) throw std::invalid_argument("bad polymer");
+    const BRParams p{b, zpar, zperp, bending, temp, dt, tether};
+    std::vector<double> R0(3 * (size_t)N), FB(3 * (size_t)N), T((size_t)N, 0.0);
+    br_first_step(p, seed, step, N, R, Fext, R0.data(), FB.data(), T.data());
+    *iters = br_second_step(p, N, R, Fext, R0.data(), FB.data(), T.data(), itermax);
+  });
+}
+int orc_philox4x64_10(const uint64_t *ctr, const uint64_t *key, uint64_t *out) {
+  uint64_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  philox4x64_10_host(c, key[0], key[1]);
+  std::copy(c, c + 4, out);
+  return 0;
+}
 // Neighbor::setup, src/neighbor.cpp:39-68, then CommBrick::setup
 int orc_neighbor(orc_sys *s, double skin) {
   return guard(s, [&] {
@@ -1521,6 +1942,8 @@ int orc_setup(orc_sys *s, double dt) {
       for (int i = 0; i < s->nowned; i++) unmap(s, &s->xs[3 * i], s->images[i], &s->uxs[3 * i]);
     }
     std::fill(s->Fs.begin(), s->Fs.end(), 0.0);
+    for (auto &f : s->fixes)
+      if (f.kind == FixDef::SEMIFLEX) semiflex_check_bonds(s, f);
     s->setup_done = true;
   });
 }
@@ -1746,6 +2169,10 @@ int orc_energy(orc_sys *s, double *out) {
       if (f.kind == FixDef::CHAIN)
         for (size_t c = 0; c < f.start.size(); c++)
           ech += chain_energy(&s->uxs[3 * (size_t)f.start[c]], f.end[c] - f.start[c], f.kbond, f.r0, f.kbend);
+      else if (f.kind == FixDef::SEMIFLEX)
+        for (size_t c = 0; c < f.start.size(); c++)
+          if (f.end[c] > f.start[c])
+            ech += br_energy(&s->uxs[3 * (size_t)f.start[c]], f.end[c] - f.start[c], f.bending);
     out[0] = efh; out[1] = ega; out[2] = elj; out[3] = ech;
   });
 }

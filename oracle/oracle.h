@@ -1,8 +1,8 @@
 /*
  * oracle.h -- C API of the CPU FP64 oracle for PHAFD's per-timestep hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  The oracle is a plain, single-threaded C++
- * restatement of the reference's CPU algorithm (one MPI rank holding the whole
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a plain C++ (OpenMP over FFT
+ * lines and pointwise loops only) restatement of the reference's CPU algorithm (one MPI rank holding the whole
  * box).  It is the checker that the HIP path is compared against: only the
  * tests, the smoke check and the CPU-baseline leg of the benchmark may load it.
  * Nothing in the product links or calls it.
@@ -50,6 +50,13 @@ int orc_mt19937_uniform(int seed, int n, double *out);
 int orc_make_unique_seed(int seed);
 /* OpenMP threads of the FFT lines and pointwise loops (results do not
  * depend on the count: every line and point is computed alone) */
+/* bead-rod semiflexible polymers (tether 0 none, 1 bead 0, 2 both ends) */
+int orc_add_fix_semiflexible(orc_sys *s, int tether, int nchunk, const int *start, const int *end, int seed,
+                             double bondlength, double zpar, double zperp, double bending, double temp);
+int orc_semiflexible_seeds(orc_sys *s, int ifix, int *out);
+int orc_philox4x64_10(const uint64_t *ctr, const uint64_t *key, uint64_t *out);
+int orc_beadrod_step(int tether, int N, double *R, const double *Fext, uint64_t seed, int64_t step, double b,
+                     double zpar, double zperp, double bending, double temp, double dt, int itermax, int *iters);
 int orc_set_threads(int n);
 int orc_threads(void);
 int orc_mt19937_raw(int seed, int n, uint32_t *out);

@@ -48,6 +48,13 @@ def lib():
         L.orc_add_fix_gradphi.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_add_fix_drag.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip, ctypes.c_double]
         L.orc_add_fix_chain.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip] + [ctypes.c_double] * 4
+        L.orc_add_fix_semiflexible.argtypes = ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.c_int]
+                                               + [ctypes.c_double] * 5)
+        L.orc_semiflexible_seeds.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
+        _u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.orc_philox4x64_10.argtypes = [_u64p, _u64p, _u64p]
+        L.orc_beadrod_step.argtypes = ([ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_uint64, ctypes.c_int64]
+                                       + [ctypes.c_double] * 6 + [ctypes.c_int, _ip])
         L.orc_neighbor.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_setup.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -180,6 +187,29 @@ def make_unique_seed(seed):
     return lib().orc_make_unique_seed(seed)
 
 
+def philox4x64_10(counter, key):
+    """The four output words of Philox4x64-10 for one counter and key."""
+    c = np.ascontiguousarray(counter, dtype=np.uint64)
+    k = np.ascontiguousarray(key, dtype=np.uint64)
+    out = np.zeros(4, dtype=np.uint64)
+    p = ctypes.POINTER(ctypes.c_uint64)
+    lib().orc_philox4x64_10(c.ctypes.data_as(p), k.ctypes.data_as(p), out.ctypes.data_as(p))
+    return out
+
+
+def beadrod_step(R, Fext, tether=0, seed=1, step=1, b=1.0, zpar=1.0, zperp=1.0, bending=0.0, temp=0.0, dt=1e-3,
+                 itermax=50):
+    """One bead-rod timestep of one polymer; returns (new R, corrector iterations)."""
+    R = np.array(R, dtype=np.float64, order="C")
+    F = np.ascontiguousarray(Fext, dtype=np.float64)
+    it = np.zeros(1, dtype=np.int32)
+    rc = lib().orc_beadrod_step(tether, len(R), dptr(R), dptr(F), seed, step, b, zpar, zperp, bending, temp, dt,
+                                itermax, iptr(it))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error(None).decode())
+    return R, int(it[0])
+
+
 def set_threads(n):
     """OpenMP threads of the oracle's FFT lines and pointwise loops."""
     if lib().orc_set_threads(int(n)) != 0:
@@ -308,6 +338,18 @@ class System:
         s = np.ascontiguousarray(starts, dtype=np.int32)
         e = np.ascontiguousarray(ends, dtype=np.int32)
         self._chk(lib().orc_add_fix_chain(self.h, len(s), iptr(s), iptr(e), r0, kbond, kbend, drag))
+
+    def fix_semiflexible(self, tether, starts, ends, seed, bondlength, zeta_para, zeta_perp, bending, temp):
+        """fix atom/semiflexible/{notether,singletether,doubletether} (tether 0, 1, 2)."""
+        s = np.ascontiguousarray(starts, dtype=np.int32)
+        e = np.ascontiguousarray(ends, dtype=np.int32)
+        self._chk(lib().orc_add_fix_semiflexible(self.h, tether, len(s), iptr(s), iptr(e), seed, bondlength,
+                                                 zeta_para, zeta_perp, bending, temp))
+
+    def semiflexible_seeds(self, ifix, npoly):
+        out = np.zeros(npoly, dtype=np.int32)
+        self._chk(lib().orc_semiflexible_seeds(self.h, ifix, iptr(out)))
+        return out
 
     def neighbor(self, skin):
         self._chk(lib().orc_neighbor(self.h, skin))

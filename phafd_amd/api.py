@@ -84,6 +84,8 @@ def lib():
         "phafd_hip_fix_gradphi": ([V, ctypes.c_int], ctypes.c_int),
         "phafd_hip_fix_drag": ([V, ctypes.c_int, _ip, _ip, ctypes.c_double], ctypes.c_int),
         "phafd_hip_fix_chain": ([V, ctypes.c_int, _ip, _ip] + [ctypes.c_double] * 4, ctypes.c_int),
+        "phafd_hip_fix_semiflexible": ([V, ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.c_int] + [ctypes.c_double] * 5,
+                                       ctypes.c_int),
         "phafd_hip_neighbor": ([V, ctypes.c_double], ctypes.c_int),
         "phafd_hip_setup": ([V, ctypes.c_double], ctypes.c_int),
         "phafd_hip_run": ([V, ctypes.c_int], ctypes.c_int),
@@ -600,6 +602,13 @@ class System:
         s = np.ascontiguousarray(starts, dtype=np.int32)
         e = np.ascontiguousarray(ends, dtype=np.int32)
         self._c(lib().phafd_hip_fix_chain(self.h, len(s), _i(s), _i(e), r0, kbond, kbend, drag))
+
+    def fix_semiflexible(self, tether, starts, ends, seed, bondlength, zeta_para, zeta_perp, bending, temp):
+        """fix atom/semiflexible/{notether,singletether,doubletether} (tether 0, 1, 2) over group chunks."""
+        s = np.ascontiguousarray(starts, dtype=np.int32)
+        e = np.ascontiguousarray(ends, dtype=np.int32)
+        self._c(lib().phafd_hip_fix_semiflexible(self.h, tether, len(s), _i(s), _i(e), seed, bondlength, zeta_para,
+                                                 zeta_perp, bending, temp))
 
     def neighbor(self, skin):
         self._c(lib().phafd_hip_neighbor(self.h, skin))

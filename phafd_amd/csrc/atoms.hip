@@ -6,6 +6,7 @@
 // reference's exact expressions (built with -ffp-contract=off), so lists and
 // bins are bit-identical to the reference's on the same positions.
 #include "atoms.hpp"
+#include "spectral_ops.hpp"
 
 namespace phafd {
 namespace {
@@ -1116,6 +1117,368 @@ __global__ void k_chain_energy(int n, const int *members, const int *cstart, con
   atomicAdd(energy, e);
 }
 
+// ------------------------------------------------------- bead-rod chains
+// fix atom/semiflexible/{notether,singletether,doubletether}
+// (src/fixatom_semiflexible.cpp:144-191): the bead-rod Brownian dynamics of
+// Montesi, Morse & Pasquali (J. Chem. Phys. 122, 084903, 2005) -- rigid rods,
+// bending kb (1 - u_j.u_{j+1}), mobility w_i [t t / zeta_para + (I - t t) /
+// zeta_perp] about the bead tangent, the metric pseudo-potential
+// (kT/2) ln det A of the rod-tension matrix A = G H G^T, Brownian force
+// sqrt(2 kT / dt) zeta^(1/2) xi at the step start, midpoint predictor with
+// first-order tensions, corrector with Newton-iterated tensions (at most
+// itermax).  One 64-thread block per chain; the chain lives in LDS (29
+// doubles per bead), serial tridiagonal sweeps on lane 0.
+
+constexpr int BR_PER_BEAD = 29;
+constexpr double BR_TOL = 1e-12;   // |R_{j+1} - R_j|^2 - b^2 within BR_TOL b^2
+constexpr unsigned long long BR_KEY1 = 0x4245414452ull;
+
+struct BRLds {
+  double *Rx, *Ry, *Rz, *Fx, *Fy, *Fz, *tx, *ty, *tz;           // per bead
+  double *ux, *uy, *uz, *il, *cs, *T;                            // per bond
+  double *hAx, *hAy, *hAz, *hBx, *hBy, *hBz, *Dx, *Dy, *Dz;      // per bond
+  double *r, *lo, *di, *up, *cp;                                 // per bond (tridiagonal)
+  __device__ void carve(double *sh, int L) {
+    double **a[BR_PER_BEAD] = {&Rx, &Ry, &Rz, &Fx, &Fy, &Fz, &tx, &ty, &tz, &ux, &uy, &uz, &il, &cs, &T,
+                               &hAx, &hAy, &hAz, &hBx, &hBy, &hBz, &Dx, &Dy, &Dz, &r, &lo, &di, &up, &cp};
+    for (int k = 0; k < BR_PER_BEAD; k++) *a[k] = sh + (size_t)k * L;
+  }
+};
+
+__device__ __forceinline__ double br_w(const BRParamsDev &p, int i, int N) {
+  return ((p.tether >= 1 && i == 0) || (p.tether >= 2 && i == N - 1)) ? 0.0 : 1.0;
+}
+// out = H_i v
+__device__ __forceinline__ void br_mob(const BRParamsDev &p, const BRLds &S, int i, int N, double vx, double vy,
+                                       double vz, double &ox, double &oy, double &oz) {
+  if (br_w(p, i, N) == 0.0) { ox = oy = oz = 0.0; return; }
+  const double t0 = S.tx[i], t1 = S.ty[i], t2 = S.tz[i];
+  const double tv = t0 * vx + t1 * vy + t2 * vz;
+  const double a = 1.0 / p.zperp, c = 1.0 / p.zpar - 1.0 / p.zperp;
+  ox = vx * a + c * tv * t0;
+  oy = vy * a + c * tv * t1;
+  oz = vz * a + c * tv * t2;
+}
+
+// bonds, tangents and cosines of the positions in S.R
+__device__ void br_geom(BRLds &S, int N) {
+  const int m = N - 1, tid = threadIdx.x;
+  for (int j = tid; j < m; j += blockDim.x) {
+    const double d0 = S.Rx[j + 1] - S.Rx[j], d1 = S.Ry[j + 1] - S.Ry[j], d2 = S.Rz[j + 1] - S.Rz[j];
+    const double l = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    const double il = 1.0 / l;
+    S.il[j] = il;
+    S.ux[j] = d0 * il;
+    S.uy[j] = d1 * il;
+    S.uz[j] = d2 * il;
+  }
+  __syncthreads();
+  for (int i = tid; i < N; i += blockDim.x) {
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    if (m > 0) {
+      if (i == 0 || i == N - 1) {
+        const int j = i == 0 ? 0 : m - 1;
+        t0 = S.ux[j]; t1 = S.uy[j]; t2 = S.uz[j];
+      } else {
+        const double s0 = S.ux[i - 1] + S.ux[i], s1 = S.uy[i - 1] + S.uy[i], s2 = S.uz[i - 1] + S.uz[i];
+        const double n = sqrt(s0 * s0 + s1 * s1 + s2 * s2);
+        if (n > 1e-12) { t0 = s0 / n; t1 = s1 / n; t2 = s2 / n; }
+        else { t0 = S.ux[i]; t1 = S.uy[i]; t2 = S.uz[i]; }
+      }
+    }
+    S.tx[i] = t0; S.ty[i] = t1; S.tz[i] = t2;
+  }
+  for (int j = tid; j + 1 < m; j += blockDim.x)
+    S.cs[j] = S.ux[j] * S.ux[j + 1] + S.uy[j] * S.uy[j + 1] + S.uz[j] * S.uz[j + 1];
+  __syncthreads();
+}
+
+// lane 0: lo_j x_{j-1} + di_j x_j + up_j x_{j+1} = r_j, x into r (Thomas);
+// a vanishing pivot leaves its row's unknown 0
+__device__ void br_thomas(BRLds &S, int m) {
+  if (threadIdx.x == 0) {
+    double cprev = 0.0, rprev = 0.0;
+    for (int j = 0; j < m; j++) {
+      const double den = S.di[j] - (j > 0 ? S.lo[j] * cprev : 0.0);
+      double cj, rj;
+      if (fabs(den) > 1e-300) {
+        cj = S.up[j] / den;
+        rj = (S.r[j] - (j > 0 ? S.lo[j] * rprev : 0.0)) / den;
+      } else {
+        cj = 0.0;
+        rj = 0.0;
+      }
+      S.cp[j] = cj;
+      S.r[j] = rj;
+      cprev = cj;
+      rprev = rj;
+    }
+    for (int j = m - 2; j >= 0; j--) {
+      rprev = S.r[j] - S.cp[j] * rprev;
+      S.r[j] = rprev;
+    }
+  }
+  __syncthreads();
+}
+
+// S.F += bending and metric pseudo-potential forces (the tension matrix A
+// as a function of the cosines: A_jj = (w_j + w_{j+1}) / zeta_perp + kh
+// (w_j (t_j.u_j)^2 + w_{j+1} (t_{j+1}.u_j)^2), A_{j,j+1} = -w_{j+1} (c_j /
+// zeta_perp + kh (1 + c_j) / 2), kh = 1/zeta_para - 1/zeta_perp; the forward
+// and backward pivots give diag and first off-diagonal of A^-1).  The
+// scratch arrays of the Newton iteration hold a, off, pivots and weights.
+__device__ void br_bend_ps(const BRParamsDev &p, BRLds &S, int N) {
+  const int m = N - 1, tid = threadIdx.x;
+  if (m < 2) return;
+  double *a = S.Dx, *off = S.Dy, *al = S.Dz, *be = S.lo, *dinv = S.di, *e = S.up;
+  const double ip = 1.0 / p.zperp, kh = 1.0 / p.zpar - 1.0 / p.zperp;
+  for (int j = tid; j < m; j += blockDim.x) {
+    const double aL = j == 0 ? 1.0 : 0.5 * (1.0 + S.cs[j - 1]);
+    const double aR = j + 1 == N - 1 ? 1.0 : 0.5 * (1.0 + S.cs[j]);
+    const double w0 = br_w(p, j, N), w1 = br_w(p, j + 1, N);
+    a[j] = (w0 + w1) * ip + kh * (w0 * aL + w1 * aR);
+    off[j] = j + 1 < m ? -w1 * (S.cs[j] * ip + kh * 0.5 * (1.0 + S.cs[j])) : 0.0;
+  }
+  __syncthreads();
+  // forward pivots on lane 0, backward pivots on lane 1
+  if (tid == 0) {
+    double v = a[0];
+    al[0] = v;
+    for (int j = 1; j < m; j++) {
+      v = fabs(v) > 1e-300 ? a[j] - off[j - 1] * off[j - 1] / v : a[j];
+      al[j] = v;
+    }
+  } else if (tid == 1) {
+    double v = a[m - 1];
+    be[m - 1] = v;
+    for (int j = m - 2; j >= 0; j--) {
+      v = fabs(v) > 1e-300 ? a[j] - off[j] * off[j] / v : a[j];
+      be[j] = v;
+    }
+  }
+  __syncthreads();
+  for (int j = tid; j < m; j += blockDim.x) {
+    const double den = al[j] + be[j] - a[j];
+    dinv[j] = fabs(den) > 1e-300 ? 1.0 / den : 0.0;
+  }
+  __syncthreads();
+  for (int j = tid; j + 1 < m; j += blockDim.x) {
+    double ej = p.kb;
+    if (p.temp != 0.0) {
+      const double oinv = fabs(be[j + 1]) > 1e-300 ? -off[j] / be[j + 1] * dinv[j] : 0.0;
+      const double w1 = br_w(p, j + 1, N);
+      const double dlog = 2.0 * oinv * (-w1 * (ip + 0.5 * kh)) + 0.5 * kh * w1 * (dinv[j] + dinv[j + 1]);
+      ej -= 0.5 * p.temp * dlog;
+    }
+    e[j] = ej;
+  }
+  __syncthreads();
+  // bead i gathers, in the order of the bond-pair loop: pair i-2 (+e q),
+  // pair i-1 (+e (p - q)), pair i (-e p)
+  for (int i = tid; i < N; i += blockDim.x) {
+    double f[3] = {S.Fx[i], S.Fy[i], S.Fz[i]};
+    for (int j = i - 2; j <= i; j++) {
+      if (j < 0 || j + 1 >= m) continue;
+      const double u0[3] = {S.ux[j], S.uy[j], S.uz[j]}, u1[3] = {S.ux[j + 1], S.uy[j + 1], S.uz[j + 1]};
+      const double c = S.cs[j];
+      for (int q = 0; q < 3; q++) {
+        const double pj = S.il[j] * (u1[q] - c * u0[q]);
+        const double qj = S.il[j + 1] * (u0[q] - c * u1[q]);
+        if (j == i - 2) f[q] += e[j] * qj;
+        else if (j == i - 1) f[q] += e[j] * (pj - qj);
+        else f[q] -= e[j] * pj;
+      }
+    }
+    S.Fx[i] = f[0]; S.Fy[i] = f[1]; S.Fz[i] = f[2];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double4 br_map(double x, double y, double z, int img, const Box &bx) {
+  const int xb = (img & IMGMASK) - IMGMAX, yb = (img >> IMGBITS & IMGMASK) - IMGMAX, zb = (img >> IMG2BITS) - IMGMAX;
+  return make_double4(x - xb * bx.period[0], y - yb * bx.period[1], z - zb * bx.period[2], 0.0);
+}
+
+// FixAtomSemiFlexible::initial_integrate -> first_step
+__global__ __launch_bounds__(64) void k_br_first(const int *chs, const int *che, const int *seeds, const double4 *F,
+                                                 double4 *ux, double4 *x, const int *image, double4 *R0,
+                                                 double4 *FB, double *Tg, BRParamsDev p, Box bx, int L) {
+  extern __shared__ double sh[];
+  BRLds S;
+  S.carve(sh, L);
+  const int s = chs[blockIdx.x], N = che[blockIdx.x] - s, m = N - 1, tid = threadIdx.x;
+  for (int i = tid; i < N; i += blockDim.x) {
+    const double4 u = ux[s + i];
+    S.Rx[i] = u.x; S.Ry[i] = u.y; S.Rz[i] = u.z;
+    R0[s + i] = u;
+  }
+  __syncthreads();
+  br_geom(S, N);
+  const double amp = p.temp > 0 ? sqrt(2.0 * p.temp / p.dt) : 0.0;
+  const double s12 = sqrt(12.0), sp = sqrt(p.zpar), sq = sqrt(p.zperp);
+  const unsigned long long key0 = (unsigned long long)(unsigned)seeds[blockIdx.x];
+  for (int i = tid; i < N; i += blockDim.x) {
+    double xi[3] = {0.0, 0.0, 0.0};
+    if (amp != 0.0) {
+      unsigned long long w[4] = {(unsigned long long)i, p.step, 0ull, 0ull};
+      philox4x64_10(w, key0, BR_KEY1);
+      for (int q = 0; q < 3; q++) xi[q] = s12 * unit_centered(w[q]);
+    }
+    const double t[3] = {S.tx[i], S.ty[i], S.tz[i]};
+    const double tx = t[0] * xi[0] + t[1] * xi[1] + t[2] * xi[2];
+    double fb[3];
+    for (int q = 0; q < 3; q++) fb[q] = amp * (sp * tx * t[q] + sq * (xi[q] - tx * t[q]));
+    FB[s + i] = make_double4(fb[0], fb[1], fb[2], 0.0);
+    const double4 f = F[s + i];
+    S.Fx[i] = f.x + fb[0];
+    S.Fy[i] = f.y + fb[1];
+    S.Fz[i] = f.z + fb[2];
+  }
+  __syncthreads();
+  br_bend_ps(p, S, N);
+  // first-order tensions: u_j.(V_{j+1} - V_j) = 0
+  for (int j = tid; j < m; j += blockDim.x) {
+    const double u0 = S.ux[j], u1 = S.uy[j], u2 = S.uz[j];
+    double h0x, h0y, h0z, h1x, h1y, h1z;
+    br_mob(p, S, j, N, u0, u1, u2, h0x, h0y, h0z);
+    br_mob(p, S, j + 1, N, u0, u1, u2, h1x, h1y, h1z);
+    S.di[j] = -((u0 * h1x + u1 * h1y + u2 * h1z) + (u0 * h0x + u1 * h0y + u2 * h0z));
+    double lo = 0.0, up = 0.0;
+    if (j > 0) {
+      double hx, hy, hz;
+      br_mob(p, S, j, N, S.ux[j - 1], S.uy[j - 1], S.uz[j - 1], hx, hy, hz);
+      lo = u0 * hx + u1 * hy + u2 * hz;
+    }
+    if (j + 1 < m) {
+      double hx, hy, hz;
+      br_mob(p, S, j + 1, N, S.ux[j + 1], S.uy[j + 1], S.uz[j + 1], hx, hy, hz);
+      up = u0 * hx + u1 * hy + u2 * hz;
+    }
+    S.lo[j] = lo;
+    S.up[j] = up;
+    double f0x, f0y, f0z, f1x, f1y, f1z;
+    br_mob(p, S, j, N, S.Fx[j], S.Fy[j], S.Fz[j], f0x, f0y, f0z);
+    br_mob(p, S, j + 1, N, S.Fx[j + 1], S.Fy[j + 1], S.Fz[j + 1], f1x, f1y, f1z);
+    const double dfx = f1x - f0x, dfy = f1y - f0y, dfz = f1z - f0z;
+    S.r[j] = -(u0 * dfx + u1 * dfy + u2 * dfz);
+  }
+  __syncthreads();
+  br_thomas(S, m);
+  for (int i = tid; i < N; i += blockDim.x) {
+    double tot[3] = {S.Fx[i], S.Fy[i], S.Fz[i]};
+    const double uu[3] = {i < m ? S.ux[i] : 0.0, i < m ? S.uy[i] : 0.0, i < m ? S.uz[i] : 0.0};
+    const double ul[3] = {i > 0 ? S.ux[i - 1] : 0.0, i > 0 ? S.uy[i - 1] : 0.0, i > 0 ? S.uz[i - 1] : 0.0};
+    for (int q = 0; q < 3; q++) {
+      if (i < m) tot[q] += S.r[i] * uu[q];
+      if (i > 0) tot[q] -= S.r[i - 1] * ul[q];
+    }
+    double vx, vy, vz;
+    br_mob(p, S, i, N, tot[0], tot[1], tot[2], vx, vy, vz);
+    const double4 r0 = R0[s + i];
+    const double nx = r0.x + 0.5 * p.dt * vx, ny = r0.y + 0.5 * p.dt * vy, nz = r0.z + 0.5 * p.dt * vz;
+    ux[s + i] = make_double4(nx, ny, nz, r0.w);
+    x[s + i] = br_map(nx, ny, nz, image[s + i], bx);
+    if (i < m) Tg[s + i] = S.r[i];
+  }
+}
+
+// FixAtomSemiFlexible::final_integrate -> second_step(..., itermax): R1 = R0
+// + dt V(R*), tensions by Newton from the first step's; fail |= 1 when the
+// rods were not brought to length within itermax iterations
+__global__ __launch_bounds__(64) void k_br_second(const int *chs, const int *che, const double4 *F, double4 *ux,
+                                                  double4 *x, const int *image, const double4 *R0,
+                                                  const double4 *FB, double *Tg, int *fail, BRParamsDev p, Box bx,
+                                                  int L) {
+  extern __shared__ double sh[];
+  BRLds S;
+  S.carve(sh, L);
+  const int s = chs[blockIdx.x], N = che[blockIdx.x] - s, m = N - 1, tid = threadIdx.x;
+  for (int i = tid; i < N; i += blockDim.x) {
+    const double4 u = ux[s + i];
+    S.Rx[i] = u.x; S.Ry[i] = u.y; S.Rz[i] = u.z;
+  }
+  __syncthreads();
+  br_geom(S, N);
+  for (int i = tid; i < N; i += blockDim.x) {
+    const double4 f = F[s + i], fb = FB[s + i];
+    S.Fx[i] = f.x + fb.x;
+    S.Fy[i] = f.y + fb.y;
+    S.Fz[i] = f.z + fb.z;
+  }
+  __syncthreads();
+  br_bend_ps(p, S, N);
+  // P = R0 + dt H F (into R), hA_j = H_j u_j, hB_j = H_{j+1} u_j, T from the first step
+  for (int i = tid; i < N; i += blockDim.x) {
+    double vx, vy, vz;
+    br_mob(p, S, i, N, S.Fx[i], S.Fy[i], S.Fz[i], vx, vy, vz);
+    const double4 r0 = R0[s + i];
+    S.Rx[i] = r0.x + p.dt * vx;
+    S.Ry[i] = r0.y + p.dt * vy;
+    S.Rz[i] = r0.z + p.dt * vz;
+  }
+  for (int j = tid; j < m; j += blockDim.x) {
+    br_mob(p, S, j, N, S.ux[j], S.uy[j], S.uz[j], S.hAx[j], S.hAy[j], S.hAz[j]);
+    br_mob(p, S, j + 1, N, S.ux[j], S.uy[j], S.uz[j], S.hBx[j], S.hBy[j], S.hBz[j]);
+    S.T[j] = Tg[s + j];
+  }
+  __syncthreads();
+  const double tol = BR_TOL * p.b * p.b, dt = p.dt;
+  int it = 0;
+  for (;; it++) {
+    double gmax = 0.0;
+    for (int j = tid; j < m; j += blockDim.x) {
+      double tq0 = -S.T[j] * S.hBx[j] - S.T[j] * S.hAx[j];
+      double tq1 = -S.T[j] * S.hBy[j] - S.T[j] * S.hAy[j];
+      double tq2 = -S.T[j] * S.hBz[j] - S.T[j] * S.hAz[j];
+      if (j + 1 < m) {
+        tq0 += S.T[j + 1] * S.hAx[j + 1];
+        tq1 += S.T[j + 1] * S.hAy[j + 1];
+        tq2 += S.T[j + 1] * S.hAz[j + 1];
+      }
+      if (j > 0) {
+        tq0 += S.T[j - 1] * S.hBx[j - 1];
+        tq1 += S.T[j - 1] * S.hBy[j - 1];
+        tq2 += S.T[j - 1] * S.hBz[j - 1];
+      }
+      const double d0 = (S.Rx[j + 1] - S.Rx[j]) + dt * tq0;
+      const double d1 = (S.Ry[j + 1] - S.Ry[j]) + dt * tq1;
+      const double d2 = (S.Rz[j + 1] - S.Rz[j]) + dt * tq2;
+      S.Dx[j] = d0; S.Dy[j] = d1; S.Dz[j] = d2;
+      const double g = (d0 * d0 + d1 * d1 + d2 * d2) - p.b * p.b;
+      S.r[j] = g;
+      gmax = fmax(gmax, fabs(g));
+    }
+    for (int o = 32; o > 0; o >>= 1) gmax = fmax(gmax, __shfl_xor(gmax, o, 64));
+    if (gmax <= tol) break;                      // uniform: every lane holds the block maximum
+    if (it == p.itermax) { it = p.itermax + 1; break; }
+    __syncthreads();
+    for (int j = tid; j < m; j += blockDim.x) {
+      const double d0 = S.Dx[j], d1 = S.Dy[j], d2 = S.Dz[j];
+      S.di[j] = -2.0 * dt * (d0 * (S.hAx[j] + S.hBx[j]) + d1 * (S.hAy[j] + S.hBy[j]) + d2 * (S.hAz[j] + S.hBz[j]));
+      S.lo[j] = j > 0 ? 2.0 * dt * (d0 * S.hBx[j - 1] + d1 * S.hBy[j - 1] + d2 * S.hBz[j - 1]) : 0.0;
+      S.up[j] = j + 1 < m ? 2.0 * dt * (d0 * S.hAx[j + 1] + d1 * S.hAy[j + 1] + d2 * S.hAz[j + 1]) : 0.0;
+    }
+    __syncthreads();
+    br_thomas(S, m);
+    for (int j = tid; j < m; j += blockDim.x) S.T[j] -= S.r[j];
+    __syncthreads();
+  }
+  __syncthreads();
+  for (int i = tid; i < N; i += blockDim.x) {
+    double v[3] = {S.Rx[i], S.Ry[i], S.Rz[i]};
+    if (i < m) { v[0] += dt * S.T[i] * S.hAx[i]; v[1] += dt * S.T[i] * S.hAy[i]; v[2] += dt * S.T[i] * S.hAz[i]; }
+    if (i > 0) {
+      v[0] -= dt * S.T[i - 1] * S.hBx[i - 1];
+      v[1] -= dt * S.T[i - 1] * S.hBy[i - 1];
+      v[2] -= dt * S.T[i - 1] * S.hBz[i - 1];
+    }
+    const double w = ux[s + i].w;
+    ux[s + i] = make_double4(v[0], v[1], v[2], w);
+    x[s + i] = br_map(v[0], v[1], v[2], image[s + i], bx);
+  }
+  if (tid == 0 && it > p.itermax) atomicOr(fail, 1);
+}
+
 __global__ void k_fill_i32(int n, int *p, int v) { GRID_STRIDE(i, n) p[i] = v; }
 __global__ void k_fill_d4(int n, double4 *p) { GRID_STRIDE(i, n) p[i] = make_double4(0, 0, 0, 0); }
 
@@ -1424,6 +1787,32 @@ void launch_chain_energy(int n, const int *members, const int *cstart, const int
                          const ChainParams &cp, double *energy, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_chain_energy, dim3(nblk(n)), dim3(TPB), 0, s, n, members, cstart, cend, ux, cp, energy);
+  HIPCHK(hipGetLastError());
+}
+size_t br_lds_bytes(int maxlen) { return sizeof(double) * BR_PER_BEAD * (size_t)maxlen; }
+void br_lds_allow(const void *k, size_t bytes) {
+  if (bytes > 65536) HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+void launch_br_first(int nchains, int maxlen, const int *chs, const int *che, const int *seeds, const double4 *F,
+                     double4 *ux, double4 *x, const int *image, double4 *R0, double4 *FB, double *T,
+                     const BRParamsDev &p, const Box &b, hipStream_t s) {
+  if (nchains <= 0 || maxlen <= 0) return;
+  if (maxlen > BR_MAXLEN) throw std::invalid_argument("bead-rod chains longer than 700 beads are not supported");
+  const size_t lds = br_lds_bytes(maxlen);
+  br_lds_allow((const void *)k_br_first, lds);
+  hipLaunchKernelGGL(k_br_first, dim3(nchains), dim3(64), lds, s, chs, che, seeds, F, ux, x, image, R0, FB, T, p, b,
+                     maxlen);
+  HIPCHK(hipGetLastError());
+}
+void launch_br_second(int nchains, int maxlen, const int *chs, const int *che, const double4 *F, double4 *ux,
+                      double4 *x, const int *image, const double4 *R0, const double4 *FB, double *T, int *fail,
+                      const BRParamsDev &p, const Box &b, hipStream_t s) {
+  if (nchains <= 0 || maxlen <= 0) return;
+  if (maxlen > BR_MAXLEN) throw std::invalid_argument("bead-rod chains longer than 700 beads are not supported");
+  const size_t lds = br_lds_bytes(maxlen);
+  br_lds_allow((const void *)k_br_second, lds);
+  hipLaunchKernelGGL(k_br_second, dim3(nchains), dim3(64), lds, s, chs, che, F, ux, x, image, R0, FB, T, fail, p,
+                     b, maxlen);
   HIPCHK(hipGetLastError());
 }
 void launch_fill_i32(int n, int *p, int v, hipStream_t s) {

@@ -156,6 +156,22 @@ void launch_chain_step(int nchains, int maxlen, const int *chs, const int *che, 
                        int halves = 1, StepGate gate = StepGate());
 void launch_chain_energy(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,
                          const ChainParams &cp, double *energy, hipStream_t s);
+// atom/semiflexible bead-rod polymers (one chain per block, in LDS)
+constexpr int BR_MAXLEN = 700;
+struct BRParamsDev {
+  double b, zpar, zperp, kb, temp, dt;
+  int tether, itermax;              // tether 0 none, 1 bead 0, 2 both ends
+  unsigned long long step;          // first_step calls (noise counter)
+};
+// first_step: ux (R0 -> R*), x = map(ux); R0, FB (Brownian force) and T
+// (first-order tensions) kept per atom for second_step
+void launch_br_first(int nchains, int maxlen, const int *chs, const int *che, const int *seeds, const double4 *F,
+                     double4 *ux, double4 *x, const int *image, double4 *R0, double4 *FB, double *T,
+                     const BRParamsDev &p, const Box &b, hipStream_t s);
+// second_step: ux (R* -> R1), x = map(ux); fail |= 1 past itermax iterations
+void launch_br_second(int nchains, int maxlen, const int *chs, const int *che, const double4 *F, double4 *ux,
+                      double4 *x, const int *image, const double4 *R0, const double4 *FB, double *T, int *fail,
+                      const BRParamsDev &p, const Box &b, hipStream_t s);
 void launch_fill_i32(int n, int *p, int v, hipStream_t s);
 void launch_fill_d4(int n, double4 *p, hipStream_t s);
 void launch_add4(int n, const double4 *a, double4 *b, hipStream_t s);   // b.xyz += a.xyz

@@ -12,6 +12,9 @@
 #include <thread>
 
 #include "../../include/phafd_hip.h"
+#include <cstdio>
+#include <random>
+
 #include "../../include/phafd_host.h"
 #include "atoms.hpp"
 #include "internal.hpp"
@@ -39,7 +42,7 @@ struct PairDef {
 bool bead_pair(const PairDef &p) { return p.style == 2 || p.style == 4; }
 
 struct FixDef {
-  enum Kind { CONJ, FH, GRADPHI, DRAG, CHAIN } kind;
+  enum Kind { CONJ, FH, GRADPHI, DRAG, CHAIN, SEMIFLEX } kind;
   double mobility = 1, temp = 1, volFH = 1, gamma = 1, chi = 1;
   int seed = 0, noise_mode = 0, once = 0;
   unsigned long long noise_key = 0;   // device noise: make_unique_seed(seed)
@@ -48,6 +51,13 @@ struct FixDef {
   int nmem = 0, nchain = 0, maxlen = 0;
   DBuf<int> members, cstart, cend;
   DBuf<int> chs, che;   // chains (group chunks) as atom ranges [chs, che)
+  // atom/semiflexible/*: bead-rod polymers, one per group chunk
+  int tether = 0;
+  double bondlength = 1, zpar = 1, zperp = 1, bending = 0, brtemp = 0;
+  DBuf<int> d_seed;              // per non-empty chain: the polymer's seed
+  DBuf<double4> br_R0, br_FB;    // per owned atom: step-start position, Brownian force
+  DBuf<double> br_T;             // per owned atom: its bond's tension (first step -> second step)
+  unsigned long long brstep = 0; // first_step calls
 };
 
 struct SwapDev {
@@ -726,8 +736,22 @@ void enqueue_decide(phafd_hip_ctx *c) {
 void enqueue_flags_readback(phafd_hip_ctx *c) {
   HIPCHK(hipMemcpyAsync(c->h_flags + 8, c->flags.p, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
 }
+// the bead-rod fix's failure flag: flags[3] (read back with each step's
+// flags); FixAtomSemiFlexible::final_integrate's error (src/fixatom_semiflexible.cpp:176-188)
+void semiflex_failed(phafd_hip_ctx *c) {
+  std::fprintf(stderr, "bad timestep from no_tether of polymer on processor 0 at timestep %lld\n",
+               (long long)c->timestep);
+  throw std::runtime_error("Failed to maintain polymer bond-length constraints.");
+}
+bool has_semiflex(const phafd_hip_ctx *c) {
+  for (auto &f : c->fixes)
+    if (f->kind == FixDef::SEMIFLEX) return true;
+  return false;
+}
+
 // the checks after step `step`'s synchronise (src/integrate.cpp:190-201)
 void check_step_flags(phafd_hip_ctx *c, int64_t step) {
+  if (c->h_flags[11]) semiflex_failed(c);
   if (c->neigh_set) {
     const int f = c->h_flags[8 + 4 * (int)(step & 1)];
     if (f & 2) throw std::runtime_error("Non-numeric positions - simulation unstable");
@@ -955,7 +979,21 @@ void phase_pair(phafd_hip_ctx *c, double *energy_ga = nullptr, double *energy_lj
   reverse_comm(c);
 }
 
-void integrate_half(phafd_hip_ctx *c, FixDef &f) {
+// second: the final_integrate half (FixAtomDrag and the harmonic chains do
+// the same in both halves; the bead-rod fix runs first_step, then second_step)
+void integrate_half(phafd_hip_ctx *c, FixDef &f, bool second = false) {
+  if (f.kind == FixDef::SEMIFLEX) {
+    BRParamsDev p{f.bondlength, f.zpar, f.zperp, f.bending, f.brtemp, c->dt, f.tether, 50, f.brstep};
+    if (!second) {
+      p.step = ++f.brstep;
+      launch_br_first(f.nchain, f.maxlen, f.chs.p, f.che.p, f.d_seed.p, c->F.p, c->ux.p, c->x.p, c->image.p,
+                      f.br_R0.p, f.br_FB.p, f.br_T.p, p, c->box(), c->stream);
+    } else {
+      launch_br_second(f.nchain, f.maxlen, f.chs.p, f.che.p, c->F.p, c->ux.p, c->x.p, c->image.p, f.br_R0.p,
+                       f.br_FB.p, f.br_T.p, c->flags.p + 3, p, c->box(), c->stream);
+    }
+    return;
+  }
   if (f.kind == FixDef::DRAG) {
     launch_drag(f.nmem, f.members.p, c->x.p, c->F.p, f.drag, c->dt, c->stream);
   } else if (f.kind == FixDef::CHAIN) {
@@ -1015,7 +1053,7 @@ void phase_final_t(phafd_hip_ctx *c, bool atoms = true) {
       const auto *nz = take_noise<R>(c, f);
       spectral_update(c->plan, F.phihat.p, F.muhat.p, nz, update_params(c, f), origin_ptr(c), c->stream);
     } else if (atoms) {
-      integrate_half(c, f);
+      integrate_half(c, f, true);
     }
   }
 }
@@ -1049,7 +1087,7 @@ void phase_final_split(phafd_hip_ctx *c, bool atoms = true, bool next = false) {
       fft_z_update(c->plan, F.muhat.p, F.phihat.p, nz, update_params(c, f), origin_ptr(c), c->stream, Slab(),
                    c->store_muhat);
     } else if (atoms) {
-      integrate_half(c, f);
+      integrate_half(c, f, true);
     }
   }
 }
@@ -1084,6 +1122,7 @@ void run_phase(phafd_hip_ctx *c, int ph) {
     case PHAFD_PH_FINAL:
       if (c->fp32) phase_final_t<float>(c);
       else phase_final_t<double>(c);
+      if (has_semiflex(c) && read_int(c, c->flags.p + 3)) semiflex_failed(c);
       break;
     case PHAFD_PH_POSTFINAL:
       if (c->fp32) phase_postfinal_t<float>(c);
@@ -1276,7 +1315,8 @@ void build_members(phafd_hip_ctx *c, FixDef &f) {
       throw std::invalid_argument("fix group chunk out of range");
     if (f.end[k] == f.start[k]) continue;
     for (int j = f.start[k]; j < f.end[k]; j++) {
-      if (seen[j] && f.kind == FixDef::CHAIN) throw std::invalid_argument("chain chunks overlap");
+      if (seen[j] && (f.kind == FixDef::CHAIN || f.kind == FixDef::SEMIFLEX))
+        throw std::invalid_argument("chain chunks overlap");
       seen[j] = 1;
       mem.push_back(j);
       cs.push_back(f.start[k]);
@@ -1288,6 +1328,8 @@ void build_members(phafd_hip_ctx *c, FixDef &f) {
   }
   if (f.kind == FixDef::CHAIN && f.maxlen > 2048)
     throw std::invalid_argument("chains longer than 2048 beads are not supported");
+  if (f.kind == FixDef::SEMIFLEX && f.maxlen > BR_MAXLEN)
+    throw std::invalid_argument("bead-rod chains longer than 700 beads are not supported");
   f.nmem = (int)mem.size();
   f.nchain = (int)hs.size();
   f.members.alloc(mem.size() + 1);
@@ -2165,6 +2207,42 @@ int phafd_hip_fix_chain(phafd_hip_ctx *c, int nchunk, const int *start, const in
   });
 }
 
+int phafd_hip_fix_semiflexible(phafd_hip_ctx *c, int tether, int nchunk, const int *start, const int *end, int seed,
+                               double bondlength, double zeta_para, double zeta_perp, double bending, double temp) {
+  return guard(c, [&] {
+    if (c->ntypes <= 0) throw std::runtime_error("Fix requires atoms to be created.");
+    if (tether < 0 || tether > 2) throw std::invalid_argument("tether must be 0 (notether), 1 (singletether) or 2");
+    if (!(bondlength > 0) || !(zeta_para > 0) || !(zeta_perp > 0) || !(temp >= 0))
+      throw std::invalid_argument("bondlength, zeta_para, zeta_perp must be positive, temp non-negative");
+    if (nchunk < 0 || (nchunk > 0 && (!start || !end))) throw std::invalid_argument("bad group chunks");
+    auto f = std::make_unique<FixDef>();
+    f->kind = FixDef::SEMIFLEX;
+    f->tether = tether;
+    f->start.assign(start, start + nchunk);
+    f->end.assign(end, end + nchunk);
+    f->bondlength = bondlength; f->zpar = zeta_para; f->zperp = zeta_perp; f->bending = bending; f->brtemp = temp;
+    build_members(c, *f);
+    // one seed per chunk, drawn as src/fixatom_semiflexible.cpp:44-80 does;
+    // the empty chunks' draws are skipped with them
+    std::mt19937 gen;
+    std::uniform_int_distribution<int> integer_dist;
+    gen.seed(phafd_make_unique_seed(seed));
+    std::vector<int> seeds;
+    for (int k = 0; k < nchunk; k++) {
+      const int v = integer_dist(gen);
+      if (end[k] > start[k]) seeds.push_back(v);
+    }
+    f->d_seed.alloc(seeds.size() + 1);
+    if (!seeds.empty())
+      HIPCHK(hipMemcpy(f->d_seed.p, seeds.data(), sizeof(int) * seeds.size(), hipMemcpyHostToDevice));
+    f->br_R0.alloc(c->nowned + 1);
+    f->br_FB.alloc(c->nowned + 1);
+    f->br_T.alloc(c->nowned + 1);
+    c->fixes.push_back(std::move(f));
+    return 0;
+  });
+}
+
 int phafd_hip_neighbor(phafd_hip_ctx *c, double skin) {
   return guard(c, [&] {
     if (c->pairs.empty() && c->ntypes > 0) throw std::runtime_error("Must call neighbor after pairs are given.");
@@ -2196,8 +2274,24 @@ int phafd_hip_setup(phafd_hip_ctx *c, double dt) {
     }
     if (c->nowned > 0) launch_fill_d4(c->nowned, c->F.p, c->stream);
     if (c->ngathered > 0) launch_fill_d4(c->ngathered, c->gF.p, c->stream);
+    launch_fill_i32(1, c->flags.p + 3, 0, c->stream);
     c->pending_decision = -1;
     sync(c);
+    // FixAtomSemiFlexible::setup -> check_bonds (src/fixatom_semiflexible.cpp:84-140)
+    if (has_semiflex(c)) {
+      std::vector<double4> u(c->nowned);
+      if (c->nowned) HIPCHK(hipMemcpy(u.data(), c->ux.p, sizeof(double4) * c->nowned, hipMemcpyDeviceToHost));
+      for (auto &fp : c->fixes) {
+        if (fp->kind != FixDef::SEMIFLEX) continue;
+        for (size_t k = 0; k < fp->start.size(); k++)
+          for (int mu = fp->start[k] + 1; mu < fp->end[k]; mu++) {
+            const double d0 = u[mu].x - u[mu - 1].x, d1 = u[mu].y - u[mu - 1].y, d2 = u[mu].z - u[mu - 1].z;
+            if (std::fabs(std::sqrt(d0 * d0 + d1 * d1 + d2 * d2) - fp->bondlength) > 1e-3)
+              throw std::runtime_error("Adjacent atom displacements do not match the FixAtomSemiFlexible "
+                                       "bondlength specified.");
+          }
+      }
+    }
     c->setup_done = true;
     return 0;
   });
@@ -2367,6 +2461,9 @@ int phafd_hip_energy(phafd_hip_ctx *c, double *out) {
                            f->temp / f->volFH * f->gamma, c->esum.p + 2, c->stream);
       } else if (f->kind == FixDef::CHAIN) {
         ChainParams cp{f->r0, f->kbond, f->kbend, f->drag};
+        launch_chain_energy(f->nmem, f->members.p, f->cstart.p, f->cend.p, c->ux.p, cp, c->esum.p + 6, c->stream);
+      } else if (f->kind == FixDef::SEMIFLEX) {   // rigid rods: the bending energy kb (1 - cos) alone
+        ChainParams cp{f->bondlength, 0.0, f->bending, 0.0};
         launch_chain_energy(f->nmem, f->members.p, f->cstart.p, f->cend.p, c->ux.p, cp, c->esum.p + 6, c->stream);
       }
     }
@@ -2786,7 +2883,7 @@ void batch_steps(phafd_hip_batch *b, int nsteps) {
       phafd_hip_ctx *c = b->members[i].get();
       if (!fused_chain_fix(c))
         for (auto &f : c->fixes)
-          if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f);
+          if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f, true);
       if (c->neigh_set) enqueue_decide(c);   // next step's rebuild decision (Neighbor::decide)
       enqueue_flags_readback(c);
       HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->fd.phi.p, sizeof(double), hipMemcpyDeviceToHost, b->stream));
@@ -2863,6 +2960,8 @@ int phafd_hip_batch_setup(phafd_hip_batch *b, double dt) {
         if (f->kind == FixDef::CONJ) { nconj++; cj = f.get(); }
         if (f->kind == FixDef::FH) { nfh++; fh = f.get(); }
         if (f->kind == FixDef::GRADPHI) throw std::invalid_argument("batch members take no grid/gradphi fix");
+        if (f->kind == FixDef::SEMIFLEX)
+          throw std::invalid_argument("batch members take no atom/semiflexible fix (plain runs only)");
       }
       if (nconj != 1 || nfh != 1)
         throw std::invalid_argument("each batch member needs one grid/conjugate/volfrac and one grid/floryhuggins fix");
@@ -3002,6 +3101,8 @@ SurrogateCfg surrogate_cfg(phafd_hip_ctx *c) {
     if (f->kind == FixDef::CONJ) { nconj++; g.cj = f.get(); }
     if (f->kind == FixDef::FH) { nfh++; g.fh = f.get(); }
     if (f->kind == FixDef::GRADPHI) throw std::invalid_argument("the surrogate takes no grid/gradphi fix");
+    if (f->kind == FixDef::SEMIFLEX)
+      throw std::invalid_argument("the surrogate takes no atom/semiflexible fix (plain runs only)");
   }
   if (nconj != 1 || nfh != 1)
     throw std::invalid_argument("the surrogate needs one grid/conjugate/volfrac and one grid/floryhuggins fix");
@@ -3176,7 +3277,7 @@ void atom_steps_frozen_field(phafd_hip_ctx *c, const SurrogateCfg &g, int K) {
       } else {
         phase_initial(c);
         for (auto &f : c->fixes)
-          if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f);
+          if (f->kind != FixDef::CONJ && f->kind != FixDef::FH) integrate_half(c, *f, true);
       }
       // next step's rebuild decision (Neighbor::decide)
       if (W > 1) launch_decide(c->nowned, c->x.p, c->xhold.p, c->triggersq, c->sg_win.p + 1, nullptr, c->stream, gate);
