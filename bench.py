@@ -408,6 +408,20 @@ def bench_batch(args):
     for i in range(len(members)):
         m = b.member(i)
         got.append((m.get_field(P.PHI), m.get_atoms()[0], m.get_forces()))
+    # the other pair-phase mode (one launch per pass for every member vs
+    # per-member launches on the member streams), timed over as many steps
+    # both modes at the same point of the run: blocks of 10 steps ordered
+    # A B B A (a drift of the step time over the blocks cancels), the first
+    # step of each block untimed
+    fi = b.fused_info()
+    mode_ms = {0: 0.0, 1: 0.0}
+    for mode in (1, 0, 0, 1):
+        b.set_fused(mode)
+        b.run(1)
+        t1 = time.perf_counter()
+        b.run(10)
+        mode_ms[mode] += 1e3 * (time.perf_counter() - t1) / 20
+    b.set_fused(1 if fi["fused"] else 0)
     b.set_graphs(False)
     m0 = b.member(0)
     m0.probe(args.probe)
@@ -462,6 +476,15 @@ def bench_batch(args):
         "sequential_solo": {"value": N * args.steps / solo, "ms_per_step": 1e3 * solo / args.steps,
                             "batch_speedup": solo / elapsed},
         "members_bit_identical_to_solo": f"{same}/{len(members)}",
+        "pair_phase": {"mode": "one launch per pass for every member" if fi["fused"] else "per-member launches",
+                       "picked_by": "autotuner: whole steps 16-47 timed, the modes in blocks of 8 ordered A B B A",
+                       "step_ms_per_member_launches_trial": fi["step_ms_per_member"],
+                       "step_ms_one_launch_per_pass_trial": fi["step_ms_one_launch_per_pass"],
+                       "setup_ms_one_launch_per_pass": fi["tune_fused_ms"],
+                       "setup_ms_per_member_launches": fi["tune_member_ms"],
+                       "tuned_shape": {"atoms_per_item": fi["atoms_per_item"], "grid_cap": fi["grid_cap"],
+                                       "setup_ms_by_shape": fi["tune_shape_ms"]},
+                       "ms_per_step_abba": {"one_launch_per_pass": mode_ms[1], "per_member_launches": mode_ms[0]}},
         "roofline": {"bound": "hbm", "kernel": args.probe, "achieved": achieved, "peak": peak,
                      "unit": "GB/s", "frac": (achieved / peak) if achieved else None, "traffic": None,
                      "avg_launch_ms": avg_ms, "launches": plaunch,

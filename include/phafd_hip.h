@@ -362,6 +362,28 @@ int phafd_hip_batch_set_graphs(phafd_hip_batch *batch, int on);
 /* {members, members per launch, member streams, graphs on, gradient-block ms
  *  for the four launch sizes tried, pair-pass ms for the four stream counts} */
 int phafd_hip_batch_info(phafd_hip_batch *batch, double *out12);
+/* The members' pair phase (grid-atom force walk, mu sums, lj/cut, each
+ * ghost-force fold level, the harmonic chains' two half steps) as one
+ * launch per pass over every member (items: blocks of 64-256 atoms of one
+ * member, taken round-robin by a persistent grid; both sizes autotuned)
+ * instead of per-member
+ * launches on the member streams.  mode -1 (default): the autotuner of
+ * phafd_hip_batch_setup times both on the set-up state and keeps the faster;
+ * 0: per-member; 1: one launch per pass.  The arithmetic is the same either
+ * way (bit-identical members).  Members with other pair styles than one
+ * grid-atom style and lj/cut run per-member. */
+int phafd_hip_batch_set_fused(phafd_hip_batch *batch, int mode);
+/* out14 = {one launch per pass in use (0/1), its set-up time in ms (best
+ * shape), the best per-member time in ms, the tuned shape: atoms per item,
+ * persistent grid cap, the set-up times in ms of the six shapes tried
+ * (256 x {1024, 2048, 4096}, 128 x {2048, 4096}, 64 x 4096), then the
+ * whole-step choice of mode -1: decided (0/1), mean step ms per-member, mean
+ * step ms one launch per pass}.  In mode -1 the pair-phase timing of set-up
+ * picks the mode of the first steps; steps 16-47 then run the modes in
+ * blocks of 8 ordered A B B A (the same numbers either way; a steady drift
+ * of the step time cancels), every step but a block's first timed whole,
+ * and the faster mode stays. */
+int phafd_hip_batch_fused_info(phafd_hip_batch *batch, double *out14);
 int phafd_hip_batch_destroy(phafd_hip_batch *batch);
 const char *phafd_hip_batch_last_error(const phafd_hip_batch *batch);
 

@@ -112,6 +112,8 @@ def lib():
         "phafd_hip_batch_run": ([V, ctypes.c_int], ctypes.c_int),
         "phafd_hip_batch_set_graphs": ([V, ctypes.c_int], ctypes.c_int),
         "phafd_hip_batch_info": ([V, _dp], ctypes.c_int),
+        "phafd_hip_batch_set_fused": ([V, ctypes.c_int], ctypes.c_int),
+        "phafd_hip_batch_fused_info": ([V, _dp], ctypes.c_int),
         "phafd_hip_batch_destroy": ([V], ctypes.c_int),
         "phafd_hip_batch_last_error": ([V], ctypes.c_char_p),
         "phafd_unet_create": ([ctypes.c_int, ctypes.c_ulonglong, ctypes.POINTER(V)], ctypes.c_int),
@@ -759,6 +761,21 @@ class Batch:
 
     def set_graphs(self, on):
         self._c(lib().phafd_hip_batch_set_graphs(self.h, 1 if on else 0))
+
+    def set_fused(self, mode):
+        """-1: autotuned (default), 0: per-member pair launches, 1: one launch per pass for every member."""
+        self._c(lib().phafd_hip_batch_set_fused(self.h, int(mode)))
+
+    def fused_info(self):
+        o = np.zeros(14)
+        self._c(lib().phafd_hip_batch_fused_info(self.h, _d(o)))
+        return {"fused": bool(o[0]), "tune_fused_ms": float(o[1]), "tune_member_ms": float(o[2]),
+                "atoms_per_item": int(o[3]), "grid_cap": int(o[4]),
+                "tune_shape_ms": {f"{a}x{c}": float(v) for (a, c), v in
+                                  zip([(256, 1024), (256, 2048), (256, 4096), (128, 2048), (128, 4096), (64, 4096)],
+                                      o[5:11])},
+                "step_choice_decided": bool(o[11]), "step_ms_per_member": float(o[12]),
+                "step_ms_one_launch_per_pass": float(o[13])}
 
     def info(self):
         o = np.zeros(12)

@@ -779,6 +779,84 @@ __global__ __launch_bounds__(256) void k_ga_pm_sort(size_t N, const int *__restr
 // order, so that up to GA_B pairs' loads are in flight.  Grid sides are
 // powers of two (lx = log2 nx, lxy = log2 (nx ny)).
 constexpr int GA_B = 8;
+// the force walk of gathered atom number q of the bin-sorted order (ZI: gF
+// starts from zero instead of its stored value, and atoms without pairs
+// get zero written -- the fill and the walk in one)
+template <class R, bool ZI>
+__device__ __forceinline__ void ga_force_atom(size_t q, const GridViewT<R> &gv, const GAParams &ga,
+                                              const double4 *__restrict__ gx, const int *__restrict__ order,
+                                              const int *__restrict__ aoff, const int2 *__restrict__ ps,
+                                              R *__restrict__ mterm, double4 *__restrict__ gF, double &esum,
+                                              bool want_energy, int lx, int lxy) {
+  const int mx = gv.nx - 1, my = gv.ny - 1;
+  const int s = aoff[q], e = aoff[q + 1];
+  const int a = order[q];
+  if (s == e) {
+    if (ZI) gF[a] = make_double4(0.0, 0.0, 0.0, 0.0);
+    return;
+  }
+  const double4 x = gx[a];
+  const double4 F0 = ZI ? make_double4(0.0, 0.0, 0.0, 0.0) : gF[a];
+  const int t = (int)x.w;
+  const double cs = ga.cutsq[t];
+  const R phit = (R)ga.phit[t];
+  ForceAcc<R> acc;
+  acc.F = F0;
+  bool any = false;
+  for (int c0 = s; c0 < e; c0 += GA_B) {
+    int pp[GA_B], so[GA_B];
+#pragma unroll
+    for (int u = 0; u < GA_B; u++) {
+      const int2 v = c0 + u < e ? ps[c0 + u] : make_int2(0, 0);
+      pp[u] = v.x;
+      so[u] = v.y;
+    }
+    bool in[GA_B];
+    double rs[GA_B];
+    R ph[GA_B], fx[GA_B], fy[GA_B], fz[GA_B];
+#pragma unroll
+    for (int u = 0; u < GA_B; u++) {
+      const int p = pp[u];
+      const int k = p & mx, j = (p >> lx) & my, i = p >> lxy;
+      const double delz = (i * gv.dz + gv.lo[2]) - x.z;
+      const double dely = (j * gv.dy + gv.lo[1]) - x.y;
+      const double delx = (k * gv.dx + gv.lo[0]) - x.x;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      in[u] = c0 + u < e && ga_in(ga, rsq, cs);   // NaN positions: false
+      rs[u] = rsq;
+      ph[u] = fx[u] = fy[u] = fz[u] = R(0);
+      if (in[u]) {
+        ph[u] = gv.phi[p];
+        fx[u] = gv.gx[p];
+        fy[u] = gv.gy[p];
+        fz[u] = gv.gz[p];
+      } else if (c0 + u < e) {
+        mterm[so[u]] = __builtin_nan("");
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < GA_B; u++) {
+      if (!in[u]) continue;
+      R expfac = ga_expfac<R>(ga, rs[u] / cs, t);
+      if (ga.style == GA_LJISH_LINEAR) {
+        // PairGridAtomLJishLinear::compute (src/pair_gridatom_lj_ish_linear.cpp:
+        // 93-160): e = -1 * bump * eps, mu += e, force weight e dv, no energy
+        expfac = -expfac;
+        mterm[so[u]] = expfac;
+        expfac *= (R)gv.dv;
+      } else {
+        const R delphi = (ph[u] - phit);
+        mterm[so[u]] = 2 * delphi * expfac;
+        if (want_energy) esum += (double)delphi * (double)delphi * (double)expfac * gv.dv;
+        expfac *= 2 * delphi * (R)gv.dv;
+      }
+      acc.sub(fx[u] * expfac, fy[u] * expfac, fz[u] * expfac);
+      any = true;
+    }
+  }
+  if (any || ZI) gF[a] = acc.get();
+}
+
 template <class R>
 __global__ __launch_bounds__(256) void k_ga_force(int ng, GridViewT<R> gv, GAParams ga,
                                                   const double4 *__restrict__ gx, const int *__restrict__ order,
@@ -787,73 +865,36 @@ __global__ __launch_bounds__(256) void k_ga_force(int ng, GridViewT<R> gv, GAPar
                                                   double4 *__restrict__ gF, double *energy, int lx, int lxy) {
   double esum = 0.0;
   const size_t bl = xcd_block(blockIdx.x, gridDim.x);
-  const int mx = gv.nx - 1, my = gv.ny - 1;
-  for (size_t q = bl * blockDim.x + threadIdx.x; q < (size_t)ng; q += (size_t)gridDim.x * blockDim.x) {
-    const int s = aoff[q], e = aoff[q + 1];
-    const int a = order[q];
-    if (s == e) continue;
-    const double4 x = gx[a];
-    const double4 F0 = gF[a];
-    const int t = (int)x.w;
-    const double cs = ga.cutsq[t];
-    const R phit = (R)ga.phit[t];
-    ForceAcc<R> acc;
-    acc.F = F0;
-    bool any = false;
-    for (int c0 = s; c0 < e; c0 += GA_B) {
-      int pp[GA_B], so[GA_B];
-#pragma unroll
-      for (int u = 0; u < GA_B; u++) {
-        const int2 v = c0 + u < e ? ps[c0 + u] : make_int2(0, 0);
-        pp[u] = v.x;
-        so[u] = v.y;
-      }
-      bool in[GA_B];
-      double rs[GA_B];
-      R ph[GA_B], fx[GA_B], fy[GA_B], fz[GA_B];
-#pragma unroll
-      for (int u = 0; u < GA_B; u++) {
-        const int p = pp[u];
-        const int k = p & mx, j = (p >> lx) & my, i = p >> lxy;
-        const double delz = (i * gv.dz + gv.lo[2]) - x.z;
-        const double dely = (j * gv.dy + gv.lo[1]) - x.y;
-        const double delx = (k * gv.dx + gv.lo[0]) - x.x;
-        const double rsq = delx * delx + dely * dely + delz * delz;
-        in[u] = c0 + u < e && ga_in(ga, rsq, cs);   // NaN positions: false
-        rs[u] = rsq;
-        ph[u] = fx[u] = fy[u] = fz[u] = R(0);
-        if (in[u]) {
-          ph[u] = gv.phi[p];
-          fx[u] = gv.gx[p];
-          fy[u] = gv.gy[p];
-          fz[u] = gv.gz[p];
-        } else if (c0 + u < e) {
-          mterm[so[u]] = __builtin_nan("");
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < GA_B; u++) {
-        if (!in[u]) continue;
-        R expfac = ga_expfac<R>(ga, rs[u] / cs, t);
-        if (ga.style == GA_LJISH_LINEAR) {
-          // PairGridAtomLJishLinear::compute (src/pair_gridatom_lj_ish_linear.cpp:
-          // 93-160): e = -1 * bump * eps, mu += e, force weight e dv, no energy
-          expfac = -expfac;
-          mterm[so[u]] = expfac;
-          expfac *= (R)gv.dv;
-        } else {
-          const R delphi = (ph[u] - phit);
-          mterm[so[u]] = 2 * delphi * expfac;
-          if (energy) esum += (double)delphi * (double)delphi * (double)expfac * gv.dv;
-          expfac *= 2 * delphi * (R)gv.dv;
-        }
-        acc.sub(fx[u] * expfac, fy[u] * expfac, fz[u] * expfac);
-        any = true;
-      }
-    }
-    if (any) gF[a] = acc.get();
-  }
+  for (size_t q = bl * blockDim.x + threadIdx.x; q < (size_t)ng; q += (size_t)gridDim.x * blockDim.x)
+    ga_force_atom<R, false>(q, gv, ga, gx, order, aoff, ps, mterm, gF, esum, energy != nullptr, lx, lxy);
   if (energy && esum != 0.0) atomicAdd(energy, esum);
+}
+
+// the member whose item range [boff[m], boff[m + 1]) holds item
+__device__ __forceinline__ int member_of(const int *__restrict__ boff, int nm, int item) {
+  int lo = 0, hi = nm;   // boff[lo] <= item < boff[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (boff[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// every member's force walk in one launch: item = a block of 256 gathered
+// atoms of one member, items taken round-robin by a grid of at most a few
+// blocks per CU (gF written from zero: the fill folded in)
+__global__ __launch_bounds__(256) void k_ga_force_multi(const GAMemberDev *__restrict__ d,
+                                                        const int *__restrict__ boff, int nm, int lx, int lxy) {
+  const int nitems = boff[nm];
+  for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int m = member_of(boff, nm, item);
+    const GAMemberDev &g = d[m];
+    const size_t q = (size_t)(item - boff[m]) * blockDim.x + threadIdx.x;
+    double esum = 0.0;
+    if (q < (size_t)g.ng)
+      ga_force_atom<double, true>(q, g.gv, g.ga, g.gx, g.order, g.aoff, g.ps, g.mterm, g.gF, esum, false, lx, lxy);
+  }
 }
 
 // The step's grid-atom pass, grid side: mu(g) (first: from zero, else added
@@ -862,9 +903,10 @@ __global__ __launch_bounds__(256) void k_ga_force(int ng, GridViewT<R> gv, GAPar
 // 64 * GM_U consecutive points, GM_U per lane 64 apart, and issues the loads
 // of its points (offsets, then up to four terms each) before using them.
 template <class R>
-__global__ __launch_bounds__(256) void k_ga_list_mu(size_t N, R *__restrict__ mu, const int *__restrict__ poff,
-                                                    const R *__restrict__ mterm, int first) {
-  const size_t wbase = ((size_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * (64 * GM_U);
+__device__ __forceinline__ void ga_list_mu_block(size_t blk, size_t N, R *__restrict__ mu,
+                                                 const int *__restrict__ poff, const R *__restrict__ mterm,
+                                                 int first) {
+  const size_t wbase = (blk * (blockDim.x / 64) + threadIdx.x / 64) * (64 * GM_U);
   const int lane = threadIdx.x & 63;
   int s[GM_U], n[GM_U];
   R m[GM_U];
@@ -901,6 +943,16 @@ __global__ __launch_bounds__(256) void k_ga_list_mu(size_t N, R *__restrict__ mu
     }
     mu[p] = acc.get();
   }
+}
+template <class R>
+__global__ __launch_bounds__(256) void k_ga_list_mu(size_t N, R *__restrict__ mu, const int *__restrict__ poff,
+                                                    const R *__restrict__ mterm, int first) {
+  ga_list_mu_block<R>(blockIdx.x, N, mu, poff, mterm, first);
+}
+// every member's mu sums in one launch (blockIdx.y: the member; mu from zero)
+__global__ __launch_bounds__(256) void k_ga_list_mu_multi(const GAMemberDev *__restrict__ d, size_t N) {
+  const GAMemberDev &g = d[blockIdx.y];
+  ga_list_mu_block<double>(blockIdx.x, N, g.mu, g.poff, g.mterm, 1);
 }
 
 // PairLJCut::compute, src/pair_lj_cut.cpp:46-119, gathered per atom: atom k
@@ -958,6 +1010,35 @@ struct HarmLaw {
 // one bead-bead pair style over the pruned half list, gathered per atom in
 // the reference's slot order (the law: lj/cut or harmonic/cut)
 template <class Law>
+__device__ __forceinline__ void pair_force_atom(int a, const Law &law, const double4 *gx, const int *rowof,
+                                                const int *hoff, const int *hneigh, const int *toff,
+                                                const int *tpair, const int *ti, int nowned, double4 *gF,
+                                                bool want_energy, double &esum, int zero_init) {
+  const double4 xa = gx[a];
+  double4 F = zero_init ? make_double4(0.0, 0.0, 0.0, 0.0) : gF[a];
+  const int row = rowof[a];
+  const int rs = row >= 0 ? hoff[row] : 0x7fffffff;
+  const int re = row >= 0 ? hoff[row + 1] : 0x7fffffff;
+  int q = toff[a];
+  const int q1 = toff[a + 1];
+  double fx, fy, fz, e;
+  for (; q < q1 && tpair[q] < rs; q++) {
+    if (law(gx[ti[q]], xa, fx, fy, fz, e)) { F.x -= fx; F.y -= fy; F.z -= fz; }
+  }
+  if (row >= 0) {
+    for (int s = rs; s < re; s++) {
+      if (law(xa, gx[hneigh[s] - nowned], fx, fy, fz, e)) {
+        F.x += fx; F.y += fy; F.z += fz;
+        if (want_energy) esum += e;
+      }
+    }
+  }
+  for (; q < q1; q++) {
+    if (law(gx[ti[q]], xa, fx, fy, fz, e)) { F.x -= fx; F.y -= fy; F.z -= fz; }
+  }
+  gF[a] = F;
+}
+template <class Law>
 __global__ void k_pair_force(int ng, Law law, const double4 *gx, const int *rowof,
                              const int *hoff, const int *hneigh, const int *toff, const int *tpair,
                              const int *ti, int nowned, double4 *gF, double *energy, int zero_init,
@@ -965,31 +1046,56 @@ __global__ void k_pair_force(int ng, Law law, const double4 *gx, const int *rowo
   if (gate_off(gate)) return;
   double esum = 0.0;
   GRID_STRIDE(a, ng) {
-    const double4 xa = gx[a];
-    double4 F = zero_init ? make_double4(0.0, 0.0, 0.0, 0.0) : gF[a];
-    const int row = rowof[a];
-    const int rs = row >= 0 ? hoff[row] : 0x7fffffff;
-    const int re = row >= 0 ? hoff[row + 1] : 0x7fffffff;
-    int q = toff[a];
-    const int q1 = toff[a + 1];
-    double fx, fy, fz, e;
-    for (; q < q1 && tpair[q] < rs; q++) {
-      if (law(gx[ti[q]], xa, fx, fy, fz, e)) { F.x -= fx; F.y -= fy; F.z -= fz; }
-    }
-    if (row >= 0) {
-      for (int s = rs; s < re; s++) {
-        if (law(xa, gx[hneigh[s] - nowned], fx, fy, fz, e)) {
-          F.x += fx; F.y += fy; F.z += fz;
-          if (energy) esum += e;
-        }
-      }
-    }
-    for (; q < q1; q++) {
-      if (law(gx[ti[q]], xa, fx, fy, fz, e)) { F.x -= fx; F.y -= fy; F.z -= fz; }
-    }
-    gF[a] = F;
+    pair_force_atom(a, law, gx, rowof, hoff, hneigh, toff, tpair, ti, nowned, gF, energy != nullptr, esum,
+                    zero_init);
   }
   if (energy) atomicAdd(energy, esum);
+}
+// every member's lj/cut in one launch (items: blocks of 256 gathered atoms)
+__global__ __launch_bounds__(256) void k_lj_multi(const LJMemberDev *__restrict__ d, const int *__restrict__ boff,
+                                                  int nm) {
+  const int nitems = boff[nm];
+  for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int m = member_of(boff, nm, item);
+    const LJMemberDev &g = d[m];
+    const int a = (item - boff[m]) * blockDim.x + threadIdx.x;
+    double esum = 0.0;
+    if (a < g.ng)
+      pair_force_atom(a, LJLaw{g.lj}, g.gx, g.rowof, g.hoff, g.hneigh, g.toff, g.tpair, g.ti, g.nowned, g.gF,
+                      false, esum, 0);
+  }
+}
+// one fold level of every member (a swap, or a dimension's two disjoint
+// swaps): target[send[r]] (+)= src[first + r]; zero: the target starts from
+// zero (the fill folded in, one-pass copy swap)
+__global__ __launch_bounds__(256) void k_fold_multi(const FoldDev *__restrict__ d, const int *__restrict__ boff,
+                                                    int nm, int zero) {
+  const int nitems = boff[nm];
+  for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int m = member_of(boff, nm, item);
+    const FoldDev &f = d[m];
+    const int r = (item - boff[m]) * blockDim.x + threadIdx.x;
+    if (r >= f.ca + f.cb) continue;
+    const bool ina = r < f.ca;
+    const int rr = ina ? r : r - f.ca;
+    const int j = ina ? f.sa[rr] : f.sb[rr];
+    const double4 b = f.src[(ina ? f.fa : f.fb) + rr];
+    double4 t = zero ? make_double4(0.0, 0.0, 0.0, 0.0) : f.target[j];
+    t.x += b.x;
+    t.y += b.y;
+    t.z += b.z;
+    f.target[j] = t;
+  }
+}
+// the owned forces of every member set to zero (before a fold that adds)
+__global__ __launch_bounds__(256) void k_fill_multi(const FoldDev *__restrict__ d, const int *__restrict__ boff,
+                                                    int nm) {
+  const int nitems = boff[nm];
+  for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int m = member_of(boff, nm, item);
+    const int r = (item - boff[m]) * blockDim.x + threadIdx.x;
+    if (r < d[m].ca) d[m].target[r] = make_double4(0.0, 0.0, 0.0, 0.0);
+  }
 }
 
 // ---------------------------------------------------------- integrators
@@ -1051,12 +1157,9 @@ __device__ __forceinline__ void chain_bead_force(const double4 *u, int b, int nb
 // halves = 2: both half steps of a timestep in one launch (F is the same for
 // both; the first half's positions stay in LDS, in the second buffer at
 // us + stride, and only the second half is written out)
-__global__ void k_chain_step(const int *chs, const int *che, const double4 *F, double4 *ux, double4 *x,
-                             const int *image, ChainParams cp, double dt, Box bx, int halves, int stride,
-                             StepGate gate) {
-  if (gate_off(gate)) return;   // uniform over the block: *gate.stop is fixed during this launch
-  extern __shared__ double4 us[];
-  const int s = chs[blockIdx.x], nb = che[blockIdx.x] - s;
+__device__ __forceinline__ void chain_step_body(int s, int nb, const double4 *F, double4 *ux, double4 *x,
+                                                const int *image, const ChainParams &cp, double dt, const Box &bx,
+                                                int halves, int stride, double4 *us) {
   double4 *cur = us, *nxt = us + stride;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) cur[b] = ux[s + b];
   __syncthreads();
@@ -1091,6 +1194,25 @@ __global__ void k_chain_step(const int *chs, const int *che, const double4 *F, d
       nxt = t;
     }
   }
+}
+__global__ void k_chain_step(const int *chs, const int *che, const double4 *F, double4 *ux, double4 *x,
+                             const int *image, ChainParams cp, double dt, Box bx, int halves, int stride,
+                             StepGate gate) {
+  if (gate_off(gate)) return;   // uniform over the block: *gate.stop is fixed during this launch
+  extern __shared__ double4 us[];
+  const int s = chs[blockIdx.x], nb = che[blockIdx.x] - s;
+  chain_step_body(s, nb, F, ux, x, image, cp, dt, bx, halves, stride, us);
+}
+// both half steps of every member's chains in one launch: block = one chain
+// (coff: the members' chain offsets)
+__global__ void k_chain_multi(const ChainMemberDev *__restrict__ d, const int *__restrict__ coff, int nm, double dt,
+                              int stride) {
+  extern __shared__ double4 us[];
+  const int m = member_of(coff, nm, blockIdx.x);
+  const ChainMemberDev &g = d[m];
+  const int c = blockIdx.x - coff[m];
+  const int s = g.chs[c], nb = g.che[c] - s;
+  chain_step_body(s, nb, g.F, g.ux, g.x, g.image, g.cp, dt, g.bx, 2, stride, us);
 }
 __global__ void k_chain_energy(int n, const int *members, const int *cstart, const int *cend,
                                const double4 *ux, ChainParams cp, double *energy) {
@@ -1813,6 +1935,59 @@ void launch_br_second(int nchains, int maxlen, const int *chs, const int *che, c
   br_lds_allow((const void *)k_br_second, lds);
   hipLaunchKernelGGL(k_br_second, dim3(nchains), dim3(64), lds, s, chs, che, F, ux, x, image, R0, FB, T, fail, p,
                      b, maxlen);
+  HIPCHK(hipGetLastError());
+}
+// the batched pair phase: grids of at most 8 blocks per CU take the items
+// round-robin
+unsigned multi_grid(int nitems, int cap) { return (unsigned)std::max(1, std::min(nitems, cap)); }
+void check_shape(const MultiShape &sh) {
+  if ((sh.threads != 64 && sh.threads != 128 && sh.threads != 256) || sh.cap < 1)
+    throw std::invalid_argument("batched pair phase: items of 64, 128 or 256 atoms, a positive grid cap");
+}
+void launch_ga_force_multi(int nm, const GAMemberDev *d, const int *boff, int nitems, int nx, int ny,
+                           const MultiShape &sh, hipStream_t s) {
+  if (nm <= 0 || nitems <= 0) return;
+  check_shape(sh);
+  const int lx = ilog2(nx), lxy = lx + ilog2(ny);
+  hipLaunchKernelGGL(k_ga_force_multi, dim3(multi_grid(nitems, sh.cap)), dim3(sh.threads), 0, s, d, boff, nm, lx,
+                     lxy);
+  HIPCHK(hipGetLastError());
+}
+void launch_ga_list_mu_multi(int nm, size_t N, const GAMemberDev *d, hipStream_t s) {
+  if (nm <= 0) return;
+  const size_t per_block = 4 * 64 * GM_U;
+  hipLaunchKernelGGL(k_ga_list_mu_multi, dim3((unsigned)((N + per_block - 1) / per_block), nm), dim3(256), 0, s, d,
+                     N);
+  HIPCHK(hipGetLastError());
+}
+void launch_lj_multi(int nm, const LJMemberDev *d, const int *boff, int nitems, const MultiShape &sh,
+                     hipStream_t s) {
+  if (nm <= 0 || nitems <= 0) return;
+  check_shape(sh);
+  hipLaunchKernelGGL(k_lj_multi, dim3(multi_grid(nitems, sh.cap)), dim3(sh.threads), 0, s, d, boff, nm);
+  HIPCHK(hipGetLastError());
+}
+void launch_fold_multi(int nm, const FoldDev *d, const int *boff, int nitems, bool zero, const MultiShape &sh,
+                       hipStream_t s) {
+  if (nm <= 0 || nitems <= 0) return;
+  check_shape(sh);
+  hipLaunchKernelGGL(k_fold_multi, dim3(multi_grid(nitems, sh.cap)), dim3(sh.threads), 0, s, d, boff, nm,
+                     zero ? 1 : 0);
+  HIPCHK(hipGetLastError());
+}
+void launch_fill_multi(int nm, const FoldDev *d, const int *boff, int nitems, const MultiShape &sh, hipStream_t s) {
+  if (nm <= 0 || nitems <= 0) return;
+  check_shape(sh);
+  hipLaunchKernelGGL(k_fill_multi, dim3(multi_grid(nitems, sh.cap)), dim3(sh.threads), 0, s, d, boff, nm);
+  HIPCHK(hipGetLastError());
+}
+void launch_chain_multi(int nm, const ChainMemberDev *d, const int *coff, int nchains, int maxlen, double dt,
+                        hipStream_t s) {
+  if (nm <= 0 || nchains <= 0 || maxlen <= 0) return;
+  if (maxlen > 1024) throw std::invalid_argument("fused chain steps take chains of at most 1024 beads");
+  const int threads = maxlen <= 64 ? 64 : (maxlen <= 128 ? 128 : 256);
+  hipLaunchKernelGGL(k_chain_multi, dim3(nchains), dim3(threads), sizeof(double4) * maxlen * 2, s, d, coff, nm, dt,
+                     maxlen);
   HIPCHK(hipGetLastError());
 }
 void launch_fill_i32(int n, int *p, int v, hipStream_t s) {

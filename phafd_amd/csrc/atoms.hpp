@@ -156,6 +156,64 @@ void launch_chain_step(int nchains, int maxlen, const int *chs, const int *che, 
                        int halves = 1, StepGate gate = StepGate());
 void launch_chain_energy(int n, const int *members, const int *cstart, const int *cend, const double4 *ux,
                          const ChainParams &cp, double *energy, hipStream_t s);
+// The batched pair phase: every member of a batch in one launch per pass.
+// Items are blocks of 256 atoms (or entries) of one member; boff[m] the
+// first item of member m, boff[nm] their total (device arrays).
+struct GAMemberDev {
+  GridViewT<double> gv;
+  GAParams ga;
+  const double4 *gx;
+  const int *order, *aoff;
+  const int2 *ps;
+  double *mterm;
+  double4 *gF;
+  int ng;
+  double *mu;          // list_mu target (written from zero)
+  const int *poff;
+};
+struct LJMemberDev {
+  const double4 *gx;
+  const int *rowof, *hoff, *hneigh, *toff, *tpair, *ti;
+  int nowned, ng;
+  double4 *gF;
+  LJParams lj;
+};
+// one fold level of one member: swap a (ca entries) and, for a dimension's
+// two disjoint swaps, swap b (cb entries; 0 otherwise)
+struct FoldDev {
+  int ca, fa, cb, fb;
+  const int *sa, *sb;
+  double4 *target;
+  const double4 *src;
+};
+struct ChainMemberDev {
+  const int *chs, *che;
+  const double4 *F;
+  double4 *ux, *x;
+  const int *image;
+  ChainParams cp;
+  Box bx;
+};
+// launch shape of the item kernels: threads per block = atoms (entries)
+// per item, and the cap on the persistent grid; the item offsets boff are
+// counted in items of `threads`
+struct MultiShape {
+  int threads = 256, cap = 2048;
+};
+// gF of every gathered atom written (from zero: the fill folded in)
+void launch_ga_force_multi(int nm, const GAMemberDev *d, const int *boff, int nitems, int nx, int ny,
+                           const MultiShape &sh, hipStream_t s);
+void launch_ga_list_mu_multi(int nm, size_t N, const GAMemberDev *d, hipStream_t s);
+void launch_lj_multi(int nm, const LJMemberDev *d, const int *boff, int nitems, const MultiShape &sh,
+                     hipStream_t s);
+void launch_fold_multi(int nm, const FoldDev *d, const int *boff, int nitems, bool zero, const MultiShape &sh,
+                       hipStream_t s);
+// target[0 .. ca) = 0 of every member
+void launch_fill_multi(int nm, const FoldDev *d, const int *boff, int nitems, const MultiShape &sh, hipStream_t s);
+// both half steps of the harmonic chains of every member (coff: chain offsets)
+void launch_chain_multi(int nm, const ChainMemberDev *d, const int *coff, int nchains, int maxlen, double dt,
+                        hipStream_t s);
+
 // atom/semiflexible bead-rod polymers (one chain per block, in LDS)
 constexpr int BR_MAXLEN = 700;
 struct BRParamsDev {

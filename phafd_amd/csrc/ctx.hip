@@ -2730,6 +2730,24 @@ struct phafd_hip_batch {
   bool setup_done = false, use_graphs = true, sep = false, zsplit = false;
   int nstreams = 1, chunk = 1;
   double tune_grad_ms[4] = {0, 0, 0, 0}, tune_pair_ms[4] = {0, 0, 0, 0};
+  // the batched pair phase: every member in one launch per pass
+  int fused_mode = -1;                 // -1: picked by the autotuner, 0: per-member launches, 1: one launch per pass
+  bool fused = false;                  // what runs
+  double tune_fused_ms = 0;            // the pair phase in one launch per pass, at setup (best shape)
+  MultiShape shape;                    // its launch shape (autotuned)
+  // mode -1: steps [BATCH_TRIAL_START, + 4 * BATCH_TRIAL) run the two modes
+  // in blocks A B B A (real steps: the modes compute the same numbers),
+  // every step but a block's first timed whole; the faster mode is kept
+  int64_t steps_run = 0;
+  int trial = 0;                       // trial steps done
+  bool decided = false;
+  double trial_ms[2] = {0, 0};         // mean step time: per-member, one launch per pass
+  double tune_shape_ms[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<hipEvent_t> ev_n;        // member neighbour stage done
+  hipEvent_t ev_pair = nullptr;        // batched pair phase done
+  unsigned char *h_desc = nullptr;     // pinned staging of the descriptors
+  size_t h_desc_bytes = 0;
+  DBuf<unsigned char> d_desc;
   UpdateParams u{};
   SpecConst sc{};
   double tv = 1;
@@ -2858,25 +2876,221 @@ void capture(phafd_hip_batch *b, hipGraph_t *g, hipGraphExec_t *x, Fn fn) {
   HIPCHK(hipGraphInstantiate(x, *g, nullptr, nullptr, 0));
 }
 
+// The pair phase of every member in one launch per pass (grid-atom force
+// walk, mu sums, lj/cut, each fold level, the chains' two half steps):
+// descriptors of every member packed into one pinned blob, copied with one
+// transfer.  The per-member arithmetic is that of phase_pair +
+// integrate_both_halves.  Returns false (nothing launched) when a member
+// does not fit: other pair styles, no grid-atom style, folds of another
+// shape than member 0's.
+struct FusedPlan {
+  size_t off_ga, off_lj, off_fold, off_ch, off_int, bytes;
+  int nlev = 0, items_ga = 0, nch = 0, maxlen = 0;
+  bool lj = false, chains = false, onepass = false;
+  std::vector<int> ints;   // boff_ga (nm+1), fold boffs (nlev+1)*(nm+1), coff (nm+1)
+};
+
+bool fused_eligible(phafd_hip_batch *b) {
+  phafd_hip_ctx *c0 = b->members[0].get();
+  for (auto &m : b->members) {
+    phafd_hip_ctx *c = m.get();
+    if (!c->neigh_set || c->eager_gridlist || c->fp32) return false;
+    int nga = 0, nlj = 0;
+    for (auto &p : c->pairs) {
+      if (p->style == 2) nlj++;
+      else if (!bead_pair(*p)) nga++;
+      else return false;
+    }
+    if (nga != 1 || nlj > 1 || nlj != (int)std::count_if(c0->pairs.begin(), c0->pairs.end(),
+                                                          [](const std::unique_ptr<PairDef> &p) {
+                                                            return p->style == 2;
+                                                          }))
+      return false;
+    if (c->swaps.size() != c0->swaps.size()) return false;
+    for (size_t k = 0; k < c->swaps.size(); k++)
+      if (c->swaps[k].disjoint_pair != c0->swaps[k].disjoint_pair) return false;
+    if ((fused_chain_fix(c) != nullptr) != (fused_chain_fix(c0) != nullptr)) return false;
+  }
+  return true;
+}
+
+// the members' lists and pair arrays for this step (member streams)
+void fused_prepare(phafd_hip_ctx *c) {
+  if (c->ga_calls != c->ncalls) build_ga_pairs(c);
+}
+
+void batch_pair_fused(phafd_hip_batch *b, bool with_chains) {
+  const int nm = b->n;
+  const int T = b->shape.threads;
+  phafd_hip_ctx *c0 = b->members[0].get();
+  FusedPlan P;
+  P.lj = std::any_of(c0->pairs.begin(), c0->pairs.end(), [](const std::unique_ptr<PairDef> &p) {
+    return p->style == 2;
+  });
+  P.chains = with_chains && fused_chain_fix(c0) != nullptr;
+  // fold levels (the order of reverse_comm): pairs of disjoint swaps, single
+  // swaps, then the copy swap into F
+  std::vector<std::pair<int, int>> levels;   // (swap a, swap b or -1)
+  for (int iswap = (int)c0->swaps.size() - 1; iswap >= 1; iswap--) {
+    if (c0->swaps[iswap].disjoint_pair) {
+      levels.push_back({iswap, iswap - 1});
+      iswap--;
+    } else {
+      levels.push_back({iswap, -1});
+    }
+  }
+  levels.push_back({0, -1});
+  P.nlev = (int)levels.size();
+  P.onepass = true;
+  for (auto &m : b->members) P.onepass = P.onepass && reverse_from_one_pass(m.get());
+  std::vector<GAMemberDev> ga(nm);
+  std::vector<LJMemberDev> lj(nm);
+  std::vector<FoldDev> fold((size_t)(P.nlev + 1) * nm);   // levels, then the fill of F
+  std::vector<ChainMemberDev> ch(nm);
+  const int stride = nm + 1;
+  P.ints.assign((size_t)stride * (P.nlev + 4), 0);
+  int *boff_ga = P.ints.data(), *boff_fold = boff_ga + stride, *coff = boff_fold + (size_t)stride * (P.nlev + 1);
+  for (int i = 0; i < nm; i++) {
+    phafd_hip_ctx *c = b->members[i].get();
+    PairDef *pga = nullptr, *plj = nullptr;
+    for (auto &p : c->pairs) (p->style == 2 ? plj : pga) = p.get();
+    GAMemberDev &g = ga[i];
+    g.gv = c->gridview<double>();
+    g.ga = ga_params(*pga);
+    g.gx = c->gx.p; g.order = c->binatoms.p; g.aoff = c->ga_aoff.p; g.ps = c->ga_ps.p;
+    g.mterm = c->ga_mterm.p; g.gF = c->gF.p; g.ng = c->ngathered;
+    g.mu = c->fd.mu.p; g.poff = c->ga_poff.p;
+    boff_ga[i + 1] = boff_ga[i] + (c->ngathered + T - 1) / T;
+    if (P.lj) {
+      LJMemberDev &l = lj[i];
+      l.gx = c->gx.p; l.rowof = c->h_rowof.p; l.hoff = c->h_off.p; l.hneigh = c->h_neigh.p;
+      l.toff = c->t_off.p; l.tpair = c->t_pair.p; l.ti = c->t_i.p; l.nowned = c->nowned; l.ng = c->ngathered;
+      l.gF = c->gF.p; l.lj = LJParams{plj->lj1, plj->lj2, plj->ljcutsq};
+    }
+    for (int L = 0; L < P.nlev; L++) {
+      FoldDev &f = fold[(size_t)L * nm + i];
+      const SwapDev &wa = c->swaps[levels[L].first];
+      f.ca = wa.count; f.fa = wa.first; f.sa = wa.send.p;
+      f.cb = 0; f.fb = 0; f.sb = nullptr;
+      if (levels[L].second >= 0) {
+        const SwapDev &wb = c->swaps[levels[L].second];
+        f.cb = wb.count; f.fb = wb.first; f.sb = wb.send.p;
+      }
+      f.target = levels[L].first == 0 ? c->F.p : c->gF.p;
+      f.src = c->gF.p;
+      int *bo = boff_fold + (size_t)L * stride;
+      bo[i + 1] = bo[i] + (f.ca + f.cb + T - 1) / T;
+    }
+    FoldDev &fz = fold[(size_t)P.nlev * nm + i];   // the fill of F
+    fz = FoldDev{c->nowned, 0, 0, 0, nullptr, nullptr, c->F.p, nullptr};
+    int *bz = boff_fold + (size_t)P.nlev * stride;
+    bz[i + 1] = bz[i] + (c->nowned + T - 1) / T;
+    if (P.chains) {
+      FixDef &f = *fused_chain_fix(c);
+      ch[i] = ChainMemberDev{f.chs.p, f.che.p, c->F.p, c->ux.p, c->x.p, c->image.p,
+                             ChainParams{f.r0, f.kbond, f.kbend, f.drag}, c->box()};
+      coff[i + 1] = coff[i] + f.nchain;
+      P.maxlen = std::max(P.maxlen, f.maxlen);
+    } else {
+      coff[i + 1] = coff[i];
+    }
+  }
+  P.items_ga = boff_ga[nm];
+  P.nch = coff[nm];
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  P.off_ga = 0;
+  P.off_lj = al(P.off_ga + sizeof(GAMemberDev) * nm);
+  P.off_fold = al(P.off_lj + sizeof(LJMemberDev) * nm);
+  P.off_ch = al(P.off_fold + sizeof(FoldDev) * fold.size());
+  P.off_int = al(P.off_ch + sizeof(ChainMemberDev) * nm);
+  P.bytes = al(P.off_int + sizeof(int) * P.ints.size());
+  if (P.bytes > b->h_desc_bytes) {
+    if (b->h_desc) HIPCHK(hipHostFree(b->h_desc));
+    b->h_desc = nullptr;
+    HIPCHK(hipHostMalloc((void **)&b->h_desc, P.bytes, hipHostMallocDefault));
+    b->h_desc_bytes = P.bytes;
+  }
+  b->d_desc.alloc(P.bytes);
+  unsigned char *h = b->h_desc;
+  std::memcpy(h + P.off_ga, ga.data(), sizeof(GAMemberDev) * nm);
+  std::memcpy(h + P.off_lj, lj.data(), sizeof(LJMemberDev) * nm);
+  std::memcpy(h + P.off_fold, fold.data(), sizeof(FoldDev) * fold.size());
+  std::memcpy(h + P.off_ch, ch.data(), sizeof(ChainMemberDev) * nm);
+  std::memcpy(h + P.off_int, P.ints.data(), sizeof(int) * P.ints.size());
+  unsigned char *d = b->d_desc.p;
+  HIPCHK(hipMemcpyAsync(d, h, P.bytes, hipMemcpyHostToDevice, b->stream));
+  const GAMemberDev *dga = (const GAMemberDev *)(d + P.off_ga);
+  const LJMemberDev *dlj = (const LJMemberDev *)(d + P.off_lj);
+  const FoldDev *dfold = (const FoldDev *)(d + P.off_fold);
+  const ChainMemberDev *dch = (const ChainMemberDev *)(d + P.off_ch);
+  const int *di = (const int *)(d + P.off_int);
+  const int *dboff_ga = di, *dboff_fold = di + stride, *dcoff = dboff_fold + (size_t)stride * (P.nlev + 1);
+  double ng_all = 0;
+  for (auto &g : ga) ng_all += g.ng;
+  probe_begin(K_GA_FORCE, b->stream);
+  launch_ga_force_multi(nm, dga, dboff_ga, P.items_ga, c0->nx, c0->ny, b->shape, b->stream);
+  probe_end(K_GA_FORCE, b->stream, 64.0 * ng_all);
+  launch_ga_list_mu_multi(nm, b->N, dga, b->stream);
+  if (P.lj) launch_lj_multi(nm, dlj, dboff_ga, P.items_ga, b->shape, b->stream);
+  for (int L = 0; L < P.nlev; L++) {
+    const bool copy_swap = levels[L].first == 0;
+    if (copy_swap && !P.onepass)
+      launch_fill_multi(nm, dfold + (size_t)P.nlev * nm, dboff_fold + (size_t)P.nlev * stride,
+                        boff_fold[(size_t)P.nlev * stride + nm], b->shape, b->stream);
+    launch_fold_multi(nm, dfold + (size_t)L * nm, dboff_fold + (size_t)L * stride, boff_fold[(size_t)L * stride + nm],
+                      copy_swap && P.onepass, b->shape, b->stream);
+  }
+  if (P.chains) launch_chain_multi(nm, dch, dcoff, P.nch, P.maxlen, c0->dt, b->stream);
+}
+
+// the online choice of the pair-phase mode: after the set-up state has
+// given way to the run's (neighbour rebuilds under way), four blocks of
+// BATCH_TRIAL steps in the order A B B A
+constexpr int BATCH_TRIAL_START = 16, BATCH_TRIAL = 8;
+
 void batch_steps(phafd_hip_batch *b, int nsteps) {
   if (!b->setup_done) throw std::invalid_argument("call phafd_hip_batch_setup before running");
   for (auto &m : b->members)
     if (m->neigh_set) clear_decide_slot(m.get(), m->timestep + 1);
   for (int it = 0; it < nsteps; it++) {
+    const bool eligible = fused_eligible(b);
+    const bool trial = b->fused_mode < 0 && !b->decided && eligible && b->steps_run >= BATCH_TRIAL_START;
+    if (trial) {   // blocks in the order one launch per pass, per-member, per-member, one launch per pass
+      const int blk = b->trial / BATCH_TRIAL;   // (a drift of the step time over the trial cancels)
+      b->fused = blk == 0 || blk == 3;
+    }
+    const auto t_step = std::chrono::steady_clock::now();
     for (auto &m : b->members) m->timestep++;
     if (b->use_graphs) HIPCHK(hipGraphLaunch(b->x_grad, b->stream));
     else batch_grad(b, b->chunk);
     HIPCHK(hipEventRecord(b->ev_grad, b->stream));
+    const bool fused = b->fused && eligible;
+    // a member's neighbour stage (decide, or pbc + ghosts + bins + lists,
+    // whose sizes the host reads back) needs positions only: it runs beside
+    // the gradient block, and the pair phase waits for the gradient
     for (int i = 0; i < b->n; i++) {
       phafd_hip_ctx *c = b->members[i].get();
-      HIPCHK(hipStreamWaitEvent(c->stream, b->ev_grad, 0));
       phase_neigh(c);
-      phase_pair(c);
-      if (FixDef *fc = fused_chain_fix(c)) integrate_both_halves(c, *fc);
-      else phase_initial(c);
+      HIPCHK(hipStreamWaitEvent(c->stream, b->ev_grad, 0));
+      if (fused) {
+        fused_prepare(c);
+      } else {
+        phase_pair(c);
+        if (FixDef *fc = fused_chain_fix(c)) integrate_both_halves(c, *fc);
+        else phase_initial(c);
+      }
       HIPCHK(hipEventRecord(b->ev_m[i], c->stream));
     }
     for (int i = 0; i < b->n; i++) HIPCHK(hipStreamWaitEvent(b->stream, b->ev_m[i], 0));
+    if (fused) {
+      batch_pair_fused(b, true);
+      HIPCHK(hipEventRecord(b->ev_pair, b->stream));
+      for (int i = 0; i < b->n; i++) {
+        phafd_hip_ctx *c = b->members[i].get();
+        HIPCHK(hipStreamWaitEvent(c->stream, b->ev_pair, 0));
+        if (!fused_chain_fix(c)) phase_initial(c);
+      }
+    }
     if (b->use_graphs) HIPCHK(hipGraphLaunch(b->x_spec, b->stream));
     else batch_spec(b, b->chunk);
     for (int i = 0; i < b->n; i++) {
@@ -2889,6 +3103,15 @@ void batch_steps(phafd_hip_batch *b, int nsteps) {
       HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->fd.phi.p, sizeof(double), hipMemcpyDeviceToHost, b->stream));
     }
     batch_sync(b);
+    b->steps_run++;
+    if (trial) {
+      const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_step).count();
+      if (b->trial % BATCH_TRIAL != 0) b->trial_ms[fused ? 1 : 0] += ms / (2 * (BATCH_TRIAL - 1));
+      if (++b->trial == 4 * BATCH_TRIAL) {
+        b->decided = true;
+        b->fused = b->trial_ms[1] < b->trial_ms[0];
+      }
+    }
     probe_collect();
     for (int i = 0; i < b->n; i++) {   // the checks of src/integrate.cpp:190-201, per member
       phafd_hip_ctx *c = b->members[i].get();
@@ -2929,6 +3152,7 @@ int phafd_hip_batch_create(const phafd_hip_desc *desc, int nmembers, phafd_hip_b
       b->members.push_back(std::move(c));
     }
     HIPCHK(hipEventCreateWithFlags(&b->ev_grad, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&b->ev_pair, hipEventDisableTiming));
     b->ev_m.resize(b->n);
     for (auto &e : b->ev_m) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return 0;
@@ -3011,6 +3235,30 @@ int phafd_hip_batch_setup(phafd_hip_batch *b, double dt) {
       if (b->tune_pair_ms[k] < best) { best = b->tune_pair_ms[k]; bestk = ks[k]; }
     }
     batch_assign_streams(b, bestk);
+    // the pair phase as one launch per pass for every member, timed on the
+    // same state (the chains' step left out of both timings)
+    b->tune_fused_ms = 0;
+    b->fused = false;
+    MultiShape best_shape;
+    if (fused_eligible(b)) {
+      for (auto &m : b->members) fused_prepare(m.get());
+      batch_sync(b);
+      // the launch shape: atoms per item (threads) x persistent grid cap
+      const MultiShape shapes[6] = {{256, 1024}, {256, 2048}, {256, 4096}, {128, 2048}, {128, 4096}, {64, 4096}};
+      double bs = 1e30;
+      for (int k = 0; k < 6; k++) {
+        b->shape = shapes[k];
+        b->tune_shape_ms[k] = time_ms(b, 2, [&] { batch_pair_fused(b, false); });
+        if (b->tune_shape_ms[k] < bs) { bs = b->tune_shape_ms[k]; best_shape = shapes[k]; }
+      }
+      b->shape = best_shape;
+      b->tune_fused_ms = bs;
+      b->fused = b->fused_mode == 1 || (b->fused_mode < 0 && b->tune_fused_ms < best);
+      b->trial = 0;
+      b->steps_run = 0;
+      b->decided = false;
+      b->trial_ms[0] = b->trial_ms[1] = 0;
+    }
     batch_grad(b, b->chunk);   // the state the first step expects (start_of_step recomputes it anyway)
     batch_sync(b);
     destroy_graphs(b);
@@ -3035,6 +3283,33 @@ int phafd_hip_batch_set_graphs(phafd_hip_batch *b, int on) {
   });
 }
 
+int phafd_hip_batch_set_fused(phafd_hip_batch *b, int mode) {
+  return bguard(b, [&] {
+    if (mode < -1 || mode > 1) throw std::invalid_argument("mode must be -1 (autotuned), 0 or 1");
+    b->fused_mode = mode;
+    if (b->setup_done && mode >= 0) b->fused = mode == 1 && fused_eligible(b);
+    return 0;
+  });
+}
+
+int phafd_hip_batch_fused_info(phafd_hip_batch *b, double *out14) {
+  return bguard(b, [&] {
+    out14[0] = b->fused ? 1 : 0;
+    out14[1] = b->tune_fused_ms;
+    double best = 1e30;
+    for (int k = 0; k < 4; k++)
+      if (b->tune_pair_ms[k] > 0) best = std::min(best, b->tune_pair_ms[k]);
+    out14[2] = best < 1e30 ? best : 0.0;
+    out14[3] = b->shape.threads;
+    out14[4] = b->shape.cap;
+    for (int k = 0; k < 6; k++) out14[5 + k] = b->tune_shape_ms[k];
+    out14[11] = b->decided ? 1 : 0;
+    out14[12] = b->trial_ms[0];
+    out14[13] = b->trial_ms[1];
+    return 0;
+  });
+}
+
 int phafd_hip_batch_info(phafd_hip_batch *b, double *out12) {
   return bguard(b, [&] {
     out12[0] = b->n;
@@ -3055,6 +3330,8 @@ int phafd_hip_batch_destroy(phafd_hip_batch *b) {
   destroy_graphs(b);
   for (auto &m : b->members) (void)phafd_hip_destroy(m.release());
   if (b->ev_grad) (void)hipEventDestroy(b->ev_grad);
+  if (b->ev_pair) (void)hipEventDestroy(b->ev_pair);
+  if (b->h_desc) (void)hipHostFree(b->h_desc);
   for (auto e : b->ev_m) if (e) (void)hipEventDestroy(e);
   for (auto s : b->mstreams) {
     scan_forget_stream(s);

@@ -64,6 +64,32 @@ def test_members_match_solo_runs_bitwise(n):
     b.close()
 
 
+@pytest.mark.parametrize("n", [64, 32])
+def test_one_launch_per_pass_matches_per_member_launches_bitwise(n):
+    # the members' pair phase in one launch per pass (grid-atom force walk,
+    # mu sums, lj/cut, fold levels, chain steps over every member) against
+    # per-member launches and against the solo runs, through rebuilds
+    ws = _members(n, 4)
+    f = _batch(ws)
+    f.set_fused(1)
+    m = _batch(ws)
+    m.set_fused(0)
+    assert f.fused_info()["fused"] and not m.fused_info()["fused"]
+    for _ in range(3):
+        f.run(5)
+        m.run(5)
+        for i in range(len(ws)):
+            _assert_same(_state(f.member(i)), _state(m.member(i)), f"member {i}")
+    assert f.member(3).counts()["ncalls"] > 1
+    for i, w in enumerate(ws):
+        s = make_gpu(w)
+        s.set_option(P.OPT_NEXT_GRADIENT, 0)
+        s.run(15)
+        _assert_same(_state(f.member(i)), _state(s), f"member {i} vs solo")
+    f.close()
+    m.close()
+
+
 def test_graphs_match_eager_launches_bitwise():
     ws = _members(64, 4)
     g = _batch(ws, graphs=True)
