@@ -136,7 +136,8 @@ struct phafd_hip_ctx {
   DBuf<unsigned long long> counters;   // [nless, ngreat]
   DBuf<int> flags;                     // [0] rebuild, [1] overflow
   DBuf<double> esum;                   // energy accumulators
-  int *h_flags = nullptr;              // pinned, 16 ints: [2..3] phi(0,0,0), [4] scalar reads, [6..7] window
+  int *h_flags = nullptr;              // pinned, 32 ints: [2..3] phi(0,0,0), [4] scalar reads, [6..7] window,
+                                       // [8..15] the step flags, [16..17] phi(0,0,0) of odd pipelined steps
                                        // stop step and flags, [8..15] flags
   // owned atoms
   int nowned = 0, ntypes = 0;
@@ -210,6 +211,7 @@ struct phafd_hip_ctx {
   // timers
   hipEvent_t ev[9] = {};
   hipEvent_t ev_pre[2] = {};   // a start_of_step enqueued ahead of the last read-back
+  hipEvent_t evp[2][10] = {};  // pipelined steps: hook events 0-8 and the flags' read-back, per step parity
   double tacc[8] = {0};
   bool fuse_fh = true;
   bool fuse_spectral = true;
@@ -1196,6 +1198,96 @@ void capture_spectrum(phafd_hip_ctx *c, int k, const float2 *src) {
   HIPCHK(hipGetLastError());
 }
 
+// see run_steps_t: the steps with the flags read back mid-step, step k + 1
+// enqueued before step k ends
+template <class R>
+void run_steps_pipelined(phafd_hip_ctx *c, int nsteps, bool fused, bool zsplit, bool next_ok, FHPreT<R> &pre,
+                         FixDef *fc) {
+  auto phi0_slot = [&](int set) { return c->h_flags + (set ? 16 : 2); };
+  // the timers and the NaN check of a completed step
+  auto finish = [&](int set, int64_t step) {
+    hipEvent_t *E = c->evp[set];
+    for (int k = 0; k < 7; k++) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, E[k], E[k + 1]));
+      c->tacc[k] += ms;
+    }
+    float tot = 0;
+    HIPCHK(hipEventElapsedTime(&tot, E[0], E[8]));
+    c->tacc[7] += tot;
+    R phi0;
+    std::memcpy(&phi0, phi0_slot(set), sizeof(R));
+    if (std::isnan(phi0)) {
+      sync(c);   // a later step in flight ends first
+      c->timestep = step;
+      c->grad_ready = false;
+      c->pending_decision = -1;
+      throw std::runtime_error("NAN encountered in phi.");
+    }
+  };
+  c->grad_ready = false;
+  int64_t prev_step = -1;
+  for (int it = 0; it < nsteps; it++) {
+    const int set = it & 1;
+    hipEvent_t *E = c->evp[set];
+    c->timestep++;
+    const int64_t step = c->timestep;
+    const bool last = it == nsteps - 1;
+    c->store_muhat = last;
+    HIPCHK(hipEventRecord(E[0], c->stream));
+    phase_start(c);
+    HIPCHK(hipEventRecord(E[1], c->stream));
+    phase_neigh(c);
+    HIPCHK(hipEventRecord(E[2], c->stream));
+    phase_pair(c);
+    HIPCHK(hipEventRecord(E[3], c->stream));
+    if (fc) integrate_both_halves(c, *fc);
+    else phase_initial(c);
+    HIPCHK(hipEventRecord(E[4], c->stream));
+    // next step's rebuild decision (Neighbor::decide): the atoms are final
+    if (c->neigh_set) enqueue_decide(c);
+    enqueue_flags_readback(c);
+    HIPCHK(hipEventRecord(E[9], c->stream));
+    if (!fused) phase_postforce_t<R>(c);
+    if (zsplit) phase_prefinal_split<R>(c, fused ? &pre : nullptr);
+    else phase_prefinal_t<R>(c, fused ? &pre : nullptr);
+    HIPCHK(hipEventRecord(E[5], c->stream));
+    const bool capture = c->fourier_capture && last;
+    if (capture) capture_spectrum(c, 0, c->fields<R>().phihat.p);   // in_fourier: ft_phi before the update
+    const bool next = next_ok && !last;
+    if (zsplit) phase_final_split<R>(c, !fc, next);
+    else phase_final_t<R>(c, !fc);
+    HIPCHK(hipEventRecord(E[6], c->stream));
+    if (zsplit) phase_postfinal_split<R>(c, next);
+    else phase_postfinal_t<R>(c);
+    if (capture) {   // ft_chempot: the last step of a run stores the complete muhat
+      capture_spectrum(c, 1, c->fields<R>().muhat.p);
+      c->cap_valid = true;
+    }
+    HIPCHK(hipEventRecord(E[7], c->stream));
+    // NaN check of phi(0,0,0) (src/integrate.cpp:190-201)
+    HIPCHK(hipMemcpyAsync(phi0_slot(set), c->fields<R>().phi.p, sizeof(R), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(E[8], c->stream));
+    // this step's flags (mid-step); the previous step is complete by now
+    HIPCHK(hipEventSynchronize(E[9]));
+    if (prev_step >= 0) finish(set ^ 1, prev_step);
+    try {
+      check_step_flags(c, step);
+    } catch (...) {
+      // the reference checks phi at the end of the step, before the next
+      // step's neighbour decision: a NaN in phi takes precedence
+      sync(c);
+      c->grad_ready = false;
+      finish(set, step);
+      throw;
+    }
+    prev_step = step;
+  }
+  sync(c);
+  probe_collect();
+  if (prev_step >= 0) finish((nsteps - 1) & 1, prev_step);
+}
+
 template <class R>
 void run_steps_t(phafd_hip_ctx *c, int nsteps) {
   FHPreT<R> pre;
@@ -1211,6 +1303,21 @@ void run_steps_t(phafd_hip_ctx *c, int nsteps) {
   c->grad_ready = false;
   FixDef *fc = fused_chain_fix(c);
   if (c->neigh_set) clear_decide_slot(c, c->timestep + 1);
+  // Pipelined steps: when every atom move of a step happens at
+  // initial_integrate (the fused chain fix, or no atom integrator), the next
+  // step's rebuild decision is known right after it.  The decision and the
+  // flags are then read back mid-step, and the host enqueues step k + 1
+  // while the device still runs step k's grid passes; the NaN check of step
+  // k's phi(0,0,0) happens once step k + 1 is enqueued (after PHAFD_E_NAN the
+  // fields hold one step more than the step reported).
+  bool atom_final = false;
+  for (auto &f : c->fixes)
+    if (f->kind != FixDef::CONJ && f->kind != FixDef::FH && f->kind != FixDef::GRADPHI && f.get() != fc)
+      atom_final = true;
+  if (!c->touch && !atom_final) {   // an active kernel probe collects its events after the run
+    run_steps_pipelined<R>(c, nsteps, fused, zsplit, next_ok, pre, fc);
+    return;
+  }
   bool started = false;   // this step's start_of_step went in before the last read-back
   for (int it = 0; it < nsteps; it++) {
     if (!started) c->timestep++;
@@ -1468,9 +1575,11 @@ void init_ctx(phafd_hip_ctx *c, const phafd_hip_desc *desc, const MemberSlot *sl
     c->mass64.alloc(1);
     c->flags.alloc(8);
     c->esum.alloc(8);
-    HIPCHK(hipHostMalloc((void **)&c->h_flags, 64, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&c->h_flags, 128, hipHostMallocDefault));
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
     for (auto &e : c->ev_pre) HIPCHK(hipEventCreate(&e));
+    for (auto &set : c->evp)
+      for (auto &e : set) HIPCHK(hipEventCreate(&e));
     for (int w = PHAFD_PHI; w <= PHAFD_GRADZ; w++)
       fill_zero(c->fp32 ? (void *)field_ptr<float>(c, w) : (void *)field_ptr<double>(c, w),
                 c->real_bytes() * c->N, c->stream);
@@ -1512,6 +1621,8 @@ int phafd_hip_destroy(phafd_hip_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
   for (auto &e : c->ev_pre) if (e) (void)hipEventDestroy(e);
+  for (auto &set : c->evp)
+    for (auto &e : set) if (e) (void)hipEventDestroy(e);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->h_ends) (void)hipHostFree(c->h_ends);
   for (auto &d : c->dump) {
