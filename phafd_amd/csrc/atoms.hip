@@ -803,14 +803,20 @@ __device__ __forceinline__ void ga_force_atom(size_t q, const GridViewT<R> &gv, 
   ForceAcc<R> acc;
   acc.F = F0;
   bool any = false;
+  // the pair entries of the next batch load while this batch's grid
+  // values load and its terms are formed
+  int2 nx[GA_B];
+#pragma unroll
+  for (int u = 0; u < GA_B; u++) nx[u] = s + u < e ? ps[s + u] : make_int2(0, 0);
   for (int c0 = s; c0 < e; c0 += GA_B) {
     int pp[GA_B], so[GA_B];
 #pragma unroll
     for (int u = 0; u < GA_B; u++) {
-      const int2 v = c0 + u < e ? ps[c0 + u] : make_int2(0, 0);
-      pp[u] = v.x;
-      so[u] = v.y;
+      pp[u] = nx[u].x;
+      so[u] = nx[u].y;
     }
+#pragma unroll
+    for (int u = 0; u < GA_B; u++) nx[u] = c0 + GA_B + u < e ? ps[c0 + GA_B + u] : make_int2(0, 0);
     bool in[GA_B];
     double rs[GA_B];
     R ph[GA_B], fx[GA_B], fy[GA_B], fz[GA_B];
