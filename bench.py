@@ -505,7 +505,11 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4"])
-    ap.add_argument("--probe", default="c2r_x", help="kernel timed with HIP events for the roofline")
+    ap.add_argument("--probe", default=None,
+                    help="kernel timed with HIP events for the roofline (default: the step's longest kernel, "
+                         "ga_force; c2r_x for the batched workload)")
+    ap.add_argument("--fft-probe", default="z_update",
+                    help="FFT-side kernel also timed, over extra steps after the timed region")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (counter calibration) and exit")
     ap.add_argument("--cpu-steps", type=int, default=10)
@@ -532,8 +536,10 @@ def main():
         return
 
     if args.workload == "c4":
+        args.probe = args.probe or "c2r_x"
         bench_batch(args)
         return
+    args.probe = args.probe or "ga_force"
 
     import phafd_amd as P
     wl = WORKLOADS[args.workload]
@@ -551,6 +557,20 @@ def main():
     timers = s.timers()
     counts = s.counts()
     ref = snapshot(s, P)
+    fft_roof = None
+    if args.fft_probe and args.fft_probe != "none":
+        # the longest FFT-side kernel, over 20 further steps (outside the
+        # timed region; the fields compared below are those of the timed run)
+        s.probe(args.fft_probe)
+        s.run(20)
+        fms, flaunch, fbytes = s.probe(None)
+        if flaunch:
+            favg = fms / flaunch
+            fach = fbytes / flaunch / (favg * 1e-3) / 1e9
+            fft_roof = {"bound": "hbm", "kernel": args.fft_probe, "achieved": fach, "peak": 8.0e3, "unit": "GB/s",
+                        "frac": fach / 8.0e3, "avg_launch_ms": favg, "launches": flaunch,
+                        "algorithmic_bytes_per_launch": fbytes / flaunch,
+                        "traffic": read_traffic(args.fft_probe, args.workload)}
     s.close()
 
     N = n ** 3
@@ -607,6 +627,8 @@ def main():
                      "hipmemcpy_d2d_GBs": copy_gbs,
                      "frac_of_hipmemcpy_d2d": (achieved / copy_gbs) if (achieved and copy_gbs) else None},
     }
+    if fft_roof is not None:
+        out["roofline_fft"] = fft_roof
     if fp32 is not None:
         out["fp32"] = fp32
     if sur is not None:

@@ -333,7 +333,8 @@ int phafd_hip_set_option(phafd_hip_ctx *ctx, int key, double value);
  * r2c x, y fwd, y inv, z fwd, z inv, c2r x, grad multiply, update, FH,
  * grid-atom deposit, grid-atom force, grid list count, grid list fill,
  * half list count, half list fill, lj/cut, fused z gradient, fused z
- * update, the surrogate's full-resolution 16->16 convolution); kernel_id < 0
+ * update, the surrogate's full-resolution 16->16 convolution, the c2r x
+ * pass of phi with its x gradient); kernel_id < 0
  * stops it.  out3 = {total ms, launches, algorithmic bytes (flops for the
  * convolution)} accumulated so far; also records during
  * phafd_hip_run_surrogate. */

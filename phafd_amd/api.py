@@ -710,7 +710,7 @@ class System:
 
     KERNELS = ["r2c_x", "y_fwd", "y_inv", "z_fwd", "z_inv", "c2r_x", "grad_multiply", "update", "fh",
                "ga_deposit", "ga_force", "grid_count", "grid_fill", "half_count", "half_fill", "lj",
-               "z_grad", "z_update", "unet_conv"]
+               "z_grad", "z_update", "unet_conv", "c2r_xd"]
 
     def probe(self, kernel=None):
         """Start (kernel name) or stop (None) HIP-event timing of one kernel;

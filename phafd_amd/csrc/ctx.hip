@@ -929,10 +929,12 @@ void grid_atom_pass(phafd_hip_ctx *c, PairDef &p, double *energy, bool first) {
   if (c->ga_calls != c->ncalls) build_ga_pairs(c);
   DBuf<R> &mt = ga_mterm<R>(c);
   mt.alloc((size_t)c->ga_nent + 1);
+  // algorithmic bytes: phi and its gradient read once (4 fields), each
+  // gathered atom's position read and force written
   probe_begin(K_GA_FORCE, c->stream);
   launch_ga_force(c->ngathered, gvr, ga, c->gx.p, c->binatoms.p, c->ga_aoff.p, c->ga_ps.p, mt.p, c->gF.p, energy,
                   c->stream);
-  probe_end(K_GA_FORCE, c->stream, 64.0 * c->ngathered);
+  probe_end(K_GA_FORCE, c->stream, 4.0 * sizeof(R) * c->N + 64.0 * c->ngathered);
   probe_begin(K_GA_DEPOSIT, c->stream);
   if (!(c->eager_gridlist && c->grid_built)) {
     launch_ga_list_mu(c->N, Fs.mu.p, c->ga_poff.p, mt.p, first, c->stream);
@@ -3140,7 +3142,7 @@ void batch_pair_fused(phafd_hip_batch *b, bool with_chains) {
   for (auto &g : ga) ng_all += g.ng;
   probe_begin(K_GA_FORCE, b->stream);
   launch_ga_force_multi(nm, dga, dboff_ga, P.items_ga, c0->nx, c0->ny, b->shape, b->stream);
-  probe_end(K_GA_FORCE, b->stream, 64.0 * ng_all);
+  probe_end(K_GA_FORCE, b->stream, 4.0 * sizeof(double) * b->N * nm + 64.0 * ng_all);
   launch_ga_list_mu_multi(nm, b->N, dga, b->stream);
   if (P.lj) launch_lj_multi(nm, dlj, dboff_ga, P.items_ga, b->shape, b->stream);
   for (int L = 0; L < P.nlev; L++) {

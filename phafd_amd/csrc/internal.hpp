@@ -73,7 +73,7 @@ struct DBuf {   // device buffer owned by the context (or a view into another's)
 enum KernelId {
   K_R2C_X = 0, K_Y_FWD, K_Y_INV, K_Z_FWD, K_Z_INV, K_C2R_X, K_GRAD_MUL, K_UPDATE, K_FH,
   K_GA_DEPOSIT, K_GA_FORCE, K_GRID_COUNT, K_GRID_FILL, K_HALF_COUNT, K_HALF_FILL, K_LJ,
-  K_Z_GRAD, K_Z_UPDATE, K_UNET_CONV, K_NKERNELS
+  K_Z_GRAD, K_Z_UPDATE, K_UNET_CONV, K_C2R_XD, K_NKERNELS
 };
 struct KernelProbe {
   int id = -1;
