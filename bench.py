@@ -66,7 +66,7 @@ def setup_inputs(wl):
     return phi, xyz, mol
 
 
-def configure(s, wl, phi, xyz, mol, is_oracle=False, setup=True):
+def configure(s, wl, phi, xyz, mol, is_oracle=False, setup=True, noise=0, noise_temp=1.0):
     nch, nb = wl["nchains"], wl["nbeads"]
     s.set_field(0, phi)
     s.set_atoms(xyz, np.zeros(len(xyz), np.int32), tags=np.arange(len(xyz), dtype=np.int32), mols=mol)
@@ -77,7 +77,7 @@ def configure(s, wl, phi, xyz, mol, is_oracle=False, setup=True):
         s.add_pair(2, 1.122462048, 1.0, 1.0)
     else:
         s.pair_ljcut(1.122462048, 1.0, 1.0)
-    s.fix_conjugate(5678, 1.0, 1.0, 1.0, 1.0, 0)
+    s.fix_conjugate(5678, 1.0, noise_temp, 1.0, 1.0, noise)
     s.fix_floryhuggins(1.0, 1.0, wl["chi"])
     st = (np.arange(nch) * nb).astype(np.int32)
     s.fix_chain(st, st + nb, 1.0, 100.0, 10.0, 1.0)
@@ -516,6 +516,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
+    ap.add_argument("--no-noise", action="store_true", help="skip the thermal-noise leg")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="steps of the timed plain-vs-surrogate comparison (0: --steps); rounded up to a multiple "
                          "of the network's K")
@@ -592,6 +593,22 @@ def main():
                 "rel_err_vs_fp64": err, "steps_compared": args.warmup + args.steps,
                 "form": "FP32 FFT + fields, Kahan-compensated grid-atom sums, FP64 k=0 mode, atoms and energies"}
 
+    # thermal noise on (grid/conjugate/volfrac temp 0.01, generated on the
+    # device per update; at temp 1 the noise of a unit-spacing grid, about
+    # 0.14 per point per step at dt 1e-3, drives phi out of [-1, 2] and the
+    # FH logarithm to NaN): the same workload, timed over as many steps
+    noise_leg = None
+    if not args.no_noise:
+        sn = P.System(n, n, n, [0, 0, 0], [wl["box"]] * 3, zghost=wl["zghost"])
+        configure(sn, wl, phi, xyz, mol, noise=P.NOISE_DEVICE, noise_temp=0.01)
+        sn.run(args.warmup)
+        t2 = time.perf_counter()
+        sn.run(args.steps)
+        eln = time.perf_counter() - t2
+        sn.close()
+        noise_leg = {"ms_per_step": 1e3 * eln / args.steps, "value": N * args.steps / eln, "unit": "grid-pt·steps/s",
+                     "form": "device Philox noise (PHAFD_NOISE_DEVICE), grid/conjugate/volfrac temp 0.01, mobility 1"}
+
     sur = None
     if not args.no_surrogate and not wl["ljish"] and os.path.exists(SURROGATE):
         sur = surrogate_leg(args, wl, phi, xyz, mol)
@@ -629,6 +646,8 @@ def main():
     }
     if fft_roof is not None:
         out["roofline_fft"] = fft_roof
+    if noise_leg is not None:
+        out["noise_device"] = noise_leg
     if fp32 is not None:
         out["fp32"] = fp32
     if sur is not None:
