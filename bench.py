@@ -246,19 +246,30 @@ def read_traffic(kernel, workload):
     return None
 
 
-SURROGATE = os.path.join(ROOT, "phafd_amd", "data", "surrogate_gaussian64.npz")
+DATA = os.path.join(ROOT, "phafd_amd", "data")
+SURROGATE = os.path.join(DATA, "surrogate_gaussian64.npz")          # K = 100, trained at 64^3
+SURROGATE_K10 = os.path.join(DATA, "surrogate_gaussian128_k10.npz")  # K = 10, trained at 128^3
+SURROGATE_LJISH = os.path.join(DATA, "surrogate_ljish64.npz")       # K = 100, LJish coupling, 64^3
 
 
-def surrogate_leg(args, wl, phi, xyz, mol):
+def surrogate_networks(wl):
+    """Committed networks for the workload's grid-atom coupling (the first
+    is the headline one)."""
+    names = [SURROGATE_LJISH] if wl["ljish"] else [SURROGATE, SURROGATE_K10]
+    return [p for p in names if os.path.exists(p)]
+
+
+def surrogate_leg(args, wl, phi, xyz, mol, weights):
     """The surrogate-accelerated form on the same workload and initial state:
-    A steps (accuracy_steps, a multiple of the network's K; default 1000) run
-    by the plain FP64 path and by blocks of K (network + check, plain
-    fallback), both timed; phi compared at the end."""
+    A steps (accuracy_steps, a multiple of every network's K; default 1000)
+    run by the plain FP64 path once and, per network, by blocks of K
+    (network + check, plain fallback), all timed; phi and the bead positions
+    compared at the end.  One result per network."""
     import phafd_amd as P
-    z = np.load(SURROGATE)
-    K, tol = int(z["K"][0]), float(z["tol"][0])
+    Ks = [int(np.load(w)["K"][0]) for w in weights]
+    L = int(np.lcm.reduce(Ks))
     A = args.accuracy_steps if args.accuracy_steps else args.steps
-    A = max(K, (A + K - 1) // K * K)
+    A = max(L, (A + L - 1) // L * L)
     n = wl["n"]
 
     def system():
@@ -274,8 +285,17 @@ def surrogate_leg(args, wl, phi, xyz, mol):
     ref = a.get_field(P.PHI)
     uref = a.get_atoms()[1]
     a.close()
+    return [surrogate_run(P, system, w, A, plain, ref, uref, u0, n) for w in weights]
+
+
+def surrogate_run(P, system, weights, A, plain, ref, uref, u0, n):
+    """A steps of one network's surrogate form from a fresh system, timed,
+    against the plain run's phi (ref) and bead positions (uref)."""
+    z = np.load(weights)
+    K, tol = int(z["K"][0]), float(z["tol"][0])
+    trained = int(z["n"][0]) if "n" in z.files else 64
     net = P.UNet(seed=1)
-    net.load(SURROGATE)
+    net.load(weights)
     s = system()
     s.run_surrogate(net, 0, K, tol)   # allocations outside the timed region
     s.probe("unet_conv")
@@ -288,7 +308,8 @@ def surrogate_leg(args, wl, phi, xyz, mol):
     s.close()
     net.close()
     conv_tflops = pflops / (pms * 1e-3) / 1e12 if plaunch else None
-    return {"value": n ** 3 * A / el, "unit": "grid-pt·steps/s", "ms_per_step": 1e3 * el / A, "steps": A,
+    return {"network": os.path.basename(weights), "trained_grid": [trained] * 3,
+            "value": n ** 3 * A / el, "unit": "grid-pt·steps/s", "ms_per_step": 1e3 * el / A, "steps": A,
             "plain_fp64_ms_per_step_same_run": 1e3 * plain / A, "speedup_vs_fp64": plain / el,
             f"phi_rel_l2_vs_fp64_after_{A}_steps": rel_err(got, ref), "K": K, "tol": tol,
             "blocks": int(st["blocks"]), "accepted": int(st["accepted"]), "fallbacks": int(st["fallbacks"]),
@@ -517,6 +538,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
     ap.add_argument("--no-noise", action="store_true", help="skip the thermal-noise leg")
+    ap.add_argument("--surrogate-net", action="append", default=None, metavar="NPZ",
+                    help="network(s) of the surrogate leg (default: the committed ones for the workload's coupling)")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="steps of the timed plain-vs-surrogate comparison (0: --steps); rounded up to a multiple "
                          "of the network's K")
@@ -610,8 +633,9 @@ def main():
                      "form": "device Philox noise (PHAFD_NOISE_DEVICE), grid/conjugate/volfrac temp 0.01, mobility 1"}
 
     sur = None
-    if not args.no_surrogate and not wl["ljish"] and os.path.exists(SURROGATE):
-        sur = surrogate_leg(args, wl, phi, xyz, mol)
+    nets = args.surrogate_net if args.surrogate_net else surrogate_networks(wl)
+    if not args.no_surrogate and nets:
+        sur = surrogate_leg(args, wl, phi, xyz, mol, nets)
 
     nbeads = wl["nchains"] * wl["nbeads"]
     ms_per_step = 1e3 * elapsed / args.steps
@@ -651,7 +675,9 @@ def main():
     if fp32 is not None:
         out["fp32"] = fp32
     if sur is not None:
-        out["surrogate"] = sur
+        out["surrogate"] = sur[0]
+        if len(sur) > 1:
+            out["surrogate_other_networks"] = sur[1:]
     if args.dump_every > 0:
         out["dumps"] = dump_leg(args, wl, phi, xyz, mol, ms_per_step)
     if args.diagnostics:

@@ -31,12 +31,13 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int TPB = 256;
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-// round to nearest even (the conversion PyTorch applies)
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+// round to nearest even (the conversion PyTorch applies), by gfx950's
+// v_cvt_pk_bf16_f32 (one instruction per pair)
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+__device__ __forceinline__ uint32_t f2bf2(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
 }
 __device__ __forceinline__ bool bf_pos(uint16_t h) { return !(h & 0x8000u) && h != 0; }
 
@@ -45,6 +46,21 @@ __device__ __forceinline__ int wrap1(int i, int n) { return i < 0 ? i + n : (i >
 size_t nblk(size_t n) {
   size_t b = (n + TPB - 1) / TPB;
   return b < 1 ? 1 : (b > 65535u * 4 ? 65535u * 4 : b);
+}
+
+// 8 bf16 channels <-> float
+__device__ __forceinline__ void unpack8(const u32x4 &u, float f[8]) {
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    f[2 * e] = __uint_as_float(u[e] << 16);
+    f[2 * e + 1] = __uint_as_float(u[e] & 0xffff0000u);
+  }
+}
+__device__ __forceinline__ u32x4 pack8(const float f[8]) {
+  u32x4 u;
+#pragma unroll
+  for (int e = 0; e < 4; e++) u[e] = f2bf2(f[2 * e], f[2 * e + 1]);
+  return u;
 }
 
 // ------------------------------------------------------------ convolution
@@ -58,6 +74,30 @@ size_t nblk(size_t n) {
 // group][y][x], 16 bytes per entry): the 16 lanes of a row read 256
 // contiguous bytes, free of bank conflicts.  Grid: (nx/16) * (ny/(4 MTR)) *
 // (nz/zb) blocks.
+// K order of a layer's packed weights: chunk c and lane group kg (16 lanes,
+// one 8-channel group each) -> the tap (return value, -1 past the last
+// group) and the channel group cg.  3^3 kernels with two or more channel
+// groups: chunks come in threes, one per dy (dy = c % 3 - 1), over load
+// phases p = c / 3 of four (dz, dx, cg) columns (column 4p + kg), so a wave
+// reads a column's rows once for the three dy taps.  Otherwise group
+// g = 4c + kg, tap g / CG, cg g % CG.
+__host__ __device__ inline int conv_k_group(int c, int kg, int CG, int taps, int &cg) {
+  if (taps == 27 && CG >= 2) {
+    const int q = 4 * (c / 3) + kg;
+    if (q >= 9 * CG) return -1;
+    const int comb = q / CG;   // dz * 3 + dx (each 0..2)
+    cg = q % CG;
+    return (comb / 3) * 9 + (c % 3) * 3 + comb % 3;
+  }
+  const int g = 4 * c + kg;
+  if (g >= taps * CG) return -1;
+  cg = g % CG;
+  return g / CG;
+}
+__host__ __device__ constexpr int conv_chunks(int CG, int taps) {
+  return (taps == 27 && CG >= 2) ? 3 * ((9 * CG + 3) / 4) : (taps * CG + 3) / 4;
+}
+
 template <int CIN, int TAPS, int MTR>
 struct ConvTile {
   static constexpr int CG = CIN / 8;
@@ -68,15 +108,20 @@ struct ConvTile {
   static constexpr int PLANE = HY * HX * CG;   // 16-byte entries per plane
 };
 
-template <int CIN, int COUT, int TAPS, bool RELU, int MTR>
-__global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const u32x4 *__restrict__ W,
-                                              const float *__restrict__ bias, uint16_t *__restrict__ Y, int nx,
-                                              int ny, int nz, int zb) {
+// Q (optional): a coarse (half-resolution) input of the same channels added
+// on load, X[v] + Q[parent(v)] rounded to bf16 -- the decoder's skip sum,
+// bit for bit what k_upadd would have stored.
+template <int CIN, int COUT, int TAPS, bool RELU, int MTR, bool UP>
+__global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const u32x4 *__restrict__ Q,
+                                              const u32x4 *__restrict__ W, const float *__restrict__ bias,
+                                              uint16_t *__restrict__ Y, int nx, int ny, int nz, int zb) {
   using T = ConvTile<CIN, TAPS, MTR>;
   constexpr int CG = T::CG;
   constexpr int G = TAPS * CG;
-  constexpr int NC = (G + 3) / 4;
+  constexpr int NC = conv_chunks(CG, TAPS);
   constexpr int NT = COUT / 16;
+  constexpr bool DYR = TAPS == 27 && CG >= 2;   // rows reused across the dy taps
+  constexpr int NP = NC / 3;                     // load phases (DYR)
   constexpr int H = TAPS == 27 ? 1 : 0;   // halo width
   constexpr int PER = (T::PLANE + 255) / 256;
   __shared__ u32x4 tile[T::HZ * T::PLANE];
@@ -92,7 +137,7 @@ __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const
   const int lx = lane & 15;
   // this thread's share of a plane load: offset within a z plane of X (group
   // fastest, coalesced) and the group-major LDS slot; the same every plane
-  int goff[PER], loff[PER];
+  int goff[PER], loff[PER], qoff[PER];
 #pragma unroll
   for (int k = 0; k < PER; k++) {
     const int t = threadIdx.x + 256 * k;
@@ -101,13 +146,42 @@ __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const
     const int xx = wrap1(x0 - H + hx, nx), yy = wrap1(y0 - H + hy, ny);
     goff[k] = t < T::PLANE ? (yy * nx + xx) * CG + cg : -1;
     loff[k] = (cg * T::HY + hy) * T::HX + hx;
+    qoff[k] = ((yy >> 1) * (nx >> 1) + (xx >> 1)) * CG + cg;
   }
   const size_t zstride = (size_t)ny * nx * CG;
-  auto load_plane = [&](u32x4 *dst, int z) {
-    const u32x4 *Xz = X + (size_t)wrap1(z, nz) * zstride;
+  const size_t qzstride = (size_t)(ny >> 1) * (nx >> 1) * CG;
+  // plane loads in two halves: fetch (global -> registers) and put
+  // (registers -> LDS), so the fetch of the next plane is in flight while
+  // the current one is computed
+  u32x4 px[PER], pq[UP ? PER : 1];
+  auto fetch = [&](int z) {
+    const int wz = wrap1(z, nz);
+    const u32x4 *Xz = X + (size_t)wz * zstride;
 #pragma unroll
     for (int k = 0; k < PER; k++)
-      if (goff[k] >= 0) dst[loff[k]] = Xz[goff[k]];
+      if (goff[k] >= 0) px[k] = Xz[goff[k]];
+    if (UP) {
+      const u32x4 *Qz = Q + (size_t)(wz >> 1) * qzstride;
+#pragma unroll
+      for (int k = 0; k < (UP ? PER : 0); k++)
+        if (goff[k] >= 0) pq[k] = Qz[qoff[k]];
+    }
+  };
+  auto put = [&](u32x4 *dst) {
+#pragma unroll
+    for (int k = 0; k < PER; k++)
+      if (goff[k] >= 0) {
+        if (UP) {
+          float a[8], q[8];
+          unpack8(px[k], a);
+          unpack8(pq[UP ? k : 0], q);
+#pragma unroll
+          for (int e = 0; e < 8; e++) a[e] = a[e] + q[e];
+          dst[loff[k]] = pack8(a);
+        } else {
+          dst[loff[k]] = px[k];
+        }
+      }
   };
   // this lane's A (points) offset for K chunk c within a ring plane and the
   // plane it reads (dz + 1), packed as (dz + 1) << 20 | offset; -1 past the
@@ -124,11 +198,22 @@ __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const
     const int off = (cg * T::HY + wave * MTR + dy + H) * T::HX + (lx + dx + H);
     return g < G ? (dzi << 20) | off : -1;
   };
-  constexpr bool PRE = NC <= 16;   // codes kept in registers (fully unrolled K loop)
-  int code[PRE ? NC : 1];
+  // DYR: this lane's column of load phase p -- its ring plane (dz + 1) and
+  // the offset of its first row (y0 + wave MTR - 1), packed as above; the
+  // phase reads rows 0 .. MTR + 1 from there, and dy tap e (dy = e - 1) of
+  // output row r is row r + e
+  auto col_code = [&](int p) -> int {
+    const int q = 4 * p + kg;
+    const int comb = q / CG, cg = q % CG;
+    const int off = (cg * T::HY + wave * MTR) * T::HX + (lx + comb % 3);
+    return q < 9 * CG ? ((comb / 3) << 20) | off : -1;
+  };
+  constexpr bool PRE = DYR ? NP <= 9 : NC <= 16;   // codes kept in registers (fully unrolled K loop)
+  constexpr int NCODE = PRE ? (DYR ? NP : NC) : 1;
+  int code[NCODE];
   if (PRE) {
 #pragma unroll
-    for (int c = 0; c < (PRE ? NC : 1); c++) code[c] = chunk_code(c);
+    for (int c = 0; c < NCODE; c++) code[c] = DYR ? col_code(c) : chunk_code(c);
   }
   // the output channels of this lane: 4 * kg + j of each n-tile
   float bb[NT][4];
@@ -137,16 +222,28 @@ __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const
 #pragma unroll
     for (int j = 0; j < 4; j++) bb[t][j] = bias ? bias[t * 16 + 4 * kg + j] : 0.f;
   if (TAPS == 27) {
-    load_plane(tile, z0 - 1);
-    load_plane(tile + T::PLANE, z0);
+    fetch(z0 - 1);
+    put(tile);
+    fetch(z0);
+    put(tile + T::PLANE);
+    fetch(z0 + 1);
+  } else {
+    fetch(z0);
   }
+  // the weight loads do not depend on z: kept in registers across the z
+  // loop when they fit (16 chunk tiles, 64 VGPRs), else reloaded per z (L1 /
+  // L2 hits) through a lane index the compiler cannot hoist
+  constexpr bool WHOIST = NC * NT <= 16;
   for (int zi = 0; zi < zb; zi++) {
     const int z = z0 + zi;
+    int wl = lane;
+    if (!WHOIST) asm volatile("" : "+v"(wl));
     // plane z + d lives in ring slot (zi + 1 + d) % 3; a 1x1 kernel reads its
-    // one plane z from slot 0
-    if (TAPS == 27) load_plane(tile + ((zi + 2) % 3) * T::PLANE, z + 1);
-    else load_plane(tile, z);
+    // one plane z from slot 0.  The registers hold plane z + 1 (z for 1x1);
+    // the next one is fetched once they are in LDS.
+    put(TAPS == 27 ? tile + ((zi + 2) % 3) * T::PLANE : tile);
     __syncthreads();
+    if (zi + 1 < zb) fetch(TAPS == 27 ? z + 2 : z + 1);
     const int sb0 = TAPS == 27 ? (zi % 3) * T::PLANE : 0;
     const int sb1 = TAPS == 27 ? ((zi + 1) % 3) * T::PLANE : 0;
     const int sb2 = TAPS == 27 ? ((zi + 2) % 3) * T::PLANE : 0;
@@ -168,14 +265,43 @@ __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const
       }
 #pragma unroll
       for (int t = 0; t < NT; t++) {
-        const bf16x8 w = __builtin_bit_cast(bf16x8, W[((size_t)c * NT + t) * 64 + lane]);
+        const bf16x8 w = __builtin_bit_cast(bf16x8, W[((size_t)c * NT + t) * 64 + wl]);
         // weights as the A operand (rows = output channels), points as B
 #pragma unroll
         for (int r = 0; r < MTR; r++)
           acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, a[r], acc[r][t], 0, 0, 0);
       }
     };
-    if (PRE) {
+    auto phase = [&](int p, int cd) {
+      bf16x8 a[MTR + 2];
+      if (cd >= 0) {
+        const int pz = cd >> 20;
+        const int base = (pz == 0 ? sb0 : (pz == 1 ? sb1 : sb2)) + (cd & 0xfffff);
+#pragma unroll
+        for (int r = 0; r < MTR + 2; r++) a[r] = __builtin_bit_cast(bf16x8, tile[base + r * T::HX]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < MTR + 2; r++) a[r] = __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u});
+      }
+#pragma unroll
+      for (int e = 0; e < 3; e++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const bf16x8 w = __builtin_bit_cast(bf16x8, W[((size_t)(3 * p + e) * NT + t) * 64 + wl]);
+#pragma unroll
+          for (int r = 0; r < MTR; r++)
+            acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, a[r + e], acc[r][t], 0, 0, 0);
+        }
+    };
+    if (DYR) {
+      if (PRE) {
+#pragma unroll
+        for (int p = 0; p < NP; p++) phase(p, code[PRE ? p : 0]);
+      } else {
+#pragma unroll 2
+        for (int p = 0; p < NP; p++) phase(p, col_code(p));
+      }
+    } else if (PRE) {
 #pragma unroll
       for (int c = 0; c < NC; c++) chunk(c, code[PRE ? c : 0]);
     } else {
@@ -196,8 +322,8 @@ __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const
           if (RELU) v[j] = fmaxf(v[j], 0.f);
         }
         uint2 o;
-        o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-        o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+        o.x = f2bf2(v[0], v[1]);
+        o.y = f2bf2(v[2], v[3]);
         *reinterpret_cast<uint2 *>(Y + pt * COUT + t * 16 + 4 * kg) = o;
       }
     }
@@ -353,19 +479,6 @@ __device__ __forceinline__ size_t parent(size_t f, int fx_n, int fy_n) {
 
 // One thread per (point, group of 8 channels), 16-byte loads and stores;
 // C channels = C / 8 groups per point.
-__device__ __forceinline__ void unpack8(const u32x4 &u, float f[8]) {
-#pragma unroll
-  for (int e = 0; e < 4; e++) {
-    f[2 * e] = __uint_as_float(u[e] << 16);
-    f[2 * e + 1] = __uint_as_float(u[e] & 0xffff0000u);
-  }
-}
-__device__ __forceinline__ u32x4 pack8(const float f[8]) {
-  u32x4 u;
-#pragma unroll
-  for (int e = 0; e < 4; e++) u[e] = (uint32_t)f2bf(f[2 * e]) | ((uint32_t)f2bf(f[2 * e + 1]) << 16);
-  return u;
-}
 
 // P[p] = mean of the 8 children (coarse dims cx, cy; Nc coarse points)
 __global__ void k_pool(const u32x4 *__restrict__ X, u32x4 *__restrict__ P, size_t Nc, int CG, int cx, int cy) {
@@ -524,24 +637,25 @@ __global__ void k_adam(float *__restrict__ p, float *__restrict__ g, float *__re
   }
 }
 
-// packed B operand of a layer: out[((c * NT + t) * 64 + l) * 8 + j] = bf16 of
-// W[co][ci][tap] with g = 4c + l/16, tap = g / (CIN/8), ci = (g % (CIN/8)) * 8
+// packed weight operand of a layer: out[((c * NT + t) * 64 + l) * 8 + j] =
+// bf16 of W[co][ci][tap] with (tap, cg) = conv_k_group(c, l / 16), ci = cg * 8
 // + j, co = 16 t + l % 16; zero past the last group.  transposed: the weights
 // of the data gradient (CIN' = cout, COUT' = cin, tap' = taps - 1 - tap).
 __global__ void k_pack(const float *__restrict__ W, int cin, int cout, int taps, bool transposed,
                        uint16_t *__restrict__ out) {
   const int CIN = transposed ? cout : cin, COUT = transposed ? cin : cout;
-  const int CG = CIN / 8, G = taps * CG, NC = (G + 3) / 4, NT = COUT / 16;
+  const int CG = CIN / 8, NC = conv_chunks(CG, taps), NT = COUT / 16;
   const size_t total = (size_t)NC * NT * 64 * 8;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
     const int j = (int)(i % 8);
     const int l = (int)((i / 8) % 64);
     const int t = (int)((i / 512) % NT);
     const int c = (int)(i / (512 * (size_t)NT));
-    const int g = 4 * c + (l >> 4);
+    int cg = 0;
+    const int tap = conv_k_group(c, l >> 4, CG, taps, cg);
     float w = 0.f;
-    if (g < G) {
-      const int tap = g / CG, ci = (g % CG) * 8 + j, co = t * 16 + (l & 15);
+    if (tap >= 0) {
+      const int ci = cg * 8 + j, co = t * 16 + (l & 15);
       if (!transposed) w = W[((size_t)co * cin + ci) * taps + tap];
       else w = W[((size_t)ci * cin + co) * taps + (taps - 1 - tap)];   // W[co'=ci][ci'=co]
     }
@@ -549,8 +663,7 @@ __global__ void k_pack(const float *__restrict__ W, int cin, int cout, int taps,
   }
 }
 size_t packed_size(int cin, int cout, int taps) {
-  const int CG = cin / 8, G = taps * CG, NC = (G + 3) / 4, NT = cout / 16;
-  return (size_t)NC * NT * 64 * 8;
+  return (size_t)conv_chunks(cin / 8, taps) * (cout / 16) * 64 * 8;
 }
 
 // ------------------------------------------------------ features / apply
@@ -596,18 +709,26 @@ int conv_zb(int tiles_xy, int nz) {
 // channels: 2)
 template <int CIN, int COUT, int TAPS, bool RELU>
 void conv(const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y, int nx, int ny, int nz,
-          hipStream_t s) {
+          hipStream_t s, const uint16_t *Q = nullptr) {
   const bool four = CIN < 64 && COUT < 64 && ny % 16 == 0;
   const int ty = four ? 16 : 8;
   const int tiles = (nx / 16) * (ny / ty);
   const int zb = conv_zb(tiles, nz);
   const unsigned blocks = (unsigned)(tiles * (nz / zb));
-  if (four)
-    hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 4>), dim3(blocks), dim3(256), 0, s,
-                       reinterpret_cast<const u32x4 *>(X), reinterpret_cast<const u32x4 *>(W), bias, Y, nx, ny, nz, zb);
+  const u32x4 *x = reinterpret_cast<const u32x4 *>(X), *q = reinterpret_cast<const u32x4 *>(Q),
+              *w = reinterpret_cast<const u32x4 *>(W);
+  if (four && Q)
+    hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 4, true>), dim3(blocks), dim3(256), 0, s, x, q, w, bias, Y, nx,
+                       ny, nz, zb);
+  else if (four)
+    hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 4, false>), dim3(blocks), dim3(256), 0, s, x, q, w, bias, Y,
+                       nx, ny, nz, zb);
+  else if (Q)
+    hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 2, true>), dim3(blocks), dim3(256), 0, s, x, q, w, bias, Y, nx,
+                       ny, nz, zb);
   else
-    hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 2>), dim3(blocks), dim3(256), 0, s,
-                       reinterpret_cast<const u32x4 *>(X), reinterpret_cast<const u32x4 *>(W), bias, Y, nx, ny, nz, zb);
+    hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 2, false>), dim3(blocks), dim3(256), 0, s, x, q, w, bias, Y,
+                       nx, ny, nz, zb);
   HIPCHK(hipGetLastError());
 }
 
@@ -746,13 +867,21 @@ void UNet::forward(const uint16_t *X0, float *out, hipStream_t s) {
   hipLaunchKernelGGL(k_pool, dim3(nblk(N3 * 4)), dim3(TPB), 0, s, V(A.s2.p), V(A.p2.p), N3, 4, n3x, n3y);
   conv<32, 64, 27, true>(A.p2.p, W + L[3].pf, p + L[3].b, A.a3.p, n3x, n3y, n3z, s);
   conv<64, 32, 1, false>(A.a3.p, W + L[4].pf, p + L[4].b, A.q2.p, n3x, n3y, n3z, s);
-  hipLaunchKernelGGL(k_upadd, dim3(nblk(N2 * 4)), dim3(TPB), 0, s, V(A.q2.p), V(A.s2.p), V(A.t2.p), N2, 4, n2x, n2y);
-  conv<32, 32, 27, true>(A.t2.p, W + L[5].pf, p + L[5].b, A.a2.p, n2x, n2y, n2z, s);
+  // the skip sums t = s + up(q): stored when training (the backward pass
+  // reads them), else formed by the next convolution as it loads its planes
+  if (A.train)
+    hipLaunchKernelGGL(k_upadd, dim3(nblk(N2 * 4)), dim3(TPB), 0, s, V(A.q2.p), V(A.s2.p), V(A.t2.p), N2, 4, n2x,
+                       n2y);
+  if (A.train) conv<32, 32, 27, true>(A.t2.p, W + L[5].pf, p + L[5].b, A.a2.p, n2x, n2y, n2z, s);
+  else conv<32, 32, 27, true>(A.s2.p, W + L[5].pf, p + L[5].b, A.a2.p, n2x, n2y, n2z, s, A.q2.p);
   conv<32, 16, 1, false>(A.a2.p, W + L[6].pf, p + L[6].b, A.q1.p, n2x, n2y, n2z, s);
-  hipLaunchKernelGGL(k_upadd, dim3(nblk(N1 * 2)), dim3(TPB), 0, s, V(A.q1.p), V(A.s1.p), V(A.t1.p), N1, 2, n1x, n1y);
-  probe_begin(K_UNET_CONV, s);
-  conv<16, 16, 27, true>(A.t1.p, W + L[7].pf, p + L[7].b, A.a1.p, n1x, n1y, n1z, s);
-  probe_end(K_UNET_CONV, s, conv_flops);
+  if (A.train) {
+    hipLaunchKernelGGL(k_upadd, dim3(nblk(N1 * 2)), dim3(TPB), 0, s, V(A.q1.p), V(A.s1.p), V(A.t1.p), N1, 2, n1x,
+                       n1y);
+    conv<16, 16, 27, true>(A.t1.p, W + L[7].pf, p + L[7].b, A.a1.p, n1x, n1y, n1z, s);
+  } else {
+    conv<16, 16, 27, true>(A.s1.p, W + L[7].pf, p + L[7].b, A.a1.p, n1x, n1y, n1z, s, A.q1.p);
+  }
   hipLaunchKernelGGL(k_head, dim3(nblk(N1)), dim3(TPB), 0, s, reinterpret_cast<const u32x4 *>(A.a1.p),
                      p + L[8].w, p + L[8].b, out, N1);
   ew_launch_check();

@@ -38,8 +38,10 @@ MIXES = {
 
 def make_system(P, n, fam, seed):
     """A plain-solver system of one family (the same script commands as the
-    benchmark workloads)."""
+    benchmark workloads) on an n^3 grid; the chain count scales with the
+    volume, so the bead density is the family's at 64^3."""
     _, phi0, var, chi, nch, nb, style, cut, width = fam
+    nch = nch * (n // 64) ** 3 if n >= 64 else max(1, nch * n ** 3 // 64 ** 3)
     s = P.System(n, n, n, [0, 0, 0], [float(n)] * 3, zghost=P.ZGHOST_PERIODIC)
     s.set_field(P.PHI, P.ic_noisy_constant((n, n, n), phi0, var, 1234 + seed))
     xyz, mol = P.ic_random_chains(nch, nb, 1.0, 1.0, [0, 0, 0], [float(n)] * 3, 42 + seed)
@@ -130,6 +132,7 @@ def main(argv=None):
     ap.add_argument("--holdout", type=int, default=4, help="trajectories kept for validation")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mix", default="gaussian", choices=sorted(MIXES))
+    ap.add_argument("--n", type=int, default=64, help="training grid (n^3, dx = 1)")
     args = ap.parse_args(argv)
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     import phafd_amd as P
@@ -138,7 +141,8 @@ def main(argv=None):
         print(msg, flush=True)
 
     net = P.UNet(seed=args.seed)
-    meta, norm = build_dataset(P, net, mix=args.mix, ntraj=args.trajectories, nblocks=args.blocks, K=args.K, log=log)
+    meta, norm = build_dataset(P, net, mix=args.mix, n=args.n, ntraj=args.trajectories, nblocks=args.blocks, K=args.K,
+                               log=log)
     ntrain = (args.trajectories - args.holdout) * args.blocks
     nval = len(meta) - ntrain
     t0 = time.time()
@@ -159,14 +163,14 @@ def main(argv=None):
             f"{time.time() - t0:.1f} s")
     # relative error of the predicted change on the validation blocks
     val = net.eval(ntrain, nval) if nval else np.zeros(0)
-    N = 64 ** 3   # the training grid
+    N = args.n ** 3   # the training grid
     err_pred = np.sqrt(val * N) * norm[4]
     rel = err_pred / meta[ntrain:, 0] if nval else np.zeros(0)
     base_only = meta[ntrain:, 1] / meta[ntrain:, 0] if nval else np.zeros(0)
     vres, verr = calibrate(P, net, norm, args.mix, args.trajectories - args.holdout, args.holdout, args.blocks,
-                           args.K)
+                           args.K, n=args.n)
     tol = 1.25 * float(vres.max())
-    summary = {"mix": args.mix, "norm": norm.tolist(), "tol": tol, "K": args.K, "train_samples": ntrain,
+    summary = {"mix": args.mix, "n": args.n, "norm": norm.tolist(), "tol": tol, "K": args.K, "train_samples": ntrain,
                "validation_samples": nval,
                "history": hist, "validation_rel_err_change_mean": float(rel.mean()) if nval else None,
                "validation_rel_err_change_max": float(rel.max()) if nval else None,
@@ -174,7 +178,8 @@ def main(argv=None):
                "plain_residual_max": float(meta[:, 2].max()),
                "network_residual_validation_max": float(vres.max()),
                "network_residual_validation_mean": float(vres.mean()), "train_seconds": time.time() - t0}
-    np.savez(args.out, params=net.get_params(), norm=norm, tol=np.array([tol]), K=np.array([args.K]))
+    np.savez(args.out, params=net.get_params(), norm=norm, tol=np.array([tol]), K=np.array([args.K]),
+             n=np.array([args.n]), mix=np.array([args.mix]))
     log(json.dumps(summary))
     net.close()
 
