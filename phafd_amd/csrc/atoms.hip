@@ -335,6 +335,7 @@ __global__ void k_grid_list(GridView gv, BinGeom b, const int *stencil, int nst,
     const int ibin = coord2bin(b, xtmp, ytmp, ztmp);
     int n = 0;
     OffT w = FILL ? off[p] : 0;
+    const OffT wend = FILL ? off[p + 1] : 0;   // the row's scanned segment
     for (int m = 0; m < nst; m++) {
       const int bb = ibin + stencil[m];
       const int s = binstart[bb], e = binstart[bb + 1];
@@ -346,7 +347,10 @@ __global__ void k_grid_list(GridView gv, BinGeom b, const int *stencil, int nst,
         const double delz = ztmp - x.z;
         const double rsq = delx * delx + dely * dely + delz * delz;
         if (rsq <= cutneighsq) {
-          if (FILL) neigh[w++] = nowned + a;
+          if (FILL) {
+            if (w < wend) neigh[w++] = nowned + a;
+            else atomicOr(overflow, 2);
+          }
           n++;
         }
       }
@@ -386,6 +390,7 @@ __global__ void k_half_list(int nrows, const int *ilist, const int *a2b, const i
     const double xtmp = xi.x, ytmp = xi.y, ztmp = xi.z;
     int n = 0;
     int w = FILL ? off[r] : 0;
+    const int wend = FILL ? off[r + 1] : 0;   // the row's scanned segment
     const int ibin = a2b[i];
     // rest of i's own bin (entries after i in the bin's list order)
     const int s0 = binstart[ibin], e0 = binstart[ibin + 1];
@@ -404,7 +409,10 @@ __global__ void k_half_list(int nrows, const int *ilist, const int *a2b, const i
       const double delx = xtmp - xj.x, dely = ytmp - xj.y, delz = ztmp - xj.z;
       const double rsq = delx * delx + dely * dely + delz * delz;
       if (rsq <= cutneighsq) {
-        if (FILL) neigh[w++] = nowned + j;
+        if (FILL) {
+          if (w < wend) neigh[w++] = nowned + j;
+          else atomicOr(overflow, 2);
+        }
         n++;
       }
     }
@@ -417,7 +425,10 @@ __global__ void k_half_list(int nrows, const int *ilist, const int *a2b, const i
         const double delx = xtmp - xj.x, dely = ytmp - xj.y, delz = ztmp - xj.z;
         const double rsq = delx * delx + dely * dely + delz * delz;
         if (rsq <= cutneighsq) {
-          if (FILL) neigh[w++] = nowned + j;
+          if (FILL) {
+          if (w < wend) neigh[w++] = nowned + j;
+          else atomicOr(overflow, 2);
+        }
           n++;
         }
       }

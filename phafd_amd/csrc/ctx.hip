@@ -592,6 +592,18 @@ void bin_atoms(phafd_hip_ctx *c) {
   launch_bin_sort(nb, c->binstart.p, c->binatoms.p, c->glabel.p, c->stream);
 }
 
+// flags[1], read at a build's count synchronise: bit 0 a row of more than
+// Neighbor::oneatom (2000) entries, which the reference reports on stderr
+// and goes on (src/npair_grid_bin.cpp:118-121, src/npair_half_bin_newton.cpp:
+// 117-120); bit 1 a fill that found more entries than its count (the writes
+// past the row's scanned segment were dropped), an error
+void list_flags(phafd_hip_ctx *c, int ovf) {
+  if (!ovf) return;
+  launch_fill_i32(1, c->flags.p + 1, 0, c->stream);
+  if (ovf & 2) throw std::runtime_error("neighbor list fill exceeded the entries counted for its row");
+  std::fprintf(stderr, "Neighbor list overflow, terrible things are going to happen.\n");
+}
+
 // NPairGridBin::build (src/npair_grid_bin.cpp:22-127)
 void build_gridlist(phafd_hip_ctx *c) {
   GridView gv = c->gridview<double>();
@@ -606,8 +618,11 @@ void build_gridlist(phafd_hip_ctx *c) {
   probe_end(K_GRID_COUNT, c->stream, 4.0 * c->N);
   scan_exclusive_i32_to_i64(c->g_cnt.p, c->g_off.p, c->N, c->stream);
   int64_t tot = 0;
+  int ovf = 0;
   HIPCHK(hipMemcpyAsync(&tot, c->g_off.p + c->N, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&ovf, c->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   sync(c);
+  list_flags(c, ovf);
   c->g_nent = tot;
   c->g_neigh.alloc((size_t)tot + 1);
   probe_begin(K_GRID_FILL, c->stream);
@@ -642,7 +657,11 @@ void build_halflist(phafd_hip_ctx *c, bool full) {
   launch_half_list(false, nrows, c->h_ilist.p, c->a2b.p, c->binstart.p, c->binatoms.p, c->glabel.p,
                    st.p, nst, pos, cutneighsq, c->nowned, c->h_cnt.p, nullptr, nullptr, c->flags.p + 1, c->stream);
   scan_exclusive_i32(c->h_cnt.p, c->h_off.p, (size_t)nrows, c->stream);
-  const int nent = read_int(c, c->h_off.p + nrows);
+  int nent = 0, ovf = 0;
+  HIPCHK(hipMemcpyAsync(&nent, c->h_off.p + nrows, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&ovf, c->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  list_flags(c, ovf);
   c->h_nent = nent;
   c->h_neigh.alloc((size_t)nent + 1);
   launch_half_list(true, nrows, c->h_ilist.p, c->a2b.p, c->binstart.p, c->binatoms.p, c->glabel.p,

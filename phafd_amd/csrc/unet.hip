@@ -111,10 +111,32 @@ struct ConvTile {
 // Q (optional): a coarse (half-resolution) input of the same channels added
 // on load, X[v] + Q[parent(v)] rounded to bf16 -- the decoder's skip sum,
 // bit for bit what k_upadd would have stored.
-template <int CIN, int COUT, int TAPS, bool RELU, int MTR, bool UP>
+// Fused epilogues (EPI): EPI_POOL also writes P, the 2x2x2 mean of the
+// stored (bf16) outputs, bit for bit what k_pool computes from Y (needs an
+// even zb); EPI_HEAD (16 output channels) writes out[v] = head(Y[v]) as
+// k_head does, in place of Y (Y may be null).
+enum { EPI_NONE = 0, EPI_POOL = 1, EPI_HEAD = 2 };
+// the value of the lane's x neighbour (lx ^ 1) by DPP quad_perm [1, 0, 3, 2]
+__device__ __forceinline__ float xpair(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+// the head's sum order (shared by k_head): four partials over consecutive
+// channel quads, then bias + ((p0 + p1) + (p2 + p3))
+__device__ __forceinline__ float head_quad(const float *w, const float a[4]) {
+  float p = w[0] * a[0];
+  p = fmaf(w[1], a[1], p);
+  p = fmaf(w[2], a[2], p);
+  return fmaf(w[3], a[3], p);
+}
+
+template <int CIN, int COUT, int TAPS, bool RELU, int MTR, bool UP, int EPI>
 __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const u32x4 *__restrict__ Q,
                                               const u32x4 *__restrict__ W, const float *__restrict__ bias,
-                                              uint16_t *__restrict__ Y, int nx, int ny, int nz, int zb) {
+                                              uint16_t *__restrict__ Y, int nx, int ny, int nz, int zb,
+                                              uint16_t *__restrict__ P, const float *__restrict__ hw,
+                                              const float *__restrict__ hb, float *__restrict__ hout) {
+  static_assert(EPI != EPI_HEAD || COUT == 16, "the fused head reads 16 channels");
+  static_assert(EPI != EPI_POOL || MTR % 2 == 0, "the fused pool pairs rows");
   using T = ConvTile<CIN, TAPS, MTR>;
   constexpr int CG = T::CG;
   constexpr int G = TAPS * CG;
@@ -234,6 +256,7 @@ __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const
   // loop when they fit (16 chunk tiles, 64 VGPRs), else reloaded per z (L1 /
   // L2 hits) through a lane index the compiler cannot hoist
   constexpr bool WHOIST = NC * NT <= 16;
+  float pool_part[EPI == EPI_POOL ? MTR / 2 : 1][NT][4];
   for (int zi = 0; zi < zb; zi++) {
     const int z = z0 + zi;
     int wl = lane;
@@ -310,6 +333,7 @@ __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const
     }
     // D layout: lane holds rows 4 * kg + j (output channels of the n-tile),
     // column lx (the point x0 + lx): 4 consecutive channels, one 8-byte store
+    float fe[NT][4];   // EPI_POOL: the even row's stored values
 #pragma unroll
     for (int r = 0; r < MTR; r++) {
       const size_t pt = ((size_t)z * ny + (y0 + wave * MTR + r)) * nx + x0 + lx;
@@ -324,7 +348,47 @@ __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const
         uint2 o;
         o.x = f2bf2(v[0], v[1]);
         o.y = f2bf2(v[2], v[3]);
-        *reinterpret_cast<uint2 *>(Y + pt * COUT + t * 16 + 4 * kg) = o;
+        if (EPI != EPI_HEAD) *reinterpret_cast<uint2 *>(Y + pt * COUT + t * 16 + 4 * kg) = o;
+        float a[4] = {__uint_as_float(o.x << 16), __uint_as_float(o.x & 0xffff0000u), __uint_as_float(o.y << 16),
+                      __uint_as_float(o.y & 0xffff0000u)};
+        if (EPI == EPI_POOL && !(r & 1)) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) fe[t][j] = a[j];
+        }
+        if (EPI == EPI_POOL && (r & 1)) {
+          // children in k_pool's order: x fastest, then y, then z; the sum
+          // over the even plane is kept in registers until the odd one
+          const int rp = r >> 1;
+          float m[4];
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const float c0 = fe[t][j], c2 = a[j];
+            const float c1 = xpair(c0), c3 = xpair(c2);
+            float sum = (zi & 1) ? pool_part[EPI == EPI_POOL ? rp : 0][t][j] : 0.f;
+            sum = sum + c0;
+            sum = sum + c1;
+            sum = sum + c2;
+            sum = sum + c3;
+            pool_part[EPI == EPI_POOL ? rp : 0][t][j] = sum;
+            m[j] = sum * 0.125f;
+          }
+          if ((zi & 1) && !(lx & 1)) {
+            const size_t cp = ((size_t)(z >> 1) * (ny >> 1) + ((y0 + wave * MTR + r) >> 1)) * (nx >> 1) +
+                              ((x0 + lx) >> 1);
+            uint2 q;
+            q.x = f2bf2(m[0], m[1]);
+            q.y = f2bf2(m[2], m[3]);
+            *reinterpret_cast<uint2 *>(P + cp * COUT + t * 16 + 4 * kg) = q;
+          }
+        }
+        if (EPI == EPI_HEAD) {
+          // lanes kg = 0..3 hold the channel quads of point pt: partials,
+          // then (p0 + p1) + (p2 + p3) across the lane groups
+          float pq = head_quad(hw + 4 * kg, a);
+          pq = pq + __shfl_xor(pq, 16);
+          pq = pq + __shfl_xor(pq, 32);
+          if (kg == 0) hout[pt] = hb[0] + pq;
+        }
       }
     }
     __syncthreads();   // the ring slot read here is reloaded next
@@ -555,17 +619,18 @@ __global__ void k_mask(uint16_t *__restrict__ D, const uint16_t *__restrict__ A,
 __global__ void k_head(const u32x4 *__restrict__ A, const float *__restrict__ w, const float *__restrict__ b,
                        float *__restrict__ out, size_t N) {
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < N; v += (size_t)gridDim.x * blockDim.x) {
-    float s = b[0];
+    float p[4];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const u32x4 u = A[v * 2 + q];
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        s += w[q * 8 + 2 * e] * __uint_as_float(u[e] << 16);
-        s += w[q * 8 + 2 * e + 1] * __uint_as_float(u[e] & 0xffff0000u);
+      for (int h = 0; h < 2; h++) {
+        const float a[4] = {__uint_as_float(u[2 * h] << 16), __uint_as_float(u[2 * h] & 0xffff0000u),
+                            __uint_as_float(u[2 * h + 1] << 16), __uint_as_float(u[2 * h + 1] & 0xffff0000u)};
+        p[2 * q + h] = head_quad(w + 8 * q + 4 * h, a);
       }
     }
-    out[v] = s;
+    out[v] = b[0] + ((p[0] + p[1]) + (p[2] + p[3]));
   }
 }
 // dA[v][c] = (A[v][c] > 0) ? dout[v] w[c] : 0; part[blk][17] = (sum dout A[.][c], sum dout)
@@ -707,28 +772,39 @@ int conv_zb(int tiles_xy, int nz) {
 // rows per wave: 4 when the rows of a level allow it, the halo planes fit
 // comfortably in LDS and the accumulators in registers (64 input or output
 // channels: 2)
-template <int CIN, int COUT, int TAPS, bool RELU>
+// the fused epilogue's outputs (EPI_POOL: P; EPI_HEAD: hw, hb, hout)
+struct ConvEpi {
+  uint16_t *P = nullptr;
+  const float *hw = nullptr, *hb = nullptr;
+  float *hout = nullptr;
+};
+
+template <int CIN, int COUT, int TAPS, bool RELU, int EPI = EPI_NONE>
 void conv(const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y, int nx, int ny, int nz,
-          hipStream_t s, const uint16_t *Q = nullptr) {
+          hipStream_t s, const uint16_t *Q = nullptr, const ConvEpi &e = ConvEpi()) {
   const bool four = CIN < 64 && COUT < 64 && ny % 16 == 0;
   const int ty = four ? 16 : 8;
   const int tiles = (nx / 16) * (ny / ty);
   const int zb = conv_zb(tiles, nz);
+  if (EPI == EPI_POOL && zb % 2) {   // z pairs would straddle blocks: separate pool
+    conv<CIN, COUT, TAPS, RELU>(X, W, bias, Y, nx, ny, nz, s, Q);
+    const size_t Nc = (size_t)nx * ny * nz / 8;
+    hipLaunchKernelGGL(k_pool, dim3(nblk(Nc * (COUT / 8))), dim3(TPB), 0, s, reinterpret_cast<const u32x4 *>(Y),
+                       reinterpret_cast<u32x4 *>(e.P), Nc, COUT / 8, nx / 2, ny / 2);
+    HIPCHK(hipGetLastError());
+    return;
+  }
   const unsigned blocks = (unsigned)(tiles * (nz / zb));
   const u32x4 *x = reinterpret_cast<const u32x4 *>(X), *q = reinterpret_cast<const u32x4 *>(Q),
               *w = reinterpret_cast<const u32x4 *>(W);
-  if (four && Q)
-    hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 4, true>), dim3(blocks), dim3(256), 0, s, x, q, w, bias, Y, nx,
-                       ny, nz, zb);
-  else if (four)
-    hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 4, false>), dim3(blocks), dim3(256), 0, s, x, q, w, bias, Y,
-                       nx, ny, nz, zb);
-  else if (Q)
-    hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 2, true>), dim3(blocks), dim3(256), 0, s, x, q, w, bias, Y, nx,
-                       ny, nz, zb);
-  else
-    hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, 2, false>), dim3(blocks), dim3(256), 0, s, x, q, w, bias, Y,
-                       nx, ny, nz, zb);
+#define PHAFD_CONV_LAUNCH(M, U)                                                                                     \
+  hipLaunchKernelGGL((k_conv<CIN, COUT, TAPS, RELU, M, U, EPI>), dim3(blocks), dim3(256), 0, s, x, q, w, bias, Y,  \
+                     nx, ny, nz, zb, e.P, e.hw, e.hb, e.hout)
+  if (four && Q) PHAFD_CONV_LAUNCH(4, true);
+  else if (four) PHAFD_CONV_LAUNCH(4, false);
+  else if (Q) PHAFD_CONV_LAUNCH(2, true);
+  else PHAFD_CONV_LAUNCH(2, false);
+#undef PHAFD_CONV_LAUNCH
   HIPCHK(hipGetLastError());
 }
 
@@ -859,12 +935,14 @@ void UNet::forward(const uint16_t *X0, float *out, hipStream_t s) {
   const float *p = P.p;
   conv<8, 16, 27, true>(X0, W + L[0].pf, p + L[0].b, A.ae1.p, n1x, n1y, n1z, s);
   const double conv_flops = 2.0 * 27 * 16 * 16 * (double)N1;   // 3^3 x 16 -> 16 at full resolution
+  // the encoder's convolutions also write the pooled level below
+  ConvEpi e1, e2;
+  e1.P = A.p1.p;
+  e2.P = A.p2.p;
   probe_begin(K_UNET_CONV, s);
-  conv<16, 16, 27, true>(A.ae1.p, W + L[1].pf, p + L[1].b, A.s1.p, n1x, n1y, n1z, s);
+  conv<16, 16, 27, true, EPI_POOL>(A.ae1.p, W + L[1].pf, p + L[1].b, A.s1.p, n1x, n1y, n1z, s, nullptr, e1);
   probe_end(K_UNET_CONV, s, conv_flops);
-  hipLaunchKernelGGL(k_pool, dim3(nblk(N2 * 2)), dim3(TPB), 0, s, V(A.s1.p), V(A.p1.p), N2, 2, n2x, n2y);
-  conv<16, 32, 27, true>(A.p1.p, W + L[2].pf, p + L[2].b, A.s2.p, n2x, n2y, n2z, s);
-  hipLaunchKernelGGL(k_pool, dim3(nblk(N3 * 4)), dim3(TPB), 0, s, V(A.s2.p), V(A.p2.p), N3, 4, n3x, n3y);
+  conv<16, 32, 27, true, EPI_POOL>(A.p1.p, W + L[2].pf, p + L[2].b, A.s2.p, n2x, n2y, n2z, s, nullptr, e2);
   conv<32, 64, 27, true>(A.p2.p, W + L[3].pf, p + L[3].b, A.a3.p, n3x, n3y, n3z, s);
   conv<64, 32, 1, false>(A.a3.p, W + L[4].pf, p + L[4].b, A.q2.p, n3x, n3y, n3z, s);
   // the skip sums t = s + up(q): stored when training (the backward pass
@@ -879,11 +957,16 @@ void UNet::forward(const uint16_t *X0, float *out, hipStream_t s) {
     hipLaunchKernelGGL(k_upadd, dim3(nblk(N1 * 2)), dim3(TPB), 0, s, V(A.q1.p), V(A.s1.p), V(A.t1.p), N1, 2, n1x,
                        n1y);
     conv<16, 16, 27, true>(A.t1.p, W + L[7].pf, p + L[7].b, A.a1.p, n1x, n1y, n1z, s);
+    hipLaunchKernelGGL(k_head, dim3(nblk(N1)), dim3(TPB), 0, s, reinterpret_cast<const u32x4 *>(A.a1.p),
+                       p + L[8].w, p + L[8].b, out, N1);
   } else {
-    conv<16, 16, 27, true>(A.s1.p, W + L[7].pf, p + L[7].b, A.a1.p, n1x, n1y, n1z, s, A.q1.p);
+    // the head applied in the last convolution's epilogue (a1 not stored)
+    ConvEpi eh;
+    eh.hw = p + L[8].w;
+    eh.hb = p + L[8].b;
+    eh.hout = out;
+    conv<16, 16, 27, true, EPI_HEAD>(A.s1.p, W + L[7].pf, p + L[7].b, nullptr, n1x, n1y, n1z, s, A.q1.p, eh);
   }
-  hipLaunchKernelGGL(k_head, dim3(nblk(N1)), dim3(TPB), 0, s, reinterpret_cast<const u32x4 *>(A.a1.p),
-                     p + L[8].w, p + L[8].b, out, N1);
   ew_launch_check();
   (void)n3z;
 }

@@ -133,12 +133,30 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mix", default="gaussian", choices=sorted(MIXES))
     ap.add_argument("--n", type=int, default=64, help="training grid (n^3, dx = 1)")
+    ap.add_argument("--recalibrate", metavar="NPZ", default=None,
+                    help="only set the block-check tolerance of a trained network again, on its held-out "
+                         "trajectories (--trajectories, --holdout, --blocks as trained, --blocks possibly longer)")
     args = ap.parse_args(argv)
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     import phafd_amd as P
 
     def log(msg):
         print(msg, flush=True)
+
+    if args.recalibrate:
+        z = np.load(args.recalibrate)
+        K, n, mix, norm = int(z["K"][0]), int(z["n"][0]), str(z["mix"][0]), z["norm"]
+        net = P.UNet(seed=args.seed)
+        net.load(args.recalibrate)
+        vres, verr = calibrate(P, net, norm, mix, args.trajectories - args.holdout, args.holdout, args.blocks, K, n=n)
+        tol = 1.25 * float(vres.max())
+        np.savez(args.out, params=z["params"], norm=norm, tol=np.array([tol]), K=z["K"], n=z["n"], mix=z["mix"])
+        log(json.dumps({"recalibrated": args.recalibrate, "old_tol": float(z["tol"][0]), "tol": tol,
+                        "held_out_blocks": int(vres.size), "network_residual_validation_max": float(vres.max()),
+                        "network_residual_validation_mean": float(vres.mean()),
+                        "phi_err_per_block_max": float(verr.max())}))
+        net.close()
+        return
 
     net = P.UNet(seed=args.seed)
     meta, norm = build_dataset(P, net, mix=args.mix, n=args.n, ntraj=args.trajectories, nblocks=args.blocks, K=args.K,
@@ -153,7 +171,12 @@ def main(argv=None):
     w = max(1, per // 6)
     schedule = [(w, 0.1), (w, 0.3), (w, 0.6), (per - 3 * w, 1.0)] + [(per, c) for c in (0.7, 0.5, 0.3, 0.15, 0.05)]
     hist = []
-    for nst, c in schedule:
+    # the validation error of predicting no correction at all: a network
+    # that stays at this level has lost every ReLU of a layer and only
+    # predicts a constant (which the mean removal cancels)
+    # (the targets d / norm[4] have zero mean: both phi1 and the base keep the mass)
+    null_mse = float(np.mean((meta[ntrain:, 1] / norm[4]) ** 2 / args.n ** 3)) if nval else None
+    for k, (nst, c) in enumerate(schedule):
         if nst <= 0:
             continue
         losses = net.train(nst, args.batch, ntrain, args.lr * c)
@@ -161,6 +184,11 @@ def main(argv=None):
         hist.append((float(losses.mean()), float(val.mean()) if nval else None))
         log(f"lr {args.lr * c:.2e}: train mse {hist[-1][0]:.4e}, validation mse {hist[-1][1]}, "
             f"{time.time() - t0:.1f} s")
+        if nval and k >= 4 and hist[-1][1] >= 0.9 * null_mse:
+            net.close()
+            raise SystemExit(f"training collapsed: validation mse {hist[-1][1]:.4e} stays at the no-correction "
+                             f"level {null_mse:.4e} (a layer's ReLUs all inactive); rerun with a lower --lr or "
+                             f"another --seed")
     # relative error of the predicted change on the validation blocks
     val = net.eval(ntrain, nval) if nval else np.zeros(0)
     N = args.n ** 3   # the training grid

@@ -103,6 +103,32 @@ def test_setup_ghosts_bins_lists_bit_exact(zghost):
     assert _lists_equal(g.halflist(), o.halflist())
 
 
+def test_list_overflow_is_reported_like_the_reference(capfd):
+    # more than Neighbor::oneatom (2000) atoms within reach of a grid point:
+    # the reference prints a warning and goes on
+    # (src/npair_grid_bin.cpp:118-121); the list stays complete.  The step
+    # walks its own grid-atom entries; the reference's grid list (and its
+    # warning) is built when it is asked for.
+    w = workload(n=16, nchains=2, nbeads=5, lj=False, chain=False, seed=5)
+    rng = np.random.default_rng(5)
+    w["xyz"] = 7.0 + 2.0 * rng.random((2100, 3))
+    w["mol"] = np.zeros(2100, np.int32)
+    w["types"] = np.zeros(2100, np.int32)
+    g = make_gpu(w)
+    o = make_oracle(w)
+    try:
+        capfd.readouterr()
+        gl = g.gridlist()
+        err = capfd.readouterr().err
+        assert "Neighbor list overflow, terrible things are going to happen." in err
+        ol = o.gridlist()
+        assert gl[1].max() > 2000   # numneigh of the densest grid point
+        assert _lists_equal(gl, ol)
+    finally:
+        g.close()
+        o.close()
+
+
 @pytest.mark.parametrize("ljish", [False, True, "linear"])
 def test_one_step_phases(ljish):
     w = workload(n=32, nchains=10, nbeads=30, ljish=ljish, seed=4, nucwidth=2.0)
