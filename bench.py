@@ -250,12 +250,13 @@ DATA = os.path.join(ROOT, "phafd_amd", "data")
 SURROGATE = os.path.join(DATA, "surrogate_gaussian64.npz")          # K = 100, trained at 64^3
 SURROGATE_K10 = os.path.join(DATA, "surrogate_gaussian128_k10.npz")  # K = 10, trained at 128^3
 SURROGATE_LJISH = os.path.join(DATA, "surrogate_ljish64.npz")       # K = 100, LJish coupling, 64^3
+SURROGATE_LJISH_K10 = os.path.join(DATA, "surrogate_ljish128_k10.npz")  # K = 10, LJish coupling, 128^3
 
 
 def surrogate_networks(wl):
     """Committed networks for the workload's grid-atom coupling (the first
     is the headline one)."""
-    names = [SURROGATE_LJISH] if wl["ljish"] else [SURROGATE, SURROGATE_K10]
+    names = [SURROGATE_LJISH_K10, SURROGATE_LJISH] if wl["ljish"] else [SURROGATE, SURROGATE_K10]
     return [p for p in names if os.path.exists(p)]
 
 
