@@ -183,6 +183,70 @@ def test_trajectory_200_steps(zghost):
     assert np.allclose(ge, oe, rtol=1e-9, atol=1e-12)
 
 
+def test_grid_only_run_and_pair_before_atoms():
+    # no atoms: the field alone (FH + conjugate) against the oracle; a pair
+    # style before any atom is the reference's error (src/input.cpp:103)
+    n = 32
+    phi = P.ic_noisy_constant((n, n, n), 0.5, 0.02, 77)
+    runs = []
+    for is_oracle in (False, True):
+        s = (O.System if is_oracle else P.System)(n, n, n, [0, 0, 0], [float(n)] * 3)
+        s.set_field(P.PHI, phi)
+        with pytest.raises((RuntimeError, P.PhafdError), match="Cannot create pair style before atoms are made."):
+            if is_oracle:
+                s.add_pair(0, 3.0)
+            else:
+                s.pair_gridatom(0, 3.0)
+        s.fix_conjugate(5678, 1.0, 1.0, 1.0, 1.0, P.NOISE_OFF)
+        s.fix_floryhuggins(1.0, 1.0, 3.0)
+        s.setup(1e-3)
+        s.run(100)
+        runs.append((s.get_field(P.PHI), s.get_field(P.MU)))
+        s.close()
+    assert rel_l2(runs[0][0], runs[1][0]) < 1e-10
+    assert rel_l2(runs[0][1], runs[1][1]) < 1e-10
+
+
+@pytest.mark.parametrize("zghost", [0, 1])
+def test_atoms_on_and_beyond_the_box_faces(zghost):
+    # Domain::pbc / image counters (src/domain.cpp:102-208): beads exactly on
+    # the upper faces, on the lower ones, and one period outside (given
+    # unwrapped; pbc shifts by one period, as the reference's does); wrapped
+    # positions, images and the gathered atoms bitwise at set-up, then a
+    # short trajectory against the oracle
+    w = workload(n=32, nchains=2, nbeads=5, lj=False, chain=False, drag=True, seed=8, zghost=zghost)
+    L = w["box"]
+    rng = np.random.default_rng(8)
+    base = 4.0 + 24.0 * rng.random((40, 3))
+    base[0] = [L, 5.0, 6.0]
+    base[1] = [7.0, L, 8.0]
+    base[2] = [9.0, 10.0, L]
+    base[3] = [0.0, 0.0, 0.0]
+    base[4] = [L, L, L]
+    shift = rng.integers(-1, 2, (40, 3)) * L
+    shift[:5] = 0
+    w["xyz"] = base + shift
+    w["mol"] = np.zeros(40, np.int32)
+    w["types"] = np.zeros(40, np.int32)
+    w["starts"], w["ends"] = np.arange(40, dtype=np.int32), np.arange(1, 41, dtype=np.int32)
+    g = make_gpu(w)
+    o = make_oracle(w)
+    try:
+        for a, b in zip(g.get_atoms(), o.get_atoms()):
+            assert np.array_equal(a, b)
+        for a, b in zip(g.gathered(), o.gathered()):
+            assert np.array_equal(a, b)
+        assert _lists_equal(g.gridlist(), o.gridlist())
+        g.run(20)
+        o.run(20)
+        assert rel_l2(g.get_field(P.PHI), o.get_field(P.PHI)) < 1e-10
+        assert rel_l2(g.get_atoms()[0], o.get_atoms()[0]) < 1e-12   # atom/drag moves x
+        assert np.array_equal(g.get_atoms()[2], o.get_atoms()[2])
+    finally:
+        g.close()
+        o.close()
+
+
 def test_reference_noise_field_steps():
     w = workload(n=32, nchains=5, nbeads=20, seed=6, noise=P.NOISE_FIELD)
     g = make_gpu(w)
