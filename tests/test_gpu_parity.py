@@ -378,11 +378,14 @@ def _fields_only(cls, shape, period, noise, seed=0):
     return s
 
 
-# grids whose z pass takes the fused kernels (Nz in {16, 64, 256}); the
-# anisotropic periods check that the wave-number scales go to the right axes
+# grids whose z pass takes the fused kernels (Nz in {16, 32, 64, 128, 256};
+# 32 and 128 split unevenly, 8 x 4 and 16 x 8); the anisotropic periods
+# check that the wave-number scales go to the right axes
 @pytest.mark.parametrize("shape,period", [((16, 64, 32), [32.0, 96.0, 12.0]),
                                           ((256, 16, 8), [8.0, 16.0, 256.0]),
-                                          ((64, 64, 64), [64.0, 64.0, 64.0])])
+                                          ((64, 64, 64), [64.0, 64.0, 64.0]),
+                                          ((128, 32, 16), [16.0, 40.0, 128.0]),
+                                          ((32, 128, 64), [64.0, 128.0, 32.0])])
 def test_fused_spectral_path(shape, period):
     nz, ny, nx = shape
     fused = _fields_only(P.System, shape, period, P.NOISE_FIELD)
@@ -397,11 +400,15 @@ def test_fused_spectral_path(shape, period):
         fused.run(1)
         plain.run(1)
         o.run(1)
+        # mu = (T/v)(ln phi - ln(1 - phi)) + chi (1 - 2 phi) is small at
+        # phi ~ 0.5 while its slope is ~2: a 1e-13 change of phi moves it by
+        # a few 1e-13 relative, so mu's bar is 1e-12 (measured 3.4e-13 on
+        # the 32 x 128 x 64 grid); an indexing mistake would show at O(1)
         for f in (P.PHI, P.MU, P.GRADX, P.GRADY, P.GRADZ):
-            assert rel_l2(fused.get_field(f), plain.get_field(f)) < 1e-13, f
-        assert rel_l2(fused.get_spectrum(P.MUHAT), plain.get_spectrum(P.MUHAT)) < 1e-13
+            assert rel_l2(fused.get_field(f), plain.get_field(f)) < (1e-12 if f == P.MU else 1e-13), f
+        assert rel_l2(fused.get_spectrum(P.MUHAT), plain.get_spectrum(P.MUHAT)) < 1e-12
     for f in (P.PHI, P.MU, P.GRADX, P.GRADY, P.GRADZ):
-        assert rel_l2(fused.get_field(f), o.get_field(f)) < 1e-12, f
+        assert rel_l2(fused.get_field(f), o.get_field(f)) < (1e-10 if f == P.MU else 1e-12), f
 
 
 def test_fused_spectral_probe_counts():
@@ -429,6 +436,27 @@ def test_fused_spectral_probe_counts():
     g.run(3)
     ms, launches, nbytes = g.probe(None)
     assert launches == 3 and nbytes == (2 * 80.0 + 64.0) * 31 * 64 * 64
+
+
+def test_fused_kernels_run_at_128():
+    # 128 = 16 x 8 (an uneven split): the fused z update, the separable
+    # gradient and the next step's gradient all run on a 128^3 grid
+    w = workload(n=128, nchains=4, nbeads=10, seed=9)
+    g = make_gpu(w)
+    try:
+        g.set_option(P.OPT_NEXT_GRADIENT, 0)
+        g.probe("z_update")
+        g.run(2)
+        assert g.probe(None)[1] == 2
+        g.probe("z_grad")
+        g.run(2)
+        assert g.probe(None)[1] == 2
+        g.set_option(P.OPT_NEXT_GRADIENT, 1)
+        g.probe("c2r_xd")
+        g.run(3)
+        assert g.probe(None)[1] == 2   # every step of a run but its last
+    finally:
+        g.close()
 
 
 def test_pruned_lj_list_matches_full_list_bitwise():
