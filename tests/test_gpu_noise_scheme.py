@@ -271,7 +271,9 @@ def test_drag_trajectory_200_steps(zghost, chain):
                                           ((256, 16, 8), [8.0, 16.0, 256.0]),
                                           ((64, 64, 64), [64.0, 64.0, 64.0]),
                                           ((128, 32, 16), [16.0, 40.0, 128.0]),
-                                          ((32, 128, 64), [64.0, 128.0, 32.0])])
+                                          ((32, 128, 64), [64.0, 128.0, 32.0]),
+                                          ((512, 16, 32), [32.0, 16.0, 512.0]),
+                                          ((16, 512, 32), [32.0, 512.0, 16.0])])
 def test_next_step_gradient_from_spectrum(shape, period):
     from tests.test_gpu_parity import _fields_only
     nz, ny, nx = shape

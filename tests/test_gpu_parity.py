@@ -378,14 +378,16 @@ def _fields_only(cls, shape, period, noise, seed=0):
     return s
 
 
-# grids whose z pass takes the fused kernels (Nz in {16, 32, 64, 128, 256};
-# 32 and 128 split unevenly, 8 x 4 and 16 x 8); the anisotropic periods
-# check that the wave-number scales go to the right axes
+# grids whose z pass takes the fused kernels (Nz in {16, ..., 512}; 32, 128
+# and 512 split unevenly, 8 x 4, 16 x 8 and 16 x 32); the anisotropic
+# periods check that the wave-number scales go to the right axes
 @pytest.mark.parametrize("shape,period", [((16, 64, 32), [32.0, 96.0, 12.0]),
                                           ((256, 16, 8), [8.0, 16.0, 256.0]),
                                           ((64, 64, 64), [64.0, 64.0, 64.0]),
                                           ((128, 32, 16), [16.0, 40.0, 128.0]),
-                                          ((32, 128, 64), [64.0, 128.0, 32.0])])
+                                          ((32, 128, 64), [64.0, 128.0, 32.0]),
+                                          ((512, 16, 32), [32.0, 16.0, 512.0]),
+                                          ((16, 512, 32), [32.0, 512.0, 16.0])])
 def test_fused_spectral_path(shape, period):
     nz, ny, nx = shape
     fused = _fields_only(P.System, shape, period, P.NOISE_FIELD)
