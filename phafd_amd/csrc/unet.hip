@@ -76,13 +76,12 @@ __device__ __forceinline__ u32x4 pack8(const float f[8]) {
 // (nz/zb) blocks.
 // K order of a layer's packed weights: chunk c and lane group kg (16 lanes,
 // one 8-channel group each) -> the tap (return value, -1 past the last
-// group) and the channel group cg.  3^3 kernels with two or more channel
-// groups: chunks come in threes, one per dy (dy = c % 3 - 1), over load
-// phases p = c / 3 of four (dz, dx, cg) columns (column 4p + kg), so a wave
-// reads a column's rows once for the three dy taps.  Otherwise group
-// g = 4c + kg, tap g / CG, cg g % CG.
+// group) and the channel group cg.  3^3 kernels: chunks come in threes,
+// one per dy (dy = c % 3 - 1), over load phases p = c / 3 of four (dz, dx,
+// cg) columns (column 4p + kg), so a wave reads a column's rows once for the
+// three dy taps.  1x1 kernels: group g = 4c + kg, cg = g (one tap).
 __host__ __device__ inline int conv_k_group(int c, int kg, int CG, int taps, int &cg) {
-  if (taps == 27 && CG >= 2) {
+  if (taps == 27) {
     const int q = 4 * (c / 3) + kg;
     if (q >= 9 * CG) return -1;
     const int comb = q / CG;   // dz * 3 + dx (each 0..2)
@@ -95,7 +94,7 @@ __host__ __device__ inline int conv_k_group(int c, int kg, int CG, int taps, int
   return g / CG;
 }
 __host__ __device__ constexpr int conv_chunks(int CG, int taps) {
-  return (taps == 27 && CG >= 2) ? 3 * ((9 * CG + 3) / 4) : (taps * CG + 3) / 4;
+  return taps == 27 ? 3 * ((9 * CG + 3) / 4) : (taps * CG + 3) / 4;
 }
 
 template <int CIN, int TAPS, int MTR>
@@ -142,7 +141,7 @@ __global__ __launch_bounds__(256) void k_conv(const u32x4 *__restrict__ X, const
   constexpr int G = TAPS * CG;
   constexpr int NC = conv_chunks(CG, TAPS);
   constexpr int NT = COUT / 16;
-  constexpr bool DYR = TAPS == 27 && CG >= 2;   // rows reused across the dy taps
+  constexpr bool DYR = TAPS == 27;   // rows reused across the dy taps
   constexpr int NP = NC / 3;                     // load phases (DYR)
   constexpr int H = TAPS == 27 ? 1 : 0;   // halo width
   constexpr int PER = (T::PLANE + 255) / 256;
