@@ -267,7 +267,10 @@ int phafd_hip_phase(phafd_hip_ctx *ctx, int phase);
  *  src/fixgrid_floryhuggins.cpp:74-80) */
 int phafd_hip_counts(phafd_hip_ctx *ctx, int64_t *out8);
 /* NPairGridBin list as ilist / numneigh / flattened neighbours
- * (src/npair_grid_bin.cpp:22-127); call with ilist == NULL to query sizes */
+ * (src/npair_grid_bin.cpp:22-127); call with ilist == NULL to query sizes.
+ * Building it prints the reference's "Neighbor list overflow" warning to
+ * stderr when a row holds more than 2000 atoms (src/npair_grid_bin.cpp:
+ * 118-121), as the half list's build does for its rows */
 int phafd_hip_gridlist(phafd_hip_ctx *ctx, int64_t cap, int *ilist, int *numneigh,
                        int *neigh, int64_t *inum, int64_t *nentries);
 /* NPairHalfBinNewton list (src/npair_half_bin_newton.cpp:38-124) */
@@ -297,7 +300,8 @@ int phafd_hip_timers(phafd_hip_ctx *ctx, double *ms8, int reset);
  *    mu when the two hooks are adjacent (same arithmetic per point).
  *  PHAFD_OPT_FUSE_SPECTRAL [1]: fold the k-space gradient multiply and the
  *    spectral update into the z pass of the FFT (forward z, pointwise map,
- *    inverse z in one kernel) where the grid allows it (Nz in {16, 64, 256}).
+ *    inverse z in one kernel) where the grid allows it (Nz in {16, 32, 64,
+ *    128, 256, 512}).
  *    Same per-mode arithmetic; the inverse z transform factors differently,
  *    so results agree with the unfused path to rounding, not bitwise.  With
  *    it on, phafd_hip_run() leaves muhat complete and phihat consumed by the
