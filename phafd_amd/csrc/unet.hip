@@ -770,7 +770,8 @@ int conv_zb(int tiles_xy, int nz) {
 
 // rows per wave: 4 when the rows of a level allow it, the halo planes fit
 // comfortably in LDS and the accumulators in registers (64 input or output
-// channels: 2)
+// channels: 2; a skip sum formed on load: 2, whose registers for the
+// coarse input would otherwise cost a wave per SIMD)
 // the fused epilogue's outputs (EPI_POOL: P; EPI_HEAD: hw, hb, hout)
 struct ConvEpi {
   uint16_t *P = nullptr;
@@ -781,7 +782,7 @@ struct ConvEpi {
 template <int CIN, int COUT, int TAPS, bool RELU, int EPI = EPI_NONE>
 void conv(const uint16_t *X, const uint16_t *W, const float *bias, uint16_t *Y, int nx, int ny, int nz,
           hipStream_t s, const uint16_t *Q = nullptr, const ConvEpi &e = ConvEpi()) {
-  const bool four = CIN < 64 && COUT < 64 && ny % 16 == 0;
+  const bool four = CIN < 64 && COUT < 64 && ny % 16 == 0 && !Q;
   const int ty = four ? 16 : 8;
   const int tiles = (nx / 16) * (ny / ty);
   const int zb = conv_zb(tiles, nz);
